@@ -1,0 +1,336 @@
+"""globe_amd: MI355X-native matrix-free LGMM solve (ctypes binding of libglobe_amd.so).
+
+The product is the C ABI in include/globe.h (libglobe_amd.so, HIP kernels for
+gfx950) and the C++ CLIs built on it (globe_amd/bin/runPCGsolver, runAIREML).
+This module is the Python binding used by the tests and bench.py; its method
+names follow the reference's C++ functions so parity tests read like the
+reference's own call sites.
+
+There is no CPU fallback: if libglobe_amd.so is missing or no GPU is visible the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libglobe_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "globe.h")
+
+GLOBE_OK = 0
+VX_FULL, VX_GRM_ONLY = 0, 1
+PREC_FP64, PREC_MIXED, PREC_FP64_VALU = 0, 1, 2
+RULE_MAX_RATIO, RULE_ALL_COLUMNS, RULE_PER_COLUMN = 0, 1, 2
+X0_ZERO, X0_PRECOND, X0_GIVEN = 0, 1, 2
+
+_LIB = None
+
+
+class GlobeError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"globe error {code}: {msg}")
+        self.code = code
+
+
+class PcgOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("max_iter", C.c_int32), ("x0_mode", C.c_int32),
+                ("rule", C.c_int32), ("stagnation", C.c_int32), ("precision", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class PcgStats(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("stalled", C.c_int32), ("max_ratio", C.c_double),
+                ("seconds", C.c_double)]
+
+
+def header_symbols() -> list[str]:
+    """Every function declared in include/globe.h."""
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(globe_[a-z0-9_]+)\s*\(", txt)))
+
+
+def load_library(path: str = LIB_PATH):
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built: run __graft_entry__.build() or python globe_amd/build.py")
+    L = C.CDLL(path)
+    P = C.c_void_p
+    i64, i32, dbl = C.c_int64, C.c_int32, C.c_double
+    sig = {
+        "globe_abi_version": ([], C.c_int),
+        "globe_device_count": ([], C.c_int),
+        "globe_create": ([C.c_int, C.POINTER(P)], C.c_int),
+        "globe_destroy": ([P], None),
+        "globe_last_error": ([P], C.c_char_p),
+        "globe_set_cohort": ([P, i64, i32, i32, P, P, P, i64], C.c_int),
+        "globe_load_bed": ([P, P, i64, i64, C.c_float, C.c_float, P], C.c_int),
+        "globe_get_qc": ([P, P, P, P, P], C.c_int),
+        "globe_get_freq": ([P, P], C.c_int),
+        "globe_load_grm": ([P, P], C.c_int),
+        "globe_genetic_diag": ([P, P], C.c_int),
+        "globe_set_varcomp": ([P, P], C.c_int),
+        "globe_vx": ([P, P, i64, C.c_int, P, i64, C.c_int, C.c_int], C.c_int),
+        "globe_vx_dev": ([P, P, i64, C.c_int, P, i64, C.c_int, C.c_int], C.c_int),
+        "globe_precond_apply": ([P, P, i64, C.c_int, P, i64], C.c_int),
+        "globe_pcg": ([P, P, i64, C.c_int, P, i64, P, P, P, P], C.c_int),
+        "globe_pcg_dev": ([P, P, i64, C.c_int, P, i64, P, P, P, P], C.c_int),
+        "globe_gls_blup": ([P, P, P, P, P, P, P, P], C.c_int),
+        "globe_aireml_eval": ([P, P, P, P, P, C.c_int, P, P, P, P, P], C.c_int),
+        "globe_dev_alloc": ([P, i64, C.POINTER(P)], C.c_int),
+        "globe_dev_free": ([P, P], C.c_int),
+        "globe_memcpy_h2d": ([P, P, P, i64], C.c_int),
+        "globe_memcpy_d2h": ([P, P, P, i64], C.c_int),
+        "globe_synchronize": ([P], C.c_int),
+        "globe_synth_bed": ([P, i64, C.c_uint64, C.c_float, C.c_float, C.c_float, C.c_float,
+                             C.c_float, P], C.c_int),
+        "globe_geno_Wa": ([P, P, P], C.c_int),
+        "globe_get_geno_rows": ([P, i64, i64, P, P], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _LIB = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _cm(a, dtype=np.float64):
+    a = np.asarray(a, dtype=dtype)
+    if a.ndim == 1:
+        a = a[:, None]
+    return np.asfortranarray(a)
+
+
+def device_count() -> int:
+    return load_library().globe_device_count()
+
+
+class DeviceBuffer:
+    """Device allocation owned by a context (for the *_dev entry points)."""
+
+    def __init__(self, ctx: "Globe", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        ptr = C.c_void_p()
+        ctx._check(ctx.L.globe_dev_alloc(ctx.h, self.nbytes, C.byref(ptr)))
+        self.ptr = ptr.value
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr) if not arr.flags.f_contiguous else arr
+        assert arr.nbytes <= self.nbytes
+        self.ctx._check(self.ctx.L.globe_memcpy_h2d(self.ctx.h, self.ptr, _p(arr), arr.nbytes))
+
+    def download(self, arr):
+        assert arr.nbytes <= self.nbytes
+        self.ctx._check(self.ctx.L.globe_memcpy_d2h(self.ctx.h, _p(arr), self.ptr, arr.nbytes))
+        return arr
+
+    def free(self):
+        if self.ptr:
+            self.ctx.L.globe_dev_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+
+class Globe:
+    """One device context: cohort + resident genetics + variance components."""
+
+    def __init__(self, device: int = 0):
+        self.L = load_library()
+        h = C.c_void_p()
+        rc = self.L.globe_create(device, C.byref(h))
+        if rc != GLOBE_OK:
+            raise GlobeError(rc, f"globe_create(device={device}) failed "
+                                 f"({self.L.globe_device_count()} device(s) visible)")
+        self.h = h.value
+        self.nrec = self.n = self.p = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.globe_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != GLOBE_OK:
+            msg = self.L.globe_last_error(self.h)
+            raise GlobeError(rc, msg.decode() if msg else "")
+
+    # ---- cohort / genetics (readInputData, prepGenotypeInput, readFullGRM)
+    def set_cohort(self, indiv, times, X):
+        self._indiv = np.ascontiguousarray(indiv, dtype=np.int32)
+        self._times = np.ascontiguousarray(times, dtype=np.float64)
+        X = _cm(X)
+        self.nrec = int(self._indiv.shape[0])
+        self.n = int(self._indiv.max()) + 1 if self.nrec else 0
+        self.p = int(X.shape[1]) - 1
+        self._check(self.L.globe_set_cohort(self.h, self.nrec, self.n, self.p, _p(self._indiv),
+                                            _p(self._times), _p(X), X.shape[0]))
+
+    def prepGenotypeInput(self, rows, min_maf=0.01, max_miss=0.2):
+        rows = np.ascontiguousarray(rows, dtype=np.uint8)
+        kept = C.c_int64()
+        self._check(self.L.globe_load_bed(self.h, _p(rows), rows.shape[0], rows.shape[1],
+                                          min_maf, max_miss, C.byref(kept)))
+        self.m = kept.value
+        self.m_total = rows.shape[0]
+        return kept.value
+
+    def synth_bed(self, m, seed, f_lo=0.01, f_hi=0.5, miss=0.0, min_maf=0.01, max_miss=0.2):
+        kept = C.c_int64()
+        self._check(self.L.globe_synth_bed(self.h, m, seed, f_lo, f_hi, miss, min_maf, max_miss,
+                                           C.byref(kept)))
+        self.m = kept.value
+        self.m_total = m
+        return kept.value
+
+    def qc(self):
+        m = self.m_total
+        alt = np.zeros(m, dtype=np.int64)
+        miss = np.zeros(m, dtype=np.int64)
+        keep = np.zeros(m, dtype=np.uint8)
+        maf = np.zeros(m, dtype=np.float32)
+        self._check(self.L.globe_get_qc(self.h, _p(alt), _p(miss), _p(keep), _p(maf)))
+        return {"alt": alt, "miss": miss, "keep": keep.astype(bool), "maf": maf}
+
+    def allele_freqs(self):
+        f = np.zeros(max(self.m, 1), dtype=np.float32)
+        self._check(self.L.globe_get_freq(self.h, _p(f)))
+        return f[: self.m]
+
+    def geno_rows(self, first=0, count=None):
+        count = self.m - first if count is None else count
+        pitch = C.c_int64()
+        self._check(self.L.globe_get_geno_rows(self.h, 0, 0, None, C.byref(pitch)))
+        out = np.zeros((count, pitch.value), dtype=np.uint8)
+        self._check(self.L.globe_get_geno_rows(self.h, first, count, _p(out), C.byref(pitch)))
+        return out
+
+    def readFullGRM(self, grm_lt):
+        self._grm = np.ascontiguousarray(grm_lt, dtype=np.float32)
+        self._check(self.L.globe_load_grm(self.h, _p(self._grm)))
+
+    def computeGeneticDiagonal(self):
+        out = np.zeros(self.n)
+        self._check(self.L.globe_genetic_diag(self.h, _p(out)))
+        return out
+
+    def set_varcomp(self, theta):
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        self._check(self.L.globe_set_varcomp(self.h, _p(th)))
+
+    # ---- operator (computeVx_multi) and preconditioner (applyPreconditioner)
+    def computeVx_multi(self, x, multGRM_uhat=False, precision=PREC_FP64):
+        x = _cm(x)
+        k = x.shape[1]
+        rows = self.n if multGRM_uhat else self.nrec
+        y = np.zeros((rows, k), order="F")
+        self._check(self.L.globe_vx(self.h, _p(x), x.shape[0], k, _p(y), rows,
+                                    VX_GRM_ONLY if multGRM_uhat else VX_FULL, precision))
+        return y
+
+    def applyPreconditioner(self, r):
+        r = _cm(r)
+        z = np.zeros_like(r, order="F")
+        self._check(self.L.globe_precond_apply(self.h, _p(r), r.shape[0], r.shape[1], _p(z),
+                                               z.shape[0]))
+        return z
+
+    # ---- solver (runPCG_multi / runPCG_multi_VC)
+    def pcg(self, b, *, tol=1e-5, max_iter=1000, x0_mode=X0_ZERO, rule=RULE_MAX_RATIO,
+            stagnation=True, precision=PREC_FP64, x0=None):
+        b = _cm(b)
+        k = b.shape[1]
+        x = np.zeros_like(b, order="F") if x0 is None else _cm(x0).copy(order="F")
+        o = PcgOpts(tol, max_iter, x0_mode, rule, int(stagnation), precision, 0)
+        st = PcgStats()
+        ratio = np.zeros(k)
+        its = np.zeros(k, dtype=np.int32)
+        self._check(self.L.globe_pcg(self.h, _p(b), b.shape[0], k, _p(x), x.shape[0],
+                                     C.byref(o), C.byref(st), _p(ratio), _p(its)))
+        return x, {"iters": st.iters, "stalled": st.stalled, "max_ratio": st.max_ratio,
+                   "seconds": st.seconds, "col_ratio": ratio, "col_iters": its}
+
+    def runPCG_multi(self, b, **kw):
+        kw.setdefault("rule", RULE_MAX_RATIO)
+        return self.pcg(b, **kw)
+
+    def runPCG_multi_VC(self, b, **kw):
+        kw.setdefault("rule", RULE_ALL_COLUMNS)
+        kw.setdefault("stagnation", False)
+        return self.pcg(b, **kw)
+
+    # ---- estimators (runPCGsolver main body, runAIREML derivatives)
+    def gls_blup(self, y, *, tol=1e-5, max_iter=1000, x0_mode=X0_ZERO, rule=RULE_MAX_RATIO,
+                 stagnation=True, precision=PREC_FP64):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        beta = np.zeros(self.p + 1)
+        vr = np.zeros(self.nrec)
+        uh = np.zeros(self.n)
+        bh = np.zeros(2 * self.n)
+        o = PcgOpts(tol, max_iter, x0_mode, rule, int(stagnation), precision, 0)
+        st = PcgStats()
+        self._check(self.L.globe_gls_blup(self.h, _p(y), C.byref(o), _p(beta), _p(vr), _p(uh),
+                                          _p(bh), C.byref(st)))
+        return {"beta": beta, "vinv_r": vr, "uhat": uh, "bhat": bh, "iters": st.iters,
+                "stalled": st.stalled, "seconds": st.seconds}
+
+    def aireml_eval(self, y, probes, weights, probe_x0=None, *, tol=1e-5, max_iter=1000,
+                    precision=PREC_FP64):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        probes = _cm(probes)
+        weights = _cm(weights)
+        x0 = None if probe_x0 is None else _cm(probe_x0)
+        nmc = probes.shape[1]
+        grad = np.zeros(5)
+        ai = np.zeros(25)
+        first = np.zeros(5)
+        tr = np.zeros(5)
+        o = PcgOpts(tol, max_iter, X0_PRECOND, RULE_ALL_COLUMNS, 0, precision, 0)
+        self._check(self.L.globe_aireml_eval(self.h, _p(y), _p(probes), _p(weights), _p(x0), nmc,
+                                             C.byref(o), _p(grad), _p(ai), _p(first), _p(tr)))
+        return {"grad": grad, "ai": ai.reshape(5, 5), "first": first, "trace": tr}
+
+    def geno_Wa(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        out = np.zeros(self.n)
+        self._check(self.L.globe_geno_Wa(self.h, _p(a), _p(out)))
+        return out
+
+    # ---- device-resident buffers
+    def alloc(self, nbytes) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def pcg_dev(self, bbuf, xbuf, k, *, tol=1e-5, max_iter=1000, x0_mode=X0_ZERO,
+                rule=RULE_PER_COLUMN, stagnation=False, precision=PREC_FP64):
+        o = PcgOpts(tol, max_iter, x0_mode, rule, int(stagnation), precision, 0)
+        st = PcgStats()
+        ratio = np.zeros(k)
+        its = np.zeros(k, dtype=np.int32)
+        self._check(self.L.globe_pcg_dev(self.h, bbuf.ptr, self.nrec, k, xbuf.ptr, self.nrec,
+                                         C.byref(o), C.byref(st), _p(ratio), _p(its)))
+        return {"iters": st.iters, "stalled": st.stalled, "max_ratio": st.max_ratio,
+                "seconds": st.seconds, "col_ratio": ratio, "col_iters": its}
+
+    def vx_dev(self, xbuf, ybuf, k, mode=VX_FULL, precision=PREC_FP64):
+        rows = self.n if mode == VX_GRM_ONLY else self.nrec
+        self._check(self.L.globe_vx_dev(self.h, xbuf.ptr, self.nrec, k, ybuf.ptr, rows, mode,
+                                        precision))
+
+    def synchronize(self):
+        self._check(self.L.globe_synchronize(self.h))

@@ -1,0 +1,955 @@
+// libglobe_amd.so: the C ABI (include/globe.h) over the HIP kernels.
+//
+// Host code owns orchestration only: staging, the operator sequence, the PCG loop
+// (whose scalars live on the device), the small FP64 dense algebra of GLS and
+// AIREML ((p+1)^2 and 5x5 systems), and error reporting. Every vector operation
+// runs on the device stream of the context.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "context.h"
+
+using globe::DevBuf;
+using globe::Error;
+
+namespace globe {
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        throw Error(e == hipErrorOutOfMemory ? GLOBE_E_NOMEM : GLOBE_E_DEVICE,
+                    std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void DevBuf::ensure(size_t b) {
+    if (b <= bytes && p) return;
+    release();
+    if (b == 0) b = 256;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+        p = nullptr;
+        bytes = 0;
+        (void)hipGetLastError();
+        throw Error(GLOBE_E_NOMEM, "device allocation of " + std::to_string(b) + " bytes failed: " +
+                                       hipGetErrorString(e));
+    }
+    bytes = b;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+void require(bool cond, int code, const std::string& msg) {
+    if (!cond) throw Error(code, msg);
+}
+
+// Gauss-Jordan inverse with partial pivoting (the reference inverts X^T V^-1 X and
+// the AI matrix with Eigen's inverse(), PCGsolver.cpp:280, varCompEst.cpp:1485).
+static bool invert(int n, const std::vector<double>& a, std::vector<double>& out) {
+    std::vector<double> w((size_t)n * 2 * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            w[i * 2 * n + j] = a[i * n + j];
+            w[i * 2 * n + n + j] = i == j ? 1.0 : 0.0;
+        }
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(w[r * 2 * n + c]) > std::fabs(w[piv * 2 * n + c])) piv = r;
+        if (w[piv * 2 * n + c] == 0.0) return false;
+        if (piv != c)
+            for (int j = 0; j < 2 * n; ++j) std::swap(w[c * 2 * n + j], w[piv * 2 * n + j]);
+        const double d = w[c * 2 * n + c];
+        for (int j = 0; j < 2 * n; ++j) w[c * 2 * n + j] /= d;
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            const double f = w[r * 2 * n + c];
+            if (f == 0.0) continue;
+            for (int j = 0; j < 2 * n; ++j) w[r * 2 * n + j] -= f * w[c * 2 * n + j];
+        }
+    }
+    out.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) out[i * n + j] = w[i * 2 * n + n + j];
+    return true;
+}
+
+void ensure_theta(globe_ctx* c, int k) {
+    require(c->has_theta, GLOBE_E_STATE, "variance components not set (globe_set_varcomp)");
+    if (c->theta_k >= k) return;
+    c->d_theta.ensure(sizeof(double) * 5 * kMaxCols);
+    std::vector<double> h((size_t)5 * kMaxCols);
+    for (int j = 0; j < kMaxCols; ++j)
+        for (int q = 0; q < 5; ++q) h[j * 5 + q] = c->theta[q];
+    HIP_CHECK(hipMemcpyAsync(c->d_theta.p, h.data(), h.size() * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->theta_k = kMaxCols;
+}
+
+void ensure_gdiag(globe_ctx* c) {
+    if (c->gdiag_ok) return;
+    require(c->gen != 0, GLOBE_E_STATE, "no genetics loaded (globe_load_bed / globe_load_grm)");
+    c->d_gdiag.ensure(sizeof(double) * (size_t)c->n);
+    if (c->gen == 1) {
+        launch_gdiag(c->stream, c->d_ind.as<uint8_t>(), c->pitch_i, c->n, c->m,
+                     c->d_twof.as<double>(), c->d_inv.as<double>(), c->d_gdiag.as<double>());
+    }
+    // GRM: the diagonal was uploaded by globe_load_grm
+    c->gdiag_ok = true;
+}
+
+static void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int precision,
+                            const int* skip) {
+    const int32_t n = c->n;
+    hipStream_t s = c->stream;
+    const double* th = c->d_theta.as<double>();
+    if (c->gen == 2) {
+        launch_grm_product(s, c->d_grm.as<float>(), n, S0, n, k, th, GZ, n, skip);
+        return;
+    }
+    if (c->m == 0) {
+        launch_fill_cols(s, n, GZ, n, k, 0.0);
+        return;
+    }
+    if (precision != GLOBE_PREC_FP64_VALU) {
+        const int limbs = precision == GLOBE_PREC_MIXED ? kLimbsMixed : kLimbsFP64;
+        if (k * limbs <= 128) {
+            mfma_geno_product(c, S0, n, k, GZ, n, skip, limbs, th);
+            return;
+        }
+        // wider blocks run in column groups that fit one contraction
+        const int kg = 128 / limbs;
+        for (int c0 = 0; c0 < k; c0 += kg) {
+            const int kk = std::min(kg, k - c0);
+            mfma_geno_product(c, S0 + (size_t)c0 * n, n, kk, GZ + (size_t)c0 * n, n, skip, limbs,
+                              th + (size_t)c0 * 5);
+        }
+        return;
+    }
+    const int64_t m = c->m;
+    c->w_P1.ensure(sizeof(double) * (size_t)m * k);
+    c->w_U.ensure(sizeof(double) * (size_t)m * k);
+    c->w_P2.ensure(sizeof(double) * (size_t)n * k);
+    c->w_T.ensure(sizeof(double) * kMaxCols);
+    c->w_C.ensure(sizeof(double) * kMaxCols);
+    c->w_part2.ensure(sizeof(double) * kMaxParts * kMaxCols);
+    double* P1 = c->w_P1.as<double>();
+    double* U = c->w_U.as<double>();
+    double* P2 = c->w_P2.as<double>();
+    double* part = c->w_part2.as<double>();
+    // pass 1 (PCGsolver.cpp:1143-1171): P1[c][s] = sum_i g_is S0[c][i]
+    launch_rowdot(s, c->d_snp.as<uint8_t>(), c->pitch_s, m, n, S0, n, k, P1, m, skip);
+    launch_colsum_parts(s, n, S0, n, k, part, skip);
+    launch_reduce_parts_skip(s, part, parts_for(n), k, c->w_T.as<double>(), skip);
+    launch_center_snp(s, m, k, P1, m, c->w_T.as<double>(), c->d_twof.as<double>(),
+                      c->d_inv.as<double>(), th, 1.0 / (double)m, U, m, part, skip);
+    launch_reduce_parts_skip(s, part, parts_for(m), k, c->w_C.as<double>(), skip);
+    // pass 2 (PCGsolver.cpp:1173-1199): GZ[c][i] = sum_s g_is u[c][s] - C[c]
+    launch_rowdot(s, c->d_ind.as<uint8_t>(), c->pitch_i, n, m, U, m, k, P2, n, skip);
+    launch_center_ind(s, n, k, P2, n, c->w_C.as<double>(), GZ, n, skip);
+}
+
+void op_vx(globe_ctx* c, const double* x, int64_t ldx, int k, double* y, int64_t ldy, int mode,
+           int precision, const int* skip, double* part_dot) {
+    require(c->has_cohort, GLOBE_E_STATE, "cohort not set (globe_set_cohort)");
+    require(c->gen != 0, GLOBE_E_STATE, "no genetics loaded (globe_load_bed / globe_load_grm)");
+    require(k >= 1 && k <= kMaxCols, GLOBE_E_INVALID, "k must be in [1, 64]");
+    ensure_theta(c, k);
+    const int32_t n = c->n;
+    c->w_S0.ensure(sizeof(double) * (size_t)n * k);
+    c->w_S1.ensure(sizeof(double) * (size_t)n * k);
+    c->w_GZ.ensure(sizeof(double) * (size_t)n * k);
+    double* S0 = c->w_S0.as<double>();
+    double* S1 = c->w_S1.as<double>();
+    double* GZ = c->w_GZ.as<double>();
+    hipStream_t s = c->stream;
+    launch_seg_sums(s, n, c->d_blk_off.as<int64_t>(), c->d_blk_ind.as<int32_t>(),
+                    c->d_times.as<double>(), x, ldx, k, S0, S1, n, skip);
+    if (mode == GLOBE_VX_GRM_ONLY) {
+        genetic_product(c, S0, k, GZ, precision, skip);
+        launch_copy_cols(s, n, GZ, n, y, ldy, k);
+        return;
+    }
+    genetic_product(c, S0, k, GZ, precision, skip);
+    launch_record_out(s, c->nrec, c->d_indiv.as<int32_t>(), c->d_times.as<double>(), x, ldx, k, GZ,
+                      S0, S1, n, c->d_theta.as<double>(), y, ldy, part_dot, skip);
+}
+
+static void precond(globe_ctx* c, const double* r, int64_t ldr, int k, double* z, int64_t ldz,
+                    double* part_rz, double* part_rr, const PcgState* st) {
+    ensure_gdiag(c);
+    ensure_theta(c, k);
+    launch_precond(c->stream, c->n, c->d_blk_off.as<int64_t>(), c->d_blk_ind.as<int32_t>(),
+                   c->d_times.as<double>(), c->d_gdiag.as<double>(), c->d_theta.as<double>(), r,
+                   ldr, k, z, ldz, part_rz, part_rr, st, 0);
+}
+
+// PCG with the reference's control flow (PCGsolver.cpp:1324-1444 for rule 0,
+// varCompEst.cpp:453-528 for rule 1), all scalars on the device. The host enqueues
+// batches of iterations and reads back only the done flag between batches.
+void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64_t ldx,
+             const globe_pcg_opts& o, globe_pcg_stats* st, double* col_ratio, int32_t* col_iters) {
+    require(c->has_cohort && c->gen != 0, GLOBE_E_STATE, "cohort and genetics required");
+    require(k >= 1 && k <= kMaxCols, GLOBE_E_INVALID, "k must be in [1, 64]");
+    require(o.rule >= 0 && o.rule <= 2, GLOBE_E_INVALID, "unknown PCG rule");
+    require(o.x0_mode >= 0 && o.x0_mode <= 2, GLOBE_E_INVALID, "unknown x0 mode");
+    require(o.precision >= GLOBE_PREC_FP64 && o.precision <= GLOBE_PREC_FP64_VALU, GLOBE_E_INVALID,
+            "unknown precision");
+    ensure_gdiag(c);
+    ensure_theta(c, k);
+    const int64_t nrec = c->nrec;
+    hipStream_t s = c->stream;
+    const size_t vb = sizeof(double) * (size_t)nrec * k;
+    c->p_r.ensure(vb);
+    c->p_z.ensure(vb);
+    c->p_p.ensure(vb);
+    c->p_vp.ensure(vb);
+    c->p_state.ensure(sizeof(PcgState));
+    c->w_part.ensure(sizeof(double) * kMaxParts * kMaxCols);
+    c->w_part3.ensure(sizeof(double) * kMaxParts * kMaxCols);
+    c->w_part4.ensure(sizeof(double) * kMaxParts * kMaxCols);
+    double* r = c->p_r.as<double>();
+    double* z = c->p_z.as<double>();
+    double* p = c->p_p.as<double>();
+    double* vp = c->p_vp.as<double>();
+    double* part_a = c->w_part.as<double>();
+    double* part_b = c->w_part3.as<double>();
+    double* part_c = c->w_part4.as<double>();
+    PcgState* dst = c->p_state.as<PcgState>();
+    PcgState* hs = c->h_state;
+    std::memset(hs, 0, sizeof(PcgState));
+    hs->max_iter = o.max_iter;
+    hs->rule = o.rule;
+    hs->stagnation = o.stagnation;
+    hs->k = k;
+    hs->tol = o.tol;
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, s));
+    HIP_CHECK(hipMemcpyAsync(dst, hs, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    const int np = parts_for(nrec);
+    const int npb = parts_for(c->n);
+    // starting point and residual
+    if (o.x0_mode == GLOBE_X0_ZERO) {
+        launch_fill_cols(s, nrec, X, ldx, k, 0.0);
+        launch_copy_cols(s, nrec, B, ldb, r, nrec, k);
+    } else {
+        if (o.x0_mode == GLOBE_X0_PRECOND) precond(c, B, ldb, k, X, ldx, nullptr, nullptr, nullptr);
+        op_vx(c, X, ldx, k, vp, nrec, GLOBE_VX_FULL, o.precision, nullptr, nullptr);
+        launch_sub_cols(s, nrec, B, ldb, vp, nrec, r, nrec, k);
+    }
+    launch_dot_parts(s, nrec, B, ldb, B, ldb, k, part_a);
+    precond(c, r, nrec, k, z, nrec, part_c, part_b, nullptr);
+    launch_copy_cols(s, nrec, z, nrec, p, nrec, k);
+    // partial slot counts: dots over records use np, precond uses npb
+    // (k_pcg_init reads up to max(np, npb) slots; unused slots are zero-filled below)
+    const int nmax = std::max(np, npb);
+    if (npb < nmax) {
+        for (int j = 0; j < k; ++j) {
+            launch_fill(s, part_b + (size_t)j * kMaxParts + npb, nmax - npb, 0.0);
+            launch_fill(s, part_c + (size_t)j * kMaxParts + npb, nmax - npb, 0.0);
+        }
+    }
+    if (np < nmax)
+        for (int j = 0; j < k; ++j) launch_fill(s, part_a + (size_t)j * kMaxParts + np, nmax - np, 0.0);
+    launch_pcg_init(s, dst, part_a, part_b, part_c, nmax, k);
+    const int* skip = &dst->done;
+    int batch = 1, queued = 0;
+    while (true) {
+        for (int b = 0; b < batch; ++b) {
+            op_vx(c, p, nrec, k, vp, nrec, GLOBE_VX_FULL, o.precision, skip, part_a);
+            launch_pcg_alpha(s, dst, part_a, np, k);
+            launch_pcg_xr(s, dst, nrec, X, ldx, r, p, vp, nrec, k, part_b);
+            launch_pcg_check(s, dst, part_b, np, k);
+            precond(c, r, nrec, k, z, nrec, part_c, nullptr, dst);
+            launch_pcg_beta(s, dst, part_c, npb, k);
+            launch_pcg_p(s, dst, nrec, p, z, nrec, k);
+        }
+        queued += batch;
+        HIP_CHECK(hipMemcpyAsync(hs, dst, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        if (hs->done || queued >= o.max_iter + 1) break;
+        batch = std::min(batch * 2, 16);
+    }
+    HIP_CHECK(hipEventRecord(e1, s));
+    HIP_CHECK(hipMemcpyAsync(hs, dst, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st) {
+        st->iters = hs->iters;
+        st->stalled = hs->stalled;
+        st->max_ratio = hs->max_ratio;
+        st->seconds = ms * 1e-3;
+    }
+    for (int j = 0; j < k; ++j) {
+        if (col_ratio) col_ratio[j] = hs->ratio[j];
+        if (col_iters) col_iters[j] = hs->col_iters[j];
+    }
+}
+
+}  // namespace globe
+
+using namespace globe;
+
+// ----------------------------------------------------------------------------- ABI
+#define API_BEGIN(ctx)                                 \
+    if (!(ctx)) return GLOBE_E_INVALID;                \
+    try {                                              \
+        HIP_CHECK(hipSetDevice((ctx)->device));
+#define API_END(ctx)                                   \
+    return GLOBE_OK;                                   \
+    }                                                  \
+    catch (const Error& e) {                           \
+        (ctx)->err = e.what();                         \
+        return e.code;                                 \
+    }                                                  \
+    catch (const std::exception& e) {                  \
+        (ctx)->err = e.what();                         \
+        return GLOBE_E_INVALID;                        \
+    }
+
+extern "C" {
+
+int globe_abi_version(void) { return GLOBE_ABI_VERSION; }
+
+int globe_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int globe_create(int device, globe_ctx** out) {
+    if (!out) return GLOBE_E_INVALID;
+    *out = nullptr;
+    int n = globe_device_count();
+    if (device < 0 || device >= n) return GLOBE_E_DEVICE;
+    globe_ctx* c = new globe_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_state, sizeof(PcgState), hipHostMallocDefault) != hipSuccess) {
+        delete c;
+        return GLOBE_E_DEVICE;
+    }
+    *out = c;
+    return GLOBE_OK;
+}
+
+void globe_destroy(globe_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->h_state) (void)hipHostFree(c->h_state);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* globe_last_error(const globe_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int globe_set_cohort(globe_ctx* c, int64_t nrec, int32_t n, int32_t p, const int32_t* indiv,
+                     const double* times, const double* X, int64_t ldx) {
+    API_BEGIN(c)
+    require(nrec > 0 && n > 0 && p >= 0 && indiv && times && X && ldx >= nrec, GLOBE_E_INVALID,
+            "globe_set_cohort: bad sizes or null pointers");
+    // blocks = runs of equal individual (PCGsolver.cpp:1004-1011); each individual
+    // must form exactly one run and every index 0..n-1 must occur.
+    std::vector<int64_t> off;
+    std::vector<int32_t> ind;
+    std::vector<uint8_t> seen((size_t)n, 0);
+    for (int64_t r = 0; r < nrec; ++r) {
+        const int32_t i = indiv[r];
+        require(i >= 0 && i < n, GLOBE_E_INVALID,
+                "individual index " + std::to_string(i) + " at record " + std::to_string(r) +
+                    " outside [0, n)");
+        if (r == 0 || indiv[r - 1] != i) {
+            require(!seen[i], GLOBE_E_INVALID,
+                    "records of individual " + std::to_string(i + 1) + " are not contiguous");
+            seen[i] = 1;
+            off.push_back(r);
+            ind.push_back(i);
+        }
+    }
+    off.push_back(nrec);
+    require((int64_t)ind.size() == n, GLOBE_E_INVALID,
+            "individual IDs must be exactly 1..N: " + std::to_string(ind.size()) +
+                " distinct individuals for N = " + std::to_string(n));
+    c->nrec = nrec;
+    c->n = n;
+    c->p = p;
+    c->h_indiv.assign(indiv, indiv + nrec);
+    c->h_times.assign(times, times + nrec);
+    c->h_X.resize((size_t)nrec * (p + 1));
+    for (int q = 0; q <= p; ++q)
+        std::memcpy(&c->h_X[(size_t)q * nrec], X + (size_t)q * ldx, sizeof(double) * nrec);
+    c->h_blk_off = off;
+    c->h_blk_ind = ind;
+    c->d_indiv.ensure(sizeof(int32_t) * nrec);
+    c->d_times.ensure(sizeof(double) * nrec);
+    c->d_X.ensure(sizeof(double) * nrec * (p + 1));
+    c->d_blk_off.ensure(sizeof(int64_t) * (n + 1));
+    c->d_blk_ind.ensure(sizeof(int32_t) * n);
+    HIP_CHECK(hipMemcpy(c->d_indiv.p, indiv, sizeof(int32_t) * nrec, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(c->d_times.p, times, sizeof(double) * nrec, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(c->d_X.p, c->h_X.data(), sizeof(double) * nrec * (p + 1), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(c->d_blk_off.p, off.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(c->d_blk_ind.p, ind.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    c->has_cohort = true;
+    // genetics must match n
+    if ((c->gen == 1 || c->gen == 2) && c->gdiag_ok && c->d_gdiag.bytes < sizeof(double) * (size_t)n)
+        c->gdiag_ok = false;
+    API_END(c)
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ genotype staging
+namespace {
+
+// FP32 keep rule of PCGsolver.cpp:726-732, evaluated exactly as the reference does.
+inline bool keep_snp(int64_t alt, int64_t miss, int32_t n, float min_maf, float max_miss,
+                     float* maf_out) {
+    const int n_obs = n - (int)miss;
+    volatile float denom = 2.0f * (float)n_obs;
+    const float maf = (n_obs > 0) ? (float)alt / denom : 0.0f;
+    const float mr = (float)miss / (float)n;
+    *maf_out = maf;
+    return maf >= min_maf && maf <= (1.0f - min_maf) && mr <= max_miss;
+}
+
+// Stage m input SNP rows in chunks: fill(chunk_dev, s0, cnt) writes cnt rows of
+// `pitch_in` bytes; counts + QC + compaction run per chunk, then the
+// individual-major copy is built from the kept rows.
+template <class Fill>
+void stage_rows(globe_ctx* c, int64_t m, int64_t pitch_in, float min_maf, float max_miss,
+                Fill fill) {
+    require(c->has_cohort, GLOBE_E_STATE, "cohort must be set before genotypes");
+    const int32_t n = c->n;
+    require(pitch_in >= (n + 3) / 4, GLOBE_E_INVALID, "row pitch smaller than ceil(N/4)");
+    require(min_maf >= 0.0f && min_maf <= 0.5f && max_miss >= 0.0f && max_miss <= 1.0f,
+            GLOBE_E_INVALID, "invalid MAF or missing rate thresholds");
+    hipStream_t s = c->stream;
+    c->gen = 0;
+    c->gdiag_ok = false;
+    c->d_grm.release();
+    c->pitch_s = round_up((n + 3) / 4, kPitchAlign);
+    c->d_snp.release();
+    c->d_ind.release();
+    c->d_snp.ensure((size_t)std::max<int64_t>(m, 1) * c->pitch_s);
+    const int64_t chunk_rows = std::max<int64_t>(1, std::min<int64_t>(m, (int64_t)(1ll << 30) / pitch_in));
+    DevBuf raw, d_alt, d_miss, d_idx;
+    raw.ensure((size_t)chunk_rows * pitch_in);
+    d_alt.ensure(sizeof(int64_t) * chunk_rows);
+    d_miss.ensure(sizeof(int64_t) * chunk_rows);
+    d_idx.ensure(sizeof(int64_t) * chunk_rows);
+    c->qc_alt.assign(m, 0);
+    c->qc_miss.assign(m, 0);
+    c->qc_keep.assign(m, 0);
+    c->qc_maf.assign(m, 0.f);
+    c->h_freq.clear();
+    int64_t kept = 0;
+    std::vector<int64_t> idx;
+    for (int64_t s0 = 0; s0 < m; s0 += chunk_rows) {
+        const int64_t cnt = std::min(chunk_rows, m - s0);
+        fill(raw.as<uint8_t>(), s0, cnt);
+        launch_bed_counts(s, raw.as<uint8_t>(), cnt, pitch_in, n, d_alt.as<int64_t>(),
+                          d_miss.as<int64_t>());
+        HIP_CHECK(hipMemcpyAsync(&c->qc_alt[s0], d_alt.p, sizeof(int64_t) * cnt, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipMemcpyAsync(&c->qc_miss[s0], d_miss.p, sizeof(int64_t) * cnt, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        idx.clear();
+        for (int64_t q = 0; q < cnt; ++q) {
+            float maf;
+            const bool k = keep_snp(c->qc_alt[s0 + q], c->qc_miss[s0 + q], n, min_maf, max_miss, &maf);
+            c->qc_maf[s0 + q] = maf;
+            c->qc_keep[s0 + q] = k;
+            if (k) {
+                idx.push_back(q);
+                c->h_freq.push_back(maf);
+            }
+        }
+        if (!idx.empty()) {
+            HIP_CHECK(hipMemcpyAsync(d_idx.p, idx.data(), sizeof(int64_t) * idx.size(), hipMemcpyHostToDevice, s));
+            launch_compact_rows(s, raw.as<uint8_t>(), pitch_in, d_idx.as<int64_t>(), (int64_t)idx.size(), n,
+                                c->d_snp.as<uint8_t>() + kept * c->pitch_s, c->pitch_s);
+            HIP_CHECK(hipStreamSynchronize(s));
+        }
+        kept += (int64_t)idx.size();
+    }
+    raw.release();
+    c->m_total = m;
+    c->m = kept;
+    c->pitch_i = round_up((kept + 3) / 4, kPitchAlign);
+    c->d_ind.ensure((size_t)n * c->pitch_i);
+    HIP_CHECK(hipMemsetAsync(c->d_ind.p, 0xFF, (size_t)n * c->pitch_i, s));
+    launch_transpose_geno(s, c->d_snp.as<uint8_t>(), kept, c->pitch_s, n, c->d_ind.as<uint8_t>(), c->pitch_i);
+    c->d_freq.ensure(sizeof(float) * std::max<int64_t>(kept, 1));
+    c->d_twof.ensure(sizeof(double) * std::max<int64_t>(kept, 1));
+    c->d_inv.ensure(sizeof(double) * std::max<int64_t>(kept, 1));
+    if (kept > 0) {
+        HIP_CHECK(hipMemcpyAsync(c->d_freq.p, c->h_freq.data(), sizeof(float) * kept, hipMemcpyHostToDevice, s));
+        launch_snp_consts(s, c->d_freq.as<float>(), kept, c->d_twof.as<double>(), c->d_inv.as<double>());
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    c->gen = 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int globe_load_bed(globe_ctx* c, const uint8_t* rows, int64_t m, int64_t pitch, float min_maf,
+                   float max_miss, int64_t* m_kept) {
+    API_BEGIN(c)
+    require(rows != nullptr || m == 0, GLOBE_E_INVALID, "null genotype rows");
+    require(m >= 0, GLOBE_E_INVALID, "negative SNP count");
+    stage_rows(c, m, pitch, min_maf, max_miss, [&](uint8_t* dst, int64_t s0, int64_t cnt) {
+        HIP_CHECK(hipMemcpyAsync(dst, rows + s0 * pitch, (size_t)cnt * pitch, hipMemcpyHostToDevice,
+                                 c->stream));
+    });
+    if (m_kept) *m_kept = c->m;
+    API_END(c)
+}
+
+int globe_synth_bed(globe_ctx* c, int64_t m, uint64_t seed, float f_lo, float f_hi, float miss,
+                    float min_maf, float max_miss, int64_t* m_kept) {
+    API_BEGIN(c)
+    require(m >= 0 && f_lo >= 0.f && f_hi <= 1.f && f_lo <= f_hi && miss >= 0.f && miss <= 1.f,
+            GLOBE_E_INVALID, "globe_synth_bed: bad parameters");
+    require(c->has_cohort, GLOBE_E_STATE, "cohort must be set before genotypes");
+    const int64_t pitch = round_up((c->n + 3) / 4, kPitchAlign);
+    stage_rows(c, m, pitch, min_maf, max_miss, [&](uint8_t* dst, int64_t s0, int64_t cnt) {
+        launch_synth_bed(c->stream, dst, cnt, pitch, c->n, seed + 0x51ED2701ull * (uint64_t)s0, f_lo,
+                         f_hi, miss);
+    });
+    if (m_kept) *m_kept = c->m;
+    API_END(c)
+}
+
+int globe_get_qc(globe_ctx* c, int64_t* alt, int64_t* miss, uint8_t* keep, float* maf) {
+    API_BEGIN(c)
+    require(c->gen == 1, GLOBE_E_STATE, "no genotypes loaded");
+    const size_t m = c->qc_alt.size();
+    if (alt) std::memcpy(alt, c->qc_alt.data(), sizeof(int64_t) * m);
+    if (miss) std::memcpy(miss, c->qc_miss.data(), sizeof(int64_t) * m);
+    if (keep) std::memcpy(keep, c->qc_keep.data(), m);
+    if (maf) std::memcpy(maf, c->qc_maf.data(), sizeof(float) * m);
+    API_END(c)
+}
+
+int globe_get_freq(globe_ctx* c, float* freq) {
+    API_BEGIN(c)
+    require(c->gen == 1, GLOBE_E_STATE, "no genotypes loaded");
+    if (freq) std::memcpy(freq, c->h_freq.data(), sizeof(float) * c->h_freq.size());
+    API_END(c)
+}
+
+int globe_get_geno_rows(globe_ctx* c, int64_t first, int64_t count, uint8_t* out, int64_t* pitch) {
+    API_BEGIN(c)
+    require(c->gen == 1, GLOBE_E_STATE, "no genotypes loaded");
+    require(first >= 0 && count >= 0 && first + count <= c->m, GLOBE_E_INVALID, "row range");
+    if (pitch) *pitch = c->pitch_s;
+    if (out && count > 0)
+        HIP_CHECK(hipMemcpy(out, c->d_snp.as<uint8_t>() + first * c->pitch_s, (size_t)count * c->pitch_s,
+                            hipMemcpyDeviceToHost));
+    API_END(c)
+}
+
+int globe_load_grm(globe_ctx* c, const float* lt) {
+    API_BEGIN(c)
+    require(c->has_cohort, GLOBE_E_STATE, "cohort must be set before the GRM");
+    require(lt != nullptr, GLOBE_E_INVALID, "null GRM");
+    const int64_t n = c->n;
+    const size_t nel = (size_t)n * (n + 1) / 2;
+    c->gen = 0;
+    c->d_snp.release();
+    c->d_ind.release();
+    c->m = 0;
+    c->d_grm.ensure(sizeof(float) * nel);
+    HIP_CHECK(hipMemcpy(c->d_grm.p, lt, sizeof(float) * nel, hipMemcpyHostToDevice));
+    std::vector<double> gd((size_t)n);
+    for (int64_t i = 0; i < n; ++i) gd[i] = (double)lt[(size_t)i * (i + 1) / 2 + i];
+    c->d_gdiag.ensure(sizeof(double) * n);
+    HIP_CHECK(hipMemcpy(c->d_gdiag.p, gd.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    c->gdiag_ok = true;
+    c->gen = 2;
+    API_END(c)
+}
+
+int globe_genetic_diag(globe_ctx* c, double* out) {
+    API_BEGIN(c)
+    ensure_gdiag(c);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (out) HIP_CHECK(hipMemcpy(out, c->d_gdiag.p, sizeof(double) * c->n, hipMemcpyDeviceToHost));
+    API_END(c)
+}
+
+int globe_set_varcomp(globe_ctx* c, const double theta[5]) {
+    API_BEGIN(c)
+    require(theta != nullptr, GLOBE_E_INVALID, "null theta");
+    for (int q = 0; q < 5; ++q)
+        require(std::isfinite(theta[q]), GLOBE_E_INVALID, "non-finite variance component");
+    require(theta[0] > 0.0, GLOBE_E_INVALID, "sig2e must be positive");
+    for (int q = 0; q < 5; ++q) c->theta[q] = theta[q];
+    c->has_theta = true;
+    c->theta_k = -1;
+    ensure_theta(c, kMaxCols);
+    API_END(c)
+}
+
+int globe_vx_dev(globe_ctx* c, const double* X, int64_t ldx, int k, double* Y, int64_t ldy,
+                 int mode, int precision) {
+    API_BEGIN(c)
+    require(mode == GLOBE_VX_FULL || mode == GLOBE_VX_GRM_ONLY, GLOBE_E_INVALID, "unknown mode");
+    require(precision >= GLOBE_PREC_FP64 && precision <= GLOBE_PREC_FP64_VALU, GLOBE_E_INVALID,
+            "unknown precision");
+    const int64_t rows_out = mode == GLOBE_VX_FULL ? c->nrec : c->n;
+    require(ldx >= c->nrec && ldy >= rows_out, GLOBE_E_INVALID, "leading dimension too small");
+    op_vx(c, X, ldx, k, Y, ldy, mode, precision, nullptr, nullptr);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    API_END(c)
+}
+
+int globe_vx(globe_ctx* c, const double* X, int64_t ldx, int k, double* Y, int64_t ldy, int mode,
+             int precision) {
+    API_BEGIN(c)
+    require(X && Y && k >= 1 && k <= kMaxCols, GLOBE_E_INVALID, "globe_vx: bad arguments");
+    require(c->has_cohort, GLOBE_E_STATE, "cohort not set");
+    const int64_t rows_out = mode == GLOBE_VX_FULL ? c->nrec : c->n;
+    require(ldx >= c->nrec && ldy >= rows_out, GLOBE_E_INVALID, "leading dimension too small");
+    c->s_in.ensure(sizeof(double) * c->nrec * k);
+    c->s_out.ensure(sizeof(double) * rows_out * k);
+    HIP_CHECK(hipMemcpy2D(c->s_in.p, sizeof(double) * c->nrec, X, sizeof(double) * ldx,
+                          sizeof(double) * c->nrec, k, hipMemcpyHostToDevice));
+    int rc = globe_vx_dev(c, c->s_in.as<double>(), c->nrec, k, c->s_out.as<double>(), rows_out, mode,
+                          precision);
+    if (rc != GLOBE_OK) return rc;
+    HIP_CHECK(hipMemcpy2D(Y, sizeof(double) * ldy, c->s_out.p, sizeof(double) * rows_out,
+                          sizeof(double) * rows_out, k, hipMemcpyDeviceToHost));
+    API_END(c)
+}
+
+int globe_precond_apply(globe_ctx* c, const double* R, int64_t ldr, int k, double* Z, int64_t ldz) {
+    API_BEGIN(c)
+    require(R && Z && k >= 1 && k <= kMaxCols && ldr >= c->nrec && ldz >= c->nrec, GLOBE_E_INVALID,
+            "globe_precond_apply: bad arguments");
+    require(c->has_cohort && c->gen != 0, GLOBE_E_STATE, "cohort and genetics required");
+    c->s_in.ensure(sizeof(double) * c->nrec * k);
+    c->s_out.ensure(sizeof(double) * c->nrec * k);
+    HIP_CHECK(hipMemcpy2D(c->s_in.p, sizeof(double) * c->nrec, R, sizeof(double) * ldr,
+                          sizeof(double) * c->nrec, k, hipMemcpyHostToDevice));
+    precond(c, c->s_in.as<double>(), c->nrec, k, c->s_out.as<double>(), c->nrec, nullptr, nullptr, nullptr);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    HIP_CHECK(hipMemcpy2D(Z, sizeof(double) * ldz, c->s_out.p, sizeof(double) * c->nrec,
+                          sizeof(double) * c->nrec, k, hipMemcpyDeviceToHost));
+    API_END(c)
+}
+
+int globe_pcg_dev(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64_t ldx,
+                  const globe_pcg_opts* opts, globe_pcg_stats* stats, double* col_ratio,
+                  int32_t* col_iters) {
+    API_BEGIN(c)
+    require(B && X && opts && ldb >= c->nrec && ldx >= c->nrec, GLOBE_E_INVALID,
+            "globe_pcg: bad arguments");
+    run_pcg(c, B, ldb, k, X, ldx, *opts, stats, col_ratio, col_iters);
+    API_END(c)
+}
+
+int globe_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64_t ldx,
+              const globe_pcg_opts* opts, globe_pcg_stats* stats, double* col_ratio,
+              int32_t* col_iters) {
+    API_BEGIN(c)
+    require(B && X && opts && k >= 1 && k <= kMaxCols && ldb >= c->nrec && ldx >= c->nrec,
+            GLOBE_E_INVALID, "globe_pcg: bad arguments");
+    require(c->has_cohort, GLOBE_E_STATE, "cohort not set");
+    const int64_t nrec = c->nrec;
+    c->s_in.ensure(sizeof(double) * nrec * k);
+    c->s_out.ensure(sizeof(double) * nrec * k);
+    HIP_CHECK(hipMemcpy2D(c->s_in.p, sizeof(double) * nrec, B, sizeof(double) * ldb,
+                          sizeof(double) * nrec, k, hipMemcpyHostToDevice));
+    if (opts->x0_mode == GLOBE_X0_GIVEN)
+        HIP_CHECK(hipMemcpy2D(c->s_out.p, sizeof(double) * nrec, X, sizeof(double) * ldx,
+                              sizeof(double) * nrec, k, hipMemcpyHostToDevice));
+    run_pcg(c, c->s_in.as<double>(), nrec, k, c->s_out.as<double>(), nrec, *opts, stats, col_ratio,
+            col_iters);
+    HIP_CHECK(hipMemcpy2D(X, sizeof(double) * ldx, c->s_out.p, sizeof(double) * nrec,
+                          sizeof(double) * nrec, k, hipMemcpyDeviceToHost));
+    API_END(c)
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------ estimators
+namespace {
+
+struct Fit {
+    std::vector<double> beta, xtvx_inv;  // (p+1), (p+1)^2 row-major
+    globe_pcg_stats stats{};
+};
+
+// fitMME core (PCGsolver.cpp:262-290, varCompEst.cpp:636-665): solve V [X | y] into
+// c->e_sol (nrec x (p+2)), then beta and V^-1 r (left in column p+1 of e_sol... copied
+// to vinv_r).
+Fit fit_gls(globe_ctx* c, const double* y_dev, const globe_pcg_opts& o, double* vinv_r_dev) {
+    const int64_t nrec = c->nrec;
+    const int q = c->p + 1, k = c->p + 2;
+    require(k <= kMaxCols, GLOBE_E_INVALID, "too many covariates");
+    hipStream_t s = c->stream;
+    c->e_xy.ensure(sizeof(double) * nrec * k);
+    c->e_sol.ensure(sizeof(double) * nrec * k);
+    double* xy = c->e_xy.as<double>();
+    double* sol = c->e_sol.as<double>();
+    launch_copy_cols(s, nrec, c->d_X.as<double>(), nrec, xy, nrec, q);
+    launch_copy_cols(s, nrec, y_dev, nrec, xy + (size_t)q * nrec, nrec, 1);
+    Fit f;
+    run_pcg(c, xy, nrec, k, sol, nrec, o, &f.stats, nullptr, nullptr);
+    // X^T [V^-1 X | V^-1 y]
+    c->e_small.ensure(sizeof(double) * kMaxCols * kMaxCols);
+    c->w_part.ensure(sizeof(double) * kMaxParts * kMaxCols * kMaxCols);
+    launch_gram(s, nrec, c->d_X.as<double>(), nrec, q, sol, nrec, k, c->w_part.as<double>(),
+                c->e_small.as<double>());
+    std::vector<double> g((size_t)q * k);
+    HIP_CHECK(hipMemcpyAsync(g.data(), c->e_small.p, sizeof(double) * q * k, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<double> A((size_t)q * q), Ai;
+    for (int a = 0; a < q; ++a)
+        for (int b = 0; b < q; ++b) A[a * q + b] = g[a * k + b];
+    require(invert(q, A, Ai), GLOBE_E_NUMERIC, "X^T V^-1 X is singular");
+    f.beta.assign(q, 0.0);
+    for (int a = 0; a < q; ++a)
+        for (int b = 0; b < q; ++b) f.beta[a] += Ai[a * q + b] * g[b * k + q];
+    f.xtvx_inv = Ai;
+    // V^-1 r = V^-1 y - V^-1 X beta
+    launch_copy_cols(s, nrec, sol + (size_t)q * nrec, nrec, vinv_r_dev, nrec, 1);
+    c->e_coef.ensure(sizeof(double) * kMaxCols * kMaxCols);
+    HIP_CHECK(hipMemcpyAsync(c->e_coef.p, f.beta.data(), sizeof(double) * q, hipMemcpyHostToDevice, s));
+    launch_axpy_cols(s, nrec, vinv_r_dev, nrec, sol, nrec, q, c->e_coef.as<double>(), 1);
+    HIP_CHECK(hipStreamSynchronize(s));
+    return f;
+}
+
+// P v = V^-1 v - V^-1 X (X^T V^-1 X)^-1 X^T V^-1 v for ncols columns in place
+// (varCompEst.cpp:960, :1028-1035); sv holds V^-1 v, V^-1 X is e_sol[:, :p+1].
+void apply_P(globe_ctx* c, const Fit& f, double* sv, int ncols) {
+    const int64_t nrec = c->nrec;
+    const int q = c->p + 1;
+    hipStream_t s = c->stream;
+    c->e_small.ensure(sizeof(double) * kMaxCols * kMaxCols);
+    c->w_part.ensure(sizeof(double) * kMaxParts * kMaxCols * kMaxCols);
+    launch_gram(s, nrec, c->d_X.as<double>(), nrec, q, sv, nrec, ncols, c->w_part.as<double>(),
+                c->e_small.as<double>());
+    std::vector<double> xv((size_t)q * ncols), coef((size_t)q * ncols, 0.0);
+    HIP_CHECK(hipMemcpyAsync(xv.data(), c->e_small.p, sizeof(double) * q * ncols, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    for (int a = 0; a < q; ++a)
+        for (int j = 0; j < ncols; ++j)
+            for (int b = 0; b < q; ++b) coef[a * ncols + j] += f.xtvx_inv[a * q + b] * xv[b * ncols + j];
+    c->e_coef.ensure(sizeof(double) * kMaxCols * kMaxCols);
+    HIP_CHECK(hipMemcpyAsync(c->e_coef.p, coef.data(), sizeof(double) * q * ncols, hipMemcpyHostToDevice, s));
+    launch_axpy_cols(s, nrec, sv, nrec, c->e_sol.as<double>(), nrec, q, c->e_coef.as<double>(), ncols);
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+extern "C" {
+
+int globe_gls_blup(globe_ctx* c, const double* y, const globe_pcg_opts* opts, double* beta,
+                   double* vinv_r, double* uhat, double* bhat, globe_pcg_stats* stats) {
+    API_BEGIN(c)
+    require(y && opts, GLOBE_E_INVALID, "globe_gls_blup: null argument");
+    require(c->has_cohort && c->gen != 0, GLOBE_E_STATE, "cohort and genetics required");
+    const int64_t nrec = c->nrec;
+    hipStream_t s = c->stream;
+    c->e_probe.ensure(sizeof(double) * nrec);
+    c->e_pv.ensure(sizeof(double) * nrec);
+    HIP_CHECK(hipMemcpy(c->e_probe.p, y, sizeof(double) * nrec, hipMemcpyHostToDevice));
+    Fit f = fit_gls(c, c->e_probe.as<double>(), *opts, c->e_pv.as<double>());
+    if (beta) std::copy(f.beta.begin(), f.beta.end(), beta);
+    if (stats) *stats = f.stats;
+    if (vinv_r) HIP_CHECK(hipMemcpy(vinv_r, c->e_pv.p, sizeof(double) * nrec, hipMemcpyDeviceToHost));
+    if (uhat) {
+        // compute_uhat (PCGsolver.cpp:1447-1473): sig2g G J^T V^-1 r
+        c->e_us.ensure(sizeof(double) * c->n);
+        op_vx(c, c->e_pv.as<double>(), nrec, 1, c->e_us.as<double>(), c->n, GLOBE_VX_GRM_ONLY,
+              opts->precision, nullptr, nullptr);
+        HIP_CHECK(hipStreamSynchronize(s));
+        HIP_CHECK(hipMemcpy(uhat, c->e_us.p, sizeof(double) * c->n, hipMemcpyDeviceToHost));
+    }
+    if (bhat) {
+        c->e_fs.ensure(sizeof(double) * 2 * c->n);
+        launch_bhat(s, c->n, c->d_blk_off.as<int64_t>(), c->d_blk_ind.as<int32_t>(),
+                    c->d_times.as<double>(), c->e_pv.as<double>(), c->d_theta.as<double>(),
+                    c->e_fs.as<double>());
+        HIP_CHECK(hipStreamSynchronize(s));
+        HIP_CHECK(hipMemcpy(bhat, c->e_fs.p, sizeof(double) * 2 * c->n, hipMemcpyDeviceToHost));
+    }
+    API_END(c)
+}
+
+int globe_aireml_eval(globe_ctx* c, const double* y, const double* probes, const double* weights,
+                      const double* probe_x0, int nmc, const globe_pcg_opts* opts, double* grad,
+                      double* ai, double* first_terms, double* trace) {
+    API_BEGIN(c)
+    require(y && opts && grad && ai, GLOBE_E_INVALID, "globe_aireml_eval: null argument");
+    require(nmc >= 0 && nmc <= kMaxCols / 5, GLOBE_E_INVALID, "nmc must be in [0, 12]");
+    require(nmc == 0 || (probes && weights), GLOBE_E_INVALID, "probes/weights required");
+    require(c->has_cohort && c->gen != 0, GLOBE_E_STATE, "cohort and genetics required");
+    const int64_t nrec = c->nrec;
+    const int32_t n = c->n;
+    hipStream_t s = c->stream;
+    const double sg = c->theta[1];
+    require(sg > 0.0, GLOBE_E_INVALID, "sig2g must be positive for AIREML (f_theta divides by it)");
+    // 1. fitMME at theta (varCompEst.cpp:587-685): x0 = Q^-1 b
+    globe_pcg_opts o = *opts;
+    o.x0_mode = GLOBE_X0_PRECOND;
+    c->e_probe.ensure(sizeof(double) * nrec * std::max(1, nmc));
+    c->e_pv.ensure(sizeof(double) * nrec * std::max(5, nmc));
+    HIP_CHECK(hipMemcpy(c->e_probe.p, y, sizeof(double) * nrec, hipMemcpyHostToDevice));
+    c->e_w.ensure(sizeof(double) * nrec);
+    double* py = c->e_w.as<double>();
+    Fit f = fit_gls(c, c->e_probe.as<double>(), o, py);
+    // 2. uhat and f_theta (varCompEst.cpp:693-760)
+    c->e_us.ensure(sizeof(double) * (size_t)n * std::max(1, nmc));
+    op_vx(c, py, nrec, 1, c->e_us.as<double>(), n, GLOBE_VX_GRM_ONLY, o.precision, nullptr, nullptr);
+    c->e_f.ensure(sizeof(double) * nrec * 5);
+    double* fth = c->e_f.as<double>();
+    launch_f_theta(s, n, c->d_blk_off.as<int64_t>(), c->d_blk_ind.as<int32_t>(), c->d_times.as<double>(),
+                   py, nrec, c->e_us.as<double>(), n, 1, 1.0 / sg, fth, nrec);
+    c->e_small.ensure(sizeof(double) * kMaxCols * kMaxCols);
+    c->w_part.ensure(sizeof(double) * kMaxParts * kMaxCols * kMaxCols);
+    launch_gram(s, nrec, py, nrec, 1, fth, nrec, 5, c->w_part.as<double>(), c->e_small.as<double>());
+    double first[5];
+    HIP_CHECK(hipMemcpyAsync(first, c->e_small.p, sizeof(double) * 5, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    // 3. trace terms (varCompEst.cpp:816-977) with injected probes
+    double tr[5] = {0, 0, 0, 0, 0};
+    if (nmc > 0) {
+        c->e_sv.ensure(sizeof(double) * nrec * nmc);
+        c->e_fs.ensure(sizeof(double) * nrec * 5 * nmc);
+        c->s_aux.ensure(sizeof(double) * nrec * nmc);
+        double* sv = c->e_sv.as<double>();
+        HIP_CHECK(hipMemcpy(c->e_probe.p, probes, sizeof(double) * nrec * nmc, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(c->s_aux.p, weights, sizeof(double) * nrec * nmc, hipMemcpyHostToDevice));
+        globe_pcg_opts ot = o;
+        if (probe_x0) {
+            HIP_CHECK(hipMemcpy(sv, probe_x0, sizeof(double) * nrec * nmc, hipMemcpyHostToDevice));
+            ot.x0_mode = GLOBE_X0_GIVEN;
+        }
+        run_pcg(c, c->e_probe.as<double>(), nrec, nmc, sv, nrec, ot, nullptr, nullptr, nullptr);
+        apply_P(c, f, sv, nmc);
+        op_vx(c, sv, nrec, nmc, c->e_us.as<double>(), n, GLOBE_VX_GRM_ONLY, o.precision, nullptr, nullptr);
+        double* fs = c->e_fs.as<double>();
+        launch_f_theta(s, n, c->d_blk_off.as<int64_t>(), c->d_blk_ind.as<int32_t>(),
+                       c->d_times.as<double>(), sv, nrec, c->e_us.as<double>(), n, nmc, 1.0 / sg, fs, nrec);
+        for (int smp = 0; smp < nmc; ++smp) {
+            launch_gram(s, nrec, c->s_aux.as<double>() + (size_t)smp * nrec, nrec, 1,
+                        fs + (size_t)5 * smp * nrec, nrec, 5, c->w_part.as<double>(),
+                        c->e_small.as<double>() + 5 * smp);
+        }
+        std::vector<double> t((size_t)5 * nmc);
+        HIP_CHECK(hipMemcpyAsync(t.data(), c->e_small.p, sizeof(double) * 5 * nmc, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        for (int smp = 0; smp < nmc; ++smp)
+            for (int j = 0; j < 5; ++j) tr[j] += t[smp * 5 + j];
+        for (int j = 0; j < 5; ++j) tr[j] /= (double)nmc;
+    }
+    for (int j = 0; j < 5; ++j) grad[j] = 0.5 * (first[j] - tr[j]);
+    if (first_terms) std::copy(first, first + 5, first_terms);
+    if (trace) std::copy(tr, tr + 5, trace);
+    // 4. AI matrix (varCompEst.cpp:1015-1041): PCG(f, x0 = Q^-1 f), P f, AI = f^T P f / 2
+    double* pf = c->e_pv.as<double>();
+    globe_pcg_opts oa = o;
+    oa.x0_mode = GLOBE_X0_PRECOND;
+    run_pcg(c, fth, nrec, 5, pf, nrec, oa, nullptr, nullptr, nullptr);
+    apply_P(c, f, pf, 5);
+    launch_gram(s, nrec, fth, nrec, 5, pf, nrec, 5, c->w_part.as<double>(), c->e_small.as<double>());
+    double aim[25];
+    HIP_CHECK(hipMemcpyAsync(aim, c->e_small.p, sizeof(double) * 25, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    for (int q = 0; q < 25; ++q) ai[q] = 0.5 * aim[q];
+    API_END(c)
+}
+
+// ------------------------------------------------------------------ device helpers
+int globe_dev_alloc(globe_ctx* c, int64_t bytes, void** ptr) {
+    API_BEGIN(c)
+    require(ptr && bytes >= 0, GLOBE_E_INVALID, "globe_dev_alloc: bad arguments");
+    *ptr = nullptr;
+    hipError_t e = hipMalloc(ptr, bytes > 0 ? (size_t)bytes : 256);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(GLOBE_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    API_END(c)
+}
+
+int globe_dev_free(globe_ctx* c, void* ptr) {
+    API_BEGIN(c)
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (ptr) HIP_CHECK(hipFree(ptr));
+    API_END(c)
+}
+
+int globe_memcpy_h2d(globe_ctx* c, void* dst, const void* src, int64_t bytes) {
+    API_BEGIN(c)
+    require(dst && src && bytes >= 0, GLOBE_E_INVALID, "globe_memcpy_h2d: bad arguments");
+    HIP_CHECK(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    API_END(c)
+}
+
+int globe_memcpy_d2h(globe_ctx* c, void* dst, const void* src, int64_t bytes) {
+    API_BEGIN(c)
+    require(dst && src && bytes >= 0, GLOBE_E_INVALID, "globe_memcpy_d2h: bad arguments");
+    HIP_CHECK(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    API_END(c)
+}
+
+int globe_synchronize(globe_ctx* c) {
+    API_BEGIN(c)
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    API_END(c)
+}
+
+int globe_geno_Wa(globe_ctx* c, const double* a, double* out) {
+    API_BEGIN(c)
+    require(a && out, GLOBE_E_INVALID, "globe_geno_Wa: null argument");
+    require(c->gen == 1, GLOBE_E_STATE, "no genotypes loaded");
+    const int64_t m = c->m;
+    const int32_t n = c->n;
+    hipStream_t s = c->stream;
+    // out_i = sum_s g_is inv_s a_s - sum_s 2 f_s inv_s a_s: scale a on the host, then
+    // one pass over the individual-major rows.
+    std::vector<double> inv((size_t)m), twof((size_t)m), u((size_t)m);
+    HIP_CHECK(hipMemcpy(inv.data(), c->d_inv.p, sizeof(double) * m, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(twof.data(), c->d_twof.p, sizeof(double) * m, hipMemcpyDeviceToHost));
+    double csum = 0.0;
+    for (int64_t q = 0; q < m; ++q) {
+        u[q] = inv[q] * a[q];
+        csum += twof[q] * u[q];
+    }
+    c->w_U.ensure(sizeof(double) * std::max<int64_t>(m, 1));
+    c->w_P2.ensure(sizeof(double) * n);
+    HIP_CHECK(hipMemcpy(c->w_U.p, u.data(), sizeof(double) * m, hipMemcpyHostToDevice));
+    launch_rowdot(s, c->d_ind.as<uint8_t>(), c->pitch_i, n, m, c->w_U.as<double>(), m, 1,
+                  c->w_P2.as<double>(), n, nullptr);
+    HIP_CHECK(hipStreamSynchronize(s));
+    HIP_CHECK(hipMemcpy(out, c->w_P2.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    for (int32_t i = 0; i < n; ++i) out[i] -= csum;
+    API_END(c)
+}
+
+}  // extern "C"

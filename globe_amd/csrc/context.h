@@ -1,0 +1,71 @@
+// The context behind the C ABI: cohort, resident genetics, variance components,
+// workspaces and the device PCG state of one solve stream.
+#pragma once
+#include <array>
+
+#include "globe_internal.h"
+
+struct globe_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // cohort
+    bool has_cohort = false;
+    int64_t nrec = 0;
+    int32_t n = 0, p = 0;
+    std::vector<int32_t> h_indiv, h_blk_ind;
+    std::vector<int64_t> h_blk_off;
+    std::vector<double> h_times, h_X;
+    globe::DevBuf d_indiv, d_times, d_X, d_blk_off, d_blk_ind;
+
+    // genetics: 0 none, 1 packed genotypes, 2 GRM
+    int gen = 0;
+    int64_t m_total = 0, m = 0, pitch_s = 0, pitch_i = 0;
+    globe::DevBuf d_snp, d_ind, d_freq, d_twof, d_inv, d_grm;
+    std::vector<int64_t> qc_alt, qc_miss;
+    std::vector<uint8_t> qc_keep;
+    std::vector<float> qc_maf, h_freq;
+    globe::DevBuf d_gdiag;
+    bool gdiag_ok = false;
+
+    // variance components
+    std::array<double, 5> theta{0, 0, 0, 0, 0};
+    bool has_theta = false;
+    globe::DevBuf d_theta;       // [kMaxCols][5]
+    int theta_k = -1;            // columns of d_theta filled with `theta`
+
+    // operator workspaces
+    globe::DevBuf w_S0, w_S1, w_GZ, w_P1, w_U, w_P2, w_T, w_C, w_part, w_part2, w_part3, w_part4;
+    globe::DevBuf w_mix;         // mixed-precision contraction scratch
+
+    // PCG workspaces and state
+    globe::DevBuf p_state, p_r, p_z, p_p, p_vp;
+    globe::PcgState* h_state = nullptr;  // pinned mirror
+
+    // host-pointer API staging
+    globe::DevBuf s_in, s_out, s_aux;
+
+    // estimator workspaces
+    globe::DevBuf e_xy, e_sol, e_f, e_probe, e_w, e_sv, e_pv, e_us, e_fs, e_coef, e_small;
+};
+
+namespace globe {
+
+// device operator: y = V x (mode FULL) or sig2g G J^T x (GRM_ONLY); optional fused
+// partials of sum x.y per column; skip = device done flag (nullptr: always run).
+void op_vx(globe_ctx* c, const double* x, int64_t ldx, int k, double* y, int64_t ldy, int mode,
+           int precision, const int* skip, double* part_dot);
+void ensure_theta(globe_ctx* c, int k);
+void ensure_gdiag(globe_ctx* c);
+void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64_t ldx,
+             const globe_pcg_opts& o, globe_pcg_stats* st, double* col_ratio, int32_t* col_iters);
+void require(bool cond, int code, const std::string& msg);
+// packed-genotype contraction on the int8 matrix cores (k_mfma.hip): GZ = sig2g G S0
+// with `limbs` int8 digits per value of the right-hand side
+void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, double* GZ,
+                       int64_t ldg, const int* skip, int limbs, const double* theta_cols);
+constexpr int kLimbsFP64 = 7;
+constexpr int kLimbsMixed = 4;
+
+}  // namespace globe

@@ -1,0 +1,126 @@
+// Internal declarations of libglobe_amd.so: device buffers, the context, and the
+// host-side launch wrappers of the HIP kernels (k_record.hip, k_geno.hip,
+// k_grm.hip, k_mfma.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/globe.h"
+
+namespace globe {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what);
+#define HIP_CHECK(x) ::globe::hip_check((x), #x)
+
+// Grow-only device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t b);
+    void release();
+    ~DevBuf() { release(); }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+constexpr int kBlock = 256;         // threads per block of the streaming kernels
+constexpr int kMaxParts = 1024;     // partial-sum slots per column (grid cap)
+constexpr int kMaxCols = 64;        // max right-hand sides per solve
+constexpr int kGenoTile = 512;      // genotypes per K-step of the row-dot kernels
+constexpr int64_t kPitchAlign = 128;  // bytes: row pitch multiple (512 genotypes)
+
+inline int parts_for(int64_t count) {
+    int64_t b = (count + kBlock - 1) / kBlock;
+    if (b < 1) b = 1;
+    return (int)(b > kMaxParts ? kMaxParts : b);
+}
+inline int64_t round_up(int64_t a, int64_t m) { return (a + m - 1) / m * m; }
+
+// Device-side PCG state: scalars then per-column arrays (k <= kMaxCols).
+struct PcgState {
+    int done, iters, stag, stalled;
+    int max_iter, rule, stagnation, k;
+    double tol, last, max_ratio, pad;
+    double rs[kMaxCols], alpha[kMaxCols], beta[kMaxCols], bnorm[kMaxCols], ratio[kMaxCols];
+    int active[kMaxCols], col_iters[kMaxCols];
+};
+
+// ---------------------------------------------------------------- record space
+void launch_seg_sums(hipStream_t s, int32_t nblk, const int64_t* blk_off, const int32_t* blk_ind,
+                     const double* times, const double* x, int64_t ldx, int k, double* S0,
+                     double* S1, int64_t ldn, const int* skip);
+void launch_record_out(hipStream_t s, int64_t nrec, const int32_t* indiv, const double* times,
+                       const double* x, int64_t ldx, int k, const double* GZ, const double* S0,
+                       const double* S1, int64_t ldn, const double* theta_cols, double* y,
+                       int64_t ldy, double* part_dot, const int* skip);
+void launch_precond(hipStream_t s, int32_t nblk, const int64_t* blk_off, const int32_t* blk_ind,
+                    const double* times, const double* gdiag, const double* theta_cols,
+                    const double* r, int64_t ldr, int k, double* z, int64_t ldz, double* part_rz,
+                    double* part_rr, const PcgState* st, int tilde_v);
+void launch_dot_parts(hipStream_t s, int64_t nrec, const double* a, int64_t lda, const double* b,
+                      int64_t ldb, int k, double* part);
+void launch_reduce_parts(hipStream_t s, const double* part, int nparts, int k, double* out);
+void launch_gram(hipStream_t s, int64_t nrec, const double* A, int64_t lda, int ka,
+                 const double* B, int64_t ldb, int kb, double* part, double* out);
+void launch_axpy_cols(hipStream_t s, int64_t nrec, double* y, int64_t ldy, const double* x,
+                      int64_t ldx, int kx, const double* coef, int ky);
+void launch_pcg_init(hipStream_t s, PcgState* st, const double* part_bb, const double* part_rr,
+                     const double* part_rz, int nparts, int k);
+void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* part_pvp, int nparts, int k);
+void launch_pcg_xr(hipStream_t s, PcgState* st, int64_t nrec, double* x, int64_t ldx, double* r,
+                   const double* p, const double* vp, int64_t ld, int k, double* part_rr);
+void launch_pcg_check(hipStream_t s, PcgState* st, const double* part_rr, int nparts, int k);
+void launch_pcg_beta(hipStream_t s, PcgState* st, const double* part_rz, int nparts, int k);
+void launch_pcg_p(hipStream_t s, PcgState* st, int64_t nrec, double* p, const double* z,
+                  int64_t ld, int k);
+void launch_bhat(hipStream_t s, int32_t nblk, const int64_t* blk_off, const int32_t* blk_ind,
+                 const double* times, const double* v, const double* theta, double* bhat);
+void launch_f_theta(hipStream_t s, int32_t nblk, const int64_t* blk_off, const int32_t* blk_ind,
+                    const double* times, const double* py, int64_t ldpy, const double* uhat,
+                    int64_t ldu, int ncols, double inv_sg, double* f, int64_t ldf);
+void launch_copy_cols(hipStream_t s, int64_t rows, const double* src, int64_t lds, double* dst,
+                      int64_t ldd, int k);
+void launch_fill(hipStream_t s, double* dst, int64_t count, double v);
+void launch_sub_cols(hipStream_t s, int64_t rows, const double* a, int64_t lda, const double* b,
+                     int64_t ldb, double* d, int64_t ldd, int k);
+void launch_fill_cols(hipStream_t s, int64_t rows, double* d, int64_t ldd, int k, double v);
+
+// ---------------------------------------------------------------- genotypes
+void launch_bed_counts(hipStream_t s, const uint8_t* rows, int64_t m, int64_t pitch, int32_t n,
+                       int64_t* alt, int64_t* miss);
+void launch_compact_rows(hipStream_t s, const uint8_t* src, int64_t pitch_src, const int64_t* idx,
+                         int64_t m_kept, int32_t n, uint8_t* dst, int64_t pitch_dst);
+void launch_transpose_geno(hipStream_t s, const uint8_t* snp, int64_t m, int64_t pitch_s, int32_t n,
+                           uint8_t* ind, int64_t pitch_i);
+void launch_synth_bed(hipStream_t s, uint8_t* rows, int64_t m, int64_t pitch, int32_t n,
+                      uint64_t seed, float f_lo, float f_hi, float miss);
+void launch_snp_consts(hipStream_t s, const float* freq, int64_t m, double* twof, double* inv);
+void launch_gdiag(hipStream_t s, const uint8_t* ind, int64_t pitch_i, int32_t n, int64_t m,
+                  const double* twof, const double* inv, double* gdiag);
+// out[c][r] = sum_k dosage(rows[r], k) * v[c][k] over nk genotypes (FP64).
+void launch_rowdot(hipStream_t s, const uint8_t* rows, int64_t pitch, int64_t nrows, int64_t nk,
+                   const double* v, int64_t ldv, int k, double* out, int64_t ldo, const int* skip);
+void launch_center_snp(hipStream_t s, int64_t m, int k, const double* P1, int64_t ld1,
+                       const double* T, const double* twof, const double* inv,
+                       const double* theta_cols, double inv_m, double* u, int64_t ldu,
+                       double* part_c, const int* skip);
+void launch_center_ind(hipStream_t s, int32_t n, int k, const double* P2, int64_t ld2,
+                       const double* Csum, double* GZ, int64_t ldg, const int* skip);
+void launch_colsum_parts(hipStream_t s, int64_t rows, const double* v, int64_t ldv, int k,
+                         double* part, const int* skip);
+void launch_reduce_parts_skip(hipStream_t s, const double* part, int nparts, int k, double* out,
+                              const int* skip);
+
+// ---------------------------------------------------------------- GRM
+void launch_grm_product(hipStream_t s, const float* lt, int32_t n, const double* S0, int64_t ldn,
+                        int k, const double* theta_cols, double* GZ, int64_t ldg, const int* skip);
+
+}  // namespace globe

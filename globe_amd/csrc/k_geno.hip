@@ -1,0 +1,447 @@
+// Packed-genotype kernels.
+//
+// HBM layout (both resident for the life of the cohort):
+//   snp-major  : m kept SNP rows x pitch_s bytes, individual i of SNP s in bits
+//                2(i%4) of byte i/4 (PLINK 1 SNP-major, PCGsolver.cpp:825-827);
+//   indiv-major: n individual rows x pitch_i bytes, SNP s in bits 2(s%4) of byte s/4.
+// Pitches are multiples of 128 bytes (512 genotypes). Phantom codes of the last
+// byte and all padding hold code 11 (dosage 0), so padding never contributes.
+// Dosage of a code (PCGsolver.cpp:83-94, missing -> 0 at :837): 00 -> 2, 10 -> 1,
+// 11 -> 0, 01 -> 0; branch-free d = (1 - (c & 1)) * (2 - (c >> 1)).
+#include "globe_internal.h"
+
+namespace globe {
+
+namespace {
+
+__device__ inline int dosage(uint32_t code) { return (1 - (int)(code & 1u)) * (2 - (int)(code >> 1)); }
+
+// ------------------------------------------------------------------ QC counts
+// Per SNP: A1 allele count 2*#00 + #10 and missing count #01 over the first n
+// individuals (PCGsolver.cpp:711-724; excluding phantom pairs is what
+// subtractPhantomGenotypes :590-613 achieves). Integer exact.
+__global__ __launch_bounds__(kBlock) void k_bed_counts(const uint8_t* __restrict__ rows, int64_t m,
+                                                       int64_t pitch, int32_t n,
+                                                       int64_t* __restrict__ alt,
+                                                       int64_t* __restrict__ miss) {
+    __shared__ int sh_a[kBlock / 64], sh_m[kBlock / 64];
+    const int64_t nbytes = (n + 3) / 4;
+    for (int64_t s = blockIdx.x; s < m; s += gridDim.x) {
+        const uint8_t* row = rows + s * pitch;
+        int a = 0, mi = 0;
+        for (int64_t j = threadIdx.x; j < nbytes; j += kBlock) {
+            const uint32_t b = row[j];
+            const int64_t valid = n - 4 * j;
+            const uint32_t mask = valid >= 4 ? 0x55u : (0x55u & ((1u << (2 * valid)) - 1u));
+            const uint32_t L = b & 0x55u, H = (b >> 1) & 0x55u;
+            a += 2 * __popc(~L & ~H & mask) + __popc(H & ~L & mask);
+            mi += __popc(L & ~H & mask);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_down(a, o, 64);
+            mi += __shfl_down(mi, o, 64);
+        }
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) {
+            sh_a[threadIdx.x >> 6] = a;
+            sh_m[threadIdx.x >> 6] = mi;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int64_t ta = 0, tm = 0;
+            for (int w = 0; w < kBlock / 64; ++w) {
+                ta += sh_a[w];
+                tm += sh_m[w];
+            }
+            alt[s] = ta;
+            miss[s] = tm;
+        }
+    }
+}
+
+// Copy kept rows, force phantom codes of the last byte to 11 and fill the pitch
+// padding with 0xFF.
+__global__ __launch_bounds__(kBlock) void k_compact_rows(const uint8_t* __restrict__ src,
+                                                         int64_t pitch_src,
+                                                         const int64_t* __restrict__ idx,
+                                                         int64_t m_kept, int32_t n,
+                                                         uint8_t* __restrict__ dst,
+                                                         int64_t pitch_dst) {
+    const int64_t nbytes = (n + 3) / 4;
+    const int rem = n & 3;
+    const uint8_t phantom = rem == 0 ? 0 : (uint8_t)(0xFFu << (2 * rem));
+    for (int64_t r = blockIdx.x; r < m_kept; r += gridDim.x) {
+        const uint8_t* s = src + idx[r] * pitch_src;
+        uint8_t* d = dst + r * pitch_dst;
+        for (int64_t j = threadIdx.x; j < pitch_dst; j += kBlock) {
+            uint8_t v = 0xFF;
+            if (j < nbytes) {
+                v = s[j];
+                if (j == nbytes - 1) v |= phantom;
+            }
+            d[j] = v;
+        }
+    }
+}
+
+// SNP-major -> individual-major. One block per tile of 256 SNPs x 64 individuals:
+// rows of 16 bytes are staged in LDS, each thread emits 16 output bytes (64 SNPs)
+// of one individual.
+__global__ __launch_bounds__(kBlock) void k_transpose(const uint8_t* __restrict__ snp, int64_t m,
+                                                      int64_t pitch_s, int32_t n,
+                                                      uint8_t* __restrict__ ind, int64_t pitch_i) {
+    __shared__ uint8_t tile[256][17];
+    const int64_t s0 = (int64_t)blockIdx.x * 256;
+    const int64_t i0 = (int64_t)blockIdx.y * 64;
+    const int t = threadIdx.x;
+    {
+        const int64_t s = s0 + t;
+        uint4 v = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+        if (s < m) v = *reinterpret_cast<const uint4*>(snp + s * pitch_s + i0 / 4);
+        const uint8_t* vb = reinterpret_cast<const uint8_t*>(&v);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) tile[t][q] = vb[q];
+    }
+    __syncthreads();
+    const int il = t >> 2;         // individual within tile
+    const int part = t & 3;        // which 64-SNP group
+    const int64_t i = i0 + il;
+    if (i >= n) return;
+    uint8_t out[16];
+#pragma unroll
+    for (int ob = 0; ob < 16; ++ob) {
+        uint32_t byte = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int sl = part * 64 + ob * 4 + q;
+            const uint32_t code = (tile[sl][il >> 2] >> (2 * (il & 3))) & 3u;
+            byte |= code << (2 * q);
+        }
+        out[ob] = (uint8_t)byte;
+    }
+    *reinterpret_cast<uint4*>(ind + i * pitch_i + s0 / 4 + part * 16) =
+        *reinterpret_cast<const uint4*>(out);
+}
+
+// SplitMix64 finalizer (Steele, Lea & Flood, "Fast splittable pseudorandom number
+// generators", OOPSLA 2014) as a counter-based hash.
+__device__ inline uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Synthetic SNP-major rows: f_s ~ U(f_lo, f_hi); per genotype two Bernoulli(f_s)
+// draws (Binomial(2, f)) and a Bernoulli(miss) missing flag, from one 64-bit hash
+// split into three 21-bit uniforms.
+__global__ __launch_bounds__(kBlock) void k_synth_bed(uint8_t* __restrict__ rows, int64_t m,
+                                                      int64_t pitch, int32_t n, uint64_t seed,
+                                                      float f_lo, float f_hi, float miss) {
+    const int64_t nbytes = (n + 3) / 4;
+    const double inv21 = 1.0 / 2097152.0;
+    for (int64_t s = blockIdx.y; s < m; s += gridDim.y) {
+        const uint64_t hs = mix64(seed ^ (0xA5A5A5A5ull + (uint64_t)s * 0x100000001B3ull));
+        const double f = f_lo + (f_hi - f_lo) * ((double)(hs >> 11) * (1.0 / 9007199254740992.0));
+        uint8_t* row = rows + s * pitch;
+        for (int64_t j = blockIdx.x * (int64_t)kBlock + threadIdx.x; j < pitch;
+             j += (int64_t)gridDim.x * kBlock) {
+            uint32_t byte = 0xFF;
+            if (j < nbytes) {
+                byte = 0;
+                for (int q = 0; q < 4; ++q) {
+                    const int64_t i = 4 * j + q;
+                    uint32_t code = 3u;
+                    if (i < n) {
+                        const uint64_t h = mix64(hs + (uint64_t)i * 0x9E3779B97F4A7C15ull);
+                        const double u1 = (double)(h & 0x1FFFFF) * inv21;
+                        const double u2 = (double)((h >> 21) & 0x1FFFFF) * inv21;
+                        const double u3 = (double)((h >> 42) & 0x1FFFFF) * inv21;
+                        const int d = (u1 < f) + (u2 < f);
+                        code = d == 2 ? 0u : (d == 1 ? 2u : 3u);
+                        if (u3 < miss) code = 1u;
+                    }
+                    byte |= code << (2 * q);
+                }
+            }
+            row[j] = (uint8_t)byte;
+        }
+    }
+}
+
+// twof = 2 f, inv = 1/sqrt(2 f (1 - f)) or 1 when 2 f (1 - f) <= 1e-12 (FP64 from the
+// stored FP32 frequency; the reference evaluates the same formula in FP32,
+// PCGsolver.cpp:1159-1161).
+__global__ __launch_bounds__(kBlock) void k_snp_consts(const float* __restrict__ freq, int64_t m,
+                                                       double* __restrict__ twof,
+                                                       double* __restrict__ inv) {
+    for (int64_t s = blockIdx.x * (int64_t)kBlock + threadIdx.x; s < m;
+         s += (int64_t)gridDim.x * kBlock) {
+        const double f = (double)freq[s];
+        const double d = 2.0 * f * (1.0 - f);
+        twof[s] = 2.0 * f;
+        inv[s] = d > 1e-12 ? 1.0 / sqrt(d) : 1.0;
+    }
+}
+
+// diag(G)_i = (1/m) sum_s ((g_is - 2 f_s) inv_s)^2 (PCGsolver.cpp:959-987) from the
+// individual-major rows; the three possible values per SNP are staged in LDS.
+__global__ __launch_bounds__(kBlock) void k_gdiag(const uint8_t* __restrict__ ind, int64_t pitch_i,
+                                                  int32_t n, int64_t m,
+                                                  const double* __restrict__ twof,
+                                                  const double* __restrict__ inv,
+                                                  double* __restrict__ gdiag) {
+    __shared__ double cv[3][kGenoTile];
+    const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    const uint8_t* row = ind + i * pitch_i;
+    double acc = 0.0;
+    for (int64_t k0 = 0; k0 < m; k0 += kGenoTile) {
+        __syncthreads();
+        for (int q = threadIdx.x; q < kGenoTile; q += kBlock) {
+            const int64_t s = k0 + q;
+            double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+            if (s < m) {
+                const double a = twof[s], w = inv[s] * inv[s];
+                c0 = a * a * w;
+                c1 = (1.0 - a) * (1.0 - a) * w;
+                c2 = (2.0 - a) * (2.0 - a) * w;
+            }
+            cv[0][q] = c0;
+            cv[1][q] = c1;
+            cv[2][q] = c2;
+        }
+        __syncthreads();
+        if (i < n) {
+            const uint4* q4 = reinterpret_cast<const uint4*>(row + k0 / 4);
+#pragma unroll 2
+            for (int j = 0; j < kGenoTile / 64; ++j) {
+                const uint4 w = q4[j];
+                const uint32_t wd[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                for (int d = 0; d < 4; ++d)
+#pragma unroll
+                    for (int g = 0; g < 16; ++g) {
+                        const int kk = j * 64 + d * 16 + g;
+                        acc += cv[dosage((wd[d] >> (2 * g)) & 3u)][kk];
+                    }
+            }
+        }
+    }
+    if (i < n) gdiag[i] = m > 0 ? acc / (double)m : 0.0;
+}
+
+// out[c][r] = sum_k dosage(r, k) v[c][k] in FP64: one thread per packed row, the
+// K-tile of v staged in LDS and read as a broadcast by the whole block.
+template <int KC>
+__global__ __launch_bounds__(kBlock) void k_rowdot(const uint8_t* __restrict__ rows, int64_t pitch,
+                                                   int64_t nrows, int64_t nk,
+                                                   const double* __restrict__ v, int64_t ldv,
+                                                   double* __restrict__ out, int64_t ldo,
+                                                   const int* skip) {
+    if (skip && *skip) return;
+    __shared__ double sv[KC][kGenoTile];
+    const int64_t r = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    const uint8_t* row = rows + r * pitch;
+    double acc[KC];
+#pragma unroll
+    for (int c = 0; c < KC; ++c) acc[c] = 0.0;
+    for (int64_t k0 = 0; k0 < nk; k0 += kGenoTile) {
+        __syncthreads();
+        for (int q = threadIdx.x; q < KC * kGenoTile; q += kBlock) {
+            const int c = q / kGenoTile, kk = q % kGenoTile;
+            const int64_t kg = k0 + kk;
+            sv[c][kk] = kg < nk ? v[c * ldv + kg] : 0.0;
+        }
+        __syncthreads();
+        if (r < nrows) {
+            const uint4* q4 = reinterpret_cast<const uint4*>(row + k0 / 4);
+            for (int j = 0; j < kGenoTile / 64; ++j) {
+                const uint4 w = q4[j];
+                const uint32_t wd[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                for (int d = 0; d < 4; ++d)
+#pragma unroll
+                    for (int g = 0; g < 16; ++g) {
+                        const int kk = j * 64 + d * 16 + g;
+                        const double dv = (double)dosage((wd[d] >> (2 * g)) & 3u);
+#pragma unroll
+                        for (int c = 0; c < KC; ++c) acc[c] = fma(dv, sv[c][kk], acc[c]);
+                    }
+            }
+        }
+    }
+    if (r < nrows)
+#pragma unroll
+        for (int c = 0; c < KC; ++c) out[c * ldo + r] = acc[c];
+}
+
+__device__ inline double wave_sum_d(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+__device__ inline double block_sum_d(double v) {
+    __shared__ double sh[kBlock / 64];
+    v = wave_sum_d(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kBlock / 64; ++i) r += sh[i];
+    return r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_colsum_parts(int64_t rows, const double* __restrict__ v,
+                                                         int64_t ldv, double* part,
+                                                         const int* skip) {
+    if (skip && *skip) return;
+    const int c = blockIdx.y;
+    double acc = 0.0;
+    for (int64_t q = blockIdx.x * (int64_t)kBlock + threadIdx.x; q < rows;
+         q += (int64_t)gridDim.x * kBlock)
+        acc += v[c * ldv + q];
+    const double s = block_sum_d(acc);
+    if (threadIdx.x == 0) part[c * kMaxParts + blockIdx.x] = s;
+}
+
+// Pass-1 epilogue (centering as a rank-one correction of the uncentred product):
+//   sum_i w_is z_i = inv_s (P1_s - 2 f_s T), T = sum_i z_i;
+//   u_s = (sig2g / M) inv_s^2 (P1_s - 2 f_s T)  so that  GZ_i = sum_s g_is u_s - C,
+//   C = sum_s 2 f_s u_s (partials here).
+__global__ __launch_bounds__(kBlock) void k_center_snp(int64_t m, const double* __restrict__ P1,
+                                                       int64_t ld1, const double* __restrict__ T,
+                                                       const double* __restrict__ twof,
+                                                       const double* __restrict__ inv,
+                                                       const double* __restrict__ th,
+                                                       double inv_m, double* __restrict__ u,
+                                                       int64_t ldu, double* part_c,
+                                                       const int* skip) {
+    if (skip && *skip) return;
+    const int c = blockIdx.y;
+    const double scale = th[c * 5 + 1] * inv_m, tc = T[c];
+    double acc = 0.0;
+    for (int64_t s = blockIdx.x * (int64_t)kBlock + threadIdx.x; s < m;
+         s += (int64_t)gridDim.x * kBlock) {
+        const double iv = inv[s];
+        const double uv = scale * iv * iv * (P1[c * ld1 + s] - twof[s] * tc);
+        u[c * ldu + s] = uv;
+        acc += twof[s] * uv;
+    }
+    const double sum = block_sum_d(acc);
+    if (threadIdx.x == 0) part_c[c * kMaxParts + blockIdx.x] = sum;
+}
+
+__global__ __launch_bounds__(kBlock) void k_center_ind(int32_t n, const double* __restrict__ P2,
+                                                       int64_t ld2, const double* __restrict__ Cs,
+                                                       double* __restrict__ GZ, int64_t ldg,
+                                                       const int* skip) {
+    if (skip && *skip) return;
+    const int c = blockIdx.y;
+    const double cc = Cs[c];
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        GZ[c * ldg + i] = P2[c * ld2 + i] - cc;
+}
+
+}  // namespace
+
+#define LAUNCH_CHECK() HIP_CHECK(hipGetLastError())
+
+void launch_bed_counts(hipStream_t s, const uint8_t* rows, int64_t m, int64_t pitch, int32_t n,
+                       int64_t* alt, int64_t* miss) {
+    int64_t g = m < 65536 ? m : 65536;
+    if (g < 1) return;
+    k_bed_counts<<<(unsigned)g, kBlock, 0, s>>>(rows, m, pitch, n, alt, miss);
+    LAUNCH_CHECK();
+}
+
+void launch_compact_rows(hipStream_t s, const uint8_t* src, int64_t pitch_src, const int64_t* idx,
+                         int64_t m_kept, int32_t n, uint8_t* dst, int64_t pitch_dst) {
+    int64_t g = m_kept < 65536 ? m_kept : 65536;
+    if (g < 1) return;
+    k_compact_rows<<<(unsigned)g, kBlock, 0, s>>>(src, pitch_src, idx, m_kept, n, dst, pitch_dst);
+    LAUNCH_CHECK();
+}
+
+void launch_transpose_geno(hipStream_t s, const uint8_t* snp, int64_t m, int64_t pitch_s, int32_t n,
+                           uint8_t* ind, int64_t pitch_i) {
+    if (m == 0 || n == 0) return;
+    dim3 g((unsigned)((m + 255) / 256), (unsigned)((n + 63) / 64));
+    k_transpose<<<g, kBlock, 0, s>>>(snp, m, pitch_s, n, ind, pitch_i);
+    LAUNCH_CHECK();
+}
+
+void launch_synth_bed(hipStream_t s, uint8_t* rows, int64_t m, int64_t pitch, int32_t n,
+                      uint64_t seed, float f_lo, float f_hi, float miss) {
+    if (m == 0) return;
+    unsigned gx = (unsigned)((pitch + kBlock - 1) / kBlock);
+    unsigned gy = (unsigned)(m < 65535 ? m : 65535);
+    k_synth_bed<<<dim3(gx, gy), kBlock, 0, s>>>(rows, m, pitch, n, seed, f_lo, f_hi, miss);
+    LAUNCH_CHECK();
+}
+
+void launch_snp_consts(hipStream_t s, const float* freq, int64_t m, double* twof, double* inv) {
+    if (m == 0) return;
+    k_snp_consts<<<parts_for(m), kBlock, 0, s>>>(freq, m, twof, inv);
+    LAUNCH_CHECK();
+}
+
+void launch_gdiag(hipStream_t s, const uint8_t* ind, int64_t pitch_i, int32_t n, int64_t m,
+                  const double* twof, const double* inv, double* gdiag) {
+    k_gdiag<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(ind, pitch_i, n, m, twof, inv,
+                                                                     gdiag);
+    LAUNCH_CHECK();
+}
+
+void launch_rowdot(hipStream_t s, const uint8_t* rows, int64_t pitch, int64_t nrows, int64_t nk,
+                   const double* v, int64_t ldv, int k, double* out, int64_t ldo, const int* skip) {
+    if (nrows == 0) return;
+    const unsigned g = (unsigned)((nrows + kBlock - 1) / kBlock);
+    int c = 0;
+    while (c < k) {
+        const int left = k - c;
+        if (left >= 8) {
+            k_rowdot<8><<<g, kBlock, 0, s>>>(rows, pitch, nrows, nk, v + c * ldv, ldv, out + c * ldo, ldo, skip);
+            c += 8;
+        } else if (left >= 4) {
+            k_rowdot<4><<<g, kBlock, 0, s>>>(rows, pitch, nrows, nk, v + c * ldv, ldv, out + c * ldo, ldo, skip);
+            c += 4;
+        } else if (left >= 2) {
+            k_rowdot<2><<<g, kBlock, 0, s>>>(rows, pitch, nrows, nk, v + c * ldv, ldv, out + c * ldo, ldo, skip);
+            c += 2;
+        } else {
+            k_rowdot<1><<<g, kBlock, 0, s>>>(rows, pitch, nrows, nk, v + c * ldv, ldv, out + c * ldo, ldo, skip);
+            c += 1;
+        }
+        LAUNCH_CHECK();
+    }
+}
+
+void launch_colsum_parts(hipStream_t s, int64_t rows, const double* v, int64_t ldv, int k,
+                         double* part, const int* skip) {
+    dim3 g(parts_for(rows), k);
+    k_colsum_parts<<<g, kBlock, 0, s>>>(rows, v, ldv, part, skip);
+    LAUNCH_CHECK();
+}
+
+void launch_center_snp(hipStream_t s, int64_t m, int k, const double* P1, int64_t ld1,
+                       const double* T, const double* twof, const double* inv,
+                       const double* theta_cols, double inv_m, double* u, int64_t ldu,
+                       double* part_c, const int* skip) {
+    dim3 g(parts_for(m), k);
+    k_center_snp<<<g, kBlock, 0, s>>>(m, P1, ld1, T, twof, inv, theta_cols, inv_m, u, ldu, part_c,
+                                      skip);
+    LAUNCH_CHECK();
+}
+
+void launch_center_ind(hipStream_t s, int32_t n, int k, const double* P2, int64_t ld2,
+                       const double* Csum, double* GZ, int64_t ldg, const int* skip) {
+    dim3 g(parts_for(n), k);
+    k_center_ind<<<g, kBlock, 0, s>>>(n, P2, ld2, Csum, GZ, ldg, skip);
+    LAUNCH_CHECK();
+}
+
+}  // namespace globe

@@ -1,0 +1,628 @@
+// Record-space kernels: CSR segmented sums over an individual's records, the
+// record-space output of V x, the per-individual Woodbury preconditioner, and the
+// fused PCG vector updates with deterministic two-stage reductions (per-block
+// partials in a fixed slot, then one ordered pass).
+//
+// Layout: record-space matrices column-major [k][nrec] (leading dimension ld);
+// individual-space matrices [k][n]; theta_cols is [k][5] = per column
+// {sig2e, sig2g, sig2b0, sigb01, sig2b1}.
+#include "globe_internal.h"
+
+namespace globe {
+
+namespace {
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// Sum over a 256-thread block, result valid in thread 0. Fixed order: deterministic.
+__device__ inline double block_sum(double v) {
+    __shared__ double sh[kBlock / 64];
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kBlock / 64; ++i) r += sh[i];
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// S0[c][i] = sum_{r in block} x[c][r], S1[c][i] = sum t_r x[c][r]
+// (PCGsolver.cpp:1119-1125 scatter, and the t-weighted sum the factorized time
+// term needs instead of the n_b^2 loop at :1237-1254).
+__global__ __launch_bounds__(kBlock) void k_seg_sums(int32_t nblk, const int64_t* __restrict__ off,
+                                                     const int32_t* __restrict__ ind,
+                                                     const double* __restrict__ t,
+                                                     const double* __restrict__ x, int64_t ldx,
+                                                     double* __restrict__ S0,
+                                                     double* __restrict__ S1, int64_t ldn,
+                                                     const int* skip) {
+    if (skip && *skip) return;
+    const int c = blockIdx.y;
+    const double* xc = x + c * ldx;
+    for (int64_t b = blockIdx.x * (int64_t)kBlock + threadIdx.x; b < nblk;
+         b += (int64_t)gridDim.x * kBlock) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int64_t r = off[b]; r < off[b + 1]; ++r) {
+            double v = xc[r];
+            s0 += v;
+            s1 += t[r] * v;
+        }
+        const int32_t i = ind[b];
+        S0[c * ldn + i] = s0;
+        if (S1) S1[c * ldn + i] = s1;
+    }
+}
+
+// y[c][r] = GZ_i + sig2b0 S0_i + sigb01 (t S0_i + S1_i) + sig2b1 t S1_i + sig2e x_r
+// (the factorized form of PCGsolver.cpp:1220-1265). Optional fused partial of
+// sum_r x[c][r] y[c][r] (the p . Vp of PCG, PCGsolver.cpp:1381).
+__global__ __launch_bounds__(kBlock) void k_record_out(
+    int64_t nrec, const int32_t* __restrict__ indiv, const double* __restrict__ t,
+    const double* __restrict__ x, int64_t ldx, const double* __restrict__ GZ,
+    const double* __restrict__ S0, const double* __restrict__ S1, int64_t ldn,
+    const double* __restrict__ th, double* __restrict__ y, int64_t ldy, double* part_dot,
+    const int* skip) {
+    if (skip && *skip) return;
+    const int c = blockIdx.y;
+    const double se = th[c * 5 + 0], sb0 = th[c * 5 + 2], sb01 = th[c * 5 + 3],
+                 sb1 = th[c * 5 + 4];
+    const double* xc = x + c * ldx;
+    double* yc = y + c * ldy;
+    const double* g = GZ + c * ldn;
+    const double* s0 = S0 + c * ldn;
+    const double* s1 = S1 + c * ldn;
+    double acc = 0.0;
+    for (int64_t r = blockIdx.x * (int64_t)kBlock + threadIdx.x; r < nrec;
+         r += (int64_t)gridDim.x * kBlock) {
+        const int32_t i = indiv[r];
+        const double tr = t[r], a = s0[i], b = s1[i], xv = xc[r];
+        const double v = g[i] + sb0 * a + sb01 * (tr * a + b) + sb1 * tr * b + se * xv;
+        yc[r] = v;
+        acc += xv * v;
+    }
+    if (part_dot) {
+        double s = block_sum(acc);
+        if (threadIdx.x == 0) part_dot[c * kMaxParts + blockIdx.x] = s;
+    }
+}
+
+// z = Q_i^-1 r per individual block by the 2x2 Woodbury form of
+// Q_i = sig2e I + U_i C_i U_i^T, U_i = [1, t], C_i = [[sig2g G_ii + sig2b0, sigb01],
+// [sigb01, sig2b1]] -- the same block the reference inverts explicitly
+// (PCGsolver.cpp:1026-1045, applied at :1060-1093):
+//   w = U^T r, c = (sig2e I + C U^T U)^-1 C w, z = (r - U c) / sig2e.
+// tilde_v = 1 uses G_ii := 1 (tilde-V, varCompEst.cpp:839-858).
+// Fused partials: sum r.z and sum r.r per column.
+__global__ __launch_bounds__(kBlock) void k_precond(
+    int32_t nblk, const int64_t* __restrict__ off, const int32_t* __restrict__ ind,
+    const double* __restrict__ t, const double* __restrict__ gdiag, const double* __restrict__ th,
+    const double* __restrict__ r, int64_t ldr, double* __restrict__ z, int64_t ldz,
+    double* part_rz, double* part_rr, const PcgState* st, int tilde_v) {
+    if (st && st->done) return;
+    const int c = blockIdx.y;
+    if (st && !st->active[c]) {
+        // frozen column: keep the partial slots defined
+        if (threadIdx.x == 0) {
+            if (part_rz) part_rz[c * kMaxParts + blockIdx.x] = 0.0;
+            if (part_rr) part_rr[c * kMaxParts + blockIdx.x] = 0.0;
+        }
+        return;
+    }
+    const double se = th[c * 5 + 0], sg = th[c * 5 + 1], sb0 = th[c * 5 + 2],
+                 sb01 = th[c * 5 + 3], sb1 = th[c * 5 + 4];
+    const double* rc = r + c * ldr;
+    double* zc = z + c * ldz;
+    const double inv_se = 1.0 / se;
+    double arz = 0.0, arr = 0.0;
+    for (int64_t b = blockIdx.x * (int64_t)kBlock + threadIdx.x; b < nblk;
+         b += (int64_t)gridDim.x * kBlock) {
+        const int64_t r0 = off[b], r1 = off[b + 1];
+        double a01 = 0.0, a11 = 0.0, w0 = 0.0, w1 = 0.0;
+        for (int64_t q = r0; q < r1; ++q) {
+            const double tq = t[q], rv = rc[q];
+            a01 += tq;
+            a11 += tq * tq;
+            w0 += rv;
+            w1 += tq * rv;
+        }
+        const double a00 = (double)(r1 - r0);
+        const double gv = sg * (tilde_v ? 1.0 : gdiag[ind[b]]);
+        const double c00 = gv + sb0, c01 = sb01, c11 = sb1;
+        // M2 = se I + C A (A = U^T U symmetric)
+        const double m00 = se + c00 * a00 + c01 * a01;
+        const double m01 = c00 * a01 + c01 * a11;
+        const double m10 = c01 * a00 + c11 * a01;
+        const double m11 = se + c01 * a01 + c11 * a11;
+        const double v0 = c00 * w0 + c01 * w1;
+        const double v1 = c01 * w0 + c11 * w1;
+        const double det = m00 * m11 - m01 * m10;
+        const double cc0 = (m11 * v0 - m01 * v1) / det;
+        const double cc1 = (m00 * v1 - m10 * v0) / det;
+        for (int64_t q = r0; q < r1; ++q) {
+            const double rv = rc[q];
+            const double zv = (rv - cc0 - cc1 * t[q]) * inv_se;
+            zc[q] = zv;
+            arz += rv * zv;
+            arr += rv * rv;
+        }
+    }
+    if (part_rz) {
+        double s = block_sum(arz);
+        if (threadIdx.x == 0) part_rz[c * kMaxParts + blockIdx.x] = s;
+    }
+    if (part_rr) {
+        double s = block_sum(arr);
+        if (threadIdx.x == 0) part_rr[c * kMaxParts + blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot_parts(int64_t nrec, const double* __restrict__ a,
+                                                      int64_t lda, const double* __restrict__ b,
+                                                      int64_t ldb, double* part) {
+    const int c = blockIdx.y;
+    const double* ac = a + c * lda;
+    const double* bc = b + c * ldb;
+    double acc = 0.0;
+    for (int64_t r = blockIdx.x * (int64_t)kBlock + threadIdx.x; r < nrec;
+         r += (int64_t)gridDim.x * kBlock)
+        acc += ac[r] * bc[r];
+    double s = block_sum(acc);
+    if (threadIdx.x == 0) part[c * kMaxParts + blockIdx.x] = s;
+}
+
+// out[c] = sum_j part[c][j], j ascending (one thread per column).
+__global__ void k_reduce_parts(const double* __restrict__ part, int nparts, int k, double* out,
+                               const int* skip) {
+    if (skip && *skip) return;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= k) return;
+    double s = 0.0;
+    for (int j = 0; j < nparts; ++j) s += part[c * kMaxParts + j];
+    out[c] = s;
+}
+
+// A^T B partials: part[(a*kb + b)][block]
+__global__ __launch_bounds__(kBlock) void k_gram_parts(int64_t nrec, const double* __restrict__ A,
+                                                       int64_t lda, const double* __restrict__ B,
+                                                       int64_t ldb, int kb, double* part) {
+    const int a = blockIdx.y / kb, bb = blockIdx.y % kb;
+    const double* ac = A + a * lda;
+    const double* bc = B + bb * ldb;
+    double acc = 0.0;
+    for (int64_t r = blockIdx.x * (int64_t)kBlock + threadIdx.x; r < nrec;
+         r += (int64_t)gridDim.x * kBlock)
+        acc += ac[r] * bc[r];
+    double s = block_sum(acc);
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.y * kMaxParts + blockIdx.x] = s;
+}
+
+// y[:, j] -= sum_q x[:, q] coef[q * ky + j]   (V^-1 r = V^-1 y - V^-1 X beta, P f)
+__global__ __launch_bounds__(kBlock) void k_axpy_cols(int64_t nrec, double* __restrict__ y,
+                                                      int64_t ldy, const double* __restrict__ x,
+                                                      int64_t ldx, int kx,
+                                                      const double* __restrict__ coef, int ky) {
+    const int j = blockIdx.y;
+    for (int64_t r = blockIdx.x * (int64_t)kBlock + threadIdx.x; r < nrec;
+         r += (int64_t)gridDim.x * kBlock) {
+        double v = y[j * ldy + r];
+        for (int q = 0; q < kx; ++q) v -= x[q * ldx + r] * coef[q * ky + j];
+        y[j * ldy + r] = v;
+    }
+}
+
+// ---------------------------------------------------------------- PCG control
+// PCG per PCGsolver.cpp:1343-1443 / varCompEst.cpp:482-527 with every scalar on
+// the device; once `done` is set every later kernel of the solve is a no-op, so the
+// host may enqueue iterations ahead of the convergence test.
+__global__ void k_pcg_init(PcgState* st, const double* __restrict__ part_bb,
+                           const double* __restrict__ part_rr, const double* __restrict__ part_rz,
+                           int nparts, int k) {
+    const int c = threadIdx.x;
+    __shared__ double ratio_s[kMaxCols];
+    __shared__ int act_s[kMaxCols];
+    if (c < k) {
+        double bb = 0.0, rr = 0.0, rz = 0.0;
+        for (int j = 0; j < nparts; ++j) {
+            bb += part_bb[c * kMaxParts + j];
+            rr += part_rr[c * kMaxParts + j];
+            rz += part_rz[c * kMaxParts + j];
+        }
+        const double nb = sqrt(bb), nr = sqrt(rr);
+        st->bnorm[c] = nb;
+        st->rs[c] = rz;
+        st->alpha[c] = 0.0;
+        st->beta[c] = 0.0;
+        st->col_iters[c] = 0;
+        st->ratio[c] = nb > 0.0 ? nr / nb : 0.0;
+        ratio_s[c] = nb > 1e-12 ? nr / nb : 0.0;
+        int act = 1;
+        if (st->rule == GLOBE_RULE_PER_COLUMN && (nb <= 1e-12 || nr < st->tol * nb)) act = 0;
+        st->active[c] = act;
+        act_s[c] = act;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double last = 0.0;
+        int any = 0;
+        for (int j = 0; j < k; ++j) {
+            if (ratio_s[j] > last) last = ratio_s[j];
+            any |= act_s[j];
+        }
+        st->last = last;
+        st->max_ratio = last;
+        st->iters = 0;
+        st->stag = 0;
+        st->stalled = 0;
+        st->done = (st->rule == GLOBE_RULE_PER_COLUMN && !any) || st->max_iter <= 0;
+    }
+}
+
+__global__ void k_pcg_alpha(PcgState* st, const double* __restrict__ part_pvp, int nparts, int k) {
+    if (st->done) return;
+    const int c = threadIdx.x;
+    if (c >= k) return;
+    if (!st->active[c]) {
+        st->alpha[c] = 0.0;
+        return;
+    }
+    double s = 0.0;
+    for (int j = 0; j < nparts; ++j) s += part_pvp[c * kMaxParts + j];
+    st->alpha[c] = st->rs[c] / s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcg_xr(PcgState* st, int64_t nrec, double* __restrict__ x,
+                                                   int64_t ldx, double* __restrict__ r,
+                                                   const double* __restrict__ p,
+                                                   const double* __restrict__ vp, int64_t ld,
+                                                   double* part_rr) {
+    if (st->done) return;
+    const int c = blockIdx.y;
+    const double a = st->alpha[c];
+    const bool act = st->active[c] != 0;
+    double acc = 0.0;
+    for (int64_t q = blockIdx.x * (int64_t)kBlock + threadIdx.x; q < nrec;
+         q += (int64_t)gridDim.x * kBlock) {
+        double rv = r[c * ld + q];
+        if (act) {
+            x[c * ldx + q] += a * p[c * ld + q];
+            rv -= a * vp[c * ld + q];
+            r[c * ld + q] = rv;
+        }
+        acc += rv * rv;
+    }
+    double s = block_sum(acc);
+    if (threadIdx.x == 0) part_rr[c * kMaxParts + blockIdx.x] = s;
+}
+
+// Convergence test after the x/r update of one iteration.
+//   rule 0 (PCGsolver.cpp:1388-1415): max ratio over columns with |b| > 1e-12 below tol;
+//          stall exit when the max ratio improves by < 5e-3 of its last value for 10
+//          consecutive iterations.
+//   rule 1 (varCompEst.cpp:505): every column |r| < tol |b|.
+//   rule 2: per-column freeze; done when no column is active.
+__global__ void k_pcg_check(PcgState* st, const double* __restrict__ part_rr, int nparts, int k) {
+    if (st->done) return;
+    __shared__ double nr_s[kMaxCols];
+    const int c = threadIdx.x;
+    if (c < k) {
+        double s = 0.0;
+        for (int j = 0; j < nparts; ++j) s += part_rr[c * kMaxParts + j];
+        nr_s[c] = sqrt(s);
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const int it = st->iters + 1;
+    int done = 0;
+    double maxr = 0.0;
+    if (st->rule == GLOBE_RULE_MAX_RATIO) {
+        for (int j = 0; j < k; ++j) {
+            const double nb = st->bnorm[j];
+            const double ratio = nb > 1e-12 ? nr_s[j] / nb : 0.0;
+            st->ratio[j] = ratio;
+            if (ratio > maxr) maxr = ratio;
+        }
+        if (maxr < st->tol) {
+            done = 1;
+        } else if (st->stagnation) {
+            const double impr = st->last - maxr, req = 5e-3 * st->last;
+            st->stag = impr < req ? st->stag + 1 : 0;
+            st->last = maxr;
+            if (st->stag >= 10) {
+                done = 1;
+                st->stalled = 1;
+            }
+        } else {
+            st->last = maxr;
+        }
+        for (int j = 0; j < k; ++j) st->col_iters[j] = it;
+    } else if (st->rule == GLOBE_RULE_ALL_COLUMNS) {
+        int all = 1;
+        for (int j = 0; j < k; ++j) {
+            const double nb = st->bnorm[j];
+            if (!(nr_s[j] < st->tol * nb)) all = 0;
+            const double ratio = nb > 0.0 ? nr_s[j] / nb : 0.0;
+            st->ratio[j] = ratio;
+            if (ratio > maxr) maxr = ratio;
+            st->col_iters[j] = it;
+        }
+        done = all;
+    } else {
+        int any = 0;
+        for (int j = 0; j < k; ++j) {
+            const double nb = st->bnorm[j];
+            const double ratio = nb > 1e-12 ? nr_s[j] / nb : 0.0;
+            if (st->active[j]) {
+                st->ratio[j] = nb > 0.0 ? nr_s[j] / nb : 0.0;
+                st->col_iters[j] = it;
+                if (nr_s[j] < st->tol * nb) st->active[j] = 0;
+            }
+            if (ratio > maxr) maxr = ratio;
+            any |= st->active[j];
+        }
+        done = !any;
+    }
+    st->max_ratio = maxr;
+    st->iters = it;
+    if (it >= st->max_iter) done = 1;
+    st->done = done;
+}
+
+__global__ void k_pcg_beta(PcgState* st, const double* __restrict__ part_rz, int nparts, int k) {
+    if (st->done) return;
+    const int c = threadIdx.x;
+    if (c >= k || !st->active[c]) return;
+    double s = 0.0;
+    for (int j = 0; j < nparts; ++j) s += part_rz[c * kMaxParts + j];
+    st->beta[c] = s / st->rs[c];
+    st->rs[c] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcg_p(PcgState* st, int64_t nrec, double* __restrict__ p,
+                                                  const double* __restrict__ z, int64_t ld) {
+    if (st->done) return;
+    const int c = blockIdx.y;
+    if (!st->active[c]) return;
+    const double b = st->beta[c];
+    for (int64_t q = blockIdx.x * (int64_t)kBlock + threadIdx.x; q < nrec;
+         q += (int64_t)gridDim.x * kBlock)
+        p[c * ld + q] = z[c * ld + q] + b * p[c * ld + q];
+}
+
+// bhat[2i] = sum v_r (sig2b0 + sigb01 t_r), bhat[2i+1] = sum v_r (sigb01 + sig2b1 t_r)
+// (PCGsolver.cpp:1491-1504), one thread per individual: fixed summation order.
+__global__ __launch_bounds__(kBlock) void k_bhat(int32_t nblk, const int64_t* __restrict__ off,
+                                                 const int32_t* __restrict__ ind,
+                                                 const double* __restrict__ t,
+                                                 const double* __restrict__ v,
+                                                 const double* __restrict__ th,
+                                                 double* __restrict__ bhat) {
+    const double sb0 = th[2], sb01 = th[3], sb1 = th[4];
+    for (int64_t b = blockIdx.x * (int64_t)kBlock + threadIdx.x; b < nblk;
+         b += (int64_t)gridDim.x * kBlock) {
+        double u0 = 0.0, u1 = 0.0;
+        for (int64_t q = off[b]; q < off[b + 1]; ++q) {
+            u0 += v[q] * (sb0 + sb01 * t[q]);
+            u1 += v[q] * (sb01 + sb1 * t[q]);
+        }
+        const int32_t i = ind[b];
+        bhat[2 * i] = u0;
+        bhat[2 * i + 1] = u1;
+    }
+}
+
+// f_theta for ncols P-vectors (varCompEst.cpp:693-760): f[(5*j + q)][r], q = 0..4
+// from py[j] and uhat[j].
+__global__ __launch_bounds__(kBlock) void k_f_theta(int32_t nblk, const int64_t* __restrict__ off,
+                                                    const int32_t* __restrict__ ind,
+                                                    const double* __restrict__ t,
+                                                    const double* __restrict__ py, int64_t ldpy,
+                                                    const double* __restrict__ uhat, int64_t ldu,
+                                                    double inv_sg, double* __restrict__ f,
+                                                    int64_t ldf) {
+    const int j = blockIdx.y;
+    const double* pj = py + j * ldpy;
+    const double* uj = uhat + j * ldu;
+    double* fj = f + (int64_t)5 * j * ldf;
+    for (int64_t b = blockIdx.x * (int64_t)kBlock + threadIdx.x; b < nblk;
+         b += (int64_t)gridDim.x * kBlock) {
+        double w = 0.0, tw = 0.0;
+        for (int64_t q = off[b]; q < off[b + 1]; ++q) {
+            w += pj[q];
+            tw += pj[q] * t[q];
+        }
+        const double u = uj[ind[b]] * inv_sg;
+        for (int64_t q = off[b]; q < off[b + 1]; ++q) {
+            const double tq = t[q];
+            fj[0 * ldf + q] = pj[q];
+            fj[1 * ldf + q] = u;
+            fj[2 * ldf + q] = w;
+            fj[3 * ldf + q] = tq * w + tw;
+            fj[4 * ldf + q] = tw * tq;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy_cols(int64_t rows, const double* __restrict__ src,
+                                                      int64_t lds, double* __restrict__ dst,
+                                                      int64_t ldd) {
+    const int c = blockIdx.y;
+    for (int64_t q = blockIdx.x * (int64_t)kBlock + threadIdx.x; q < rows;
+         q += (int64_t)gridDim.x * kBlock)
+        dst[c * ldd + q] = src[c * lds + q];
+}
+
+// d[c][q] = a[c][q] - b[c][q]  (r = b - V x0)
+__global__ __launch_bounds__(kBlock) void k_sub_cols(int64_t rows, const double* __restrict__ a,
+                                                     int64_t lda, const double* __restrict__ b,
+                                                     int64_t ldb, double* __restrict__ d,
+                                                     int64_t ldd) {
+    const int c = blockIdx.y;
+    for (int64_t q = blockIdx.x * (int64_t)kBlock + threadIdx.x; q < rows;
+         q += (int64_t)gridDim.x * kBlock)
+        d[c * ldd + q] = a[c * lda + q] - b[c * ldb + q];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_cols(int64_t rows, double* __restrict__ d,
+                                                      int64_t ldd, double v) {
+    const int c = blockIdx.y;
+    for (int64_t q = blockIdx.x * (int64_t)kBlock + threadIdx.x; q < rows;
+         q += (int64_t)gridDim.x * kBlock)
+        d[c * ldd + q] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(double* dst, int64_t count, double v) {
+    for (int64_t q = blockIdx.x * (int64_t)kBlock + threadIdx.x; q < count;
+         q += (int64_t)gridDim.x * kBlock)
+        dst[q] = v;
+}
+
+}  // namespace
+
+#define LAUNCH_CHECK() HIP_CHECK(hipGetLastError())
+
+void launch_seg_sums(hipStream_t s, int32_t nblk, const int64_t* blk_off, const int32_t* blk_ind,
+                     const double* times, const double* x, int64_t ldx, int k, double* S0,
+                     double* S1, int64_t ldn, const int* skip) {
+    dim3 g(parts_for(nblk), k);
+    k_seg_sums<<<g, kBlock, 0, s>>>(nblk, blk_off, blk_ind, times, x, ldx, S0, S1, ldn, skip);
+    LAUNCH_CHECK();
+}
+
+void launch_record_out(hipStream_t s, int64_t nrec, const int32_t* indiv, const double* times,
+                       const double* x, int64_t ldx, int k, const double* GZ, const double* S0,
+                       const double* S1, int64_t ldn, const double* theta_cols, double* y,
+                       int64_t ldy, double* part_dot, const int* skip) {
+    dim3 g(parts_for(nrec), k);
+    k_record_out<<<g, kBlock, 0, s>>>(nrec, indiv, times, x, ldx, GZ, S0, S1, ldn, theta_cols, y,
+                                      ldy, part_dot, skip);
+    LAUNCH_CHECK();
+}
+
+void launch_precond(hipStream_t s, int32_t nblk, const int64_t* blk_off, const int32_t* blk_ind,
+                    const double* times, const double* gdiag, const double* theta_cols,
+                    const double* r, int64_t ldr, int k, double* z, int64_t ldz, double* part_rz,
+                    double* part_rr, const PcgState* st, int tilde_v) {
+    dim3 g(parts_for(nblk), k);
+    k_precond<<<g, kBlock, 0, s>>>(nblk, blk_off, blk_ind, times, gdiag, theta_cols, r, ldr, z,
+                                   ldz, part_rz, part_rr, st, tilde_v);
+    LAUNCH_CHECK();
+}
+
+void launch_dot_parts(hipStream_t s, int64_t nrec, const double* a, int64_t lda, const double* b,
+                      int64_t ldb, int k, double* part) {
+    dim3 g(parts_for(nrec), k);
+    k_dot_parts<<<g, kBlock, 0, s>>>(nrec, a, lda, b, ldb, part);
+    LAUNCH_CHECK();
+}
+
+void launch_reduce_parts(hipStream_t s, const double* part, int nparts, int k, double* out) {
+    k_reduce_parts<<<(k + 63) / 64, 64, 0, s>>>(part, nparts, k, out, nullptr);
+    LAUNCH_CHECK();
+}
+
+void launch_reduce_parts_skip(hipStream_t s, const double* part, int nparts, int k, double* out,
+                              const int* skip) {
+    k_reduce_parts<<<(k + 63) / 64, 64, 0, s>>>(part, nparts, k, out, skip);
+    LAUNCH_CHECK();
+}
+
+void launch_gram(hipStream_t s, int64_t nrec, const double* A, int64_t lda, int ka,
+                 const double* B, int64_t ldb, int kb, double* part, double* out) {
+    const int np = parts_for(nrec);
+    dim3 g(np, ka * kb);
+    k_gram_parts<<<g, kBlock, 0, s>>>(nrec, A, lda, B, ldb, kb, part);
+    LAUNCH_CHECK();
+    k_reduce_parts<<<(ka * kb + 63) / 64, 64, 0, s>>>(part, np, ka * kb, out, nullptr);
+    LAUNCH_CHECK();
+}
+
+void launch_axpy_cols(hipStream_t s, int64_t nrec, double* y, int64_t ldy, const double* x,
+                      int64_t ldx, int kx, const double* coef, int ky) {
+    dim3 g(parts_for(nrec), ky);
+    k_axpy_cols<<<g, kBlock, 0, s>>>(nrec, y, ldy, x, ldx, kx, coef, ky);
+    LAUNCH_CHECK();
+}
+
+void launch_pcg_init(hipStream_t s, PcgState* st, const double* part_bb, const double* part_rr,
+                     const double* part_rz, int nparts, int k) {
+    k_pcg_init<<<1, kMaxCols, 0, s>>>(st, part_bb, part_rr, part_rz, nparts, k);
+    LAUNCH_CHECK();
+}
+
+void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* part_pvp, int nparts, int k) {
+    k_pcg_alpha<<<1, kMaxCols, 0, s>>>(st, part_pvp, nparts, k);
+    LAUNCH_CHECK();
+}
+
+void launch_pcg_xr(hipStream_t s, PcgState* st, int64_t nrec, double* x, int64_t ldx, double* r,
+                   const double* p, const double* vp, int64_t ld, int k, double* part_rr) {
+    dim3 g(parts_for(nrec), k);
+    k_pcg_xr<<<g, kBlock, 0, s>>>(st, nrec, x, ldx, r, p, vp, ld, part_rr);
+    LAUNCH_CHECK();
+}
+
+void launch_pcg_check(hipStream_t s, PcgState* st, const double* part_rr, int nparts, int k) {
+    k_pcg_check<<<1, kMaxCols, 0, s>>>(st, part_rr, nparts, k);
+    LAUNCH_CHECK();
+}
+
+void launch_pcg_beta(hipStream_t s, PcgState* st, const double* part_rz, int nparts, int k) {
+    k_pcg_beta<<<1, kMaxCols, 0, s>>>(st, part_rz, nparts, k);
+    LAUNCH_CHECK();
+}
+
+void launch_pcg_p(hipStream_t s, PcgState* st, int64_t nrec, double* p, const double* z,
+                  int64_t ld, int k) {
+    dim3 g(parts_for(nrec), k);
+    k_pcg_p<<<g, kBlock, 0, s>>>(st, nrec, p, z, ld);
+    LAUNCH_CHECK();
+}
+
+void launch_bhat(hipStream_t s, int32_t nblk, const int64_t* blk_off, const int32_t* blk_ind,
+                 const double* times, const double* v, const double* theta, double* bhat) {
+    k_bhat<<<parts_for(nblk), kBlock, 0, s>>>(nblk, blk_off, blk_ind, times, v, theta, bhat);
+    LAUNCH_CHECK();
+}
+
+void launch_f_theta(hipStream_t s, int32_t nblk, const int64_t* blk_off, const int32_t* blk_ind,
+                    const double* times, const double* py, int64_t ldpy, const double* uhat,
+                    int64_t ldu, int ncols, double inv_sg, double* f, int64_t ldf) {
+    dim3 g(parts_for(nblk), ncols);
+    k_f_theta<<<g, kBlock, 0, s>>>(nblk, blk_off, blk_ind, times, py, ldpy, uhat, ldu, inv_sg, f,
+                                   ldf);
+    LAUNCH_CHECK();
+}
+
+void launch_copy_cols(hipStream_t s, int64_t rows, const double* src, int64_t lds, double* dst,
+                      int64_t ldd, int k) {
+    dim3 g(parts_for(rows), k);
+    k_copy_cols<<<g, kBlock, 0, s>>>(rows, src, lds, dst, ldd);
+    LAUNCH_CHECK();
+}
+
+void launch_sub_cols(hipStream_t s, int64_t rows, const double* a, int64_t lda, const double* b,
+                     int64_t ldb, double* d, int64_t ldd, int k) {
+    dim3 g(parts_for(rows), k);
+    k_sub_cols<<<g, kBlock, 0, s>>>(rows, a, lda, b, ldb, d, ldd);
+    LAUNCH_CHECK();
+}
+
+void launch_fill_cols(hipStream_t s, int64_t rows, double* d, int64_t ldd, int k, double v) {
+    dim3 g(parts_for(rows), k);
+    k_fill_cols<<<g, kBlock, 0, s>>>(rows, d, ldd, v);
+    LAUNCH_CHECK();
+}
+
+void launch_fill(hipStream_t s, double* dst, int64_t count, double v) {
+    k_fill<<<parts_for(count), kBlock, 0, s>>>(dst, count, v);
+    LAUNCH_CHECK();
+}
+
+}  // namespace globe

@@ -1,0 +1,176 @@
+/*
+ * globe.h -- C ABI of libglobe_amd.so, the MI355X (gfx950) matrix-free LGMM solve.
+ *
+ * Plain C types only: host buffers are caller-owned and copied on entry; the
+ * *_dev entry points take device pointers (allocated with globe_dev_alloc or any
+ * HIP allocation on the context's device). Matrices are column-major with an
+ * explicit leading dimension (the reference keeps Eigen column-major float
+ * matrices); record-space matrices have nrec rows, individual-space ones n rows.
+ *
+ * One context per host thread (not re-entrant); all device work of a context is
+ * ordered on one HIP stream. Every call returns a globe_status; on failure
+ * globe_last_error() describes it. Nothing here exits the process (the
+ * reference calls exit(1) on bad input; the CLIs built on this ABI keep that).
+ *
+ * Each declaration cites the reference interface it replaces (ahmadazim/GLOBE).
+ */
+#ifndef GLOBE_H
+#define GLOBE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GLOBE_ABI_VERSION 1
+
+typedef struct globe_ctx globe_ctx;
+
+typedef enum {
+    GLOBE_OK = 0,
+    GLOBE_E_INVALID = 1,  /* bad argument, shape or cohort layout */
+    GLOBE_E_STATE = 2,    /* call order (no cohort / genetics / variance components) */
+    GLOBE_E_DEVICE = 3,   /* HIP runtime error or no usable device */
+    GLOBE_E_NOMEM = 4,    /* device allocation failed */
+    GLOBE_E_NUMERIC = 5   /* singular system */
+} globe_status;
+
+/* globe_vx modes */
+enum { GLOBE_VX_FULL = 0, GLOBE_VX_GRM_ONLY = 1 };
+/* Arithmetic of the packed-genotype contraction W (W^T z); everything else is FP64.
+ *   FP64:      int8 matrix cores on 7 int8 digits of a 54-bit block-fixed-point z,
+ *              exact int32 accumulation -- as accurate as an FP64 dot product;
+ *   MIXED:     the same with 4 digits (30-bit fixed point);
+ *   FP64_VALU: FP64 fused multiply-adds on the vector ALUs (validation path).
+ * The GRM product always accumulates in FP64. */
+enum { GLOBE_PREC_FP64 = 0, GLOBE_PREC_MIXED = 1, GLOBE_PREC_FP64_VALU = 2 };
+/* PCG stopping rules */
+enum {
+    GLOBE_RULE_MAX_RATIO = 0,   /* runPCG_multi: max_j |r_j|/|b_j| < tol (+ optional stall exit) */
+    GLOBE_RULE_ALL_COLUMNS = 1, /* runPCG_multi_VC: every |r_j| < tol |b_j| */
+    GLOBE_RULE_PER_COLUMN = 2   /* per-column masking: converged columns freeze */
+};
+/* PCG starting points */
+enum { GLOBE_X0_ZERO = 0, GLOBE_X0_PRECOND = 1, GLOBE_X0_GIVEN = 2 };
+
+typedef struct {
+    double tol;         /* reference: 1e-5 hard-coded (PCGsolver.cpp:54) */
+    int32_t max_iter;   /* reference: 1000 (PCGsolver.cpp:55) */
+    int32_t x0_mode;    /* GLOBE_X0_* */
+    int32_t rule;       /* GLOBE_RULE_* */
+    int32_t stagnation; /* rule 0 only: 10-iteration / 5e-3 stall exit (PCGsolver.cpp:1374-1415) */
+    int32_t precision;  /* GLOBE_PREC_* */
+    int32_t reserved;
+} globe_pcg_opts;
+
+typedef struct {
+    int32_t iters;      /* iterations run */
+    int32_t stalled;    /* 1 if the stall exit ended the solve */
+    double max_ratio;   /* max_j |r_j| / |b_j| at exit */
+    double seconds;     /* wall time of the solve on the device stream */
+} globe_pcg_stats;
+
+/* ---- context ------------------------------------------------------------------ */
+/* No reference counterpart (the reference is one host process without a device). */
+int globe_create(int device, globe_ctx** out);
+void globe_destroy(globe_ctx* ctx);
+const char* globe_last_error(const globe_ctx* ctx);
+int globe_abi_version(void);
+int globe_device_count(void);
+
+/* ---- cohort ------------------------------------------------------------------- */
+/* Replaces readInputData (PCGsolver.cpp:529-587) + block detection
+ * (computePreconditioner_Block, :1004-1011). indiv: 0-based individual per record,
+ * records of one individual contiguous, every index 0..n-1 present (the reference
+ * silently mis-computes otherwise; here it is GLOBE_E_INVALID). X: (p+1) columns,
+ * TIME last, leading dimension ldx >= nrec. */
+int globe_set_cohort(globe_ctx* ctx, int64_t nrec, int32_t n, int32_t p,
+                     const int32_t* indiv, const double* times, const double* X, int64_t ldx);
+
+/* ---- genetics ------------------------------------------------------------------ */
+/* Replaces prepGenotypeInput (PCGsolver.cpp:615-763): SNP-major PLINK rows
+ * (m rows of `pitch` >= ceil(n/4) bytes, the .bed body after its 3 magic bytes) are
+ * staged to HBM, counted on the device (bit-exact A1 / missing counts), filtered with
+ * the reference's FP32 MAF / missing-rate rule and kept resident in two layouts. */
+int globe_load_bed(globe_ctx* ctx, const uint8_t* rows, int64_t m, int64_t pitch,
+                   float min_maf, float max_miss, int64_t* m_kept);
+/* Per input SNP of the last globe_load_bed / globe_synth_bed: A1 allele count, missing
+ * count, keep flag and FP32 A1 frequency (PCGsolver.cpp:711-734). Any pointer may be NULL. */
+int globe_get_qc(globe_ctx* ctx, int64_t* alt, int64_t* miss, uint8_t* keep, float* maf);
+/* Kept-SNP allele frequencies (float, as GenotypeData::allele_freqs, PCGsolver.cpp:64). */
+int globe_get_freq(globe_ctx* ctx, float* freq);
+/* Replaces readFullGRM (PCGsolver.cpp:778-809) / readGRM (varCompEst.cpp:270-295):
+ * n(n+1)/2 float32 lower triangle, row-major (GCTA .grm.bin). */
+int globe_load_grm(globe_ctx* ctx, const float* grm_lt);
+/* Replaces computeGeneticDiagonal (PCGsolver.cpp:925-989): diag(G), n doubles. */
+int globe_genetic_diag(globe_ctx* ctx, double* out);
+
+/* ---- variance components ------------------------------------------------------------- */
+/* Replaces SolvingData's sig2e..sig2b1 (PCGsolver.cpp:47-52) and the rebuild of the
+ * block preconditioner (computePreconditioner_Block :991-1058, computePreconditioner
+ * varCompEst.cpp:355-415). theta = {sig2e, sig2g, sig2b0, sigb01, sig2b1}. */
+int globe_set_varcomp(globe_ctx* ctx, const double theta[5]);
+
+/* ---- operator ----------------------------------------------------------------------- */
+/* Replaces computeVx_multi (PCGsolver.cpp:1095-1266) and computeVx_multi_VC
+ * (varCompEst.cpp:322-352). mode GLOBE_VX_FULL: Y (nrec x k) = V X;
+ * GLOBE_VX_GRM_ONLY (multGRM_uhat): Y (n x k) = sig2g G J^T X. */
+int globe_vx(globe_ctx* ctx, const double* X, int64_t ldx, int k, double* Y, int64_t ldy,
+             int mode, int precision);
+int globe_vx_dev(globe_ctx* ctx, const double* X, int64_t ldx, int k, double* Y, int64_t ldy,
+                 int mode, int precision);
+/* Replaces applyPreconditioner (PCGsolver.cpp:1060-1093, varCompEst.cpp:432-450):
+ * Z = Q^-1 R per individual block, all k columns. */
+int globe_precond_apply(globe_ctx* ctx, const double* R, int64_t ldr, int k, double* Z, int64_t ldz);
+
+/* ---- solver --------------------------------------------------------------------------- */
+/* Replaces runPCG_multi (PCGsolver.cpp:1324-1444) and runPCG_multi_VC
+ * (varCompEst.cpp:453-528). Solves V X = B for k columns; X holds x0 on entry when
+ * x0_mode == GLOBE_X0_GIVEN. col_ratio (k) and col_iters (k) may be NULL. */
+int globe_pcg(globe_ctx* ctx, const double* B, int64_t ldb, int k, double* X, int64_t ldx,
+              const globe_pcg_opts* opts, globe_pcg_stats* stats, double* col_ratio,
+              int32_t* col_iters);
+int globe_pcg_dev(globe_ctx* ctx, const double* B, int64_t ldb, int k, double* X, int64_t ldx,
+                  const globe_pcg_opts* opts, globe_pcg_stats* stats, double* col_ratio,
+                  int32_t* col_iters);
+
+/* ---- estimators ---------------------------------------------------------------------- */
+/* Replaces runPCGsolver's main body (PCGsolver.cpp:262-302): PCG on [X | y], GLS
+ * beta (p+1), V^-1 r (nrec), uhat = sig2g G J^T V^-1 r (n, compute_uhat :1447-1473),
+ * bhat (2n interleaved [b0, b1], compute_bhat :1475-1513). Output pointers may be NULL. */
+int globe_gls_blup(globe_ctx* ctx, const double* y, const globe_pcg_opts* opts, double* beta,
+                   double* vinv_r, double* uhat, double* bhat, globe_pcg_stats* stats);
+/* Replaces computeDerivative_first + estimateTrace_PdV + computeDerivative_AImat
+ * (varCompEst.cpp:980-1041) at the current variance components: fitMME (x0 = Q^-1 b,
+ * all-columns rule), f_theta, trace of P dV_theta from nmc injected probes
+ * (probes nrec x nmc; weights nrec x nmc: tilde-V^-1 probe for the reference's
+ * Gaussian tilde-V probes, the probe itself for Rademacher probes; probe_x0 nrec x nmc
+ * PCG starting point or NULL for Q^-1 probe), gradient (5) and AI matrix (5x5,
+ * row-major). first_terms / trace (5 each) may be NULL. */
+int globe_aireml_eval(globe_ctx* ctx, const double* y, const double* probes,
+                      const double* weights, const double* probe_x0, int nmc,
+                      const globe_pcg_opts* opts, double* grad, double* ai,
+                      double* first_terms, double* trace);
+
+/* ---- device memory and synthetic cohorts (no reference counterpart) ---------------- */
+/* Device buffers for the *_dev entry points and for measurement with inputs resident in HBM. */
+int globe_dev_alloc(globe_ctx* ctx, int64_t bytes, void** ptr);
+int globe_dev_free(globe_ctx* ctx, void* ptr);
+int globe_memcpy_h2d(globe_ctx* ctx, void* dst, const void* src, int64_t bytes);
+int globe_memcpy_d2h(globe_ctx* ctx, void* dst, const void* src, int64_t bytes);
+int globe_synchronize(globe_ctx* ctx);
+/* Generates an m-SNP x n-individual PLINK cohort directly in HBM (counter-based hash
+ * RNG, seeded): f_s ~ U(f_lo, f_hi), dosage ~ Binomial(2, f_s), missing w.p. miss;
+ * then runs the same QC and staging as globe_load_bed. For cohorts too large to
+ * write to disk; the generator never runs inside a timed region. */
+int globe_synth_bed(globe_ctx* ctx, int64_t m, uint64_t seed, float f_lo, float f_hi,
+                    float miss, float min_maf, float max_miss, int64_t* m_kept);
+/* out (n) = sum_s w_is a_s: standardized kept genotypes times an SNP-space vector
+ * (the synthetic phenotype recipe's u = sqrt(sig2g / M) W a). */
+int globe_geno_Wa(globe_ctx* ctx, const double* a, double* out);
+/* Raw rows of the resident SNP-major layout (pitch bytes each) for checks. */
+int globe_get_geno_rows(globe_ctx* ctx, int64_t first, int64_t count, uint8_t* out, int64_t* pitch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,267 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use it as the checker. The product path never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build(force: bool = False) -> str:
+    path = os.path.join(_HERE, "liboracle.so")
+    if force or not os.path.exists(path):
+        subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle.so"])
+    return path
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = C.CDLL(build())
+        _declare(_LIB)
+    return _LIB
+
+
+class OrcModel(C.Structure):
+    _fields_ = [
+        ("nrec", C.c_int64), ("n", C.c_int32), ("p", C.c_int32),
+        ("indiv", C.c_void_p), ("times", C.c_void_p), ("X", C.c_void_p),
+        ("use_grm", C.c_int32), ("geno", C.c_void_p), ("m", C.c_int64), ("pitch", C.c_int64),
+        ("freq", C.c_void_p), ("grm_lt", C.c_void_p), ("theta", C.c_double * 5),
+    ]
+
+
+class PcgOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("max_iter", C.c_int32), ("x0_mode", C.c_int32),
+                ("rule", C.c_int32), ("stagnation", C.c_int32)]
+
+
+class PcgStats(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("stalled", C.c_int32), ("max_ratio", C.c_double)]
+
+
+P = C.c_void_p
+
+
+def _declare(L):
+    L.orc_decode.argtypes = [C.c_uint8]
+    L.orc_decode.restype = C.c_int
+    L.orc_bed_counts.argtypes = [P, C.c_int64, C.c_int32, C.c_int64, P, P]
+    L.orc_bed_qc.argtypes = [P, C.c_int64, C.c_int32, C.c_int64, C.c_float, C.c_float, P, P, P]
+    L.orc_bed_qc.restype = C.c_int64
+    L.orc_inv_std.argtypes = [C.c_float]
+    L.orc_inv_std.restype = C.c_double
+    L.orc_genetic_diag.argtypes = [P, P]
+    L.orc_vx.argtypes = [P, P, C.c_int, P, C.c_int]
+    L.orc_vx_dense.argtypes = [P, P, C.c_int, P]
+    L.orc_geno_Wa.argtypes = [P, P, P]
+    L.orc_vx_ref32.argtypes = [P, P, C.c_int, P, C.c_int]
+    L.orc_blocks.argtypes = [P, P]
+    L.orc_blocks.restype = C.c_int64
+    L.orc_precond_size.argtypes = [P]
+    L.orc_precond_size.restype = C.c_int64
+    L.orc_precond_build.argtypes = [P, P, P, C.c_int]
+    L.orc_precond_apply.argtypes = [P, P, P, C.c_int, P]
+    L.orc_pcg.argtypes = [P, P, P, C.c_int, P, P, P, P, P]
+    L.orc_inverse.argtypes = [C.c_int, P, P]
+    L.orc_inverse.restype = C.c_int
+    L.orc_gls_blup.argtypes = [P, P, P, P, P, P, P, P, P, P, P]
+    L.orc_bhat.argtypes = [P, P, P]
+    L.orc_f_theta.argtypes = [P, P, P, P]
+    L.orc_tildeV_solve.argtypes = [P, P, C.c_int, P]
+    L.orc_sample_tildeV.argtypes = [P, P, P, P, P]
+    L.orc_aireml_eval.argtypes = [P, P, P, P, P, C.c_int, P, P, P, P, P]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _cm(a):
+    """[rows, k] -> column-major contiguous buffer (Fortran order)."""
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+class Oracle:
+    """FP64 oracle model over one cohort + genetics (genotypes or GRM)."""
+
+    def __init__(self, indiv, times, X, theta, *, geno_rows=None, freq=None, n=None,
+                 grm_lt=None):
+        self.indiv = np.ascontiguousarray(indiv, dtype=np.int32)
+        self.times = np.ascontiguousarray(times, dtype=np.float64)
+        self.X = _cm(X)
+        self.nrec = int(self.indiv.shape[0])
+        self.n = int(n if n is not None else self.indiv.max() + 1)
+        self.p = int(self.X.shape[1] - 1)
+        m = OrcModel()
+        m.nrec, m.n, m.p = self.nrec, self.n, self.p
+        m.indiv, m.times, m.X = _ptr(self.indiv), _ptr(self.times), _ptr(self.X)
+        if grm_lt is not None:
+            self.grm_lt = np.ascontiguousarray(grm_lt, dtype=np.float32)
+            m.use_grm = 1
+            m.grm_lt = _ptr(self.grm_lt)
+        else:
+            self.geno = np.ascontiguousarray(geno_rows, dtype=np.uint8)
+            self.freq = np.ascontiguousarray(freq, dtype=np.float32)
+            m.use_grm = 0
+            m.geno = _ptr(self.geno)
+            m.m = int(self.geno.shape[0])
+            m.pitch = int(self.geno.shape[1]) if self.geno.ndim == 2 else (self.n + 3) // 4
+            m.freq = _ptr(self.freq)
+        self._m = m
+        self.set_theta(theta)
+
+    @property
+    def ref(self):
+        return C.byref(self._m)
+
+    def set_theta(self, theta):
+        for i in range(5):
+            self._m.theta[i] = float(theta[i])
+        self.theta = tuple(float(t) for t in theta)
+
+    def genetic_diag(self):
+        out = np.zeros(self.n)
+        lib().orc_genetic_diag(self.ref, _ptr(out))
+        return out
+
+    def vx(self, x, grm_only=False):
+        x = _cm(x if np.ndim(x) == 2 else np.asarray(x)[:, None])
+        k = x.shape[1]
+        y = np.zeros((self.n if grm_only else self.nrec, k), order="F")
+        lib().orc_vx(self.ref, _ptr(x), k, _ptr(y), int(grm_only))
+        return y
+
+    def vx_dense(self, x):
+        x = _cm(x if np.ndim(x) == 2 else np.asarray(x)[:, None])
+        k = x.shape[1]
+        y = np.zeros((self.nrec, k), order="F")
+        lib().orc_vx_dense(self.ref, _ptr(x), k, _ptr(y))
+        return y
+
+    def vx_ref32(self, x, grm_only=False):
+        x = np.asfortranarray(x if np.ndim(x) == 2 else np.asarray(x)[:, None], dtype=np.float32)
+        k = x.shape[1]
+        y = np.zeros((self.n if grm_only else self.nrec, k), order="F", dtype=np.float32)
+        lib().orc_vx_ref32(self.ref, _ptr(x), k, _ptr(y), int(grm_only))
+        return y
+
+    def geno_Wa(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        out = np.zeros(self.n)
+        lib().orc_geno_Wa(self.ref, _ptr(a), _ptr(out))
+        return out
+
+    def precond(self, no_grm=False):
+        gd = self.genetic_diag()
+        q = np.zeros(max(1, lib().orc_precond_size(self.ref)))
+        lib().orc_precond_build(self.ref, _ptr(gd), _ptr(q), int(no_grm))
+        return q
+
+    def precond_apply(self, q, r):
+        r = _cm(r if np.ndim(r) == 2 else np.asarray(r)[:, None])
+        z = np.zeros_like(r, order="F")
+        lib().orc_precond_apply(self.ref, _ptr(q), _ptr(r), r.shape[1], _ptr(z))
+        return z
+
+    def pcg(self, b, *, tol=1e-5, max_iter=1000, x0_mode=0, rule=0, stagnation=True,
+            x0=None, q=None):
+        b = _cm(b if np.ndim(b) == 2 else np.asarray(b)[:, None])
+        k = b.shape[1]
+        q = self.precond() if q is None else q
+        x = np.zeros_like(b, order="F") if x0 is None else _cm(x0).copy(order="F")
+        o = PcgOpts(tol, max_iter, x0_mode, rule, int(stagnation))
+        st = PcgStats()
+        ratio = np.zeros(k)
+        its = np.zeros(k, dtype=np.int32)
+        lib().orc_pcg(self.ref, _ptr(q), _ptr(b), k, _ptr(x), C.byref(o), C.byref(st),
+                      _ptr(ratio), _ptr(its))
+        return x, {"iters": st.iters, "stalled": st.stalled, "max_ratio": st.max_ratio,
+                   "col_ratio": ratio, "col_iters": its}
+
+    def gls_blup(self, y, *, tol=1e-5, max_iter=1000, x0_mode=0, rule=0, stagnation=True):
+        q = self.precond()
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        beta = np.zeros(self.p + 1)
+        vr = np.zeros(self.nrec)
+        uh = np.zeros(self.n)
+        bh = np.zeros(2 * self.n)
+        vx = np.zeros((self.nrec, self.p + 1), order="F")
+        xi = np.zeros((self.p + 1, self.p + 1))
+        o = PcgOpts(tol, max_iter, x0_mode, rule, int(stagnation))
+        st = PcgStats()
+        lib().orc_gls_blup(self.ref, _ptr(q), _ptr(y), C.byref(o), _ptr(beta), _ptr(vr),
+                           _ptr(uh), _ptr(bh), _ptr(vx), _ptr(xi), C.byref(st))
+        return {"beta": beta, "vinv_r": vr, "uhat": uh, "bhat": bh, "vinv_x": vx,
+                "xtvx_inv": xi, "iters": st.iters, "stalled": st.stalled}
+
+    def f_theta(self, py, uhat):
+        f = np.zeros((self.nrec, 5), order="F")
+        lib().orc_f_theta(self.ref, _ptr(np.ascontiguousarray(py, dtype=np.float64)),
+                          _ptr(np.ascontiguousarray(uhat, dtype=np.float64)), _ptr(f))
+        return f
+
+    def tildeV_solve(self, yv):
+        yv = _cm(yv if np.ndim(yv) == 2 else np.asarray(yv)[:, None])
+        out = np.zeros_like(yv, order="F")
+        lib().orc_tildeV_solve(self.ref, _ptr(yv), yv.shape[1], _ptr(out))
+        return out
+
+    def sample_tildeV(self, z1, z2, z3):
+        out = np.zeros(self.nrec)
+        lib().orc_sample_tildeV(self.ref, _ptr(np.ascontiguousarray(z1, dtype=np.float64)),
+                                _ptr(np.ascontiguousarray(z2, dtype=np.float64)),
+                                _ptr(np.ascontiguousarray(z3, dtype=np.float64)), _ptr(out))
+        return out
+
+    def aireml_eval(self, y, probes, weights, probe_x0=None, *, tol=1e-5, max_iter=1000):
+        probes = _cm(probes)
+        weights = _cm(weights)
+        nmc = probes.shape[1]
+        x0 = None if probe_x0 is None else _cm(probe_x0)
+        grad = np.zeros(5)
+        ai = np.zeros(25)
+        first = np.zeros(5)
+        tr = np.zeros(5)
+        o = PcgOpts(tol, max_iter, 1, 1, 0)
+        lib().orc_aireml_eval(self.ref, _ptr(np.ascontiguousarray(y, dtype=np.float64)),
+                              _ptr(probes), _ptr(weights), _ptr(x0), nmc, C.byref(o),
+                              _ptr(grad), _ptr(ai), _ptr(first), _ptr(tr))
+        return {"grad": grad, "ai": ai.reshape(5, 5), "first": first, "trace": tr}
+
+
+def oracle_from_synth(d, theta=None) -> Oracle:
+    c = d.cohort
+    theta = d.theta if theta is None else theta
+    if d.grm_lt is not None:
+        return Oracle(c.indiv, c.times, c.X, theta, grm_lt=d.grm_lt, n=c.n)
+    rows = d.bed_rows[d.keep]
+    return Oracle(c.indiv, c.times, c.X, theta, geno_rows=rows, freq=d.freq, n=c.n)
+
+
+def bed_counts(rows, n):
+    rows = np.ascontiguousarray(rows, dtype=np.uint8)
+    m = rows.shape[0]
+    alt = np.zeros(m, dtype=np.int64)
+    miss = np.zeros(m, dtype=np.int64)
+    lib().orc_bed_counts(_ptr(rows), m, n, rows.shape[1], _ptr(alt), _ptr(miss))
+    return alt, miss
+
+
+def bed_qc(rows, n, min_maf, max_miss):
+    rows = np.ascontiguousarray(rows, dtype=np.uint8)
+    m = rows.shape[0]
+    keep = np.zeros(m, dtype=np.uint8)
+    maf = np.zeros(m, dtype=np.float32)
+    mr = np.zeros(m, dtype=np.float32)
+    kept = lib().orc_bed_qc(_ptr(rows), m, n, rows.shape[1], min_maf, max_miss,
+                            _ptr(keep), _ptr(maf), _ptr(mr))
+    return keep.astype(bool), maf, mr, int(kept)

@@ -1,0 +1,227 @@
+"""HIP path vs the FP64 oracle on the same seeded inputs (calls through the C ABI).
+
+Bars: bit-exact for decode / counts / QC / staged rows; <= 1e-12 relative for the
+FP64 operator pieces; <= 1e-7 relative for FP64 solutions at a PCG tolerance of
+1e-9 (the oracle runs the same control flow); mixed precision within 1e-6 of the
+FP64 solution.
+"""
+import numpy as np
+import pytest
+
+import globe_amd as G
+from globe_amd import synth
+from helpers import golden_oracle, load_golden, rel
+
+pytestmark = pytest.mark.gpu
+
+GENO = ["geno_n203", "geno_n150_odd", "geno_qc_n257"]
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+def device_for(d, ctx_factory, theta=None):
+    g = ctx_factory()
+    c = d["cohort"]
+    g.set_cohort(c.indiv, c.times, c.X)
+    if "grm_lt" in d:
+        g.readFullGRM(d["grm_lt"])
+    else:
+        g.prepGenotypeInput(d["bed_rows"], float(d["min_maf"]), float(d["max_miss"]))
+    g.set_varcomp(tuple(d["theta"]) if theta is None else theta)
+    return g
+
+
+@pytest.mark.parametrize("name", GENO)
+def test_qc_and_staging_bit_exact(name, gpu_ctx_factory):
+    _say("qc", name)
+    d = load_golden(name)
+    g = device_for(d, gpu_ctx_factory)
+    n = d["cohort"].n
+    keep, maf, _, alt, miss = synth.qc_reference(d["bed_rows"], n, float(d["min_maf"]), float(d["max_miss"]))
+    q = g.qc()
+    assert np.array_equal(q["alt"], alt) and np.array_equal(q["miss"], miss)
+    assert np.array_equal(q["keep"], keep) and np.array_equal(q["maf"], maf)
+    assert g.m == int(d["ref_kept"]) == int(keep.sum())
+    assert np.array_equal(g.allele_freqs(), maf[keep])
+    rows = g.geno_rows()
+    nb = (n + 3) // 4
+    kept_rows = d["bed_rows"][keep]
+    assert np.array_equal(synth.unpack_codes(rows[:, :nb], n), synth.unpack_codes(kept_rows, n))
+    # phantom pairs and padding hold code 11 (dosage 0)
+    codes = synth.unpack_codes(rows, rows.shape[1] * 4)
+    assert (codes[:, n:] == 3).all()
+
+
+@pytest.mark.parametrize("n", [1, 3, 5, 130])
+def test_qc_tiny_and_ragged(n, gpu_ctx_factory):
+    _say("qc-ragged", n)
+    rng = np.random.default_rng(n)
+    c = synth.simulate_cohort(n, 1.0, rng)
+    rows = rng.integers(0, 256, size=(29, (n + 3) // 4), dtype=np.uint8)
+    g = gpu_ctx_factory()
+    g.set_cohort(c.indiv, c.times, c.X)
+    g.prepGenotypeInput(rows, 0.0, 1.0)
+    keep, maf, _, alt, miss = synth.qc_reference(rows, n, 0.0, 1.0)
+    q = g.qc()
+    assert np.array_equal(q["alt"], alt) and np.array_equal(q["miss"], miss)
+    assert np.array_equal(q["keep"], keep)
+
+
+@pytest.mark.parametrize("name", GENO + ["grm_n180"])
+def test_genetic_diag(name, gpu_ctx_factory):
+    _say("gdiag", name)
+    d = load_golden(name)
+    g = device_for(d, gpu_ctx_factory)
+    assert rel(g.computeGeneticDiagonal(), golden_oracle(d).genetic_diag()) < 1e-13
+
+
+@pytest.mark.parametrize("name", GENO + ["grm_n180"])
+@pytest.mark.parametrize("prec", [G.PREC_FP64, G.PREC_FP64_VALU])
+def test_vx_fp64(name, prec, gpu_ctx_factory):
+    _say("vx", name, prec)
+    d = load_golden(name)
+    g = device_for(d, gpu_ctx_factory)
+    orc = golden_oracle(d)
+    c = d["cohort"]
+    x = np.random.default_rng(3).standard_normal((c.nrec, 5))
+    assert rel(g.computeVx_multi(x, precision=prec), orc.vx(x)) < 1e-12
+    assert rel(g.computeVx_multi(x, multGRM_uhat=True, precision=prec), orc.vx(x, grm_only=True)) < 1e-12
+    x1 = x[:, :1] * 1e3 + 7.0  # large mean: centering cancels
+    assert rel(g.computeVx_multi(x1, precision=prec), orc.vx(x1)) < 1e-12
+
+
+@pytest.mark.parametrize("k", [1, 3, 8, 32, 40])
+def test_vx_mixed_and_wide(k, gpu_ctx_factory):
+    _say("vx-mixed", k)
+    d = load_golden("geno_n203")
+    g = device_for(d, gpu_ctx_factory)
+    orc = golden_oracle(d)
+    x = np.random.default_rng(k).standard_normal((d["cohort"].nrec, k))
+    ref = orc.vx(x)
+    assert rel(g.computeVx_multi(x, precision=G.PREC_FP64), ref) < 1e-12
+    e = rel(g.computeVx_multi(x, precision=G.PREC_MIXED), ref)
+    assert e < 1e-7, e
+
+
+def test_precond_matches_dense_inverse(gpu_ctx_factory):
+    _say("precond")
+    for name in ["geno_n150_odd", "grm_n180"]:
+        d = load_golden(name)
+        g = device_for(d, gpu_ctx_factory)
+        orc = golden_oracle(d)
+        r = np.random.default_rng(1).standard_normal((d["cohort"].nrec, 4))
+        assert rel(g.applyPreconditioner(r), orc.precond_apply(orc.precond(), r)) < 1e-11
+
+
+@pytest.mark.parametrize("rule", [G.RULE_MAX_RATIO, G.RULE_ALL_COLUMNS, G.RULE_PER_COLUMN])
+@pytest.mark.parametrize("x0_mode", [G.X0_ZERO, G.X0_PRECOND])
+def test_pcg_matches_oracle(rule, x0_mode, gpu_ctx_factory):
+    _say("pcg", rule, x0_mode)
+    d = load_golden("geno_n203")
+    g = device_for(d, gpu_ctx_factory)
+    orc = golden_oracle(d)
+    c = d["cohort"]
+    b = np.column_stack([c.X, d["y"]])
+    xg, sg = g.pcg(b, tol=1e-9, rule=rule, x0_mode=x0_mode, stagnation=False)
+    xo, so = orc.pcg(b, tol=1e-9, rule=rule, x0_mode=x0_mode, stagnation=False)
+    assert rel(xg, xo) < 1e-7
+    assert abs(sg["iters"] - so["iters"]) <= 1
+    # the reference's own tolerance and stall rule: same iteration count
+    xg, sg = g.runPCG_multi(b, tol=1e-5)
+    xo, so = orc.pcg(b, tol=1e-5, rule=0, stagnation=True)
+    assert sg["iters"] == so["iters"] and rel(xg, xo) < 1e-9
+
+
+def test_pcg_given_x0_and_stall(gpu_ctx_factory):
+    _say("pcg-x0-stall")
+    d = load_golden("geno_n150_odd")
+    c = d["cohort"]
+    g = device_for(d, gpu_ctx_factory)
+    orc = golden_oracle(d)
+    b = np.column_stack([c.X, d["y"]])
+    x0 = np.random.default_rng(2).standard_normal(b.shape) * 0.01
+    xg, _ = g.pcg(b, tol=1e-10, x0_mode=G.X0_GIVEN, x0=x0, rule=G.RULE_ALL_COLUMNS, stagnation=False)
+    xo, _ = orc.pcg(b, tol=1e-10, x0_mode=2, x0=x0, rule=1, stagnation=False)
+    assert rel(xg, xo) < 1e-8
+    th = (1e-9, 10.0, 1e-4, 0.0, 1e-4)
+    g.set_varcomp(th)
+    ill = golden_oracle(d, theta=th)
+    _, sg = g.pcg(b, tol=1e-14, max_iter=2000, rule=G.RULE_MAX_RATIO, stagnation=True)
+    _, so = ill.pcg(b, tol=1e-14, max_iter=2000, rule=0, stagnation=True)
+    assert sg["stalled"] == 1 and so["stalled"] == 1
+    _, sg = g.pcg(b, tol=1e-14, max_iter=37, rule=G.RULE_MAX_RATIO, stagnation=False)
+    assert sg["iters"] == 37
+
+
+@pytest.mark.parametrize("name", GENO + ["grm_n180"])
+def test_gls_blup_fp64_matches_oracle(name, gpu_ctx_factory):
+    _say("gls", name)
+    d = load_golden(name)
+    g = device_for(d, gpu_ctx_factory)
+    orc = golden_oracle(d)
+    rg = g.gls_blup(d["y"], tol=1e-9, stagnation=False)
+    ro = orc.gls_blup(d["y"], tol=1e-9, stagnation=False)
+    for key in ("beta", "uhat", "bhat", "vinv_r"):
+        assert rel(rg[key], ro[key]) < 1e-7, key
+    # and the compiled reference's FP32 outputs to its own tolerance
+    rg5 = g.gls_blup(d["y"], tol=1e-5)
+    assert rel(rg5["beta"], d["ref_beta"]) < 5e-4
+    assert rel(rg5["uhat"], d["ref_uhat"]) < 5e-4
+    assert rel(rg5["bhat"], d["ref_bhat"]) < 5e-4
+
+
+def test_mixed_solve_within_1e6_of_fp64(gpu_ctx_factory):
+    _say("mixed-solve")
+    d = load_golden("geno_qc_n257")
+    g = device_for(d, gpu_ctx_factory)
+    c = d["cohort"]
+    rng = np.random.default_rng(7)
+    b = np.column_stack([d["y"], rng.choice([-1.0, 1.0], size=(c.nrec, 31))])
+    x64, s64 = g.pcg(b, tol=1e-9, rule=G.RULE_PER_COLUMN, precision=G.PREC_FP64)
+    xm, sm = g.pcg(b, tol=1e-9, rule=G.RULE_PER_COLUMN, precision=G.PREC_MIXED)
+    errs = np.linalg.norm(xm - x64, axis=0) / np.linalg.norm(x64, axis=0)
+    assert errs.max() < 1e-6, errs.max()
+
+
+def test_aireml_eval_matches_oracle(gpu_ctx_factory):
+    _say("aireml-eval")
+    d = load_golden("grm_n180")
+    th = (0.5, 0.5, 0.3, 0.05, 0.3)
+    g = device_for(d, gpu_ctx_factory, theta=th)
+    orc = golden_oracle(d, theta=th)
+    c = d["cohort"]
+    rng = np.random.default_rng(11)
+    probes = rng.choice([-1.0, 1.0], size=(c.nrec, 7))
+    og = g.aireml_eval(d["y"], probes, probes, tol=1e-10)
+    oo = orc.aireml_eval(d["y"], probes, probes, tol=1e-10)
+    for key in ("first", "trace", "grad", "ai"):
+        assert rel(og[key], oo[key]) < 1e-6, key
+    # reference-style Gaussian tilde-V probes with x0 = weights = tilde-V^-1 probe
+    z1, z2, z3 = rng.standard_normal(c.n), rng.standard_normal(2 * c.n), rng.standard_normal(c.nrec)
+    yv = orc.sample_tildeV(z1, z2, z3)[:, None]
+    wv = orc.tildeV_solve(yv)
+    og = g.aireml_eval(d["y"], yv, wv, wv, tol=1e-10)
+    oo = orc.aireml_eval(d["y"], yv, wv, wv, tol=1e-10)
+    assert rel(og["grad"], oo["grad"]) < 1e-6 and rel(og["ai"], oo["ai"]) < 1e-6
+
+
+def test_invalid_cohorts_rejected(gpu_ctx_factory):
+    _say("invalid")
+    g = gpu_ctx_factory()
+    t = np.zeros(4)
+    X = np.ones((4, 2))
+    with pytest.raises(G.GlobeError):  # individual 0 appears in two runs
+        g.set_cohort(np.array([0, 1, 0, 1]), t, X)
+    ind = np.array([0, 0, 1, 3], dtype=np.int32)  # id 3 outside [0, n = 3)
+    Xf = np.asfortranarray(X)
+    assert g.L.globe_set_cohort(g.h, 4, 3, 1, ind.ctypes.data, t.ctypes.data, Xf.ctypes.data, 4) == 1
+    ind = np.array([0, 0, 2, 2], dtype=np.int32)  # n = 3 but id 1 never occurs
+    assert g.L.globe_set_cohort(g.h, 4, 3, 1, ind.ctypes.data, t.ctypes.data, Xf.ctypes.data, 4) == 1
+    g.set_cohort(np.array([0, 0, 1, 1], dtype=np.int32), t, X)
+    with pytest.raises(G.GlobeError):  # sig2e must be positive
+        g.set_varcomp((0.0, 0.1, 0.1, 0.0, 0.1))
+    with pytest.raises(G.GlobeError):  # no genetics yet
+        g.set_varcomp((0.5, 0.1, 0.1, 0.0, 0.1))
+        g.computeVx_multi(np.ones(4))
