@@ -691,7 +691,7 @@ int globe_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int6
 }  // extern "C"
 
 // ------------------------------------------------------------------------ estimators
-namespace {
+namespace globe {
 
 struct Fit {
     std::vector<double> beta, xtvx_inv;  // (p+1), (p+1)^2 row-major
@@ -761,7 +761,7 @@ void apply_P(globe_ctx* c, const Fit& f, double* sv, int ncols) {
     HIP_CHECK(hipStreamSynchronize(s));
 }
 
-}  // namespace
+}  // namespace globe
 
 extern "C" {
 
@@ -801,8 +801,28 @@ int globe_gls_blup(globe_ctx* c, const double* y, const globe_pcg_opts* opts, do
 int globe_aireml_eval(globe_ctx* c, const double* y, const double* probes, const double* weights,
                       const double* probe_x0, int nmc, const globe_pcg_opts* opts, double* grad,
                       double* ai, double* first_terms, double* trace) {
+    if (!c) return GLOBE_E_INVALID;
+    if (!y || !grad || !ai) {
+        c->err = "globe_aireml_eval: null argument";
+        return GLOBE_E_INVALID;
+    }
+    return globe::aireml_eval_impl(c, y, probes, weights, probe_x0, nmc, opts, true, grad, ai,
+                                   first_terms, trace);
+}
+
+}  // extern "C"
+
+namespace globe {
+
+// computeDerivative_first + estimateTrace_PdV + computeDerivative_AImat
+// (varCompEst.cpp:980-1041). y == nullptr reuses the fit, f_theta and X^T V^-1 X of
+// the previous call at the same variance components (the AI matrix of an accepted
+// step, varCompEst.cpp:1321-1330, needs no new fit).
+int aireml_eval_impl(globe_ctx* c, const double* y, const double* probes, const double* weights,
+                     const double* probe_x0, int nmc, const globe_pcg_opts* opts, bool want_ai,
+                     double* grad, double* ai, double* first_terms, double* trace) {
     API_BEGIN(c)
-    require(y && opts && grad && ai, GLOBE_E_INVALID, "globe_aireml_eval: null argument");
+    require(opts != nullptr, GLOBE_E_INVALID, "globe_aireml_eval: null options");
     require(nmc >= 0 && nmc <= kMaxCols / 5, GLOBE_E_INVALID, "nmc must be in [0, 12]");
     require(nmc == 0 || (probes && weights), GLOBE_E_INVALID, "probes/weights required");
     require(c->has_cohort && c->gen != 0, GLOBE_E_STATE, "cohort and genetics required");
@@ -811,26 +831,36 @@ int globe_aireml_eval(globe_ctx* c, const double* y, const double* probes, const
     hipStream_t s = c->stream;
     const double sg = c->theta[1];
     require(sg > 0.0, GLOBE_E_INVALID, "sig2g must be positive for AIREML (f_theta divides by it)");
-    // 1. fitMME at theta (varCompEst.cpp:587-685): x0 = Q^-1 b
     globe_pcg_opts o = *opts;
     o.x0_mode = GLOBE_X0_PRECOND;
     c->e_probe.ensure(sizeof(double) * nrec * std::max(1, nmc));
     c->e_pv.ensure(sizeof(double) * nrec * std::max(5, nmc));
-    HIP_CHECK(hipMemcpy(c->e_probe.p, y, sizeof(double) * nrec, hipMemcpyHostToDevice));
     c->e_w.ensure(sizeof(double) * nrec);
-    double* py = c->e_w.as<double>();
-    Fit f = fit_gls(c, c->e_probe.as<double>(), o, py);
-    // 2. uhat and f_theta (varCompEst.cpp:693-760)
     c->e_us.ensure(sizeof(double) * (size_t)n * std::max(1, nmc));
-    op_vx(c, py, nrec, 1, c->e_us.as<double>(), n, GLOBE_VX_GRM_ONLY, o.precision, nullptr, nullptr);
     c->e_f.ensure(sizeof(double) * nrec * 5);
-    double* fth = c->e_f.as<double>();
-    launch_f_theta(s, n, c->d_blk_off.as<int64_t>(), c->d_blk_ind.as<int32_t>(), c->d_times.as<double>(),
-                   py, nrec, c->e_us.as<double>(), n, 1, 1.0 / sg, fth, nrec);
     c->e_small.ensure(sizeof(double) * kMaxCols * kMaxCols);
     c->w_part.ensure(sizeof(double) * kMaxParts * kMaxCols * kMaxCols);
+    double* py = c->e_w.as<double>();
+    double* fth = c->e_f.as<double>();
+    Fit f;
+    if (y) {
+        // 1. fitMME at theta (varCompEst.cpp:587-685): x0 = Q^-1 b
+        HIP_CHECK(hipMemcpy(c->e_probe.p, y, sizeof(double) * nrec, hipMemcpyHostToDevice));
+        f = fit_gls(c, c->e_probe.as<double>(), o, py);
+        // 2. uhat and f_theta (varCompEst.cpp:693-760)
+        op_vx(c, py, nrec, 1, c->e_us.as<double>(), n, GLOBE_VX_GRM_ONLY, o.precision, nullptr, nullptr);
+        launch_f_theta(s, n, c->d_blk_off.as<int64_t>(), c->d_blk_ind.as<int32_t>(),
+                       c->d_times.as<double>(), py, nrec, c->e_us.as<double>(), n, 1, 1.0 / sg, fth, nrec);
+        c->fit_xtvx_inv = f.xtvx_inv;
+        c->fit_theta = c->theta;
+        c->fit_valid = true;
+    } else {
+        require(c->fit_valid && c->fit_theta == c->theta, GLOBE_E_STATE,
+                "no fit at the current variance components to reuse");
+        f.xtvx_inv = c->fit_xtvx_inv;
+    }
+    double first[5] = {0, 0, 0, 0, 0};
     launch_gram(s, nrec, py, nrec, 1, fth, nrec, 5, c->w_part.as<double>(), c->e_small.as<double>());
-    double first[5];
     HIP_CHECK(hipMemcpyAsync(first, c->e_small.p, sizeof(double) * 5, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     // 3. trace terms (varCompEst.cpp:816-977) with injected probes
@@ -853,11 +883,10 @@ int globe_aireml_eval(globe_ctx* c, const double* y, const double* probes, const
         double* fs = c->e_fs.as<double>();
         launch_f_theta(s, n, c->d_blk_off.as<int64_t>(), c->d_blk_ind.as<int32_t>(),
                        c->d_times.as<double>(), sv, nrec, c->e_us.as<double>(), n, nmc, 1.0 / sg, fs, nrec);
-        for (int smp = 0; smp < nmc; ++smp) {
+        for (int smp = 0; smp < nmc; ++smp)
             launch_gram(s, nrec, c->s_aux.as<double>() + (size_t)smp * nrec, nrec, 1,
                         fs + (size_t)5 * smp * nrec, nrec, 5, c->w_part.as<double>(),
                         c->e_small.as<double>() + 5 * smp);
-        }
         std::vector<double> t((size_t)5 * nmc);
         HIP_CHECK(hipMemcpyAsync(t.data(), c->e_small.p, sizeof(double) * 5 * nmc, hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
@@ -865,22 +894,29 @@ int globe_aireml_eval(globe_ctx* c, const double* y, const double* probes, const
             for (int j = 0; j < 5; ++j) tr[j] += t[smp * 5 + j];
         for (int j = 0; j < 5; ++j) tr[j] /= (double)nmc;
     }
-    for (int j = 0; j < 5; ++j) grad[j] = 0.5 * (first[j] - tr[j]);
+    if (grad)
+        for (int j = 0; j < 5; ++j) grad[j] = 0.5 * (first[j] - tr[j]);
     if (first_terms) std::copy(first, first + 5, first_terms);
     if (trace) std::copy(tr, tr + 5, trace);
-    // 4. AI matrix (varCompEst.cpp:1015-1041): PCG(f, x0 = Q^-1 f), P f, AI = f^T P f / 2
-    double* pf = c->e_pv.as<double>();
-    globe_pcg_opts oa = o;
-    oa.x0_mode = GLOBE_X0_PRECOND;
-    run_pcg(c, fth, nrec, 5, pf, nrec, oa, nullptr, nullptr, nullptr);
-    apply_P(c, f, pf, 5);
-    launch_gram(s, nrec, fth, nrec, 5, pf, nrec, 5, c->w_part.as<double>(), c->e_small.as<double>());
-    double aim[25];
-    HIP_CHECK(hipMemcpyAsync(aim, c->e_small.p, sizeof(double) * 25, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    for (int q = 0; q < 25; ++q) ai[q] = 0.5 * aim[q];
+    if (want_ai && ai) {
+        // 4. AI matrix (varCompEst.cpp:1015-1041): PCG(f, x0 = Q^-1 f), P f, AI = f^T P f / 2
+        double* pf = c->e_pv.as<double>();
+        globe_pcg_opts oa = o;
+        oa.x0_mode = GLOBE_X0_PRECOND;
+        run_pcg(c, fth, nrec, 5, pf, nrec, oa, nullptr, nullptr, nullptr);
+        apply_P(c, f, pf, 5);
+        launch_gram(s, nrec, fth, nrec, 5, pf, nrec, 5, c->w_part.as<double>(), c->e_small.as<double>());
+        double aim[25];
+        HIP_CHECK(hipMemcpyAsync(aim, c->e_small.p, sizeof(double) * 25, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        for (int q = 0; q < 25; ++q) ai[q] = 0.5 * aim[q];
+    }
     API_END(c)
 }
+
+}  // namespace globe
+
+extern "C" {
 
 // ------------------------------------------------------------------ device helpers
 int globe_dev_alloc(globe_ctx* c, int64_t bytes, void** ptr) {

@@ -46,6 +46,11 @@ struct globe_ctx {
     // host-pointer API staging
     globe::DevBuf s_in, s_out, s_aux;
 
+    // last fitMME (reused by the AI matrix of an accepted AIREML step)
+    std::vector<double> fit_xtvx_inv;
+    std::array<double, 5> fit_theta{0, 0, 0, 0, 0};
+    bool fit_valid = false;
+
     // estimator workspaces
     globe::DevBuf e_xy, e_sol, e_f, e_probe, e_w, e_sv, e_pv, e_us, e_fs, e_coef, e_small;
 };
@@ -65,6 +70,11 @@ void require(bool cond, int code, const std::string& msg);
 // with `limbs` int8 digits per value of the right-hand side
 void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, double* GZ,
                        int64_t ldg, const int* skip, int limbs, const double* theta_cols);
+// AIREML derivative terms at the current variance components; y == nullptr reuses
+// the previous fit at the same components (for the AI matrix of an accepted step)
+int aireml_eval_impl(globe_ctx* c, const double* y, const double* probes, const double* weights,
+                     const double* probe_x0, int nmc, const globe_pcg_opts* opts, bool want_ai,
+                     double* grad, double* ai, double* first_terms, double* trace);
 constexpr int kLimbsFP64 = 7;
 constexpr int kLimbsMixed = 4;
 

@@ -1,0 +1,41 @@
+// File formats of the reference, for the drop-in CLIs.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace globe_io {
+
+struct Cohort {
+    int64_t nrec = 0;
+    int32_t n = 0, p = 0;
+    std::vector<int32_t> indiv;   // 0-based
+    std::vector<double> y, times;
+    std::vector<double> X;        // (p+1) columns, column-major, TIME last
+    std::vector<std::string> cov_names;
+};
+
+// Whitespace table with header INDIV PHENO TIMES <covariates...>
+// (checkInputs / readInputData, PCGsolver.cpp:465-587). Values are parsed as
+// float, as the reference does, then widened. Throws std::runtime_error.
+Cohort read_input(const std::string& path);
+
+// Memory-mapped PLINK .bed (SNP-major, magic 6C 1B 01) + .bim line count
+// (prepGenotypeInput, PCGsolver.cpp:624-667).
+struct BedMap {
+    int64_t m = 0;           // SNPs in .bim
+    int64_t row_bytes = 0;   // ceil(n/4)
+    const uint8_t* rows = nullptr;
+    void* base = nullptr;
+    size_t len = 0;
+    ~BedMap();
+};
+void open_bed(const std::string& prefix, int32_t n, BedMap& out);
+
+// <prefix>.grm.bin: n(n+1)/2 float32 lower triangle (readFullGRM, :778-809).
+std::vector<float> read_grm(const std::string& prefix, int32_t n);
+
+// int32 n + n float32 (saveVector, utils.cpp:5-21).
+void save_vector(const std::vector<double>& v, const std::string& path);
+
+}  // namespace globe_io

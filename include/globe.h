@@ -151,6 +151,36 @@ int globe_aireml_eval(globe_ctx* ctx, const double* y, const double* probes,
                       const globe_pcg_opts* opts, double* grad, double* ai,
                       double* first_terms, double* trace);
 
+/* Replaces the NLopt LD_MMA trust-region step of oneIter_AIREML (varCompEst.cpp:1162-1214,
+ * objective :1044-1065, constraints :1068-1145): minimise -(g.p - p^T AI p / 2) over
+ * -10 <= p_i <= 10 subject to sum_i AI_ii p_i <= delta, theta_i + p_i >= 1e-5 (i != 3)
+ * and (theta_3+p_3)^2 - (theta_2+p_2)(theta_4+p_4) + 1e-3 <= 0. Returns the Newton step
+ * AI^-1 g when it is feasible, else the constrained minimiser from an augmented-Lagrangian
+ * projected-gradient solve. Host only (no context). GLOBE_E_NUMERIC when no feasible
+ * step is found (the reference then shrinks the trust region, :1201-1214). */
+int globe_aireml_step(const double grad[5], const double ai[25], const double theta[5],
+                      double delta, double step[5]);
+
+typedef void (*globe_log_fn)(const char* line, void* user);
+
+typedef struct {
+    int32_t max_iter;   /* --maxiter, reference default 50 (varCompEst.cpp:21) */
+    double tol;         /* --tol, reference default 1e-3 (:22) */
+    int32_t num_mc;     /* --numMC, reference default 15 (:23) */
+    int32_t probes;     /* 0: Gaussian tilde-V probes (reference, :775-797); 1: Rademacher */
+    uint64_t seed;      /* probe and restart RNG seed (the reference's is irreproducible) */
+    globe_pcg_opts pcg; /* inner solves; reference: tol 1e-5, max 1000, all-columns rule */
+} globe_aireml_opts;
+
+/* Replaces runAIREML's main loop + oneIter_AIREML (varCompEst.cpp:1148-1491): AI-REML with
+ * the reference's trust-region control (step halving, rho test, Delta update, restarts,
+ * Delta < 1e-4 stop). theta: start values in, estimates out (sig2e, sig2g, sig2b0,
+ * sigb01, sig2b1); se = sqrt(diag(AI^-1)); ai 5x5 row-major; iters and mean seconds per
+ * iteration out; log (may be NULL) receives the progress lines the reference prints. */
+int globe_aireml_fit(globe_ctx* ctx, const double* y, const globe_aireml_opts* opts,
+                     double theta[5], double se[5], double ai[25], int32_t* iters,
+                     double* sec_per_iter, globe_log_fn log, void* user);
+
 /* ---- device memory and synthetic cohorts (no reference counterpart) ---------------- */
 /* Device buffers for the *_dev entry points and for measurement with inputs resident in HBM. */
 int globe_dev_alloc(globe_ctx* ctx, int64_t bytes, void** ptr);
