@@ -47,6 +47,19 @@ class PcgStats(C.Structure):
                 ("seconds", C.c_double)]
 
 
+class KernelStats(C.Structure):
+    _fields_ = [("launches", C.c_int64), ("ms_total", C.c_double), ("bytes_per_launch", C.c_double),
+                ("flops_per_launch", C.c_double), ("issued_ops_per_launch", C.c_double)]
+
+
+class AiremlOpts(C.Structure):
+    _fields_ = [("max_iter", C.c_int32), ("tol", C.c_double), ("num_mc", C.c_int32),
+                ("probes", C.c_int32), ("seed", C.c_uint64), ("pcg", PcgOpts)]
+
+
+LOG_FN = C.CFUNCTYPE(None, C.c_char_p, C.c_void_p)
+
+
 def header_symbols() -> list[str]:
     """Every function declared in include/globe.h."""
     txt = open(HEADER).read()
@@ -92,6 +105,10 @@ def load_library(path: str = LIB_PATH):
                              C.c_float, P], C.c_int),
         "globe_geno_Wa": ([P, P, P], C.c_int),
         "globe_get_geno_rows": ([P, i64, i64, P, P], C.c_int),
+        "globe_profile_enable": ([P, C.c_int], C.c_int),
+        "globe_profile_read": ([P, C.c_int, P], C.c_int),
+        "globe_aireml_step": ([P, P, P, dbl, P], C.c_int),
+        "globe_aireml_fit": ([P, P, P, P, P, P, P, P, P, P], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -334,3 +351,33 @@ class Globe:
 
     def synchronize(self):
         self._check(self.L.globe_synchronize(self.h))
+
+    # ---- kernel timing
+    def profile(self, on=True):
+        self._check(self.L.globe_profile_enable(self.h, int(on)))
+
+    def kernel_stats(self, which):
+        st = KernelStats()
+        self._check(self.L.globe_profile_read(self.h, which, C.byref(st)))
+        return {"launches": st.launches, "ms_total": st.ms_total,
+                "ms_avg": st.ms_total / st.launches if st.launches else 0.0,
+                "bytes": st.bytes_per_launch, "flops": st.flops_per_launch,
+                "issued": st.issued_ops_per_launch}
+
+    # ---- AI-REML (runAIREML)
+    def aireml_fit(self, y, theta0, *, max_iter=50, tol=1e-3, num_mc=15, probes="gaussian",
+                   seed=20261203, pcg_tol=1e-5, precision=PREC_FP64, log=None):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        o = AiremlOpts(max_iter, tol, num_mc, 0 if probes == "gaussian" else 1, seed,
+                       PcgOpts(pcg_tol, 1000, X0_PRECOND, RULE_ALL_COLUMNS, 0, precision, 0))
+        th = np.ascontiguousarray(theta0, dtype=np.float64).copy()
+        se = np.zeros(5)
+        ai = np.zeros(25)
+        it = C.c_int32()
+        spi = C.c_double()
+        lines = []
+        cb = LOG_FN(lambda s, u: (lines.append(s.decode()), log(s.decode()) if log else None))
+        self._check(self.L.globe_aireml_fit(self.h, _p(y), C.byref(o), _p(th), _p(se), _p(ai),
+                                            C.byref(it), C.byref(spi), cb, None))
+        return {"theta": th, "se": se, "ai": ai.reshape(5, 5), "iters": it.value,
+                "sec_per_iter": spi.value, "log": lines}

@@ -262,8 +262,10 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
     launch_pcg_init(s, dst, part_a, part_b, part_c, nmax, k);
     const int* skip = &dst->done;
     int batch = 1, queued = 0;
+    const size_t prof_mark = c->prof.size();
     while (true) {
         for (int b = 0; b < batch; ++b) {
+            c->prof_iter = queued + b;
             op_vx(c, p, nrec, k, vp, nrec, GLOBE_VX_FULL, o.precision, skip, part_a);
             launch_pcg_alpha(s, dst, part_a, np, k);
             launch_pcg_xr(s, dst, nrec, X, ldx, r, p, vp, nrec, k, part_b);
@@ -281,6 +283,21 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
     HIP_CHECK(hipEventRecord(e1, s));
     HIP_CHECK(hipMemcpyAsync(hs, dst, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
+    c->prof_iter = -1;
+    if (c->prof_on) {
+        // launches of iterations after convergence returned at once: drop them
+        std::vector<globe_ctx::ProfEvent> keep(c->prof.begin(), c->prof.begin() + prof_mark);
+        for (size_t q = prof_mark; q < c->prof.size(); ++q) {
+            const auto& e = c->prof[q];
+            if (e.iter < hs->iters) {
+                keep.push_back(e);
+            } else {
+                (void)hipEventDestroy(e.a);
+                (void)hipEventDestroy(e.b);
+            }
+        }
+        c->prof.swap(keep);
+    }
     float ms = 0.f;
     HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
@@ -351,6 +368,10 @@ void globe_destroy(globe_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& e : c->prof) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
     if (c->h_state) (void)hipHostFree(c->h_state);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -951,6 +972,37 @@ int globe_memcpy_d2h(globe_ctx* c, void* dst, const void* src, int64_t bytes) {
     require(dst && src && bytes >= 0, GLOBE_E_INVALID, "globe_memcpy_d2h: bad arguments");
     HIP_CHECK(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
+    API_END(c)
+}
+
+int globe_profile_enable(globe_ctx* c, int on) {
+    API_BEGIN(c)
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (auto& e : c->prof) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    c->prof.clear();
+    c->prof_on = on != 0;
+    API_END(c)
+}
+
+int globe_profile_read(globe_ctx* c, int which, globe_kernel_stats* out) {
+    API_BEGIN(c)
+    require(out != nullptr, GLOBE_E_INVALID, "null output");
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    globe_kernel_stats st{};
+    for (const auto& e : c->prof) {
+        if (e.pass != which) continue;
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e.a, e.b));
+        st.launches += 1;
+        st.ms_total += ms;
+        st.bytes_per_launch = e.bytes;
+        st.flops_per_launch = e.flops;
+        st.issued_ops_per_launch = e.issued;
+    }
+    *out = st;
     API_END(c)
 }
 

@@ -99,6 +99,15 @@ int main(int argc, char** argv) {
         std::cout << "[checkInputs] Found:\n   N = " << c.n << " unique individuals\n   p = " << c.p
                   << " fixed effects (not counting TIME)\n   Nrec = " << c.nrec << " records total." << std::endl;
 
+    // open every input before touching the device, so file errors exit like the reference
+    globe_io::BedMap bed;
+    std::vector<float> lt;
+    try {
+        if (!geno.empty()) globe_io::open_bed(geno, c.n, bed);
+        else lt = globe_io::read_grm(grm, c.n);
+    } catch (const std::exception& e) {
+        die(e.what());
+    }
     globe_ctx* ctx = nullptr;
     if (globe_create(device, &ctx) != GLOBE_OK)
         die("[Error] no usable GPU (device " + std::to_string(device) + ", " +
@@ -106,23 +115,11 @@ int main(int argc, char** argv) {
     check(ctx, globe_set_cohort(ctx, c.nrec, c.n, c.p, c.indiv.data(), c.times.data(), c.X.data(), c.nrec),
           "cohort");
     if (!geno.empty()) {
-        globe_io::BedMap bed;
-        try {
-            globe_io::open_bed(geno, c.n, bed);
-        } catch (const std::exception& e) {
-            die(e.what());
-        }
         int64_t kept = 0;
         check(ctx, globe_load_bed(ctx, bed.rows, bed.m, bed.row_bytes, min_maf, max_miss, &kept), "genotypes");
         std::cout << "[prepGenotypeInput] Processed " << bed.m << " / " << bed.m << " SNPs." << std::endl;
         if (debug) std::cout << "[prepGenotypeInput] Filtered SNPs = " << kept << " / " << bed.m << std::endl;
     } else {
-        std::vector<float> lt;
-        try {
-            lt = globe_io::read_grm(grm, c.n);
-        } catch (const std::exception& e) {
-            die(e.what());
-        }
         check(ctx, globe_load_grm(ctx, lt.data()), "GRM");
     }
     std::cout << "[prepareAllInputs] Input prepared successfully." << std::endl;

@@ -43,6 +43,17 @@ struct globe_ctx {
     globe::DevBuf p_state, p_r, p_z, p_p, p_vp;
     globe::PcgState* h_state = nullptr;  // pinned mirror
 
+    // kernel timing: event pairs around contraction launches, tagged with the PCG
+    // iteration that issued them (-1 outside PCG)
+    struct ProfEvent {
+        hipEvent_t a, b;
+        int pass, iter;
+        double bytes, flops, issued;
+    };
+    bool prof_on = false;
+    int prof_iter = -1;
+    std::vector<ProfEvent> prof;
+
     // host-pointer API staging
     globe::DevBuf s_in, s_out, s_aux;
 

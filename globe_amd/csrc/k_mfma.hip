@@ -378,7 +378,27 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
                                                   part, skip);
     k_reduce_sum<<<1, kMaxCols, 0, s>>>(part, qg1, k, c->w_T.as<double>(), skip);
     HIP_CHECK(hipGetLastError());
-    launch_mfma(s, RT, JF, c->d_snp.as<uint8_t>(), c->pitch_s, m, nss1, sp1, Bq, out, skip);
+    // algorithmic work per pass: rows x ceil(K/4) packed bytes, 2 N M k flops
+    const double flops = 2.0 * (double)n * (double)m * k;
+    auto timed = [&](int pass, int64_t rows, int64_t K, auto&& fn) {
+        if (!c->prof_on) {
+            fn();
+            return;
+        }
+        globe_ctx::ProfEvent e{};
+        HIP_CHECK(hipEventCreate(&e.a));
+        HIP_CHECK(hipEventCreate(&e.b));
+        e.pass = pass;
+        e.iter = c->prof_iter;
+        e.bytes = (double)rows * (double)((K + 3) / 4);
+        e.flops = flops;
+        e.issued = 2.0 * (double)rows * (double)(pass == 0 ? Kpad1 : Kpad2) * (JF * 16);
+        HIP_CHECK(hipEventRecord(e.a, s));
+        fn();
+        HIP_CHECK(hipEventRecord(e.b, s));
+        c->prof.push_back(e);
+    };
+    timed(0, m, n, [&] { launch_mfma(s, RT, JF, c->d_snp.as<uint8_t>(), c->pitch_s, m, nss1, sp1, Bq, out, skip); });
     const int npm = parts_for(m);
     k_epi1<<<dim3(npm, k), 256, 0, s>>>(out, sp1, m, JF * 16, limbs, sh1, c->w_T.as<double>(),
                                          c->d_twof.as<double>(), c->d_inv.as<double>(), th,
@@ -391,7 +411,7 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
                                                   c->d_twof.as<double>(), Bq, part, skip);
     k_reduce_sum<<<1, kMaxCols, 0, s>>>(part, qg2, k, c->w_C.as<double>(), skip);
     HIP_CHECK(hipGetLastError());
-    launch_mfma(s, RT, JF, c->d_ind.as<uint8_t>(), c->pitch_i, n, nss2, sp2, Bq, out, skip);
+    timed(1, n, m, [&] { launch_mfma(s, RT, JF, c->d_ind.as<uint8_t>(), c->pitch_i, n, nss2, sp2, Bq, out, skip); });
     k_epi2<<<dim3(npn, k), 256, 0, s>>>(out, sp2, n, JF * 16, limbs, sh2, c->w_C.as<double>(), GZ,
                                          ldg, skip);
     HIP_CHECK(hipGetLastError());

@@ -181,6 +181,21 @@ int globe_aireml_fit(globe_ctx* ctx, const double* y, const globe_aireml_opts* o
                      double theta[5], double se[5], double ai[25], int32_t* iters,
                      double* sec_per_iter, globe_log_fn log, void* user);
 
+/* ---- kernel timing (no reference counterpart) --------------------------------------- */
+/* When enabled, every packed-genotype contraction launch is bracketed by HIP events on
+ * the context stream (launches that a finished PCG skips are discarded). which: 0 = pass 1
+ * (SNP-major rows, W^T z), 1 = pass 2 (individual-major rows, W u). bytes / flops are
+ * the algorithmic figures per launch: rows x ceil(K/4) packed bytes, 2 N M k flops. */
+typedef struct {
+    int64_t launches;
+    double ms_total;
+    double bytes_per_launch;
+    double flops_per_launch;
+    double issued_ops_per_launch; /* int8 MFMA operations actually issued (digits, padding) */
+} globe_kernel_stats;
+int globe_profile_enable(globe_ctx* ctx, int on);
+int globe_profile_read(globe_ctx* ctx, int which, globe_kernel_stats* out);
+
 /* ---- device memory and synthetic cohorts (no reference counterpart) ---------------- */
 /* Device buffers for the *_dev entry points and for measurement with inputs resident in HBM. */
 int globe_dev_alloc(globe_ctx* ctx, int64_t bytes, void** ptr);
