@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--seed", type=int, default=20261206)
     ap.add_argument("--cpu-sample-snps", type=int, default=20_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="first run 3 plain 1 GiB device copies (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; the multi-GPU SNP sharding is "
@@ -98,6 +100,13 @@ def main():
 
     import globe_amd as G
     g = G.Globe(0)
+    if args.calib:
+        # reference dispatches for the HBM byte counters: 3 plain copies of 1 GiB
+        a, b = g.alloc(1 << 30), g.alloc(1 << 30)
+        for _ in range(3):
+            g._check(g.L.globe_dev_copy(g.h, b.ptr, a.ptr, 1 << 30))
+        a.free()
+        b.free()
     c, y, kept = build_cohort(g, args.n, args.m, args.lam, args.seed)
     nrec, k = c.nrec, args.k
     rng = np.random.default_rng(args.seed + 1)

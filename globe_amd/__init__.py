@@ -105,6 +105,7 @@ def load_library(path: str = LIB_PATH):
                              C.c_float, P], C.c_int),
         "globe_geno_Wa": ([P, P, P], C.c_int),
         "globe_get_geno_rows": ([P, i64, i64, P, P], C.c_int),
+        "globe_dev_copy": ([P, P, P, i64], C.c_int),
         "globe_profile_enable": ([P, C.c_int], C.c_int),
         "globe_profile_read": ([P, C.c_int, P], C.c_int),
         "globe_aireml_step": ([P, P, P, dbl, P], C.c_int),

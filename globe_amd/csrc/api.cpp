@@ -1006,6 +1006,15 @@ int globe_profile_read(globe_ctx* c, int which, globe_kernel_stats* out) {
     API_END(c)
 }
 
+int globe_dev_copy(globe_ctx* c, void* dst, const void* src, int64_t bytes) {
+    API_BEGIN(c)
+    require(dst && src && bytes >= 0 && bytes % 16 == 0, GLOBE_E_INVALID,
+            "globe_dev_copy: bytes must be a non-negative multiple of 16");
+    launch_copy16(c->stream, dst, src, bytes);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    API_END(c)
+}
+
 int globe_synchronize(globe_ctx* c) {
     API_BEGIN(c)
     HIP_CHECK(hipStreamSynchronize(c->stream));

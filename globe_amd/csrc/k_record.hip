@@ -477,6 +477,13 @@ __global__ __launch_bounds__(kBlock) void k_fill_cols(int64_t rows, double* __re
         d[c * ldd + q] = v;
 }
 
+__global__ __launch_bounds__(kBlock) void k_copy16(uint4* __restrict__ dst,
+                                                   const uint4* __restrict__ src, int64_t n16) {
+    for (int64_t q = blockIdx.x * (int64_t)kBlock + threadIdx.x; q < n16;
+         q += (int64_t)gridDim.x * kBlock)
+        dst[q] = src[q];
+}
+
 __global__ __launch_bounds__(kBlock) void k_fill(double* dst, int64_t count, double v) {
     for (int64_t q = blockIdx.x * (int64_t)kBlock + threadIdx.x; q < count;
          q += (int64_t)gridDim.x * kBlock)
@@ -617,6 +624,12 @@ void launch_sub_cols(hipStream_t s, int64_t rows, const double* a, int64_t lda, 
 void launch_fill_cols(hipStream_t s, int64_t rows, double* d, int64_t ldd, int k, double v) {
     dim3 g(parts_for(rows), k);
     k_fill_cols<<<g, kBlock, 0, s>>>(rows, d, ldd, v);
+    LAUNCH_CHECK();
+}
+
+void launch_copy16(hipStream_t s, void* dst, const void* src, int64_t bytes) {
+    const int64_t n16 = bytes / 16;
+    k_copy16<<<2048, kBlock, 0, s>>>(static_cast<uint4*>(dst), static_cast<const uint4*>(src), n16);
     LAUNCH_CHECK();
 }
 

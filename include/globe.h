@@ -203,6 +203,9 @@ int globe_dev_free(globe_ctx* ctx, void* ptr);
 int globe_memcpy_h2d(globe_ctx* ctx, void* dst, const void* src, int64_t bytes);
 int globe_memcpy_d2h(globe_ctx* ctx, void* dst, const void* src, int64_t bytes);
 int globe_synchronize(globe_ctx* ctx);
+/* Plain device-to-device copy kernel (16-byte vector loads / stores); the reference
+ * point for HBM byte counters (bytes a multiple of 16). */
+int globe_dev_copy(globe_ctx* ctx, void* dst, const void* src, int64_t bytes);
 /* Generates an m-SNP x n-individual PLINK cohort directly in HBM (counter-based hash
  * RNG, seeded): f_s ~ U(f_lo, f_hi), dosage ~ Binomial(2, f_s), missing w.p. miss;
  * then runs the same QC and staging as globe_load_bed. For cohorts too large to
