@@ -261,8 +261,12 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
         for (int j = 0; j < k; ++j) launch_fill(s, part_a + (size_t)j * kMaxParts + np, nmax - np, 0.0);
     launch_pcg_init(s, dst, part_a, part_b, part_c, nmax, k);
     const int* skip = &dst->done;
+    // Iterations are enqueued in batches that double while an iteration is short
+    // (launch-bound problems); when one iteration takes over 2 ms the host checks the
+    // done flag after every iteration, so no skipped launches are issued.
     int batch = 1, queued = 0;
     const size_t prof_mark = c->prof.size();
+    auto tb = std::chrono::steady_clock::now();
     while (true) {
         for (int b = 0; b < batch; ++b) {
             c->prof_iter = queued + b;
@@ -278,7 +282,10 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
         HIP_CHECK(hipMemcpyAsync(hs, dst, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
         if (hs->done || queued >= o.max_iter + 1) break;
-        batch = std::min(batch * 2, 16);
+        const auto tn = std::chrono::steady_clock::now();
+        const double per_iter = std::chrono::duration<double>(tn - tb).count() / batch;
+        tb = tn;
+        batch = per_iter > 2e-3 ? 1 : std::min(batch * 2, 16);
     }
     HIP_CHECK(hipEventRecord(e1, s));
     HIP_CHECK(hipMemcpyAsync(hs, dst, sizeof(PcgState), hipMemcpyDeviceToHost, s));
