@@ -1,4 +1,4 @@
-// Packed-genotype contraction on the int8 matrix cores (v_mfma_i32_16x16x64_i8).
+// Packed-genotype contraction on the int8 matrix cores.
 //
 //   Out[r][c] = sum_kappa dosage(r, kappa) * Z[c][kappa]
 //
@@ -9,22 +9,36 @@
 // is exact for the quantized input; the only rounding is the quantization
 // (2^-(8L-2) of the column max) and the final FP64 combination of the L digit sums.
 //   L = 7: 54-bit fixed point, as accurate as an FP64 dot product (GLOBE_PREC_FP64)
-//   L = 4: 30-bit fixed point (GLOBE_PREC_MIXED), half the MFMA work of L = 7... of
-//          the same cost as BF16 hi+lo pieces, with exact accumulation.
-//
+//   L = 4: 30-bit fixed point (GLOBE_PREC_MIXED); 4 int8 digits cost the MFMA time of
+//          two BF16 pieces and keep 30 bits with exact accumulation.
 // Centering and scaling (w = (g - 2f) inv) are applied afterwards as the FP64
-// rank-one correction sum_i w_is z_i = inv_s (P_s - 2 f_s T), T = sum_i z_i computed
-// from the quantized z, so the operator is applied exactly to the quantized vector.
+// rank-one correction sum_i w_is z_i = inv_s (P_s - 2 f_s T), T = sum_i z_i of the
+// quantized z, so the operator is applied exactly to the quantized vector.
 //
-// Operand layouts (16x16x64 i8, lane l, group g = l / 16):
-//   A: row l % 16, 16 K-slots; B: column l % 16, the same 16 K-slots;
-//   D: element v of lane l is (row 4 g + v, column l % 16).
-// A lane of row fragment rt loads 16 packed bytes (64 genotypes) at byte
-// 64 ss + 16 g of its row per super-step ss (4 K-steps); K-step s uses dword s.
-// The dword's 16 codes are decoded with two masks and four v_perm_b32 byte-table
-// lookups, which yields genotype order {0,4,8,12, 1,5,9,13, 2,6,10,14, 3,7,11,15};
-// the digit matrix Bq is stored in exactly that order:
-//   Bq[ks][jf][lane][16 bytes], ks = K-step, jf = 16-column fragment.
+// Two kernels share the K order (a super-step is 256 genotypes = 64 packed bytes of
+// a row; a lane decodes one packed dword = 16 genotypes per MFMA K-step):
+//   thin (digit columns <= 16, the 1-RHS solve): v_mfma_i32_16x16x64_i8, 8 row
+//     fragments of 16 rows per wave (512 rows per block), the digit fragments read
+//     straight from global memory (L1 / L2), no LDS and no barrier;
+//     A lane l: row l % 16, packed bytes 64 ss + 16 (l / 16) + 4 s for K-step s < 4.
+//   wide (digit columns 17..128): v_mfma_i32_32x32x32_i8, 2 row fragments of 32 rows
+//     and up to 4 column fragments of 32 per wave (256 rows per block), the digit
+//     tile of the next super-step staged in a second LDS buffer while the current
+//     one is consumed (one barrier per super-step);
+//     A lane l: row l % 32, packed bytes 64 ss + 32 (l / 32) + 4 s for K-step s < 8.
+// Decode: two masks and four v_perm_b32 byte-table lookups turn a dword of 16 codes
+// into 16 int8 dosages in the order {0,4,8,12, 1,5,9,13, 2,6,10,14, 3,7,11,15}; the
+// digit matrix Bq holds each lane's 16 digits in exactly that order.
+// Output fragments (probed on gfx950): 16x16: element v of lane l is (row 4 (l/16) + v,
+// column l % 16); 32x32: (row 8 (v/4) + 4 (l/32) + v % 4, column l % 32).
+//
+// Split-K is a power of two S; block b works on K-split b % S of row block b / S for
+// S <= 8, and for S > 8 every XCD (b % 8) walks its splits one after another
+// (split = b % 8 + 8 ((b / 8) / R)), so the blocks resident on one XCD share one
+// K-chunk of Bq in that XCD's L2. Partial sums per split go to out[split][row][J16]
+// and are added exactly (int64) in the epilogue.
+#include <algorithm>
+
 #include "context.h"
 
 namespace globe {
@@ -32,9 +46,11 @@ namespace globe {
 namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
 
 // dosage byte table for codes 0..3: 00 -> 2, 01 (missing) -> 0, 10 -> 1, 11 -> 0
 constexpr uint32_t kDosTable = 0x00010002u;
+constexpr uint32_t kZeroCodes = 0xFFFFFFFFu;  // 16 codes 11: dosage 0
 
 __device__ inline v4i decode16(uint32_t w) {
     const uint32_t e = w & 0x33333333u;
@@ -47,91 +63,207 @@ __device__ inline v4i decode16(uint32_t w) {
     return d;
 }
 
-// RT row fragments x JF column fragments per wave; 4 waves per block.
-template <int RT, int JF>
-__global__ __launch_bounds__(256) void k_geno_mfma(const uint8_t* __restrict__ rows, int64_t pitch,
+__device__ inline uint32_t dword_of(const uint4& v, int s) {
+    return s == 0 ? v.x : s == 1 ? v.y : s == 2 ? v.z : v.w;
+}
+
+// block b -> (row block, split)
+__device__ inline void block_coords(int64_t b, int64_t R, int S, int64_t& rb, int& sp) {
+    if (S <= 8) {
+        sp = (int)(b % S);
+        rb = b / S;
+    } else {
+        const int64_t t = b / 8;
+        sp = (int)(b % 8) + 8 * (int)(t / R);
+        rb = t % R;
+    }
+}
+
+// ---------------------------------------------------------------- thin kernel
+constexpr int kThinRT = 8;                      // row fragments of 16 per wave
+constexpr int kThinRows = 4 * 16 * kThinRT;      // rows per block
+
+__global__ __launch_bounds__(256) void k_geno_thin(const uint8_t* __restrict__ rows, int64_t pitch,
                                                    int64_t nrows, int64_t n_ss, int64_t ss_per_split,
-                                                   const uint8_t* __restrict__ Bq,
-                                                   int32_t* __restrict__ out, int64_t out_rows,
-                                                   const int* skip) {
+                                                   int64_t R, int S, const uint8_t* __restrict__ Bq,
+                                                   int32_t* __restrict__ out, const int* skip) {
     if (skip && *skip) return;
-    __shared__ __attribute__((aligned(16))) uint8_t lds[4 * JF * 1024];
+    int64_t rb;
+    int sp;
+    block_coords(blockIdx.x, R, S, rb, sp);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, rr = lane & 15;
-    const int64_t row_base = (int64_t)blockIdx.x * (64 * RT) + wave * (16 * RT);
-    const int64_t ss0 = (int64_t)blockIdx.y * ss_per_split;
-    int64_t ss1 = ss0 + ss_per_split;
-    if (ss1 > n_ss) ss1 = n_ss;
+    const int64_t row_base = rb * kThinRows + wave * (16 * kThinRT);
+    const int64_t ss0 = (int64_t)sp * ss_per_split;
+    const int64_t ss1 = ss0 + ss_per_split < n_ss ? ss0 + ss_per_split : n_ss;
 
-    v4i acc[RT][JF];
+    v4i acc[kThinRT];
 #pragma unroll
-    for (int a = 0; a < RT; ++a)
+    for (int a = 0; a < kThinRT; ++a) acc[a] = v4i{0, 0, 0, 0};
+    const uint8_t* rp[kThinRT];
+    bool rv[kThinRT];
 #pragma unroll
-        for (int b = 0; b < JF; ++b) acc[a][b] = v4i{0, 0, 0, 0};
-
-    const uint8_t* rp[RT];
-    bool rv[RT];
-#pragma unroll
-    for (int a = 0; a < RT; ++a) {
+    for (int a = 0; a < kThinRT; ++a) {
         const int64_t r = row_base + a * 16 + rr;
         rv[a] = r < nrows;
         rp[a] = rows + (rv[a] ? r : 0) * pitch + 16 * g;
     }
-
-    uint4 araw[RT];
+    const uint4* bq = reinterpret_cast<const uint4*>(Bq) + lane;  // [ks][64 lanes] x 16 B
+    uint4 an[kThinRT], bn[4];
+    if (ss0 < ss1) {
 #pragma unroll
-    for (int a = 0; a < RT; ++a)
-        araw[a] = (rv[a] && ss0 < ss1) ? *reinterpret_cast<const uint4*>(rp[a] + ss0 * 64)
-                                       : make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
-
+        for (int a = 0; a < kThinRT; ++a)
+            an[a] = rv[a] ? *reinterpret_cast<const uint4*>(rp[a] + ss0 * 64)
+                          : make_uint4(kZeroCodes, kZeroCodes, kZeroCodes, kZeroCodes);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) bn[s] = bq[(4 * ss0 + s) * 64];
+    }
     for (int64_t ss = ss0; ss < ss1; ++ss) {
-        __syncthreads();
-        {
-            const uint4* src = reinterpret_cast<const uint4*>(Bq + ss * (4 * JF * 1024));
-            uint4* dst = reinterpret_cast<uint4*>(lds);
+        uint4 ac[kThinRT], bc[4];
 #pragma unroll
-            for (int q = 0; q < JF; ++q) dst[q * 256 + threadIdx.x] = src[q * 256 + threadIdx.x];
-        }
-        __syncthreads();
-        uint4 cur[RT];
+        for (int a = 0; a < kThinRT; ++a) ac[a] = an[a];
 #pragma unroll
-        for (int a = 0; a < RT; ++a) cur[a] = araw[a];
-        // prefetch the next super-step's packed genotypes
+        for (int s = 0; s < 4; ++s) bc[s] = bn[s];
         if (ss + 1 < ss1) {
 #pragma unroll
-            for (int a = 0; a < RT; ++a)
-                if (rv[a]) araw[a] = *reinterpret_cast<const uint4*>(rp[a] + (ss + 1) * 64);
+            for (int a = 0; a < kThinRT; ++a)
+                if (rv[a]) an[a] = *reinterpret_cast<const uint4*>(rp[a] + (ss + 1) * 64);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) bn[s] = bq[(4 * (ss + 1) + s) * 64];
         }
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            v4i av[RT];
+            const v4i bv = *reinterpret_cast<const v4i*>(&bc[s]);
 #pragma unroll
-            for (int a = 0; a < RT; ++a) {
-                const uint32_t w = s == 0 ? cur[a].x : s == 1 ? cur[a].y : s == 2 ? cur[a].z : cur[a].w;
-                av[a] = decode16(w);
-            }
-#pragma unroll
-            for (int b = 0; b < JF; ++b) {
-                const v4i bv = *reinterpret_cast<const v4i*>(lds + ((s * JF + b) * 64 + lane) * 16);
-#pragma unroll
-                for (int a = 0; a < RT; ++a)
-                    acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[a], bv, acc[a][b], 0, 0, 0);
-            }
+            for (int a = 0; a < kThinRT; ++a)
+                acc[a] = __builtin_amdgcn_mfma_i32_16x16x64_i8(decode16(dword_of(ac[a], s)), bv,
+                                                               acc[a], 0, 0, 0);
         }
     }
-    // D element v of lane -> (row 4 g + v, column rr)
-    int32_t* o = out + (int64_t)blockIdx.y * out_rows * (JF * 16);
+    int32_t* o = out + (int64_t)sp * nrows * 16;
 #pragma unroll
-    for (int a = 0; a < RT; ++a)
+    for (int a = 0; a < kThinRT; ++a)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int64_t r = row_base + a * 16 + 4 * g + v;
-            if (r < nrows)
-#pragma unroll
-                for (int b = 0; b < JF; ++b) o[r * (JF * 16) + b * 16 + rr] = acc[a][b][v];
+            if (r < nrows) o[r * 16 + rr] = acc[a][v];
         }
 }
 
+// ---------------------------------------------------------------- wide kernel
+constexpr int kWideRows = 4 * 64;  // 4 waves x 2 fragments of 32 rows
+
+template <int JT>
+__global__ __launch_bounds__(256) void k_geno_wide(const uint8_t* __restrict__ rows, int64_t pitch,
+                                                   int64_t nrows, int64_t n_ss, int64_t ss_per_split,
+                                                   int64_t R, int S, const uint8_t* __restrict__ Bq,
+                                                   int32_t* __restrict__ out, const int* skip) {
+    if (skip && *skip) return;
+    // two buffers of one super-step of digits: 8 K-steps x JT fragments x 1 KB
+    __shared__ __attribute__((aligned(16))) uint8_t lds[2][8 * JT * 1024];
+    int64_t rb;
+    int sp;
+    block_coords(blockIdx.x, R, S, rb, sp);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 5, rr = lane & 31;
+    const int64_t row_base = rb * kWideRows + wave * 64;
+    const int64_t ss0 = (int64_t)sp * ss_per_split;
+    const int64_t ss1 = ss0 + ss_per_split < n_ss ? ss0 + ss_per_split : n_ss;
+
+    v16i acc[2][JT];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < JT; ++b)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[a][b][v] = 0;
+    const uint8_t* rp[2];
+    bool rv[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int64_t r = row_base + a * 32 + rr;
+        rv[a] = r < nrows;
+        rp[a] = rows + (rv[a] ? r : 0) * pitch + 32 * g;
+    }
+    // digits of one super-step: 8 * JT KB = 512 * JT uint4; thread t copies uint4
+    // t + 256 q for q < 2 JT, spread over the 8 K-steps (JT / 4 per K-step and thread
+    // when JT = 4, one copy every 4 / JT K-steps otherwise)
+    const uint4* bsrc = reinterpret_cast<const uint4*>(Bq);
+    constexpr int kCopies = 2 * JT;
+    uint4 an[2][2];
+    if (ss0 < ss1) {
+        const uint4* s4 = bsrc + ss0 * (512 * JT);
+        uint4* d4 = reinterpret_cast<uint4*>(lds[0]);
+#pragma unroll
+        for (int q = 0; q < kCopies; ++q) d4[threadIdx.x + 256 * q] = s4[threadIdx.x + 256 * q];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            an[a][0] = rv[a] ? *reinterpret_cast<const uint4*>(rp[a] + ss0 * 64)
+                             : make_uint4(kZeroCodes, kZeroCodes, kZeroCodes, kZeroCodes);
+            an[a][1] = rv[a] ? *reinterpret_cast<const uint4*>(rp[a] + ss0 * 64 + 16)
+                             : make_uint4(kZeroCodes, kZeroCodes, kZeroCodes, kZeroCodes);
+        }
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int64_t ss = ss0; ss < ss1; ++ss) {
+        uint4 ac[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            ac[a][0] = an[a][0];
+            ac[a][1] = an[a][1];
+        }
+        const bool more = ss + 1 < ss1;
+        if (more) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+                if (rv[a]) {
+                    an[a][0] = *reinterpret_cast<const uint4*>(rp[a] + (ss + 1) * 64);
+                    an[a][1] = *reinterpret_cast<const uint4*>(rp[a] + (ss + 1) * 64 + 16);
+                }
+        }
+        const uint4* nsrc = bsrc + (ss + 1) * (512 * JT);
+        uint4* ndst = reinterpret_cast<uint4*>(lds[cur ^ 1]);
+        const uint8_t* cl = lds[cur];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            // stage a share of the next super-step's digits
+            uint4 pre[kCopies / 8 > 0 ? kCopies / 8 : 1];
+            constexpr int per = kCopies >= 8 ? kCopies / 8 : 1;
+            const bool mine = kCopies >= 8 || (s % (8 / kCopies)) == 0;
+            const int qbase = kCopies >= 8 ? s * per : s / (8 / kCopies);
+            if (more && mine)
+#pragma unroll
+                for (int q = 0; q < per; ++q) pre[q] = nsrc[threadIdx.x + 256 * (qbase + q)];
+            const v4i a0 = decode16(dword_of(s < 4 ? ac[0][0] : ac[0][1], s & 3));
+            const v4i a1 = decode16(dword_of(s < 4 ? ac[1][0] : ac[1][1], s & 3));
+#pragma unroll
+            for (int b = 0; b < JT; ++b) {
+                const v4i bv = *reinterpret_cast<const v4i*>(cl + ((s * JT + b) * 64 + lane) * 16);
+                acc[0][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, bv, acc[0][b], 0, 0, 0);
+                acc[1][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, bv, acc[1][b], 0, 0, 0);
+            }
+            if (more && mine)
+#pragma unroll
+                for (int q = 0; q < per; ++q) ndst[threadIdx.x + 256 * (qbase + q)] = pre[q];
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    constexpr int J16 = 32 * JT;
+    int32_t* o = out + (int64_t)sp * nrows * J16;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int64_t r = row_base + a * 32 + 8 * (v >> 2) + 4 * g + (v & 3);
+            if (r < nrows)
+#pragma unroll
+                for (int b = 0; b < JT; ++b) o[r * J16 + b * 32 + rr] = acc[a][b][v];
+        }
+}
+
+// ---------------------------------------------------------------- reductions
 __device__ inline double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -175,40 +307,53 @@ __global__ __launch_bounds__(256) void k_colmax_parts(int64_t rows, const double
     if (threadIdx.x == 0) part[c * kMaxParts + blockIdx.x] = r;
 }
 
-// scale exponent per column: sh_c = (8L - 2) - e_c with max|v| < 2^e_c
-__global__ void k_scales(const double* __restrict__ part, int nparts, int k, int limbs, int* sh,
-                         const int* skip) {
+// One wave per column: max (want_max) or ordered sum of the column's partials; the
+// scale exponent sh_c = (8L - 2) - e_c with max|v| < 2^e_c when sh != nullptr.
+__global__ __launch_bounds__(256) void k_finish_cols(const double* __restrict__ part, int nparts,
+                                                     int k, int want_max, int limbs, int* sh,
+                                                     double* out, const int* skip) {
     if (skip && *skip) return;
-    const int c = threadIdx.x;
-    if (c >= k) return;
-    double m = 0.0;
-    for (int j = 0; j < nparts; ++j) m = fmax(m, part[c * kMaxParts + j]);
-    int e = 0;
-    if (m > 0.0) (void)frexp(m, &e);  // m < 2^e
-    sh[c] = (m > 0.0 && isfinite(m)) ? (8 * limbs - 2) - e : 0;
+    const int lane = threadIdx.x & 63;
+    for (int c = threadIdx.x >> 6; c < k; c += 4) {
+        double v = 0.0;
+        for (int j = lane; j < nparts; j += 64) {
+            const double x = part[c * kMaxParts + j];
+            v = want_max ? fmax(v, x) : v + x;
+        }
+        v = want_max ? wave_max(v) : wave_sum(v);
+        if (lane == 0) {
+            if (out) out[c] = v;
+            if (sh) {
+                int e = 0;
+                if (v > 0.0) (void)frexp(v, &e);  // v < 2^e
+                sh[c] = (v > 0.0 && isfinite(v)) ? (8 * limbs - 2) - e : 0;
+            }
+        }
+    }
 }
 
-// Quantize v (k columns, K rows valid, Kpad rows total) into the digit layout and
-// emit per-block partials of sum_kappa w_kappa * q_kappa 2^-sh (T for pass 1 with
-// w = 1, C for pass 2 with w = 2 f).
+// Quantize v (k columns, K rows valid, Kpad total) into the digit layout of the thin
+// (wide = 0) or wide (wide = 1) kernel and emit per-block partials of
+// sum_kappa w_kappa q_kappa 2^-sh (T for pass 1 with w = 1, C for pass 2 with w = 2 f).
 __global__ __launch_bounds__(256) void k_quantize(const double* __restrict__ v, int64_t ldv,
-                                                  int64_t K, int64_t Kpad, int k, int limbs, int JF,
-                                                  const int* __restrict__ sh,
+                                                  int64_t K, int64_t Kpad, int k, int limbs,
+                                                  int nfrag, int wide, const int* __restrict__ sh,
                                                   const double* __restrict__ wts,
                                                   uint8_t* __restrict__ Bq, double* part,
                                                   const int* skip) {
     if (skip && *skip) return;
-    const int j16 = blockIdx.y;
-    const int c = j16 / limbs, l = j16 % limbs;
+    const int j = blockIdx.y;  // digit column
+    const int c = j / limbs, l = j % limbs;
     const bool live = c < k;
+    const int fw = wide ? 32 : 16;
+    const int jf = j / fw, jj = j % fw;
     double acc = 0.0;
     for (int64_t kap = blockIdx.x * 256ll + threadIdx.x; kap < Kpad; kap += (int64_t)gridDim.x * 256) {
         int dig = 0;
         if (live && kap < K) {
             const double z = v[c * ldv + kap];
-            long long q = llrint(ldexp(z, sh[c]));
-            if (l == 0 && wts) acc += wts[kap] * ldexp((double)q, -sh[c]);
-            if (l == 0 && !wts) acc += ldexp((double)q, -sh[c]);
+            const long long q = llrint(ldexp(z, sh[c]));
+            if (l == 0) acc += (wts ? wts[kap] : 1.0) * ldexp((double)q, -sh[c]);
             long long t = q;
             for (int d = 0; d <= l; ++d) {
                 int b = (int)(t & 0xFF);
@@ -219,17 +364,29 @@ __global__ __launch_bounds__(256) void k_quantize(const double* __restrict__ v, 
         }
         const int64_t ss = kap >> 8;
         const int rem = (int)(kap & 255);
-        const int gg = rem >> 6, s = (rem >> 4) & 3, w = rem & 15;
+        const int w = rem & 15;
         const int qslot = 4 * (w & 3) + (w >> 2);
-        const int64_t ks = 4 * ss + s;
-        const int jf = j16 >> 4, jj = j16 & 15;
-        Bq[((ks * JF + jf) * 64 + (16 * gg + jj)) * 16 + qslot] = (uint8_t)(int8_t)dig;
+        int64_t ks;
+        int ln;
+        if (wide) {
+            ks = 8 * ss + ((rem >> 4) & 7);
+            ln = 32 * (rem >> 7) + jj;
+        } else {
+            ks = 4 * ss + ((rem >> 4) & 3);
+            ln = 16 * (rem >> 6) + jj;
+        }
+        Bq[((ks * nfrag + jf) * 64 + ln) * 16 + qslot] = (uint8_t)(int8_t)dig;
     }
     if (live && l == 0) {
         const double r = block_sum(acc);
         if (threadIdx.x == 0) part[c * kMaxParts + blockIdx.x] = r;
     }
 }
+
+// Epilogues: a block takes 64 rows x k columns. Phase 1 reads the digit sums with
+// consecutive threads on consecutive digits (coalesced), phase 2 writes the FP64
+// results column-major with consecutive threads on consecutive rows.
+constexpr int kEpiRows = 64;
 
 __device__ inline double combine(const int32_t* __restrict__ out, int nsplit, int64_t split_stride,
                                  int64_t base, int limbs) {
@@ -242,100 +399,78 @@ __device__ inline double combine(const int32_t* __restrict__ out, int nsplit, in
     return v;
 }
 
-// pass-1 epilogue: P = combine * 2^-sh; u = sig2g / M * inv^2 * (P - 2 f T); colmax of u
-__global__ __launch_bounds__(256) void k_epi1(const int32_t* __restrict__ out, int nsplit,
-                                              int64_t m, int J16, int limbs,
-                                              const int* __restrict__ sh,
-                                              const double* __restrict__ T,
-                                              const double* __restrict__ twof,
-                                              const double* __restrict__ inv,
-                                              const double* __restrict__ th, double inv_m,
-                                              double* __restrict__ u, int64_t ldu, double* part,
-                                              const int* skip) {
+// pass 1 (mode 0): u = sig2g / M * inv^2 * (P - 2 f T), per-block colmax of u
+// pass 2 (mode 1): GZ = P - C
+__global__ __launch_bounds__(256) void k_epilogue(const int32_t* __restrict__ out, int nsplit,
+                                                  int64_t rows, int J16, int limbs, int k,
+                                                  const int* __restrict__ sh, int mode,
+                                                  const double* __restrict__ TC,
+                                                  const double* __restrict__ twof,
+                                                  const double* __restrict__ inv,
+                                                  const double* __restrict__ th, double inv_m,
+                                                  double* __restrict__ res, int64_t ldr,
+                                                  double* part, const int* skip) {
     if (skip && *skip) return;
-    const int c = blockIdx.y;
-    const double scale = th[c * 5 + 1] * inv_m, tc = T[c];
-    const int shc = sh[c];
-    double mx = 0.0;
-    for (int64_t s = blockIdx.x * 256ll + threadIdx.x; s < m; s += (int64_t)gridDim.x * 256) {
-        const double P = ldexp(combine(out, nsplit, m * J16, s * J16 + c * limbs, limbs), -shc);
-        const double iv = inv[s];
-        const double uv = scale * iv * iv * (P - twof[s] * tc);
-        u[c * ldu + s] = uv;
-        mx = fmax(mx, fabs(uv));
+    __shared__ double tile[kEpiRows][kMaxCols + 1];
+    __shared__ double cmax[kMaxCols];
+    const int64_t split_stride = rows * J16;
+    const int64_t ntiles = (rows + kEpiRows - 1) / kEpiRows;
+    for (int c = threadIdx.x; c < k; c += 256) cmax[c] = 0.0;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int64_t r0 = t * kEpiRows;
+        __syncthreads();
+        for (int q = threadIdx.x; q < kEpiRows * k; q += 256) {
+            const int rl = q / k, c = q % k;
+            const int64_t r = r0 + rl;
+            double val = 0.0;
+            if (r < rows) {
+                const double P =
+                    ldexp(combine(out, nsplit, split_stride, r * J16 + c * limbs, limbs), -sh[c]);
+                if (mode == 0) {
+                    const double iv = inv[r];
+                    val = th[c * 5 + 1] * inv_m * iv * iv * (P - twof[r] * TC[c]);
+                } else {
+                    val = P - TC[c];
+                }
+            }
+            tile[rl][c] = val;
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < kEpiRows * k; q += 256) {
+            const int c = q / kEpiRows, rl = q % kEpiRows;
+            const int64_t r = r0 + rl;
+            if (r < rows) res[c * ldr + r] = tile[rl][c];
+        }
+        if (mode == 0) {
+            // running per-column maxima (one wave per column, fixed order)
+            for (int c = threadIdx.x >> 6; c < k; c += 4) {
+                double m = 0.0;
+                const int rl = threadIdx.x & 63;
+                if (r0 + rl < rows) m = fabs(tile[rl][c]);
+                m = wave_max(m);
+                if ((threadIdx.x & 63) == 0) cmax[c] = fmax(cmax[c], m);
+            }
+        }
     }
-    const double r = block_max(mx);
-    if (threadIdx.x == 0) part[c * kMaxParts + blockIdx.x] = r;
+    __syncthreads();
+    if (mode == 0 && part)
+        for (int c = threadIdx.x; c < k; c += 256) part[c * kMaxParts + blockIdx.x] = cmax[c];
 }
 
-// pass-2 epilogue: GZ = combine * 2^-sh - C
-__global__ __launch_bounds__(256) void k_epi2(const int32_t* __restrict__ out, int nsplit,
-                                              int64_t n, int J16, int limbs,
-                                              const int* __restrict__ sh,
-                                              const double* __restrict__ Cs,
-                                              double* __restrict__ GZ, int64_t ldg,
-                                              const int* skip) {
-    if (skip && *skip) return;
-    const int c = blockIdx.y;
-    const int shc = sh[c];
-    const double cc = Cs[c];
-    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
-        GZ[c * ldg + i] = ldexp(combine(out, nsplit, n * J16, i * J16 + c * limbs, limbs), -shc) - cc;
+int ceil_pow2(int64_t x) {
+    int p = 1;
+    while (p < x) p <<= 1;
+    return p;
 }
 
-__global__ void k_reduce_sum(const double* __restrict__ part, int nparts, int k, double* out,
-                             const int* skip) {
-    if (skip && *skip) return;
-    const int c = threadIdx.x;
-    if (c >= k) return;
-    double s = 0.0;
-    for (int j = 0; j < nparts; ++j) s += part[c * kMaxParts + j];
-    out[c] = s;
-}
-
-struct Launch {
-    int rt, jf;
-};
-
-int jf_for(int J) {
-    const int f = (J + 15) / 16;
-    return f <= 1 ? 1 : f <= 2 ? 2 : f <= 4 ? 4 : 8;
-}
-
-void launch_mfma(hipStream_t s, int RT, int JF, const uint8_t* rows, int64_t pitch, int64_t nrows,
-                 int64_t n_ss, int nsplit, const uint8_t* Bq, int32_t* out, const int* skip) {
-    const int64_t per = (n_ss + nsplit - 1) / nsplit;
-    dim3 grid((unsigned)((nrows + 64 * RT - 1) / (64 * RT)), (unsigned)nsplit);
-#define GLOBE_MFMA_CASE(R, F)                                                                 \
-    if (RT == R && JF == F) {                                                                 \
-        k_geno_mfma<R, F><<<grid, 256, 0, s>>>(rows, pitch, nrows, n_ss, per, Bq, out, nrows, \
-                                               skip);                                          \
-        HIP_CHECK(hipGetLastError());                                                         \
-        return;                                                                               \
-    }
-    GLOBE_MFMA_CASE(4, 1)
-    GLOBE_MFMA_CASE(4, 2)
-    GLOBE_MFMA_CASE(2, 4)
-    GLOBE_MFMA_CASE(2, 8)
-    GLOBE_MFMA_CASE(2, 1)
-    GLOBE_MFMA_CASE(2, 2)
-    GLOBE_MFMA_CASE(1, 4)
-    GLOBE_MFMA_CASE(1, 8)
-#undef GLOBE_MFMA_CASE
-    throw Error(GLOBE_E_INVALID, "no MFMA tile for this shape");
-}
-
-int rt_for(int JF) { return JF <= 2 ? 4 : 2; }
-
-// split K so that the grid has at least ~2048 blocks, keeping >= 4 super-steps each
-int split_for(int64_t nrows, int RT, int64_t n_ss) {
-    const int64_t rb = (nrows + 64 * RT - 1) / (64 * RT);
-    int64_t sp = (2048 + rb - 1) / rb;
-    const int64_t maxsp = n_ss / 4 > 1 ? n_ss / 4 : 1;
-    if (sp > maxsp) sp = maxsp;
-    if (sp > 64) sp = 64;
-    if (sp < 1) sp = 1;
-    return (int)sp;
+// Split-K: enough blocks to fill the chip (>= 4 per CU), each split >= 8 super-steps;
+// wide contractions split at least 8 ways so each XCD streams one K-chunk of Bq.
+int choose_split(int64_t R, int64_t n_ss, bool wide) {
+    int S = 1;
+    while (R * S < 1024 && S < 64) S <<= 1;
+    if (wide && S < 8) S = 8;
+    while (S > 1 && n_ss / S < 8) S >>= 1;
+    return S;
 }
 
 }  // namespace
@@ -346,16 +481,21 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     hipStream_t s = c->stream;
     const int64_t n = c->n, m = c->m;
     const int J = k * limbs;
-    const int JF = jf_for(J);
     require(J <= 128, GLOBE_E_INVALID, "k * limbs must be <= 128 for the MFMA contraction");
-    const int RT = rt_for(JF);
+    const bool wide = J > 16;
+    const int JT = wide ? (J <= 32 ? 1 : J <= 64 ? 2 : 4) : 1;
+    const int J16 = wide ? 32 * JT : 16;
+    const int nfrag = wide ? JT : 1;
+    const int rows_blk = wide ? kWideRows : kThinRows;
     const int64_t Kpad1 = c->pitch_s * 4, Kpad2 = c->pitch_i * 4;  // genotypes per padded row
     const int64_t nss1 = Kpad1 / 256, nss2 = Kpad2 / 256;
-    const int sp1 = split_for(m, RT, nss1), sp2 = split_for(n, RT, nss2);
-    const size_t bq1 = (size_t)Kpad1 * JF * 16, bq2 = (size_t)Kpad2 * JF * 16;
-    const size_t o1 = (size_t)sp1 * m * JF * 16 * 4, o2 = (size_t)sp2 * n * JF * 16 * 4;
-    // scratch: Bq (max of both passes) | out32 (max) | sh (2 x kMaxCols ints)
-    const size_t bq = bq1 > bq2 ? bq1 : bq2, ob = o1 > o2 ? o1 : o2;
+    const int64_t R1 = (m + rows_blk - 1) / rows_blk, R2 = (n + rows_blk - 1) / rows_blk;
+    const int sp1 = choose_split(R1, nss1, wide), sp2 = choose_split(R2, nss2, wide);
+    const size_t bq1 = (size_t)Kpad1 * J16, bq2 = (size_t)Kpad2 * J16;
+    const size_t o1 = (size_t)sp1 * m * J16 * 4, o2 = (size_t)sp2 * n * J16 * 4;
+    // the wide kernel prefetches one super-step past the end of its split: pad Bq
+    const size_t bq = (bq1 > bq2 ? bq1 : bq2) + (size_t)512 * 16 * JT;
+    const size_t ob = o1 > o2 ? o1 : o2;
     c->w_mix.ensure(round_up(bq, 256) + round_up(ob, 256) + 2 * kMaxCols * sizeof(int));
     uint8_t* Bq = c->w_mix.as<uint8_t>();
     int32_t* out = reinterpret_cast<int32_t*>(Bq + round_up(bq, 256));
@@ -368,19 +508,8 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     double* part = c->w_part2.as<double>();
     double* U = c->w_U.as<double>();
     const double* th = theta_cols;
-
-    // pass 1: P1[s][c] = sum_i g_is S0[c][i] on the SNP-major rows
-    const int npn = parts_for(n);
-    k_colmax_parts<<<dim3(npn, k), 256, 0, s>>>(n, S0, ldn, part, skip);
-    k_scales<<<1, kMaxCols, 0, s>>>(part, npn, k, limbs, sh1, skip);
-    const int qg1 = parts_for(Kpad1);
-    k_quantize<<<dim3(qg1, JF * 16), 256, 0, s>>>(S0, ldn, n, Kpad1, k, limbs, JF, sh1, nullptr, Bq,
-                                                  part, skip);
-    k_reduce_sum<<<1, kMaxCols, 0, s>>>(part, qg1, k, c->w_T.as<double>(), skip);
-    HIP_CHECK(hipGetLastError());
-    // algorithmic work per pass: rows x ceil(K/4) packed bytes, 2 N M k flops
     const double flops = 2.0 * (double)n * (double)m * k;
-    auto timed = [&](int pass, int64_t rows, int64_t K, auto&& fn) {
+    auto timed = [&](int pass, int64_t rows, int64_t K, int64_t Kpad, auto&& fn) {
         if (!c->prof_on) {
             fn();
             return;
@@ -392,28 +521,54 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
         e.iter = c->prof_iter;
         e.bytes = (double)rows * (double)((K + 3) / 4);
         e.flops = flops;
-        e.issued = 2.0 * (double)rows * (double)(pass == 0 ? Kpad1 : Kpad2) * (JF * 16);
+        e.issued = 2.0 * (double)rows * (double)Kpad * J16;
         HIP_CHECK(hipEventRecord(e.a, s));
         fn();
         HIP_CHECK(hipEventRecord(e.b, s));
         c->prof.push_back(e);
     };
-    timed(0, m, n, [&] { launch_mfma(s, RT, JF, c->d_snp.as<uint8_t>(), c->pitch_s, m, nss1, sp1, Bq, out, skip); });
-    const int npm = parts_for(m);
-    k_epi1<<<dim3(npm, k), 256, 0, s>>>(out, sp1, m, JF * 16, limbs, sh1, c->w_T.as<double>(),
-                                         c->d_twof.as<double>(), c->d_inv.as<double>(), th,
-                                         1.0 / (double)m, U, m, part, skip);
+    auto contract = [&](const uint8_t* rows, int64_t pitch, int64_t nrows, int64_t nss, int64_t R, int S) {
+        const int64_t per = (nss + S - 1) / S;
+        const unsigned grid = (unsigned)(R * S);
+        if (!wide) {
+            k_geno_thin<<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, S, Bq, out, skip);
+        } else if (JT == 1) {
+            k_geno_wide<1><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, S, Bq, out, skip);
+        } else if (JT == 2) {
+            k_geno_wide<2><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, S, Bq, out, skip);
+        } else {
+            k_geno_wide<4><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, S, Bq, out, skip);
+        }
+        HIP_CHECK(hipGetLastError());
+    };
+
+    // pass 1: P1[s][c] = sum_i g_is S0[c][i] on the SNP-major rows
+    const int npn = parts_for(n);
+    k_colmax_parts<<<dim3(npn, k), 256, 0, s>>>(n, S0, ldn, part, skip);
+    k_finish_cols<<<1, 256, 0, s>>>(part, npn, k, 1, limbs, sh1, nullptr, skip);
+    const int qg1 = parts_for(Kpad1);
+    k_quantize<<<dim3(qg1, J16), 256, 0, s>>>(S0, ldn, n, Kpad1, k, limbs, nfrag, wide, sh1, nullptr,
+                                              Bq, part, skip);
+    k_finish_cols<<<1, 256, 0, s>>>(part, qg1, k, 0, limbs, nullptr, c->w_T.as<double>(), skip);
+    HIP_CHECK(hipGetLastError());
+    timed(0, m, n, Kpad1, [&] { contract(c->d_snp.as<uint8_t>(), c->pitch_s, m, nss1, R1, sp1); });
+    const int64_t eb1 = std::min<int64_t>((m + kEpiRows - 1) / kEpiRows, kMaxParts);
+    k_epilogue<<<(unsigned)eb1, 256, 0, s>>>(out, sp1, m, J16, limbs, k, sh1, 0, c->w_T.as<double>(),
+                                             c->d_twof.as<double>(), c->d_inv.as<double>(), th,
+                                             1.0 / (double)m, U, m, part, skip);
     HIP_CHECK(hipGetLastError());
     // pass 2: GZ[c][i] = sum_s g_is u[c][s] - C[c] on the individual-major rows
-    k_scales<<<1, kMaxCols, 0, s>>>(part, npm, k, limbs, sh2, skip);
+    k_finish_cols<<<1, 256, 0, s>>>(part, (int)(eb1 < kMaxParts ? eb1 : kMaxParts), k, 1, limbs, sh2,
+                                    nullptr, skip);
     const int qg2 = parts_for(Kpad2);
-    k_quantize<<<dim3(qg2, JF * 16), 256, 0, s>>>(U, m, m, Kpad2, k, limbs, JF, sh2,
-                                                  c->d_twof.as<double>(), Bq, part, skip);
-    k_reduce_sum<<<1, kMaxCols, 0, s>>>(part, qg2, k, c->w_C.as<double>(), skip);
+    k_quantize<<<dim3(qg2, J16), 256, 0, s>>>(U, m, m, Kpad2, k, limbs, nfrag, wide, sh2,
+                                              c->d_twof.as<double>(), Bq, part, skip);
+    k_finish_cols<<<1, 256, 0, s>>>(part, qg2, k, 0, limbs, nullptr, c->w_C.as<double>(), skip);
     HIP_CHECK(hipGetLastError());
-    timed(1, n, m, [&] { launch_mfma(s, RT, JF, c->d_ind.as<uint8_t>(), c->pitch_i, n, nss2, sp2, Bq, out, skip); });
-    k_epi2<<<dim3(npn, k), 256, 0, s>>>(out, sp2, n, JF * 16, limbs, sh2, c->w_C.as<double>(), GZ,
-                                         ldg, skip);
+    timed(1, n, m, Kpad2, [&] { contract(c->d_ind.as<uint8_t>(), c->pitch_i, n, nss2, R2, sp2); });
+    const int64_t eb2 = std::min<int64_t>((n + kEpiRows - 1) / kEpiRows, kMaxParts);
+    k_epilogue<<<(unsigned)eb2, 256, 0, s>>>(out, sp2, n, J16, limbs, k, sh2, 1, c->w_C.as<double>(),
+                                             nullptr, nullptr, nullptr, 0.0, GZ, ldg, nullptr, skip);
     HIP_CHECK(hipGetLastError());
 }
 

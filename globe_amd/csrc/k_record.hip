@@ -221,19 +221,32 @@ __global__ __launch_bounds__(kBlock) void k_axpy_cols(int64_t nrec, double* __re
 // PCG per PCGsolver.cpp:1343-1443 / varCompEst.cpp:482-527 with every scalar on
 // the device; once `done` is set every later kernel of the solve is a no-op, so the
 // host may enqueue iterations ahead of the convergence test.
-__global__ void k_pcg_init(PcgState* st, const double* __restrict__ part_bb,
-                           const double* __restrict__ part_rr, const double* __restrict__ part_rz,
-                           int nparts, int k) {
+// sums[c] = sum_j part[c][j] for c < k: one wave per column, lanes over partials,
+// then a fixed shuffle tree (deterministic). Ends with a barrier.
+__device__ void wave_col_sums(const double* __restrict__ part, int nparts, int k, double* sums) {
+    const int lane = threadIdx.x & 63;
+    for (int c = threadIdx.x >> 6; c < k; c += blockDim.x >> 6) {
+        double v = 0.0;
+        for (int j = lane; j < nparts; j += 64) v += part[c * kMaxParts + j];
+        v = wave_sum(v);
+        if (lane == 0) sums[c] = v;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcg_init(PcgState* st, const double* __restrict__ part_bb,
+                                                     const double* __restrict__ part_rr,
+                                                     const double* __restrict__ part_rz, int nparts,
+                                                     int k) {
     const int c = threadIdx.x;
     __shared__ double ratio_s[kMaxCols];
     __shared__ int act_s[kMaxCols];
+    __shared__ double sbb[kMaxCols], srr[kMaxCols], srz[kMaxCols];
+    wave_col_sums(part_bb, nparts, k, sbb);
+    wave_col_sums(part_rr, nparts, k, srr);
+    wave_col_sums(part_rz, nparts, k, srz);
     if (c < k) {
-        double bb = 0.0, rr = 0.0, rz = 0.0;
-        for (int j = 0; j < nparts; ++j) {
-            bb += part_bb[c * kMaxParts + j];
-            rr += part_rr[c * kMaxParts + j];
-            rz += part_rz[c * kMaxParts + j];
-        }
+        const double bb = sbb[c], rr = srr[c], rz = srz[c];
         const double nb = sqrt(bb), nr = sqrt(rr);
         st->bnorm[c] = nb;
         st->rs[c] = rz;
@@ -264,17 +277,14 @@ __global__ void k_pcg_init(PcgState* st, const double* __restrict__ part_bb,
     }
 }
 
-__global__ void k_pcg_alpha(PcgState* st, const double* __restrict__ part_pvp, int nparts, int k) {
+__global__ __launch_bounds__(kBlock) void k_pcg_alpha(PcgState* st, const double* __restrict__ part_pvp,
+                                                      int nparts, int k) {
     if (st->done) return;
+    __shared__ double sums[kMaxCols];
+    wave_col_sums(part_pvp, nparts, k, sums);
     const int c = threadIdx.x;
     if (c >= k) return;
-    if (!st->active[c]) {
-        st->alpha[c] = 0.0;
-        return;
-    }
-    double s = 0.0;
-    for (int j = 0; j < nparts; ++j) s += part_pvp[c * kMaxParts + j];
-    st->alpha[c] = st->rs[c] / s;
+    st->alpha[c] = st->active[c] ? st->rs[c] / sums[c] : 0.0;
 }
 
 __global__ __launch_bounds__(kBlock) void k_pcg_xr(PcgState* st, int64_t nrec, double* __restrict__ x,
@@ -307,15 +317,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_xr(PcgState* st, int64_t nrec, d
 //          consecutive iterations.
 //   rule 1 (varCompEst.cpp:505): every column |r| < tol |b|.
 //   rule 2: per-column freeze; done when no column is active.
-__global__ void k_pcg_check(PcgState* st, const double* __restrict__ part_rr, int nparts, int k) {
+__global__ __launch_bounds__(kBlock) void k_pcg_check(PcgState* st, const double* __restrict__ part_rr,
+                                                      int nparts, int k) {
     if (st->done) return;
     __shared__ double nr_s[kMaxCols];
-    const int c = threadIdx.x;
-    if (c < k) {
-        double s = 0.0;
-        for (int j = 0; j < nparts; ++j) s += part_rr[c * kMaxParts + j];
-        nr_s[c] = sqrt(s);
-    }
+    wave_col_sums(part_rr, nparts, k, nr_s);
+    if (threadIdx.x < k) nr_s[threadIdx.x] = sqrt(nr_s[threadIdx.x]);
     __syncthreads();
     if (threadIdx.x != 0) return;
     const int it = st->iters + 1;
@@ -374,12 +381,14 @@ __global__ void k_pcg_check(PcgState* st, const double* __restrict__ part_rr, in
     st->done = done;
 }
 
-__global__ void k_pcg_beta(PcgState* st, const double* __restrict__ part_rz, int nparts, int k) {
+__global__ __launch_bounds__(kBlock) void k_pcg_beta(PcgState* st, const double* __restrict__ part_rz,
+                                                     int nparts, int k) {
     if (st->done) return;
+    __shared__ double sums[kMaxCols];
+    wave_col_sums(part_rz, nparts, k, sums);
     const int c = threadIdx.x;
     if (c >= k || !st->active[c]) return;
-    double s = 0.0;
-    for (int j = 0; j < nparts; ++j) s += part_rz[c * kMaxParts + j];
+    const double s = sums[c];
     st->beta[c] = s / st->rs[c];
     st->rs[c] = s;
 }
@@ -559,12 +568,12 @@ void launch_axpy_cols(hipStream_t s, int64_t nrec, double* y, int64_t ldy, const
 
 void launch_pcg_init(hipStream_t s, PcgState* st, const double* part_bb, const double* part_rr,
                      const double* part_rz, int nparts, int k) {
-    k_pcg_init<<<1, kMaxCols, 0, s>>>(st, part_bb, part_rr, part_rz, nparts, k);
+    k_pcg_init<<<1, kBlock, 0, s>>>(st, part_bb, part_rr, part_rz, nparts, k);
     LAUNCH_CHECK();
 }
 
 void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* part_pvp, int nparts, int k) {
-    k_pcg_alpha<<<1, kMaxCols, 0, s>>>(st, part_pvp, nparts, k);
+    k_pcg_alpha<<<1, kBlock, 0, s>>>(st, part_pvp, nparts, k);
     LAUNCH_CHECK();
 }
 
@@ -576,12 +585,12 @@ void launch_pcg_xr(hipStream_t s, PcgState* st, int64_t nrec, double* x, int64_t
 }
 
 void launch_pcg_check(hipStream_t s, PcgState* st, const double* part_rr, int nparts, int k) {
-    k_pcg_check<<<1, kMaxCols, 0, s>>>(st, part_rr, nparts, k);
+    k_pcg_check<<<1, kBlock, 0, s>>>(st, part_rr, nparts, k);
     LAUNCH_CHECK();
 }
 
 void launch_pcg_beta(hipStream_t s, PcgState* st, const double* part_rz, int nparts, int k) {
-    k_pcg_beta<<<1, kMaxCols, 0, s>>>(st, part_rz, nparts, k);
+    k_pcg_beta<<<1, kBlock, 0, s>>>(st, part_rz, nparts, k);
     LAUNCH_CHECK();
 }
 
