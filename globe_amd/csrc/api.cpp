@@ -474,7 +474,9 @@ void stage_rows(globe_ctx* c, int64_t m, int64_t pitch_in, float min_maf, float 
     c->pitch_s = round_up((n + 3) / 4, kPitchAlign);
     c->d_snp.release();
     c->d_ind.release();
-    c->d_snp.ensure((size_t)std::max<int64_t>(m, 1) * c->pitch_s);
+    // rows are padded to a multiple of kRowPad with code-11 bytes so the contraction
+    // kernels load whole row blocks without bounds checks
+    c->d_snp.ensure((size_t)round_up(std::max<int64_t>(m, 1), kRowPad) * c->pitch_s);
     const int64_t chunk_rows = std::max<int64_t>(1, std::min<int64_t>(m, (int64_t)(1ll << 30) / pitch_in));
     DevBuf raw, d_alt, d_miss, d_idx;
     raw.ensure((size_t)chunk_rows * pitch_in);
@@ -518,9 +520,14 @@ void stage_rows(globe_ctx* c, int64_t m, int64_t pitch_in, float min_maf, float 
     raw.release();
     c->m_total = m;
     c->m = kept;
+    {
+        const int64_t pad_rows = round_up(std::max<int64_t>(kept, 1), kRowPad) - kept;
+        HIP_CHECK(hipMemsetAsync(c->d_snp.as<uint8_t>() + kept * c->pitch_s, 0xFF,
+                                 (size_t)pad_rows * c->pitch_s, s));
+    }
     c->pitch_i = round_up((kept + 3) / 4, kPitchAlign);
-    c->d_ind.ensure((size_t)n * c->pitch_i);
-    HIP_CHECK(hipMemsetAsync(c->d_ind.p, 0xFF, (size_t)n * c->pitch_i, s));
+    c->d_ind.ensure((size_t)round_up(n, kRowPad) * c->pitch_i);
+    HIP_CHECK(hipMemsetAsync(c->d_ind.p, 0xFF, (size_t)round_up(n, kRowPad) * c->pitch_i, s));
     launch_transpose_geno(s, c->d_snp.as<uint8_t>(), kept, c->pitch_s, n, c->d_ind.as<uint8_t>(), c->pitch_i);
     c->d_freq.ensure(sizeof(float) * std::max<int64_t>(kept, 1));
     c->d_twof.ensure(sizeof(double) * std::max<int64_t>(kept, 1));

@@ -2,6 +2,7 @@
 // workspaces and the device PCG state of one solve stream.
 #pragma once
 #include <array>
+#include <map>
 
 #include "globe_internal.h"
 
@@ -38,6 +39,8 @@ struct globe_ctx {
     // operator workspaces
     globe::DevBuf w_S0, w_S1, w_GZ, w_P1, w_U, w_P2, w_T, w_C, w_part, w_part2, w_part3, w_part4;
     globe::DevBuf w_mix;         // mixed-precision contraction scratch
+    // tuned contraction variant per shape key -> (rows per wave, split-K)
+    std::map<std::string, std::pair<int, int>> tuned;
 
     // PCG workspaces and state
     globe::DevBuf p_state, p_r, p_z, p_p, p_vp;

@@ -36,6 +36,7 @@ constexpr int kMaxParts = 1024;     // partial-sum slots per column (grid cap)
 constexpr int kMaxCols = 64;        // max right-hand sides per solve
 constexpr int kGenoTile = 512;      // genotypes per K-step of the row-dot kernels
 constexpr int64_t kPitchAlign = 128;  // bytes: row pitch multiple (512 genotypes)
+constexpr int64_t kRowPad = 512;      // packed layouts hold a multiple of this many rows
 
 inline int parts_for(int64_t count) {
     int64_t b = (count + kBlock - 1) / kBlock;

@@ -80,9 +80,32 @@ __device__ inline void block_coords(int64_t b, int64_t R, int S, int64_t& rb, in
 }
 
 // ---------------------------------------------------------------- thin kernel
-constexpr int kThinRT = 8;                      // row fragments of 16 per wave
-constexpr int kThinRows = 4 * 16 * kThinRT;      // rows per block
+// RT row fragments of 16 rows per wave (64 RT rows per block). Row blocks are read
+// without bounds checks: the packed layouts are padded to a multiple of 512 rows.
+// Two register sets alternate so the next super-step's loads overlap the current
+// one's decode and MFMAs.
+template <int RT>
+__device__ inline void thin_load(const uint8_t* base, int64_t frag_stride, uint32_t loff, int64_t ss,
+                                 const uint4* bq, uint4 (&A)[RT], uint4 (&B)[4]) {
+#pragma unroll
+    for (int a = 0; a < RT; ++a)
+        A[a] = *reinterpret_cast<const uint4*>(base + a * frag_stride + loff + ss * 64);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) B[s] = bq[(4 * ss + s) * 64];
+}
 
+template <int RT>
+__device__ inline void thin_mma(const uint4 (&A)[RT], const uint4 (&B)[4], v4i (&acc)[RT]) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const v4i bv = *reinterpret_cast<const v4i*>(&B[s]);
+#pragma unroll
+        for (int a = 0; a < RT; ++a)
+            acc[a] = __builtin_amdgcn_mfma_i32_16x16x64_i8(decode16(dword_of(A[a], s)), bv, acc[a], 0, 0, 0);
+    }
+}
+
+template <int RT>
 __global__ __launch_bounds__(256) void k_geno_thin(const uint8_t* __restrict__ rows, int64_t pitch,
                                                    int64_t nrows, int64_t n_ss, int64_t ss_per_split,
                                                    int64_t R, int S, const uint8_t* __restrict__ Bq,
@@ -93,56 +116,29 @@ __global__ __launch_bounds__(256) void k_geno_thin(const uint8_t* __restrict__ r
     block_coords(blockIdx.x, R, S, rb, sp);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, rr = lane & 15;
-    const int64_t row_base = rb * kThinRows + wave * (16 * kThinRT);
+    const int64_t row_base = rb * (64 * RT) + wave * (16 * RT);
+    const uint8_t* base = rows + row_base * pitch;  // uniform per wave
+    const int64_t fstride = 16 * pitch;
+    const uint32_t loff = (uint32_t)(rr * pitch + 16 * g);
     const int64_t ss0 = (int64_t)sp * ss_per_split;
     const int64_t ss1 = ss0 + ss_per_split < n_ss ? ss0 + ss_per_split : n_ss;
-
-    v4i acc[kThinRT];
-#pragma unroll
-    for (int a = 0; a < kThinRT; ++a) acc[a] = v4i{0, 0, 0, 0};
-    const uint8_t* rp[kThinRT];
-    bool rv[kThinRT];
-#pragma unroll
-    for (int a = 0; a < kThinRT; ++a) {
-        const int64_t r = row_base + a * 16 + rr;
-        rv[a] = r < nrows;
-        rp[a] = rows + (rv[a] ? r : 0) * pitch + 16 * g;
-    }
     const uint4* bq = reinterpret_cast<const uint4*>(Bq) + lane;  // [ks][64 lanes] x 16 B
-    uint4 an[kThinRT], bn[4];
-    if (ss0 < ss1) {
+
+    v4i acc[RT];
 #pragma unroll
-        for (int a = 0; a < kThinRT; ++a)
-            an[a] = rv[a] ? *reinterpret_cast<const uint4*>(rp[a] + ss0 * 64)
-                          : make_uint4(kZeroCodes, kZeroCodes, kZeroCodes, kZeroCodes);
-#pragma unroll
-        for (int s = 0; s < 4; ++s) bn[s] = bq[(4 * ss0 + s) * 64];
-    }
-    for (int64_t ss = ss0; ss < ss1; ++ss) {
-        uint4 ac[kThinRT], bc[4];
-#pragma unroll
-        for (int a = 0; a < kThinRT; ++a) ac[a] = an[a];
-#pragma unroll
-        for (int s = 0; s < 4; ++s) bc[s] = bn[s];
-        if (ss + 1 < ss1) {
-#pragma unroll
-            for (int a = 0; a < kThinRT; ++a)
-                if (rv[a]) an[a] = *reinterpret_cast<const uint4*>(rp[a] + (ss + 1) * 64);
-#pragma unroll
-            for (int s = 0; s < 4; ++s) bn[s] = bq[(4 * (ss + 1) + s) * 64];
-        }
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const v4i bv = *reinterpret_cast<const v4i*>(&bc[s]);
-#pragma unroll
-            for (int a = 0; a < kThinRT; ++a)
-                acc[a] = __builtin_amdgcn_mfma_i32_16x16x64_i8(decode16(dword_of(ac[a], s)), bv,
-                                                               acc[a], 0, 0, 0);
-        }
+    for (int a = 0; a < RT; ++a) acc[a] = v4i{0, 0, 0, 0};
+    uint4 A0[RT], A1[RT], B0[4], B1[4];
+    if (ss0 < ss1) thin_load<RT>(base, fstride, loff, ss0, bq, A0, B0);
+    for (int64_t ss = ss0; ss < ss1; ss += 2) {
+        if (ss + 1 < ss1) thin_load<RT>(base, fstride, loff, ss + 1, bq, A1, B1);
+        thin_mma<RT>(A0, B0, acc);
+        if (ss + 1 >= ss1) break;
+        if (ss + 2 < ss1) thin_load<RT>(base, fstride, loff, ss + 2, bq, A0, B0);
+        thin_mma<RT>(A1, B1, acc);
     }
     int32_t* o = out + (int64_t)sp * nrows * 16;
 #pragma unroll
-    for (int a = 0; a < kThinRT; ++a)
+    for (int a = 0; a < RT; ++a)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int64_t r = row_base + a * 16 + 4 * g + v;
@@ -167,6 +163,9 @@ __global__ __launch_bounds__(256) void k_geno_wide(const uint8_t* __restrict__ r
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 5, rr = lane & 31;
     const int64_t row_base = rb * kWideRows + wave * 64;
+    const uint8_t* base = rows + row_base * pitch;  // uniform per wave
+    const int64_t fstride = 32 * pitch;
+    const uint32_t loff = (uint32_t)(rr * pitch + 32 * g);
     const int64_t ss0 = (int64_t)sp * ss_per_split;
     const int64_t ss1 = ss0 + ss_per_split < n_ss ? ss0 + ss_per_split : n_ss;
 
@@ -177,75 +176,51 @@ __global__ __launch_bounds__(256) void k_geno_wide(const uint8_t* __restrict__ r
         for (int b = 0; b < JT; ++b)
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[a][b][v] = 0;
-    const uint8_t* rp[2];
-    bool rv[2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        const int64_t r = row_base + a * 32 + rr;
-        rv[a] = r < nrows;
-        rp[a] = rows + (rv[a] ? r : 0) * pitch + 32 * g;
-    }
-    // digits of one super-step: 8 * JT KB = 512 * JT uint4; thread t copies uint4
-    // t + 256 q for q < 2 JT, spread over the 8 K-steps (JT / 4 per K-step and thread
-    // when JT = 4, one copy every 4 / JT K-steps otherwise)
-    const uint4* bsrc = reinterpret_cast<const uint4*>(Bq);
+    // digits of one super-step: 8 JT KB = 512 JT uint4, thread t copies t + 256 q, q < 2 JT
     constexpr int kCopies = 2 * JT;
-    uint4 an[2][2];
+    const uint4* bsrc = reinterpret_cast<const uint4*>(Bq) + threadIdx.x;
+    uint4 an[2][2], bn[kCopies];
     if (ss0 < ss1) {
-        const uint4* s4 = bsrc + ss0 * (512 * JT);
-        uint4* d4 = reinterpret_cast<uint4*>(lds[0]);
+        uint4* d4 = reinterpret_cast<uint4*>(lds[0]) + threadIdx.x;
 #pragma unroll
-        for (int q = 0; q < kCopies; ++q) d4[threadIdx.x + 256 * q] = s4[threadIdx.x + 256 * q];
+        for (int q = 0; q < kCopies; ++q) d4[256 * q] = bsrc[ss0 * (512 * JT) + 256 * q];
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-            an[a][0] = rv[a] ? *reinterpret_cast<const uint4*>(rp[a] + ss0 * 64)
-                             : make_uint4(kZeroCodes, kZeroCodes, kZeroCodes, kZeroCodes);
-            an[a][1] = rv[a] ? *reinterpret_cast<const uint4*>(rp[a] + ss0 * 64 + 16)
-                             : make_uint4(kZeroCodes, kZeroCodes, kZeroCodes, kZeroCodes);
+            an[a][0] = *reinterpret_cast<const uint4*>(base + a * fstride + loff + ss0 * 64);
+            an[a][1] = *reinterpret_cast<const uint4*>(base + a * fstride + loff + ss0 * 64 + 16);
         }
     }
     __syncthreads();
     int cur = 0;
     for (int64_t ss = ss0; ss < ss1; ++ss) {
-        uint4 ac[2][2];
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            ac[a][0] = an[a][0];
-            ac[a][1] = an[a][1];
-        }
+        const uint4 c00 = an[0][0], c01 = an[0][1], c10 = an[1][0], c11 = an[1][1];
         const bool more = ss + 1 < ss1;
         if (more) {
+            // the whole next super-step (packed rows and digits) is requested up front
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
-                if (rv[a]) {
-                    an[a][0] = *reinterpret_cast<const uint4*>(rp[a] + (ss + 1) * 64);
-                    an[a][1] = *reinterpret_cast<const uint4*>(rp[a] + (ss + 1) * 64 + 16);
-                }
+            for (int a = 0; a < 2; ++a) {
+                an[a][0] = *reinterpret_cast<const uint4*>(base + a * fstride + loff + (ss + 1) * 64);
+                an[a][1] = *reinterpret_cast<const uint4*>(base + a * fstride + loff + (ss + 1) * 64 + 16);
+            }
+#pragma unroll
+            for (int q = 0; q < kCopies; ++q) bn[q] = bsrc[(ss + 1) * (512 * JT) + 256 * q];
         }
-        const uint4* nsrc = bsrc + (ss + 1) * (512 * JT);
-        uint4* ndst = reinterpret_cast<uint4*>(lds[cur ^ 1]);
         const uint8_t* cl = lds[cur];
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-            // stage a share of the next super-step's digits
-            uint4 pre[kCopies / 8 > 0 ? kCopies / 8 : 1];
-            constexpr int per = kCopies >= 8 ? kCopies / 8 : 1;
-            const bool mine = kCopies >= 8 || (s % (8 / kCopies)) == 0;
-            const int qbase = kCopies >= 8 ? s * per : s / (8 / kCopies);
-            if (more && mine)
-#pragma unroll
-                for (int q = 0; q < per; ++q) pre[q] = nsrc[threadIdx.x + 256 * (qbase + q)];
-            const v4i a0 = decode16(dword_of(s < 4 ? ac[0][0] : ac[0][1], s & 3));
-            const v4i a1 = decode16(dword_of(s < 4 ? ac[1][0] : ac[1][1], s & 3));
+            const v4i a0 = decode16(dword_of(s < 4 ? c00 : c01, s & 3));
+            const v4i a1 = decode16(dword_of(s < 4 ? c10 : c11, s & 3));
 #pragma unroll
             for (int b = 0; b < JT; ++b) {
                 const v4i bv = *reinterpret_cast<const v4i*>(cl + ((s * JT + b) * 64 + lane) * 16);
                 acc[0][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, bv, acc[0][b], 0, 0, 0);
                 acc[1][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, bv, acc[1][b], 0, 0, 0);
             }
-            if (more && mine)
+        }
+        if (more) {
+            uint4* d4 = reinterpret_cast<uint4*>(lds[cur ^ 1]) + threadIdx.x;
 #pragma unroll
-                for (int q = 0; q < per; ++q) ndst[threadIdx.x + 256 * (qbase + q)] = pre[q];
+            for (int q = 0; q < kCopies; ++q) d4[256 * q] = bn[q];
         }
         __syncthreads();
         cur ^= 1;
@@ -457,12 +432,6 @@ __global__ __launch_bounds__(256) void k_epilogue(const int32_t* __restrict__ ou
         for (int c = threadIdx.x; c < k; c += 256) part[c * kMaxParts + blockIdx.x] = cmax[c];
 }
 
-int ceil_pow2(int64_t x) {
-    int p = 1;
-    while (p < x) p <<= 1;
-    return p;
-}
-
 // Split-K: enough blocks to fill the chip (>= 4 per CU), each split >= 8 super-steps;
 // wide contractions split at least 8 ways so each XCD streams one K-chunk of Bq.
 int choose_split(int64_t R, int64_t n_ss, bool wide) {
@@ -473,7 +442,55 @@ int choose_split(int64_t R, int64_t n_ss, bool wide) {
     return S;
 }
 
+struct Variant {
+    int rt;  // thin: row fragments per wave (4 or 8); wide: 2
+    int S;   // split-K
+};
+
+int64_t rows_per_block(bool wide, int rt) { return wide ? kWideRows : 64 * rt; }
+
+void launch_variant(hipStream_t s, bool wide, int JT, const Variant& v, const uint8_t* rows,
+                    int64_t pitch, int64_t nrows, int64_t nss, const uint8_t* Bq, int32_t* out,
+                    const int* skip) {
+    const int64_t R = (nrows + rows_per_block(wide, v.rt) - 1) / rows_per_block(wide, v.rt);
+    const int64_t per = (nss + v.S - 1) / v.S;
+    const unsigned grid = (unsigned)(R * v.S);
+    if (!wide) {
+        if (v.rt == 8)
+            k_geno_thin<8><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip);
+        else
+            k_geno_thin<4><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip);
+    } else if (JT == 1) {
+        k_geno_wide<1><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip);
+    } else if (JT == 2) {
+        k_geno_wide<2><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip);
+    } else {
+        k_geno_wide<4><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace
+
+// Tile autotuner: the first contraction of each shape (pass, rows, K, digit columns)
+// times every candidate variant once on the live data with HIP events and keeps the
+// fastest; every variant computes the same exact integer sums, so the choice never
+// changes a result. Candidates: thin rows-per-wave 4 / 8, split-K S/2, S, 2 S around
+// choose_split().
+static std::vector<Variant> candidates(bool wide, int64_t nrows, int64_t nss) {
+    std::vector<Variant> v;
+    const int rts[2] = {8, 4};
+    for (int ri = 0; ri < (wide ? 1 : 2); ++ri) {
+        const int rt = wide ? 2 : rts[ri];
+        const int64_t R = (nrows + rows_per_block(wide, rt) - 1) / rows_per_block(wide, rt);
+        const int S0 = choose_split(R, nss, wide);
+        for (int S : {S0 / 2, S0, S0 * 2}) {
+            if (S < 1 || S > 64 || (S > 1 && nss / S < 4)) continue;
+            v.push_back({rt, S});
+        }
+    }
+    return v;
+}
 
 // GZ (n x k) = sig2g G S0 for the packed-genotype cohort with `limbs` int8 digits.
 void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, double* GZ, int64_t ldg,
@@ -486,16 +503,15 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     const int JT = wide ? (J <= 32 ? 1 : J <= 64 ? 2 : 4) : 1;
     const int J16 = wide ? 32 * JT : 16;
     const int nfrag = wide ? JT : 1;
-    const int rows_blk = wide ? kWideRows : kThinRows;
     const int64_t Kpad1 = c->pitch_s * 4, Kpad2 = c->pitch_i * 4;  // genotypes per padded row
     const int64_t nss1 = Kpad1 / 256, nss2 = Kpad2 / 256;
-    const int64_t R1 = (m + rows_blk - 1) / rows_blk, R2 = (n + rows_blk - 1) / rows_blk;
-    const int sp1 = choose_split(R1, nss1, wide), sp2 = choose_split(R2, nss2, wide);
+    // the largest split any candidate may use bounds the partial-sum buffer
     const size_t bq1 = (size_t)Kpad1 * J16, bq2 = (size_t)Kpad2 * J16;
-    const size_t o1 = (size_t)sp1 * m * J16 * 4, o2 = (size_t)sp2 * n * J16 * 4;
-    // the wide kernel prefetches one super-step past the end of its split: pad Bq
-    const size_t bq = (bq1 > bq2 ? bq1 : bq2) + (size_t)512 * 16 * JT;
-    const size_t ob = o1 > o2 ? o1 : o2;
+    int smax1 = 1, smax2 = 1;
+    for (const Variant& v : candidates(wide, m, nss1)) smax1 = std::max(smax1, v.S);
+    for (const Variant& v : candidates(wide, n, nss2)) smax2 = std::max(smax2, v.S);
+    const size_t ob = std::max((size_t)smax1 * m, (size_t)smax2 * n) * J16 * 4;
+    const size_t bq = std::max(bq1, bq2);
     c->w_mix.ensure(round_up(bq, 256) + round_up(ob, 256) + 2 * kMaxCols * sizeof(int));
     uint8_t* Bq = c->w_mix.as<uint8_t>();
     int32_t* out = reinterpret_cast<int32_t*>(Bq + round_up(bq, 256));
@@ -509,37 +525,54 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     double* U = c->w_U.as<double>();
     const double* th = theta_cols;
     const double flops = 2.0 * (double)n * (double)m * k;
-    auto timed = [&](int pass, int64_t rows, int64_t K, int64_t Kpad, auto&& fn) {
-        if (!c->prof_on) {
-            fn();
-            return;
+
+    // run one pass with the tuned variant (tuning it first if this shape is new)
+    auto contract = [&](int pass, const uint8_t* rows, int64_t pitch, int64_t nrows, int64_t K,
+                        int64_t Kpad, int64_t nss) -> int {
+        const std::string key = std::to_string(pass) + ":" + std::to_string(nrows) + ":" +
+                                std::to_string(Kpad) + ":" + std::to_string(J16);
+        auto it = c->tuned.find(key);
+        if (it == c->tuned.end()) {
+            const std::vector<Variant> cand = candidates(wide, nrows, nss);
+            hipEvent_t e0, e1;
+            HIP_CHECK(hipEventCreate(&e0));
+            HIP_CHECK(hipEventCreate(&e1));
+            float best = 1e30f;
+            int bi = 0;
+            for (size_t q = 0; q < cand.size(); ++q) {
+                HIP_CHECK(hipEventRecord(e0, s));
+                launch_variant(s, wide, JT, cand[q], rows, pitch, nrows, nss, Bq, out, nullptr);
+                HIP_CHECK(hipEventRecord(e1, s));
+                HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    bi = (int)q;
+                }
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            it = c->tuned.emplace(key, std::make_pair(cand[bi].rt, cand[bi].S)).first;
         }
-        globe_ctx::ProfEvent e{};
-        HIP_CHECK(hipEventCreate(&e.a));
-        HIP_CHECK(hipEventCreate(&e.b));
-        e.pass = pass;
-        e.iter = c->prof_iter;
-        e.bytes = (double)rows * (double)((K + 3) / 4);
-        e.flops = flops;
-        e.issued = 2.0 * (double)rows * (double)Kpad * J16;
-        HIP_CHECK(hipEventRecord(e.a, s));
-        fn();
-        HIP_CHECK(hipEventRecord(e.b, s));
-        c->prof.push_back(e);
-    };
-    auto contract = [&](const uint8_t* rows, int64_t pitch, int64_t nrows, int64_t nss, int64_t R, int S) {
-        const int64_t per = (nss + S - 1) / S;
-        const unsigned grid = (unsigned)(R * S);
-        if (!wide) {
-            k_geno_thin<<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, S, Bq, out, skip);
-        } else if (JT == 1) {
-            k_geno_wide<1><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, S, Bq, out, skip);
-        } else if (JT == 2) {
-            k_geno_wide<2><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, S, Bq, out, skip);
+        const Variant v{it->second.first, it->second.second};
+        if (c->prof_on) {
+            globe_ctx::ProfEvent e{};
+            HIP_CHECK(hipEventCreate(&e.a));
+            HIP_CHECK(hipEventCreate(&e.b));
+            e.pass = pass;
+            e.iter = c->prof_iter;
+            e.bytes = (double)nrows * (double)((K + 3) / 4);
+            e.flops = flops;
+            e.issued = 2.0 * (double)nrows * (double)Kpad * J16;
+            HIP_CHECK(hipEventRecord(e.a, s));
+            launch_variant(s, wide, JT, v, rows, pitch, nrows, nss, Bq, out, skip);
+            HIP_CHECK(hipEventRecord(e.b, s));
+            c->prof.push_back(e);
         } else {
-            k_geno_wide<4><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, S, Bq, out, skip);
+            launch_variant(s, wide, JT, v, rows, pitch, nrows, nss, Bq, out, skip);
         }
-        HIP_CHECK(hipGetLastError());
+        return v.S;
     };
 
     // pass 1: P1[s][c] = sum_i g_is S0[c][i] on the SNP-major rows
@@ -551,21 +584,20 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
                                               Bq, part, skip);
     k_finish_cols<<<1, 256, 0, s>>>(part, qg1, k, 0, limbs, nullptr, c->w_T.as<double>(), skip);
     HIP_CHECK(hipGetLastError());
-    timed(0, m, n, Kpad1, [&] { contract(c->d_snp.as<uint8_t>(), c->pitch_s, m, nss1, R1, sp1); });
+    const int sp1 = contract(0, c->d_snp.as<uint8_t>(), c->pitch_s, m, n, Kpad1, nss1);
     const int64_t eb1 = std::min<int64_t>((m + kEpiRows - 1) / kEpiRows, kMaxParts);
     k_epilogue<<<(unsigned)eb1, 256, 0, s>>>(out, sp1, m, J16, limbs, k, sh1, 0, c->w_T.as<double>(),
                                              c->d_twof.as<double>(), c->d_inv.as<double>(), th,
                                              1.0 / (double)m, U, m, part, skip);
     HIP_CHECK(hipGetLastError());
     // pass 2: GZ[c][i] = sum_s g_is u[c][s] - C[c] on the individual-major rows
-    k_finish_cols<<<1, 256, 0, s>>>(part, (int)(eb1 < kMaxParts ? eb1 : kMaxParts), k, 1, limbs, sh2,
-                                    nullptr, skip);
+    k_finish_cols<<<1, 256, 0, s>>>(part, (int)eb1, k, 1, limbs, sh2, nullptr, skip);
     const int qg2 = parts_for(Kpad2);
     k_quantize<<<dim3(qg2, J16), 256, 0, s>>>(U, m, m, Kpad2, k, limbs, nfrag, wide, sh2,
                                               c->d_twof.as<double>(), Bq, part, skip);
     k_finish_cols<<<1, 256, 0, s>>>(part, qg2, k, 0, limbs, nullptr, c->w_C.as<double>(), skip);
     HIP_CHECK(hipGetLastError());
-    timed(1, n, m, Kpad2, [&] { contract(c->d_ind.as<uint8_t>(), c->pitch_i, n, nss2, R2, sp2); });
+    const int sp2 = contract(1, c->d_ind.as<uint8_t>(), c->pitch_i, n, m, Kpad2, nss2);
     const int64_t eb2 = std::min<int64_t>((n + kEpiRows - 1) / kEpiRows, kMaxParts);
     k_epilogue<<<(unsigned)eb2, 256, 0, s>>>(out, sp2, n, J16, limbs, k, sh2, 1, c->w_C.as<double>(),
                                              nullptr, nullptr, nullptr, 0.0, GZ, ldg, nullptr, skip);
