@@ -38,6 +38,8 @@
 // K-chunk of Bq in that XCD's L2. Partial sums per split go to out[split][row][J16]
 // and are added exactly (int64) in the epilogue.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 
 #include "context.h"
 
@@ -53,13 +55,12 @@ constexpr uint32_t kDosTable = 0x00010002u;
 constexpr uint32_t kZeroCodes = 0xFFFFFFFFu;  // 16 codes 11: dosage 0
 
 __device__ inline v4i decode16(uint32_t w) {
-    const uint32_t e = w & 0x33333333u;
-    const uint32_t o = (w >> 2) & 0x33333333u;
+    // byte j of (w >> 2d) & 0x03030303 is the code of genotype 4 j + d
     v4i d;
-    d[0] = (int)__builtin_amdgcn_perm(0u, kDosTable, e & 0x0F0F0F0Fu);
-    d[1] = (int)__builtin_amdgcn_perm(0u, kDosTable, o & 0x0F0F0F0Fu);
-    d[2] = (int)__builtin_amdgcn_perm(0u, kDosTable, (e >> 4) & 0x0F0F0F0Fu);
-    d[3] = (int)__builtin_amdgcn_perm(0u, kDosTable, (o >> 4) & 0x0F0F0F0Fu);
+    d[0] = (int)__builtin_amdgcn_perm(0u, kDosTable, w & 0x03030303u);
+    d[1] = (int)__builtin_amdgcn_perm(0u, kDosTable, (w >> 2) & 0x03030303u);
+    d[2] = (int)__builtin_amdgcn_perm(0u, kDosTable, (w >> 4) & 0x03030303u);
+    d[3] = (int)__builtin_amdgcn_perm(0u, kDosTable, (w >> 6) & 0x03030303u);
     return d;
 }
 
@@ -194,17 +195,17 @@ __global__ __launch_bounds__(256) void k_geno_wide(const uint8_t* __restrict__ r
     int cur = 0;
     for (int64_t ss = ss0; ss < ss1; ++ss) {
         const uint4 c00 = an[0][0], c01 = an[0][1], c10 = an[1][0], c11 = an[1][1];
-        const bool more = ss + 1 < ss1;
-        if (more) {
-            // the whole next super-step (packed rows and digits) is requested up front
+        // the whole next super-step (packed rows and digits) is requested up front; on
+        // the last super-step the current one is re-read instead (in bounds, unused), so
+        // the prefetch registers are written unconditionally and stay in VGPRs
+        const int64_t nx = ss + 1 < ss1 ? ss + 1 : ss;
 #pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                an[a][0] = *reinterpret_cast<const uint4*>(base + a * fstride + loff + (ss + 1) * 64);
-                an[a][1] = *reinterpret_cast<const uint4*>(base + a * fstride + loff + (ss + 1) * 64 + 16);
-            }
-#pragma unroll
-            for (int q = 0; q < kCopies; ++q) bn[q] = bsrc[(ss + 1) * (512 * JT) + 256 * q];
+        for (int a = 0; a < 2; ++a) {
+            an[a][0] = *reinterpret_cast<const uint4*>(base + a * fstride + loff + nx * 64);
+            an[a][1] = *reinterpret_cast<const uint4*>(base + a * fstride + loff + nx * 64 + 16);
         }
+#pragma unroll
+        for (int q = 0; q < kCopies; ++q) bn[q] = bsrc[nx * (512 * JT) + 256 * q];
         const uint8_t* cl = lds[cur];
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
@@ -217,7 +218,7 @@ __global__ __launch_bounds__(256) void k_geno_wide(const uint8_t* __restrict__ r
                 acc[1][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, bv, acc[1][b], 0, 0, 0);
             }
         }
-        if (more) {
+        {
             uint4* d4 = reinterpret_cast<uint4*>(lds[cur ^ 1]) + threadIdx.x;
 #pragma unroll
             for (int q = 0; q < kCopies; ++q) d4[256 * q] = bn[q];
@@ -546,6 +547,9 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
                 HIP_CHECK(hipEventSynchronize(e1));
                 float ms = 0.f;
                 HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                if (std::getenv("GLOBE_TUNE_LOG"))
+                    std::fprintf(stderr, "[globe tune] %s rows/wave-frags %d split %d: %.3f ms\n",
+                                 key.c_str(), cand[q].rt, cand[q].S, ms);
                 if (ms < best) {
                     best = ms;
                     bi = (int)q;
