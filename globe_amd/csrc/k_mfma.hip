@@ -206,17 +206,28 @@ __global__ __launch_bounds__(256) void k_geno_wide(const uint8_t* __restrict__ r
         }
 #pragma unroll
         for (int q = 0; q < kCopies; ++q) bn[q] = bsrc[nx * (512 * JT) + 256 * q];
-        const uint8_t* cl = lds[cur];
+        const uint8_t* cl = lds[cur] + lane * 16;
+        // digit fragments of K-step s + 1 are read from LDS while step s multiplies
+        v4i bcur[JT];
+#pragma unroll
+        for (int b = 0; b < JT; ++b) bcur[b] = *reinterpret_cast<const v4i*>(cl + b * 1024);
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
+            v4i bnx[JT];
+            if (s < 7)
+#pragma unroll
+                for (int b = 0; b < JT; ++b)
+                    bnx[b] = *reinterpret_cast<const v4i*>(cl + ((s + 1) * JT + b) * 1024);
             const v4i a0 = decode16(dword_of(s < 4 ? c00 : c01, s & 3));
             const v4i a1 = decode16(dword_of(s < 4 ? c10 : c11, s & 3));
 #pragma unroll
             for (int b = 0; b < JT; ++b) {
-                const v4i bv = *reinterpret_cast<const v4i*>(cl + ((s * JT + b) * 64 + lane) * 16);
-                acc[0][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, bv, acc[0][b], 0, 0, 0);
-                acc[1][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, bv, acc[1][b], 0, 0, 0);
+                acc[0][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, bcur[b], acc[0][b], 0, 0, 0);
+                acc[1][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, bcur[b], acc[1][b], 0, 0, 0);
             }
+            if (s < 7)
+#pragma unroll
+                for (int b = 0; b < JT; ++b) bcur[b] = bnx[b];
         }
         {
             uint4* d4 = reinterpret_cast<uint4*>(lds[cur ^ 1]) + threadIdx.x;
