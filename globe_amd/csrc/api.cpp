@@ -474,7 +474,7 @@ void stage_rows(globe_ctx* c, int64_t m, int64_t pitch_in, float min_maf, float 
     c->pitch_s = round_up((n + 3) / 4, kPitchAlign);
     c->d_snp.release();
     c->d_ind.release();
-    // rows are padded to a multiple of kRowPad with code-11 bytes so the contraction
+    // rows are padded to a multiple of kRowPad with zero (dosage-0) bytes so the contraction
     // kernels load whole row blocks without bounds checks
     c->d_snp.ensure((size_t)round_up(std::max<int64_t>(m, 1), kRowPad) * c->pitch_s);
     const int64_t chunk_rows = std::max<int64_t>(1, std::min<int64_t>(m, (int64_t)(1ll << 30) / pitch_in));
@@ -522,12 +522,12 @@ void stage_rows(globe_ctx* c, int64_t m, int64_t pitch_in, float min_maf, float 
     c->m = kept;
     {
         const int64_t pad_rows = round_up(std::max<int64_t>(kept, 1), kRowPad) - kept;
-        HIP_CHECK(hipMemsetAsync(c->d_snp.as<uint8_t>() + kept * c->pitch_s, 0xFF,
+        HIP_CHECK(hipMemsetAsync(c->d_snp.as<uint8_t>() + kept * c->pitch_s, 0,
                                  (size_t)pad_rows * c->pitch_s, s));
     }
     c->pitch_i = round_up((kept + 3) / 4, kPitchAlign);
     c->d_ind.ensure((size_t)round_up(n, kRowPad) * c->pitch_i);
-    HIP_CHECK(hipMemsetAsync(c->d_ind.p, 0xFF, (size_t)round_up(n, kRowPad) * c->pitch_i, s));
+    HIP_CHECK(hipMemsetAsync(c->d_ind.p, 0, (size_t)round_up(n, kRowPad) * c->pitch_i, s));
     launch_transpose_geno(s, c->d_snp.as<uint8_t>(), kept, c->pitch_s, n, c->d_ind.as<uint8_t>(), c->pitch_i);
     c->d_freq.ensure(sizeof(float) * std::max<int64_t>(kept, 1));
     c->d_twof.ensure(sizeof(double) * std::max<int64_t>(kept, 1));
@@ -595,9 +595,21 @@ int globe_get_geno_rows(globe_ctx* c, int64_t first, int64_t count, uint8_t* out
     require(c->gen == 1, GLOBE_E_STATE, "no genotypes loaded");
     require(first >= 0 && count >= 0 && first + count <= c->m, GLOBE_E_INVALID, "row range");
     if (pitch) *pitch = c->pitch_s;
-    if (out && count > 0)
-        HIP_CHECK(hipMemcpy(out, c->d_snp.as<uint8_t>() + first * c->pitch_s, (size_t)count * c->pitch_s,
-                            hipMemcpyDeviceToHost));
+    if (out && count > 0) {
+        const size_t nb = (size_t)count * c->pitch_s;
+        HIP_CHECK(hipMemcpy(out, c->d_snp.as<uint8_t>() + first * c->pitch_s, nb, hipMemcpyDeviceToHost));
+        // 2-bit dosage d -> PLINK code: 2 -> 00, 1 -> 10, 0 -> 11
+        uint8_t lut[256];
+        for (int b = 0; b < 256; ++b) {
+            int v = 0;
+            for (int q = 0; q < 4; ++q) {
+                const int d = (b >> (2 * q)) & 3;
+                v |= (d == 2 ? 0 : d == 1 ? 2 : 3) << (2 * q);
+            }
+            lut[b] = (uint8_t)v;
+        }
+        for (size_t q = 0; q < nb; ++q) out[q] = lut[out[q]];
+    }
     API_END(c)
 }
 

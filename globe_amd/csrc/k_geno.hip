@@ -4,17 +4,26 @@
 //   snp-major  : m kept SNP rows x pitch_s bytes, individual i of SNP s in bits
 //                2(i%4) of byte i/4 (PLINK 1 SNP-major, PCGsolver.cpp:825-827);
 //   indiv-major: n individual rows x pitch_i bytes, SNP s in bits 2(s%4) of byte s/4.
-// Pitches are multiples of 128 bytes (512 genotypes). Phantom codes of the last
-// byte and all padding hold code 11 (dosage 0), so padding never contributes.
-// Dosage of a code (PCGsolver.cpp:83-94, missing -> 0 at :837): 00 -> 2, 10 -> 1,
-// 11 -> 0, 01 -> 0; branch-free d = (1 - (c & 1)) * (2 - (c >> 1)).
+// Pitches are multiples of 128 bytes (512 genotypes).
+// The resident layouts hold each genotype as its 2-bit dosage (0, 1, 2), not the
+// PLINK code: 00 -> 2, 10 -> 1, 11 -> 0, and a missing call 01 -> 0 (PCGsolver.cpp:
+// 83-94, missing -> 0 at :837). After QC a missing call and code 11 enter every
+// product identically, so the conversion loses nothing, and a dosage field is
+// turned into an int8 byte with one mask. Phantom genotypes of the last byte and
+// all padding hold dosage 0 (zero bytes), so padding never contributes.
 #include "globe_internal.h"
 
 namespace globe {
 
 namespace {
 
-__device__ inline int dosage(uint32_t code) { return (1 - (int)(code & 1u)) * (2 - (int)(code >> 1)); }
+// one packed byte of PLINK codes -> the same four genotypes as 2-bit dosages
+__device__ inline uint32_t codes_to_dosages(uint32_t b) {
+    const uint32_t L = b & 0x55u, H = (b >> 1) & 0x55u;
+    const uint32_t one = H & ~L;            // code 10
+    const uint32_t two = ~H & ~L & 0x55u;   // code 00
+    return one | (two << 1);
+}
 
 // ------------------------------------------------------------------ QC counts
 // Per SNP: A1 allele count 2*#00 + #10 and missing count #01 over the first n
@@ -60,8 +69,8 @@ __global__ __launch_bounds__(kBlock) void k_bed_counts(const uint8_t* __restrict
     }
 }
 
-// Copy kept rows, force phantom codes of the last byte to 11 and fill the pitch
-// padding with 0xFF.
+// Copy kept rows as 2-bit dosages, with phantom genotypes of the last byte and the
+// pitch padding set to dosage 0.
 __global__ __launch_bounds__(kBlock) void k_compact_rows(const uint8_t* __restrict__ src,
                                                          int64_t pitch_src,
                                                          const int64_t* __restrict__ idx,
@@ -70,17 +79,17 @@ __global__ __launch_bounds__(kBlock) void k_compact_rows(const uint8_t* __restri
                                                          int64_t pitch_dst) {
     const int64_t nbytes = (n + 3) / 4;
     const int rem = n & 3;
-    const uint8_t phantom = rem == 0 ? 0 : (uint8_t)(0xFFu << (2 * rem));
+    const uint32_t live = rem == 0 ? 0xFFu : (1u << (2 * rem)) - 1u;  // fields of the last byte
     for (int64_t r = blockIdx.x; r < m_kept; r += gridDim.x) {
         const uint8_t* s = src + idx[r] * pitch_src;
         uint8_t* d = dst + r * pitch_dst;
         for (int64_t j = threadIdx.x; j < pitch_dst; j += kBlock) {
-            uint8_t v = 0xFF;
+            uint32_t v = 0;
             if (j < nbytes) {
-                v = s[j];
-                if (j == nbytes - 1) v |= phantom;
+                v = codes_to_dosages(s[j]);
+                if (j == nbytes - 1) v &= live;
             }
-            d[j] = v;
+            d[j] = (uint8_t)v;
         }
     }
 }
@@ -97,7 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_transpose(const uint8_t* __restrict_
     const int t = threadIdx.x;
     {
         const int64_t s = s0 + t;
-        uint4 v = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
         if (s < m) v = *reinterpret_cast<const uint4*>(snp + s * pitch_s + i0 / 4);
         const uint8_t* vb = reinterpret_cast<const uint8_t*>(&v);
 #pragma unroll
@@ -115,8 +124,8 @@ __global__ __launch_bounds__(kBlock) void k_transpose(const uint8_t* __restrict_
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int sl = part * 64 + ob * 4 + q;
-            const uint32_t code = (tile[sl][il >> 2] >> (2 * (il & 3))) & 3u;
-            byte |= code << (2 * q);
+            const uint32_t dose = (tile[sl][il >> 2] >> (2 * (il & 3))) & 3u;
+            byte |= dose << (2 * q);
         }
         out[ob] = (uint8_t)byte;
     }
@@ -223,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_gdiag(const uint8_t* __restrict__ in
 #pragma unroll
                     for (int g = 0; g < 16; ++g) {
                         const int kk = j * 64 + d * 16 + g;
-                        acc += cv[dosage((wd[d] >> (2 * g)) & 3u)][kk];
+                        acc += cv[(wd[d] >> (2 * g)) & 3u][kk];
                     }
             }
         }
@@ -264,7 +273,7 @@ __global__ __launch_bounds__(kBlock) void k_rowdot(const uint8_t* __restrict__ r
 #pragma unroll
                     for (int g = 0; g < 16; ++g) {
                         const int kk = j * 64 + d * 16 + g;
-                        const double dv = (double)dosage((wd[d] >> (2 * g)) & 3u);
+                        const double dv = (double)((wd[d] >> (2 * g)) & 3u);
 #pragma unroll
                         for (int c = 0; c < KC; ++c) acc[c] = fma(dv, sv[c][kk], acc[c]);
                     }

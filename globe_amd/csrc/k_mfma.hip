@@ -15,20 +15,27 @@
 // rank-one correction sum_i w_is z_i = inv_s (P_s - 2 f_s T), T = sum_i z_i of the
 // quantized z, so the operator is applied exactly to the quantized vector.
 //
-// Two kernels share the K order (a super-step is 256 genotypes = 64 packed bytes of
-// a row; a lane decodes one packed dword = 16 genotypes per MFMA K-step):
-//   thin (digit columns <= 16, the 1-RHS solve): v_mfma_i32_16x16x64_i8, 8 row
-//     fragments of 16 rows per wave (512 rows per block), the digit fragments read
-//     straight from global memory (L1 / L2), no LDS and no barrier;
-//     A lane l: row l % 16, packed bytes 64 ss + 16 (l / 16) + 4 s for K-step s < 4.
-//   wide (digit columns 17..128): v_mfma_i32_32x32x32_i8, 2 row fragments of 32 rows
-//     and up to 4 column fragments of 32 per wave (256 rows per block), the digit
-//     tile of the next super-step staged in a second LDS buffer while the current
-//     one is consumed (one barrier per super-step);
-//     A lane l: row l % 32, packed bytes 64 ss + 32 (l / 32) + 4 s for K-step s < 8.
-// Decode: two masks and four v_perm_b32 byte-table lookups turn a dword of 16 codes
-// into 16 int8 dosages in the order {0,4,8,12, 1,5,9,13, 2,6,10,14, 3,7,11,15}; the
-// digit matrix Bq holds each lane's 16 digits in exactly that order.
+// The packed rows hold 2-bit dosages (k_geno.hip), so a dword of 16 genotypes
+// becomes 16 int8 dosages with one shift and one mask per 4 genotypes, in the order
+// {0,4,8,12, 1,5,9,13, 2,6,10,14, 3,7,11,15}; the digit matrix Bq holds each lane's
+// 16 digits in exactly that order.
+//
+// Two kernels:
+//   thin (digit columns <= 16, the 1-RHS solve): v_mfma_i32_16x16x64_i8, RT row
+//     fragments of 16 rows per wave, the digit fragments read straight from global
+//     memory (L1 / L2), no LDS and no barrier. A super-step is 256 genotypes = 64
+//     packed bytes of a row; A lane l: row l % 16, packed bytes 64 ss + 16 (l / 16)
+//     + 4 s for K-step s < 4.
+//   wide (digit columns 17..128): v_mfma_i32_32x32x32_i8, W waves of 2 row fragments
+//     of 32 rows and JT column fragments of 32 (64 W rows per block; W = 8 with one
+//     block per CU or W = 4 with two). A stage is 128 genotypes = 32 packed bytes of a
+//     row: A lane l: row l % 32, packed bytes 32 H + 16 (l / 32) + 4 s for K-step
+//     s < 4 of stage H. Both operands of a stage (2 W KB of packed rows, 4 JT KB of
+//     digits) are copied global -> LDS by the buffer-to-LDS path
+//     (global_load_lds_dwordx4: 64 lanes x 16 B land as one contiguous KB) into a
+//     ring of NS slots, NS - 1 stages ahead of the one being multiplied; one wait +
+//     barrier per stage. No loaded value lives in VGPRs across a stage, so the
+//     compiler adds no waits of its own on the copies.
 // Output fragments (probed on gfx950): 16x16: element v of lane l is (row 4 (l/16) + v,
 // column l % 16); 32x32: (row 8 (v/4) + 4 (l/32) + v % 4, column l % 32).
 //
@@ -49,18 +56,17 @@ namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+typedef __attribute__((address_space(3))) void lds_void;
 
-// dosage byte table for codes 0..3: 00 -> 2, 01 (missing) -> 0, 10 -> 1, 11 -> 0
-constexpr uint32_t kDosTable = 0x00010002u;
-constexpr uint32_t kZeroCodes = 0xFFFFFFFFu;  // 16 codes 11: dosage 0
+constexpr uint32_t kFieldMask = 0x03030303u;
 
 __device__ inline v4i decode16(uint32_t w) {
-    // byte j of (w >> 2d) & 0x03030303 is the code of genotype 4 j + d
+    // byte j of (w >> 2d) & 0x03030303 is the dosage of genotype 4 j + d
     v4i d;
-    d[0] = (int)__builtin_amdgcn_perm(0u, kDosTable, w & 0x03030303u);
-    d[1] = (int)__builtin_amdgcn_perm(0u, kDosTable, (w >> 2) & 0x03030303u);
-    d[2] = (int)__builtin_amdgcn_perm(0u, kDosTable, (w >> 4) & 0x03030303u);
-    d[3] = (int)__builtin_amdgcn_perm(0u, kDosTable, (w >> 6) & 0x03030303u);
+    d[0] = (int)(w & kFieldMask);
+    d[1] = (int)((w >> 2) & kFieldMask);
+    d[2] = (int)((w >> 4) & kFieldMask);
+    d[3] = (int)((w >> 6) & kFieldMask);
     return d;
 }
 
@@ -148,105 +154,176 @@ __global__ __launch_bounds__(256) void k_geno_thin(const uint8_t* __restrict__ r
 }
 
 // ---------------------------------------------------------------- wide kernel
-constexpr int kWideRows = 4 * 64;  // 4 waves x 2 fragments of 32 rows
+// Block shape <W waves, NS ring stages, CG column groups, RA row fragments per wave>:
+// the waves form W / CG row groups x CG column groups; a wave owns RA row fragments of
+// 32 rows and CB = JT / CG column fragments of 32, so a block covers
+// (W / CG) 32 RA rows x all 32 JT digit columns. Per stage a wave reads RA KB of
+// packed rows (4 K-steps each) and 4 CB KB of digits from LDS for 4 RA CB MFMAs, so
+// the taller tiles (RA = 4) need a third less LDS traffic per MFMA than RA = 2.
+template <int JT, int W, int CG, int RA>
+struct WideCfg {
+    static constexpr int kW = W, kJT = JT, kRA = RA;
+    static constexpr int kCB = JT / CG;                 // column fragments per wave
+    static constexpr int kRowGroups = W / CG;
+    static constexpr int kRows = kRowGroups * 32 * RA;  // rows per block
+    static constexpr int kAFrags = kRowGroups * RA;     // 1 KB row fragments per stage
+    static constexpr int kAPerWave = kAFrags / W;
+    static constexpr int kA = kAFrags * 1024;
+    static constexpr int kB = 4 * JT * 1024;            // digits of a stage
+    static constexpr int kSlot = kA + kB;
+    static constexpr int kBChunks = 4 * JT;             // 1 KB digit chunks
+    static constexpr int kBPerWave = kBChunks >= W ? kBChunks / W : 1;
+    static constexpr int kSink = kBChunks < W ? (W - kBChunks) * 1024 : 0;
+    static constexpr int kOps = kAPerWave + kBPerWave;  // copies per wave per stage
+    static_assert(JT % CG == 0 && W % CG == 0, "column groups");
+    static_assert(kAFrags % W == 0, "row fragments split evenly over waves");
+    static_assert(kBChunks < W || kBChunks % W == 0, "digit chunks split evenly over waves");
+};
 
-template <int JT>
-__global__ __launch_bounds__(256) void k_geno_wide(const uint8_t* __restrict__ rows, int64_t pitch,
-                                                   int64_t nrows, int64_t n_ss, int64_t ss_per_split,
-                                                   int64_t R, int S, const uint8_t* __restrict__ Bq,
-                                                   int32_t* __restrict__ out, const int* skip) {
+// Issue the global -> LDS copies of one stage into ring slot `slot`: kAPerWave row
+// fragments (lane l copies the 16 packed bytes of row l % 32, half l / 32; they land
+// lane-ordered) and kBPerWave digit chunks. With fewer chunks than waves the spare
+// waves copy into a private sink, so every wave issues the same number of copies and
+// one vmcnt fits all. arow: this lane's row pointer of fragment 0 at the stage.
+template <class C>
+__device__ inline void wide_issue(const uint8_t* __restrict__ arow, int64_t frag_stride,
+                                  const uint8_t* __restrict__ bstage, uint8_t* __restrict__ slot,
+                                  uint8_t* __restrict__ sink, int wave, int lane) {
+#pragma unroll
+    for (int q = 0; q < C::kAPerWave; ++q) {
+        const int f = wave + C::kW * q;
+        __builtin_amdgcn_global_load_lds((const void*)(arow + f * frag_stride),
+                                         (lds_void*)(slot + f * 1024), 16, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < C::kBPerWave; ++q) {
+        const int chunk = wave + C::kW * q;
+        if (chunk < C::kBChunks)
+            __builtin_amdgcn_global_load_lds((const void*)(bstage + chunk * 1024 + lane * 16),
+                                             (lds_void*)(slot + C::kA + chunk * 1024), 16, 0, 0);
+        else
+            __builtin_amdgcn_global_load_lds((const void*)(bstage + (chunk % C::kBChunks) * 1024 + lane * 16),
+                                             (lds_void*)(sink + (chunk - C::kBChunks) * 1024), 16, 0, 0);
+    }
+}
+
+// 4 K-steps of one stage from LDS: the wave's RA row fragments, then its digit
+// fragments one K-step ahead of the MFMAs that use them.
+template <class C>
+__device__ inline void wide_mma(const uint8_t* __restrict__ slot, int rgroup, int cgroup, int lane,
+                                v16i (&acc)[4][4]) {
+    constexpr int RA = C::kRA, CB = C::kCB, JT = C::kJT;
+    uint4 av[RA];
+#pragma unroll
+    for (int a = 0; a < RA; ++a)
+        av[a] = *reinterpret_cast<const uint4*>(slot + (rgroup * RA + a) * 1024 + lane * 16);
+    const uint8_t* cl = slot + C::kA + (cgroup * CB) * 1024 + lane * 16;
+    v4i bcur[CB];
+#pragma unroll
+    for (int b = 0; b < CB; ++b) bcur[b] = *reinterpret_cast<const v4i*>(cl + b * 1024);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        v4i bnx[CB];
+        if (s < 3)
+#pragma unroll
+            for (int b = 0; b < CB; ++b)
+                bnx[b] = *reinterpret_cast<const v4i*>(cl + ((s + 1) * JT + b) * 1024);
+#pragma unroll
+        for (int a = 0; a < RA; ++a) {
+            const v4i ad = decode16(dword_of(av[a], s));
+#pragma unroll
+            for (int b = 0; b < CB; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ad, bcur[b], acc[a][b], 0, 0, 0);
+        }
+        if (s < 3)
+#pragma unroll
+            for (int b = 0; b < CB; ++b) bcur[b] = bnx[b];
+    }
+}
+
+// Wait until this wave's copies of the stage about to be multiplied have landed (at
+// most the copies of the NS - 2 later stages still in flight) and its LDS reads of
+// the previous stage are complete, then synchronize the block, so that every wave's
+// copies are visible and the slot refilled next is no longer read.
+template <class C, int NS>
+__device__ inline void wide_stage_barrier() {
+    constexpr int kAhead = (NS - 2) * C::kOps;
+    static_assert(kAhead < 64, "vmcnt field");
+    asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(kAhead) : "memory");
+}
+
+// All operands pass through LDS (no loaded value lives in VGPRs across a stage), so
+// the only waits on the copies are the explicit per-stage ones above.
+template <int JT, int W, int NS, int CG, int RA>
+__global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(2))) void k_geno_wide(
+    const uint8_t* __restrict__ rows, int64_t pitch, int64_t nrows, int64_t n_ss,
+    int64_t ss_per_split, int64_t R, int S, const uint8_t* __restrict__ Bq,
+    int32_t* __restrict__ out, const int* skip) {
     if (skip && *skip) return;
-    // two buffers of one super-step of digits: 8 K-steps x JT fragments x 1 KB
-    __shared__ __attribute__((aligned(16))) uint8_t lds[2][8 * JT * 1024];
+    using C = WideCfg<JT, W, CG, RA>;
+    constexpr int CB = C::kCB;
+    __shared__ __attribute__((aligned(1024))) uint8_t lds[NS * C::kSlot + C::kSink];
     int64_t rb;
     int sp;
     block_coords(blockIdx.x, R, S, rb, sp);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rgroup = wave / CG, cgroup = wave % CG;
     const int g = lane >> 5, rr = lane & 31;
-    const int64_t row_base = rb * kWideRows + wave * 64;
-    const uint8_t* base = rows + row_base * pitch;  // uniform per wave
-    const int64_t fstride = 32 * pitch;
-    const uint32_t loff = (uint32_t)(rr * pitch + 32 * g);
+    const int64_t blk_row = rb * C::kRows;
     const int64_t ss0 = (int64_t)sp * ss_per_split;
     const int64_t ss1 = ss0 + ss_per_split < n_ss ? ss0 + ss_per_split : n_ss;
+    const int64_t H0 = 2 * ss0;
+    const int64_t nt = ss1 > ss0 ? 2 * (ss1 - ss0) : 0;  // stages of this split
+    // fragment f of the block = rows blk_row + 32 f .. + 31
+    const uint8_t* arow = rows + (blk_row + rr) * pitch + 16 * g + 32 * H0;
+    const int64_t fstride = 32 * pitch;
+    const uint8_t* bst = Bq + H0 * C::kB;
+    uint8_t* sink = lds + NS * C::kSlot;
 
-    v16i acc[2][JT];
+    v16i acc[4][4];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < JT; ++b)
+        for (int b = 0; b < 4; ++b)
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[a][b][v] = 0;
-    // digits of one super-step: 8 JT KB = 512 JT uint4, thread t copies t + 256 q, q < 2 JT
-    constexpr int kCopies = 2 * JT;
-    const uint4* bsrc = reinterpret_cast<const uint4*>(Bq) + threadIdx.x;
-    uint4 an[2][2], bn[kCopies];
-    if (ss0 < ss1) {
-        uint4* d4 = reinterpret_cast<uint4*>(lds[0]) + threadIdx.x;
+    if (nt > 0) {
+        // prologue: stages 0 .. NS - 2 (clamped to the last stage; a clamped copy
+        // lands in a slot that is never multiplied)
 #pragma unroll
-        for (int q = 0; q < kCopies; ++q) d4[256 * q] = bsrc[ss0 * (512 * JT) + 256 * q];
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            an[a][0] = *reinterpret_cast<const uint4*>(base + a * fstride + loff + ss0 * 64);
-            an[a][1] = *reinterpret_cast<const uint4*>(base + a * fstride + loff + ss0 * 64 + 16);
+        for (int u = 0; u < NS - 1; ++u) {
+            const int64_t t = u < nt ? u : nt - 1;
+            wide_issue<C>(arow + 32 * t, fstride, bst + t * C::kB, lds + u * C::kSlot, sink, wave, lane);
         }
-    }
-    __syncthreads();
-    int cur = 0;
-    for (int64_t ss = ss0; ss < ss1; ++ss) {
-        const uint4 c00 = an[0][0], c01 = an[0][1], c10 = an[1][0], c11 = an[1][1];
-        // the whole next super-step (packed rows and digits) is requested up front; on
-        // the last super-step the current one is re-read instead (in bounds, unused), so
-        // the prefetch registers are written unconditionally and stay in VGPRs
-        const int64_t nx = ss + 1 < ss1 ? ss + 1 : ss;
+        for (int64_t t0 = 0; t0 < nt; t0 += NS) {
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            an[a][0] = *reinterpret_cast<const uint4*>(base + a * fstride + loff + nx * 64);
-            an[a][1] = *reinterpret_cast<const uint4*>(base + a * fstride + loff + nx * 64 + 16);
-        }
-#pragma unroll
-        for (int q = 0; q < kCopies; ++q) bn[q] = bsrc[nx * (512 * JT) + 256 * q];
-        const uint8_t* cl = lds[cur] + lane * 16;
-        // digit fragments of K-step s + 1 are read from LDS while step s multiplies
-        v4i bcur[JT];
-#pragma unroll
-        for (int b = 0; b < JT; ++b) bcur[b] = *reinterpret_cast<const v4i*>(cl + b * 1024);
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            v4i bnx[JT];
-            if (s < 7)
-#pragma unroll
-                for (int b = 0; b < JT; ++b)
-                    bnx[b] = *reinterpret_cast<const v4i*>(cl + ((s + 1) * JT + b) * 1024);
-            const v4i a0 = decode16(dword_of(s < 4 ? c00 : c01, s & 3));
-            const v4i a1 = decode16(dword_of(s < 4 ? c10 : c11, s & 3));
-#pragma unroll
-            for (int b = 0; b < JT; ++b) {
-                acc[0][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, bcur[b], acc[0][b], 0, 0, 0);
-                acc[1][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, bcur[b], acc[1][b], 0, 0, 0);
+            for (int u = 0; u < NS; ++u) {
+                const int64_t t = t0 + u;
+                if (t >= nt) break;
+                wide_stage_barrier<C, NS>();
+                // refill the slot multiplied in the previous stage with stage t + NS - 1,
+                // issued before this stage's MFMAs (the scheduler may not sink it)
+                const int64_t tn = t + NS - 1 < nt ? t + NS - 1 : nt - 1;
+                wide_issue<C>(arow + 32 * tn, fstride, bst + tn * C::kB,
+                              lds + ((u + NS - 1) % NS) * C::kSlot, sink, wave, lane);
+                __builtin_amdgcn_sched_barrier(0);
+                wide_mma<C>(lds + u * C::kSlot, rgroup, cgroup, lane, acc);
             }
-            if (s < 7)
-#pragma unroll
-                for (int b = 0; b < JT; ++b) bcur[b] = bnx[b];
         }
-        {
-            uint4* d4 = reinterpret_cast<uint4*>(lds[cur ^ 1]) + threadIdx.x;
-#pragma unroll
-            for (int q = 0; q < kCopies; ++q) d4[256 * q] = bn[q];
-        }
-        __syncthreads();
-        cur ^= 1;
+        // no copy into LDS may be in flight when the block ends
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     }
     constexpr int J16 = 32 * JT;
     int32_t* o = out + (int64_t)sp * nrows * J16;
+    const int64_t row_base = blk_row + rgroup * 32 * RA;
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < RA; ++a)
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const int64_t r = row_base + a * 32 + 8 * (v >> 2) + 4 * g + (v & 3);
             if (r < nrows)
 #pragma unroll
-                for (int b = 0; b < JT; ++b) o[r * J16 + b * 32 + rr] = acc[a][b][v];
+                for (int b = 0; b < CB; ++b) o[r * J16 + (cgroup * CB + b) * 32 + rr] = acc[a][b][v];
         }
 }
 
@@ -356,8 +433,9 @@ __global__ __launch_bounds__(256) void k_quantize(const double* __restrict__ v, 
         int64_t ks;
         int ln;
         if (wide) {
-            ks = 8 * ss + ((rem >> 4) & 7);
-            ln = 32 * (rem >> 7) + jj;
+            // stage 2 ss + rem / 128; within it lane half (rem / 64) % 2, K-step (rem / 16) % 4
+            ks = 8 * ss + 4 * (rem >> 7) + ((rem >> 4) & 3);
+            ln = 32 * ((rem >> 6) & 1) + jj;
         } else {
             ks = 4 * ss + ((rem >> 4) & 3);
             ln = 16 * (rem >> 6) + jj;
@@ -455,29 +533,71 @@ int choose_split(int64_t R, int64_t n_ss, bool wide) {
 }
 
 struct Variant {
-    int rt;  // thin: row fragments per wave (4 or 8); wide: 2
-    int S;   // split-K
+    int cfg;  // thin: row fragments of 16 rows per wave (8 or 4); wide: index into kWide
+    int S;    // split-K
 };
 
-int64_t rows_per_block(bool wide, int rt) { return wide ? kWideRows : 64 * rt; }
+// wide block shapes: waves, ring stages, column groups, row fragments per wave
+struct WideShape {
+    int W, NS, CG, RA;
+};
+constexpr WideShape kWide[4] = {{8, 4, 1, 2}, {4, 3, 1, 2}, {8, 4, 2, 4}, {4, 3, 2, 4}};
+
+int64_t rows_per_block(bool wide, int cfg) {
+    if (!wide) return 64 * cfg;
+    const WideShape& w = kWide[cfg];
+    return (int64_t)(w.W / w.CG) * 32 * w.RA;
+}
+
+template <int JT, int Q>
+void launch_wide_q(hipStream_t s, unsigned grid, const uint8_t* rows, int64_t pitch, int64_t nrows,
+                   int64_t nss, int64_t per, int64_t R, int S, const uint8_t* Bq, int32_t* out,
+                   const int* skip) {
+    constexpr WideShape w = kWide[Q];
+    k_geno_wide<JT, w.W, w.NS, w.CG, w.RA><<<grid, w.W * 64, 0, s>>>(rows, pitch, nrows, nss, per, R,
+                                                                    S, Bq, out, skip);
+}
+
+template <int JT>
+void launch_wide(hipStream_t s, const Variant& v, unsigned grid, const uint8_t* rows, int64_t pitch,
+                 int64_t nrows, int64_t nss, int64_t per, int64_t R, const uint8_t* Bq, int32_t* out,
+                 const int* skip) {
+    switch (v.cfg) {
+        case 0: launch_wide_q<JT, 0>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
+        case 1: launch_wide_q<JT, 1>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
+        case 2:
+            if constexpr (JT % 2 == 0) {
+                launch_wide_q<JT, 2>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip);
+                break;
+            }
+            [[fallthrough]];
+        case 3:
+            if constexpr (JT % 2 == 0) {
+                launch_wide_q<JT, 3>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip);
+                break;
+            }
+            [[fallthrough]];
+        default: require(false, GLOBE_E_INVALID, "wide contraction shape not available for this width");
+    }
+}
 
 void launch_variant(hipStream_t s, bool wide, int JT, const Variant& v, const uint8_t* rows,
                     int64_t pitch, int64_t nrows, int64_t nss, const uint8_t* Bq, int32_t* out,
                     const int* skip) {
-    const int64_t R = (nrows + rows_per_block(wide, v.rt) - 1) / rows_per_block(wide, v.rt);
+    const int64_t R = (nrows + rows_per_block(wide, v.cfg) - 1) / rows_per_block(wide, v.cfg);
     const int64_t per = (nss + v.S - 1) / v.S;
     const unsigned grid = (unsigned)(R * v.S);
     if (!wide) {
-        if (v.rt == 8)
+        if (v.cfg == 8)
             k_geno_thin<8><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip);
         else
             k_geno_thin<4><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip);
     } else if (JT == 1) {
-        k_geno_wide<1><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip);
+        launch_wide<1>(s, v, grid, rows, pitch, nrows, nss, per, R, Bq, out, skip);
     } else if (JT == 2) {
-        k_geno_wide<2><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip);
+        launch_wide<2>(s, v, grid, rows, pitch, nrows, nss, per, R, Bq, out, skip);
     } else {
-        k_geno_wide<4><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip);
+        launch_wide<4>(s, v, grid, rows, pitch, nrows, nss, per, R, Bq, out, skip);
     }
     HIP_CHECK(hipGetLastError());
 }
@@ -487,18 +607,24 @@ void launch_variant(hipStream_t s, bool wide, int JT, const Variant& v, const ui
 // Tile autotuner: the first contraction of each shape (pass, rows, K, digit columns)
 // times every candidate variant once on the live data with HIP events and keeps the
 // fastest; every variant computes the same exact integer sums, so the choice never
-// changes a result. Candidates: thin rows-per-wave 4 / 8, split-K S/2, S, 2 S around
+// changes a result. Candidates: thin rows-per-wave 8 / 4, the wide block shapes of
+// kWide (the RA = 4 ones need an even JT), each with split-K S/2, S, 2 S around
 // choose_split().
-static std::vector<Variant> candidates(bool wide, int64_t nrows, int64_t nss) {
+static std::vector<Variant> candidates(bool wide, int JT, int64_t nrows, int64_t nss) {
     std::vector<Variant> v;
-    const int rts[2] = {8, 4};
-    for (int ri = 0; ri < (wide ? 1 : 2); ++ri) {
-        const int rt = wide ? 2 : rts[ri];
-        const int64_t R = (nrows + rows_per_block(wide, rt) - 1) / rows_per_block(wide, rt);
+    std::vector<int> cfgs;
+    if (wide) {
+        for (int q = 0; q < 4; ++q)
+            if (JT % kWide[q].CG == 0) cfgs.push_back(q);
+    } else {
+        cfgs = {8, 4};
+    }
+    for (int cfg : cfgs) {
+        const int64_t R = (nrows + rows_per_block(wide, cfg) - 1) / rows_per_block(wide, cfg);
         const int S0 = choose_split(R, nss, wide);
         for (int S : {S0 / 2, S0, S0 * 2}) {
             if (S < 1 || S > 64 || (S > 1 && nss / S < 4)) continue;
-            v.push_back({rt, S});
+            v.push_back({cfg, S});
         }
     }
     return v;
@@ -520,8 +646,8 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     // the largest split any candidate may use bounds the partial-sum buffer
     const size_t bq1 = (size_t)Kpad1 * J16, bq2 = (size_t)Kpad2 * J16;
     int smax1 = 1, smax2 = 1;
-    for (const Variant& v : candidates(wide, m, nss1)) smax1 = std::max(smax1, v.S);
-    for (const Variant& v : candidates(wide, n, nss2)) smax2 = std::max(smax2, v.S);
+    for (const Variant& v : candidates(wide, JT, m, nss1)) smax1 = std::max(smax1, v.S);
+    for (const Variant& v : candidates(wide, JT, n, nss2)) smax2 = std::max(smax2, v.S);
     const size_t ob = std::max((size_t)smax1 * m, (size_t)smax2 * n) * J16 * 4;
     const size_t bq = std::max(bq1, bq2);
     c->w_mix.ensure(round_up(bq, 256) + round_up(ob, 256) + 2 * kMaxCols * sizeof(int));
@@ -545,7 +671,7 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
                                 std::to_string(Kpad) + ":" + std::to_string(J16);
         auto it = c->tuned.find(key);
         if (it == c->tuned.end()) {
-            const std::vector<Variant> cand = candidates(wide, nrows, nss);
+            const std::vector<Variant> cand = candidates(wide, JT, nrows, nss);
             hipEvent_t e0, e1;
             HIP_CHECK(hipEventCreate(&e0));
             HIP_CHECK(hipEventCreate(&e1));
@@ -559,8 +685,8 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
                 float ms = 0.f;
                 HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
                 if (std::getenv("GLOBE_TUNE_LOG"))
-                    std::fprintf(stderr, "[globe tune] %s rows/wave-frags %d split %d: %.3f ms\n",
-                                 key.c_str(), cand[q].rt, cand[q].S, ms);
+                    std::fprintf(stderr, "[globe tune] %s shape %d split %d: %.3f ms\n",
+                                 key.c_str(), cand[q].cfg, cand[q].S, ms);
                 if (ms < best) {
                     best = ms;
                     bi = (int)q;
@@ -568,7 +694,7 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
             }
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
-            it = c->tuned.emplace(key, std::make_pair(cand[bi].rt, cand[bi].S)).first;
+            it = c->tuned.emplace(key, std::make_pair(cand[bi].cfg, cand[bi].S)).first;
         }
         const Variant v{it->second.first, it->second.second};
         if (c->prof_on) {

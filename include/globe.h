@@ -215,7 +215,9 @@ int globe_synth_bed(globe_ctx* ctx, int64_t m, uint64_t seed, float f_lo, float 
 /* out (n) = sum_s w_is a_s: standardized kept genotypes times an SNP-space vector
  * (the synthetic phenotype recipe's u = sqrt(sig2g / M) W a). */
 int globe_geno_Wa(globe_ctx* ctx, const double* a, double* out);
-/* Raw rows of the resident SNP-major layout (pitch bytes each) for checks. */
+/* Rows of the resident SNP-major layout (pitch bytes each) for checks, written back
+ * in PLINK codes: dosage 2 -> 00, 1 -> 10, 0 -> 11. A missing call (01) held as
+ * dosage 0 therefore reads back as 11, and padding reads as 11. */
 int globe_get_geno_rows(globe_ctx* ctx, int64_t first, int64_t count, uint8_t* out, int64_t* pitch);
 
 #ifdef __cplusplus
