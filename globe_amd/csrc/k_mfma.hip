@@ -21,11 +21,10 @@
 // 16 digits in exactly that order.
 //
 // Two kernels:
-//   thin (digit columns <= 16, the 1-RHS solve): v_mfma_i32_16x16x64_i8, RT row
-//     fragments of 16 rows per wave, the digit fragments read straight from global
-//     memory (L1 / L2), no LDS and no barrier. A super-step is 256 genotypes = 64
-//     packed bytes of a row; A lane l: row l % 16, packed bytes 64 ss + 16 (l / 16)
-//     + 4 s for K-step s < 4.
+//   thin (digit columns <= 16, the 1-RHS solve): v_mfma_i32_16x16x64_i8, W waves of RT
+//     row fragments of 16 rows. A stage is one super-step of 256 genotypes = 64 packed
+//     bytes of a row; A lane l: row l % 16, packed bytes 64 ss + 16 (l / 16) + 4 s for
+//     K-step s < 4.
 //   wide (digit columns 17..128): v_mfma_i32_32x32x32_i8, W waves of 2 row fragments
 //     of 32 rows and JT column fragments of 32 (64 W rows per block; W = 8 with one
 //     block per CU or W = 4 with two). A stage is 128 genotypes = 32 packed bytes of a
@@ -34,8 +33,11 @@
 //     digits) are copied global -> LDS by the buffer-to-LDS path
 //     (global_load_lds_dwordx4: 64 lanes x 16 B land as one contiguous KB) into a
 //     ring of NS slots, NS - 1 stages ahead of the one being multiplied; one wait +
-//     barrier per stage. No loaded value lives in VGPRs across a stage, so the
-//     compiler adds no waits of its own on the copies.
+//     barrier per stage.
+// Both kernels stage every operand through LDS this way: no loaded value lives in
+// VGPRs across a stage. Copies into LDS and plain VGPR loads share vmcnt, so a
+// compiler-generated wait on a VGPR load would have to be vmcnt(0) and drain the ring;
+// with LDS-only operands the explicit per-stage waits are the only ones.
 // Output fragments (probed on gfx950): 16x16: element v of lane l is (row 4 (l/16) + v,
 // column l % 16); 32x32: (row 8 (v/4) + 4 (l/32) + v % 4, column l % 32).
 //
@@ -87,61 +89,113 @@ __device__ inline void block_coords(int64_t b, int64_t R, int S, int64_t& rb, in
 }
 
 // ---------------------------------------------------------------- thin kernel
-// RT row fragments of 16 rows per wave (64 RT rows per block). Row blocks are read
-// without bounds checks: the packed layouts are padded to a multiple of 512 rows.
-// Two register sets alternate so the next super-step's loads overlap the current
-// one's decode and MFMAs.
-template <int RT>
-__device__ inline void thin_load(const uint8_t* base, int64_t frag_stride, uint32_t loff, int64_t ss,
-                                 const uint4* bq, uint4 (&A)[RT], uint4 (&B)[4]) {
+// W waves x RT row fragments of 16 rows (16 W RT rows per block), one column fragment
+// of 16 digits. A stage is one super-step (256 genotypes = 64 packed bytes of a row):
+// the wave's own RT row fragments (lane l: row l % 16, 16 bytes at 16 (l / 16)) and the
+// stage's 4 digit chunks (waves 0-3 one each; with W = 8 the other waves copy into a
+// private sink so that every wave issues the same number of copies) are copied
+// global -> LDS into a ring of NS slots, NS - 1 stages ahead; one wait + barrier per
+// stage (the digit chunks are shared by the block). Row blocks are read without bounds
+// checks: the packed layouts are padded to a multiple of 512 rows.
+template <int W, int RT>
+struct ThinCfg {
+    static constexpr int kA = W * RT * 1024;  // packed rows of a stage
+    static constexpr int kB = 4 * 1024;       // digits of a stage (4 K-steps)
+    static constexpr int kSlot = kA + kB;
+    static constexpr int kSink = W > 4 ? (W - 4) * 1024 : 0;
+    static constexpr int kOps = RT + 1;       // copies per wave per stage
+    static_assert(W == 4 || W == 8, "4 or 8 waves");
+};
+
+template <int W, int RT>
+__device__ inline void thin_issue(const uint8_t* __restrict__ arow, int64_t frag_stride,
+                                  const uint8_t* __restrict__ bstage, uint8_t* __restrict__ slot,
+                                  uint8_t* __restrict__ sink, int wave, int lane) {
+    using C = ThinCfg<W, RT>;
 #pragma unroll
-    for (int a = 0; a < RT; ++a)
-        A[a] = *reinterpret_cast<const uint4*>(base + a * frag_stride + loff + ss * 64);
-#pragma unroll
-    for (int s = 0; s < 4; ++s) B[s] = bq[(4 * ss + s) * 64];
+    for (int q = 0; q < RT; ++q) {
+        const int f = wave * RT + q;
+        __builtin_amdgcn_global_load_lds((const void*)(arow + q * frag_stride),
+                                         (lds_void*)(slot + f * 1024), 16, 0, 0);
+    }
+    if (wave < 4)
+        __builtin_amdgcn_global_load_lds((const void*)(bstage + wave * 1024 + lane * 16),
+                                         (lds_void*)(slot + C::kA + wave * 1024), 16, 0, 0);
+    else
+        __builtin_amdgcn_global_load_lds((const void*)(bstage + (wave & 3) * 1024 + lane * 16),
+                                         (lds_void*)(sink + (wave - 4) * 1024), 16, 0, 0);
 }
 
-template <int RT>
-__device__ inline void thin_mma(const uint4 (&A)[RT], const uint4 (&B)[4], v4i (&acc)[RT]) {
+template <int W, int RT>
+__device__ inline void thin_mma(const uint8_t* __restrict__ slot, int wave, int lane,
+                                v4i (&acc)[RT]) {
+    using C = ThinCfg<W, RT>;
+    uint4 av[RT];
+#pragma unroll
+    for (int q = 0; q < RT; ++q)
+        av[q] = *reinterpret_cast<const uint4*>(slot + (wave * RT + q) * 1024 + lane * 16);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-        const v4i bv = *reinterpret_cast<const v4i*>(&B[s]);
+        const v4i bv = *reinterpret_cast<const v4i*>(slot + C::kA + s * 1024 + lane * 16);
 #pragma unroll
-        for (int a = 0; a < RT; ++a)
-            acc[a] = __builtin_amdgcn_mfma_i32_16x16x64_i8(decode16(dword_of(A[a], s)), bv, acc[a], 0, 0, 0);
+        for (int q = 0; q < RT; ++q)
+            acc[q] = __builtin_amdgcn_mfma_i32_16x16x64_i8(decode16(dword_of(av[q], s)), bv, acc[q], 0, 0, 0);
     }
 }
 
-template <int RT>
-__global__ __launch_bounds__(256) void k_geno_thin(const uint8_t* __restrict__ rows, int64_t pitch,
-                                                   int64_t nrows, int64_t n_ss, int64_t ss_per_split,
-                                                   int64_t R, int S, const uint8_t* __restrict__ Bq,
-                                                   int32_t* __restrict__ out, const int* skip) {
+template <int W, int RT, int NS>
+__device__ inline void thin_stage_barrier() {
+    constexpr int kAhead = (NS - 2) * ThinCfg<W, RT>::kOps;
+    static_assert(kAhead < 64, "vmcnt field");
+    asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(kAhead) : "memory");
+}
+
+template <int W, int RT, int NS>
+__global__ __launch_bounds__(W * 64) void k_geno_thin(const uint8_t* __restrict__ rows, int64_t pitch,
+                                                      int64_t nrows, int64_t n_ss, int64_t ss_per_split,
+                                                      int64_t R, int S, const uint8_t* __restrict__ Bq,
+                                                      int32_t* __restrict__ out, const int* skip) {
     if (skip && *skip) return;
+    using C = ThinCfg<W, RT>;
+    __shared__ __attribute__((aligned(1024))) uint8_t lds[NS * C::kSlot + C::kSink];
     int64_t rb;
     int sp;
     block_coords(blockIdx.x, R, S, rb, sp);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, rr = lane & 15;
-    const int64_t row_base = rb * (64 * RT) + wave * (16 * RT);
-    const uint8_t* base = rows + row_base * pitch;  // uniform per wave
-    const int64_t fstride = 16 * pitch;
-    const uint32_t loff = (uint32_t)(rr * pitch + 16 * g);
+    const int64_t row_base = rb * (16 * W * RT) + wave * (16 * RT);
     const int64_t ss0 = (int64_t)sp * ss_per_split;
     const int64_t ss1 = ss0 + ss_per_split < n_ss ? ss0 + ss_per_split : n_ss;
-    const uint4* bq = reinterpret_cast<const uint4*>(Bq) + lane;  // [ks][64 lanes] x 16 B
+    const int64_t nt = ss1 > ss0 ? ss1 - ss0 : 0;
+    const uint8_t* arow = rows + (row_base + rr) * pitch + 16 * g + 64 * ss0;
+    const int64_t fstride = 16 * pitch;
+    const uint8_t* bst = Bq + ss0 * C::kB;
+    uint8_t* sink = lds + NS * C::kSlot;
 
     v4i acc[RT];
 #pragma unroll
     for (int a = 0; a < RT; ++a) acc[a] = v4i{0, 0, 0, 0};
-    uint4 A0[RT], A1[RT], B0[4], B1[4];
-    if (ss0 < ss1) thin_load<RT>(base, fstride, loff, ss0, bq, A0, B0);
-    for (int64_t ss = ss0; ss < ss1; ss += 2) {
-        if (ss + 1 < ss1) thin_load<RT>(base, fstride, loff, ss + 1, bq, A1, B1);
-        thin_mma<RT>(A0, B0, acc);
-        if (ss + 1 >= ss1) break;
-        if (ss + 2 < ss1) thin_load<RT>(base, fstride, loff, ss + 2, bq, A0, B0);
-        thin_mma<RT>(A1, B1, acc);
+    if (nt > 0) {
+#pragma unroll
+        for (int u = 0; u < NS - 1; ++u) {
+            const int64_t t = u < nt ? u : nt - 1;
+            thin_issue<W, RT>(arow + 64 * t, fstride, bst + t * C::kB, lds + u * C::kSlot, sink, wave,
+                              lane);
+        }
+        for (int64_t t0 = 0; t0 < nt; t0 += NS) {
+#pragma unroll
+            for (int u = 0; u < NS; ++u) {
+                const int64_t t = t0 + u;
+                if (t >= nt) break;
+                thin_stage_barrier<W, RT, NS>();
+                const int64_t tn = t + NS - 1 < nt ? t + NS - 1 : nt - 1;
+                thin_issue<W, RT>(arow + 64 * tn, fstride, bst + tn * C::kB,
+                                  lds + ((u + NS - 1) % NS) * C::kSlot, sink, wave, lane);
+                __builtin_amdgcn_sched_barrier(0);
+                thin_mma<W, RT>(lds + u * C::kSlot, wave, lane, acc);
+            }
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     }
     int32_t* o = out + (int64_t)sp * nrows * 16;
 #pragma unroll
@@ -533,7 +587,7 @@ int choose_split(int64_t R, int64_t n_ss, bool wide) {
 }
 
 struct Variant {
-    int cfg;  // thin: row fragments of 16 rows per wave (8 or 4); wide: index into kWide
+    int cfg;  // index into kThin (thin) or kWide (wide)
     int S;    // split-K
 };
 
@@ -542,9 +596,14 @@ struct WideShape {
     int W, NS, CG, RA;
 };
 constexpr WideShape kWide[4] = {{8, 4, 1, 2}, {4, 3, 1, 2}, {8, 4, 2, 4}, {4, 3, 2, 4}};
+// thin block shapes: waves, row fragments of 16 rows per wave, ring stages
+struct ThinShape {
+    int W, RT, NS;
+};
+constexpr ThinShape kThin[4] = {{4, 4, 3}, {4, 4, 4}, {8, 2, 3}, {8, 2, 4}};
 
 int64_t rows_per_block(bool wide, int cfg) {
-    if (!wide) return 64 * cfg;
+    if (!wide) return (int64_t)16 * kThin[cfg].W * kThin[cfg].RT;
     const WideShape& w = kWide[cfg];
     return (int64_t)(w.W / w.CG) * 32 * w.RA;
 }
@@ -588,10 +647,12 @@ void launch_variant(hipStream_t s, bool wide, int JT, const Variant& v, const ui
     const int64_t per = (nss + v.S - 1) / v.S;
     const unsigned grid = (unsigned)(R * v.S);
     if (!wide) {
-        if (v.cfg == 8)
-            k_geno_thin<8><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip);
-        else
-            k_geno_thin<4><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip);
+        switch (v.cfg) {
+            case 0: k_geno_thin<4, 4, 3><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
+            case 1: k_geno_thin<4, 4, 4><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
+            case 2: k_geno_thin<8, 2, 3><<<grid, 512, 0, s>>>(rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
+            default: k_geno_thin<8, 2, 4><<<grid, 512, 0, s>>>(rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
+        }
     } else if (JT == 1) {
         launch_wide<1>(s, v, grid, rows, pitch, nrows, nss, per, R, Bq, out, skip);
     } else if (JT == 2) {
@@ -607,9 +668,9 @@ void launch_variant(hipStream_t s, bool wide, int JT, const Variant& v, const ui
 // Tile autotuner: the first contraction of each shape (pass, rows, K, digit columns)
 // times every candidate variant once on the live data with HIP events and keeps the
 // fastest; every variant computes the same exact integer sums, so the choice never
-// changes a result. Candidates: thin rows-per-wave 8 / 4, the wide block shapes of
-// kWide (the RA = 4 ones need an even JT), each with split-K S/2, S, 2 S around
-// choose_split().
+// changes a result. Candidates: the thin block shapes of kThin, the wide block shapes of
+// kWide (the RA = 4 ones need an even JT), each with split-K S/2, S, 2 S (and 4 S for
+// thin) around choose_split().
 static std::vector<Variant> candidates(bool wide, int JT, int64_t nrows, int64_t nss) {
     std::vector<Variant> v;
     std::vector<int> cfgs;
@@ -617,12 +678,13 @@ static std::vector<Variant> candidates(bool wide, int JT, int64_t nrows, int64_t
         for (int q = 0; q < 4; ++q)
             if (JT % kWide[q].CG == 0) cfgs.push_back(q);
     } else {
-        cfgs = {8, 4};
+        cfgs = {0, 1, 2, 3};
     }
     for (int cfg : cfgs) {
         const int64_t R = (nrows + rows_per_block(wide, cfg) - 1) / rows_per_block(wide, cfg);
         const int S0 = choose_split(R, nss, wide);
-        for (int S : {S0 / 2, S0, S0 * 2}) {
+        for (int S : {S0 / 2, S0, S0 * 2, S0 * 4}) {
+            if (wide && S == S0 * 4) continue;
             if (S < 1 || S > 64 || (S > 1 && nss / S < 4)) continue;
             v.push_back({cfg, S});
         }
