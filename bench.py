@@ -52,6 +52,22 @@ def build_cohort(g, n, m, lam, seed):
     return c, y, kept
 
 
+def measured_traffic(prefix):
+    """HBM bytes per launch of the dominant kernel from the committed rocprof summary
+    (calibrated FETCH_SIZE + WRITE_SIZE; profiles/k_geno_mfma_n200k_m500k.md has the
+    commands), or None when no summary of that kernel is present."""
+    path = os.path.join(ROOT, "profiles", "k_geno_mfma_n200k_m500k.traffic.json")
+    try:
+        with open(path) as f:
+            tr = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    for name, v in tr.items():
+        if name.startswith(prefix) and isinstance(v, dict):
+            return v["fetch_bytes"] + v["write_bytes"], name
+    return None, None
+
+
 def cpu_baseline(g, c, kept, iters, sample_snps, threads):
     """FP32 OpenMP restatement of computeVx_multi (oracle/) on the first sample_snps kept
     SNPs at full N, k = 1; per-V.x time scaled by M / sample and by the FP64 iteration
@@ -94,8 +110,8 @@ def main():
                     help="first run 3 plain 1 GiB device copies (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; the multi-GPU SNP sharding is "
-                                   "design-only (DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; SNP-sharded runs over several "
+                                   "GPUs are not implemented", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import globe_amd as G
@@ -160,6 +176,9 @@ def main():
     # 1-RHS FP64 pass HBM roofline (algorithmic packed bytes)
     gbps64 = [p["bytes"] / (p["ms_avg"] * 1e-3) / 1e9 if p["launches"] else 0.0 for p in p64]
 
+    traffic, traffic_src = measured_traffic("k_geno_wide")
+    if traffic_src:
+        traffic_src = f"profiles/k_geno_mfma_n200k_m500k.md ({traffic_src})"
     cpu = None
     if not args.no_cpu:
         log("CPU baseline ...")
@@ -190,8 +209,9 @@ def main():
         "mixed_solution_rel_err_y": err_y,
         "mixed_iters": [st["iters"] for st in stats],
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_INT8_TOPS,
-                     "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_INT8_TOPS, "traffic": None,
-                     "kernel": "k_geno_mfma (both passes)", "ms_per_launch": tot_ms / max(launches, 1),
+                     "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_INT8_TOPS, "traffic": traffic,
+                     "traffic_source": traffic_src, "algorithmic_bytes": pm[0]["bytes"],
+                     "kernel": "k_geno_wide (both passes)", "ms_per_launch": tot_ms / max(launches, 1),
                      "issued_int8_TOPs": issued_tops, "issued_frac": issued_tops / PEAK_INT8_TOPS},
         "cpu_baseline": cpu,
     }

@@ -170,7 +170,7 @@ void op_vx(globe_ctx* c, const double* x, int64_t ldx, int k, double* y, int64_t
     double* GZ = c->w_GZ.as<double>();
     hipStream_t s = c->stream;
     launch_seg_sums(s, n, c->d_blk_off.as<int64_t>(), c->d_blk_ind.as<int32_t>(),
-                    c->d_times.as<double>(), x, ldx, k, S0, S1, n, skip);
+                    c->d_times.as<double>(), x, ldx, k, S0, S1, n, skip, c->tiles);
     if (mode == GLOBE_VX_GRM_ONLY) {
         genetic_product(c, S0, k, GZ, precision, skip);
         launch_copy_cols(s, n, GZ, n, y, ldy, k);
@@ -187,7 +187,7 @@ static void precond(globe_ctx* c, const double* r, int64_t ldr, int k, double* z
     ensure_theta(c, k);
     launch_precond(c->stream, c->n, c->d_blk_off.as<int64_t>(), c->d_blk_ind.as<int32_t>(),
                    c->d_times.as<double>(), c->d_gdiag.as<double>(), c->d_theta.as<double>(), r,
-                   ldr, k, z, ldz, part_rz, part_rr, st, 0);
+                   ldr, k, z, ldz, part_rz, part_rr, st, 0, c->tiles);
 }
 
 // PCG with the reference's control flow (PCGsolver.cpp:1324-1444 for rule 0,
@@ -235,7 +235,7 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
     HIP_CHECK(hipEventRecord(e0, s));
     HIP_CHECK(hipMemcpyAsync(dst, hs, sizeof(PcgState), hipMemcpyHostToDevice, s));
     const int np = parts_for(nrec);
-    const int npb = parts_for(c->n);
+    const int npb = precond_parts(c->n, c->tiles);
     // starting point and residual
     if (o.x0_mode == GLOBE_X0_ZERO) {
         launch_fill_cols(s, nrec, X, ldx, k, 0.0);
@@ -433,6 +433,34 @@ int globe_set_cohort(globe_ctx* c, int64_t nrec, int32_t n, int32_t p, const int
     HIP_CHECK(hipMemcpy(c->d_X.p, c->h_X.data(), sizeof(double) * nrec * (p + 1), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(c->d_blk_off.p, off.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(c->d_blk_ind.p, ind.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    // record tiles: greedy runs of whole blocks of at most kTileRecords records
+    {
+        std::vector<int32_t> tile_blk{0}, rec_blk((size_t)nrec);
+        int64_t maxblk = 0, in_tile = 0;
+        for (int32_t b = 0; b < n; ++b) {
+            const int64_t len = off[b + 1] - off[b];
+            maxblk = std::max(maxblk, len);
+            if (in_tile > 0 && in_tile + len > kTileRecords) {
+                tile_blk.push_back(b);
+                in_tile = 0;
+            }
+            in_tile += len;
+            for (int64_t r = off[b]; r < off[b + 1]; ++r) rec_blk[r] = b;
+        }
+        tile_blk.push_back(n);
+        c->tiles = RecTiles{};
+        if (maxblk <= kTileMaxRecords) {
+            c->d_tile_blk.ensure(sizeof(int32_t) * tile_blk.size());
+            c->d_rec_blk.ensure(sizeof(int32_t) * std::max<int64_t>(nrec, 1));
+            HIP_CHECK(hipMemcpy(c->d_tile_blk.p, tile_blk.data(), sizeof(int32_t) * tile_blk.size(),
+                                hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(c->d_rec_blk.p, rec_blk.data(), sizeof(int32_t) * nrec, hipMemcpyHostToDevice));
+            c->tiles.tile_blk = c->d_tile_blk.as<int32_t>();
+            c->tiles.rec_blk = c->d_rec_blk.as<int32_t>();
+            c->tiles.ntiles = (int32_t)tile_blk.size() - 1;
+            c->tiles.cap = kTileRecords;
+        }
+    }
     c->has_cohort = true;
     // genetics must match n
     if ((c->gen == 1 || c->gen == 2) && c->gdiag_ok && c->d_gdiag.bytes < sizeof(double) * (size_t)n)

@@ -19,6 +19,8 @@ struct globe_ctx {
     std::vector<int64_t> h_blk_off;
     std::vector<double> h_times, h_X;
     globe::DevBuf d_indiv, d_times, d_X, d_blk_off, d_blk_ind;
+    globe::DevBuf d_tile_blk, d_rec_blk;  // record tiles (see RecTiles)
+    globe::RecTiles tiles;
 
     // genetics: 0 none, 1 packed genotypes, 2 GRM
     int gen = 0;

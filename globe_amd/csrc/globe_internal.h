@@ -55,9 +55,23 @@ struct PcgState {
 };
 
 // ---------------------------------------------------------------- record space
+// Record tiles: runs of whole individual blocks holding at most `cap` records each
+// (tile t = blocks tile_blk[t] .. tile_blk[t+1]-1); rec_blk[r] = block of record r.
+// ntiles == 0 selects the thread-per-block kernels (an individual with more than
+// kTileMaxRecords records).
+struct RecTiles {
+    const int32_t* tile_blk = nullptr;
+    const int32_t* rec_blk = nullptr;
+    int32_t ntiles = 0;
+    int32_t cap = 0;
+};
+constexpr int kTileRecords = 2048;
+constexpr int kTileMaxRecords = 2048;  // LDS: 4 doubles per record slot (64 KB)
+// partial-sum slots written by launch_precond
+int precond_parts(int32_t nblk, const RecTiles& tiles);
 void launch_seg_sums(hipStream_t s, int32_t nblk, const int64_t* blk_off, const int32_t* blk_ind,
                      const double* times, const double* x, int64_t ldx, int k, double* S0,
-                     double* S1, int64_t ldn, const int* skip);
+                     double* S1, int64_t ldn, const int* skip, const RecTiles& tiles);
 void launch_record_out(hipStream_t s, int64_t nrec, const int32_t* indiv, const double* times,
                        const double* x, int64_t ldx, int k, const double* GZ, const double* S0,
                        const double* S1, int64_t ldn, const double* theta_cols, double* y,
@@ -65,7 +79,7 @@ void launch_record_out(hipStream_t s, int64_t nrec, const int32_t* indiv, const 
 void launch_precond(hipStream_t s, int32_t nblk, const int64_t* blk_off, const int32_t* blk_ind,
                     const double* times, const double* gdiag, const double* theta_cols,
                     const double* r, int64_t ldr, int k, double* z, int64_t ldz, double* part_rz,
-                    double* part_rr, const PcgState* st, int tilde_v);
+                    double* part_rr, const PcgState* st, int tilde_v, const RecTiles& tiles);
 void launch_dot_parts(hipStream_t s, int64_t nrec, const double* a, int64_t lda, const double* b,
                       int64_t ldb, int k, double* part);
 void launch_reduce_parts(hipStream_t s, const double* part, int nparts, int k, double* out);

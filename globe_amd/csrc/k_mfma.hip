@@ -509,6 +509,22 @@ constexpr int kEpiRows = 64;
 
 __device__ inline double combine(const int32_t* __restrict__ out, int nsplit, int64_t split_stride,
                                  int64_t base, int limbs) {
+    if (limbs == 4) {
+        // the 4 digit sums of a column are 16-byte aligned (J16 and the split stride are
+        // multiples of 16): one vector load per split, same combination order
+        long long s[4] = {0, 0, 0, 0};
+        for (int sp = 0; sp < nsplit; ++sp) {
+            const int4 d = *reinterpret_cast<const int4*>(out + sp * split_stride + base);
+            s[0] += d.x;
+            s[1] += d.y;
+            s[2] += d.z;
+            s[3] += d.w;
+        }
+        double v = 0.0;
+#pragma unroll
+        for (int l = 0; l < 4; ++l) v += ldexp((double)s[l], 8 * l);
+        return v;
+    }
     double v = 0.0;
     for (int l = 0; l < limbs; ++l) {
         long long s = 0;

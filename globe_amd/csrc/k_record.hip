@@ -6,6 +6,8 @@
 // Layout: record-space matrices column-major [k][nrec] (leading dimension ld);
 // individual-space matrices [k][n]; theta_cols is [k][5] = per column
 // {sig2e, sig2g, sig2b0, sigb01, sig2b1}.
+#include <algorithm>
+
 #include "globe_internal.h"
 
 namespace globe {
@@ -90,6 +92,125 @@ __global__ __launch_bounds__(kBlock) void k_record_out(
     if (part_dot) {
         double s = block_sum(acc);
         if (threadIdx.x == 0) part_dot[c * kMaxParts + blockIdx.x] = s;
+    }
+}
+
+// Tiled forms of the two per-block kernels: a workgroup stages the records of a tile
+// of whole individual blocks in LDS with coalesced loads, one thread per block sums its
+// records from LDS in record order (so the block sums equal the thread-per-block
+// kernels' bit for bit), and the preconditioner writes z back one thread per record.
+__global__ __launch_bounds__(kBlock) void k_seg_sums_tiled(
+    int32_t ntiles, const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ off,
+    const int32_t* __restrict__ ind, const double* __restrict__ t, const double* __restrict__ x,
+    int64_t ldx, double* __restrict__ S0, double* __restrict__ S1, int64_t ldn, int cap,
+    const int* skip) {
+    if (skip && *skip) return;
+    extern __shared__ double sm[];
+    double* sx = sm;
+    double* st = sm + cap;
+    const int c = blockIdx.y;
+    const double* xc = x + c * ldx;
+    for (int32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int32_t b0 = tile_blk[tile], b1 = tile_blk[tile + 1];
+        const int64_t r0 = off[b0];
+        const int nr = (int)(off[b1] - r0);
+        __syncthreads();
+        for (int q = threadIdx.x; q < nr; q += kBlock) {
+            sx[q] = xc[r0 + q];
+            st[q] = t[r0 + q];
+        }
+        __syncthreads();
+        for (int32_t b = b0 + threadIdx.x; b < b1; b += kBlock) {
+            const int q0 = (int)(off[b] - r0), q1 = (int)(off[b + 1] - r0);
+            double s0 = 0.0, s1 = 0.0;
+            for (int q = q0; q < q1; ++q) {
+                const double v = sx[q];
+                s0 += v;
+                s1 += st[q] * v;
+            }
+            const int32_t i = ind[b];
+            S0[c * ldn + i] = s0;
+            if (S1) S1[c * ldn + i] = s1;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_precond_tiled(
+    int32_t ntiles, const int32_t* __restrict__ tile_blk, const int32_t* __restrict__ rec_blk,
+    const int64_t* __restrict__ off, const int32_t* __restrict__ ind, const double* __restrict__ t,
+    const double* __restrict__ gdiag, const double* __restrict__ th, const double* __restrict__ r,
+    int64_t ldr, double* __restrict__ z, int64_t ldz, double* part_rz, double* part_rr,
+    const PcgState* stt, int tilde_v, int cap) {
+    if (stt && stt->done) return;
+    const int c = blockIdx.y;
+    if (stt && !stt->active[c]) {
+        if (threadIdx.x == 0) {
+            if (part_rz) part_rz[c * kMaxParts + blockIdx.x] = 0.0;
+            if (part_rr) part_rr[c * kMaxParts + blockIdx.x] = 0.0;
+        }
+        return;
+    }
+    extern __shared__ double sm[];
+    double* sr = sm;             // r of the tile's records
+    double* st = sm + cap;       // t of the tile's records
+    double* sc0 = sm + 2 * cap;  // per block: Woodbury coefficients
+    double* sc1 = sm + 3 * cap;
+    const double se = th[c * 5 + 0], sg = th[c * 5 + 1], sb0 = th[c * 5 + 2],
+                 sb01 = th[c * 5 + 3], sb1 = th[c * 5 + 4];
+    const double* rc = r + c * ldr;
+    double* zc = z + c * ldz;
+    const double inv_se = 1.0 / se;
+    double arz = 0.0, arr = 0.0;
+    for (int32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int32_t b0 = tile_blk[tile], b1 = tile_blk[tile + 1];
+        const int64_t r0 = off[b0];
+        const int nr = (int)(off[b1] - r0);
+        __syncthreads();
+        for (int q = threadIdx.x; q < nr; q += kBlock) {
+            sr[q] = rc[r0 + q];
+            st[q] = t[r0 + q];
+        }
+        __syncthreads();
+        for (int32_t b = b0 + threadIdx.x; b < b1; b += kBlock) {
+            const int q0 = (int)(off[b] - r0), q1 = (int)(off[b + 1] - r0);
+            double a01 = 0.0, a11 = 0.0, w0 = 0.0, w1 = 0.0;
+            for (int q = q0; q < q1; ++q) {
+                const double tq = st[q], rv = sr[q];
+                a01 += tq;
+                a11 += tq * tq;
+                w0 += rv;
+                w1 += tq * rv;
+            }
+            const double a00 = (double)(q1 - q0);
+            const double gv = sg * (tilde_v ? 1.0 : gdiag[ind[b]]);
+            const double c00 = gv + sb0, c01 = sb01, c11 = sb1;
+            const double m00 = se + c00 * a00 + c01 * a01;
+            const double m01 = c00 * a01 + c01 * a11;
+            const double m10 = c01 * a00 + c11 * a01;
+            const double m11 = se + c01 * a01 + c11 * a11;
+            const double v0 = c00 * w0 + c01 * w1;
+            const double v1 = c01 * w0 + c11 * w1;
+            const double det = m00 * m11 - m01 * m10;
+            sc0[b - b0] = (m11 * v0 - m01 * v1) / det;
+            sc1[b - b0] = (m00 * v1 - m10 * v0) / det;
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < nr; q += kBlock) {
+            const int bl = rec_blk[r0 + q] - b0;
+            const double rv = sr[q];
+            const double zv = (rv - sc0[bl] - sc1[bl] * st[q]) * inv_se;
+            zc[r0 + q] = zv;
+            arz += rv * zv;
+            arr += rv * rv;
+        }
+    }
+    if (part_rz) {
+        double s = block_sum(arz);
+        if (threadIdx.x == 0) part_rz[c * kMaxParts + blockIdx.x] = s;
+    }
+    if (part_rr) {
+        double s = block_sum(arr);
+        if (threadIdx.x == 0) part_rr[c * kMaxParts + blockIdx.x] = s;
     }
 }
 
@@ -503,11 +624,21 @@ __global__ __launch_bounds__(kBlock) void k_fill(double* dst, int64_t count, dou
 
 #define LAUNCH_CHECK() HIP_CHECK(hipGetLastError())
 
+int precond_parts(int32_t nblk, const RecTiles& tiles) {
+    return tiles.ntiles > 0 ? std::min<int>(tiles.ntiles, kMaxParts) : parts_for(nblk);
+}
+
 void launch_seg_sums(hipStream_t s, int32_t nblk, const int64_t* blk_off, const int32_t* blk_ind,
                      const double* times, const double* x, int64_t ldx, int k, double* S0,
-                     double* S1, int64_t ldn, const int* skip) {
-    dim3 g(parts_for(nblk), k);
-    k_seg_sums<<<g, kBlock, 0, s>>>(nblk, blk_off, blk_ind, times, x, ldx, S0, S1, ldn, skip);
+                     double* S1, int64_t ldn, const int* skip, const RecTiles& tiles) {
+    if (tiles.ntiles > 0) {
+        dim3 g(std::min<int>(tiles.ntiles, kMaxParts), k);
+        k_seg_sums_tiled<<<g, kBlock, sizeof(double) * 2 * tiles.cap, s>>>(
+            tiles.ntiles, tiles.tile_blk, blk_off, blk_ind, times, x, ldx, S0, S1, ldn, tiles.cap, skip);
+    } else {
+        dim3 g(parts_for(nblk), k);
+        k_seg_sums<<<g, kBlock, 0, s>>>(nblk, blk_off, blk_ind, times, x, ldx, S0, S1, ldn, skip);
+    }
     LAUNCH_CHECK();
 }
 
@@ -524,10 +655,17 @@ void launch_record_out(hipStream_t s, int64_t nrec, const int32_t* indiv, const 
 void launch_precond(hipStream_t s, int32_t nblk, const int64_t* blk_off, const int32_t* blk_ind,
                     const double* times, const double* gdiag, const double* theta_cols,
                     const double* r, int64_t ldr, int k, double* z, int64_t ldz, double* part_rz,
-                    double* part_rr, const PcgState* st, int tilde_v) {
-    dim3 g(parts_for(nblk), k);
-    k_precond<<<g, kBlock, 0, s>>>(nblk, blk_off, blk_ind, times, gdiag, theta_cols, r, ldr, z,
-                                   ldz, part_rz, part_rr, st, tilde_v);
+                    double* part_rr, const PcgState* st, int tilde_v, const RecTiles& tiles) {
+    if (tiles.ntiles > 0) {
+        dim3 g(precond_parts(nblk, tiles), k);
+        k_precond_tiled<<<g, kBlock, sizeof(double) * 4 * tiles.cap, s>>>(
+            tiles.ntiles, tiles.tile_blk, tiles.rec_blk, blk_off, blk_ind, times, gdiag, theta_cols, r,
+            ldr, z, ldz, part_rz, part_rr, st, tilde_v, tiles.cap);
+    } else {
+        dim3 g(parts_for(nblk), k);
+        k_precond<<<g, kBlock, 0, s>>>(nblk, blk_off, blk_ind, times, gdiag, theta_cols, r, ldr, z,
+                                       ldz, part_rz, part_rr, st, tilde_v);
+    }
     LAUNCH_CHECK();
 }
 

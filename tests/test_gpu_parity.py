@@ -228,3 +228,38 @@ def test_invalid_cohorts_rejected(gpu_ctx_factory):
     with pytest.raises(G.GlobeError):  # no genetics yet
         g.set_varcomp((0.5, 0.1, 0.1, 0.0, 0.1))
         g.computeVx_multi(np.ones(4))
+
+
+@pytest.mark.parametrize("big_block", [False, True])
+def test_record_tiles_and_large_block_fallback(big_block, gpu_ctx_factory):
+    """Cohorts spanning many record tiles (2048-record LDS tiles of whole individuals)
+    and, with one individual holding more than 2048 records, the thread-per-individual
+    kernels: operator, preconditioner and a PCG solve against the oracle."""
+    import oracle as O
+    _say("record-tiles", big_block)
+    d = synth.make_geno_case(n=1400, m=700, lam=6, seed=77)
+    c = d.cohort
+    if big_block:
+        counts = np.bincount(c.indiv, minlength=c.n)
+        counts[5] = 2100
+        indiv = np.repeat(np.arange(c.n, dtype=np.int32), counts)
+        rng = np.random.default_rng(5)
+        times = synth.f32(rng.uniform(0.0, 10.0, size=indiv.shape[0]))
+        covars = np.stack([np.ones(indiv.shape[0]), (indiv % 2).astype(np.float64),
+                           synth.f32(40.0 + (indiv % 30))], axis=1)
+        c = synth.Cohort(n=c.n, indiv=indiv, times=times, covars=covars)
+        d.cohort = c
+        d.y = rng.standard_normal(c.nrec)
+    g = gpu_ctx_factory()
+    g.set_cohort(c.indiv, c.times, c.X)
+    g.prepGenotypeInput(d.bed_rows, 0.01, 0.2)
+    g.set_varcomp(d.theta)
+    orc = O.oracle_from_synth(d)
+    x = np.random.default_rng(9).standard_normal((c.nrec, 3))
+    assert rel(g.computeVx_multi(x), orc.vx(x)) < 1e-12
+    q = orc.precond()
+    assert rel(g.applyPreconditioner(x), orc.precond_apply(q, x)) < 1e-11
+    b = np.asarray(d.y)[:, None]
+    xg, _ = g.pcg(b, tol=1e-9, rule=G.RULE_PER_COLUMN, precision=G.PREC_FP64, stagnation=False)
+    xo, _ = orc.pcg(b, tol=1e-9, rule=2, stagnation=False, q=q)
+    assert rel(xg, xo) < 1e-7
