@@ -57,6 +57,21 @@ class AiremlOpts(C.Structure):
                 ("probes", C.c_int32), ("seed", C.c_uint64), ("pcg", PcgOpts)]
 
 
+class Request(C.Structure):
+    _fields_ = [("nrec", C.c_int64), ("indiv", C.c_void_p), ("times", C.c_void_p), ("y", C.c_void_p),
+                ("theta", C.c_double * 5)]
+
+
+class ServeOpts(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("max_records", C.c_int64), ("hbm_budget", C.c_int64),
+                ("tol", C.c_double), ("max_iter", C.c_int32), ("precision", C.c_int32)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("id", C.c_int64), ("iters", C.c_int32), ("converged", C.c_int32), ("ratio", C.c_double),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
 LOG_FN = C.CFUNCTYPE(None, C.c_char_p, C.c_void_p)
 
 
@@ -110,6 +125,12 @@ def load_library(path: str = LIB_PATH):
         "globe_profile_read": ([P, C.c_int, P], C.c_int),
         "globe_aireml_step": ([P, P, P, dbl, P], C.c_int),
         "globe_aireml_fit": ([P, P, P, P, P, P, P, P, P, P], C.c_int),
+        "globe_serve_open": ([P, P], C.c_int),
+        "globe_serve_submit": ([P, P, P], C.c_int),
+        "globe_serve_step": ([P, P, P, P], C.c_int),
+        "globe_serve_take": ([P, i64, P, P], C.c_int),
+        "globe_serve_finished": ([P, P, i32, P], C.c_int),
+        "globe_serve_close": ([P], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -364,6 +385,48 @@ class Globe:
                 "ms_avg": st.ms_total / st.launches if st.launches else 0.0,
                 "bytes": st.bytes_per_launch, "flops": st.flops_per_launch,
                 "issued": st.issued_ops_per_launch}
+
+    # ---- serving loop
+    def serve_open(self, *, slots=32, max_records, hbm_budget=0, tol=1e-6, max_iter=1000,
+                   precision=PREC_MIXED):
+        o = ServeOpts(slots, max_records, hbm_budget, tol, max_iter, precision)
+        self._check(self.L.globe_serve_open(self.h, C.byref(o)))
+        self._serve_nrec = {}
+
+    def serve_submit(self, indiv, times, y, theta):
+        """Queue a solve V x = y for the records (indiv, times); returns the request id."""
+        indiv = np.ascontiguousarray(indiv, dtype=np.int32)
+        times = np.ascontiguousarray(times, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        r = Request(indiv.shape[0], indiv.ctypes.data, times.ctypes.data, y.ctypes.data,
+                    (C.c_double * 5)(*theta))
+        rid = C.c_int64()
+        self._check(self.L.globe_serve_submit(self.h, C.byref(r), C.byref(rid)))
+        self._serve_nrec[rid.value] = indiv.shape[0]
+        return rid.value
+
+    def serve_step(self):
+        """One admission + block-PCG iteration; returns (finished, active, queued)."""
+        f, a, q = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self.L.globe_serve_step(self.h, C.byref(f), C.byref(a), C.byref(q)))
+        return f.value, a.value, q.value
+
+    def serve_finished(self, max_ids=4096):
+        ids = np.zeros(max_ids, dtype=np.int64)
+        n = C.c_int32()
+        self._check(self.L.globe_serve_finished(self.h, _p(ids), max_ids, C.byref(n)))
+        return [int(v) for v in ids[:n.value]]
+
+    def serve_take(self, rid):
+        x = np.zeros(self._serve_nrec[rid])
+        res = ServeResult()
+        self._check(self.L.globe_serve_take(self.h, rid, _p(x), C.byref(res)))
+        del self._serve_nrec[rid]
+        return x, {"id": res.id, "iters": res.iters, "converged": bool(res.converged), "ratio": res.ratio,
+                   "t_submit": res.t_submit, "t_admit": res.t_admit, "t_done": res.t_done}
+
+    def serve_close(self):
+        self._check(self.L.globe_serve_close(self.h))
 
     # ---- AI-REML (runAIREML)
     def aireml_fit(self, y, theta0, *, max_iter=50, tol=1e-3, num_mc=15, probes="gaussian",

@@ -104,11 +104,10 @@ void ensure_gdiag(globe_ctx* c) {
     c->gdiag_ok = true;
 }
 
-static void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int precision,
-                            const int* skip) {
+void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int precision,
+                     const int* skip, const double* th) {
     const int32_t n = c->n;
     hipStream_t s = c->stream;
-    const double* th = c->d_theta.as<double>();
     if (c->gen == 2) {
         launch_grm_product(s, c->d_grm.as<float>(), n, S0, n, k, th, GZ, n, skip);
         return;
@@ -169,25 +168,23 @@ void op_vx(globe_ctx* c, const double* x, int64_t ldx, int k, double* y, int64_t
     double* S1 = c->w_S1.as<double>();
     double* GZ = c->w_GZ.as<double>();
     hipStream_t s = c->stream;
-    launch_seg_sums(s, n, c->d_blk_off.as<int64_t>(), c->d_blk_ind.as<int32_t>(),
-                    c->d_times.as<double>(), x, ldx, k, S0, S1, n, skip, c->tiles);
+    launch_seg_sums(s, c->meta, x, ldx, k, S0, S1, n, skip);
     if (mode == GLOBE_VX_GRM_ONLY) {
-        genetic_product(c, S0, k, GZ, precision, skip);
+        genetic_product(c, S0, k, GZ, precision, skip, c->d_theta.as<double>());
         launch_copy_cols(s, n, GZ, n, y, ldy, k);
         return;
     }
-    genetic_product(c, S0, k, GZ, precision, skip);
-    launch_record_out(s, c->nrec, c->d_indiv.as<int32_t>(), c->d_times.as<double>(), x, ldx, k, GZ,
-                      S0, S1, n, c->d_theta.as<double>(), y, ldy, part_dot, skip);
+    genetic_product(c, S0, k, GZ, precision, skip, c->d_theta.as<double>());
+    launch_record_out(s, c->meta, x, ldx, k, GZ, S0, S1, n, c->d_theta.as<double>(), y, ldy, part_dot,
+                      skip);
 }
 
 static void precond(globe_ctx* c, const double* r, int64_t ldr, int k, double* z, int64_t ldz,
                     double* part_rz, double* part_rr, const PcgState* st) {
     ensure_gdiag(c);
     ensure_theta(c, k);
-    launch_precond(c->stream, c->n, c->d_blk_off.as<int64_t>(), c->d_blk_ind.as<int32_t>(),
-                   c->d_times.as<double>(), c->d_gdiag.as<double>(), c->d_theta.as<double>(), r,
-                   ldr, k, z, ldz, part_rz, part_rr, st, 0, c->tiles);
+    launch_precond(c->stream, c->meta, c->d_gdiag.as<double>(), c->d_theta.as<double>(), r, ldr, k, z,
+                   ldz, part_rz, part_rr, st, 0);
 }
 
 // PCG with the reference's control flow (PCGsolver.cpp:1324-1444 for rule 0,
@@ -235,7 +232,7 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
     HIP_CHECK(hipEventRecord(e0, s));
     HIP_CHECK(hipMemcpyAsync(dst, hs, sizeof(PcgState), hipMemcpyHostToDevice, s));
     const int np = parts_for(nrec);
-    const int npb = precond_parts(c->n, c->tiles);
+    const int npb = precond_parts(c->meta);
     // starting point and residual
     if (o.x0_mode == GLOBE_X0_ZERO) {
         launch_fill_cols(s, nrec, X, ldx, k, 0.0);
@@ -326,21 +323,6 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
 using namespace globe;
 
 // ----------------------------------------------------------------------------- ABI
-#define API_BEGIN(ctx)                                 \
-    if (!(ctx)) return GLOBE_E_INVALID;                \
-    try {                                              \
-        HIP_CHECK(hipSetDevice((ctx)->device));
-#define API_END(ctx)                                   \
-    return GLOBE_OK;                                   \
-    }                                                  \
-    catch (const Error& e) {                           \
-        (ctx)->err = e.what();                         \
-        return e.code;                                 \
-    }                                                  \
-    catch (const std::exception& e) {                  \
-        (ctx)->err = e.what();                         \
-        return GLOBE_E_INVALID;                        \
-    }
 
 extern "C" {
 
@@ -375,6 +357,7 @@ void globe_destroy(globe_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    serve_release(c);
     for (auto& e : c->prof) {
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
@@ -448,17 +431,24 @@ int globe_set_cohort(globe_ctx* c, int64_t nrec, int32_t n, int32_t p, const int
             for (int64_t r = off[b]; r < off[b + 1]; ++r) rec_blk[r] = b;
         }
         tile_blk.push_back(n);
-        c->tiles = RecTiles{};
+        RecMeta& mt = c->meta;
+        mt = RecMeta{};
+        mt.indiv = c->d_indiv.as<int32_t>();
+        mt.times = c->d_times.as<double>();
+        mt.blk_off = c->d_blk_off.as<int64_t>();
+        mt.blk_ind = c->d_blk_ind.as<int32_t>();
+        mt.nrec = nrec;
+        mt.nblk = n;
         if (maxblk <= kTileMaxRecords) {
             c->d_tile_blk.ensure(sizeof(int32_t) * tile_blk.size());
             c->d_rec_blk.ensure(sizeof(int32_t) * std::max<int64_t>(nrec, 1));
             HIP_CHECK(hipMemcpy(c->d_tile_blk.p, tile_blk.data(), sizeof(int32_t) * tile_blk.size(),
                                 hipMemcpyHostToDevice));
             HIP_CHECK(hipMemcpy(c->d_rec_blk.p, rec_blk.data(), sizeof(int32_t) * nrec, hipMemcpyHostToDevice));
-            c->tiles.tile_blk = c->d_tile_blk.as<int32_t>();
-            c->tiles.rec_blk = c->d_rec_blk.as<int32_t>();
-            c->tiles.ntiles = (int32_t)tile_blk.size() - 1;
-            c->tiles.cap = kTileRecords;
+            mt.tile_blk = c->d_tile_blk.as<int32_t>();
+            mt.rec_blk = c->d_rec_blk.as<int32_t>();
+            mt.ntiles = (int32_t)tile_blk.size() - 1;
+            mt.cap = kTileRecords;
         }
     }
     c->has_cohort = true;

@@ -6,6 +6,8 @@
 
 #include "globe_internal.h"
 
+struct ServeState;  // serving loop (serve.cpp)
+
 struct globe_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -19,8 +21,8 @@ struct globe_ctx {
     std::vector<int64_t> h_blk_off;
     std::vector<double> h_times, h_X;
     globe::DevBuf d_indiv, d_times, d_X, d_blk_off, d_blk_ind;
-    globe::DevBuf d_tile_blk, d_rec_blk;  // record tiles (see RecTiles)
-    globe::RecTiles tiles;
+    globe::DevBuf d_tile_blk, d_rec_blk;  // record tiles
+    globe::RecMeta meta;                  // the cohort's record-space structure
 
     // genetics: 0 none, 1 packed genotypes, 2 GRM
     int gen = 0;
@@ -69,15 +71,41 @@ struct globe_ctx {
 
     // estimator workspaces
     globe::DevBuf e_xy, e_sol, e_f, e_probe, e_w, e_sv, e_pv, e_us, e_fs, e_coef, e_small;
+
+    // serving loop, when open
+    ServeState* serve = nullptr;
 };
 
+// Body of an ABI entry point: HIP errors and globe::Error become status codes with
+// the message kept in the context.
+#define API_BEGIN(ctx)                                 \
+    if (!(ctx)) return GLOBE_E_INVALID;                \
+    try {                                              \
+        HIP_CHECK(hipSetDevice((ctx)->device));
+#define API_END(ctx)                                   \
+    return GLOBE_OK;                                   \
+    }                                                  \
+    catch (const globe::Error& e) {                    \
+        (ctx)->err = e.what();                         \
+        return e.code;                                 \
+    }                                                  \
+    catch (const std::exception& e) {                  \
+        (ctx)->err = e.what();                         \
+        return GLOBE_E_INVALID;                        \
+    }
+
 namespace globe {
+
+void serve_release(globe_ctx* c);
 
 // device operator: y = V x (mode FULL) or sig2g G J^T x (GRM_ONLY); optional fused
 // partials of sum x.y per column; skip = device done flag (nullptr: always run).
 void op_vx(globe_ctx* c, const double* x, int64_t ldx, int k, double* y, int64_t ldy, int mode,
            int precision, const int* skip, double* part_dot);
 void ensure_theta(globe_ctx* c, int k);
+// GZ (n x k) = sig2g_c G S0 per column with the per-column components theta_cols [k][5]
+void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int precision,
+                     const int* skip, const double* theta_cols);
 void ensure_gdiag(globe_ctx* c);
 void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64_t ldx,
              const globe_pcg_opts& o, globe_pcg_stats* st, double* col_ratio, int32_t* col_iters);

@@ -55,31 +55,43 @@ struct PcgState {
 };
 
 // ---------------------------------------------------------------- record space
-// Record tiles: runs of whole individual blocks holding at most `cap` records each
-// (tile t = blocks tile_blk[t] .. tile_blk[t+1]-1); rec_blk[r] = block of record r.
-// ntiles == 0 selects the thread-per-block kernels (an individual with more than
-// kTileMaxRecords records).
-struct RecTiles {
-    const int32_t* tile_blk = nullptr;
+// Record-space structure seen by the record kernels. The records of an individual
+// are contiguous (a block); column c of a block PCG may carry its own structure (the
+// serving slots) at column strides rec_ld / blk_ld / tile_ld, or all columns share one
+// (strides 0, the cohort). Tiles are runs of whole blocks of at most `cap` records
+// (tile t = blocks tile_blk[t] .. tile_blk[t+1]-1; rec_blk[r] = block of record r),
+// staged through LDS by the tiled kernels; cap == 0 selects the thread-per-block
+// kernels (shared structure only: an individual with more than kTileMaxRecords records).
+struct RecMeta {
+    const int32_t* indiv = nullptr;    // individual of each record
+    const double* times = nullptr;
     const int32_t* rec_blk = nullptr;
-    int32_t ntiles = 0;
-    int32_t cap = 0;
+    int64_t rec_ld = 0;
+    const int64_t* blk_off = nullptr;  // nblk + 1 record offsets
+    const int32_t* blk_ind = nullptr;  // individual of each block
+    int64_t blk_ld = 0;
+    const int32_t* tile_blk = nullptr;
+    int64_t tile_ld = 0;
+    const int32_t* ntiles_col = nullptr;  // per-column tile counts (device) or null
+    int32_t ntiles = 0;                   // tiles (max over columns when ntiles_col is set)
+    const int64_t* nrec_col = nullptr;    // per-column record counts (device) or null
+    int64_t nrec = 0;                     // records (row count of the record matrices)
+    int32_t nblk = 0;                     // blocks (shared structure)
+    int32_t cap = 0;                      // records per tile; 0: thread-per-block kernels
 };
 constexpr int kTileRecords = 2048;
 constexpr int kTileMaxRecords = 2048;  // LDS: 4 doubles per record slot (64 KB)
 // partial-sum slots written by launch_precond
-int precond_parts(int32_t nblk, const RecTiles& tiles);
-void launch_seg_sums(hipStream_t s, int32_t nblk, const int64_t* blk_off, const int32_t* blk_ind,
-                     const double* times, const double* x, int64_t ldx, int k, double* S0,
-                     double* S1, int64_t ldn, const int* skip, const RecTiles& tiles);
-void launch_record_out(hipStream_t s, int64_t nrec, const int32_t* indiv, const double* times,
-                       const double* x, int64_t ldx, int k, const double* GZ, const double* S0,
-                       const double* S1, int64_t ldn, const double* theta_cols, double* y,
-                       int64_t ldy, double* part_dot, const int* skip);
-void launch_precond(hipStream_t s, int32_t nblk, const int64_t* blk_off, const int32_t* blk_ind,
-                    const double* times, const double* gdiag, const double* theta_cols,
+int precond_parts(const RecMeta& m);
+void launch_seg_sums(hipStream_t s, const RecMeta& m, const double* x, int64_t ldx, int k,
+                     double* S0, double* S1, int64_t ldn, const int* skip);
+void launch_record_out(hipStream_t s, const RecMeta& m, const double* x, int64_t ldx, int k,
+                       const double* GZ, const double* S0, const double* S1, int64_t ldn,
+                       const double* theta_cols, double* y, int64_t ldy, double* part_dot,
+                       const int* skip);
+void launch_precond(hipStream_t s, const RecMeta& m, const double* gdiag, const double* theta_cols,
                     const double* r, int64_t ldr, int k, double* z, int64_t ldz, double* part_rz,
-                    double* part_rr, const PcgState* st, int tilde_v, const RecTiles& tiles);
+                    double* part_rr, const PcgState* st, int tilde_v);
 void launch_dot_parts(hipStream_t s, int64_t nrec, const double* a, int64_t lda, const double* b,
                       int64_t ldb, int k, double* part);
 void launch_reduce_parts(hipStream_t s, const double* part, int nparts, int k, double* out);
@@ -89,6 +101,8 @@ void launch_axpy_cols(hipStream_t s, int64_t nrec, double* y, int64_t ldy, const
                       int64_t ldx, int kx, const double* coef, int ky);
 void launch_pcg_init(hipStream_t s, PcgState* st, const double* part_bb, const double* part_rr,
                      const double* part_rz, int nparts, int k);
+void launch_pcg_admit(hipStream_t s, PcgState* st, int j, const double* part_bb, int nparts_bb,
+                      const double* part_rr, const double* part_rz, int nparts);
 void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* part_pvp, int nparts, int k);
 void launch_pcg_xr(hipStream_t s, PcgState* st, int64_t nrec, double* x, int64_t ldx, double* r,
                    const double* p, const double* vp, int64_t ld, int k, double* part_rr);

@@ -66,8 +66,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_sums(int32_t nblk, const int64_t
 // (the factorized form of PCGsolver.cpp:1220-1265). Optional fused partial of
 // sum_r x[c][r] y[c][r] (the p . Vp of PCG, PCGsolver.cpp:1381).
 __global__ __launch_bounds__(kBlock) void k_record_out(
-    int64_t nrec, const int32_t* __restrict__ indiv, const double* __restrict__ t,
-    const double* __restrict__ x, int64_t ldx, const double* __restrict__ GZ,
+    RecMeta m, const double* __restrict__ x, int64_t ldx, const double* __restrict__ GZ,
     const double* __restrict__ S0, const double* __restrict__ S1, int64_t ldn,
     const double* __restrict__ th, double* __restrict__ y, int64_t ldy, double* part_dot,
     const int* skip) {
@@ -75,19 +74,25 @@ __global__ __launch_bounds__(kBlock) void k_record_out(
     const int c = blockIdx.y;
     const double se = th[c * 5 + 0], sb0 = th[c * 5 + 2], sb01 = th[c * 5 + 3],
                  sb1 = th[c * 5 + 4];
+    const int32_t* indiv = m.indiv + c * m.rec_ld;
+    const double* t = m.times + c * m.rec_ld;
+    const int64_t nvalid = m.nrec_col ? m.nrec_col[c] : m.nrec;
     const double* xc = x + c * ldx;
     double* yc = y + c * ldy;
     const double* g = GZ + c * ldn;
     const double* s0 = S0 + c * ldn;
     const double* s1 = S1 + c * ldn;
     double acc = 0.0;
-    for (int64_t r = blockIdx.x * (int64_t)kBlock + threadIdx.x; r < nrec;
+    for (int64_t r = blockIdx.x * (int64_t)kBlock + threadIdx.x; r < m.nrec;
          r += (int64_t)gridDim.x * kBlock) {
-        const int32_t i = indiv[r];
-        const double tr = t[r], a = s0[i], b = s1[i], xv = xc[r];
-        const double v = g[i] + sb0 * a + sb01 * (tr * a + b) + sb1 * tr * b + se * xv;
+        double v = 0.0;  // records past a column's own count are padding
+        if (r < nvalid) {
+            const int32_t i = indiv[r];
+            const double tr = t[r], a = s0[i], b = s1[i], xv = xc[r];
+            v = g[i] + sb0 * a + sb01 * (tr * a + b) + sb1 * tr * b + se * xv;
+            acc += xv * v;
+        }
         yc[r] = v;
-        acc += xv * v;
     }
     if (part_dot) {
         double s = block_sum(acc);
@@ -99,17 +104,21 @@ __global__ __launch_bounds__(kBlock) void k_record_out(
 // of whole individual blocks in LDS with coalesced loads, one thread per block sums its
 // records from LDS in record order (so the block sums equal the thread-per-block
 // kernels' bit for bit), and the preconditioner writes z back one thread per record.
-__global__ __launch_bounds__(kBlock) void k_seg_sums_tiled(
-    int32_t ntiles, const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ off,
-    const int32_t* __restrict__ ind, const double* __restrict__ t, const double* __restrict__ x,
-    int64_t ldx, double* __restrict__ S0, double* __restrict__ S1, int64_t ldn, int cap,
-    const int* skip) {
+__global__ __launch_bounds__(kBlock) void k_seg_sums_tiled(RecMeta m, const double* __restrict__ x,
+                                                           int64_t ldx, double* __restrict__ S0,
+                                                           double* __restrict__ S1, int64_t ldn,
+                                                           const int* skip) {
     if (skip && *skip) return;
     extern __shared__ double sm[];
     double* sx = sm;
-    double* st = sm + cap;
+    double* st = sm + m.cap;
     const int c = blockIdx.y;
     const double* xc = x + c * ldx;
+    const double* t = m.times + c * m.rec_ld;
+    const int64_t* off = m.blk_off + c * m.blk_ld;
+    const int32_t* ind = m.blk_ind + c * m.blk_ld;
+    const int32_t* tile_blk = m.tile_blk + c * m.tile_ld;
+    const int32_t ntiles = m.ntiles_col ? m.ntiles_col[c] : m.ntiles;
     for (int32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int32_t b0 = tile_blk[tile], b1 = tile_blk[tile + 1];
         const int64_t r0 = off[b0];
@@ -136,11 +145,9 @@ __global__ __launch_bounds__(kBlock) void k_seg_sums_tiled(
 }
 
 __global__ __launch_bounds__(kBlock) void k_precond_tiled(
-    int32_t ntiles, const int32_t* __restrict__ tile_blk, const int32_t* __restrict__ rec_blk,
-    const int64_t* __restrict__ off, const int32_t* __restrict__ ind, const double* __restrict__ t,
-    const double* __restrict__ gdiag, const double* __restrict__ th, const double* __restrict__ r,
-    int64_t ldr, double* __restrict__ z, int64_t ldz, double* part_rz, double* part_rr,
-    const PcgState* stt, int tilde_v, int cap) {
+    RecMeta m, const double* __restrict__ gdiag, const double* __restrict__ th,
+    const double* __restrict__ r, int64_t ldr, double* __restrict__ z, int64_t ldz,
+    double* part_rz, double* part_rr, const PcgState* stt, int tilde_v) {
     if (stt && stt->done) return;
     const int c = blockIdx.y;
     if (stt && !stt->active[c]) {
@@ -151,14 +158,20 @@ __global__ __launch_bounds__(kBlock) void k_precond_tiled(
         return;
     }
     extern __shared__ double sm[];
-    double* sr = sm;             // r of the tile's records
-    double* st = sm + cap;       // t of the tile's records
-    double* sc0 = sm + 2 * cap;  // per block: Woodbury coefficients
-    double* sc1 = sm + 3 * cap;
+    double* sr = sm;               // r of the tile's records
+    double* st = sm + m.cap;       // t of the tile's records
+    double* sc0 = sm + 2 * m.cap;  // per block: Woodbury coefficients
+    double* sc1 = sm + 3 * m.cap;
     const double se = th[c * 5 + 0], sg = th[c * 5 + 1], sb0 = th[c * 5 + 2],
                  sb01 = th[c * 5 + 3], sb1 = th[c * 5 + 4];
     const double* rc = r + c * ldr;
     double* zc = z + c * ldz;
+    const double* t = m.times + c * m.rec_ld;
+    const int32_t* rec_blk = m.rec_blk + c * m.rec_ld;
+    const int64_t* off = m.blk_off + c * m.blk_ld;
+    const int32_t* ind = m.blk_ind + c * m.blk_ld;
+    const int32_t* tile_blk = m.tile_blk + c * m.tile_ld;
+    const int32_t ntiles = m.ntiles_col ? m.ntiles_col[c] : m.ntiles;
     const double inv_se = 1.0 / se;
     double arz = 0.0, arr = 0.0;
     for (int32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -398,6 +411,36 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(PcgState* st, const double*
     }
 }
 
+// Start (or restart) column j of a running block PCG from the partials of its
+// first residual (column 0 of each partial array): |b|, |r|, r.z; the column becomes
+// active unless it is already converged, and a block whose columns had all finished
+// runs again. Used by the serving loop when a request joins between iterations.
+__global__ __launch_bounds__(64) void k_pcg_admit(PcgState* st, int j, const double* __restrict__ part_bb,
+                                                  int nparts_bb, const double* __restrict__ part_rr,
+                                                  const double* __restrict__ part_rz, int nparts) {
+    const int lane = threadIdx.x;
+    double bb = 0.0, rr = 0.0, rz = 0.0;
+    for (int q = lane; q < nparts_bb; q += 64) bb += part_bb[q];
+    for (int q = lane; q < nparts; q += 64) {
+        rr += part_rr[q];
+        rz += part_rz[q];
+    }
+    bb = wave_sum(bb);
+    rr = wave_sum(rr);
+    rz = wave_sum(rz);
+    if (lane != 0) return;
+    const double nb = sqrt(bb), nr = sqrt(rr);
+    st->bnorm[j] = nb;
+    st->rs[j] = rz;
+    st->alpha[j] = 0.0;
+    st->beta[j] = 0.0;
+    st->col_iters[j] = 0;
+    st->ratio[j] = nb > 0.0 ? nr / nb : 0.0;
+    const int act = (nb > 1e-12 && !(nr < st->tol * nb)) ? 1 : 0;
+    st->active[j] = act;
+    if (act) st->done = 0;
+}
+
 __global__ __launch_bounds__(kBlock) void k_pcg_alpha(PcgState* st, const double* __restrict__ part_pvp,
                                                       int nparts, int k) {
     if (st->done) return;
@@ -624,47 +667,43 @@ __global__ __launch_bounds__(kBlock) void k_fill(double* dst, int64_t count, dou
 
 #define LAUNCH_CHECK() HIP_CHECK(hipGetLastError())
 
-int precond_parts(int32_t nblk, const RecTiles& tiles) {
-    return tiles.ntiles > 0 ? std::min<int>(tiles.ntiles, kMaxParts) : parts_for(nblk);
+int precond_parts(const RecMeta& m) {
+    return m.cap > 0 ? std::max(1, std::min<int>(m.ntiles, kMaxParts)) : parts_for(m.nblk);
 }
 
-void launch_seg_sums(hipStream_t s, int32_t nblk, const int64_t* blk_off, const int32_t* blk_ind,
-                     const double* times, const double* x, int64_t ldx, int k, double* S0,
-                     double* S1, int64_t ldn, const int* skip, const RecTiles& tiles) {
-    if (tiles.ntiles > 0) {
-        dim3 g(std::min<int>(tiles.ntiles, kMaxParts), k);
-        k_seg_sums_tiled<<<g, kBlock, sizeof(double) * 2 * tiles.cap, s>>>(
-            tiles.ntiles, tiles.tile_blk, blk_off, blk_ind, times, x, ldx, S0, S1, ldn, tiles.cap, skip);
+void launch_seg_sums(hipStream_t s, const RecMeta& m, const double* x, int64_t ldx, int k,
+                     double* S0, double* S1, int64_t ldn, const int* skip) {
+    if (m.cap > 0) {
+        dim3 g(precond_parts(m), k);
+        k_seg_sums_tiled<<<g, kBlock, sizeof(double) * 2 * m.cap, s>>>(m, x, ldx, S0, S1, ldn, skip);
     } else {
-        dim3 g(parts_for(nblk), k);
-        k_seg_sums<<<g, kBlock, 0, s>>>(nblk, blk_off, blk_ind, times, x, ldx, S0, S1, ldn, skip);
+        dim3 g(parts_for(m.nblk), k);
+        k_seg_sums<<<g, kBlock, 0, s>>>(m.nblk, m.blk_off, m.blk_ind, m.times, x, ldx, S0, S1, ldn,
+                                         skip);
     }
     LAUNCH_CHECK();
 }
 
-void launch_record_out(hipStream_t s, int64_t nrec, const int32_t* indiv, const double* times,
-                       const double* x, int64_t ldx, int k, const double* GZ, const double* S0,
-                       const double* S1, int64_t ldn, const double* theta_cols, double* y,
-                       int64_t ldy, double* part_dot, const int* skip) {
-    dim3 g(parts_for(nrec), k);
-    k_record_out<<<g, kBlock, 0, s>>>(nrec, indiv, times, x, ldx, GZ, S0, S1, ldn, theta_cols, y,
-                                      ldy, part_dot, skip);
+void launch_record_out(hipStream_t s, const RecMeta& m, const double* x, int64_t ldx, int k,
+                       const double* GZ, const double* S0, const double* S1, int64_t ldn,
+                       const double* theta_cols, double* y, int64_t ldy, double* part_dot,
+                       const int* skip) {
+    dim3 g(parts_for(m.nrec), k);
+    k_record_out<<<g, kBlock, 0, s>>>(m, x, ldx, GZ, S0, S1, ldn, theta_cols, y, ldy, part_dot, skip);
     LAUNCH_CHECK();
 }
 
-void launch_precond(hipStream_t s, int32_t nblk, const int64_t* blk_off, const int32_t* blk_ind,
-                    const double* times, const double* gdiag, const double* theta_cols,
+void launch_precond(hipStream_t s, const RecMeta& m, const double* gdiag, const double* theta_cols,
                     const double* r, int64_t ldr, int k, double* z, int64_t ldz, double* part_rz,
-                    double* part_rr, const PcgState* st, int tilde_v, const RecTiles& tiles) {
-    if (tiles.ntiles > 0) {
-        dim3 g(precond_parts(nblk, tiles), k);
-        k_precond_tiled<<<g, kBlock, sizeof(double) * 4 * tiles.cap, s>>>(
-            tiles.ntiles, tiles.tile_blk, tiles.rec_blk, blk_off, blk_ind, times, gdiag, theta_cols, r,
-            ldr, z, ldz, part_rz, part_rr, st, tilde_v, tiles.cap);
+                    double* part_rr, const PcgState* st, int tilde_v) {
+    if (m.cap > 0) {
+        dim3 g(precond_parts(m), k);
+        k_precond_tiled<<<g, kBlock, sizeof(double) * 4 * m.cap, s>>>(m, gdiag, theta_cols, r, ldr, z,
+                                                                       ldz, part_rz, part_rr, st, tilde_v);
     } else {
-        dim3 g(parts_for(nblk), k);
-        k_precond<<<g, kBlock, 0, s>>>(nblk, blk_off, blk_ind, times, gdiag, theta_cols, r, ldr, z,
-                                       ldz, part_rz, part_rr, st, tilde_v);
+        dim3 g(parts_for(m.nblk), k);
+        k_precond<<<g, kBlock, 0, s>>>(m.nblk, m.blk_off, m.blk_ind, m.times, gdiag, theta_cols, r,
+                                       ldr, z, ldz, part_rz, part_rr, st, tilde_v);
     }
     LAUNCH_CHECK();
 }
@@ -701,6 +740,12 @@ void launch_axpy_cols(hipStream_t s, int64_t nrec, double* y, int64_t ldy, const
                       int64_t ldx, int kx, const double* coef, int ky) {
     dim3 g(parts_for(nrec), ky);
     k_axpy_cols<<<g, kBlock, 0, s>>>(nrec, y, ldy, x, ldx, kx, coef, ky);
+    LAUNCH_CHECK();
+}
+
+void launch_pcg_admit(hipStream_t s, PcgState* st, int j, const double* part_bb, int nparts_bb,
+                      const double* part_rr, const double* part_rz, int nparts) {
+    k_pcg_admit<<<1, 64, 0, s>>>(st, j, part_bb, nparts_bb, part_rr, part_rz, nparts);
     LAUNCH_CHECK();
 }
 

@@ -1,0 +1,88 @@
+"""Serving-loop driver: synthetic solve requests against a resident cohort, replayed
+with Poisson arrivals in real time through globe_serve_* (continuous batching).
+
+A request is a trait on an individual subset of the resident cohort: its own
+variance components, its own longitudinal records (1 + Poisson(lambda) per
+individual, times U(0, 10)) and its own right-hand side. The driver submits each
+request at its arrival time, steps the loop (one admission + block-PCG iteration
+per step) and collects results; latency = completion - arrival on the host clock.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+
+def draw_theta(rng):
+    """sig2g U(0.1, 0.6), sig2e U(0.3, 0.8), sig2b0 U(0.1, 0.5), sig2b1 U(0.05, 0.3),
+    correlation of the random intercept and slope U(-0.5, 0.5)."""
+    sg = rng.uniform(0.1, 0.6)
+    se = rng.uniform(0.3, 0.8)
+    sb0 = rng.uniform(0.1, 0.5)
+    sb1 = rng.uniform(0.05, 0.3)
+    rho = rng.uniform(-0.5, 0.5)
+    return (se, sg, sb0, rho * np.sqrt(sb0 * sb1), sb1)
+
+
+def make_request(rng, n_cohort, size_lo, size_hi, lam_lo, lam_hi, u_pool=None):
+    """One request: (indiv, times, y, theta). Subset size log-uniform in [size_lo,
+    size_hi]; records 1 + Poisson(lambda), lambda ~ U(lam_lo, lam_hi); y = u_i + b0_i +
+    b1_i t + e with the request's components (u from a pool of genetic values scaled
+    to the request's sig2g, or absent)."""
+    size = int(np.exp(rng.uniform(np.log(size_lo), np.log(size_hi))))
+    size = max(1, min(size, n_cohort))
+    ids = np.sort(rng.choice(n_cohort, size=size, replace=False)).astype(np.int32)
+    lam = rng.uniform(lam_lo, lam_hi)
+    counts = 1 + rng.poisson(lam, size=size)
+    indiv = np.repeat(ids, counts)
+    nrec = indiv.shape[0]
+    times = rng.uniform(0.0, 10.0, size=nrec).astype(np.float32).astype(np.float64)
+    theta = draw_theta(rng)
+    se, sg, sb0, sb01, sb1 = theta
+    L = np.linalg.cholesky(np.array([[sb0, sb01], [sb01, sb1]]))
+    b = rng.standard_normal((size, 2)) @ L.T
+    bi = np.repeat(b, counts, axis=0)
+    y = bi[:, 0] + bi[:, 1] * times + np.sqrt(se) * rng.standard_normal(nrec)
+    if u_pool is not None:
+        u = u_pool[rng.integers(len(u_pool))]
+        y += np.sqrt(sg) * u[indiv]
+    return indiv, times, y, theta
+
+
+def replay(g, requests, rate, seed=0):
+    """Submit requests at Poisson arrival times (rate per second) and run the loop until
+    all finish. Returns (solutions, stats) with per-request latency and loop counters."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(requests)))
+    arrivals -= arrivals[0]
+    sols = [None] * len(requests)
+    lat = np.zeros(len(requests))
+    iters = np.zeros(len(requests), dtype=np.int64)
+    idx = {}
+    nxt = 0
+    steps = 0
+    occupancy = 0
+    t0 = time.perf_counter()
+    while nxt < len(requests) or idx:
+        now = time.perf_counter() - t0
+        while nxt < len(requests) and arrivals[nxt] <= now:
+            indiv, times, y, theta = requests[nxt]
+            idx[g.serve_submit(indiv, times, y, theta)] = nxt
+            nxt += 1
+        _, active, queued = g.serve_step()
+        steps += 1
+        occupancy += active
+        for rid in g.serve_finished():
+            x, st = g.serve_take(rid)
+            q = idx.pop(rid)
+            sols[q] = x
+            lat[q] = (time.perf_counter() - t0) - arrivals[q]
+            iters[q] = st["iters"]
+        if not idx and nxt < len(requests):
+            wait = arrivals[nxt] - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+    wall = time.perf_counter() - t0
+    return sols, {"latency_s": lat, "iters": iters, "wall_s": wall, "steps": steps,
+                  "mean_active": occupancy / max(steps, 1)}

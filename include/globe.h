@@ -181,6 +181,56 @@ int globe_aireml_fit(globe_ctx* ctx, const double* y, const globe_aireml_opts* o
                      double theta[5], double se[5], double ai[25], int32_t* iters,
                      double* sec_per_iter, globe_log_fn log, void* user);
 
+/* ---- serving loop (no reference counterpart: the reference solves one trait per run) -- */
+/* One resident cohort (globe_set_cohort for N, genotypes or GRM loaded) serves solve
+ * requests V_j x_j = y_j, each with its own individuals, records, times and variance
+ * components. Requests are admitted into the columns ("slots") of one shared block PCG
+ * as slots and HBM free up -- continuous batching: a request joins between two
+ * iterations of the running block and leaves when its own column converges. Each
+ * column's arithmetic (per-column fixed-point scale of the contraction, its own PCG
+ * scalars and record structure) is independent of the other columns, so a result is
+ * the same alone and in the mix. While the loop is open the context runs nothing else. */
+typedef struct {
+    int64_t nrec;         /* records of the request */
+    const int32_t* indiv; /* 0-based cohort individual per record, each individual's records contiguous */
+    const double* times;  /* nrec */
+    const double* y;      /* nrec: the right-hand side (a trait) */
+    double theta[5];      /* sig2e, sig2g, sig2b0, sigb01, sig2b1 of this request */
+} globe_request;
+
+typedef struct {
+    int32_t slots;        /* columns of the shared block PCG (1..64) */
+    int64_t max_records;  /* record capacity of a slot */
+    int64_t hbm_budget;   /* bytes of request vectors resident at once (0: slots x capacity) */
+    double tol;           /* per-request |r| < tol |b| */
+    int32_t max_iter;     /* per request */
+    int32_t precision;    /* GLOBE_PREC_* of the genotype contraction */
+} globe_serve_opts;
+
+typedef struct {
+    int64_t id;
+    int32_t iters;
+    int32_t converged;
+    double ratio;         /* |r| / |b| at exit */
+    double t_submit;      /* seconds on the loop's clock (from globe_serve_open) */
+    double t_admit;
+    double t_done;
+} globe_serve_result;
+
+int globe_serve_open(globe_ctx* ctx, const globe_serve_opts* opts);
+/* Copies and queues a request; *id identifies it in globe_serve_take. */
+int globe_serve_submit(globe_ctx* ctx, const globe_request* req, int64_t* id);
+/* Admits queued requests (FIFO) into free slots while the HBM budget allows, runs one
+ * iteration of the block PCG over the occupied slots and retires the requests whose
+ * column converged (or reached max_iter). Counts after the step may be NULL. */
+int globe_serve_step(globe_ctx* ctx, int32_t* finished, int32_t* active, int32_t* queued);
+/* Ids of finished requests not yet taken (up to max, ascending); *count gets how many. */
+int globe_serve_finished(globe_ctx* ctx, int64_t* ids, int32_t max, int32_t* count);
+/* Hands over a finished request: x (its nrec values) and its result; GLOBE_E_STATE if it
+ * has not finished. Each finished request can be taken once. */
+int globe_serve_take(globe_ctx* ctx, int64_t id, double* x, globe_serve_result* res);
+int globe_serve_close(globe_ctx* ctx);
+
 /* ---- kernel timing (no reference counterpart) --------------------------------------- */
 /* When enabled, every packed-genotype contraction launch is bracketed by HIP events on
  * the context stream (launches that a finished PCG skips are discarded). which: 0 = pass 1

@@ -1,0 +1,122 @@
+"""Serving loop (continuous batching) on the GPU: requests with their own individual
+subsets, records and variance components join a running block PCG between
+iterations. Each result must equal the same request served alone (the per-column
+arithmetic does not see the other columns) and match the FP64 oracle of that
+request's own system (its records, the subset's genotypes, the cohort's allele
+frequencies)."""
+import numpy as np
+import pytest
+
+import globe_amd as G
+from globe_amd import serving, synth
+from helpers import rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def cohort():
+    return synth.make_geno_case(n=400, m=900, lam=3, seed=11, miss=0.01)
+
+
+def _requests(n, count, seed):
+    rng = np.random.default_rng(seed)
+    return [serving.make_request(rng, n, 20, n, 1.0, 5.0) for _ in range(count)]
+
+
+def _device(d, ctx_factory):
+    g = ctx_factory()
+    c = d.cohort
+    g.set_cohort(c.indiv, c.times, c.X)
+    g.prepGenotypeInput(d.bed_rows, 0.01, 0.2)
+    g.set_varcomp(d.theta)
+    return g
+
+
+def _serve(g, reqs, *, slots, precision, tol, stagger=True, cap=None):
+    # the slot capacity fixes the row partition of the reductions: alone and in the mix
+    # use the same one, so the two runs do identical arithmetic
+    cap = cap or max(r[0].shape[0] for r in reqs)
+    g.serve_open(slots=slots, max_records=cap, tol=tol, max_iter=500, precision=precision)
+    ids = {}
+    pending = list(range(len(reqs)))
+    first = pending[: len(reqs) // 2] if stagger else pending
+    for q in first:
+        ids[g.serve_submit(*reqs[q])] = q
+    rest = pending[len(first):]
+    out = [None] * len(reqs)
+    info = [None] * len(reqs)
+    steps = 0
+    while ids or rest:
+        steps += 1
+        if rest and steps % 3 == 0:  # later arrivals join a running block
+            q = rest.pop(0)
+            ids[g.serve_submit(*reqs[q])] = q
+        g.serve_step()
+        for rid in g.serve_finished():
+            x, st = g.serve_take(rid)
+            q = ids.pop(rid)
+            out[q], info[q] = x, st
+        assert steps < 5000
+    g.serve_close()
+    return out, info
+
+
+def _oracle_for(d, req):
+    import oracle as O
+    indiv, times, y, theta = req
+    ids, sub = np.unique(indiv, return_inverse=True)
+    codes = synth.unpack_codes(d.bed_rows[d.keep], d.cohort.n)[:, ids]
+    dos = np.select([codes == 0, codes == 2, codes == 3], [2, 1, 0], default=-1).astype(np.int8)
+    rows = synth.pack_bed(dos)
+    X = np.column_stack([np.ones(indiv.shape[0]), times])
+    return O.Oracle(sub.astype(np.int32), times, X, theta, geno_rows=rows, freq=d.freq, n=ids.shape[0])
+
+
+def test_serving_mix_equals_alone_and_oracle(cohort, gpu_ctx_factory):
+    _say("serve-fp64")
+    d = cohort
+    reqs = _requests(d.cohort.n, 10, seed=3)
+    g = _device(d, gpu_ctx_factory)
+    cap = max(r[0].shape[0] for r in reqs)
+    mix, info = _serve(g, reqs, slots=4, precision=G.PREC_FP64, tol=1e-10, cap=cap)
+    assert all(i["converged"] for i in info)
+    for q in (0, 4, 9):
+        alone, _ = _serve(g, [reqs[q]], slots=4, precision=G.PREC_FP64, tol=1e-10, stagger=False, cap=cap)
+        assert rel(mix[q], alone[0]) <= 1e-14, q
+        orc = _oracle_for(d, reqs[q])
+        xo, _ = orc.pcg(np.asarray(reqs[q][2])[:, None], tol=1e-10, rule=2, stagnation=False)
+        assert rel(mix[q], xo[:, 0]) < 1e-7, q
+
+
+def test_serving_mixed_precision_within_1e6(cohort, gpu_ctx_factory):
+    _say("serve-mixed")
+    d = cohort
+    reqs = _requests(d.cohort.n, 8, seed=5)
+    g = _device(d, gpu_ctx_factory)
+    x64, _ = _serve(g, reqs, slots=8, precision=G.PREC_FP64, tol=1e-10)
+    xm, info = _serve(g, reqs, slots=3, precision=G.PREC_MIXED, tol=1e-10)
+    assert all(i["converged"] for i in info)
+    for q in range(len(reqs)):
+        assert rel(xm[q], x64[q]) < 1e-6, q
+
+
+def test_serving_rejects_bad_requests(cohort, gpu_ctx_factory):
+    _say("serve-errors")
+    d = cohort
+    g = _device(d, gpu_ctx_factory)
+    g.serve_open(slots=2, max_records=50, tol=1e-8)
+    th = (0.5, 0.4, 0.3, 0.0, 0.2)
+    with pytest.raises(G.GlobeError):  # more records than a slot holds
+        g.serve_submit(np.zeros(100, np.int32), np.zeros(100), np.ones(100), th)
+    with pytest.raises(G.GlobeError):  # individual outside the cohort
+        g.serve_submit(np.array([0, 1, 10_000], np.int32), np.zeros(3), np.ones(3), th)
+    with pytest.raises(G.GlobeError):  # records of individual 0 not contiguous
+        g.serve_submit(np.array([0, 1, 0], np.int32), np.zeros(3), np.ones(3), th)
+    with pytest.raises(G.GlobeError):  # unknown / unfinished request
+        g._check(g.L.globe_serve_take(g.h, 12345, None, None))
+    g.serve_close()
