@@ -124,8 +124,9 @@ def serving_leg(g, c, kept, args):
         return r
 
     out = run(reqs, args.serve_rate)
-    # the same request mix offered ten times faster than the loop can clear it
-    out["saturated"] = run(reqs[: max(1, len(reqs) // 2)], 10.0 * args.serve_rate * 10)
+    # half of the same requests offered 50x faster (a burst the loop cannot clear as it
+    # arrives): throughput and latency with all slots busy
+    out["saturated"] = run(reqs[: max(1, len(reqs) // 2)], 50.0 * args.serve_rate)
     out.update({"slots": args.serve_slots, "precision": "mixed", "tol": args.tol,
                 "data": "synthetic requests (subset, records, components, trait) on the resident cohort"})
     return out
