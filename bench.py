@@ -132,6 +132,37 @@ def serving_leg(g, c, kept, args):
     return out
 
 
+def aireml_leg(args):
+    """AIREML on a dense GRM (N = 40,000, GRM from 80,000 synthetic SNPs built on the
+    device, ~320k records, y + 15 Rademacher probes as one 16-column block PCG, start
+    0.5 / 0.5 / 0.3 / 0 / 0.3): seconds per AIREML iteration over the first iterations."""
+    import globe_amd as G
+    from globe_amd import synth
+    g = G.Globe(0)
+    rng = np.random.default_rng(args.seed + 11)
+    c = synth.simulate_cohort(args.aireml_n, 7.0, rng)
+    g.set_cohort(c.indiv, c.times, c.X)
+    kept = g.synth_bed(args.aireml_m, seed=args.seed + 11, f_lo=0.05, f_hi=0.5, miss=0.0,
+                       min_maf=0.0, max_miss=1.0)
+    theta = (0.6, 0.4, 0.4, 0.2, 0.2)
+    g.set_varcomp(theta)
+    u = np.sqrt(theta[1] / kept) * g.geno_Wa(rng.standard_normal(kept))
+    y = synth.simulate_phenotype(c, u, theta, rng)
+    t0 = time.perf_counter()
+    g.grm_from_geno()
+    t_grm = time.perf_counter() - t0
+    log(f"aireml: GRM {c.n} x {c.n} from {kept} SNPs built in {t_grm:.2f}s")
+    fit = g.aireml_fit(y, (0.5, 0.5, 0.3, 0.0, 0.3), max_iter=args.aireml_iters, tol=1e-3, num_mc=15,
+                       probes="rademacher", seed=args.seed, pcg_tol=1e-5)
+    log(f"aireml: {fit['iters']} iterations, {fit['sec_per_iter']:.3f} s/iter, theta {np.round(fit['theta'], 4)}")
+    out = {"n": c.n, "m_snps": int(kept), "nrec": c.nrec, "probes": 15, "grm_build_s": t_grm,
+           "iters": fit["iters"], "sec_per_iter": fit["sec_per_iter"], "theta": list(map(float, fit["theta"])),
+           "precision": "fp64 (FP32 GRM, FP64 accumulation)",
+           "note": f"first {args.aireml_iters} iterations of the fit (maxiter 50, tol 1e-3 in the full run)"}
+    g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +180,9 @@ def main():
                     help="requests replayed through the serving loop (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-slots", type=int, default=32)
+    ap.add_argument("--aireml-iters", type=int, default=3, help="AIREML iterations timed (0: skip)")
+    ap.add_argument("--aireml-n", type=int, default=40_000)
+    ap.add_argument("--aireml-m", type=int, default=80_000)
     ap.add_argument("--calib", action="store_true",
                     help="first run 3 plain 1 GiB device copies (counter calibration)")
     args = ap.parse_args()
@@ -222,6 +256,7 @@ def main():
     serve = None
     if args.serve_requests > 0:
         serve = serving_leg(g, c, kept, args)
+    aireml = aireml_leg(args) if args.aireml_iters > 0 else None
     traffic, traffic_src = measured_traffic("k_geno_wide")
     if traffic_src:
         traffic_src = f"profiles/k_geno_mfma_n200k_m500k.md ({traffic_src})"
@@ -261,6 +296,7 @@ def main():
                      "issued_int8_TOPs": issued_tops, "issued_frac": issued_tops / PEAK_INT8_TOPS},
         "cpu_baseline": cpu,
         "serving": serve,
+        "aireml": aireml,
     }
     print(json.dumps(out))
     g.close()

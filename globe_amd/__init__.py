@@ -125,6 +125,8 @@ def load_library(path: str = LIB_PATH):
         "globe_profile_read": ([P, C.c_int, P], C.c_int),
         "globe_aireml_step": ([P, P, P, dbl, P], C.c_int),
         "globe_aireml_fit": ([P, P, P, P, P, P, P, P, P, P], C.c_int),
+        "globe_grm_from_geno": ([P], C.c_int),
+        "globe_get_grm": ([P, P], C.c_int),
         "globe_serve_open": ([P, P], C.c_int),
         "globe_serve_submit": ([P, P, P], C.c_int),
         "globe_serve_step": ([P, P, P, P], C.c_int),
@@ -385,6 +387,15 @@ class Globe:
                 "ms_avg": st.ms_total / st.launches if st.launches else 0.0,
                 "bytes": st.bytes_per_launch, "flops": st.flops_per_launch,
                 "issued": st.issued_ops_per_launch}
+
+    def grm_from_geno(self):
+        """Build G = W W^T / M from the resident genotypes; the context then uses the GRM."""
+        self._check(self.L.globe_grm_from_geno(self.h))
+
+    def grm_lt(self):
+        lt = np.zeros(self.n * (self.n + 1) // 2, dtype=np.float32)
+        self._check(self.L.globe_get_grm(self.h, _p(lt)))
+        return lt
 
     # ---- serving loop
     def serve_open(self, *, slots=32, max_records, hbm_budget=0, tol=1e-6, max_iter=1000,

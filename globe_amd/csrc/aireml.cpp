@@ -417,8 +417,8 @@ extern "C" int globe_aireml_fit(globe_ctx* c, const double* y, const globe_airem
     try {
         const globe_aireml_opts& o = *opts;
         Logger log{logfn, user};
-        globe::require(o.num_mc >= 1 && o.num_mc <= kMaxCols / 5, GLOBE_E_INVALID,
-                       "numMC must be in [1, 12]");
+        globe::require(o.num_mc >= 1 && o.num_mc <= kMaxCols, GLOBE_E_INVALID,
+                       "numMC must be in [1, 64]");
         double th[5];
         std::memcpy(th, theta, sizeof(th));
         Eval cur{};

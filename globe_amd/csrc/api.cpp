@@ -109,7 +109,9 @@ void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int prec
     const int32_t n = c->n;
     hipStream_t s = c->stream;
     if (c->gen == 2) {
-        launch_grm_product(s, c->d_grm.as<float>(), n, S0, n, k, th, GZ, n, skip);
+        c->w_grm_part.ensure(grm_product_part_bytes(n));
+        launch_grm_product(s, c->d_grm.as<float>(), n, S0, n, k, th, GZ, n, skip,
+                           c->w_grm_part.as<double>());
         return;
     }
     if (c->m == 0) {
@@ -652,6 +654,49 @@ int globe_load_grm(globe_ctx* c, const float* lt) {
     API_END(c)
 }
 
+int globe_grm_from_geno(globe_ctx* c) {
+    API_BEGIN(c)
+    require(c->gen == 1, GLOBE_E_STATE, "genotypes must be loaded to build the GRM");
+    const int32_t n = c->n;
+    const size_t nel = (size_t)n * (n + 1) / 2;
+    hipStream_t s = c->stream;
+    constexpr int kb = 64;  // columns of G per contraction
+    c->d_grm.ensure(sizeof(float) * std::max<size_t>(nel, 1));
+    DevBuf S0, GZ, th;
+    S0.ensure(sizeof(double) * (size_t)n * kb);
+    GZ.ensure(sizeof(double) * (size_t)n * kb);
+    th.ensure(sizeof(double) * 5 * kMaxCols);
+    std::vector<double> h((size_t)5 * kMaxCols, 0.0);
+    for (int j = 0; j < kMaxCols; ++j) h[5 * j + 1] = 1.0;  // sig2g = 1: the product is G S0
+    HIP_CHECK(hipMemcpyAsync(th.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    // G e_j exactly (a unit vector is exact in the 54-bit fixed point), rounded to FP32
+    for (int64_t j0 = 0; j0 < n; j0 += kb) {
+        const int k = (int)std::min<int64_t>(kb, n - j0);
+        launch_eye_cols(s, S0.as<double>(), n, n, j0, k);
+        genetic_product(c, S0.as<double>(), k, GZ.as<double>(), GLOBE_PREC_FP64, nullptr, th.as<double>());
+        launch_grm_scatter(s, GZ.as<double>(), n, n, j0, k, c->d_grm.as<float>());
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    // the context now works from the GRM, as after globe_load_grm
+    c->gen = 2;
+    c->d_snp.release();
+    c->d_ind.release();
+    c->m = 0;
+    c->d_gdiag.ensure(sizeof(double) * (size_t)n);
+    launch_grm_diag(s, c->d_grm.as<float>(), n, c->d_gdiag.as<double>());
+    HIP_CHECK(hipStreamSynchronize(s));
+    c->gdiag_ok = true;
+    API_END(c)
+}
+
+int globe_get_grm(globe_ctx* c, float* lt) {
+    API_BEGIN(c)
+    require(c->gen == 2, GLOBE_E_STATE, "no GRM in the context");
+    const size_t nel = (size_t)c->n * (c->n + 1) / 2;
+    if (lt) HIP_CHECK(hipMemcpy(lt, c->d_grm.p, sizeof(float) * nel, hipMemcpyDeviceToHost));
+    API_END(c)
+}
+
 int globe_genetic_diag(globe_ctx* c, double* out) {
     API_BEGIN(c)
     ensure_gdiag(c);
@@ -888,7 +933,7 @@ int aireml_eval_impl(globe_ctx* c, const double* y, const double* probes, const 
                      double* grad, double* ai, double* first_terms, double* trace) {
     API_BEGIN(c)
     require(opts != nullptr, GLOBE_E_INVALID, "globe_aireml_eval: null options");
-    require(nmc >= 0 && nmc <= kMaxCols / 5, GLOBE_E_INVALID, "nmc must be in [0, 12]");
+    require(nmc >= 0 && nmc <= kMaxCols, GLOBE_E_INVALID, "nmc must be in [0, 64]");
     require(nmc == 0 || (probes && weights), GLOBE_E_INVALID, "probes/weights required");
     require(c->has_cohort && c->gen != 0, GLOBE_E_STATE, "cohort and genetics required");
     const int64_t nrec = c->nrec;

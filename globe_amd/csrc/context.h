@@ -32,6 +32,7 @@ struct globe_ctx {
     std::vector<uint8_t> qc_keep;
     std::vector<float> qc_maf, h_freq;
     globe::DevBuf d_gdiag;
+    globe::DevBuf w_grm_part;    // GRM product strip partials
     bool gdiag_ok = false;
 
     // variance components

@@ -150,7 +150,15 @@ void launch_reduce_parts_skip(hipStream_t s, const double* part, int nparts, int
                               const int* skip);
 
 // ---------------------------------------------------------------- GRM
+// part: grm_product_part_bytes(n) of scratch for the strip partials
 void launch_grm_product(hipStream_t s, const float* lt, int32_t n, const double* S0, int64_t ldn,
-                        int k, const double* theta_cols, double* GZ, int64_t ldg, const int* skip);
+                        int k, const double* theta_cols, double* GZ, int64_t ldg, const int* skip,
+                        double* part);
+size_t grm_product_part_bytes(int32_t n);
+// lt[i(i+1)/2 + j0 + c] = G[i][j0 + c] (FP32) for i >= j0 + c, from the n x k columns Gc
+void launch_grm_scatter(hipStream_t s, const double* Gc, int64_t ldg, int32_t n, int64_t j0, int k,
+                        float* lt);
+void launch_eye_cols(hipStream_t s, double* S0, int64_t ld, int32_t n, int64_t j0, int k);
+void launch_grm_diag(hipStream_t s, const float* lt, int32_t n, double* d);
 
 }  // namespace globe

@@ -101,6 +101,14 @@ int globe_get_freq(globe_ctx* ctx, float* freq);
 /* Replaces readFullGRM (PCGsolver.cpp:778-809) / readGRM (varCompEst.cpp:270-295):
  * n(n+1)/2 float32 lower triangle, row-major (GCTA .grm.bin). */
 int globe_load_grm(globe_ctx* ctx, const float* grm_lt);
+/* GRM builder (the reference only declares one, PCGsolver.cpp:165-167): G = W W^T / M
+ * from the resident kept genotypes (W standardised as in getStdGenotype,
+ * PCGsolver.cpp:831-842), computed exactly on the int8 matrix cores column block by
+ * column block and stored as the FP32 packed lower triangle of a .grm.bin; the context
+ * then works from the GRM as after globe_load_grm (the genotype layouts are released). */
+int globe_grm_from_geno(globe_ctx* ctx);
+/* The context's GRM as the n(n+1)/2 FP32 lower triangle, row-major (GCTA .grm.bin order). */
+int globe_get_grm(globe_ctx* ctx, float* grm_lt);
 /* Replaces computeGeneticDiagonal (PCGsolver.cpp:925-989): diag(G), n doubles. */
 int globe_genetic_diag(globe_ctx* ctx, double* out);
 

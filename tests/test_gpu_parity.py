@@ -188,15 +188,17 @@ def test_mixed_solve_within_1e6_of_fp64(gpu_ctx_factory):
     assert errs.max() < 1e-6, errs.max()
 
 
-def test_aireml_eval_matches_oracle(gpu_ctx_factory):
-    _say("aireml-eval")
+@pytest.mark.parametrize("nmc", [7, 15])
+def test_aireml_eval_matches_oracle(nmc, gpu_ctx_factory):
+    """Gradient, trace and AI with injected probes; 15 = the reference's numMC default."""
+    _say("aireml-eval", nmc)
     d = load_golden("grm_n180")
     th = (0.5, 0.5, 0.3, 0.05, 0.3)
     g = device_for(d, gpu_ctx_factory, theta=th)
     orc = golden_oracle(d, theta=th)
     c = d["cohort"]
     rng = np.random.default_rng(11)
-    probes = rng.choice([-1.0, 1.0], size=(c.nrec, 7))
+    probes = rng.choice([-1.0, 1.0], size=(c.nrec, nmc))
     og = g.aireml_eval(d["y"], probes, probes, tol=1e-10)
     oo = orc.aireml_eval(d["y"], probes, probes, tol=1e-10)
     for key in ("first", "trace", "grad", "ai"):
@@ -263,3 +265,40 @@ def test_record_tiles_and_large_block_fallback(big_block, gpu_ctx_factory):
     xg, _ = g.pcg(b, tol=1e-9, rule=G.RULE_PER_COLUMN, precision=G.PREC_FP64, stagnation=False)
     xo, _ = orc.pcg(b, tol=1e-9, rule=2, stagnation=False, q=q)
     assert rel(xg, xo) < 1e-7
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 8, 16, 21])
+def test_grm_product_widths(k, gpu_ctx_factory):
+    """Tiled GRM product for every column-group width (16, 8, 4, 2, 1 and mixes)."""
+    _say("grm-widths", k)
+    d = load_golden("grm_n180")
+    g = device_for(d, gpu_ctx_factory)
+    orc = golden_oracle(d)
+    x = np.random.default_rng(k).standard_normal((d["cohort"].nrec, k))
+    assert rel(g.computeVx_multi(x), orc.vx(x)) < 1e-12
+    assert rel(g.computeVx_multi(x, multGRM_uhat=True), orc.vx(x, grm_only=True)) < 1e-12
+
+
+def test_grm_builder_matches_wwt(gpu_ctx_factory):
+    """G = W W^T / M built on the device from the resident genotypes equals the numpy
+    product to FP32 rounding, and the context then solves from it like a loaded GRM."""
+    import oracle as O
+    _say("grm-builder")
+    d = synth.make_geno_case(n=333, m=700, lam=3, seed=21, miss=0.01)
+    c = d.cohort
+    g = gpu_ctx_factory()
+    g.set_cohort(c.indiv, c.times, c.X)
+    g.prepGenotypeInput(d.bed_rows, 0.01, 0.2)
+    g.set_varcomp(d.theta)
+    x = np.random.default_rng(4).standard_normal((c.nrec, 3))
+    v_geno = g.computeVx_multi(x)
+    g.grm_from_geno()
+    W = synth.standardized(d.bed_rows[d.keep], c.n, d.freq)
+    G = (W @ W.T) / W.shape[1]
+    lt = g.grm_lt()
+    ref = G[np.tril_indices(c.n)].astype(np.float32)
+    assert np.max(np.abs(lt - ref)) <= 2.0 * np.finfo(np.float32).eps * np.max(np.abs(G))
+    assert rel(g.computeVx_multi(x), v_geno) < 1e-6
+    orc = O.Oracle(c.indiv, c.times, c.X, d.theta, grm_lt=lt, n=c.n)
+    assert rel(g.computeVx_multi(x), orc.vx(x)) < 1e-12
+    assert rel(g.computeGeneticDiagonal(), np.diag(G)) < 1e-6
