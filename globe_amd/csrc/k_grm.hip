@@ -6,10 +6,11 @@
 // Product: a workgroup owns a strip of 64 output rows and one of kGrmChunks ranges of
 // 64-column tiles of the full symmetric row strip. Each 64 x 64 tile is staged in LDS
 // with coalesced loads -- from the stored block (I, J) when J < I, transposed from the
-// stored block (J, I) when J > I, mirrored when J == I -- next to the 64 x k slice of
-// S0, and every thread accumulates a 2 x (k/8) (k >= 8) or 1 x 1 register tile of
-// outputs. The strip partials of the chunks are added in chunk order, so the result
-// is deterministic. Each stored element is read twice (once for its row strip, once
+// stored block (J, I) when J > I, mirrored when J == I -- next to the 64 x 16 slice of
+// S0, and multiplied on the FP64 matrix cores (an FP64 VALU form of the same tile
+// product is bound by LDS bandwidth: 2 operand reads per 4 FMAs); a group of up to 16
+// columns takes one pass over the triangle. The strip partials of the chunks are added
+// in chunk order, so the result is deterministic. Each stored element is read twice (once for its row strip, once
 // for its column strip); the triangle is far larger than any cache.
 #include <algorithm>
 
@@ -24,82 +25,113 @@ constexpr int kGrmChunks = 4;  // column ranges per row strip
 
 __device__ inline int64_t lt_index(int64_t i, int64_t j) { return i * (i + 1) / 2 + j; }
 
-template <int KC>
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int kGP = kGT + 4;  // LDS row pitch of a tile (floats)
+
+// Thread t holds tile elements (a, b) = (t / 64 + 4 q, t % 64), q < 16, of tile (I, J):
+// KIND 0 (J < I): G[i0 + a][j0 + b] from the stored block (I, J);
+// KIND 1 (J > I): G[i0 + b][j0 + a] from the stored block (J, I) (row j0 + a, coalesced in b);
+// KIND 2 (J == I): the diagonal block, mirrored.
+template <int KIND>
+__device__ inline void tile_load(const float* __restrict__ lt, int32_t n, int64_t i0, int64_t j0, int t,
+                                 float (&v)[16]) {
+    const int b = t & 63;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int a = (t >> 6) + 4 * q;
+        int64_t idx;
+        if (KIND == 0) {
+            const int64_t i = i0 + a;
+            idx = i < n ? lt_index(i, j0 + b) : 0;
+        } else if (KIND == 1) {
+            const int64_t j = j0 + a;
+            idx = j < n ? lt_index(j, i0 + b) : 0;
+        } else {
+            const int64_t i = i0 + a, j = j0 + b;
+            idx = (i < n && j < n) ? (b <= a ? lt_index(i, j) : lt_index(j, i)) : 0;
+        }
+        v[q] = lt[idx];
+    }
+}
+
+template <int KIND>
+__device__ inline void tile_store(float* __restrict__ g, int32_t n, int64_t i0, int64_t j0, int t,
+                                  const float (&v)[16]) {
+    const int b = t & 63;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int a = (t >> 6) + 4 * q;
+        if (KIND == 0)
+            g[a * kGP + b] = i0 + a < n ? v[q] : 0.0f;
+        else if (KIND == 1)
+            g[b * kGP + a] = j0 + a < n ? v[q] : 0.0f;
+        else
+            g[a * kGP + b] = (i0 + a < n && j0 + b < n) ? v[q] : 0.0f;
+    }
+}
+
+constexpr int kGC = 16;       // columns of S0 per pass
+
+// One pass of up to 16 columns: out (64 x 16) += G_tile (64 x 64) S0_slice (64 x 16) per
+// tile on the FP64 matrix cores (v_mfma_f64_16x16x4f64): wave w owns rows 16 w .. 16 w
+// + 15 of the strip; A lane l = G[16 w + l % 16][4 s + l / 16] (FP32 tile in LDS,
+// widened), B lane l = S0[4 s + l / 16][l % 16], D element v of lane l = (row
+// 4 v + l / 16, column l % 16) (probed on gfx950). Columns past kk are zero.
 __global__ __launch_bounds__(256) void k_grm_tiles(const float* __restrict__ lt, int32_t n,
-                                                   const double* __restrict__ S0, int64_t ldn,
+                                                   const double* __restrict__ S0, int64_t ldn, int kk,
                                                    int32_t ntiles, double* __restrict__ part,
                                                    const int* skip) {
     if (skip && *skip) return;
-    __shared__ float g[kGT][kGT + 1];
-    __shared__ double s[KC][kGT];
+    __shared__ float g[kGT * kGP];
+    __shared__ double sh[kGT * kGC];  // [k][c]
     const int I = blockIdx.x, h = blockIdx.y;
     const int64_t i0 = (int64_t)I * kGT;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int J0 = (int)((int64_t)h * ntiles / kGrmChunks), J1 = (int)((int64_t)(h + 1) * ntiles / kGrmChunks);
-    // register tile: KC >= 8: rows 2 rp, 2 rp + 1 and columns cg * CPT .. + CPT;
-    // KC < 8: row t % 64, column t / 64 (threads past KC columns idle)
-    constexpr bool kWide = KC >= 8;
-    constexpr int RPT = kWide ? 2 : 1, CPT = kWide ? KC / 8 : 1;
-    const int r0 = kWide ? 2 * (t & 31) : (t & 63);
-    const int c0 = kWide ? (t >> 5) * CPT : (t >> 6);
-    const bool live = c0 < KC;
-    double acc[RPT][CPT];
-#pragma unroll
-    for (int a = 0; a < RPT; ++a)
-#pragma unroll
-        for (int b = 0; b < CPT; ++b) acc[a][b] = 0.0;
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
     for (int J = J0; J < J1; ++J) {
         const int64_t j0 = (int64_t)J * kGT;
+        // all 16 + 4 loads of the tile are issued before the first LDS store (indices
+        // clamped in bounds; out-of-range elements become 0 when stored)
+        float v[16];
+        const int kind = J < I ? 0 : (J > I ? 1 : 2);
+        if (kind == 0)
+            tile_load<0>(lt, n, i0, j0, t, v);
+        else if (kind == 1)
+            tile_load<1>(lt, n, i0, j0, t, v);
+        else
+            tile_load<2>(lt, n, i0, j0, t, v);
+        double sv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = t + 256 * q, kq = e >> 4, c = e & 15;
+            const int64_t j = j0 + kq;
+            const bool good = c < kk && j < n;
+            const double x = S0[good ? c * ldn + j : 0];
+            sv[q] = good ? x : 0.0;
+        }
+        __syncthreads();  // the previous tile's MFMAs have read g and sh
+        if (kind == 0)
+            tile_store<0>(g, n, i0, j0, t, v);
+        else if (kind == 1)
+            tile_store<1>(g, n, i0, j0, t, v);
+        else
+            tile_store<2>(g, n, i0, j0, t, v);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sh[t + 256 * q] = sv[q];
         __syncthreads();
-        if (J < I) {
-            for (int e = t; e < kGT * kGT; e += 256) {
-                const int rr = e >> 6, cc = e & 63;
-                const int64_t i = i0 + rr, j = j0 + cc;
-                g[rr][cc] = i < n ? lt[lt_index(i, j)] : 0.0f;
-            }
-        } else if (J > I) {
-            for (int e = t; e < kGT * kGT; e += 256) {
-                const int jj = e >> 6, ii = e & 63;
-                const int64_t i = i0 + ii, j = j0 + jj;
-                g[ii][jj] = j < n ? lt[lt_index(j, i)] : 0.0f;
-            }
-        } else {
-            for (int e = t; e < kGT * kGT; e += 256) {
-                const int rr = e >> 6, cc = e & 63;
-                const int64_t i = i0 + rr, j = j0 + cc;
-                float v = 0.0f;
-                if (i < n && j < n) v = cc <= rr ? lt[lt_index(i, j)] : lt[lt_index(j, i)];
-                g[rr][cc] = v;
-            }
-        }
-        for (int e = t; e < KC * kGT; e += 256) {
-            const int c = e >> 6, jj = e & 63;
-            const int64_t j = j0 + jj;
-            s[c][jj] = j < n ? S0[c * ldn + j] : 0.0;
-        }
-        __syncthreads();
-        if (live) {
-#pragma unroll 8
-            for (int jj = 0; jj < kGT; ++jj) {
-                double gv[RPT], sv[CPT];
-#pragma unroll
-                for (int a = 0; a < RPT; ++a) gv[a] = (double)g[r0 + a][jj];
-#pragma unroll
-                for (int b = 0; b < CPT; ++b) sv[b] = s[c0 + b][jj];
-#pragma unroll
-                for (int a = 0; a < RPT; ++a)
-#pragma unroll
-                    for (int b = 0; b < CPT; ++b) acc[a][b] = fma(gv[a], sv[b], acc[a][b]);
-            }
-        }
+        const float* gr = g + (16 * w + (lane & 15)) * kGP + (lane >> 4);
+        const double* sb = sh + (lane >> 4) * kGC + (lane & 15);
+#pragma unroll 4
+        for (int s4 = 0; s4 < kGT / 4; ++s4)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)gr[4 * s4], sb[4 * s4 * kGC], acc, 0, 0, 0);
     }
-    if (!live) return;
+    const int col = lane & 15;
+    if (col >= kk) return;
 #pragma unroll
-    for (int a = 0; a < RPT; ++a) {
-        const int64_t i = i0 + r0 + a;
-        if (i < n)
-#pragma unroll
-            for (int b = 0; b < CPT; ++b) part[((int64_t)h * KC + c0 + b) * n + i] = acc[a][b];
+    for (int v = 0; v < 4; ++v) {
+        const int64_t i = i0 + 16 * w + 4 * v + (lane >> 4);
+        if (i < n) part[((int64_t)h * kGC + col) * n + i] = acc[v];
     }
 }
 
@@ -165,33 +197,14 @@ void launch_grm_product(hipStream_t s, const float* lt, int32_t n, const double*
     const int32_t ntiles = (n + kGT - 1) / kGT;
     const dim3 g(ntiles, kGrmChunks);
     const unsigned gf = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
-    int c = 0;
-    while (c < k) {
-        const int left = k - c;
-        const double* sc = S0 + c * ldn;
-        double* gc = GZ + c * ldg;
-        const double* tc = theta_cols + c * 5;
-        int kc;
-        if (left >= 16) {
-            kc = 16;
-            k_grm_tiles<16><<<g, 256, 0, s>>>(lt, n, sc, ldn, ntiles, part, skip);
-        } else if (left >= 8) {
-            kc = 8;
-            k_grm_tiles<8><<<g, 256, 0, s>>>(lt, n, sc, ldn, ntiles, part, skip);
-        } else if (left >= 4) {
-            kc = 4;
-            k_grm_tiles<4><<<g, 256, 0, s>>>(lt, n, sc, ldn, ntiles, part, skip);
-        } else if (left >= 2) {
-            kc = 2;
-            k_grm_tiles<2><<<g, 256, 0, s>>>(lt, n, sc, ldn, ntiles, part, skip);
-        } else {
-            kc = 1;
-            k_grm_tiles<1><<<g, 256, 0, s>>>(lt, n, sc, ldn, ntiles, part, skip);
-        }
+    // one pass over the triangle per group of up to 16 columns
+    for (int c = 0; c < k; c += kGC) {
+        const int kk = std::min(kGC, k - c);
+        k_grm_tiles<<<g, 256, 0, s>>>(lt, n, S0 + c * ldn, ldn, kk, ntiles, part, skip);
         HIP_CHECK(hipGetLastError());
-        k_grm_finish<<<dim3(gf, kc), 256, 0, s>>>(part, n, kc, tc, gc, ldg, skip);
+        k_grm_finish<<<dim3(gf, kk), 256, 0, s>>>(part, n, kGC, theta_cols + c * 5, GZ + c * ldg, ldg,
+                                                  skip);
         HIP_CHECK(hipGetLastError());
-        c += kc;
     }
 }
 
