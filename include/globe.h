@@ -163,8 +163,8 @@ int globe_aireml_eval(globe_ctx* ctx, const double* y, const double* probes,
  * objective :1044-1065, constraints :1068-1145): minimise -(g.p - p^T AI p / 2) over
  * -10 <= p_i <= 10 subject to sum_i AI_ii p_i <= delta, theta_i + p_i >= 1e-5 (i != 3)
  * and (theta_3+p_3)^2 - (theta_2+p_2)(theta_4+p_4) + 1e-3 <= 0. Returns the Newton step
- * AI^-1 g when it is feasible, else the constrained minimiser from an augmented-Lagrangian
- * projected-gradient solve. Host only (no context). GLOBE_E_NUMERIC when no feasible
+ * AI^-1 g when it is feasible, else the best constrained minimiser of a multi-start
+ * augmented-Lagrangian projected-Newton solve. Host only (no context). GLOBE_E_NUMERIC when no feasible
  * step is found (the reference then shrinks the trust region, :1201-1214). */
 int globe_aireml_step(const double grad[5], const double ai[25], const double theta[5],
                       double delta, double step[5]);
