@@ -99,6 +99,9 @@ def load_library(path: str = LIB_PATH):
         "globe_last_error": ([P], C.c_char_p),
         "globe_set_cohort": ([P, i64, i32, i32, P, P, P, i64], C.c_int),
         "globe_load_bed": ([P, P, i64, i64, C.c_float, C.c_float, P], C.c_int),
+        "globe_load_bed_file": ([P, C.c_char_p, C.c_float, C.c_float, P, P], C.c_int),
+        "globe_set_ingest_chunk": ([P, i64], C.c_int),
+        "globe_ingest_stats": ([P, P], C.c_int),
         "globe_get_qc": ([P, P, P, P, P], C.c_int),
         "globe_get_freq": ([P, P], C.c_int),
         "globe_load_grm": ([P, P], C.c_int),
@@ -231,6 +234,26 @@ class Globe:
         self.m = kept.value
         self.m_total = rows.shape[0]
         return kept.value
+
+    def load_bed_file(self, prefix, min_maf=0.01, max_miss=0.2):
+        """prepGenotypeInput(prefix, ...) at file level: .bim line count, .bed header and
+        size checks, .fam sample count when present, streamed ingest with device QC."""
+        total, kept = C.c_int64(), C.c_int64()
+        self._check(self.L.globe_load_bed_file(self.h, str(prefix).encode(), min_maf, max_miss,
+                                               C.byref(total), C.byref(kept)))
+        self.m = kept.value
+        self.m_total = total.value
+        return kept.value
+
+    def set_ingest_chunk(self, nbytes):
+        self._check(self.L.globe_set_ingest_chunk(self.h, int(nbytes)))
+
+    def ingest_stats(self):
+        """Last host-sourced ingest: total, host-fill and host-wait seconds, chunks, threads."""
+        out = np.zeros(5)
+        self._check(self.L.globe_ingest_stats(self.h, _p(out)))
+        return {"seconds": out[0], "fill_s": out[1], "wait_s": out[2], "chunks": int(out[3]),
+                "threads": int(out[4])}
 
     def synth_bed(self, m, seed, f_lo=0.01, f_hi=0.5, miss=0.0, min_maf=0.01, max_miss=0.2):
         kept = C.c_int64()

@@ -462,135 +462,7 @@ int globe_set_cohort(globe_ctx* c, int64_t nrec, int32_t n, int32_t p, const int
 
 }  // extern "C"
 
-// ------------------------------------------------------------------ genotype staging
-namespace {
-
-// FP32 keep rule of PCGsolver.cpp:726-732, evaluated exactly as the reference does.
-inline bool keep_snp(int64_t alt, int64_t miss, int32_t n, float min_maf, float max_miss,
-                     float* maf_out) {
-    const int n_obs = n - (int)miss;
-    volatile float denom = 2.0f * (float)n_obs;
-    const float maf = (n_obs > 0) ? (float)alt / denom : 0.0f;
-    const float mr = (float)miss / (float)n;
-    *maf_out = maf;
-    return maf >= min_maf && maf <= (1.0f - min_maf) && mr <= max_miss;
-}
-
-// Stage m input SNP rows in chunks: fill(chunk_dev, s0, cnt) writes cnt rows of
-// `pitch_in` bytes; counts + QC + compaction run per chunk, then the
-// individual-major copy is built from the kept rows.
-template <class Fill>
-void stage_rows(globe_ctx* c, int64_t m, int64_t pitch_in, float min_maf, float max_miss,
-                Fill fill) {
-    require(c->has_cohort, GLOBE_E_STATE, "cohort must be set before genotypes");
-    const int32_t n = c->n;
-    require(pitch_in >= (n + 3) / 4, GLOBE_E_INVALID, "row pitch smaller than ceil(N/4)");
-    require(min_maf >= 0.0f && min_maf <= 0.5f && max_miss >= 0.0f && max_miss <= 1.0f,
-            GLOBE_E_INVALID, "invalid MAF or missing rate thresholds");
-    hipStream_t s = c->stream;
-    c->gen = 0;
-    c->gdiag_ok = false;
-    c->d_grm.release();
-    c->pitch_s = round_up((n + 3) / 4, kPitchAlign);
-    c->d_snp.release();
-    c->d_ind.release();
-    // rows are padded to a multiple of kRowPad with zero (dosage-0) bytes so the contraction
-    // kernels load whole row blocks without bounds checks
-    c->d_snp.ensure((size_t)round_up(std::max<int64_t>(m, 1), kRowPad) * c->pitch_s);
-    const int64_t chunk_rows = std::max<int64_t>(1, std::min<int64_t>(m, (int64_t)(1ll << 30) / pitch_in));
-    DevBuf raw, d_alt, d_miss, d_idx;
-    raw.ensure((size_t)chunk_rows * pitch_in);
-    d_alt.ensure(sizeof(int64_t) * chunk_rows);
-    d_miss.ensure(sizeof(int64_t) * chunk_rows);
-    d_idx.ensure(sizeof(int64_t) * chunk_rows);
-    c->qc_alt.assign(m, 0);
-    c->qc_miss.assign(m, 0);
-    c->qc_keep.assign(m, 0);
-    c->qc_maf.assign(m, 0.f);
-    c->h_freq.clear();
-    int64_t kept = 0;
-    std::vector<int64_t> idx;
-    for (int64_t s0 = 0; s0 < m; s0 += chunk_rows) {
-        const int64_t cnt = std::min(chunk_rows, m - s0);
-        fill(raw.as<uint8_t>(), s0, cnt);
-        launch_bed_counts(s, raw.as<uint8_t>(), cnt, pitch_in, n, d_alt.as<int64_t>(),
-                          d_miss.as<int64_t>());
-        HIP_CHECK(hipMemcpyAsync(&c->qc_alt[s0], d_alt.p, sizeof(int64_t) * cnt, hipMemcpyDeviceToHost, s));
-        HIP_CHECK(hipMemcpyAsync(&c->qc_miss[s0], d_miss.p, sizeof(int64_t) * cnt, hipMemcpyDeviceToHost, s));
-        HIP_CHECK(hipStreamSynchronize(s));
-        idx.clear();
-        for (int64_t q = 0; q < cnt; ++q) {
-            float maf;
-            const bool k = keep_snp(c->qc_alt[s0 + q], c->qc_miss[s0 + q], n, min_maf, max_miss, &maf);
-            c->qc_maf[s0 + q] = maf;
-            c->qc_keep[s0 + q] = k;
-            if (k) {
-                idx.push_back(q);
-                c->h_freq.push_back(maf);
-            }
-        }
-        if (!idx.empty()) {
-            HIP_CHECK(hipMemcpyAsync(d_idx.p, idx.data(), sizeof(int64_t) * idx.size(), hipMemcpyHostToDevice, s));
-            launch_compact_rows(s, raw.as<uint8_t>(), pitch_in, d_idx.as<int64_t>(), (int64_t)idx.size(), n,
-                                c->d_snp.as<uint8_t>() + kept * c->pitch_s, c->pitch_s);
-            HIP_CHECK(hipStreamSynchronize(s));
-        }
-        kept += (int64_t)idx.size();
-    }
-    raw.release();
-    c->m_total = m;
-    c->m = kept;
-    {
-        const int64_t pad_rows = round_up(std::max<int64_t>(kept, 1), kRowPad) - kept;
-        HIP_CHECK(hipMemsetAsync(c->d_snp.as<uint8_t>() + kept * c->pitch_s, 0,
-                                 (size_t)pad_rows * c->pitch_s, s));
-    }
-    c->pitch_i = round_up((kept + 3) / 4, kPitchAlign);
-    c->d_ind.ensure((size_t)round_up(n, kRowPad) * c->pitch_i);
-    HIP_CHECK(hipMemsetAsync(c->d_ind.p, 0, (size_t)round_up(n, kRowPad) * c->pitch_i, s));
-    launch_transpose_geno(s, c->d_snp.as<uint8_t>(), kept, c->pitch_s, n, c->d_ind.as<uint8_t>(), c->pitch_i);
-    c->d_freq.ensure(sizeof(float) * std::max<int64_t>(kept, 1));
-    c->d_twof.ensure(sizeof(double) * std::max<int64_t>(kept, 1));
-    c->d_inv.ensure(sizeof(double) * std::max<int64_t>(kept, 1));
-    if (kept > 0) {
-        HIP_CHECK(hipMemcpyAsync(c->d_freq.p, c->h_freq.data(), sizeof(float) * kept, hipMemcpyHostToDevice, s));
-        launch_snp_consts(s, c->d_freq.as<float>(), kept, c->d_twof.as<double>(), c->d_inv.as<double>());
-    }
-    HIP_CHECK(hipStreamSynchronize(s));
-    c->gen = 1;
-}
-
-}  // namespace
-
 extern "C" {
-
-int globe_load_bed(globe_ctx* c, const uint8_t* rows, int64_t m, int64_t pitch, float min_maf,
-                   float max_miss, int64_t* m_kept) {
-    API_BEGIN(c)
-    require(rows != nullptr || m == 0, GLOBE_E_INVALID, "null genotype rows");
-    require(m >= 0, GLOBE_E_INVALID, "negative SNP count");
-    stage_rows(c, m, pitch, min_maf, max_miss, [&](uint8_t* dst, int64_t s0, int64_t cnt) {
-        HIP_CHECK(hipMemcpyAsync(dst, rows + s0 * pitch, (size_t)cnt * pitch, hipMemcpyHostToDevice,
-                                 c->stream));
-    });
-    if (m_kept) *m_kept = c->m;
-    API_END(c)
-}
-
-int globe_synth_bed(globe_ctx* c, int64_t m, uint64_t seed, float f_lo, float f_hi, float miss,
-                    float min_maf, float max_miss, int64_t* m_kept) {
-    API_BEGIN(c)
-    require(m >= 0 && f_lo >= 0.f && f_hi <= 1.f && f_lo <= f_hi && miss >= 0.f && miss <= 1.f,
-            GLOBE_E_INVALID, "globe_synth_bed: bad parameters");
-    require(c->has_cohort, GLOBE_E_STATE, "cohort must be set before genotypes");
-    const int64_t pitch = round_up((c->n + 3) / 4, kPitchAlign);
-    stage_rows(c, m, pitch, min_maf, max_miss, [&](uint8_t* dst, int64_t s0, int64_t cnt) {
-        launch_synth_bed(c->stream, dst, cnt, pitch, c->n, seed + 0x51ED2701ull * (uint64_t)s0, f_lo,
-                         f_hi, miss);
-    });
-    if (m_kept) *m_kept = c->m;
-    API_END(c)
-}
 
 int globe_get_qc(globe_ctx* c, int64_t* alt, int64_t* miss, uint8_t* keep, float* maf) {
     API_BEGIN(c)

@@ -118,14 +118,14 @@ int main(int argc, char** argv) {
         check(globe_load_grm(ctx, lt.data()), "GRM");
         emit("GRM read successfully.");
     } else {
-        globe_io::BedMap bed;
+        globe_io::BedInfo bed;
         try {
-            globe_io::open_bed(geno, c.n, bed);
+            bed = globe_io::check_bed(geno, c.n);
         } catch (const std::exception& e) {
             die(e.what());
         }
         int64_t kept = 0;
-        check(globe_load_bed(ctx, bed.rows, bed.m, bed.row_bytes, 0.01f, 0.2f, &kept), "genotypes");
+        check(globe_load_bed_file(ctx, geno.c_str(), 0.01f, 0.2f, nullptr, &kept), "genotypes");
         emit("Genotypes staged: " + std::to_string(kept) + " / " + std::to_string(bed.m) + " SNPs kept.");
     }
     emit("Input files read and processed successfully.");

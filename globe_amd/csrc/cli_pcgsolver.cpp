@@ -100,10 +100,10 @@ int main(int argc, char** argv) {
                   << " fixed effects (not counting TIME)\n   Nrec = " << c.nrec << " records total." << std::endl;
 
     // open every input before touching the device, so file errors exit like the reference
-    globe_io::BedMap bed;
+    globe_io::BedInfo bed;
     std::vector<float> lt;
     try {
-        if (!geno.empty()) globe_io::open_bed(geno, c.n, bed);
+        if (!geno.empty()) bed = globe_io::check_bed(geno, c.n);
         else lt = globe_io::read_grm(grm, c.n);
     } catch (const std::exception& e) {
         die(e.what());
@@ -116,7 +116,7 @@ int main(int argc, char** argv) {
           "cohort");
     if (!geno.empty()) {
         int64_t kept = 0;
-        check(ctx, globe_load_bed(ctx, bed.rows, bed.m, bed.row_bytes, min_maf, max_miss, &kept), "genotypes");
+        check(ctx, globe_load_bed_file(ctx, geno.c_str(), min_maf, max_miss, nullptr, &kept), "genotypes");
         std::cout << "[prepGenotypeInput] Processed " << bed.m << " / " << bed.m << " SNPs." << std::endl;
         if (debug) std::cout << "[prepGenotypeInput] Filtered SNPs = " << kept << " / " << bed.m << std::endl;
     } else {

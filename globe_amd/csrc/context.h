@@ -34,6 +34,10 @@ struct globe_ctx {
     globe::DevBuf d_gdiag;
     globe::DevBuf w_grm_part;    // GRM product strip partials
     bool gdiag_ok = false;
+    // genotype ingest: bytes of .bed rows per pipeline chunk, and the last ingest's
+    // timings (seconds: total, host fill, host waiting on the device; chunks; threads)
+    int64_t ingest_chunk = 256ll << 20;
+    double ingest_stats[5] = {0, 0, 0, 0, 0};
 
     // variance components
     std::array<double, 5> theta{0, 0, 0, 0, 0};

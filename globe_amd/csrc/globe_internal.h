@@ -124,10 +124,14 @@ void launch_sub_cols(hipStream_t s, int64_t rows, const double* a, int64_t lda, 
 void launch_fill_cols(hipStream_t s, int64_t rows, double* d, int64_t ldd, int k, double v);
 
 // ---------------------------------------------------------------- genotypes
-void launch_bed_counts(hipStream_t s, const uint8_t* rows, int64_t m, int64_t pitch, int32_t n,
-                       int64_t* alt, int64_t* miss);
-void launch_compact_rows(hipStream_t s, const uint8_t* src, int64_t pitch_src, const int64_t* idx,
-                         int64_t m_kept, int32_t n, uint8_t* dst, int64_t pitch_dst);
+// counts + keep rule per row (maf/keep may be null: counts only)
+void launch_bed_qc(hipStream_t s, const uint8_t* rows, int64_t m, int64_t pitch, int32_t n,
+                   float min_maf, float max_miss, int64_t* alt, int64_t* miss, float* maf,
+                   uint8_t* keep);
+// pos[q] = destination row of kept row q (or -1); *base advances by the kept count
+void launch_keep_scan(hipStream_t s, const uint8_t* keep, int64_t cnt, int64_t* base, int64_t* pos);
+void launch_compact_pos(hipStream_t s, const uint8_t* src, int64_t pitch_src, const int64_t* pos,
+                        int64_t cnt, int32_t n, uint8_t* dst, int64_t pitch_dst);
 void launch_transpose_geno(hipStream_t s, const uint8_t* snp, int64_t m, int64_t pitch_s, int32_t n,
                            uint8_t* ind, int64_t pitch_i);
 void launch_synth_bed(hipStream_t s, uint8_t* rows, int64_t m, int64_t pitch, int32_t n,

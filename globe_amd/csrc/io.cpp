@@ -2,7 +2,6 @@
 #include "io.h"
 
 #include <fcntl.h>
-#include <sys/mman.h>
 #include <sys/stat.h>
 #include <unistd.h>
 
@@ -72,41 +71,34 @@ Cohort read_input(const std::string& path) {
     return c;
 }
 
-BedMap::~BedMap() {
-    if (base) munmap(base, len);
-}
-
-void open_bed(const std::string& prefix, int32_t n, BedMap& out) {
-    const std::string bim = prefix + ".bim", bed = prefix + ".bed";
+BedInfo check_bed(const std::string& prefix, int32_t n) {
+    const std::string bim = prefix + ".bim", bed = prefix + ".bed", fam = prefix + ".fam";
+    BedInfo out;
     std::ifstream b(bim);
     if (!b.is_open()) throw std::runtime_error("[prepGenotypeInput] Error opening .bim file: " + bim);
     out.m = std::count(std::istreambuf_iterator<char>(b), std::istreambuf_iterator<char>(), '\n');
-    int fd = ::open(bed.c_str(), O_RDONLY);
-    if (fd < 0) throw std::runtime_error("[prepGenotypeInput] Error opening .bed file: " + bed);
-    struct stat st;
-    fstat(fd, &st);
-    out.len = (size_t)st.st_size;
+    std::ifstream fm(fam);
+    if (fm.is_open()) {
+        const int64_t nf = std::count(std::istreambuf_iterator<char>(fm), std::istreambuf_iterator<char>(), '\n');
+        if (nf != n)
+            throw std::runtime_error("[prepGenotypeInput] " + fam + " lists " + std::to_string(nf) +
+                                     " samples but the phenotype file has N = " + std::to_string(n));
+    }
+    std::ifstream f(bed, std::ios::binary | std::ios::ate);
+    if (!f.is_open()) throw std::runtime_error("[prepGenotypeInput] Error opening .bed file: " + bed);
+    const int64_t len = (int64_t)f.tellg();
     out.row_bytes = (n + 3) / 4;
-    if (out.len < 3) {
-        ::close(fd);
+    unsigned char magic[3] = {0, 0, 0};
+    f.seekg(0);
+    if (len < 3 || !f.read(reinterpret_cast<char*>(magic), 3) || magic[0] != 0x6C || magic[1] != 0x1B)
         throw std::runtime_error("[prepGenotypeInput] Invalid .bed format (magic mismatch).");
-    }
-    out.base = mmap(nullptr, out.len, PROT_READ, MAP_PRIVATE, fd, 0);
-    ::close(fd);
-    if (out.base == MAP_FAILED) {
-        out.base = nullptr;
-        throw std::runtime_error("[prepGenotypeInput] cannot map " + bed);
-    }
-    const uint8_t* p = static_cast<const uint8_t*>(out.base);
-    if (p[0] != 0x6C || p[1] != 0x1B)
-        throw std::runtime_error("[prepGenotypeInput] Invalid .bed format (magic mismatch).");
-    if (p[2] == 0x00)
+    if (magic[2] == 0x00)
         throw std::runtime_error(
             "[prepGenotypeInput] BED is individual-major, code expects SNP-major => aborting.");
-    if (p[2] != 0x01) throw std::runtime_error("[prepGenotypeInput] Third byte unknown => aborting.");
-    if ((int64_t)out.len < 3 + out.m * out.row_bytes)
+    if (magic[2] != 0x01) throw std::runtime_error("[prepGenotypeInput] Third byte unknown => aborting.");
+    if (len < 3 + out.m * out.row_bytes)
         throw std::runtime_error("[prepGenotypeInput] .bed holds fewer than M x ceil(N/4) genotype bytes");
-    out.rows = p + 3;
+    return out;
 }
 
 std::vector<float> read_grm(const std::string& prefix, int32_t n) {

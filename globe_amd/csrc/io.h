@@ -20,17 +20,15 @@ struct Cohort {
 // float, as the reference does, then widened. Throws std::runtime_error.
 Cohort read_input(const std::string& path);
 
-// Memory-mapped PLINK .bed (SNP-major, magic 6C 1B 01) + .bim line count
-// (prepGenotypeInput, PCGsolver.cpp:624-667).
-struct BedMap {
+// PLINK .bed (SNP-major, magic 6C 1B 01) + .bim line count (prepGenotypeInput,
+// PCGsolver.cpp:624-667), checked before the device is touched; when <prefix>.fam
+// exists its line count must equal n. The rows themselves are streamed by
+// globe_load_bed_file. Throws std::runtime_error.
+struct BedInfo {
     int64_t m = 0;           // SNPs in .bim
     int64_t row_bytes = 0;   // ceil(n/4)
-    const uint8_t* rows = nullptr;
-    void* base = nullptr;
-    size_t len = 0;
-    ~BedMap();
 };
-void open_bed(const std::string& prefix, int32_t n, BedMap& out);
+BedInfo check_bed(const std::string& prefix, int32_t n);
 
 // <prefix>.grm.bin: n(n+1)/2 float32 lower triangle (readFullGRM, :778-809).
 std::vector<float> read_grm(const std::string& prefix, int32_t n);

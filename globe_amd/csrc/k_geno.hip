@@ -25,14 +25,18 @@ __device__ inline uint32_t codes_to_dosages(uint32_t b) {
     return one | (two << 1);
 }
 
-// ------------------------------------------------------------------ QC counts
+// ------------------------------------------------------------------ QC
 // Per SNP: A1 allele count 2*#00 + #10 and missing count #01 over the first n
 // individuals (PCGsolver.cpp:711-724; excluding phantom pairs is what
-// subtractPhantomGenotypes :590-613 achieves). Integer exact.
-__global__ __launch_bounds__(kBlock) void k_bed_counts(const uint8_t* __restrict__ rows, int64_t m,
-                                                       int64_t pitch, int32_t n,
-                                                       int64_t* __restrict__ alt,
-                                                       int64_t* __restrict__ miss) {
+// subtractPhantomGenotypes :590-613 achieves), integer exact; then the keep rule of
+// :726-732 in the reference's FP32 order with IEEE round-to-nearest conversions and
+// divisions: maf = (float)alt / (2.0f * (float)n_obs), missRate = (float)miss /
+// (float)N, keep iff maf >= minMAF && maf <= 1 - minMAF && missRate <= maxMiss.
+__global__ __launch_bounds__(kBlock) void k_bed_qc(const uint8_t* __restrict__ rows, int64_t m,
+                                                   int64_t pitch, int32_t n, float min_maf,
+                                                   float max_miss, int64_t* __restrict__ alt,
+                                                   int64_t* __restrict__ miss, float* __restrict__ maf,
+                                                   uint8_t* __restrict__ keep) {
     __shared__ int sh_a[kBlock / 64], sh_m[kBlock / 64];
     const int64_t nbytes = (n + 3) / 4;
     for (int64_t s = blockIdx.x; s < m; s += gridDim.x) {
@@ -65,23 +69,80 @@ __global__ __launch_bounds__(kBlock) void k_bed_counts(const uint8_t* __restrict
             }
             alt[s] = ta;
             miss[s] = tm;
+            if (maf) {
+                const int n_obs = n - (int)tm;
+                const float f = n_obs > 0 ? __fdiv_rn(__ll2float_rn(ta), __fmul_rn(2.0f, __int2float_rn(n_obs)))
+                                          : 0.0f;
+                const float mr = __fdiv_rn(__ll2float_rn(tm), __int2float_rn(n));
+                maf[s] = f;
+                keep[s] = (f >= min_maf && f <= __fsub_rn(1.0f, min_maf) && mr <= max_miss) ? 1 : 0;
+            }
         }
     }
 }
 
-// Copy kept rows as 2-bit dosages, with phantom genotypes of the last byte and the
-// pitch padding set to dosage 0.
-__global__ __launch_bounds__(kBlock) void k_compact_rows(const uint8_t* __restrict__ src,
-                                                         int64_t pitch_src,
-                                                         const int64_t* __restrict__ idx,
-                                                         int64_t m_kept, int32_t n,
-                                                         uint8_t* __restrict__ dst,
-                                                         int64_t pitch_dst) {
+// Destination rows of one chunk's kept rows, in SNP order: pos[q] = *base + (kept
+// rows before q) for a kept row q, else -1; then *base += the chunk's kept count.
+// One block: each thread counts kScanPer consecutive flags, the block scans the
+// thread counts (wave shuffles, then across waves in LDS), tile after tile.
+constexpr int kScanThreads = 1024, kScanPer = 8;
+__global__ __launch_bounds__(kScanThreads) void k_keep_scan(const uint8_t* __restrict__ keep, int64_t cnt,
+                                                            int64_t* __restrict__ base,
+                                                            int64_t* __restrict__ pos) {
+    __shared__ int64_t wsum[kScanThreads / 64];
+    __shared__ int64_t carry;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (t == 0) carry = *base;
+    __syncthreads();
+    constexpr int64_t kTile = (int64_t)kScanThreads * kScanPer;
+    for (int64_t t0 = 0; t0 < cnt; t0 += kTile) {
+        const int64_t q0 = t0 + (int64_t)t * kScanPer;
+        int c = 0;
+#pragma unroll
+        for (int e = 0; e < kScanPer; ++e) c += (q0 + e < cnt && keep[q0 + e]) ? 1 : 0;
+        // inclusive scan of c over the wave
+        int v = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(v, o, 64);
+            if (lane >= o) v += u;
+        }
+        if (lane == 63) wsum[w] = v;
+        __syncthreads();
+        int64_t before = carry;
+        for (int q = 0; q < w; ++q) before += wsum[q];
+        before += v - c;  // exclusive within the wave
+        int64_t r = before;
+#pragma unroll
+        for (int e = 0; e < kScanPer; ++e) {
+            const int64_t q = q0 + e;
+            if (q < cnt) {
+                const bool k = keep[q] != 0;
+                pos[q] = k ? r : -1;
+                r += k ? 1 : 0;
+            }
+        }
+        __syncthreads();  // every thread has read carry and wsum
+        if (t == kScanThreads - 1) carry = r;
+        __syncthreads();
+    }
+    if (t == 0) *base = carry;
+}
+
+// Copy the rows with pos[q] >= 0 to row pos[q] of dst as 2-bit dosages, with phantom
+// genotypes of the last byte and the pitch padding set to dosage 0.
+__global__ __launch_bounds__(kBlock) void k_compact_pos(const uint8_t* __restrict__ src,
+                                                        int64_t pitch_src,
+                                                        const int64_t* __restrict__ pos, int64_t cnt,
+                                                        int32_t n, uint8_t* __restrict__ dst,
+                                                        int64_t pitch_dst) {
     const int64_t nbytes = (n + 3) / 4;
     const int rem = n & 3;
     const uint32_t live = rem == 0 ? 0xFFu : (1u << (2 * rem)) - 1u;  // fields of the last byte
-    for (int64_t r = blockIdx.x; r < m_kept; r += gridDim.x) {
-        const uint8_t* s = src + idx[r] * pitch_src;
+    for (int64_t q = blockIdx.x; q < cnt; q += gridDim.x) {
+        const int64_t r = pos[q];
+        if (r < 0) continue;
+        const uint8_t* s = src + q * pitch_src;
         uint8_t* d = dst + r * pitch_dst;
         for (int64_t j = threadIdx.x; j < pitch_dst; j += kBlock) {
             uint32_t v = 0;
@@ -359,19 +420,26 @@ __global__ __launch_bounds__(kBlock) void k_center_ind(int32_t n, const double* 
 
 #define LAUNCH_CHECK() HIP_CHECK(hipGetLastError())
 
-void launch_bed_counts(hipStream_t s, const uint8_t* rows, int64_t m, int64_t pitch, int32_t n,
-                       int64_t* alt, int64_t* miss) {
+void launch_bed_qc(hipStream_t s, const uint8_t* rows, int64_t m, int64_t pitch, int32_t n,
+                   float min_maf, float max_miss, int64_t* alt, int64_t* miss, float* maf,
+                   uint8_t* keep) {
     int64_t g = m < 65536 ? m : 65536;
     if (g < 1) return;
-    k_bed_counts<<<(unsigned)g, kBlock, 0, s>>>(rows, m, pitch, n, alt, miss);
+    k_bed_qc<<<(unsigned)g, kBlock, 0, s>>>(rows, m, pitch, n, min_maf, max_miss, alt, miss, maf, keep);
     LAUNCH_CHECK();
 }
 
-void launch_compact_rows(hipStream_t s, const uint8_t* src, int64_t pitch_src, const int64_t* idx,
-                         int64_t m_kept, int32_t n, uint8_t* dst, int64_t pitch_dst) {
-    int64_t g = m_kept < 65536 ? m_kept : 65536;
+void launch_keep_scan(hipStream_t s, const uint8_t* keep, int64_t cnt, int64_t* base, int64_t* pos) {
+    if (cnt < 1) return;
+    k_keep_scan<<<1, kScanThreads, 0, s>>>(keep, cnt, base, pos);
+    LAUNCH_CHECK();
+}
+
+void launch_compact_pos(hipStream_t s, const uint8_t* src, int64_t pitch_src, const int64_t* pos,
+                        int64_t cnt, int32_t n, uint8_t* dst, int64_t pitch_dst) {
+    int64_t g = cnt < 65536 ? cnt : 65536;
     if (g < 1) return;
-    k_compact_rows<<<(unsigned)g, kBlock, 0, s>>>(src, pitch_src, idx, m_kept, n, dst, pitch_dst);
+    k_compact_pos<<<(unsigned)g, kBlock, 0, s>>>(src, pitch_src, pos, cnt, n, dst, pitch_dst);
     LAUNCH_CHECK();
 }
 

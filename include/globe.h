@@ -31,7 +31,8 @@ typedef enum {
     GLOBE_E_STATE = 2,    /* call order (no cohort / genetics / variance components) */
     GLOBE_E_DEVICE = 3,   /* HIP runtime error or no usable device */
     GLOBE_E_NOMEM = 4,    /* device allocation failed */
-    GLOBE_E_NUMERIC = 5   /* singular system */
+    GLOBE_E_NUMERIC = 5,  /* singular system */
+    GLOBE_E_IO = 6        /* input file missing or unreadable */
 } globe_status;
 
 /* globe_vx modes */
@@ -93,6 +94,15 @@ int globe_set_cohort(globe_ctx* ctx, int64_t nrec, int32_t n, int32_t p,
  * the reference's FP32 MAF / missing-rate rule and kept resident in two layouts. */
 int globe_load_bed(globe_ctx* ctx, const uint8_t* rows, int64_t m, int64_t pitch,
                    float min_maf, float max_miss, int64_t* m_kept);
+/* Replaces prepGenotypeInput(geno_prefix, ...) (PCGsolver.cpp:615-763) at file level:
+ * M = lines of <prefix>.bim, <prefix>.bed checked (magic 6C 1B 01, at least
+ * 3 + M ceil(N/4) bytes) and streamed to the device through pinned double-buffered
+ * chunks (parallel pread, async copies) with the QC counts, keep rule and compaction
+ * on the device. When <prefix>.fam exists its line count must equal N (the reference
+ * never reads it). *m_total = M, *m_kept = SNPs kept (either may be NULL).
+ * GLOBE_E_IO when a file is missing or unreadable. */
+int globe_load_bed_file(globe_ctx* ctx, const char* prefix, float min_maf, float max_miss,
+                        int64_t* m_total, int64_t* m_kept);
 /* Per input SNP of the last globe_load_bed / globe_synth_bed: A1 allele count, missing
  * count, keep flag and FP32 A1 frequency (PCGsolver.cpp:711-734). Any pointer may be NULL. */
 int globe_get_qc(globe_ctx* ctx, int64_t* alt, int64_t* miss, uint8_t* keep, float* maf);
@@ -261,6 +271,12 @@ int globe_dev_free(globe_ctx* ctx, void* ptr);
 int globe_memcpy_h2d(globe_ctx* ctx, void* dst, const void* src, int64_t bytes);
 int globe_memcpy_d2h(globe_ctx* ctx, void* dst, const void* src, int64_t bytes);
 int globe_synchronize(globe_ctx* ctx);
+/* Bytes of .bed rows per ingest pipeline chunk (default 256 MiB); tests use small
+ * chunks to cross chunk boundaries. */
+int globe_set_ingest_chunk(globe_ctx* ctx, int64_t bytes);
+/* The last host-sourced genotype ingest: seconds in total, seconds filling pinned
+ * buffers on the host, seconds the host waited for a buffer, chunks, host threads. */
+int globe_ingest_stats(globe_ctx* ctx, double out[5]);
 /* Plain device-to-device copy kernel (16-byte vector loads / stores); the reference
  * point for HBM byte counters (bytes a multiple of 16). */
 int globe_dev_copy(globe_ctx* ctx, void* dst, const void* src, int64_t bytes);
