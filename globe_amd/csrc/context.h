@@ -48,8 +48,14 @@ struct globe_ctx {
     // operator workspaces
     globe::DevBuf w_S0, w_S1, w_GZ, w_P1, w_U, w_P2, w_T, w_C, w_part, w_part2, w_part3, w_part4;
     globe::DevBuf w_mix;         // mixed-precision contraction scratch
-    // tuned contraction variant per shape key -> (rows per wave, split-K)
-    std::map<std::string, std::pair<int, int>> tuned;
+    // tuned contraction variant per shape key: block shape, partial-sum slots and (thin
+    // stream-K) grid size
+    struct Tuned {
+        int cfg, parts;
+        int64_t grid;
+    };
+    std::map<std::string, Tuned> tuned;
+    int cu_count = 0;  // compute units of the device (stream-K grids)
 
     // PCG workspaces and state
     globe::DevBuf p_state, p_r, p_z, p_p, p_vp;

@@ -90,121 +90,185 @@ __device__ inline void block_coords(int64_t b, int64_t R, int S, int64_t& rb, in
 
 // ---------------------------------------------------------------- thin kernel
 // W waves x RT row fragments of 16 rows (16 W RT rows per block), one column fragment
-// of 16 digits. A stage is one super-step (256 genotypes = 64 packed bytes of a row):
-// the wave's own RT row fragments (lane l: row l % 16, 16 bytes at 16 (l / 16)) and the
-// stage's 4 digit chunks (waves 0-3 one each; with W = 8 the other waves copy into a
-// private sink so that every wave issues the same number of copies) are copied
-// global -> LDS into a ring of NS slots, NS - 1 stages ahead; one wait + barrier per
-// stage (the digit chunks are shared by the block). Row blocks are read without bounds
-// checks: the packed layouts are padded to a multiple of 512 rows.
-template <int W, int RT>
+// of 16 digits. A stage is SS super-steps (256 SS genotypes = 64 SS packed bytes of a
+// row): for each of the wave's RT row fragments, SS copies (lane l: row l % 16, the 16
+// bytes at 64 h + 16 (l / 16) of super-step h), and the stage's 4 SS digit chunks of
+// 1 KB (spread over the waves; with more waves than chunks, one each for the first
+// waves) go global -> LDS
+// into a ring of NS slots, NS - 1 stages ahead; one wait + barrier per stage (the digit
+// chunks are shared by the block). With SS = 2 the two 64-byte halves of every 128-byte
+// line of a row are requested back to back: measured on MI355X, 16-row fragments read
+// 64 bytes per row at a time stream at 5.2 TB/s, 128 bytes at a time at 6.1 TB/s. Row
+// blocks are read without bounds checks: the packed layouts are padded to a multiple of
+// 512 rows.
+template <int W, int RT, int SS>
 struct ThinCfg {
-    static constexpr int kA = W * RT * 1024;  // packed rows of a stage
-    static constexpr int kB = 4 * 1024;       // digits of a stage (4 K-steps)
+    static constexpr int kRowKB = W * RT * SS;           // packed rows of a stage
+    static constexpr int kDigKB = 4 * SS;                // digits of a stage (4 K-steps per super-step)
+    static constexpr int kA = kRowKB * 1024;
+    static constexpr int kB = kDigKB * 1024;
     static constexpr int kSlot = kA + kB;
-    static constexpr int kSink = W > 4 ? (W - 4) * 1024 : 0;
-    static constexpr int kOps = RT + 1;       // copies per wave per stage
-    static_assert(W == 4 || W == 8, "4 or 8 waves");
+    // digit chunks per wave: spread evenly, or one each for waves < kDigKB when there
+    // are more waves than chunks (the other waves copy only rows)
+    static constexpr int kDigPerWave = kDigKB >= W ? kDigKB / W : 1;
+    static constexpr int kOps = RT * SS + kDigPerWave;  // copies per stage, digit waves
+    static constexpr int kOpsRowOnly = RT * SS;         // copies per stage, other waves
+    static_assert(W == 4 || W == 8 || W == 16, "4, 8 or 16 waves");
+    static_assert(SS == 1 || SS == 2, "1 or 2 super-steps per stage");
+    static_assert(kDigKB < W || kDigKB % W == 0, "digit chunks must spread evenly");
 };
 
-template <int W, int RT>
+template <int W, int RT, int SS>
 __device__ inline void thin_issue(const uint8_t* __restrict__ arow, int64_t frag_stride,
                                   const uint8_t* __restrict__ bstage, uint8_t* __restrict__ slot,
-                                  uint8_t* __restrict__ sink, int wave, int lane) {
-    using C = ThinCfg<W, RT>;
-#pragma unroll
-    for (int q = 0; q < RT; ++q) {
-        const int f = wave * RT + q;
-        __builtin_amdgcn_global_load_lds((const void*)(arow + q * frag_stride),
-                                         (lds_void*)(slot + f * 1024), 16, 0, 0);
-    }
-    if (wave < 4)
-        __builtin_amdgcn_global_load_lds((const void*)(bstage + wave * 1024 + lane * 16),
-                                         (lds_void*)(slot + C::kA + wave * 1024), 16, 0, 0);
-    else
-        __builtin_amdgcn_global_load_lds((const void*)(bstage + (wave & 3) * 1024 + lane * 16),
-                                         (lds_void*)(sink + (wave - 4) * 1024), 16, 0, 0);
-}
-
-template <int W, int RT>
-__device__ inline void thin_mma(const uint8_t* __restrict__ slot, int wave, int lane,
-                                v4i (&acc)[RT]) {
-    using C = ThinCfg<W, RT>;
-    uint4 av[RT];
+                                  int wave, int lane) {
+    using C = ThinCfg<W, RT, SS>;
 #pragma unroll
     for (int q = 0; q < RT; ++q)
-        av[q] = *reinterpret_cast<const uint4*>(slot + (wave * RT + q) * 1024 + lane * 16);
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const v4i bv = *reinterpret_cast<const v4i*>(slot + C::kA + s * 1024 + lane * 16);
+        for (int h = 0; h < SS; ++h) {
+            const int f = (wave * RT + q) * SS + h;
+            __builtin_amdgcn_global_load_lds((const void*)(arow + q * frag_stride + 64 * h),
+                                             (lds_void*)(slot + f * 1024), 16, 0, 0);
+        }
+    if constexpr (C::kDigKB >= W) {
 #pragma unroll
-        for (int q = 0; q < RT; ++q)
-            acc[q] = __builtin_amdgcn_mfma_i32_16x16x64_i8(decode16(dword_of(av[q], s)), bv, acc[q], 0, 0, 0);
+        for (int d = 0; d < C::kDigPerWave; ++d) {
+            const int ch = wave * C::kDigPerWave + d;
+            __builtin_amdgcn_global_load_lds((const void*)(bstage + ch * 1024 + lane * 16),
+                                             (lds_void*)(slot + C::kA + ch * 1024), 16, 0, 0);
+        }
+    } else {
+        if (wave < C::kDigKB)
+            __builtin_amdgcn_global_load_lds((const void*)(bstage + wave * 1024 + lane * 16),
+                                             (lds_void*)(slot + C::kA + wave * 1024), 16, 0, 0);
     }
 }
 
-template <int W, int RT, int NS>
-__device__ inline void thin_stage_barrier() {
-    constexpr int kAhead = (NS - 2) * ThinCfg<W, RT>::kOps;
-    static_assert(kAhead < 64, "vmcnt field");
-    asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(kAhead) : "memory");
+template <int W, int RT, int SS>
+__device__ inline void thin_mma(const uint8_t* __restrict__ slot, int wave, int lane,
+                                v4i (&acc)[RT]) {
+    using C = ThinCfg<W, RT, SS>;
+#pragma unroll
+    for (int h = 0; h < SS; ++h) {
+        uint4 av[RT];
+#pragma unroll
+        for (int q = 0; q < RT; ++q)
+            av[q] = *reinterpret_cast<const uint4*>(slot + ((wave * RT + q) * SS + h) * 1024 + lane * 16);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const v4i bv = *reinterpret_cast<const v4i*>(slot + C::kA + (4 * h + s) * 1024 + lane * 16);
+#pragma unroll
+            for (int q = 0; q < RT; ++q)
+                acc[q] = __builtin_amdgcn_mfma_i32_16x16x64_i8(decode16(dword_of(av[q], s)), bv, acc[q], 0, 0, 0);
+        }
+    }
 }
 
-template <int W, int RT, int NS>
+// Wait until this wave's copies of the stage about to be used have landed (all but the
+// copies of the NS - 2 later stages in flight), then barrier: after it, every wave's
+// copies of the stage are visible in LDS. Waves that copy no digits issue fewer copies
+// per stage, so their count is smaller.
+template <int W, int RT, int NS, int SS>
+__device__ inline void thin_stage_barrier(int wave) {
+    using C = ThinCfg<W, RT, SS>;
+    constexpr int kAhead = (NS - 2) * C::kOps;
+    constexpr int kAheadRowOnly = (NS - 2) * C::kOpsRowOnly;
+    static_assert(kAhead < 64, "vmcnt field");
+    if (C::kDigKB >= W || wave < C::kDigKB)
+        asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(kAhead) : "memory");
+    else
+        asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(kAheadRowOnly) : "memory");
+}
+
+// Stream-K over the units u = r n_st + t (row block r of 16 W RT rows, stage t of its
+// n_st stages), u < U = R n_st: block b takes units [b U / G, (b + 1) U / G) as one
+// continuous ring of stages, so every block streams the same number of bytes and no
+// round of blocks runs the chip partly idle. A row block is covered by consecutive
+// blocks: the one holding its stage 0 writes partial slot 0, the next slot 1, ...
+// (slot = b - first block of the row block), and the block that ends a row block zeroes
+// the slots above its own up to P - 1, so the epilogue adds exactly P integer slots for
+// every row and the sum does not depend on G.
+__device__ inline int64_t sk_block_of(int64_t u, int64_t U, int64_t G) {
+    return ((u + 1) * G - 1) / U;  // the b with b U / G <= u < (b + 1) U / G
+}
+
+template <int W, int RT, int NS, int SS>
 __global__ __launch_bounds__(W * 64) void k_geno_thin(const uint8_t* __restrict__ rows, int64_t pitch,
-                                                      int64_t nrows, int64_t n_ss, int64_t ss_per_split,
-                                                      int64_t R, int S, const uint8_t* __restrict__ Bq,
+                                                      int64_t nrows, int64_t n_st, int64_t U, int64_t G,
+                                                      int P, const uint8_t* __restrict__ Bq,
                                                       int32_t* __restrict__ out, const int* skip) {
     if (skip && *skip) return;
-    using C = ThinCfg<W, RT>;
-    __shared__ __attribute__((aligned(1024))) uint8_t lds[NS * C::kSlot + C::kSink];
-    int64_t rb;
-    int sp;
-    block_coords(blockIdx.x, R, S, rb, sp);
+    using C = ThinCfg<W, RT, SS>;
+    __shared__ __attribute__((aligned(1024))) uint8_t lds[NS * C::kSlot];
+    const int64_t b = blockIdx.x;
+    const int64_t lo = b * U / G, hi = (b + 1) * U / G;
+    const int64_t nt = hi - lo;
+    if (nt <= 0) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, rr = lane & 15;
-    const int64_t row_base = rb * (16 * W * RT) + wave * (16 * RT);
-    const int64_t ss0 = (int64_t)sp * ss_per_split;
-    const int64_t ss1 = ss0 + ss_per_split < n_ss ? ss0 + ss_per_split : n_ss;
-    const int64_t nt = ss1 > ss0 ? ss1 - ss0 : 0;
-    const uint8_t* arow = rows + (row_base + rr) * pitch + 16 * g + 64 * ss0;
+    constexpr int64_t kRB = 16 * W * RT;  // rows per row block
+    constexpr int kRowStage = 64 * SS;    // packed bytes of a row per stage
+    const int64_t lane_off = (int64_t)(wave * 16 * RT + rr) * pitch + 16 * g;
     const int64_t fstride = 16 * pitch;
-    const uint8_t* bst = Bq + ss0 * C::kB;
-    uint8_t* sink = lds + NS * C::kSlot;
+
+    // issue cursor (row block, stage); it stops at the block's last unit, so the last
+    // NS - 1 issues repeat that unit into slots that are never read
+    int64_t ir = lo / n_st, it = lo % n_st, issued = 0;
+    auto issue_next = [&](int slot) {
+        thin_issue<W, RT, SS>(rows + ir * kRB * pitch + lane_off + kRowStage * it, fstride,
+                              Bq + it * C::kB, lds + slot * C::kSlot, wave, lane);
+        if (issued + 1 < nt) {
+            ++issued;
+            if (++it == n_st) {
+                it = 0;
+                ++ir;
+            }
+        }
+    };
+    // consume cursor
+    int64_t cr = lo / n_st, ct = lo % n_st;
 
     v4i acc[RT];
 #pragma unroll
     for (int a = 0; a < RT; ++a) acc[a] = v4i{0, 0, 0, 0};
-    if (nt > 0) {
 #pragma unroll
-        for (int u = 0; u < NS - 1; ++u) {
-            const int64_t t = u < nt ? u : nt - 1;
-            thin_issue<W, RT>(arow + 64 * t, fstride, bst + t * C::kB, lds + u * C::kSlot, sink, wave,
-                              lane);
-        }
-        for (int64_t t0 = 0; t0 < nt; t0 += NS) {
+    for (int u = 0; u < NS - 1; ++u) issue_next(u);
+    int cs = 0;  // ring slot being consumed
+    for (int64_t i = 0; i < nt; ++i) {
+        thin_stage_barrier<W, RT, NS, SS>(wave);
+        issue_next(cs == 0 ? NS - 1 : cs - 1);
+        __builtin_amdgcn_sched_barrier(0);
+        thin_mma<W, RT, SS>(lds + cs * C::kSlot, wave, lane, acc);
+        cs = cs + 1 == NS ? 0 : cs + 1;
+        const bool row_end = ++ct == n_st;
+        if (row_end || i == nt - 1) {
+            // this block's piece of row block cr is complete
+            const int64_t slot = b - sk_block_of(cr * n_st, U, G);
+            const int64_t r0 = cr * kRB + wave * 16 * RT + 4 * g;
 #pragma unroll
-            for (int u = 0; u < NS; ++u) {
-                const int64_t t = t0 + u;
-                if (t >= nt) break;
-                thin_stage_barrier<W, RT, NS>();
-                const int64_t tn = t + NS - 1 < nt ? t + NS - 1 : nt - 1;
-                thin_issue<W, RT>(arow + 64 * tn, fstride, bst + tn * C::kB,
-                                  lds + ((u + NS - 1) % NS) * C::kSlot, sink, wave, lane);
-                __builtin_amdgcn_sched_barrier(0);
-                thin_mma<W, RT>(lds + u * C::kSlot, wave, lane, acc);
+            for (int a = 0; a < RT; ++a)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int64_t r = r0 + a * 16 + v;
+                    if (r < nrows) {
+                        out[(slot * nrows + r) * 16 + rr] = acc[a][v];
+                        if (row_end)
+                            for (int64_t z = slot + 1; z < P; ++z) out[(z * nrows + r) * 16 + rr] = 0;
+                    }
+                }
+#pragma unroll
+            for (int a = 0; a < RT; ++a) acc[a] = v4i{0, 0, 0, 0};
+            // the stores count in vmcnt: let them (and the copies in flight) complete so
+            // that the per-stage wait counts stay exact
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            if (row_end) {
+                ct = 0;
+                ++cr;
             }
         }
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     }
-    int32_t* o = out + (int64_t)sp * nrows * 16;
-#pragma unroll
-    for (int a = 0; a < RT; ++a)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int64_t r = row_base + a * 16 + 4 * g + v;
-            if (r < nrows) o[r * 16 + rr] = acc[a][v];
-        }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
 // ---------------------------------------------------------------- wide kernel
@@ -603,8 +667,9 @@ int choose_split(int64_t R, int64_t n_ss, bool wide) {
 }
 
 struct Variant {
-    int cfg;  // index into kThin (thin) or kWide (wide)
-    int S;    // split-K
+    int cfg;     // index into kThin (thin) or kWide (wide)
+    int S;       // partial-sum slots: split-K (wide), pieces per row block (thin)
+    int64_t G;   // thin: stream-K grid (blocks); wide: 0
 };
 
 // wide block shapes: waves, ring stages, column groups, row fragments per wave
@@ -612,11 +677,54 @@ struct WideShape {
     int W, NS, CG, RA;
 };
 constexpr WideShape kWide[4] = {{8, 4, 1, 2}, {4, 3, 1, 2}, {8, 4, 2, 4}, {4, 3, 2, 4}};
-// thin block shapes: waves, row fragments of 16 rows per wave, ring stages
+// thin block shapes: waves, row fragments of 16 rows per wave, ring stages, super-steps
+// per stage
 struct ThinShape {
-    int W, RT, NS;
+    int W, RT, NS, SS;
 };
-constexpr ThinShape kThin[4] = {{4, 4, 3}, {4, 4, 4}, {8, 2, 3}, {8, 2, 4}};
+constexpr int kNumThin = 8;
+constexpr ThinShape kThin[kNumThin] = {{4, 4, 3, 1},  {8, 2, 3, 2},  {8, 2, 4, 2}, {16, 1, 3, 2},
+                                       {16, 1, 4, 2}, {8, 1, 5, 2}, {4, 4, 4, 2}, {8, 1, 6, 2}};
+
+template <int Q>
+void launch_thin_q(hipStream_t s, const Variant& v, const uint8_t* rows, int64_t pitch, int64_t nrows,
+                   int64_t nss, int64_t R, const uint8_t* Bq, int32_t* out, const int* skip) {
+    constexpr ThinShape t = kThin[Q];
+    const int64_t nst = nss / t.SS;
+    k_geno_thin<t.W, t.RT, t.NS, t.SS><<<(unsigned)v.G, t.W * 64, 0, s>>>(rows, pitch, nrows, nst, R * nst,
+                                                                         v.G, v.S, Bq, out, skip);
+}
+
+template <int Q>
+int thin_blocks_per_cu() {
+    constexpr ThinShape t = kThin[Q];
+    int nb = 0;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_geno_thin<t.W, t.RT, t.NS, t.SS>,
+                                                           t.W * 64, 0));
+    return nb;
+}
+
+int thin_occupancy(int cfg) {
+    switch (cfg) {
+        case 0: return thin_blocks_per_cu<0>();
+        case 1: return thin_blocks_per_cu<1>();
+        case 2: return thin_blocks_per_cu<2>();
+        case 3: return thin_blocks_per_cu<3>();
+        case 4: return thin_blocks_per_cu<4>();
+        case 5: return thin_blocks_per_cu<5>();
+        case 6: return thin_blocks_per_cu<6>();
+        default: return thin_blocks_per_cu<7>();
+    }
+}
+
+// Pieces per row block of the stream-K split: the most blocks any row block spans.
+int thin_pieces(int64_t R, int64_t nst, int64_t G) {
+    const int64_t U = R * nst;
+    auto block_of = [&](int64_t u) { return ((u + 1) * G - 1) / U; };
+    int64_t P = 1;
+    for (int64_t r = 0; r < R; ++r) P = std::max(P, block_of(r * nst + nst - 1) - block_of(r * nst) + 1);
+    return (int)P;
+}
 
 int64_t rows_per_block(bool wide, int cfg) {
     if (!wide) return (int64_t)16 * kThin[cfg].W * kThin[cfg].RT;
@@ -663,11 +771,17 @@ void launch_variant(hipStream_t s, bool wide, int JT, const Variant& v, const ui
     const int64_t per = (nss + v.S - 1) / v.S;
     const unsigned grid = (unsigned)(R * v.S);
     if (!wide) {
+        require(nss % kThin[v.cfg].SS == 0 && v.G >= 1 && v.G <= R * (nss / kThin[v.cfg].SS),
+                GLOBE_E_INVALID, "thin shape needs whole stages per row and at most one block per unit");
         switch (v.cfg) {
-            case 0: k_geno_thin<4, 4, 3><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
-            case 1: k_geno_thin<4, 4, 4><<<grid, 256, 0, s>>>(rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
-            case 2: k_geno_thin<8, 2, 3><<<grid, 512, 0, s>>>(rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
-            default: k_geno_thin<8, 2, 4><<<grid, 512, 0, s>>>(rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
+            case 0: launch_thin_q<0>(s, v, rows, pitch, nrows, nss, R, Bq, out, skip); break;
+            case 1: launch_thin_q<1>(s, v, rows, pitch, nrows, nss, R, Bq, out, skip); break;
+            case 2: launch_thin_q<2>(s, v, rows, pitch, nrows, nss, R, Bq, out, skip); break;
+            case 3: launch_thin_q<3>(s, v, rows, pitch, nrows, nss, R, Bq, out, skip); break;
+            case 4: launch_thin_q<4>(s, v, rows, pitch, nrows, nss, R, Bq, out, skip); break;
+            case 5: launch_thin_q<5>(s, v, rows, pitch, nrows, nss, R, Bq, out, skip); break;
+            case 6: launch_thin_q<6>(s, v, rows, pitch, nrows, nss, R, Bq, out, skip); break;
+            default: launch_thin_q<7>(s, v, rows, pitch, nrows, nss, R, Bq, out, skip); break;
         }
     } else if (JT == 1) {
         launch_wide<1>(s, v, grid, rows, pitch, nrows, nss, per, R, Bq, out, skip);
@@ -687,22 +801,34 @@ void launch_variant(hipStream_t s, bool wide, int JT, const Variant& v, const ui
 // changes a result. Candidates: the thin block shapes of kThin, the wide block shapes of
 // kWide (the RA = 4 ones need an even JT), each with split-K S/2, S, 2 S (and 4 S for
 // thin) around choose_split().
-static std::vector<Variant> candidates(bool wide, int JT, int64_t nrows, int64_t nss) {
+static std::vector<Variant> candidates(bool wide, int JT, int64_t nrows, int64_t nss, int ncu) {
     std::vector<Variant> v;
-    std::vector<int> cfgs;
     if (wide) {
-        for (int q = 0; q < 4; ++q)
-            if (JT % kWide[q].CG == 0) cfgs.push_back(q);
-    } else {
-        cfgs = {0, 1, 2, 3};
+        for (int cfg = 0; cfg < 4; ++cfg) {
+            if (JT % kWide[cfg].CG != 0) continue;
+            const int64_t R = (nrows + rows_per_block(true, cfg) - 1) / rows_per_block(true, cfg);
+            const int S0 = choose_split(R, nss, true);
+            for (int S : {S0 / 2, S0, S0 * 2}) {
+                if (S < 1 || S > 64 || (S > 1 && nss / S < 4)) continue;
+                v.push_back({cfg, S, 0});
+            }
+        }
+        return v;
     }
-    for (int cfg : cfgs) {
-        const int64_t R = (nrows + rows_per_block(wide, cfg) - 1) / rows_per_block(wide, cfg);
-        const int S0 = choose_split(R, nss, wide);
-        for (int S : {S0 / 2, S0, S0 * 2, S0 * 4}) {
-            if (wide && S == S0 * 4) continue;
-            if (S < 1 || S > 64 || (S > 1 && nss / S < 4)) continue;
-            v.push_back({cfg, S});
+    for (int cfg = 0; cfg < kNumThin; ++cfg) {
+        if (nss % kThin[cfg].SS != 0) continue;
+        const int64_t R = (nrows + rows_per_block(false, cfg) - 1) / rows_per_block(false, cfg);
+        const int64_t U = R * (nss / kThin[cfg].SS);
+        const int64_t resident = (int64_t)ncu * std::max(1, thin_occupancy(cfg));
+        // grids that fill the chip in exactly 1, 2 or 4 rounds, and one or two blocks
+        // per row block (rounds as the hardware dispatches them)
+        for (int64_t G : {resident, 2 * resident, 4 * resident, R, 2 * R}) {
+            if (G < 1 || G > U) continue;
+            const int P = thin_pieces(R, nss / kThin[cfg].SS, G);
+            if (P > 64) continue;
+            bool dup = false;
+            for (const Variant& w : v) dup |= w.cfg == cfg && w.G == G;
+            if (!dup) v.push_back({cfg, P, G});
         }
     }
     return v;
@@ -724,8 +850,14 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     // the largest split any candidate may use bounds the partial-sum buffer
     const size_t bq1 = (size_t)Kpad1 * J16, bq2 = (size_t)Kpad2 * J16;
     int smax1 = 1, smax2 = 1;
-    for (const Variant& v : candidates(wide, JT, m, nss1)) smax1 = std::max(smax1, v.S);
-    for (const Variant& v : candidates(wide, JT, n, nss2)) smax2 = std::max(smax2, v.S);
+    if (c->cu_count == 0) {
+        hipDeviceProp_t prop;
+        HIP_CHECK(hipGetDeviceProperties(&prop, c->device));
+        c->cu_count = prop.multiProcessorCount;
+    }
+    const int ncu = c->cu_count;
+    for (const Variant& v : candidates(wide, JT, m, nss1, ncu)) smax1 = std::max(smax1, v.S);
+    for (const Variant& v : candidates(wide, JT, n, nss2, ncu)) smax2 = std::max(smax2, v.S);
     const size_t ob = std::max((size_t)smax1 * m, (size_t)smax2 * n) * J16 * 4;
     const size_t bq = std::max(bq1, bq2);
     c->w_mix.ensure(round_up(bq, 256) + round_up(ob, 256) + 2 * kMaxCols * sizeof(int));
@@ -749,7 +881,7 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
                                 std::to_string(Kpad) + ":" + std::to_string(J16);
         auto it = c->tuned.find(key);
         if (it == c->tuned.end()) {
-            const std::vector<Variant> cand = candidates(wide, JT, nrows, nss);
+            const std::vector<Variant> cand = candidates(wide, JT, nrows, nss, ncu);
             hipEvent_t e0, e1;
             HIP_CHECK(hipEventCreate(&e0));
             HIP_CHECK(hipEventCreate(&e1));
@@ -763,8 +895,8 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
                 float ms = 0.f;
                 HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
                 if (std::getenv("GLOBE_TUNE_LOG"))
-                    std::fprintf(stderr, "[globe tune] %s shape %d split %d: %.3f ms\n",
-                                 key.c_str(), cand[q].cfg, cand[q].S, ms);
+                    std::fprintf(stderr, "[globe tune] %s shape %d parts %d grid %lld: %.3f ms\n",
+                                 key.c_str(), cand[q].cfg, cand[q].S, (long long)cand[q].G, ms);
                 if (ms < best) {
                     best = ms;
                     bi = (int)q;
@@ -772,9 +904,9 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
             }
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
-            it = c->tuned.emplace(key, std::make_pair(cand[bi].cfg, cand[bi].S)).first;
+            it = c->tuned.emplace(key, globe_ctx::Tuned{cand[bi].cfg, cand[bi].S, cand[bi].G}).first;
         }
-        const Variant v{it->second.first, it->second.second};
+        const Variant v{it->second.cfg, it->second.parts, it->second.grid};
         if (c->prof_on) {
             globe_ctx::ProfEvent e{};
             HIP_CHECK(hipEventCreate(&e.a));
