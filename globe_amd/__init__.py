@@ -114,6 +114,7 @@ def load_library(path: str = LIB_PATH):
         "globe_pcg_dev": ([P, P, i64, C.c_int, P, i64, P, P, P, P], C.c_int),
         "globe_gls_blup": ([P, P, P, P, P, P, P, P], C.c_int),
         "globe_aireml_eval": ([P, P, P, P, P, C.c_int, P, P, P, P, P], C.c_int),
+        "globe_aireml_eval_reduced": ([P, P, P, P, P, C.c_int, P, P, P, P, P], C.c_int),
         "globe_dev_alloc": ([P, i64, C.POINTER(P)], C.c_int),
         "globe_dev_free": ([P, P], C.c_int),
         "globe_memcpy_h2d": ([P, P, P, i64], C.c_int),
@@ -127,7 +128,9 @@ def load_library(path: str = LIB_PATH):
         "globe_profile_enable": ([P, C.c_int], C.c_int),
         "globe_profile_read": ([P, C.c_int, P], C.c_int),
         "globe_aireml_step": ([P, P, P, dbl, P], C.c_int),
+        "globe_aireml_step_reduced": ([P, P, P, dbl, P], C.c_int),
         "globe_aireml_fit": ([P, P, P, P, P, P, P, P, P, P], C.c_int),
+        "globe_aireml_fit_reduced": ([P, P, P, P, P, P, P, P, P, P], C.c_int),
         "globe_grm_from_geno": ([P], C.c_int),
         "globe_get_grm": ([P, P], C.c_int),
         "globe_serve_open": ([P, P], C.c_int),
@@ -355,20 +358,25 @@ class Globe:
                 "stalled": st.stalled, "seconds": st.seconds}
 
     def aireml_eval(self, y, probes, weights, probe_x0=None, *, tol=1e-5, max_iter=1000,
-                    precision=PREC_FP64):
+                    precision=PREC_FP64, reduced=False):
+        """computeDerivative_first + estimateTrace_PdV + computeDerivative_AImat with
+        injected probes (varCompEst.cpp:980-1041); reduced=True: the two-component model
+        of varCompEst_reducedModel_noTime.cpp (:592-705)."""
         y = np.ascontiguousarray(y, dtype=np.float64)
         probes = _cm(probes)
         weights = _cm(weights)
         x0 = None if probe_x0 is None else _cm(probe_x0)
         nmc = probes.shape[1]
-        grad = np.zeros(5)
-        ai = np.zeros(25)
-        first = np.zeros(5)
-        tr = np.zeros(5)
+        nv = 2 if reduced else 5
+        grad = np.zeros(nv)
+        ai = np.zeros(nv * nv)
+        first = np.zeros(nv)
+        tr = np.zeros(nv)
         o = PcgOpts(tol, max_iter, X0_PRECOND, RULE_ALL_COLUMNS, 0, precision, 0)
-        self._check(self.L.globe_aireml_eval(self.h, _p(y), _p(probes), _p(weights), _p(x0), nmc,
-                                             C.byref(o), _p(grad), _p(ai), _p(first), _p(tr)))
-        return {"grad": grad, "ai": ai.reshape(5, 5), "first": first, "trace": tr}
+        fn = self.L.globe_aireml_eval_reduced if reduced else self.L.globe_aireml_eval
+        self._check(fn(self.h, _p(y), _p(probes), _p(weights), _p(x0), nmc,
+                       C.byref(o), _p(grad), _p(ai), _p(first), _p(tr)))
+        return {"grad": grad, "ai": ai.reshape(nv, nv), "first": first, "trace": tr}
 
     def geno_Wa(self, a):
         a = np.ascontiguousarray(a, dtype=np.float64)
@@ -464,18 +472,27 @@ class Globe:
 
     # ---- AI-REML (runAIREML)
     def aireml_fit(self, y, theta0, *, max_iter=50, tol=1e-3, num_mc=15, probes="gaussian",
-                   seed=20261203, pcg_tol=1e-5, precision=PREC_FP64, log=None):
+                   seed=20261203, pcg_tol=1e-5, precision=PREC_FP64, log=None, reduced=False):
+        """runAIREML (varCompEst.cpp:1347-1500): theta0 has five components; reduced=True:
+        runAIREML_reducedModel (varCompEst_reducedModel_noTime.cpp:967-1099), theta0 =
+        (sig2e, sig2g). probes: "gaussian" (the reference's tilde-V probes), "rademacher"
+        or "exact" (unit probes; num_mc must equal the record count)."""
         y = np.ascontiguousarray(y, dtype=np.float64)
-        o = AiremlOpts(max_iter, tol, num_mc, 0 if probes == "gaussian" else 1, seed,
+        kind = {"gaussian": 0, "rademacher": 1, "exact": 2}[probes]
+        o = AiremlOpts(max_iter, tol, num_mc, kind, seed,
                        PcgOpts(pcg_tol, 1000, X0_PRECOND, RULE_ALL_COLUMNS, 0, precision, 0))
+        nv = 2 if reduced else 5
         th = np.ascontiguousarray(theta0, dtype=np.float64).copy()
-        se = np.zeros(5)
-        ai = np.zeros(25)
+        if th.shape != (nv,):
+            raise ValueError(f"theta0 must have {nv} components")
+        se = np.zeros(nv)
+        ai = np.zeros(nv * nv)
         it = C.c_int32()
         spi = C.c_double()
         lines = []
         cb = LOG_FN(lambda s, u: (lines.append(s.decode()), log(s.decode()) if log else None))
-        self._check(self.L.globe_aireml_fit(self.h, _p(y), C.byref(o), _p(th), _p(se), _p(ai),
-                                            C.byref(it), C.byref(spi), cb, None))
-        return {"theta": th, "se": se, "ai": ai.reshape(5, 5), "iters": it.value,
+        fn = self.L.globe_aireml_fit_reduced if reduced else self.L.globe_aireml_fit
+        self._check(fn(self.h, _p(y), C.byref(o), _p(th), _p(se), _p(ai),
+                       C.byref(it), C.byref(spi), cb, None))
+        return {"theta": th, "se": se, "ai": ai.reshape(nv, nv), "iters": it.value,
                 "sec_per_iter": spi.value, "log": lines}

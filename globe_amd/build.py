@@ -1,4 +1,5 @@
-"""Build libglobe_amd.so and the drop-in CLIs (runPCGsolver, runAIREML) for gfx950.
+"""Build libglobe_amd.so and the drop-in CLIs (runPCGsolver, runAIREML,
+runAIREML_reducedModel) for gfx950.
 
 Every HIP source is compiled with hipcc for gfx950 only, with packed FP32
 instructions disabled; objects go to globe_amd/_build/, the shared library and
@@ -24,7 +25,8 @@ ARCH = "gfx950"
 LIB_SOURCES = ["k_record.hip", "k_geno.hip", "k_grm.hip", "k_mfma.hip", "api.cpp", "aireml.cpp",
                "serve.cpp", "ingest.cpp"]
 CLI_COMMON = ["io.cpp"]
-CLIS = {"runPCGsolver": "cli_pcgsolver.cpp", "runAIREML": "cli_aireml.cpp"}
+CLIS = {"runPCGsolver": "cli_pcgsolver.cpp", "runAIREML": "cli_aireml.cpp",
+        "runAIREML_reducedModel": "cli_aireml_reduced.cpp"}
 
 DEVICE_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

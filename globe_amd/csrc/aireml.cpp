@@ -1,5 +1,12 @@
 // AI-REML outer loop and trust-region step (host FP64; the derivatives, trace and
-// AI matrix come from globe_aireml_eval on the device).
+// AI matrix come from the device, aireml_eval_impl). Two models share the code:
+//   full    -- five components (sig2e, sig2g, sig2b0, sigb01, sig2b1) of the
+//              longitudinal model (varCompEst.cpp);
+//   reduced -- two components (sig2e, sig2g), V = sig2g A + sig2e I with one record per
+//              individual and no time term (varCompEst_reducedModel_noTime.cpp). On the
+//              device it is the full operator with sig2b0 = sigb01 = sig2b1 = 0 and zero
+//              times, whose per-individual block preconditioner is then the diagonal
+//              1 / (sig2g A_ii + sig2e) of computePreconditioner (:326-335).
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -17,34 +24,33 @@ using globe::kMaxCols;
 
 namespace {
 
-constexpr int K5 = 5;
+constexpr int KMAX = 5;  // components of the full model; matrices use this stride
 
-double quad(const double* g, const double* A, const double* p) {
-    double gp = 0.0, pap = 0.0;
-    for (int i = 0; i < K5; ++i) {
-        gp += g[i] * p[i];
-        for (int j = 0; j < K5; ++j) pap += p[i] * A[i * K5 + j] * p[j];
-    }
-    return -(gp - 0.5 * pap);
-}
-
+// Trust-region subproblem of oneIter_AIREML: minimise q(p) = -(g.p - p^T A p / 2) over the
+// box lb <= p <= ub subject to the general constraints c_j(p) <= 0:
+//   j = 0: sum_i A_ii p_i - delta (the reference's signed "trust region", varCompEst.cpp
+//          :1068-1089, reduced :731-757);
+//   j = 1: (full model) (th3+p3)^2 - (th2+p2)(th4+p4) + 1e-3 (covariance, :1121-1145).
+// Positivity (:1096-1115, reduced :759-781) is part of the box.
 struct StepProblem {
-    double g[K5], A[K5 * K5], th[K5], delta;
-    double lb[K5], ub[K5];
-    // general constraints c_j(p) <= 0: j = 0 trust region (signed), j = 1 covariance
+    int nv = KMAX;
+    bool cov = true;
+    double g[KMAX], A[KMAX * KMAX], th[KMAX], delta;
+    double lb[KMAX], ub[KMAX];
+    int ncons() const { return cov ? 2 : 1; }
     double cons(int j, const double* p) const {
         if (j == 0) {
             double s = 0.0;
-            for (int i = 0; i < K5; ++i) s += A[i * K5 + i] * p[i];
+            for (int i = 0; i < nv; ++i) s += A[i * KMAX + i] * p[i];
             return s - delta;
         }
         const double b0 = th[2] + p[2], b01 = th[3] + p[3], b1 = th[4] + p[4];
         return b01 * b01 - b0 * b1 + 1e-3;
     }
     void cons_grad(int j, const double* p, double* gr) const {
-        for (int i = 0; i < K5; ++i) gr[i] = 0.0;
+        for (int i = 0; i < nv; ++i) gr[i] = 0.0;
         if (j == 0) {
-            for (int i = 0; i < K5; ++i) gr[i] = A[i * K5 + i];
+            for (int i = 0; i < nv; ++i) gr[i] = A[i * KMAX + i];
             return;
         }
         const double b0 = th[2] + p[2], b01 = th[3] + p[3], b1 = th[4] + p[4];
@@ -53,27 +59,37 @@ struct StepProblem {
         gr[4] = -b0;
     }
     bool feasible(const double* p, double tol) const {
-        for (int i = 0; i < K5; ++i)
+        for (int i = 0; i < nv; ++i)
             if (p[i] < lb[i] - tol || p[i] > ub[i] + tol) return false;
-        return cons(0, p) <= tol * (1.0 + std::fabs(delta)) && cons(1, p) <= tol;
+        if (cons(0, p) > tol * (1.0 + std::fabs(delta))) return false;
+        return !cov || cons(1, p) <= tol;
+    }
+    double quad(const double* p) const {
+        double gp = 0.0, pap = 0.0;
+        for (int i = 0; i < nv; ++i) {
+            gp += g[i] * p[i];
+            for (int j = 0; j < nv; ++j) pap += p[i] * A[i * KMAX + j] * p[j];
+        }
+        return -(gp - 0.5 * pap);
     }
 };
 
 // Augmented Lagrangian value and gradient (box handled by projection).
 double al_value(const StepProblem& P, const double* p, const double* lam, double mu, double* grad) {
-    double v = quad(P.g, P.A, p);
-    for (int i = 0; i < K5; ++i) {
+    const int nv = P.nv;
+    double v = P.quad(p);
+    for (int i = 0; i < nv; ++i) {
         double ap = 0.0;
-        for (int j = 0; j < K5; ++j) ap += P.A[i * K5 + j] * p[j];
+        for (int j = 0; j < nv; ++j) ap += P.A[i * KMAX + j] * p[j];
         grad[i] = -(P.g[i] - ap);
     }
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < P.ncons(); ++j) {
         const double t = P.cons(j, p) + lam[j] / mu;
         if (t > 0.0) {
             v += 0.5 * mu * t * t - lam[j] * lam[j] / (2.0 * mu);
-            double cg[K5];
+            double cg[KMAX];
             P.cons_grad(j, p, cg);
-            for (int i = 0; i < K5; ++i) grad[i] += mu * t * cg[i];
+            for (int i = 0; i < nv; ++i) grad[i] += mu * t * cg[i];
         } else {
             v -= lam[j] * lam[j] / (2.0 * mu);
         }
@@ -82,45 +98,46 @@ double al_value(const StepProblem& P, const double* p, const double* lam, double
 }
 
 void project(const StepProblem& P, double* p) {
-    for (int i = 0; i < K5; ++i) p[i] = std::fmin(P.ub[i], std::fmax(P.lb[i], p[i]));
+    for (int i = 0; i < P.nv; ++i) p[i] = std::fmin(P.ub[i], std::fmax(P.lb[i], p[i]));
 }
 
 // Hessian of the augmented Lagrangian at p (constraints with c_j + lam_j / mu > 0).
 void al_hessian(const StepProblem& P, const double* p, const double* lam, double mu, double* H) {
-    std::memcpy(H, P.A, sizeof(double) * K5 * K5);
-    for (int j = 0; j < 2; ++j) {
+    const int nv = P.nv;
+    std::memcpy(H, P.A, sizeof(double) * KMAX * KMAX);
+    for (int j = 0; j < P.ncons(); ++j) {
         const double t = P.cons(j, p) + lam[j] / mu;
         if (t <= 0.0) continue;
-        double cg[K5];
+        double cg[KMAX];
         P.cons_grad(j, p, cg);
-        for (int a = 0; a < K5; ++a)
-            for (int b = 0; b < K5; ++b) H[a * K5 + b] += mu * cg[a] * cg[b];
+        for (int a = 0; a < nv; ++a)
+            for (int b = 0; b < nv; ++b) H[a * KMAX + b] += mu * cg[a] * cg[b];
         if (j == 1) {  // second derivatives of (b01^2 - b0 b1)
-            H[3 * K5 + 3] += mu * t * 2.0;
-            H[2 * K5 + 4] -= mu * t;
-            H[4 * K5 + 2] -= mu * t;
+            H[3 * KMAX + 3] += mu * t * 2.0;
+            H[2 * KMAX + 4] -= mu * t;
+            H[4 * KMAX + 2] -= mu * t;
         }
     }
 }
 
 // Solve H_FF d_F = -g_F on the free set by Cholesky, adding a Levenberg shift until
 // H_FF + shift I is positive definite.
-bool newton_dir(const double* H, const double* g, const bool* fr, double* d) {
-    int idx[K5], m = 0;
-    for (int i = 0; i < K5; ++i) {
+bool newton_dir(int nv, const double* H, const double* g, const bool* fr, double* d) {
+    int idx[KMAX], m = 0;
+    for (int i = 0; i < nv; ++i) {
         d[i] = 0.0;
         if (fr[i]) idx[m++] = i;
     }
     if (m == 0) return false;
     double scale = 0.0;
-    for (int a = 0; a < m; ++a) scale = std::fmax(scale, std::fabs(H[idx[a] * K5 + idx[a]]));
+    for (int a = 0; a < m; ++a) scale = std::fmax(scale, std::fabs(H[idx[a] * KMAX + idx[a]]));
     double shift = 0.0;
     for (int attempt = 0; attempt < 40; ++attempt) {
-        double L[K5][K5] = {};
+        double L[KMAX][KMAX] = {};
         bool ok = true;
         for (int a = 0; a < m && ok; ++a)
             for (int b = 0; b <= a; ++b) {
-                double s = H[idx[a] * K5 + idx[b]] + (a == b ? shift : 0.0);
+                double s = H[idx[a] * KMAX + idx[b]] + (a == b ? shift : 0.0);
                 for (int q = 0; q < b; ++q) s -= L[a][q] * L[b][q];
                 if (a == b) {
                     if (s <= 0.0) { ok = false; break; }
@@ -130,7 +147,7 @@ bool newton_dir(const double* H, const double* g, const bool* fr, double* d) {
                 }
             }
         if (ok) {
-            double y[K5];
+            double y[KMAX];
             for (int a = 0; a < m; ++a) {
                 double s = -g[idx[a]];
                 for (int q = 0; q < a; ++q) s -= L[a][q] * y[q];
@@ -149,34 +166,35 @@ bool newton_dir(const double* H, const double* g, const bool* fr, double* d) {
 }
 
 bool solve_al(const StepProblem& P, double* p) {
+    const int nv = P.nv;
     double lam[2] = {0.0, 0.0}, mu = 10.0;
     project(P, p);
     double prev_viol = 1e300;
     for (int outer = 0; outer < 60; ++outer) {
         // projected Newton on the box-constrained augmented Lagrangian
-        double gr[K5];
+        double gr[KMAX];
         double f = al_value(P, p, lam, mu, gr);
         for (int it = 0; it < 500; ++it) {
-            bool fr[K5];
+            bool fr[KMAX];
             double pg = 0.0;
-            for (int i = 0; i < K5; ++i) {
+            for (int i = 0; i < nv; ++i) {
                 const bool at_lo = p[i] <= P.lb[i] + 1e-14 && gr[i] > 0.0;
                 const bool at_hi = p[i] >= P.ub[i] - 1e-14 && gr[i] < 0.0;
                 fr[i] = !(at_lo || at_hi);
                 if (fr[i]) pg = std::fmax(pg, std::fabs(gr[i]));
             }
             if (pg < 1e-13) break;
-            double H[K5 * K5], d[K5];
+            double H[KMAX * KMAX], d[KMAX];
             al_hessian(P, p, lam, mu, H);
-            if (!newton_dir(H, gr, fr, d)) break;
-            double a = 1.0, pn[K5], grn[K5], fn = f;
+            if (!newton_dir(nv, H, gr, fr, d)) break;
+            double a = 1.0, pn[KMAX], grn[KMAX], fn = f;
             bool ok = false;
             for (int bt = 0; bt < 60; ++bt) {
-                for (int i = 0; i < K5; ++i) pn[i] = p[i] + a * d[i];
+                for (int i = 0; i < nv; ++i) pn[i] = p[i] + a * d[i];
                 project(P, pn);
                 fn = al_value(P, pn, lam, mu, grn);
                 double dec = 0.0;
-                for (int i = 0; i < K5; ++i) dec += gr[i] * (p[i] - pn[i]);
+                for (int i = 0; i < nv; ++i) dec += gr[i] * (p[i] - pn[i]);
                 if (fn <= f - 1e-4 * std::fabs(dec) || fn < f) {
                     ok = true;
                     break;
@@ -185,16 +203,16 @@ bool solve_al(const StepProblem& P, double* p) {
             }
             if (!ok) break;
             double step = 0.0;
-            for (int i = 0; i < K5; ++i) step = std::fmax(step, std::fabs(pn[i] - p[i]));
-            std::memcpy(p, pn, sizeof(pn));
-            std::memcpy(gr, grn, sizeof(grn));
+            for (int i = 0; i < nv; ++i) step = std::fmax(step, std::fabs(pn[i] - p[i]));
+            std::memcpy(p, pn, sizeof(double) * nv);
+            std::memcpy(gr, grn, sizeof(double) * nv);
             f = fn;
             if (step < 1e-16) break;
         }
         // stop at a KKT point: feasible and multipliers no longer moving (complementary
         // slackness), otherwise update the multipliers and, if feasibility stalls, mu
         double viol = 0.0, dlam = 0.0, lmax = 0.0;
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < P.ncons(); ++j) {
             const double cj = P.cons(j, p);
             viol = std::fmax(viol, cj / (j == 0 ? 1.0 + std::fabs(P.delta) : 1.0));
             const double nl = std::fmax(0.0, lam[j] + mu * cj);
@@ -210,30 +228,94 @@ bool solve_al(const StepProblem& P, double* p) {
     return P.feasible(p, 1e-9);
 }
 
-bool invert5(const double* a, double* out) {
-    double w[K5][2 * K5];
-    for (int i = 0; i < K5; ++i)
-        for (int j = 0; j < K5; ++j) {
-            w[i][j] = a[i * K5 + j];
-            w[i][K5 + j] = i == j;
+// Gauss-Jordan inverse of an n x n matrix (n <= 5) stored with row stride `ld`.
+bool invert_n(int n, const double* a, int ld, double* out, int ldo) {
+    double w[KMAX][2 * KMAX];
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            w[i][j] = a[i * ld + j];
+            w[i][n + j] = i == j;
         }
-    for (int c = 0; c < K5; ++c) {
+    for (int c = 0; c < n; ++c) {
         int piv = c;
-        for (int r = c + 1; r < K5; ++r)
+        for (int r = c + 1; r < n; ++r)
             if (std::fabs(w[r][c]) > std::fabs(w[piv][c])) piv = r;
         if (w[piv][c] == 0.0) return false;
-        for (int j = 0; j < 2 * K5; ++j) std::swap(w[c][j], w[piv][j]);
+        for (int j = 0; j < 2 * n; ++j) std::swap(w[c][j], w[piv][j]);
         const double d = w[c][c];
-        for (int j = 0; j < 2 * K5; ++j) w[c][j] /= d;
-        for (int r = 0; r < K5; ++r)
+        for (int j = 0; j < 2 * n; ++j) w[c][j] /= d;
+        for (int r = 0; r < n; ++r)
             if (r != c) {
                 const double f = w[r][c];
-                for (int j = 0; j < 2 * K5; ++j) w[r][j] -= f * w[c][j];
+                for (int j = 0; j < 2 * n; ++j) w[r][j] -= f * w[c][j];
             }
     }
-    for (int i = 0; i < K5; ++i)
-        for (int j = 0; j < K5; ++j) out[i * K5 + j] = w[i][K5 + j];
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) out[i * ldo + j] = w[i][n + j];
     return true;
+}
+
+// The step of oneIter_AIREML for nv components (5: full, 2: reduced). ai is nv x nv
+// row-major.
+int step_impl(int nv, const double* grad, const double* ai, const double* theta, double delta,
+              double* step) {
+    StepProblem P;
+    P.nv = nv;
+    P.cov = nv == KMAX;
+    for (int i = 0; i < nv; ++i) {
+        P.g[i] = grad[i];
+        P.th[i] = theta[i];
+        for (int j = 0; j < nv; ++j) P.A[i * KMAX + j] = ai[i * nv + j];
+    }
+    P.delta = delta;
+    for (int i = 0; i < nv; ++i) {
+        P.ub[i] = 10.0;
+        // positivity for every component but the covariance sigb01 of the full model
+        P.lb[i] = (P.cov && i == 3) ? -10.0 : std::fmax(-10.0, 1e-5 - theta[i]);
+    }
+    double Ai[KMAX * KMAX], p[KMAX];
+    if (invert_n(nv, P.A, KMAX, Ai, KMAX)) {
+        for (int i = 0; i < nv; ++i) {
+            p[i] = 0.0;
+            for (int j = 0; j < nv; ++j) p[i] += Ai[i * KMAX + j] * grad[j];
+        }
+        bool spd = true;  // the Newton point minimises q only when AI is positive definite
+        for (int i = 0; i < nv; ++i) spd &= P.A[i * KMAX + i] > 0.0;
+        if (spd && P.feasible(p, 0.0)) {
+            std::memcpy(step, p, sizeof(double) * nv);
+            return GLOBE_OK;
+        }
+    } else {
+        for (int i = 0; i < nv; ++i) p[i] = 0.0;
+    }
+    // The covariance constraint is not convex, so local minimisers can differ. NLopt
+    // starts from the zero step (:1163); the zero step and three points along the
+    // (projected) Newton direction are tried here and the best feasible point is kept.
+    double starts[4][KMAX];
+    for (int i = 0; i < nv; ++i) {
+        starts[0][i] = 0.0;
+        starts[1][i] = std::fmin(P.ub[i], std::fmax(P.lb[i], p[i]));
+        starts[2][i] = 0.5 * starts[1][i];
+        starts[3][i] = 0.1 * starts[1][i];
+    }
+    bool found = false;
+    double best[KMAX], fbest = 0.0;
+    for (int s = 0; s < 4; ++s) {
+        double q[KMAX];
+        std::memcpy(q, starts[s], sizeof(double) * nv);
+        for (int i = 0; i < nv; ++i)
+            if (!std::isfinite(q[i])) q[i] = 0.0;
+        if (!solve_al(P, q)) continue;
+        const double f = P.quad(q);
+        if (!found || f < fbest) {
+            found = true;
+            fbest = f;
+            std::memcpy(best, q, sizeof(double) * nv);
+        }
+    }
+    if (!found) return GLOBE_E_NUMERIC;
+    std::memcpy(step, best, sizeof(double) * nv);
+    return GLOBE_OK;
 }
 
 struct Logger {
@@ -264,64 +346,24 @@ uint64_t mix(uint64_t z) {
 extern "C" int globe_aireml_step(const double grad[5], const double ai[25], const double theta[5],
                                  double delta, double step[5]) {
     if (!grad || !ai || !theta || !step) return GLOBE_E_INVALID;
-    StepProblem P;
-    std::memcpy(P.g, grad, sizeof(P.g));
-    std::memcpy(P.A, ai, sizeof(P.A));
-    std::memcpy(P.th, theta, sizeof(P.th));
-    P.delta = delta;
-    for (int i = 0; i < K5; ++i) {
-        P.ub[i] = 10.0;
-        P.lb[i] = i == 3 ? -10.0 : std::fmax(-10.0, 1e-5 - theta[i]);
-    }
-    double Ai[K5 * K5], p[K5];
-    if (invert5(ai, Ai)) {
-        for (int i = 0; i < K5; ++i) {
-            p[i] = 0.0;
-            for (int j = 0; j < K5; ++j) p[i] += Ai[i * K5 + j] * grad[j];
-        }
-        bool spd = true;  // the Newton point minimises q only when AI is positive definite
-        for (int i = 0; i < K5; ++i) spd &= ai[i * K5 + i] > 0.0;
-        if (spd && P.feasible(p, 0.0)) {
-            std::memcpy(step, p, sizeof(p));
-            return GLOBE_OK;
-        }
-    }
-    // The covariance constraint is not convex, so local minimisers can differ. NLopt
-    // starts from the zero step (:1163); the zero step and two points along the
-    // (projected) Newton direction are tried here and the best feasible point is kept.
-    double starts[4][K5];
-    for (int i = 0; i < K5; ++i) {
-        starts[0][i] = 0.0;
-        starts[1][i] = std::fmin(P.ub[i], std::fmax(P.lb[i], p[i]));
-        starts[2][i] = 0.5 * starts[1][i];
-        starts[3][i] = 0.1 * starts[1][i];
-    }
-    bool found = false;
-    double best[K5], fbest = 0.0;
-    for (int s = 0; s < 4; ++s) {
-        double q[K5];
-        std::memcpy(q, starts[s], sizeof(q));
-        for (int i = 0; i < K5; ++i)
-            if (!std::isfinite(q[i])) q[i] = 0.0;
-        if (!solve_al(P, q)) continue;
-        const double f = quad(P.g, P.A, q);
-        if (!found || f < fbest) {
-            found = true;
-            fbest = f;
-            std::memcpy(best, q, sizeof(q));
-        }
-    }
-    if (!found) return GLOBE_E_NUMERIC;
-    std::memcpy(step, best, sizeof(best));
-    return GLOBE_OK;
+    return step_impl(KMAX, grad, ai, theta, delta, step);
+}
+
+extern "C" int globe_aireml_step_reduced(const double grad[2], const double ai[4],
+                                         const double theta[2], double delta, double step[2]) {
+    if (!grad || !ai || !theta || !step) return GLOBE_E_INVALID;
+    return step_impl(2, grad, ai, theta, delta, step);
 }
 
 namespace {
 
 // Probes and trace weights for one gradient evaluation.
-//   Gaussian tilde-V (varCompEst.cpp:775-862): y ~ N(0, tilde-V); weights = x0 =
-//   tilde-V^-1 y by the per-individual 2x2 Woodbury solve on the host.
-//   Rademacher: z in {-1, +1}; weights = z; PCG starts from Q^-1 z.
+//   0, Gaussian tilde-V (varCompEst.cpp:775-862; reduced :563-573, :592-602): y ~ N(0,
+//      tilde-V); weights = x0 = tilde-V^-1 y by the per-individual 2x2 Woodbury solve on
+//      the host.
+//   1, Rademacher: z in {-1, +1}; weights = z; PCG starts from Q^-1 z.
+//   2, exact: the unit vectors sqrt(nrec) e_s, s < nrec (nmc must equal nrec); weights =
+//      probe, so the mean of the per-probe terms is the exact trace.
 struct Probes {
     std::vector<double> y, w;
     bool has_x0 = false;
@@ -331,9 +373,14 @@ Probes make_probes(globe_ctx* c, const double th[5], int nmc, int kind, uint64_t
     Probes P;
     const int64_t nrec = c->nrec;
     const int32_t n = c->n;
-    P.y.resize((size_t)nrec * nmc);
-    P.w.resize((size_t)nrec * nmc);
+    P.y.assign((size_t)nrec * nmc, 0.0);
+    P.w.assign((size_t)nrec * nmc, 0.0);
     const double se = th[0], sg = th[1], sb0 = th[2], sb01 = th[3], sb1 = th[4];
+    if (kind == 2) {
+        globe::require(nmc == nrec, GLOBE_E_INVALID, "exact probes need numMC equal to the record count");
+        for (int s = 0; s < nmc; ++s) P.y[(size_t)s * nrec + s] = P.w[(size_t)s * nrec + s] = std::sqrt((double)nrec);
+        return P;
+    }
     for (int s = 0; s < nmc; ++s) {
         std::mt19937_64 gen(mix(seed ^ (0x1000193ull * (uint64_t)(s + 1))));
         double* ys = P.y.data() + (size_t)s * nrec;
@@ -344,8 +391,11 @@ Probes make_probes(globe_ctx* c, const double th[5], int nmc, int kind, uint64_t
             continue;
         }
         std::normal_distribution<double> nd(0.0, 1.0);
-        const double l00 = std::sqrt(sb0), l10 = sb01 / l00;
-        double d11 = sb1 - sb01 * sb01 / sb0;
+        // Cholesky factor of B (compute_chol_2x2, :800-813); with sig2b0 = 0 (the reduced
+        // model) the slope terms vanish
+        const double l00 = sb0 > 0.0 ? std::sqrt(sb0) : 0.0;
+        const double l10 = sb0 > 0.0 ? sb01 / l00 : 0.0;
+        double d11 = sb1 - (sb0 > 0.0 ? sb01 * sb01 / sb0 : 0.0);
         if (d11 < 0) d11 = 0;  // clipped as compute_chol_2x2 does (:806-810)
         const double l11 = std::sqrt(d11);
         std::vector<double> z1(n), z2(2 * (size_t)n);
@@ -383,44 +433,62 @@ Probes make_probes(globe_ctx* c, const double th[5], int nmc, int kind, uint64_t
 }
 
 struct Eval {
-    double grad[5], ai[25];
+    double grad[KMAX], ai[KMAX * KMAX];  // grad (nv), AI nv x nv row-major
 };
 
-void evaluate(globe_ctx* c, const double* y, const double th[5], const globe_aireml_opts& o,
+// theta of nv components -> the five of the device operator
+void full_theta(int nv, const double* th, double* th5) {
+    for (int j = 0; j < KMAX; ++j) th5[j] = j < nv ? th[j] : 0.0;
+}
+
+void evaluate(globe_ctx* c, const double* y, int nv, const double* th, const globe_aireml_opts& o,
               uint64_t eval_id, bool want_ai, Eval& e) {
-    int rc = globe_set_varcomp(c, th);
+    double th5[KMAX];
+    full_theta(nv, th, th5);
+    int rc = globe_set_varcomp(c, th5);
     if (rc != GLOBE_OK) throw Error(rc, c->err);
-    Probes P = make_probes(c, th, o.num_mc, o.probes, mix(o.seed + 0x9E37ull * (eval_id + 1)));
+    Probes P = make_probes(c, th5, o.num_mc, o.probes, mix(o.seed + 0x9E37ull * (eval_id + 1)));
     globe_pcg_opts po = o.pcg;
     rc = globe::aireml_eval_impl(c, y, P.y.data(), P.w.data(), P.has_x0 ? P.w.data() : nullptr,
-                                 o.num_mc, &po, want_ai, e.grad, e.ai, nullptr, nullptr);
+                                 o.num_mc, &po, want_ai, e.grad, e.ai, nullptr, nullptr, nv);
     if (rc != GLOBE_OK) throw Error(rc, c->err);
 }
 
-double norm5(const double* v) {
+double norm_n(int nv, const double* v) {
     double s = 0;
-    for (int i = 0; i < K5; ++i) s += v[i] * v[i];
+    for (int i = 0; i < nv; ++i) s += v[i] * v[i];
     return std::sqrt(s);
 }
 
-}  // namespace
+const char* const kNames[KMAX] = {"sig2e", "sig2g", "sig2b0", "sigb01", "sig2b1"};
 
-extern "C" int globe_aireml_fit(globe_ctx* c, const double* y, const globe_aireml_opts* opts,
-                                double theta[5], double se_out[5], double ai_out[25],
-                                int32_t* iters_out, double* sec_per_iter, globe_log_fn logfn,
-                                void* user) {
-    if (!c) return GLOBE_E_INVALID;
-    if (!y || !opts || !theta) {
-        c->err = "globe_aireml_fit: null argument";
-        return GLOBE_E_INVALID;
+std::string theta_line(const char* head, int nv, const double* th) {
+    std::string s = head;
+    char buf[64];
+    for (int j = 0; j < nv; ++j) {
+        std::snprintf(buf, sizeof(buf), "%s%s = %g", j ? ", " : "  ", kNames[j], th[j]);
+        s += buf;
     }
+    return s;
+}
+
+// runAIREML's main loop + oneIter_AIREML (varCompEst.cpp:1148-1491; reduced model
+// varCompEst_reducedModel_noTime.cpp:784-1099) for nv components.
+int fit_impl(globe_ctx* c, const double* y, const globe_aireml_opts* opts, int nv, double* theta,
+             double* se_out, double* ai_out, int32_t* iters_out, double* sec_per_iter,
+             globe_log_fn logfn, void* user) {
+    const bool full = nv == KMAX;
     try {
         const globe_aireml_opts& o = *opts;
         Logger log{logfn, user};
         globe::require(o.num_mc >= 1 && o.num_mc <= kMaxCols, GLOBE_E_INVALID,
                        "numMC must be in [1, 64]");
-        double th[5];
-        std::memcpy(th, theta, sizeof(th));
+        globe::require(o.probes >= 0 && o.probes <= 2, GLOBE_E_INVALID, "unknown probe kind");
+        if (!full)
+            globe::require(c->has_cohort && c->nrec == c->n, GLOBE_E_INVALID,
+                           "the reduced model takes exactly one record per individual");
+        double th[KMAX];
+        std::memcpy(th, theta, sizeof(double) * nv);
         Eval cur{};
         double Delta = 1e20;
         bool dangerous = false, grad_check = true;
@@ -431,48 +499,55 @@ extern "C" int globe_aireml_fit(globe_ctx* c, const double* y, const globe_airem
         double total_secs = 0.0;
         int it_done = 0;
         log("Running AI-REML...");
-        log("Initial variance components:  sig2e = %g, sig2g = %g, sig2b0 = %g, sigb01 = %g, sig2b1 = %g",
-            th[0], th[1], th[2], th[3], th[4]);
+        log("%s", theta_line("Initial variance components:", nv, th).c_str());
         for (int it = 0; it < o.max_iter; ++it) {
             log("---------- Iteration %d ----------", it + 1);
             const auto t0 = std::chrono::steady_clock::now();
-            if (it == 0) evaluate(c, y, th, o, eval_id++, true, cur);
+            if (it == 0) evaluate(c, y, nv, th, o, eval_id++, true, cur);
             if (it == 1) {
-                gnt = 0.1 * norm5(cur.grad);
+                gnt = 0.1 * norm_n(nv, cur.grad);
                 log("Gradient norm threshold: %g", gnt);
             }
             if (num_rej > 5) {
-                // random restart in [0.1, 0.9] (varCompEst.cpp:1431-1451), seeded here
+                // random restart in [0.1, 0.9] (varCompEst.cpp:1431-1451, reduced :1044-1057),
+                // seeded here
                 log("Too many rejections at current point. Re-initalizing optimization.");
                 std::uniform_real_distribution<double> u(0.0, 1.0);
-                for (int j = 0; j < 5; ++j) th[j] = 0.1 + 0.8 * u(restart_rng);
-                if (th[3] > std::sqrt(th[2] * th[4])) th[3] = std::sqrt(th[2] * th[4]) - 1e-3;
-                evaluate(c, y, th, o, eval_id++, true, cur);
+                for (int j = 0; j < nv; ++j) th[j] = 0.1 + 0.8 * u(restart_rng);
+                if (full && th[3] > std::sqrt(th[2] * th[4])) th[3] = std::sqrt(th[2] * th[4]) - 1e-3;
+                evaluate(c, y, nv, th, o, eval_id++, true, cur);
                 num_rej = 0;
             }
-            const double th_old[5] = {th[0], th[1], th[2], th[3], th[4]};
-            // ---- oneIter_AIREML (varCompEst.cpp:1148-1344)
-            double p[5];
-            int rc = globe_aireml_step(cur.grad, cur.ai, th, Delta, p);
+            double th_old[KMAX];
+            std::memcpy(th_old, th, sizeof(double) * nv);
+            // ---- oneIter_AIREML (varCompEst.cpp:1148-1344, reduced :784-964)
+            double p[KMAX];
+            int rc = step_impl(nv, cur.grad, cur.ai, th, Delta, p);
             if (rc != GLOBE_OK) {
                 log("Trust-region step failed. Reducing trust region radius.");
                 Delta *= 0.75;
                 dangerous = true;
                 num_rej += 1;
             } else {
-                log("Predicted step: %g %g %g %g %g", p[0], p[1], p[2], p[3], p[4]);
+                std::string ps = "Predicted step:";
+                for (int j = 0; j < nv; ++j) {
+                    char b[32];
+                    std::snprintf(b, sizeof(b), " %g", p[j]);
+                    ps += b;
+                }
+                log("%s", ps.c_str());
                 double scale = 1.0;
                 bool accepted = false, converged_pred = false;
                 int red = 0;
                 while (!accepted && red <= 10) {
-                    double sp[5], tn[5];
-                    for (int j = 0; j < 5; ++j) {
+                    double sp[KMAX], tn[KMAX];
+                    for (int j = 0; j < nv; ++j) {
                         sp[j] = scale * p[j];
                         tn[j] = th[j] + sp[j];
                     }
                     bool violated = false;
-                    for (int j = 0; j < 5; ++j) {
-                        if (j != 3 && tn[j] < 1e-6) {
+                    for (int j = 0; j < nv; ++j) {
+                        if (!(full && j == 3) && tn[j] < 1e-6) {
                             tn[j] = 1e-6;
                             violated = true;
                         }
@@ -481,7 +556,7 @@ extern "C" int globe_aireml_fit(globe_ctx* c, const double* y, const globe_airem
                             violated = true;
                         }
                     }
-                    if (tn[3] > std::sqrt(tn[2] * tn[4])) {
+                    if (full && tn[3] > std::sqrt(tn[2] * tn[4])) {
                         tn[3] = std::sqrt(tn[2] * tn[4]) - 1e-6;
                         violated = true;
                     }
@@ -492,12 +567,12 @@ extern "C" int globe_aireml_fit(globe_ctx* c, const double* y, const globe_airem
                         continue;
                     }
                     Eval nw{};
-                    evaluate(c, y, tn, o, eval_id++, false, nw);
+                    evaluate(c, y, nv, tn, o, eval_id++, false, nw);
                     double dll = 0.0, pred = 0.0, spa = 0.0;
-                    for (int a = 0; a < 5; ++a) {
+                    for (int a = 0; a < nv; ++a) {
                         dll += sp[a] * 0.5 * (cur.grad[a] + nw.grad[a]);
                         pred += cur.grad[a] * sp[a];
-                        for (int b = 0; b < 5; ++b) spa += sp[a] * cur.ai[a * 5 + b] * sp[b];
+                        for (int b = 0; b < nv; ++b) spa += sp[a] * cur.ai[a * nv + b] * sp[b];
                     }
                     pred -= 0.5 * spa;
                     log("Predicted change in log-likelihood (scaled): %g", pred);
@@ -508,7 +583,8 @@ extern "C" int globe_aireml_fit(globe_ctx* c, const double* y, const globe_airem
                         converged_pred = true;
                         break;
                     }
-                    if (grad_check && norm5(cur.grad) > gnt && norm5(nw.grad) > 2 * norm5(cur.grad)) {
+                    if (grad_check && norm_n(nv, cur.grad) > gnt &&
+                        norm_n(nv, nw.grad) > 2 * norm_n(nv, cur.grad)) {
                         log("Dangerous model deviation detected at scale %g. Reducing step size.", scale);
                         scale *= 0.5;
                         red += 1;
@@ -517,17 +593,17 @@ extern "C" int globe_aireml_fit(globe_ctx* c, const double* y, const globe_airem
                     const double rho = dll / pred;
                     log("Rho: %g", rho);
                     double npd = 0.0;
-                    for (int j = 0; j < 5; ++j) npd += std::fabs(cur.ai[j * 5 + j] * sp[j]);
+                    for (int j = 0; j < nv; ++j) npd += std::fabs(cur.ai[j * nv + j] * sp[j]);
                     if (rho < 0.25) Delta = 0.25 * npd;
                     else if (rho > 0.99) Delta = std::fmax(Delta, 3.5 * npd);
                     if (rho > 0.25) {
-                        std::memcpy(th, tn, sizeof(th));
-                        std::memcpy(cur.grad, nw.grad, sizeof(cur.grad));
+                        std::memcpy(th, tn, sizeof(double) * nv);
+                        std::memcpy(cur.grad, nw.grad, sizeof(double) * nv);
                         // AI at the accepted point from the trial's fit and f_theta
                         // (computeDerivative_AImat, :1321-1330)
                         globe_pcg_opts po = o.pcg;
                         const int arc = globe::aireml_eval_impl(c, nullptr, nullptr, nullptr, nullptr, 0, &po,
-                                                                true, nullptr, cur.ai, nullptr, nullptr);
+                                                                true, nullptr, cur.ai, nullptr, nullptr, nv);
                         if (arc != GLOBE_OK) throw Error(arc, c->err);
                         num_rej = 0;
                         accepted = true;
@@ -544,14 +620,13 @@ extern "C" int globe_aireml_fit(globe_ctx* c, const double* y, const globe_airem
                     num_rej += 1;
                 }
             }
-            log("Estimated variance components:  sig2e = %g, sig2g = %g, sig2b0 = %g, sigb01 = %g, sig2b1 = %g",
-                th[0], th[1], th[2], th[3], th[4]);
+            log("%s", theta_line("Estimated variance components:", nv, th).c_str());
             const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             total_secs += secs;
             it_done = it + 1;
             log("Iteration %d took %g seconds.", it + 1, secs);
             double dn = 0.0, on = 0.0;
-            for (int j = 0; j < 5; ++j) {
+            for (int j = 0; j < nv; ++j) {
                 dn += (th[j] - th_old[j]) * (th[j] - th_old[j]);
                 on += th_old[j] * th_old[j];
             }
@@ -568,18 +643,17 @@ extern "C" int globe_aireml_fit(globe_ctx* c, const double* y, const globe_airem
                 break;
             }
         }
-        // SEs from the AI matrix at the final point (:1484-1491)
-        double Ai[25];
-        const bool inv_ok = invert5(cur.ai, Ai);
-        const char* nm[5] = {"sig2e", "sig2g", "sig2b0", "sigb01", "sig2b1"};
+        // SEs from the AI matrix at the final point (:1484-1491, reduced :1084-1091)
+        double Ai[KMAX * KMAX];
+        const bool inv_ok = invert_n(nv, cur.ai, nv, Ai, KMAX);
         log("Final variance components +/- SEs:");
-        for (int j = 0; j < 5; ++j) {
-            const double s = inv_ok ? std::sqrt(Ai[j * 5 + j]) : NAN;
+        for (int j = 0; j < nv; ++j) {
+            const double s = inv_ok ? std::sqrt(Ai[j * KMAX + j]) : NAN;
             if (se_out) se_out[j] = s;
-            log("  %6s = %g +/- %g", nm[j], th[j], s);
+            log("  %6s = %g +/- %g", kNames[j], th[j], s);
         }
-        std::memcpy(theta, th, sizeof(th));
-        if (ai_out) std::memcpy(ai_out, cur.ai, sizeof(cur.ai));
+        std::memcpy(theta, th, sizeof(double) * nv);
+        if (ai_out) std::memcpy(ai_out, cur.ai, sizeof(double) * nv * nv);
         if (iters_out) *iters_out = it_done;
         if (sec_per_iter) *sec_per_iter = it_done ? total_secs / it_done : 0.0;
         return GLOBE_OK;
@@ -590,4 +664,30 @@ extern "C" int globe_aireml_fit(globe_ctx* c, const double* y, const globe_airem
         c->err = e.what();
         return GLOBE_E_INVALID;
     }
+}
+
+}  // namespace
+
+extern "C" int globe_aireml_fit(globe_ctx* c, const double* y, const globe_aireml_opts* opts,
+                                double theta[5], double se_out[5], double ai_out[25],
+                                int32_t* iters_out, double* sec_per_iter, globe_log_fn logfn,
+                                void* user) {
+    if (!c) return GLOBE_E_INVALID;
+    if (!y || !opts || !theta) {
+        c->err = "globe_aireml_fit: null argument";
+        return GLOBE_E_INVALID;
+    }
+    return fit_impl(c, y, opts, KMAX, theta, se_out, ai_out, iters_out, sec_per_iter, logfn, user);
+}
+
+extern "C" int globe_aireml_fit_reduced(globe_ctx* c, const double* y, const globe_aireml_opts* opts,
+                                        double theta[2], double se_out[2], double ai_out[4],
+                                        int32_t* iters_out, double* sec_per_iter, globe_log_fn logfn,
+                                        void* user) {
+    if (!c) return GLOBE_E_INVALID;
+    if (!y || !opts || !theta) {
+        c->err = "globe_aireml_fit_reduced: null argument";
+        return GLOBE_E_INVALID;
+    }
+    return fit_impl(c, y, opts, 2, theta, se_out, ai_out, iters_out, sec_per_iter, logfn, user);
 }

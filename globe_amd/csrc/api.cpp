@@ -789,7 +789,19 @@ int globe_aireml_eval(globe_ctx* c, const double* y, const double* probes, const
         return GLOBE_E_INVALID;
     }
     return globe::aireml_eval_impl(c, y, probes, weights, probe_x0, nmc, opts, true, grad, ai,
-                                   first_terms, trace);
+                                   first_terms, trace, 5);
+}
+
+int globe_aireml_eval_reduced(globe_ctx* c, const double* y, const double* probes, const double* weights,
+                              const double* probe_x0, int nmc, const globe_pcg_opts* opts, double* grad,
+                              double* ai, double* first_terms, double* trace) {
+    if (!c) return GLOBE_E_INVALID;
+    if (!y || !grad || !ai) {
+        c->err = "globe_aireml_eval_reduced: null argument";
+        return GLOBE_E_INVALID;
+    }
+    return globe::aireml_eval_impl(c, y, probes, weights, probe_x0, nmc, opts, true, grad, ai,
+                                   first_terms, trace, 2);
 }
 
 }  // extern "C"
@@ -797,14 +809,25 @@ int globe_aireml_eval(globe_ctx* c, const double* y, const double* probes, const
 namespace globe {
 
 // computeDerivative_first + estimateTrace_PdV + computeDerivative_AImat
-// (varCompEst.cpp:980-1041). y == nullptr reuses the fit, f_theta and X^T V^-1 X of
-// the previous call at the same variance components (the AI matrix of an accepted
-// step, varCompEst.cpp:1321-1330, needs no new fit).
+// (varCompEst.cpp:980-1041; reduced model varCompEst_reducedModel_noTime.cpp:662-705).
+// y == nullptr reuses the fit, f_theta and X^T V^-1 X of the previous call at the same
+// variance components (the AI matrix of an accepted step, varCompEst.cpp:1321-1330,
+// needs no new fit). nvc = 5 (full model) or 2 (reduced model: sig2b0 = sigb01 =
+// sig2b1 = 0 and one record per individual, so V = sig2g A + sig2e I and f_theta has
+// the columns V^-1 r and A V^-1 r, compute_f_theta_i :524-548); grad, first_terms and
+// trace hold nvc values, ai nvc x nvc row-major.
 int aireml_eval_impl(globe_ctx* c, const double* y, const double* probes, const double* weights,
                      const double* probe_x0, int nmc, const globe_pcg_opts* opts, bool want_ai,
-                     double* grad, double* ai, double* first_terms, double* trace) {
+                     double* grad, double* ai, double* first_terms, double* trace, int nvc) {
     API_BEGIN(c)
     require(opts != nullptr, GLOBE_E_INVALID, "globe_aireml_eval: null options");
+    require(nvc == 5 || nvc == 2, GLOBE_E_INVALID, "2 or 5 variance components");
+    if (nvc == 2) {
+        require(c->has_cohort && c->nrec == c->n, GLOBE_E_INVALID,
+                "the reduced model takes exactly one record per individual");
+        require(c->theta[2] == 0.0 && c->theta[3] == 0.0 && c->theta[4] == 0.0, GLOBE_E_INVALID,
+                "the reduced model has no random intercept or slope (sig2b0 = sigb01 = sig2b1 = 0)");
+    }
     require(nmc >= 0 && nmc <= kMaxCols, GLOBE_E_INVALID, "nmc must be in [0, 64]");
     require(nmc == 0 || (probes && weights), GLOBE_E_INVALID, "probes/weights required");
     require(c->has_cohort && c->gen != 0, GLOBE_E_STATE, "cohort and genetics required");
@@ -877,21 +900,22 @@ int aireml_eval_impl(globe_ctx* c, const double* y, const double* probes, const 
         for (int j = 0; j < 5; ++j) tr[j] /= (double)nmc;
     }
     if (grad)
-        for (int j = 0; j < 5; ++j) grad[j] = 0.5 * (first[j] - tr[j]);
-    if (first_terms) std::copy(first, first + 5, first_terms);
-    if (trace) std::copy(tr, tr + 5, trace);
+        for (int j = 0; j < nvc; ++j) grad[j] = 0.5 * (first[j] - tr[j]);
+    if (first_terms) std::copy(first, first + nvc, first_terms);
+    if (trace) std::copy(tr, tr + nvc, trace);
     if (want_ai && ai) {
         // 4. AI matrix (varCompEst.cpp:1015-1041): PCG(f, x0 = Q^-1 f), P f, AI = f^T P f / 2
+        //    over the model's nvc columns of f_theta
         double* pf = c->e_pv.as<double>();
         globe_pcg_opts oa = o;
         oa.x0_mode = GLOBE_X0_PRECOND;
-        run_pcg(c, fth, nrec, 5, pf, nrec, oa, nullptr, nullptr, nullptr);
-        apply_P(c, f, pf, 5);
-        launch_gram(s, nrec, fth, nrec, 5, pf, nrec, 5, c->w_part.as<double>(), c->e_small.as<double>());
+        run_pcg(c, fth, nrec, nvc, pf, nrec, oa, nullptr, nullptr, nullptr);
+        apply_P(c, f, pf, nvc);
+        launch_gram(s, nrec, fth, nrec, nvc, pf, nrec, nvc, c->w_part.as<double>(), c->e_small.as<double>());
         double aim[25];
-        HIP_CHECK(hipMemcpyAsync(aim, c->e_small.p, sizeof(double) * 25, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipMemcpyAsync(aim, c->e_small.p, sizeof(double) * nvc * nvc, hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
-        for (int q = 0; q < 25; ++q) ai[q] = 0.5 * aim[q];
+        for (int q = 0; q < nvc * nvc; ++q) ai[q] = 0.5 * aim[q];
     }
     API_END(c)
 }

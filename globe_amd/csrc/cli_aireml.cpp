@@ -4,7 +4,7 @@
 //             [--numMC n] [--sig2e v] [--sig2g v] [--sig2b0 v] [--sigb01 v] [--sig2b1 v]
 // reads <grm_prefix>.grm.bin, tees everything it prints to <output_prefix>.log and
 // writes no other file, as the reference. Unknown flags are ignored (as :113-132).
-// Opt-in extensions: --probes gaussian|rademacher (default gaussian: the
+// Opt-in extensions: --probes gaussian|rademacher|exact (default gaussian: the
 // reference's tilde-V probes), --seed n, --pcgTol v (1e-5), --precision
 // fp64|mixed|fp64-valu, --device d, --geno <plinkPrefix> (packed genotypes instead of
 // the GRM; the reference's runAIREML only reads a GRM).
@@ -78,7 +78,8 @@ int main(int argc, char** argv) {
             const std::string v = next();
             if (v == "gaussian") o.probes = 0;
             else if (v == "rademacher") o.probes = 1;
-            else die("[Error] --probes must be gaussian or rademacher");
+            else if (v == "exact") o.probes = 2;
+            else die("[Error] --probes must be gaussian, rademacher or exact");
         } else if (a == "--precision") {
             const std::string v = next();
             if (v == "fp64") o.pcg.precision = GLOBE_PREC_FP64;

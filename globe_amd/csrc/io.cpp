@@ -71,6 +71,60 @@ Cohort read_input(const std::string& path) {
     return c;
 }
 
+Cohort read_input_reduced(const std::string& path) {
+    std::ifstream f(path);
+    if (!f.is_open()) throw std::runtime_error("Error opening input file: " + path);
+    std::string line;
+    std::getline(f, line);
+    std::istringstream hs(line);
+    std::string col;
+    std::vector<std::string> cols;
+    int ii = -1, pi = -1;
+    while (hs >> col) {
+        if (col == "INDIV") ii = (int)cols.size();
+        else if (col == "PHENO") pi = (int)cols.size();
+        cols.push_back(col);
+    }
+    if (ii < 0 || pi < 0) throw std::runtime_error("Error: Missing required columns in header.");
+    const int ncol = (int)cols.size();
+    const int ncov = ncol - 2;
+    if (ncov < 1) throw std::runtime_error("Error: at least one covariate column is required.");
+    Cohort c;
+    c.p = ncov - 1;
+    for (int q = 0; q < ncol; ++q)
+        if (q != ii && q != pi) c.cov_names.push_back(cols[q]);
+    std::vector<std::vector<float>> cov((size_t)ncov);
+    std::unordered_set<int> ids;
+    std::vector<float> yv;
+    while (std::getline(f, line)) {
+        std::istringstream is(line);
+        float indiv = -1.f, pheno = 0.f;
+        int cq = 0;
+        for (int q = 0; q < ncol; ++q) {
+            float v = 0.f;
+            is >> v;
+            if (q == ii) indiv = v;
+            else if (q == pi) pheno = v;
+            else cov[cq++].push_back(v);
+        }
+        if (!ids.insert((int)indiv).second)
+            throw std::runtime_error("Error: individual " + std::to_string((int)indiv) +
+                                     " appears twice; the reduced model takes one record per individual.");
+        yv.push_back(pheno);
+    }
+    c.nrec = (int64_t)yv.size();
+    c.n = (int32_t)c.nrec;
+    c.indiv.resize(c.nrec);
+    c.y.assign(yv.begin(), yv.end());
+    c.times.assign(c.nrec, 0.0);
+    c.X.resize((size_t)c.nrec * ncov);
+    for (int64_t r = 0; r < c.nrec; ++r) {
+        c.indiv[r] = (int32_t)r;
+        for (int q = 0; q < ncov; ++q) c.X[(size_t)q * c.nrec + r] = cov[q][r];
+    }
+    return c;
+}
+
 BedInfo check_bed(const std::string& prefix, int32_t n) {
     const std::string bim = prefix + ".bim", bed = prefix + ".bed", fam = prefix + ".fam";
     BedInfo out;

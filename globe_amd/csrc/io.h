@@ -20,6 +20,15 @@ struct Cohort {
 // float, as the reference does, then widened. Throws std::runtime_error.
 Cohort read_input(const std::string& path);
 
+// Whitespace table with header INDIV PHENO <covariates...> of the reduced model
+// (checkInputs / readInputData, varCompEst_reducedModel_noTime.cpp:131-248): one record
+// per individual, row r is GRM row r (the reference indexes the GRM by row and never
+// uses the INDIV values for V), so indiv[r] = r and times are 0; X holds every other
+// column (no TIME column; p + 1 = their count, at least one). INDIV values must be
+// distinct (the reference sizes its matrices by the distinct count). Throws
+// std::runtime_error.
+Cohort read_input_reduced(const std::string& path);
+
 // PLINK .bed (SNP-major, magic 6C 1B 01) + .bim line count (prepGenotypeInput,
 // PCGsolver.cpp:624-667), checked before the device is touched; when <prefix>.fam
 // exists its line count must equal n. The rows themselves are streamed by

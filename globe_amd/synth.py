@@ -189,6 +189,47 @@ def make_grm_case(n: int, m: int, lam: float, seed: int, f_lo=0.05, f_hi=0.5,
     return SynthData(cohort=c, y=y, grm_lt=lt, theta=theta)
 
 
+@dataclass
+class ReducedCase:
+    """Reduced-model cohort (varCompEst_reducedModel_noTime.cpp): one record per
+    individual in GRM order, covariates without a TIME column."""
+    y: np.ndarray          # float64 [n] (FP32-exact)
+    covars: np.ndarray     # float64 [n, p] (FP32-exact)
+    grm_lt: np.ndarray     # float32 [n(n+1)/2]
+    cov_names: list = field(default_factory=lambda: ["intercept", "sex", "age"])
+
+    @property
+    def n(self) -> int:
+        return int(self.y.shape[0])
+
+
+def make_reduced_case(n: int, m: int, seed: int, theta=(0.5, 0.5), f_lo=0.05, f_hi=0.5,
+                      beta=(1.0, 0.3, 0.01)) -> ReducedCase:
+    """y = X beta + u + e with u ~ N(0, sig2g G), e ~ N(0, sig2e I), G = W W^T / m."""
+    rng = np.random.default_rng(seed)
+    dos = simulate_dosages(n, m, rng, f_lo, f_hi, 0.0)
+    rows = pack_bed(dos)
+    _, maf, _, _, _ = qc_reference(rows, n, 0.0, 1.0)
+    W = standardized(rows, n, maf)
+    G = (W @ W.T) / m
+    lt = G[np.tril_indices(n)].astype(np.float32)
+    covars = np.stack([np.ones(n), rng.binomial(1, 0.5, size=n).astype(np.float64),
+                       f32(rng.uniform(40.0, 70.0, size=n))], axis=1)
+    u = np.sqrt(theta[1] / m) * (W @ rng.standard_normal(m))
+    y = f32(covars @ np.asarray(beta) + u + np.sqrt(theta[0]) * rng.standard_normal(n))
+    return ReducedCase(y=y, covars=covars, grm_lt=lt)
+
+
+def write_reduced_input(path: str, d: ReducedCase) -> None:
+    """Header INDIV PHENO <covariates> (readInputData, varCompEst_reducedModel_noTime.cpp:191-248)."""
+    with open(path, "w") as fh:
+        fh.write(" ".join(["INDIV", "PHENO", *d.cov_names]) + "\n")
+        for i in range(d.n):
+            vals = [str(i + 1), repr(float(np.float32(d.y[i])))]
+            vals += [repr(float(np.float32(v))) for v in d.covars[i]]
+            fh.write(" ".join(vals) + "\n")
+
+
 def grm_dense(lt: np.ndarray, n: int) -> np.ndarray:
     G = np.zeros((n, n))
     il = np.tril_indices(n)

@@ -179,13 +179,38 @@ int globe_aireml_eval(globe_ctx* ctx, const double* y, const double* probes,
 int globe_aireml_step(const double grad[5], const double ai[25], const double theta[5],
                       double delta, double step[5]);
 
+/* ---- reduced model (varCompEst_reducedModel_noTime.cpp): V = sig2g A + sig2e I ----------
+ * One record per individual (globe_set_cohort with nrec == n; records in GRM order), no
+ * time term: the covariates are passed as X with all their columns (no TIME column) and
+ * the context's components are (sig2e, sig2g, 0, 0, 0). On that cohort the per-individual
+ * block preconditioner is the reference's diagonal 1 / (sig2g A_ii + sig2e)
+ * (computePreconditioner :326-335). */
+/* Replaces computeDerivative_first + estimateTrace_PdV + computeDerivative_AImat of the
+ * reduced model (varCompEst_reducedModel_noTime.cpp:592-705): fitMME (:440-518), f_theta
+ * = [V^-1 r, A V^-1 r] (:524-548), the trace of P dV_theta from nmc injected probes (as
+ * globe_aireml_eval), gradient (2) and AI matrix (2x2 row-major). GLOBE_E_INVALID unless
+ * nrec == n and sig2b0 = sigb01 = sig2b1 = 0. */
+int globe_aireml_eval_reduced(globe_ctx* ctx, const double* y, const double* probes,
+                              const double* weights, const double* probe_x0, int nmc,
+                              const globe_pcg_opts* opts, double* grad, double* ai,
+                              double* first_terms, double* trace);
+/* Replaces the NLopt LD_MMA step of the reduced model's oneIter_AIREML
+ * (varCompEst_reducedModel_noTime.cpp:784-840, objective :707-728, constraints
+ * :731-781): minimise -(g.p - p^T AI p / 2) over -10 <= p_i <= 10 subject to
+ * sum_i AI_ii p_i <= delta and theta_i + p_i >= 1e-5 (no covariance constraint). Same
+ * solver and return codes as globe_aireml_step. */
+int globe_aireml_step_reduced(const double grad[2], const double ai[4], const double theta[2],
+                              double delta, double step[2]);
+
 typedef void (*globe_log_fn)(const char* line, void* user);
 
 typedef struct {
     int32_t max_iter;   /* --maxiter, reference default 50 (varCompEst.cpp:21) */
     double tol;         /* --tol, reference default 1e-3 (:22) */
     int32_t num_mc;     /* --numMC, reference default 15 (:23) */
-    int32_t probes;     /* 0: Gaussian tilde-V probes (reference, :775-797); 1: Rademacher */
+    int32_t probes;     /* 0: Gaussian tilde-V probes (reference, :775-797); 1: Rademacher;
+                           2: exact -- unit probes sqrt(nrec) e_s, numMC must equal nrec
+                           (small cohorts: the trace terms are then exact) */
     uint64_t seed;      /* probe and restart RNG seed (the reference's is irreproducible) */
     globe_pcg_opts pcg; /* inner solves; reference: tol 1e-5, max 1000, all-columns rule */
 } globe_aireml_opts;
@@ -198,6 +223,13 @@ typedef struct {
 int globe_aireml_fit(globe_ctx* ctx, const double* y, const globe_aireml_opts* opts,
                      double theta[5], double se[5], double ai[25], int32_t* iters,
                      double* sec_per_iter, globe_log_fn log, void* user);
+/* Replaces the reduced model's main loop + oneIter_AIREML
+ * (varCompEst_reducedModel_noTime.cpp:784-1099): the same trust-region control as
+ * globe_aireml_fit over theta = (sig2e, sig2g) (start in, estimates out), se (2), ai
+ * (2x2 row-major). The cohort must hold one record per individual. */
+int globe_aireml_fit_reduced(globe_ctx* ctx, const double* y, const globe_aireml_opts* opts,
+                             double theta[2], double se[2], double ai[4], int32_t* iters,
+                             double* sec_per_iter, globe_log_fn log, void* user);
 
 /* ---- serving loop (no reference counterpart: the reference solves one trait per run) -- */
 /* One resident cohort (globe_set_cohort for N, genotypes or GRM loaded) serves solve

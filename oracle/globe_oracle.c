@@ -739,7 +739,7 @@ void orc_sample_tildeV(const orc_model* md, const double* z1, const double* z2,
  *   grad = (first - trace) / 2;  AI = f^T P f / 2 with P f from PCG(x0 = Q^-1 f). */
 void orc_aireml_eval(const orc_model* md, const double* y, const double* probes,
                      const double* weights, const double* probe_x0, int nmc,
-                     const orc_pcg_opts* fit_opts, double* grad, double* ai,
+                     const orc_pcg_opts* fit_opts, int nvc, double* grad, double* ai,
                      double* first_terms, double* trace) {
     const int64_t nrec = md->nrec, n = md->n;
     const int q = md->p + 1;
@@ -796,11 +796,13 @@ void orc_aireml_eval(const orc_model* md, const double* y, const double* probes,
         if (nmc > 0) trace[j] /= (double)nmc;
         grad[j] = 0.5 * (first_terms[j] - trace[j]);
     }
-    /* AI (varCompEst.cpp:1015-1041) */
+    /* AI (varCompEst.cpp:1015-1041) over the model's nvc columns of f: 5, or 2 for the
+     * reduced model (varCompEst_reducedModel_noTime.cpp:677-705), whose slope columns
+     * are identically zero and would never meet the all-columns rule */
     orc_pcg_opts oa = o;
     oa.x0_mode = 1;
-    orc_pcg(md, qinv, f, 5, sv, &oa, NULL, NULL, NULL);
-    for (int j = 0; j < 5; ++j) {
+    orc_pcg(md, qinv, f, nvc, sv, &oa, NULL, NULL, NULL);
+    for (int j = 0; j < nvc; ++j) {
         const double* vs = sv + (int64_t)j * nrec;
         for (int a = 0; a < q; ++a) xv[a] = col_dot(md->X + (int64_t)a * nrec, vs, nrec);
         for (int a = 0; a < q; ++a) {
@@ -813,7 +815,7 @@ void orc_aireml_eval(const orc_model* md, const double* y, const double* probes,
             for (int a = 0; a < q; ++a) v -= vx[(int64_t)a * nrec + r] * cvec[a];
             pv[r] = v;
         }
-        for (int i2 = 0; i2 < 5; ++i2) ai[i2 * 5 + j] = 0.5 * col_dot(f + (int64_t)i2 * nrec, pv, nrec);
+        for (int i2 = 0; i2 < nvc; ++i2) ai[i2 * nvc + j] = 0.5 * col_dot(f + (int64_t)i2 * nrec, pv, nrec);
     }
     free(gdiag); free(qinv); free(py); free(uh); free(vx); free(xi); free(f);
     free(sv); free(pv); free(us); free(fs);

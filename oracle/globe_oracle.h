@@ -93,7 +93,7 @@ void orc_sample_tildeV(const orc_model* md, const double* z1, const double* z2,
                        const double* z3, double* yv);
 void orc_aireml_eval(const orc_model* md, const double* y, const double* probes,
                      const double* weights, const double* probe_x0, int nmc,
-                     const orc_pcg_opts* fit_opts, double* grad, double* ai,
+                     const orc_pcg_opts* fit_opts, int nvc, double* grad, double* ai,
                      double* first_terms, double* trace);
 
 #ifdef __cplusplus

@@ -78,7 +78,7 @@ def _declare(L):
     L.orc_f_theta.argtypes = [P, P, P, P]
     L.orc_tildeV_solve.argtypes = [P, P, C.c_int, P]
     L.orc_sample_tildeV.argtypes = [P, P, P, P, P]
-    L.orc_aireml_eval.argtypes = [P, P, P, P, P, C.c_int, P, P, P, P, P]
+    L.orc_aireml_eval.argtypes = [P, P, P, P, P, C.c_int, P, C.c_int, P, P, P, P]
 
 
 def _ptr(a):
@@ -222,20 +222,124 @@ class Oracle:
                                 _ptr(np.ascontiguousarray(z3, dtype=np.float64)), _ptr(out))
         return out
 
-    def aireml_eval(self, y, probes, weights, probe_x0=None, *, tol=1e-5, max_iter=1000):
+    def aireml_eval(self, y, probes, weights, probe_x0=None, *, tol=1e-5, max_iter=1000, nvc=5):
         probes = _cm(probes)
         weights = _cm(weights)
         nmc = probes.shape[1]
         x0 = None if probe_x0 is None else _cm(probe_x0)
         grad = np.zeros(5)
-        ai = np.zeros(25)
+        ai = np.zeros(nvc * nvc)
         first = np.zeros(5)
         tr = np.zeros(5)
         o = PcgOpts(tol, max_iter, 1, 1, 0)
         lib().orc_aireml_eval(self.ref, _ptr(np.ascontiguousarray(y, dtype=np.float64)),
-                              _ptr(probes), _ptr(weights), _ptr(x0), nmc, C.byref(o),
+                              _ptr(probes), _ptr(weights), _ptr(x0), nmc, C.byref(o), nvc,
                               _ptr(grad), _ptr(ai), _ptr(first), _ptr(tr))
-        return {"grad": grad, "ai": ai.reshape(5, 5), "first": first, "trace": tr}
+        return {"grad": grad[:nvc], "ai": ai.reshape(nvc, nvc), "first": first[:nvc],
+                "trace": tr[:nvc]}
+
+    def aireml_eval_reduced(self, y, probes, weights, probe_x0=None, *, tol=1e-5, max_iter=1000):
+        """The reduced model's computeDerivative_first + estimateTrace_PdV +
+        computeDerivative_AImat (varCompEst_reducedModel_noTime.cpp:592-705): V = sig2g A
+        + sig2e I (computeV_ij :288-296) is this oracle's V on a cohort of one record per
+        individual with sig2b0 = sigb01 = sig2b1 = 0; its f_theta columns for sig2e and
+        sig2g are the full model's first two (V^-1 r and A V^-1 r, :524-548), so the
+        gradient, trace and AI terms are orc_aireml_eval's over those two columns."""
+        if self.nrec != self.n or any(self.theta[j] != 0.0 for j in (2, 3, 4)):
+            raise ValueError("reduced model: one record per individual and sig2b* = 0")
+        return self.aireml_eval(y, probes, weights, probe_x0, tol=tol, max_iter=max_iter, nvc=2)
+
+
+def aireml_fit_reduced(orc, y, theta0, *, max_iter=50, tol=1e-3, pcg_tol=1e-10):
+    """The reduced model's AI-REML loop with exact trace terms: main (:967-1099) and
+    oneIter_AIREML (:784-964) of varCompEst_reducedModel_noTime.cpp, with the unit probes
+    sqrt(N) e_s (s < N) as the Monte Carlo sample, so every gradient is exact. The step is
+    the Newton step AI^-1 g when it is feasible (the usual case: Delta starts at 1e20,
+    :1024) and otherwise the minimiser of the same convex quadratic model under the
+    reference's box, signed trust-region and positivity constraints (SLSQP). Restarts
+    after > 5 rejections draw rand() in the reference and are not replayed (an error).
+    Returns theta, the accepted theta after every iteration, and the iteration count."""
+    from scipy.optimize import minimize
+
+    n = orc.nrec
+    probes = np.sqrt(n) * np.eye(n)
+
+    def evaluate(th):
+        orc.set_theta((th[0], th[1], 0.0, 0.0, 0.0))
+        e = orc.aireml_eval_reduced(y, probes, probes, tol=pcg_tol)
+        return e["grad"], e["ai"]
+
+    def step(g, A, th, delta):
+        lb = np.maximum(-10.0, 1e-5 - th)
+        try:
+            p = np.linalg.solve(A, g)
+            if (np.all(np.diag(A) > 0) and np.all(p >= lb) and np.all(p <= 10.0)
+                    and np.diag(A) @ p <= delta):
+                return p
+        except np.linalg.LinAlgError:
+            pass
+        cons = [{"type": "ineq", "fun": lambda q: delta - np.diag(A) @ q}]
+        r = minimize(lambda q: -(g @ q - 0.5 * q @ A @ q), np.zeros(2), method="SLSQP",
+                     bounds=list(zip(lb, [10.0, 10.0])), constraints=cons,
+                     options={"ftol": 1e-15, "maxiter": 1000})
+        return r.x
+
+    th = np.array(theta0, dtype=np.float64)
+    delta_tr, dangerous, num_rej, gnt = 1e20, False, 0, 0.0
+    traj = []
+    g = A = None
+    it_done = 0
+    for it in range(max_iter):
+        if it == 0:
+            g, A = evaluate(th)
+        if it == 1:
+            gnt = 0.1 * np.linalg.norm(g)
+        if num_rej > 5:
+            raise RuntimeError("restart (rand()) not replayed")
+        th_old = th.copy()
+        p = step(g, A, th, delta_tr)
+        scale, accepted, converged, red = 1.0, False, False, 0
+        while not accepted and red <= 10:
+            sp = scale * p
+            tn = th + sp
+            if np.any(tn < 1e-6) or np.any(tn > 10.0):
+                scale *= 0.5
+                red += 1
+                continue
+            gn, An = evaluate(tn)
+            dll = sp @ (0.5 * (g + gn))
+            pred = g @ sp - 0.5 * sp @ A @ sp
+            if scale == 1.0 and pred < 1e-2:
+                dangerous, converged = False, True
+                break
+            if np.linalg.norm(g) > gnt and np.linalg.norm(gn) > 2 * np.linalg.norm(g):
+                scale *= 0.5
+                red += 1
+                continue
+            rho = dll / pred
+            npd = float(np.sum(np.abs(np.diag(A) * sp)))
+            if rho < 0.25:
+                delta_tr = 0.25 * npd
+            elif rho > 0.99:
+                delta_tr = max(delta_tr, 3.5 * npd)
+            if rho > 0.25:
+                th, g, A = tn, gn, An
+                num_rej, accepted = 0, True
+            else:
+                scale *= 0.5
+                red += 1
+        if not accepted and not converged:
+            delta_tr *= 0.75
+            dangerous = True
+            num_rej += 1
+        traj.append(th.copy())
+        it_done = it + 1
+        if np.linalg.norm(th - th_old) / np.linalg.norm(th_old) < tol and not dangerous:
+            break
+        dangerous = False
+        if delta_tr < 1e-4:
+            break
+    return th, np.array(traj), it_done
 
 
 def oracle_from_synth(d, theta=None) -> Oracle:
