@@ -52,20 +52,19 @@ def build_cohort(g, n, m, lam, seed):
     return c, y, kept
 
 
-def measured_traffic(prefix):
+def measured_traffic(role):
     """HBM bytes per launch of the dominant kernel from the committed rocprof summary
     (calibrated FETCH_SIZE + WRITE_SIZE; profiles/k_geno_mfma_n200k_m500k.md has the
-    commands), or None when no summary of that kernel is present."""
+    commands): role "wide" (32-RHS contraction) or "thin" (1-RHS pass). None when no
+    summary of that kernel is present."""
     path = os.path.join(ROOT, "profiles", "k_geno_mfma_n200k_m500k.traffic.json")
     try:
         with open(path) as f:
             tr = json.load(f)
-    except (OSError, ValueError):
+        v = tr[role]
+        return v["fetch_bytes"] + v["write_bytes"], f"{v['kernel']} / {v['write_kernel']}"
+    except (OSError, ValueError, KeyError):
         return None, None
-    for name, v in tr.items():
-        if name.startswith(prefix) and isinstance(v, dict):
-            return v["fetch_bytes"] + v["write_bytes"], name
-    return None, None
 
 
 def cpu_baseline(g, c, kept, iters, sample_snps, threads):
@@ -308,9 +307,10 @@ def main():
         serve = serving_leg(g, c, kept, args)
     ingest = ingest_leg(g, c, args) if args.ingest_snps > 0 else None
     aireml = aireml_leg(args) if args.aireml_iters > 0 else None
-    traffic, traffic_src = measured_traffic("k_geno_wide")
+    traffic, traffic_src = measured_traffic("wide")
     if traffic_src:
         traffic_src = f"profiles/k_geno_mfma_n200k_m500k.md ({traffic_src})"
+    traffic64, traffic64_src = measured_traffic("thin")
     cpu = None
     if not args.no_cpu:
         log("CPU baseline ...")
@@ -336,7 +336,8 @@ def main():
                       "seconds": st64["seconds"],
                       "pass1_ms": p64[0]["ms_avg"], "pass2_ms": p64[1]["ms_avg"],
                       "pass1_GBps": gbps64[0], "pass2_GBps": gbps64[1],
-                      "hbm_frac": [x / PEAK_HBM_GBPS for x in gbps64]},
+                      "hbm_frac": [x / PEAK_HBM_GBPS for x in gbps64],
+                      "traffic": traffic64, "traffic_source": traffic64_src},
         "mixed_vs_fp64_speedup": value * st64["seconds"],
         "mixed_solution_rel_err_y": err_y,
         "mixed_iters": [st["iters"] for st in stats],
@@ -344,7 +345,8 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_INT8_TOPS, "traffic": traffic,
                      "traffic_source": traffic_src, "algorithmic_bytes": pm[0]["bytes"],
                      "kernel": "k_geno_wide (both passes)", "ms_per_launch": tot_ms / max(launches, 1),
-                     "issued_int8_TOPs": issued_tops, "issued_frac": issued_tops / PEAK_INT8_TOPS},
+                     "issued_int8_TOPs": issued_tops, "issued_frac": issued_tops / PEAK_INT8_TOPS,
+                     "mean_digits": (issued / launches) / flops if launches and flops else None},
         "cpu_baseline": cpu,
         "serving": serve,
         "aireml": aireml,

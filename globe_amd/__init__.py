@@ -25,6 +25,7 @@ GLOBE_OK = 0
 VX_FULL, VX_GRM_ONLY = 0, 1
 PREC_FP64, PREC_MIXED, PREC_FP64_VALU = 0, 1, 2
 RULE_MAX_RATIO, RULE_ALL_COLUMNS, RULE_PER_COLUMN = 0, 1, 2
+PCG_FIXED_DIGITS = 1  # GLOBE_PCG_FIXED_DIGITS: mixed precision keeps 4 digits
 X0_ZERO, X0_PRECOND, X0_GIVEN = 0, 1, 2
 
 _LIB = None
@@ -39,7 +40,7 @@ class GlobeError(RuntimeError):
 class PcgOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iter", C.c_int32), ("x0_mode", C.c_int32),
                 ("rule", C.c_int32), ("stagnation", C.c_int32), ("precision", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("flags", C.c_int32)]
 
 
 class PcgStats(C.Structure):
@@ -320,11 +321,11 @@ class Globe:
 
     # ---- solver (runPCG_multi / runPCG_multi_VC)
     def pcg(self, b, *, tol=1e-5, max_iter=1000, x0_mode=X0_ZERO, rule=RULE_MAX_RATIO,
-            stagnation=True, precision=PREC_FP64, x0=None):
+            stagnation=True, precision=PREC_FP64, x0=None, flags=0):
         b = _cm(b)
         k = b.shape[1]
         x = np.zeros_like(b, order="F") if x0 is None else _cm(x0).copy(order="F")
-        o = PcgOpts(tol, max_iter, x0_mode, rule, int(stagnation), precision, 0)
+        o = PcgOpts(tol, max_iter, x0_mode, rule, int(stagnation), precision, flags)
         st = PcgStats()
         ratio = np.zeros(k)
         its = np.zeros(k, dtype=np.int32)
@@ -344,13 +345,13 @@ class Globe:
 
     # ---- estimators (runPCGsolver main body, runAIREML derivatives)
     def gls_blup(self, y, *, tol=1e-5, max_iter=1000, x0_mode=X0_ZERO, rule=RULE_MAX_RATIO,
-                 stagnation=True, precision=PREC_FP64):
+                 stagnation=True, precision=PREC_FP64, flags=0):
         y = np.ascontiguousarray(y, dtype=np.float64)
         beta = np.zeros(self.p + 1)
         vr = np.zeros(self.nrec)
         uh = np.zeros(self.n)
         bh = np.zeros(2 * self.n)
-        o = PcgOpts(tol, max_iter, x0_mode, rule, int(stagnation), precision, 0)
+        o = PcgOpts(tol, max_iter, x0_mode, rule, int(stagnation), precision, flags)
         st = PcgStats()
         self._check(self.L.globe_gls_blup(self.h, _p(y), C.byref(o), _p(beta), _p(vr), _p(uh),
                                           _p(bh), C.byref(st)))
@@ -389,8 +390,8 @@ class Globe:
         return DeviceBuffer(self, nbytes)
 
     def pcg_dev(self, bbuf, xbuf, k, *, tol=1e-5, max_iter=1000, x0_mode=X0_ZERO,
-                rule=RULE_PER_COLUMN, stagnation=False, precision=PREC_FP64):
-        o = PcgOpts(tol, max_iter, x0_mode, rule, int(stagnation), precision, 0)
+                rule=RULE_PER_COLUMN, stagnation=False, precision=PREC_FP64, flags=0):
+        o = PcgOpts(tol, max_iter, x0_mode, rule, int(stagnation), precision, flags)
         st = PcgStats()
         ratio = np.zeros(k)
         its = np.zeros(k, dtype=np.int32)

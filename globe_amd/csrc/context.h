@@ -39,6 +39,9 @@ struct globe_ctx {
     int64_t ingest_chunk = 256ll << 20;
     double ingest_stats[5] = {0, 0, 0, 0, 0};
 
+    // int8 digits of GLOBE_PREC_MIXED contractions (run_pcg lowers it per iteration)
+    int limbs_mixed = 4;
+
     // variance components
     std::array<double, 5> theta{0, 0, 0, 0, 0};
     bool has_theta = false;

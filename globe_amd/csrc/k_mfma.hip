@@ -732,13 +732,23 @@ int64_t rows_per_block(bool wide, int cfg) {
     return (int64_t)(w.W / w.CG) * 32 * w.RA;
 }
 
+// A wide shape fits JT column fragments when the column groups divide them and the
+// stage's 4 JT digit chunks spread evenly over the waves (or number fewer than them).
+constexpr bool wide_fits(int JT, int q) {
+    return JT % kWide[q].CG == 0 && (4 * JT < kWide[q].W || (4 * JT) % kWide[q].W == 0);
+}
+
 template <int JT, int Q>
 void launch_wide_q(hipStream_t s, unsigned grid, const uint8_t* rows, int64_t pitch, int64_t nrows,
                    int64_t nss, int64_t per, int64_t R, int S, const uint8_t* Bq, int32_t* out,
                    const int* skip) {
-    constexpr WideShape w = kWide[Q];
-    k_geno_wide<JT, w.W, w.NS, w.CG, w.RA><<<grid, w.W * 64, 0, s>>>(rows, pitch, nrows, nss, per, R,
-                                                                    S, Bq, out, skip);
+    if constexpr (wide_fits(JT, Q)) {
+        constexpr WideShape w = kWide[Q];
+        k_geno_wide<JT, w.W, w.NS, w.CG, w.RA><<<grid, w.W * 64, 0, s>>>(rows, pitch, nrows, nss, per, R,
+                                                                        S, Bq, out, skip);
+    } else {
+        require(false, GLOBE_E_INVALID, "wide contraction shape not available for this width");
+    }
 }
 
 template <int JT>
@@ -748,19 +758,8 @@ void launch_wide(hipStream_t s, const Variant& v, unsigned grid, const uint8_t* 
     switch (v.cfg) {
         case 0: launch_wide_q<JT, 0>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
         case 1: launch_wide_q<JT, 1>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
-        case 2:
-            if constexpr (JT % 2 == 0) {
-                launch_wide_q<JT, 2>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip);
-                break;
-            }
-            [[fallthrough]];
-        case 3:
-            if constexpr (JT % 2 == 0) {
-                launch_wide_q<JT, 3>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip);
-                break;
-            }
-            [[fallthrough]];
-        default: require(false, GLOBE_E_INVALID, "wide contraction shape not available for this width");
+        case 2: launch_wide_q<JT, 2>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
+        default: launch_wide_q<JT, 3>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
     }
 }
 
@@ -787,6 +786,8 @@ void launch_variant(hipStream_t s, bool wide, int JT, const Variant& v, const ui
         launch_wide<1>(s, v, grid, rows, pitch, nrows, nss, per, R, Bq, out, skip);
     } else if (JT == 2) {
         launch_wide<2>(s, v, grid, rows, pitch, nrows, nss, per, R, Bq, out, skip);
+    } else if (JT == 3) {
+        launch_wide<3>(s, v, grid, rows, pitch, nrows, nss, per, R, Bq, out, skip);
     } else {
         launch_wide<4>(s, v, grid, rows, pitch, nrows, nss, per, R, Bq, out, skip);
     }
@@ -805,7 +806,7 @@ static std::vector<Variant> candidates(bool wide, int JT, int64_t nrows, int64_t
     std::vector<Variant> v;
     if (wide) {
         for (int cfg = 0; cfg < 4; ++cfg) {
-            if (JT % kWide[cfg].CG != 0) continue;
+            if (!wide_fits(JT, cfg)) continue;
             const int64_t R = (nrows + rows_per_block(true, cfg) - 1) / rows_per_block(true, cfg);
             const int S0 = choose_split(R, nss, true);
             for (int S : {S0 / 2, S0, S0 * 2}) {
@@ -842,7 +843,7 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     const int J = k * limbs;
     require(J <= 128, GLOBE_E_INVALID, "k * limbs must be <= 128 for the MFMA contraction");
     const bool wide = J > 16;
-    const int JT = wide ? (J <= 32 ? 1 : J <= 64 ? 2 : 4) : 1;
+    const int JT = wide ? (J + 31) / 32 : 1;  // column fragments of 32 digit columns
     const int J16 = wide ? 32 * JT : 16;
     const int nfrag = wide ? JT : 1;
     const int64_t Kpad1 = c->pitch_s * 4, Kpad2 = c->pitch_i * 4;  // genotypes per padded row

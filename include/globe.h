@@ -60,8 +60,13 @@ typedef struct {
     int32_t rule;       /* GLOBE_RULE_* */
     int32_t stagnation; /* rule 0 only: 10-iteration / 5e-3 stall exit (PCGsolver.cpp:1374-1415) */
     int32_t precision;  /* GLOBE_PREC_* */
-    int32_t reserved;
+    int32_t flags;      /* GLOBE_PCG_* bits, 0 by default */
 } globe_pcg_opts;
+
+/* GLOBE_PREC_MIXED solves relax the digits of the genotype contraction as the residual
+ * falls (4, then 3, then 2 int8 digits: inexact Krylov relaxation, the operator error of
+ * iteration j at most 0.01 tol |b| / |r_j|); this flag keeps 4 digits throughout. */
+#define GLOBE_PCG_FIXED_DIGITS 1
 
 typedef struct {
     int32_t iters;      /* iterations run */
@@ -291,7 +296,8 @@ typedef struct {
     double ms_total;
     double bytes_per_launch;
     double flops_per_launch;
-    double issued_ops_per_launch; /* int8 MFMA operations actually issued (digits, padding) */
+    double issued_ops_per_launch; /* int8 MFMA operations actually issued (digits, padding),
+                                     averaged over the launches */
 } globe_kernel_stats;
 int globe_profile_enable(globe_ctx* ctx, int on);
 int globe_profile_read(globe_ctx* ctx, int which, globe_kernel_stats* out);
