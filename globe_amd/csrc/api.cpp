@@ -533,18 +533,43 @@ int globe_get_geno_rows(globe_ctx* c, int64_t first, int64_t count, uint8_t* out
     API_END(c)
 }
 
+namespace {
+
+// Rows r0 .. r1 of the packed triangle per chunk, at most `cap` floats (one row at least).
+int64_t grm_chunk_end(int64_t r0, int64_t n, int64_t cap) {
+    auto span = [](int64_t a, int64_t b) { return (b * (b + 1) - a * (a + 1)) / 2; };
+    int64_t r1 = std::min<int64_t>(n, r0 + std::max<int64_t>(1, cap / (r0 + 1)));
+    while (r1 > r0 + 1 && span(r0, r1) > cap) r1 = r0 + (r1 - r0) / 2;
+    return r1;
+}
+
+constexpr int64_t kGrmStageFloats = 64ll << 20;  // 256 MB of packed triangle per chunk
+
+}  // namespace
+
 int globe_load_grm(globe_ctx* c, const float* lt) {
     API_BEGIN(c)
     require(c->has_cohort, GLOBE_E_STATE, "cohort must be set before the GRM");
     require(lt != nullptr, GLOBE_E_INVALID, "null GRM");
     const int64_t n = c->n;
-    const size_t nel = (size_t)n * (n + 1) / 2;
+    hipStream_t s = c->stream;
     c->gen = 0;
     c->d_snp.release();
     c->d_ind.release();
     c->m = 0;
-    c->d_grm.ensure(sizeof(float) * nel);
-    HIP_CHECK(hipMemcpy(c->d_grm.p, lt, sizeof(float) * nel, hipMemcpyHostToDevice));
+    // the packed triangle streams through a staging buffer into the tile-major layout
+    c->d_grm.ensure(grm_tile_bytes((int32_t)n));
+    HIP_CHECK(hipMemsetAsync(c->d_grm.p, 0, grm_tile_bytes((int32_t)n), s));
+    DevBuf stage;
+    stage.ensure(sizeof(float) * (size_t)std::min<int64_t>(kGrmStageFloats, n * (n + 1) / 2 + n));
+    for (int64_t r0 = 0; r0 < n;) {
+        const int64_t r1 = grm_chunk_end(r0, n, kGrmStageFloats);
+        const int64_t off = r0 * (r0 + 1) / 2, cnt = r1 * (r1 + 1) / 2 - off;
+        HIP_CHECK(hipMemcpyAsync(stage.p, lt + off, sizeof(float) * cnt, hipMemcpyHostToDevice, s));
+        launch_grm_tiles_from_lt(s, stage.as<float>(), r0, r1, c->d_grm.as<float>());
+        r0 = r1;
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
     std::vector<double> gd((size_t)n);
     for (int64_t i = 0; i < n; ++i) gd[i] = (double)lt[(size_t)i * (i + 1) / 2 + i];
     c->d_gdiag.ensure(sizeof(double) * n);
@@ -558,10 +583,10 @@ int globe_grm_from_geno(globe_ctx* c) {
     API_BEGIN(c)
     require(c->gen == 1, GLOBE_E_STATE, "genotypes must be loaded to build the GRM");
     const int32_t n = c->n;
-    const size_t nel = (size_t)n * (n + 1) / 2;
     hipStream_t s = c->stream;
-    constexpr int kb = 64;  // columns of G per contraction
-    c->d_grm.ensure(sizeof(float) * std::max<size_t>(nel, 1));
+    constexpr int kb = 64;  // columns of G per contraction = one tile column
+    c->d_grm.ensure(grm_tile_bytes(n));
+    HIP_CHECK(hipMemsetAsync(c->d_grm.p, 0, grm_tile_bytes(n), s));
     DevBuf S0, GZ, th;
     S0.ensure(sizeof(double) * (size_t)n * kb);
     GZ.ensure(sizeof(double) * (size_t)n * kb);
@@ -592,8 +617,19 @@ int globe_grm_from_geno(globe_ctx* c) {
 int globe_get_grm(globe_ctx* c, float* lt) {
     API_BEGIN(c)
     require(c->gen == 2, GLOBE_E_STATE, "no GRM in the context");
-    const size_t nel = (size_t)c->n * (c->n + 1) / 2;
-    if (lt) HIP_CHECK(hipMemcpy(lt, c->d_grm.p, sizeof(float) * nel, hipMemcpyDeviceToHost));
+    if (!lt) return GLOBE_OK;
+    const int64_t n = c->n;
+    hipStream_t s = c->stream;
+    DevBuf stage;
+    stage.ensure(sizeof(float) * (size_t)std::min<int64_t>(kGrmStageFloats, n * (n + 1) / 2 + n));
+    for (int64_t r0 = 0; r0 < n;) {
+        const int64_t r1 = grm_chunk_end(r0, n, kGrmStageFloats);
+        const int64_t off = r0 * (r0 + 1) / 2, cnt = r1 * (r1 + 1) / 2 - off;
+        launch_grm_lt_from_tiles(s, c->d_grm.as<float>(), r0, r1, stage.as<float>());
+        HIP_CHECK(hipMemcpyAsync(lt + off, stage.p, sizeof(float) * cnt, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        r0 = r1;
+    }
     API_END(c)
 }
 

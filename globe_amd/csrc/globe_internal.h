@@ -154,15 +154,21 @@ void launch_reduce_parts_skip(hipStream_t s, const double* part, int nparts, int
                               const int* skip);
 
 // ---------------------------------------------------------------- GRM
-// part: grm_product_part_bytes(n) of scratch for the strip partials
-void launch_grm_product(hipStream_t s, const float* lt, int32_t n, const double* S0, int64_t ldn,
+// The device GRM is tile-major (k_grm.hip): 64 x 64 FP32 tiles (I, J <= I) at tile
+// index I(I+1)/2 + J, diagonal tiles mirrored, zero past n; grm_tile_bytes(n) bytes.
+size_t grm_tile_bytes(int32_t n);
+// packed-triangle rows r0 .. r1 - 1 (lt_rows starts at row r0) <-> tiles
+void launch_grm_tiles_from_lt(hipStream_t s, const float* lt_rows, int64_t r0, int64_t r1, float* tiles);
+void launch_grm_lt_from_tiles(hipStream_t s, const float* tiles, int64_t r0, int64_t r1, float* lt_rows);
+// part: grm_product_part_bytes(n) of scratch for the strip partials and S0 slices
+void launch_grm_product(hipStream_t s, const float* tiles, int32_t n, const double* S0, int64_t ldn,
                         int k, const double* theta_cols, double* GZ, int64_t ldg, const int* skip,
                         double* part);
 size_t grm_product_part_bytes(int32_t n);
-// lt[i(i+1)/2 + j0 + c] = G[i][j0 + c] (FP32) for i >= j0 + c, from the n x k columns Gc
+// tile column j0 / 64 of G (FP32) for i >= j0 + c from the n x k columns Gc (j0 % 64 == 0)
 void launch_grm_scatter(hipStream_t s, const double* Gc, int64_t ldg, int32_t n, int64_t j0, int k,
-                        float* lt);
+                        float* tiles);
 void launch_eye_cols(hipStream_t s, double* S0, int64_t ld, int32_t n, int64_t j0, int k);
-void launch_grm_diag(hipStream_t s, const float* lt, int32_t n, double* d);
+void launch_grm_diag(hipStream_t s, const float* tiles, int32_t n, double* d);
 
 }  // namespace globe

@@ -1,17 +1,23 @@
-// Dense symmetric GRM product GZ = sig2g G S0 from the FP32 packed lower triangle
-// (GCTA .grm.bin, element (i, j), j <= i, at i(i+1)/2 + j; PCGsolver.cpp:878-881,
-// :1204-1212; varCompEst.cpp:336-347), FP64 accumulation, and the GRM builder that
-// writes that triangle from the resident genotypes.
+// Dense symmetric GRM product GZ = sig2g G S0 with FP64 accumulation, and the GRM
+// builder. The reference keeps G as the FP32 packed lower triangle of a GCTA .grm.bin
+// (element (i, j), j <= i, at i(i+1)/2 + j; PCGsolver.cpp:878-881, :1204-1212;
+// varCompEst.cpp:336-347). On the device the same FP32 values are kept tile-major:
+// 64 x 64 tiles (I, J), J <= I, tile index I(I+1)/2 + J, each tile row-major and 16 KB
+// contiguous; the diagonal tiles hold both halves (mirrored), and rows or columns past n
+// are zero. A packed row of the triangle starts at i(i+1)/2 floats, which is not 16-byte
+// aligned, so tiles could only be gathered 4 bytes per lane; tile-major, every tile
+// streams as 1 KB pieces of 16 bytes per lane. Conversion to and from the packed triangle
+// happens at load (globe_load_grm) and read-out (globe_get_grm).
 //
-// Product: a workgroup owns a strip of 64 output rows and one of kGrmChunks ranges of
-// 64-column tiles of the full symmetric row strip. Each 64 x 64 tile is staged in LDS
-// with coalesced loads -- from the stored block (I, J) when J < I, transposed from the
-// stored block (J, I) when J > I, mirrored when J == I -- next to the 64 x 16 slice of
-// S0, and multiplied on the FP64 matrix cores (an FP64 VALU form of the same tile
-// product is bound by LDS bandwidth: 2 operand reads per 4 FMAs); a group of up to 16
-// columns takes one pass over the triangle. The strip partials of the chunks are added
-// in chunk order, so the result is deterministic. Each stored element is read twice (once for its row strip, once
-// for its column strip); the triangle is far larger than any cache.
+// Product: a workgroup owns a strip of 64 output rows and one of kGrmChunks ranges of the
+// strip's tiles J: tile (I, J) for J <= I, tile (J, I) read transposed for J > I. Tiles
+// and the matching 64 x 16 slice of S0 (pre-transposed to rows of 16 doubles) go global
+// -> LDS (global_load_lds_dwordx4) into a ring of kGS slots, one wait + barrier per tile,
+// and each tile is multiplied on the FP64 matrix cores while the next one lands. A group
+// of up to 16 columns takes one pass over the triangle. The strip partials of the
+// chunks are added in chunk order, so the result is deterministic. Each stored tile is
+// read twice (once for its row strip, once for its column strip); the triangle is far
+// larger than any cache.
 #include <algorithm>
 
 #include "globe_internal.h"
@@ -22,112 +28,115 @@ namespace {
 
 constexpr int kGT = 64;        // tile edge
 constexpr int kGrmChunks = 4;  // column ranges per row strip
-
-__device__ inline int64_t lt_index(int64_t i, int64_t j) { return i * (i + 1) / 2 + j; }
+constexpr int kGC = 16;        // columns of S0 per pass
+constexpr int kGS = 2;         // ring slots: one tile in flight while one is multiplied
+constexpr int kTileB = kGT * kGT * 4;                 // FP32 tile bytes
+constexpr int kSliceB = kGT * kGC * 8;                // S0 slice bytes
+constexpr int kSlotB = kTileB + kSliceB;
+constexpr int kCopiesPerStage = kTileB / 1024 / 4 + kSliceB / 1024 / 4;  // per wave: 4 + 2
 
 typedef double v4d __attribute__((ext_vector_type(4)));
-constexpr int kGP = kGT + 4;  // LDS row pitch of a tile (floats)
+typedef __attribute__((address_space(3))) void lds_void;
 
-// Thread t holds tile elements (a, b) = (t / 64 + 4 q, t % 64), q < 16, of tile (I, J):
-// KIND 0 (J < I): G[i0 + a][j0 + b] from the stored block (I, J);
-// KIND 1 (J > I): G[i0 + b][j0 + a] from the stored block (J, I) (row j0 + a, coalesced in b);
-// KIND 2 (J == I): the diagonal block, mirrored.
-template <int KIND>
-__device__ inline void tile_load(const float* __restrict__ lt, int32_t n, int64_t i0, int64_t j0, int t,
-                                 float (&v)[16]) {
-    const int b = t & 63;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int a = (t >> 6) + 4 * q;
-        int64_t idx;
-        if (KIND == 0) {
-            const int64_t i = i0 + a;
-            idx = i < n ? lt_index(i, j0 + b) : 0;
-        } else if (KIND == 1) {
-            const int64_t j = j0 + a;
-            idx = j < n ? lt_index(j, i0 + b) : 0;
-        } else {
-            const int64_t i = i0 + a, j = j0 + b;
-            idx = (i < n && j < n) ? (b <= a ? lt_index(i, j) : lt_index(j, i)) : 0;
-        }
-        v[q] = lt[idx];
+__device__ inline int64_t lt_index(int64_t i, int64_t j) { return i * (i + 1) / 2 + j; }
+__device__ inline int64_t tile_index(int64_t I, int64_t J) { return I * (I + 1) / 2 + J; }  // J <= I
+
+// Element (a, b) of a tile in LDS: row a holds its 16 pieces of 4 floats rotated by a,
+// piece q at slot (q + a) & 15. 16 consecutive rows at one column (the MFMA A operand)
+// and 16 consecutive columns of one row (the transposed read) fall in different banks.
+__device__ inline int rot(int a, int b) { return a * kGT + ((((b >> 2) + a) & 15) << 2) + (b & 3); }
+
+// S0T[j][c] = S0[c][j] for c < kk and j < n, 0 elsewhere (npad = 64 ntiles rows).
+__global__ __launch_bounds__(256) void k_grm_s0t(const double* __restrict__ S0, int64_t ldn, int32_t n,
+                                                 int kk, int64_t npad, double* __restrict__ S0T,
+                                                 const int* skip) {
+    if (skip && *skip) return;
+    for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < npad * kGC; e += (int64_t)gridDim.x * 256) {
+        const int64_t j = e / kGC;
+        const int c = (int)(e % kGC);
+        S0T[e] = (c < kk && j < n) ? S0[c * ldn + j] : 0.0;
     }
 }
 
-template <int KIND>
-__device__ inline void tile_store(float* __restrict__ g, int32_t n, int64_t i0, int64_t j0, int t,
-                                  const float (&v)[16]) {
-    const int b = t & 63;
+// Copies of one stage into ring slot `slot` by wave w: tile pieces (1 KB = 4 tile rows,
+// lane l: row 4 p + l / 16, its piece (l % 16 - row) & 15, so the rows land rotated) p =
+// 4 w .. 4 w + 3, and S0T pieces 2 w, 2 w + 1 of the 8 KB slice.
+__device__ inline void grm_issue(const float* __restrict__ tile, const double* __restrict__ s0t,
+                                 uint8_t* __restrict__ slot, int w, int lane) {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int a = (t >> 6) + 4 * q;
-        if (KIND == 0)
-            g[a * kGP + b] = i0 + a < n ? v[q] : 0.0f;
-        else if (KIND == 1)
-            g[b * kGP + a] = j0 + a < n ? v[q] : 0.0f;
-        else
-            g[a * kGP + b] = (i0 + a < n && j0 + b < n) ? v[q] : 0.0f;
+    for (int q = 0; q < 4; ++q) {
+        const int p = 4 * w + q;
+        const int a = 4 * p + (lane >> 4);
+        const int piece = ((lane & 15) - a) & 15;
+        __builtin_amdgcn_global_load_lds((const void*)(tile + a * kGT + 4 * piece),
+                                         (lds_void*)(slot + p * 1024), 16, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int p = 2 * w + q;
+        __builtin_amdgcn_global_load_lds((const void*)((const uint8_t*)s0t + p * 1024 + lane * 16),
+                                         (lds_void*)(slot + kTileB + p * 1024), 16, 0, 0);
     }
 }
 
-constexpr int kGC = 16;       // columns of S0 per pass
+// 16 MFMAs of one tile for wave w: out rows 16 w + (0..15) x 16 columns. A lane l =
+// G[16 w + l % 16][4 s + l / 16] (TRANS: the stored tile read transposed), B lane l =
+// S0[j0 + 4 s + l / 16][l % 16], D element v of lane l = (row 4 v + l / 16, column
+// l % 16) (v_mfma_f64_16x16x4f64, probed on gfx950).
+template <bool TRANS>
+__device__ inline void grm_mma(const uint8_t* __restrict__ slot, int w, int lane, v4d& acc) {
+    const float* g = reinterpret_cast<const float*>(slot);
+    const double* sb = reinterpret_cast<const double*>(slot + kTileB) + (lane >> 4) * kGC + (lane & 15);
+    const int a = 16 * w + (lane & 15);
+#pragma unroll
+    for (int s4 = 0; s4 < kGT / 4; ++s4) {
+        const int b = 4 * s4 + (lane >> 4);
+        const float gv = TRANS ? g[rot(b, a)] : g[rot(a, b)];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)gv, sb[4 * s4 * kGC], acc, 0, 0, 0);
+    }
+}
 
-// One pass of up to 16 columns: out (64 x 16) += G_tile (64 x 64) S0_slice (64 x 16) per
-// tile on the FP64 matrix cores (v_mfma_f64_16x16x4f64): wave w owns rows 16 w .. 16 w
-// + 15 of the strip; A lane l = G[16 w + l % 16][4 s + l / 16] (FP32 tile in LDS,
-// widened), B lane l = S0[4 s + l / 16][l % 16], D element v of lane l = (row
-// 4 v + l / 16, column l % 16) (probed on gfx950). Columns past kk are zero.
-__global__ __launch_bounds__(256) void k_grm_tiles(const float* __restrict__ lt, int32_t n,
-                                                   const double* __restrict__ S0, int64_t ldn, int kk,
+// One pass of up to 16 columns (S0T zero past kk): part[h][c][i] = sum over the tiles J
+// of chunk h of (G_IJ S0_J)[i][c].
+__global__ __launch_bounds__(256) void k_grm_tiles(const float* __restrict__ tiles, int32_t n,
+                                                   const double* __restrict__ S0T, int kk,
                                                    int32_t ntiles, double* __restrict__ part,
                                                    const int* skip) {
     if (skip && *skip) return;
-    __shared__ float g[kGT * kGP];
-    __shared__ double sh[kGT * kGC];  // [k][c]
+    __shared__ __attribute__((aligned(1024))) uint8_t lds[kGS * kSlotB];
     const int I = blockIdx.x, h = blockIdx.y;
-    const int64_t i0 = (int64_t)I * kGT;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int J0 = (int)((int64_t)h * ntiles / kGrmChunks), J1 = (int)((int64_t)(h + 1) * ntiles / kGrmChunks);
+    const int nt = J1 - J0;
     v4d acc = {0.0, 0.0, 0.0, 0.0};
-    for (int J = J0; J < J1; ++J) {
-        const int64_t j0 = (int64_t)J * kGT;
-        // all 16 + 4 loads of the tile are issued before the first LDS store (indices
-        // clamped in bounds; out-of-range elements become 0 when stored)
-        float v[16];
-        const int kind = J < I ? 0 : (J > I ? 1 : 2);
-        if (kind == 0)
-            tile_load<0>(lt, n, i0, j0, t, v);
-        else if (kind == 1)
-            tile_load<1>(lt, n, i0, j0, t, v);
-        else
-            tile_load<2>(lt, n, i0, j0, t, v);
-        double sv[4];
+    if (nt > 0) {
+        // stage t: tile J0 + t (repeating the last one past the end, into slots never read)
+        auto issue = [&](int t, int slot) {
+            const int64_t J = J0 + (t < nt ? t : nt - 1);
+            const float* tile = tiles + (J <= I ? tile_index(I, J) : tile_index(J, I)) * (kGT * kGT);
+            grm_issue(tile, S0T + J * kGT * kGC, lds + slot * kSlotB, w, lane);
+        };
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int e = t + 256 * q, kq = e >> 4, c = e & 15;
-            const int64_t j = j0 + kq;
-            const bool good = c < kk && j < n;
-            const double x = S0[good ? c * ldn + j : 0];
-            sv[q] = good ? x : 0.0;
+        for (int u = 0; u < kGS - 1; ++u) issue(u, u);
+        int cs = 0;
+        for (int t = 0; t < nt; ++t) {
+            // this wave's copies of stage t have landed (those of the kGS - 2 later stages
+            // may be in flight); after the barrier every wave's have, and every wave is done
+            // with the slot the next copies overwrite
+            asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"((kGS - 2) * kCopiesPerStage)
+                         : "memory");
+            issue(t + kGS - 1, cs == 0 ? kGS - 1 : cs - 1);
+            __builtin_amdgcn_sched_barrier(0);
+            const uint8_t* slot = lds + cs * kSlotB;
+            if (J0 + t <= I) grm_mma<false>(slot, w, lane, acc);
+            else grm_mma<true>(slot, w, lane, acc);
+            cs = cs + 1 == kGS ? 0 : cs + 1;
         }
-        __syncthreads();  // the previous tile's MFMAs have read g and sh
-        if (kind == 0)
-            tile_store<0>(g, n, i0, j0, t, v);
-        else if (kind == 1)
-            tile_store<1>(g, n, i0, j0, t, v);
-        else
-            tile_store<2>(g, n, i0, j0, t, v);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) sh[t + 256 * q] = sv[q];
-        __syncthreads();
-        const float* gr = g + (16 * w + (lane & 15)) * kGP + (lane >> 4);
-        const double* sb = sh + (lane >> 4) * kGC + (lane & 15);
-#pragma unroll 4
-        for (int s4 = 0; s4 < kGT / 4; ++s4)
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)gr[4 * s4], sb[4 * s4 * kGC], acc, 0, 0, 0);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     }
     const int col = lane & 15;
     if (col >= kk) return;
+    const int64_t i0 = (int64_t)I * kGT;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int64_t i = i0 + 16 * w + 4 * v + (lane >> 4);
@@ -151,15 +160,51 @@ __global__ __launch_bounds__(256) void k_grm_finish(const double* __restrict__ p
     }
 }
 
-// lt[i(i+1)/2 + j] = (float) G[i][j] for the columns j0 .. j0 + k - 1 of G given in
-// Gc (n x k, column-major), rows i >= j
+// Packed rows r0 .. r1 - 1 of the triangle (lt holds them from lt_index(r0, 0) on) into
+// the tiles; one block per row. The diagonal tile gets both (a, b) and (b, a).
+__global__ __launch_bounds__(256) void k_grm_tiles_from_lt(const float* __restrict__ lt, int64_t r0,
+                                                           float* __restrict__ tiles) {
+    const int64_t i = r0 + blockIdx.x;
+    const float* row = lt + (lt_index(i, 0) - lt_index(r0, 0));
+    const int64_t I = i / kGT;
+    const int a = (int)(i % kGT);
+    for (int64_t j = threadIdx.x; j <= i; j += 256) {
+        const int64_t J = j / kGT;
+        const int b = (int)(j % kGT);
+        float* t = tiles + tile_index(I, J) * (kGT * kGT);
+        const float v = row[j];
+        t[a * kGT + b] = v;
+        if (J == I) t[b * kGT + a] = v;
+    }
+}
+
+// The inverse: packed rows r0 .. r1 - 1 from the tiles into lt (from lt_index(r0, 0) on).
+__global__ __launch_bounds__(256) void k_grm_lt_from_tiles(const float* __restrict__ tiles, int64_t r0,
+                                                           float* __restrict__ lt) {
+    const int64_t i = r0 + blockIdx.x;
+    float* row = lt + (lt_index(i, 0) - lt_index(r0, 0));
+    const int64_t I = i / kGT;
+    const int a = (int)(i % kGT);
+    for (int64_t j = threadIdx.x; j <= i; j += 256)
+        row[j] = tiles[tile_index(I, j / kGT) * (kGT * kGT) + a * kGT + (j % kGT)];
+}
+
+// Columns j0 .. j0 + k - 1 of G (Gc: n x k, column-major; j0 a multiple of 64, k <= 64)
+// into tile column J = j0 / 64, rounded to FP32: rows i >= j, mirrored in the diagonal
+// tile.
 __global__ __launch_bounds__(256) void k_grm_scatter(const double* __restrict__ Gc, int64_t ldg,
                                                      int32_t n, int64_t j0, int k,
-                                                     float* __restrict__ lt) {
+                                                     float* __restrict__ tiles) {
     const int c = blockIdx.y;
-    const int64_t j = j0 + c;
-    for (int64_t i = j + blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
-        lt[lt_index(i, j)] = (float)Gc[c * ldg + i];
+    const int64_t j = j0 + c, J = j0 / kGT;
+    for (int64_t i = j + blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int64_t I = i / kGT;
+        const int a = (int)(i % kGT);
+        float* t = tiles + tile_index(I, J) * (kGT * kGT);
+        const float v = (float)Gc[c * ldg + i];
+        t[a * kGT + c] = v;
+        if (I == J) t[c * kGT + a] = v;
+    }
 }
 
 // S0 = columns j0 .. j0 + k - 1 of the n x n identity
@@ -170,13 +215,33 @@ __global__ __launch_bounds__(256) void k_eye_cols(double* __restrict__ S0, int64
         S0[c * ld + i] = i == j0 + c ? 1.0 : 0.0;
 }
 
-__global__ __launch_bounds__(256) void k_grm_diag(const float* __restrict__ lt, int32_t n,
+__global__ __launch_bounds__(256) void k_grm_diag(const float* __restrict__ tiles, int32_t n,
                                                   double* __restrict__ d) {
-    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
-        d[i] = (double)lt[lt_index(i, i)];
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int64_t I = i / kGT;
+        const int a = (int)(i % kGT);
+        d[i] = (double)tiles[tile_index(I, I) * (kGT * kGT) + a * kGT + a];
+    }
 }
 
 }  // namespace
+
+size_t grm_tile_bytes(int32_t n) {
+    const size_t T = (size_t)(n + kGT - 1) / kGT;
+    return T * (T + 1) / 2 * kTileB;
+}
+
+void launch_grm_tiles_from_lt(hipStream_t s, const float* lt_rows, int64_t r0, int64_t r1, float* tiles) {
+    if (r1 <= r0) return;
+    k_grm_tiles_from_lt<<<(unsigned)(r1 - r0), 256, 0, s>>>(lt_rows, r0, tiles);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_grm_lt_from_tiles(hipStream_t s, const float* tiles, int64_t r0, int64_t r1, float* lt_rows) {
+    if (r1 <= r0) return;
+    k_grm_lt_from_tiles<<<(unsigned)(r1 - r0), 256, 0, s>>>(tiles, r0, lt_rows);
+    HIP_CHECK(hipGetLastError());
+}
 
 void launch_eye_cols(hipStream_t s, double* S0, int64_t ld, int32_t n, int64_t j0, int k) {
     const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 1024));
@@ -184,23 +249,29 @@ void launch_eye_cols(hipStream_t s, double* S0, int64_t ld, int32_t n, int64_t j
     HIP_CHECK(hipGetLastError());
 }
 
-void launch_grm_diag(hipStream_t s, const float* lt, int32_t n, double* d) {
+void launch_grm_diag(hipStream_t s, const float* tiles, int32_t n, double* d) {
     const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 1024));
-    k_grm_diag<<<gx, 256, 0, s>>>(lt, n, d);
+    k_grm_diag<<<gx, 256, 0, s>>>(tiles, n, d);
     HIP_CHECK(hipGetLastError());
 }
 
-void launch_grm_product(hipStream_t s, const float* lt, int32_t n, const double* S0, int64_t ldn,
+void launch_grm_product(hipStream_t s, const float* tiles, int32_t n, const double* S0, int64_t ldn,
                         int k, const double* theta_cols, double* GZ, int64_t ldg, const int* skip,
                         double* part) {
     if (n == 0) return;
     const int32_t ntiles = (n + kGT - 1) / kGT;
     const dim3 g(ntiles, kGrmChunks);
     const unsigned gf = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
-    // one pass over the triangle per group of up to 16 columns
+    // one pass over the triangle per group of up to 16 columns; S0T sits after the strip
+    // partials in the scratch (16-byte aligned: the partials are 512 n bytes)
+    const int64_t npad = (int64_t)ntiles * kGT;
+    double* S0T = part + (size_t)kGrmChunks * kGC * n;
+    const unsigned gt = (unsigned)std::min<int64_t>((npad * kGC + 255) / 256, 1024);
     for (int c = 0; c < k; c += kGC) {
         const int kk = std::min(kGC, k - c);
-        k_grm_tiles<<<g, 256, 0, s>>>(lt, n, S0 + c * ldn, ldn, kk, ntiles, part, skip);
+        k_grm_s0t<<<gt, 256, 0, s>>>(S0 + c * ldn, ldn, n, kk, npad, S0T, skip);
+        HIP_CHECK(hipGetLastError());
+        k_grm_tiles<<<g, 256, 0, s>>>(tiles, n, S0T, kk, ntiles, part, skip);
         HIP_CHECK(hipGetLastError());
         k_grm_finish<<<dim3(gf, kk), 256, 0, s>>>(part, n, kGC, theta_cols + c * 5, GZ + c * ldg, ldg,
                                                   skip);
@@ -208,14 +279,17 @@ void launch_grm_product(hipStream_t s, const float* lt, int32_t n, const double*
     }
 }
 
-size_t grm_product_part_bytes(int32_t n) { return sizeof(double) * (size_t)kGrmChunks * 16 * (size_t)n; }
+size_t grm_product_part_bytes(int32_t n) {
+    const size_t npad = (size_t)(n + kGT - 1) / kGT * kGT;
+    return sizeof(double) * ((size_t)kGrmChunks * kGC * (size_t)n + npad * kGC);
+}
 
 void launch_grm_scatter(hipStream_t s, const double* Gc, int64_t ldg, int32_t n, int64_t j0, int k,
-                        float* lt) {
+                        float* tiles) {
     if (k <= 0) return;
     const int64_t rows = n - j0;
     const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((rows + 255) / 256, 1024));
-    k_grm_scatter<<<dim3(gx, k), 256, 0, s>>>(Gc, ldg, n, j0, k, lt);
+    k_grm_scatter<<<dim3(gx, k), 256, 0, s>>>(Gc, ldg, n, j0, k, tiles);
     HIP_CHECK(hipGetLastError());
 }
 
