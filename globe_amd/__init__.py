@@ -130,6 +130,8 @@ def load_library(path: str = LIB_PATH):
         "globe_profile_read": ([P, C.c_int, P], C.c_int),
         "globe_aireml_step": ([P, P, P, dbl, P], C.c_int),
         "globe_aireml_step_reduced": ([P, P, P, dbl, P], C.c_int),
+        "globe_aireml_probes": ([P, C.c_int, C.c_int, C.c_uint64, P, P], C.c_int),
+        "globe_probe_seed": ([C.c_uint64, C.c_uint64], C.c_uint64),
         "globe_aireml_fit": ([P, P, P, P, P, P, P, P, P, P], C.c_int),
         "globe_aireml_fit_reduced": ([P, P, P, P, P, P, P, P, P, P], C.c_int),
         "globe_grm_from_geno": ([P], C.c_int),
@@ -378,6 +380,16 @@ class Globe:
         self._check(fn(self.h, _p(y), _p(probes), _p(weights), _p(x0), nmc,
                        C.byref(o), _p(grad), _p(ai), _p(first), _p(tr)))
         return {"grad": grad, "ai": ai.reshape(nv, nv), "first": first, "trace": tr}
+
+    def aireml_probes(self, kind, nmc, seed):
+        """The probes and trace weights the AI-REML fit draws on the device (Philox4x32-10
+        keyed by seed): kind "gaussian" (tilde-V probes, weights tilde-V^-1 probe),
+        "rademacher" or "exact"."""
+        k = {"gaussian": 0, "rademacher": 1, "exact": 2}[kind]
+        Y = np.zeros((self.nrec, nmc), order="F")
+        Wt = np.zeros((self.nrec, nmc), order="F")
+        self._check(self.L.globe_aireml_probes(self.h, k, nmc, C.c_uint64(seed), _p(Y), _p(Wt)))
+        return Y, Wt
 
     def geno_Wa(self, a):
         a = np.ascontiguousarray(a, dtype=np.float64)

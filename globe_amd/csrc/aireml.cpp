@@ -334,12 +334,22 @@ void Logger::operator()(const char* fmt, ...) const {
     fn(buf, user);
 }
 
+// splitmix64 finaliser (Steele, Lea and Flood, OOPSLA 2014): decorrelates nearby seeds
 uint64_t mix(uint64_t z) {
     z += 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
 }
+
+}  // namespace
+
+// Philox key of evaluation `eval_id` of a fit seeded with `seed`.
+extern "C" uint64_t globe_probe_seed(uint64_t seed, uint64_t eval_id) { return mix(seed + 0x9E37ull * (eval_id + 1)); }
+
+namespace {
+
+uint64_t probe_seed(uint64_t seed, uint64_t eval_id) { return globe_probe_seed(seed, eval_id); }
 
 }  // namespace
 
@@ -357,81 +367,6 @@ extern "C" int globe_aireml_step_reduced(const double grad[2], const double ai[4
 
 namespace {
 
-// Probes and trace weights for one gradient evaluation.
-//   0, Gaussian tilde-V (varCompEst.cpp:775-862; reduced :563-573, :592-602): y ~ N(0,
-//      tilde-V); weights = x0 = tilde-V^-1 y by the per-individual 2x2 Woodbury solve on
-//      the host.
-//   1, Rademacher: z in {-1, +1}; weights = z; PCG starts from Q^-1 z.
-//   2, exact: the unit vectors sqrt(nrec) e_s, s < nrec (nmc must equal nrec); weights =
-//      probe, so the mean of the per-probe terms is the exact trace.
-struct Probes {
-    std::vector<double> y, w;
-    bool has_x0 = false;
-};
-
-Probes make_probes(globe_ctx* c, const double th[5], int nmc, int kind, uint64_t seed) {
-    Probes P;
-    const int64_t nrec = c->nrec;
-    const int32_t n = c->n;
-    P.y.assign((size_t)nrec * nmc, 0.0);
-    P.w.assign((size_t)nrec * nmc, 0.0);
-    const double se = th[0], sg = th[1], sb0 = th[2], sb01 = th[3], sb1 = th[4];
-    if (kind == 2) {
-        globe::require(nmc == nrec, GLOBE_E_INVALID, "exact probes need numMC equal to the record count");
-        for (int s = 0; s < nmc; ++s) P.y[(size_t)s * nrec + s] = P.w[(size_t)s * nrec + s] = std::sqrt((double)nrec);
-        return P;
-    }
-    for (int s = 0; s < nmc; ++s) {
-        std::mt19937_64 gen(mix(seed ^ (0x1000193ull * (uint64_t)(s + 1))));
-        double* ys = P.y.data() + (size_t)s * nrec;
-        double* ws = P.w.data() + (size_t)s * nrec;
-        if (kind == 1) {
-            for (int64_t r = 0; r < nrec; ++r) ys[r] = (gen() >> 63) ? 1.0 : -1.0;
-            std::memcpy(ws, ys, sizeof(double) * nrec);
-            continue;
-        }
-        std::normal_distribution<double> nd(0.0, 1.0);
-        // Cholesky factor of B (compute_chol_2x2, :800-813); with sig2b0 = 0 (the reduced
-        // model) the slope terms vanish
-        const double l00 = sb0 > 0.0 ? std::sqrt(sb0) : 0.0;
-        const double l10 = sb0 > 0.0 ? sb01 / l00 : 0.0;
-        double d11 = sb1 - (sb0 > 0.0 ? sb01 * sb01 / sb0 : 0.0);
-        if (d11 < 0) d11 = 0;  // clipped as compute_chol_2x2 does (:806-810)
-        const double l11 = std::sqrt(d11);
-        std::vector<double> z1(n), z2(2 * (size_t)n);
-        for (auto& v : z1) v = nd(gen);
-        for (auto& v : z2) v = nd(gen);
-        for (int64_t r = 0; r < nrec; ++r) {
-            const int32_t i = c->h_indiv[r];
-            const double t = c->h_times[r];
-            ys[r] = std::sqrt(sg) * z1[i] + std::sqrt(se) * nd(gen) + z2[2 * i] * (l00 + l10 * t) +
-                    z2[2 * i + 1] * l11 * t;
-        }
-        // tilde-V^-1 y per individual block: Q = se I + U C U^T, C = [[sg + sb0, sb01], [sb01, sb1]]
-        for (int32_t b = 0; b < n; ++b) {
-            const int64_t r0 = c->h_blk_off[b], r1 = c->h_blk_off[b + 1];
-            double a01 = 0, a11 = 0, w0 = 0, w1 = 0;
-            for (int64_t q = r0; q < r1; ++q) {
-                const double t = c->h_times[q];
-                a01 += t;
-                a11 += t * t;
-                w0 += ys[q];
-                w1 += t * ys[q];
-            }
-            const double a00 = (double)(r1 - r0);
-            const double c00 = sg + sb0, c01 = sb01, c11 = sb1;
-            const double m00 = se + c00 * a00 + c01 * a01, m01 = c00 * a01 + c01 * a11;
-            const double m10 = c01 * a00 + c11 * a01, m11 = se + c01 * a01 + c11 * a11;
-            const double v0 = c00 * w0 + c01 * w1, v1 = c01 * w0 + c11 * w1;
-            const double det = m00 * m11 - m01 * m10;
-            const double k0 = (m11 * v0 - m01 * v1) / det, k1 = (m00 * v1 - m10 * v0) / det;
-            for (int64_t q = r0; q < r1; ++q) ws[q] = (ys[q] - k0 - k1 * c->h_times[q]) / se;
-        }
-    }
-    P.has_x0 = kind == 0;
-    return P;
-}
-
 struct Eval {
     double grad[KMAX], ai[KMAX * KMAX];  // grad (nv), AI nv x nv row-major
 };
@@ -441,16 +376,20 @@ void full_theta(int nv, const double* th, double* th5) {
     for (int j = 0; j < KMAX; ++j) th5[j] = j < nv ? th[j] : 0.0;
 }
 
+// One gradient (and AI) evaluation at th with fresh probes: generated on the device from
+// a Philox key that mixes the fit's seed with the evaluation's number (k_probe.hip;
+// globe_aireml_probes returns the same probes). The reference draws new Gaussian tilde-V
+// probes for every evaluation too, from std::random_device (varCompEst.cpp:763-797).
 void evaluate(globe_ctx* c, const double* y, int nv, const double* th, const globe_aireml_opts& o,
               uint64_t eval_id, bool want_ai, Eval& e) {
     double th5[KMAX];
     full_theta(nv, th, th5);
     int rc = globe_set_varcomp(c, th5);
     if (rc != GLOBE_OK) throw Error(rc, c->err);
-    Probes P = make_probes(c, th5, o.num_mc, o.probes, mix(o.seed + 0x9E37ull * (eval_id + 1)));
+    const globe::ProbeGen gen{o.probes, probe_seed(o.seed, eval_id)};
     globe_pcg_opts po = o.pcg;
-    rc = globe::aireml_eval_impl(c, y, P.y.data(), P.w.data(), P.has_x0 ? P.w.data() : nullptr,
-                                 o.num_mc, &po, want_ai, e.grad, e.ai, nullptr, nullptr, nv);
+    rc = globe::aireml_eval_impl(c, y, nullptr, nullptr, nullptr, o.num_mc, &po, want_ai, e.grad, e.ai,
+                                 nullptr, nullptr, nv, &gen);
     if (rc != GLOBE_OK) throw Error(rc, c->err);
 }
 
@@ -603,7 +542,8 @@ int fit_impl(globe_ctx* c, const double* y, const globe_aireml_opts* opts, int n
                         // (computeDerivative_AImat, :1321-1330)
                         globe_pcg_opts po = o.pcg;
                         const int arc = globe::aireml_eval_impl(c, nullptr, nullptr, nullptr, nullptr, 0, &po,
-                                                                true, nullptr, cur.ai, nullptr, nullptr, nv);
+                                                                true, nullptr, cur.ai, nullptr, nullptr, nv,
+                                                                nullptr);
                         if (arc != GLOBE_OK) throw Error(arc, c->err);
                         num_rej = 0;
                         accepted = true;

@@ -190,6 +190,25 @@ static void precond(globe_ctx* c, const double* r, int64_t ldr, int k, double* z
                    ldz, part_rz, part_rr, st, 0);
 }
 
+// nmc probes (nrec x nmc) and their trace weights at the context's variance components:
+// Gaussian tilde-V probes (kind 0) are weighted by tilde-V^-1 y, the per-individual
+// blocks of sig2g 1 1^T + U B U^T + sig2e I (estimateTrace_PdV, varCompEst.cpp:839-862),
+// applied by the preconditioner kernel with G_ii := 1; the others by themselves.
+void make_probes_dev(globe_ctx* c, int kind, int nmc, uint64_t seed, double* Y, double* Wt) {
+    const int64_t nrec = c->nrec;
+    hipStream_t s = c->stream;
+    launch_probes(s, kind, nrec, nmc, seed, c->d_indiv.as<int32_t>(), c->d_times.as<double>(),
+                  c->theta.data(), Y, nrec);
+    if (kind == 0) {
+        ensure_gdiag(c);
+        ensure_theta(c, nmc);
+        launch_precond(s, c->meta, c->d_gdiag.as<double>(), c->d_theta.as<double>(), Y, nrec, nmc, Wt, nrec,
+                       nullptr, nullptr, nullptr, 1);
+    } else {
+        launch_copy_cols(s, nrec, Y, nrec, Wt, nrec, nmc);
+    }
+}
+
 // PCG with the reference's control flow (PCGsolver.cpp:1324-1444 for rule 0,
 // varCompEst.cpp:453-528 for rule 1), all scalars on the device. The host enqueues
 // batches of iterations and reads back only the done flag between batches.
@@ -853,7 +872,7 @@ int globe_aireml_eval(globe_ctx* c, const double* y, const double* probes, const
         return GLOBE_E_INVALID;
     }
     return globe::aireml_eval_impl(c, y, probes, weights, probe_x0, nmc, opts, true, grad, ai,
-                                   first_terms, trace, 5);
+                                   first_terms, trace, 5, nullptr);
 }
 
 int globe_aireml_eval_reduced(globe_ctx* c, const double* y, const double* probes, const double* weights,
@@ -865,7 +884,24 @@ int globe_aireml_eval_reduced(globe_ctx* c, const double* y, const double* probe
         return GLOBE_E_INVALID;
     }
     return globe::aireml_eval_impl(c, y, probes, weights, probe_x0, nmc, opts, true, grad, ai,
-                                   first_terms, trace, 2);
+                                   first_terms, trace, 2, nullptr);
+}
+
+int globe_aireml_probes(globe_ctx* c, int kind, int nmc, uint64_t seed, double* probes, double* weights) {
+    API_BEGIN(c)
+    require(c->has_cohort && c->has_theta, GLOBE_E_STATE, "cohort and variance components required");
+    require(kind >= 0 && kind <= 2, GLOBE_E_INVALID, "unknown probe kind");
+    require(nmc >= 1 && nmc <= kMaxCols, GLOBE_E_INVALID, "nmc must be in [1, 64]");
+    require(kind != 2 || nmc == c->nrec, GLOBE_E_INVALID, "exact probes need nmc equal to the record count");
+    const size_t bytes = sizeof(double) * (size_t)c->nrec * nmc;
+    DevBuf Y, Wt;
+    Y.ensure(bytes);
+    Wt.ensure(bytes);
+    make_probes_dev(c, kind, nmc, seed, Y.as<double>(), Wt.as<double>());
+    if (probes) HIP_CHECK(hipMemcpyAsync(probes, Y.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    if (weights) HIP_CHECK(hipMemcpyAsync(weights, Wt.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    API_END(c)
 }
 
 }  // extern "C"
@@ -882,10 +918,14 @@ namespace globe {
 // trace hold nvc values, ai nvc x nvc row-major.
 int aireml_eval_impl(globe_ctx* c, const double* y, const double* probes, const double* weights,
                      const double* probe_x0, int nmc, const globe_pcg_opts* opts, bool want_ai,
-                     double* grad, double* ai, double* first_terms, double* trace, int nvc) {
+                     double* grad, double* ai, double* first_terms, double* trace, int nvc,
+                     const ProbeGen* gen) {
     API_BEGIN(c)
     require(opts != nullptr, GLOBE_E_INVALID, "globe_aireml_eval: null options");
     require(nvc == 5 || nvc == 2, GLOBE_E_INVALID, "2 or 5 variance components");
+    require(!gen || (gen->kind >= 0 && gen->kind <= 2), GLOBE_E_INVALID, "unknown probe kind");
+    require(!gen || gen->kind != 2 || nmc == c->nrec, GLOBE_E_INVALID,
+            "exact probes need numMC equal to the record count");
     if (nvc == 2) {
         require(c->has_cohort && c->nrec == c->n, GLOBE_E_INVALID,
                 "the reduced model takes exactly one record per individual");
@@ -893,7 +933,7 @@ int aireml_eval_impl(globe_ctx* c, const double* y, const double* probes, const 
                 "the reduced model has no random intercept or slope (sig2b0 = sigb01 = sig2b1 = 0)");
     }
     require(nmc >= 0 && nmc <= kMaxCols, GLOBE_E_INVALID, "nmc must be in [0, 64]");
-    require(nmc == 0 || (probes && weights), GLOBE_E_INVALID, "probes/weights required");
+    require(nmc == 0 || gen || (probes && weights), GLOBE_E_INVALID, "probes/weights required");
     require(c->has_cohort && c->gen != 0, GLOBE_E_STATE, "cohort and genetics required");
     const int64_t nrec = c->nrec;
     const int32_t n = c->n;
@@ -939,12 +979,22 @@ int aireml_eval_impl(globe_ctx* c, const double* y, const double* probes, const 
         c->e_fs.ensure(sizeof(double) * nrec * 5 * nmc);
         c->s_aux.ensure(sizeof(double) * nrec * nmc);
         double* sv = c->e_sv.as<double>();
-        HIP_CHECK(hipMemcpy(c->e_probe.p, probes, sizeof(double) * nrec * nmc, hipMemcpyHostToDevice));
-        HIP_CHECK(hipMemcpy(c->s_aux.p, weights, sizeof(double) * nrec * nmc, hipMemcpyHostToDevice));
         globe_pcg_opts ot = o;
-        if (probe_x0) {
-            HIP_CHECK(hipMemcpy(sv, probe_x0, sizeof(double) * nrec * nmc, hipMemcpyHostToDevice));
-            ot.x0_mode = GLOBE_X0_GIVEN;
+        if (gen) {
+            // probes on the device; weights = tilde-V^-1 probe (the start of its solve) for
+            // the Gaussian tilde-V probes, the probe itself otherwise
+            make_probes_dev(c, gen->kind, nmc, gen->seed, c->e_probe.as<double>(), c->s_aux.as<double>());
+            if (gen->kind == 0) {
+                launch_copy_cols(s, nrec, c->s_aux.as<double>(), nrec, sv, nrec, nmc);
+                ot.x0_mode = GLOBE_X0_GIVEN;
+            }
+        } else {
+            HIP_CHECK(hipMemcpy(c->e_probe.p, probes, sizeof(double) * nrec * nmc, hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(c->s_aux.p, weights, sizeof(double) * nrec * nmc, hipMemcpyHostToDevice));
+            if (probe_x0) {
+                HIP_CHECK(hipMemcpy(sv, probe_x0, sizeof(double) * nrec * nmc, hipMemcpyHostToDevice));
+                ot.x0_mode = GLOBE_X0_GIVEN;
+            }
         }
         run_pcg(c, c->e_probe.as<double>(), nrec, nmc, sv, nrec, ot, nullptr, nullptr, nullptr);
         apply_P(c, f, sv, nmc);

@@ -128,12 +128,21 @@ void require(bool cond, int code, const std::string& msg);
 // with `limbs` int8 digits per value of the right-hand side
 void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, double* GZ,
                        int64_t ldg, const int* skip, int limbs, const double* theta_cols);
+// Probes generated on the device (k_probe.hip): kind 0 Gaussian tilde-V, 1 Rademacher,
+// 2 exact unit probes; seed = the Philox key.
+struct ProbeGen {
+    int kind;
+    uint64_t seed;
+};
+void make_probes_dev(globe_ctx* c, int kind, int nmc, uint64_t seed, double* Y, double* Wt);
 // AIREML derivative terms at the current variance components for nvc = 5 (full) or
-// 2 (reduced) components; y == nullptr reuses the previous fit at the same components
-// (for the AI matrix of an accepted step)
+// 2 (reduced) components, with host probes (probes, weights, probe_x0) or, when gen is
+// set, probes generated on the device; y == nullptr reuses the previous fit at the same
+// components (for the AI matrix of an accepted step)
 int aireml_eval_impl(globe_ctx* c, const double* y, const double* probes, const double* weights,
                      const double* probe_x0, int nmc, const globe_pcg_opts* opts, bool want_ai,
-                     double* grad, double* ai, double* first_terms, double* trace, int nvc);
+                     double* grad, double* ai, double* first_terms, double* trace, int nvc,
+                     const ProbeGen* gen);
 constexpr int kLimbsFP64 = 7;
 constexpr int kLimbsMixed = 4;
 

@@ -153,6 +153,12 @@ void launch_colsum_parts(hipStream_t s, int64_t rows, const double* v, int64_t l
 void launch_reduce_parts_skip(hipStream_t s, const double* part, int nparts, int k, double* out,
                               const int* skip);
 
+// ---------------------------------------------------------------- probes (k_probe.hip)
+// nmc columns (leading dimension ldy) of Philox-seeded probes: kind 0 Gaussian tilde-V at
+// theta (per-record individual and time), 1 Rademacher, 2 unit vectors sqrt(nrec) e_s
+void launch_probes(hipStream_t s, int kind, int64_t nrec, int nmc, uint64_t seed, const int32_t* indiv,
+                   const double* times, const double theta[5], double* Y, int64_t ldy);
+
 // ---------------------------------------------------------------- GRM
 // The device GRM is tile-major (k_grm.hip): 64 x 64 FP32 tiles (I, J <= I) at tile
 // index I(I+1)/2 + J, diagonal tiles mirrored, zero past n; grm_tile_bytes(n) bytes.

@@ -207,6 +207,19 @@ int globe_aireml_eval_reduced(globe_ctx* ctx, const double* y, const double* pro
 int globe_aireml_step_reduced(const double grad[2], const double ai[4], const double theta[2],
                               double delta, double step[2]);
 
+/* Replaces generate_standard_normal + sample_Yvs and the tilde-V^-1 solve of
+ * estimateTrace_PdV (varCompEst.cpp:763-862): nmc probes (nrec x nmc, column-major) and
+ * their trace weights at the context's variance components, generated on the device
+ * from Philox4x32-10 keyed by `seed`. kind 0: Gaussian tilde-V probes (weights tilde-V^-1
+ * probe; the reference draws them from std::random_device); 1: Rademacher (weights = the
+ * probes); 2: unit vectors sqrt(nrec) e_s (nmc == nrec; the trace terms are then exact).
+ * Either output may be NULL. globe_aireml_fit uses seed globe_probe_seed(opts.seed, e)
+ * for its evaluation e = 0, 1, ...; these calls return the same probes, for
+ * globe_aireml_eval. */
+int globe_aireml_probes(globe_ctx* ctx, int kind, int nmc, uint64_t seed, double* probes,
+                        double* weights);
+uint64_t globe_probe_seed(uint64_t seed, uint64_t eval_id);
+
 typedef void (*globe_log_fn)(const char* line, void* user);
 
 typedef struct {

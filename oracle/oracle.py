@@ -369,3 +369,55 @@ def bed_qc(rows, n, min_maf, max_miss):
     kept = lib().orc_bed_qc(_ptr(rows), m, n, rows.shape[1], min_maf, max_miss,
                             _ptr(keep), _ptr(maf), _ptr(mr))
     return keep.astype(bool), maf, mr, int(kept)
+
+
+# ------------------------------------------------------------------ probe generator
+def philox4x32_10(c0, c1, c2, c3, k0, k1):
+    """Philox4x32-10 (Salmon, Moraes, Dror and Shaw, SC'11) on numpy arrays of counter
+    words (broadcast), key (k0, k1): the generator of the device probes. Pinned by the
+    published known-answer vectors (tests/test_oracle.py)."""
+    m = np.uint64(0xFFFFFFFF)
+    c = [np.asarray(x, dtype=np.uint64) & m for x in (c0, c1, c2, c3)]
+    k0, k1 = np.uint64(k0) & m, np.uint64(k1) & m
+    for _ in range(10):
+        p0 = np.uint64(0xD2511F53) * c[0]
+        p1 = np.uint64(0xCD9E8D57) * c[2]
+        c = [((p1 >> np.uint64(32)) ^ c[1] ^ k0) & m, p1 & m, ((p0 >> np.uint64(32)) ^ c[3] ^ k1) & m, p0 & m]
+        k0 = (k0 + np.uint64(0x9E3779B9)) & m
+        k1 = (k1 + np.uint64(0xBB67AE85)) & m
+    return c
+
+
+def probes(kind, indiv, times, theta, nmc, seed):
+    """The AI-REML probes as the device draws them, restated: kind 0 Gaussian tilde-V
+    (sample_Yvs, varCompEst.cpp:775-797, with Box-Muller normals from the Philox stream),
+    1 Rademacher, 2 unit vectors sqrt(nrec) e_s. Returns nrec x nmc."""
+    indiv = np.asarray(indiv, dtype=np.int64)
+    times = np.asarray(times, dtype=np.float64)
+    nrec = indiv.shape[0]
+    k0, k1 = seed & 0xFFFFFFFF, seed >> 32
+    r = np.arange(nrec, dtype=np.uint64)
+    out = np.zeros((nrec, nmc), order="F")
+    se, sg, sb0, sb01, sb1 = theta
+    l00 = np.sqrt(sb0) if sb0 > 0 else 0.0
+    l10 = sb01 / l00 if sb0 > 0 else 0.0
+    d11 = max(sb1 - (sb01 * sb01 / sb0 if sb0 > 0 else 0.0), 0.0)
+    l11 = np.sqrt(d11)
+    unit = lambda w: (w.astype(np.float64) + 0.5) * 2.0 ** -32  # noqa: E731
+    for s in range(nmc):
+        if kind == 1:
+            w = philox4x32_10(r, r >> np.uint64(32), s, 0, k0, k1)
+            out[:, s] = np.where(w[0] >> np.uint64(31), 1.0, -1.0)
+        elif kind == 2:
+            out[s, s] = np.sqrt(nrec)
+        else:
+            iu = indiv.astype(np.uint64)
+            a = philox4x32_10(iu, iu >> np.uint64(32), s, 1, k0, k1)
+            b = philox4x32_10(r, r >> np.uint64(32), s, 2, k0, k1)
+            ra = np.sqrt(-2.0 * np.log(unit(a[0])))
+            z1 = ra * np.cos(2 * np.pi * unit(a[1]))
+            z2a = ra * np.sin(2 * np.pi * unit(a[1]))
+            z2b = np.sqrt(-2.0 * np.log(unit(a[2]))) * np.cos(2 * np.pi * unit(a[3]))
+            z3 = np.sqrt(-2.0 * np.log(unit(b[0]))) * np.cos(2 * np.pi * unit(b[1]))
+            out[:, s] = np.sqrt(sg) * z1 + np.sqrt(se) * z3 + z2a * (l00 + l10 * times) + z2b * l11 * times
+    return out

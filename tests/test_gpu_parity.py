@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import globe_amd as G
+import oracle as O
 from globe_amd import synth
 from helpers import golden_oracle, load_golden, rel
 
@@ -242,6 +243,28 @@ def test_aireml_eval_matches_oracle(nmc, gpu_ctx_factory):
     og = g.aireml_eval(d["y"], yv, wv, wv, tol=1e-10)
     oo = orc.aireml_eval(d["y"], yv, wv, wv, tol=1e-10)
     assert rel(og["grad"], oo["grad"]) < 1e-6 and rel(og["ai"], oo["ai"]) < 1e-6
+
+
+@pytest.mark.parametrize("kind", ["rademacher", "gaussian"])
+def test_device_probes_match_oracle(kind, gpu_ctx_factory):
+    """Probes drawn on the device (Philox4x32-10) equal the oracle's restatement;
+    Gaussian tilde-V weights equal tilde-V^-1 probe. (Exact unit probes are exercised by
+    the reduced model's exact-trace fit, tests/test_reduced.py.)"""
+    _say("device-probes", kind)
+    d = load_golden("geno_n150_odd")
+    th = (0.6, 0.4, 0.4, 0.2, 0.2)
+    g = device_for(d, gpu_ctx_factory, theta=th)
+    orc = golden_oracle(d, theta=th)
+    c = d["cohort"]
+    seed = int(g.L.globe_probe_seed(20261203, 3))
+    nmc = 9
+    Y, Wt = g.aireml_probes(kind, nmc, seed)
+    want = O.probes({"gaussian": 0, "rademacher": 1}[kind], c.indiv, c.times, th, nmc, seed)
+    if kind == "gaussian":
+        assert rel(Y, want) < 1e-13
+        assert rel(Wt, orc.tildeV_solve(Y)) < 1e-12
+    else:
+        assert np.array_equal(Y, want) and np.array_equal(Wt, Y)
 
 
 def test_invalid_cohorts_rejected(gpu_ctx_factory):

@@ -227,3 +227,29 @@ def test_tildeV_sampling_covariance_and_solve():
     assert rel(A @ A.T, Vt) < 1e-12
     yv = np.random.default_rng(4).standard_normal((c.nrec, 2))
     assert rel(orc.tildeV_solve(yv), np.linalg.solve(Vt, yv)) < 1e-12
+
+
+def test_philox_known_answers_and_probe_moments():
+    """Philox4x32-10 against the published known-answer vectors (Random123), then the
+    probe generator's moments: Rademacher signs balanced, Gaussian tilde-V probes with
+    covariance tilde-V (statistical, loose)."""
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+           ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+           ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+            (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1))]
+    for ctr, key, want in kat:
+        got = O.philox4x32_10(*ctr, *key)
+        assert [int(x) for x in got] == list(want)
+    c = synth.simulate_cohort(12, 2.0, np.random.default_rng(5))
+    th = (0.6, 0.4, 0.4, 0.2, 0.2)
+    z = O.probes(1, c.indiv, c.times, th, 64, 12345)
+    assert set(np.unique(z)) == {-1.0, 1.0} and abs(z.mean()) < 0.05
+    yv = O.probes(0, c.indiv, c.times, th, 4000, 777)
+    se, sg, sb0, sb01, sb1 = th
+    t = c.times
+    same = (c.indiv[:, None] == c.indiv[None, :]).astype(float)
+    Vt = same * (sg + sb0 + sb01 * (t[:, None] + t[None, :]) + sb1 * np.outer(t, t)) + se * np.eye(c.nrec)
+    emp = yv @ yv.T / yv.shape[1]
+    assert np.abs(emp - Vt).max() < 0.1 * np.abs(Vt).max()
+    e = O.probes(2, c.indiv, c.times, th, c.nrec, 1)
+    assert np.allclose(e, np.sqrt(c.nrec) * np.eye(c.nrec))
