@@ -606,18 +606,13 @@ int globe_grm_from_geno(globe_ctx* c) {
     constexpr int kb = 64;  // columns of G per contraction = one tile column
     c->d_grm.ensure(grm_tile_bytes(n));
     HIP_CHECK(hipMemsetAsync(c->d_grm.p, 0, grm_tile_bytes(n), s));
-    DevBuf S0, GZ, th;
-    S0.ensure(sizeof(double) * (size_t)n * kb);
+    DevBuf GZ;
     GZ.ensure(sizeof(double) * (size_t)n * kb);
-    th.ensure(sizeof(double) * 5 * kMaxCols);
-    std::vector<double> h((size_t)5 * kMaxCols, 0.0);
-    for (int j = 0; j < kMaxCols; ++j) h[5 * j + 1] = 1.0;  // sig2g = 1: the product is G S0
-    HIP_CHECK(hipMemcpyAsync(th.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    // G e_j exactly (a unit vector is exact in the 54-bit fixed point), rounded to FP32
+    // G e_j in FP64 (W^T e_j is individual j's standardised row; W u runs on 7 int8
+    // digits, as accurate as FP64), rounded to FP32
     for (int64_t j0 = 0; j0 < n; j0 += kb) {
         const int k = (int)std::min<int64_t>(kb, n - j0);
-        launch_eye_cols(s, S0.as<double>(), n, n, j0, k);
-        genetic_product(c, S0.as<double>(), k, GZ.as<double>(), GLOBE_PREC_FP64, nullptr, th.as<double>());
+        geno_unit_columns(c, j0, k, GZ.as<double>());
         launch_grm_scatter(s, GZ.as<double>(), n, n, j0, k, c->d_grm.as<float>());
     }
     HIP_CHECK(hipStreamSynchronize(s));

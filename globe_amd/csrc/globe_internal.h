@@ -174,7 +174,6 @@ size_t grm_product_part_bytes(int32_t n);
 // tile column j0 / 64 of G (FP32) for i >= j0 + c from the n x k columns Gc (j0 % 64 == 0)
 void launch_grm_scatter(hipStream_t s, const double* Gc, int64_t ldg, int32_t n, int64_t j0, int k,
                         float* tiles);
-void launch_eye_cols(hipStream_t s, double* S0, int64_t ld, int32_t n, int64_t j0, int k);
 void launch_grm_diag(hipStream_t s, const float* tiles, int32_t n, double* d);
 
 }  // namespace globe

@@ -207,14 +207,6 @@ __global__ __launch_bounds__(256) void k_grm_scatter(const double* __restrict__ 
     }
 }
 
-// S0 = columns j0 .. j0 + k - 1 of the n x n identity
-__global__ __launch_bounds__(256) void k_eye_cols(double* __restrict__ S0, int64_t ld, int32_t n,
-                                                  int64_t j0, int k) {
-    const int c = blockIdx.y;
-    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
-        S0[c * ld + i] = i == j0 + c ? 1.0 : 0.0;
-}
-
 __global__ __launch_bounds__(256) void k_grm_diag(const float* __restrict__ tiles, int32_t n,
                                                   double* __restrict__ d) {
     for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
@@ -240,12 +232,6 @@ void launch_grm_tiles_from_lt(hipStream_t s, const float* lt_rows, int64_t r0, i
 void launch_grm_lt_from_tiles(hipStream_t s, const float* tiles, int64_t r0, int64_t r1, float* lt_rows) {
     if (r1 <= r0) return;
     k_grm_lt_from_tiles<<<(unsigned)(r1 - r0), 256, 0, s>>>(tiles, r0, lt_rows);
-    HIP_CHECK(hipGetLastError());
-}
-
-void launch_eye_cols(hipStream_t s, double* S0, int64_t ld, int32_t n, int64_t j0, int k) {
-    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 1024));
-    k_eye_cols<<<dim3(gx, k), 256, 0, s>>>(S0, ld, n, j0, k);
     HIP_CHECK(hipGetLastError());
 }
 

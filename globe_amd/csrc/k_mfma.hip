@@ -566,6 +566,22 @@ __global__ __launch_bounds__(256) void k_quantize(const double* __restrict__ v, 
     }
 }
 
+// Pass 1 of G e_j for the unit vectors e_j0 .. e_j0+k-1 without a contraction: W^T e_j
+// is individual j's standardised row, so u[c][s] = inv_s^2 / M (g_{s, j0+c} - 2 f_s), the
+// pass-1 epilogue's value (sig2g = 1, T = 1). g from the individual-major dosage rows.
+__global__ __launch_bounds__(256) void k_unit_u(const uint8_t* __restrict__ ind, int64_t pitch, int64_t j0,
+                                                int64_t m, const double* __restrict__ twof,
+                                                const double* __restrict__ inv, double inv_m,
+                                                double* __restrict__ U) {
+    const int c = blockIdx.y;
+    const uint8_t* row = ind + (j0 + c) * pitch;
+    for (int64_t q = blockIdx.x * 256ll + threadIdx.x; q < m; q += (int64_t)gridDim.x * 256) {
+        const int g = (row[q >> 2] >> (2 * (q & 3))) & 3;
+        const double iv = inv[q];
+        U[c * m + q] = inv_m * iv * iv * ((double)g - twof[q]);
+    }
+}
+
 // Epilogues: a block takes 64 rows x k columns. Phase 1 reads the digit sums with
 // consecutive threads on consecutive digits (coalesced), phase 2 writes the FP64
 // results column-major with consecutive threads on consecutive rows.
@@ -837,7 +853,7 @@ static std::vector<Variant> candidates(bool wide, int JT, int64_t nrows, int64_t
 
 // GZ (n x k) = sig2g G S0 for the packed-genotype cohort with `limbs` int8 digits.
 void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, double* GZ, int64_t ldg,
-                       const int* skip, int limbs, const double* theta_cols) {
+                       const int* skip, int limbs, const double* theta_cols, const double* U_given) {
     hipStream_t s = c->stream;
     const int64_t n = c->n, m = c->m;
     const int J = k * limbs;
@@ -927,6 +943,14 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
         return v.S;
     };
 
+    int64_t eb1 = 0;
+    if (U_given) {
+        // pass 1 done by the caller: u = U_given (m x k); its column maxima set the scales
+        U = const_cast<double*>(U_given);
+        eb1 = parts_for(m);
+        k_colmax_parts<<<dim3((unsigned)eb1, k), 256, 0, s>>>(m, U, m, part, skip);
+        HIP_CHECK(hipGetLastError());
+    } else {
     // pass 1: P1[s][c] = sum_i g_is S0[c][i] on the SNP-major rows
     const int npn = parts_for(n);
     k_colmax_parts<<<dim3(npn, k), 256, 0, s>>>(n, S0, ldn, part, skip);
@@ -937,11 +961,12 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     k_finish_cols<<<1, 256, 0, s>>>(part, qg1, k, 0, limbs, nullptr, c->w_T.as<double>(), skip);
     HIP_CHECK(hipGetLastError());
     const int sp1 = contract(0, c->d_snp.as<uint8_t>(), c->pitch_s, m, n, Kpad1, nss1);
-    const int64_t eb1 = std::min<int64_t>((m + kEpiRows - 1) / kEpiRows, kMaxParts);
+    eb1 = std::min<int64_t>((m + kEpiRows - 1) / kEpiRows, kMaxParts);
     k_epilogue<<<(unsigned)eb1, 256, 0, s>>>(out, sp1, m, J16, limbs, k, sh1, 0, c->w_T.as<double>(),
                                              c->d_twof.as<double>(), c->d_inv.as<double>(), th,
                                              1.0 / (double)m, U, m, part, skip);
     HIP_CHECK(hipGetLastError());
+    }
     // pass 2: GZ[c][i] = sum_s g_is u[c][s] - C[c] on the individual-major rows
     k_finish_cols<<<1, 256, 0, s>>>(part, (int)eb1, k, 1, limbs, sh2, nullptr, skip);
     const int qg2 = parts_for(Kpad2);
@@ -954,6 +979,29 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     k_epilogue<<<(unsigned)eb2, 256, 0, s>>>(out, sp2, n, J16, limbs, k, sh2, 1, c->w_C.as<double>(),
                                              nullptr, nullptr, nullptr, 0.0, GZ, ldg, nullptr, skip);
     HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace globe
+
+namespace globe {
+
+// GZ (n x k) = G [e_j0 .. e_j0+k-1] for the GRM builder: pass 1 is the direct row
+// extraction k_unit_u, pass 2 the FP64-exact contraction.
+void geno_unit_columns(globe_ctx* c, int64_t j0, int k, double* GZ) {
+    hipStream_t s = c->stream;
+    const int64_t m = c->m;
+    c->w_P1.ensure(sizeof(double) * (size_t)(m > 0 ? m : 1) * k);
+    double* U = c->w_P1.as<double>();
+    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((m + 255) / 256, 1024));
+    k_unit_u<<<dim3(gx, k), 256, 0, s>>>(c->d_ind.as<uint8_t>(), c->pitch_i, j0, m, c->d_twof.as<double>(),
+                                         c->d_inv.as<double>(), 1.0 / (double)m, U);
+    HIP_CHECK(hipGetLastError());
+    const int kg = 128 / kLimbsFP64;
+    for (int c0 = 0; c0 < k; c0 += kg) {
+        const int kk = std::min(kg, k - c0);
+        mfma_geno_product(c, nullptr, c->n, kk, GZ + (size_t)c0 * c->n, c->n, nullptr, kLimbsFP64, nullptr,
+                          U + (size_t)c0 * m);
+    }
 }
 
 }  // namespace globe
