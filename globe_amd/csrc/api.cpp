@@ -758,23 +758,15 @@ struct Fit {
     globe_pcg_stats stats{};
 };
 
-// fitMME core (PCGsolver.cpp:262-290, varCompEst.cpp:636-665): solve V [X | y] into
-// c->e_sol (nrec x (p+2)), then beta and V^-1 r (left in column p+1 of e_sol... copied
-// to vinv_r).
-Fit fit_gls(globe_ctx* c, const double* y_dev, const globe_pcg_opts& o, double* vinv_r_dev) {
+// fitMME's closing steps (PCGsolver.cpp:262-290, varCompEst.cpp:636-665) from a solved
+// block in c->e_sol (ld nrec): columns 0..p hold V^-1 X, column p+1 V^-1 y. Forms
+// X^T V^-1 [X | y], beta = (X^T V^-1 X)^-1 X^T V^-1 y and V^-1 r = V^-1 y - V^-1 X beta
+// into vinv_r_dev (which may be column p+1 itself).
+static void fit_finish(globe_ctx* c, Fit& f, double* vinv_r_dev) {
     const int64_t nrec = c->nrec;
     const int q = c->p + 1, k = c->p + 2;
-    require(k <= kMaxCols, GLOBE_E_INVALID, "too many covariates");
     hipStream_t s = c->stream;
-    c->e_xy.ensure(sizeof(double) * nrec * k);
-    c->e_sol.ensure(sizeof(double) * nrec * k);
-    double* xy = c->e_xy.as<double>();
     double* sol = c->e_sol.as<double>();
-    launch_copy_cols(s, nrec, c->d_X.as<double>(), nrec, xy, nrec, q);
-    launch_copy_cols(s, nrec, y_dev, nrec, xy + (size_t)q * nrec, nrec, 1);
-    Fit f;
-    run_pcg(c, xy, nrec, k, sol, nrec, o, &f.stats, nullptr, nullptr);
-    // X^T [V^-1 X | V^-1 y]
     c->e_small.ensure(sizeof(double) * kMaxCols * kMaxCols);
     c->w_part.ensure(sizeof(double) * kMaxParts * kMaxCols * kMaxCols);
     launch_gram(s, nrec, c->d_X.as<double>(), nrec, q, sol, nrec, k, c->w_part.as<double>(),
@@ -790,12 +782,30 @@ Fit fit_gls(globe_ctx* c, const double* y_dev, const globe_pcg_opts& o, double* 
     for (int a = 0; a < q; ++a)
         for (int b = 0; b < q; ++b) f.beta[a] += Ai[a * q + b] * g[b * k + q];
     f.xtvx_inv = Ai;
-    // V^-1 r = V^-1 y - V^-1 X beta
-    launch_copy_cols(s, nrec, sol + (size_t)q * nrec, nrec, vinv_r_dev, nrec, 1);
+    double* vinv_y = sol + (size_t)q * nrec;
+    if (vinv_r_dev != vinv_y) launch_copy_cols(s, nrec, vinv_y, nrec, vinv_r_dev, nrec, 1);
     c->e_coef.ensure(sizeof(double) * kMaxCols * kMaxCols);
     HIP_CHECK(hipMemcpyAsync(c->e_coef.p, f.beta.data(), sizeof(double) * q, hipMemcpyHostToDevice, s));
     launch_axpy_cols(s, nrec, vinv_r_dev, nrec, sol, nrec, q, c->e_coef.as<double>(), 1);
     HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// fitMME (PCGsolver.cpp:262-290, varCompEst.cpp:636-665): solve V [X | y] into c->e_sol
+// (nrec x (p+2)), then beta and V^-1 r into vinv_r_dev.
+Fit fit_gls(globe_ctx* c, const double* y_dev, const globe_pcg_opts& o, double* vinv_r_dev) {
+    const int64_t nrec = c->nrec;
+    const int q = c->p + 1, k = c->p + 2;
+    require(k <= kMaxCols, GLOBE_E_INVALID, "too many covariates");
+    hipStream_t s = c->stream;
+    c->e_xy.ensure(sizeof(double) * nrec * k);
+    c->e_sol.ensure(sizeof(double) * nrec * k);
+    double* xy = c->e_xy.as<double>();
+    double* sol = c->e_sol.as<double>();
+    launch_copy_cols(s, nrec, c->d_X.as<double>(), nrec, xy, nrec, q);
+    launch_copy_cols(s, nrec, y_dev, nrec, xy + (size_t)q * nrec, nrec, 1);
+    Fit f;
+    run_pcg(c, xy, nrec, k, sol, nrec, o, &f.stats, nullptr, nullptr);
+    fit_finish(c, f, vinv_r_dev);
     return f;
 }
 
@@ -935,72 +945,112 @@ int aireml_eval_impl(globe_ctx* c, const double* y, const double* probes, const 
     hipStream_t s = c->stream;
     const double sg = c->theta[1];
     require(sg > 0.0, GLOBE_E_INVALID, "sig2g must be positive for AIREML (f_theta divides by it)");
+    const int q = c->p + 1, kf = c->p + 2;
+    require(kf <= kMaxCols, GLOBE_E_INVALID, "too many covariates");
+    // y and the probes are solved as one block when [X | y | probes] fits in one
+    const bool merged = y && nmc > 0 && kf + nmc <= kMaxCols;
     globe_pcg_opts o = *opts;
     o.x0_mode = GLOBE_X0_PRECOND;
     c->e_probe.ensure(sizeof(double) * nrec * std::max(1, nmc));
     c->e_pv.ensure(sizeof(double) * nrec * std::max(5, nmc));
     c->e_w.ensure(sizeof(double) * nrec);
-    c->e_us.ensure(sizeof(double) * (size_t)n * std::max(1, nmc));
+    c->e_us.ensure(sizeof(double) * (size_t)n * (1 + nmc));
     c->e_f.ensure(sizeof(double) * nrec * 5);
     c->e_small.ensure(sizeof(double) * kMaxCols * kMaxCols);
     c->w_part.ensure(sizeof(double) * kMaxParts * kMaxCols * kMaxCols);
+    if (nmc > 0) {
+        c->e_sv.ensure(sizeof(double) * nrec * nmc);
+        c->e_fs.ensure(sizeof(double) * nrec * 5 * nmc);
+        c->s_aux.ensure(sizeof(double) * nrec * nmc);
+    }
     double* py = c->e_w.as<double>();
     double* fth = c->e_f.as<double>();
+    double* us = c->e_us.as<double>();      // G-products: column 0 for V^-1 r, 1.. for the probes
+    double* wts = c->s_aux.as<double>();    // trace weights of the probes
+    double* sv = c->e_sv.as<double>();      // V^-1 probes, then P probes
+    globe_pcg_opts ot = o;                  // the probes' solve
+    // The probes into Y and their PCG start into X0 (ld nrec). Device probes: weights =
+    // tilde-V^-1 probe (also the start of its solve) for the Gaussian tilde-V probes, the
+    // probe itself otherwise; injected probes: as given.
+    auto load_probes = [&](double* Y, double* X0) {
+        if (gen) {
+            make_probes_dev(c, gen->kind, nmc, gen->seed, Y, wts);
+            if (gen->kind == 0) {
+                launch_copy_cols(s, nrec, wts, nrec, X0, nrec, nmc);
+                ot.x0_mode = GLOBE_X0_GIVEN;
+            }
+        } else {
+            HIP_CHECK(hipMemcpy(Y, probes, sizeof(double) * nrec * nmc, hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(wts, weights, sizeof(double) * nrec * nmc, hipMemcpyHostToDevice));
+            if (probe_x0) {
+                HIP_CHECK(hipMemcpy(X0, probe_x0, sizeof(double) * nrec * nmc, hipMemcpyHostToDevice));
+                ot.x0_mode = GLOBE_X0_GIVEN;
+            }
+        }
+    };
     Fit f;
-    if (y) {
+    if (merged) {
+        // 1-3. fitMME (varCompEst.cpp:587-685) and the probe solves of estimateTrace_PdV
+        //    (:816-977) as one block PCG of [X | y | probes], each column from its own x0
+        //    (Q^-1 b, or the probe's given start). Under the all-columns rule the fit's
+        //    columns iterate until the probes converge too: never looser than a solve of
+        //    their own.
+        const int kb = kf + nmc;
+        c->e_xy.ensure(sizeof(double) * nrec * kb);
+        c->e_sol.ensure(sizeof(double) * nrec * kb);
+        double* xy = c->e_xy.as<double>();
+        double* sol = c->e_sol.as<double>();
+        launch_copy_cols(s, nrec, c->d_X.as<double>(), nrec, xy, nrec, q);
+        HIP_CHECK(hipMemcpy(xy + (size_t)q * nrec, y, sizeof(double) * nrec, hipMemcpyHostToDevice));
+        sv = sol + (size_t)kf * nrec;
+        load_probes(xy + (size_t)kf * nrec, sv);
+        precond(c, xy, nrec, ot.x0_mode == GLOBE_X0_GIVEN ? kf : kb, sol, nrec, nullptr, nullptr, nullptr);
+        globe_pcg_opts om = o;
+        om.x0_mode = GLOBE_X0_GIVEN;
+        run_pcg(c, xy, nrec, kb, sol, nrec, om, &f.stats, nullptr, nullptr);
+        // V^-1 r replaces V^-1 y in column p+1, beside the probes' P z: uhat and the
+        // probes' G-products take one GRM-only product over the 1 + nmc columns
+        double* vr = sol + (size_t)q * nrec;
+        fit_finish(c, f, vr);
+        apply_P(c, f, sv, nmc);
+        launch_copy_cols(s, nrec, vr, nrec, py, nrec, 1);
+        op_vx(c, vr, nrec, 1 + nmc, us, n, GLOBE_VX_GRM_ONLY, o.precision, nullptr, nullptr);
+    } else if (y) {
         // 1. fitMME at theta (varCompEst.cpp:587-685): x0 = Q^-1 b
         HIP_CHECK(hipMemcpy(c->e_probe.p, y, sizeof(double) * nrec, hipMemcpyHostToDevice));
         f = fit_gls(c, c->e_probe.as<double>(), o, py);
-        // 2. uhat and f_theta (varCompEst.cpp:693-760)
-        op_vx(c, py, nrec, 1, c->e_us.as<double>(), n, GLOBE_VX_GRM_ONLY, o.precision, nullptr, nullptr);
-        launch_f_theta(s, n, c->d_blk_off.as<int64_t>(), c->d_blk_ind.as<int32_t>(),
-                       c->d_times.as<double>(), py, nrec, c->e_us.as<double>(), n, 1, 1.0 / sg, fth, nrec);
-        c->fit_xtvx_inv = f.xtvx_inv;
-        c->fit_theta = c->theta;
-        c->fit_valid = true;
+        op_vx(c, py, nrec, 1, us, n, GLOBE_VX_GRM_ONLY, o.precision, nullptr, nullptr);
     } else {
         require(c->fit_valid && c->fit_theta == c->theta, GLOBE_E_STATE,
                 "no fit at the current variance components to reuse");
         f.xtvx_inv = c->fit_xtvx_inv;
     }
+    if (y) {
+        // 2. uhat and f_theta (varCompEst.cpp:693-760)
+        launch_f_theta(s, n, c->d_blk_off.as<int64_t>(), c->d_blk_ind.as<int32_t>(),
+                       c->d_times.as<double>(), py, nrec, us, n, 1, 1.0 / sg, fth, nrec);
+        c->fit_xtvx_inv = f.xtvx_inv;
+        c->fit_theta = c->theta;
+        c->fit_valid = true;
+    }
     double first[5] = {0, 0, 0, 0, 0};
     launch_gram(s, nrec, py, nrec, 1, fth, nrec, 5, c->w_part.as<double>(), c->e_small.as<double>());
     HIP_CHECK(hipMemcpyAsync(first, c->e_small.p, sizeof(double) * 5, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
-    // 3. trace terms (varCompEst.cpp:816-977) with injected probes
+    // 3. trace terms (varCompEst.cpp:816-977)
     double tr[5] = {0, 0, 0, 0, 0};
     if (nmc > 0) {
-        c->e_sv.ensure(sizeof(double) * nrec * nmc);
-        c->e_fs.ensure(sizeof(double) * nrec * 5 * nmc);
-        c->s_aux.ensure(sizeof(double) * nrec * nmc);
-        double* sv = c->e_sv.as<double>();
-        globe_pcg_opts ot = o;
-        if (gen) {
-            // probes on the device; weights = tilde-V^-1 probe (the start of its solve) for
-            // the Gaussian tilde-V probes, the probe itself otherwise
-            make_probes_dev(c, gen->kind, nmc, gen->seed, c->e_probe.as<double>(), c->s_aux.as<double>());
-            if (gen->kind == 0) {
-                launch_copy_cols(s, nrec, c->s_aux.as<double>(), nrec, sv, nrec, nmc);
-                ot.x0_mode = GLOBE_X0_GIVEN;
-            }
-        } else {
-            HIP_CHECK(hipMemcpy(c->e_probe.p, probes, sizeof(double) * nrec * nmc, hipMemcpyHostToDevice));
-            HIP_CHECK(hipMemcpy(c->s_aux.p, weights, sizeof(double) * nrec * nmc, hipMemcpyHostToDevice));
-            if (probe_x0) {
-                HIP_CHECK(hipMemcpy(sv, probe_x0, sizeof(double) * nrec * nmc, hipMemcpyHostToDevice));
-                ot.x0_mode = GLOBE_X0_GIVEN;
-            }
+        if (!merged) {
+            load_probes(c->e_probe.as<double>(), sv);
+            run_pcg(c, c->e_probe.as<double>(), nrec, nmc, sv, nrec, ot, nullptr, nullptr, nullptr);
+            apply_P(c, f, sv, nmc);
+            op_vx(c, sv, nrec, nmc, us + n, n, GLOBE_VX_GRM_ONLY, o.precision, nullptr, nullptr);
         }
-        run_pcg(c, c->e_probe.as<double>(), nrec, nmc, sv, nrec, ot, nullptr, nullptr, nullptr);
-        apply_P(c, f, sv, nmc);
-        op_vx(c, sv, nrec, nmc, c->e_us.as<double>(), n, GLOBE_VX_GRM_ONLY, o.precision, nullptr, nullptr);
         double* fs = c->e_fs.as<double>();
         launch_f_theta(s, n, c->d_blk_off.as<int64_t>(), c->d_blk_ind.as<int32_t>(),
-                       c->d_times.as<double>(), sv, nrec, c->e_us.as<double>(), n, nmc, 1.0 / sg, fs, nrec);
-        for (int smp = 0; smp < nmc; ++smp)
-            launch_gram(s, nrec, c->s_aux.as<double>() + (size_t)smp * nrec, nrec, 1,
-                        fs + (size_t)5 * smp * nrec, nrec, 5, c->w_part.as<double>(),
-                        c->e_small.as<double>() + 5 * smp);
+                       c->d_times.as<double>(), sv, nrec, us + n, n, nmc, 1.0 / sg, fs, nrec);
+        launch_gram_grouped(s, nrec, wts, nrec, nmc, fs, nrec, 5, c->w_part.as<double>(),
+                            c->e_small.as<double>());
         std::vector<double> t((size_t)5 * nmc);
         HIP_CHECK(hipMemcpyAsync(t.data(), c->e_small.p, sizeof(double) * 5 * nmc, hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));

@@ -97,6 +97,10 @@ void launch_dot_parts(hipStream_t s, int64_t nrec, const double* a, int64_t lda,
 void launch_reduce_parts(hipStream_t s, const double* part, int nparts, int k, double* out);
 void launch_gram(hipStream_t s, int64_t nrec, const double* A, int64_t lda, int ka,
                  const double* B, int64_t ldb, int kb, double* part, double* out);
+// out[a kb + b] = A[:, a] . B[:, a kb + b] for a < ka, b < kb (each column of A against its
+// own group of kb columns of B); part: kMaxParts x ka kb
+void launch_gram_grouped(hipStream_t s, int64_t nrec, const double* A, int64_t lda, int ka,
+                         const double* B, int64_t ldb, int kb, double* part, double* out);
 void launch_axpy_cols(hipStream_t s, int64_t nrec, double* y, int64_t ldy, const double* x,
                       int64_t ldx, int kx, const double* coef, int ky);
 void launch_pcg_init(hipStream_t s, PcgState* st, const double* part_bb, const double* part_rr,

@@ -14,7 +14,8 @@
 // and the matching 64 x 16 slice of S0 (pre-transposed to rows of 16 doubles) go global
 // -> LDS (global_load_lds_dwordx4) into a ring of kGS slots, one wait + barrier per tile,
 // and each tile is multiplied on the FP64 matrix cores while the next one lands. A group
-// of up to 16 columns takes one pass over the triangle. The strip partials of the
+// of up to 32 columns takes one pass over the triangle (blocks of 8 waves for 17-32
+// columns: two column groups share each tile). The strip partials of the
 // chunks are added in chunk order, so the result is deterministic. Each stored tile is
 // read twice (once for its row strip, once for its column strip); the triangle is far
 // larger than any cache.
@@ -28,12 +29,24 @@ namespace {
 
 constexpr int kGT = 64;        // tile edge
 constexpr int kGrmChunks = 4;  // column ranges per row strip
-constexpr int kGC = 16;        // columns of S0 per pass
+constexpr int kGC = 16;        // columns of S0 per column group (one MFMA column block)
+constexpr int kGCMax = 32;     // columns per pass: up to two column groups
 constexpr int kGS = 2;         // ring slots: one tile in flight while one is multiplied
 constexpr int kTileB = kGT * kGT * 4;                 // FP32 tile bytes
-constexpr int kSliceB = kGT * kGC * 8;                // S0 slice bytes
-constexpr int kSlotB = kTileB + kSliceB;
-constexpr int kCopiesPerStage = kTileB / 1024 / 4 + kSliceB / 1024 / 4;  // per wave: 4 + 2
+
+// A block of 4 CG waves: wave w owns rows 16 (w % 4) .. +15 of the strip and column
+// group w / 4 (16 columns); the CG groups share each tile.
+template <int CG>
+struct GrmCfg {
+    static constexpr int kCols = kGC * CG;                // columns per pass
+    static constexpr int kWaves = 4 * CG;
+    static constexpr int kSliceB = kGT * kCols * 8;       // S0 slice bytes
+    static constexpr int kSlotB = kTileB + kSliceB;
+    static constexpr int kTilePerWave = 16 / kWaves;      // 1 KB tile pieces per wave
+    static constexpr int kSlicePerWave = kSliceB / 1024 / kWaves;
+    static constexpr int kCopies = kTilePerWave + kSlicePerWave;
+    static_assert(16 % kWaves == 0 && (kSliceB / 1024) % kWaves == 0, "pieces spread evenly");
+};
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) void lds_void;
@@ -46,66 +59,74 @@ __device__ inline int64_t tile_index(int64_t I, int64_t J) { return I * (I + 1) 
 // and 16 consecutive columns of one row (the transposed read) fall in different banks.
 __device__ inline int rot(int a, int b) { return a * kGT + ((((b >> 2) + a) & 15) << 2) + (b & 3); }
 
-// S0T[j][c] = S0[c][j] for c < kk and j < n, 0 elsewhere (npad = 64 ntiles rows).
+// S0T[j][c] = S0[c][j] for c < kk and j < n, 0 elsewhere (npad = 64 ntiles rows of
+// `cols` doubles).
 __global__ __launch_bounds__(256) void k_grm_s0t(const double* __restrict__ S0, int64_t ldn, int32_t n,
-                                                 int kk, int64_t npad, double* __restrict__ S0T,
+                                                 int kk, int cols, int64_t npad, double* __restrict__ S0T,
                                                  const int* skip) {
     if (skip && *skip) return;
-    for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < npad * kGC; e += (int64_t)gridDim.x * 256) {
-        const int64_t j = e / kGC;
-        const int c = (int)(e % kGC);
+    for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < npad * cols; e += (int64_t)gridDim.x * 256) {
+        const int64_t j = e / cols;
+        const int c = (int)(e % cols);
         S0T[e] = (c < kk && j < n) ? S0[c * ldn + j] : 0.0;
     }
 }
 
 // Copies of one stage into ring slot `slot` by wave w: tile pieces (1 KB = 4 tile rows,
-// lane l: row 4 p + l / 16, its piece (l % 16 - row) & 15, so the rows land rotated) p =
-// 4 w .. 4 w + 3, and S0T pieces 2 w, 2 w + 1 of the 8 KB slice.
+// lane l: row 4 p + l / 16, its piece (l % 16 - row) & 15, so the rows land rotated)
+// p = kTilePerWave w .., and kSlicePerWave 1 KB pieces of the S0T slice.
+template <int CG>
 __device__ inline void grm_issue(const float* __restrict__ tile, const double* __restrict__ s0t,
                                  uint8_t* __restrict__ slot, int w, int lane) {
+    using C = GrmCfg<CG>;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int p = 4 * w + q;
+    for (int q = 0; q < C::kTilePerWave; ++q) {
+        const int p = C::kTilePerWave * w + q;
         const int a = 4 * p + (lane >> 4);
         const int piece = ((lane & 15) - a) & 15;
         __builtin_amdgcn_global_load_lds((const void*)(tile + a * kGT + 4 * piece),
                                          (lds_void*)(slot + p * 1024), 16, 0, 0);
     }
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int p = 2 * w + q;
+    for (int q = 0; q < C::kSlicePerWave; ++q) {
+        const int p = C::kSlicePerWave * w + q;
         __builtin_amdgcn_global_load_lds((const void*)((const uint8_t*)s0t + p * 1024 + lane * 16),
                                          (lds_void*)(slot + kTileB + p * 1024), 16, 0, 0);
     }
 }
 
-// 16 MFMAs of one tile for wave w: out rows 16 w + (0..15) x 16 columns. A lane l =
-// G[16 w + l % 16][4 s + l / 16] (TRANS: the stored tile read transposed), B lane l =
-// S0[j0 + 4 s + l / 16][l % 16], D element v of lane l = (row 4 v + l / 16, column
-// l % 16) (v_mfma_f64_16x16x4f64, probed on gfx950).
-template <bool TRANS>
-__device__ inline void grm_mma(const uint8_t* __restrict__ slot, int w, int lane, v4d& acc) {
+// 16 MFMAs of one tile for the wave of row group rg and column group cg: out rows
+// 16 rg + (0..15) x columns 16 cg + (0..15). A lane l = G[16 rg + l % 16][4 s + l / 16]
+// (TRANS: the stored tile read transposed), B lane l = S0[j0 + 4 s + l / 16][16 cg + l % 16],
+// D element v of lane l = (row 4 v + l / 16, column l % 16) (v_mfma_f64_16x16x4f64,
+// probed on gfx950).
+template <int CG, bool TRANS>
+__device__ inline void grm_mma(const uint8_t* __restrict__ slot, int rg, int cg, int lane, v4d& acc) {
+    constexpr int kCols = GrmCfg<CG>::kCols;
     const float* g = reinterpret_cast<const float*>(slot);
-    const double* sb = reinterpret_cast<const double*>(slot + kTileB) + (lane >> 4) * kGC + (lane & 15);
-    const int a = 16 * w + (lane & 15);
+    const double* sb = reinterpret_cast<const double*>(slot + kTileB) + (lane >> 4) * kCols + kGC * cg + (lane & 15);
+    const int a = 16 * rg + (lane & 15);
 #pragma unroll
     for (int s4 = 0; s4 < kGT / 4; ++s4) {
         const int b = 4 * s4 + (lane >> 4);
         const float gv = TRANS ? g[rot(b, a)] : g[rot(a, b)];
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)gv, sb[4 * s4 * kGC], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)gv, sb[4 * s4 * kCols], acc, 0, 0, 0);
     }
 }
 
-// One pass of up to 16 columns (S0T zero past kk): part[h][c][i] = sum over the tiles J
-// of chunk h of (G_IJ S0_J)[i][c].
-__global__ __launch_bounds__(256) void k_grm_tiles(const float* __restrict__ tiles, int32_t n,
-                                                   const double* __restrict__ S0T, int kk,
-                                                   int32_t ntiles, double* __restrict__ part,
-                                                   const int* skip) {
+// One pass of up to 16 CG columns (S0T zero past kk): part[h][c][i] = sum over the tiles
+// J of chunk h of (G_IJ S0_J)[i][c].
+template <int CG>
+__global__ __launch_bounds__(256 * CG) void k_grm_tiles(const float* __restrict__ tiles, int32_t n,
+                                                        const double* __restrict__ S0T, int kk,
+                                                        int32_t ntiles, double* __restrict__ part,
+                                                        const int* skip) {
     if (skip && *skip) return;
-    __shared__ __attribute__((aligned(1024))) uint8_t lds[kGS * kSlotB];
+    using C = GrmCfg<CG>;
+    __shared__ __attribute__((aligned(1024))) uint8_t lds[kGS * C::kSlotB];
     const int I = blockIdx.x, h = blockIdx.y;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int rg = w & 3, cg = w >> 2;
     const int J0 = (int)((int64_t)h * ntiles / kGrmChunks), J1 = (int)((int64_t)(h + 1) * ntiles / kGrmChunks);
     const int nt = J1 - J0;
     v4d acc = {0.0, 0.0, 0.0, 0.0};
@@ -114,7 +135,7 @@ __global__ __launch_bounds__(256) void k_grm_tiles(const float* __restrict__ til
         auto issue = [&](int t, int slot) {
             const int64_t J = J0 + (t < nt ? t : nt - 1);
             const float* tile = tiles + (J <= I ? tile_index(I, J) : tile_index(J, I)) * (kGT * kGT);
-            grm_issue(tile, S0T + J * kGT * kGC, lds + slot * kSlotB, w, lane);
+            grm_issue<CG>(tile, S0T + J * kGT * C::kCols, lds + slot * C::kSlotB, w, lane);
         };
 #pragma unroll
         for (int u = 0; u < kGS - 1; ++u) issue(u, u);
@@ -123,24 +144,24 @@ __global__ __launch_bounds__(256) void k_grm_tiles(const float* __restrict__ til
             // this wave's copies of stage t have landed (those of the kGS - 2 later stages
             // may be in flight); after the barrier every wave's have, and every wave is done
             // with the slot the next copies overwrite
-            asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"((kGS - 2) * kCopiesPerStage)
+            asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"((kGS - 2) * C::kCopies)
                          : "memory");
             issue(t + kGS - 1, cs == 0 ? kGS - 1 : cs - 1);
             __builtin_amdgcn_sched_barrier(0);
-            const uint8_t* slot = lds + cs * kSlotB;
-            if (J0 + t <= I) grm_mma<false>(slot, w, lane, acc);
-            else grm_mma<true>(slot, w, lane, acc);
+            const uint8_t* slot = lds + cs * C::kSlotB;
+            if (J0 + t <= I) grm_mma<CG, false>(slot, rg, cg, lane, acc);
+            else grm_mma<CG, true>(slot, rg, cg, lane, acc);
             cs = cs + 1 == kGS ? 0 : cs + 1;
         }
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     }
-    const int col = lane & 15;
+    const int col = kGC * cg + (lane & 15);
     if (col >= kk) return;
     const int64_t i0 = (int64_t)I * kGT;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const int64_t i = i0 + 16 * w + 4 * v + (lane >> 4);
-        if (i < n) part[((int64_t)h * kGC + col) * n + i] = acc[v];
+        const int64_t i = i0 + 16 * rg + 4 * v + (lane >> 4);
+        if (i < n) part[((int64_t)h * kGCMax + col) * n + i] = acc[v];
     }
 }
 
@@ -248,18 +269,23 @@ void launch_grm_product(hipStream_t s, const float* tiles, int32_t n, const doub
     const int32_t ntiles = (n + kGT - 1) / kGT;
     const dim3 g(ntiles, kGrmChunks);
     const unsigned gf = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
-    // one pass over the triangle per group of up to 16 columns; S0T sits after the strip
-    // partials in the scratch (16-byte aligned: the partials are 512 n bytes)
+    // one pass over the triangle per group of up to 32 columns (two column groups of 16
+    // share each tile); S0T sits after the strip partials in the scratch (16-byte aligned:
+    // the partials are 1 KB x n)
     const int64_t npad = (int64_t)ntiles * kGT;
-    double* S0T = part + (size_t)kGrmChunks * kGC * n;
-    const unsigned gt = (unsigned)std::min<int64_t>((npad * kGC + 255) / 256, 1024);
-    for (int c = 0; c < k; c += kGC) {
-        const int kk = std::min(kGC, k - c);
-        k_grm_s0t<<<gt, 256, 0, s>>>(S0 + c * ldn, ldn, n, kk, npad, S0T, skip);
+    double* S0T = part + (size_t)kGrmChunks * kGCMax * n;
+    for (int c = 0; c < k; c += kGCMax) {
+        const int kk = std::min(kGCMax, k - c);
+        const int cols = kk > kGC ? 2 * kGC : kGC;
+        const unsigned gt = (unsigned)std::min<int64_t>((npad * cols + 255) / 256, 1024);
+        k_grm_s0t<<<gt, 256, 0, s>>>(S0 + c * ldn, ldn, n, kk, cols, npad, S0T, skip);
         HIP_CHECK(hipGetLastError());
-        k_grm_tiles<<<g, 256, 0, s>>>(tiles, n, S0T, kk, ntiles, part, skip);
+        if (cols == kGC)
+            k_grm_tiles<1><<<g, 256, 0, s>>>(tiles, n, S0T, kk, ntiles, part, skip);
+        else
+            k_grm_tiles<2><<<g, 512, 0, s>>>(tiles, n, S0T, kk, ntiles, part, skip);
         HIP_CHECK(hipGetLastError());
-        k_grm_finish<<<dim3(gf, kk), 256, 0, s>>>(part, n, kGC, theta_cols + c * 5, GZ + c * ldg, ldg,
+        k_grm_finish<<<dim3(gf, kk), 256, 0, s>>>(part, n, kGCMax, theta_cols + c * 5, GZ + c * ldg, ldg,
                                                   skip);
         HIP_CHECK(hipGetLastError());
     }
@@ -267,7 +293,7 @@ void launch_grm_product(hipStream_t s, const float* tiles, int32_t n, const doub
 
 size_t grm_product_part_bytes(int32_t n) {
     const size_t npad = (size_t)(n + kGT - 1) / kGT * kGT;
-    return sizeof(double) * ((size_t)kGrmChunks * kGC * (size_t)n + npad * kGC);
+    return sizeof(double) * ((size_t)kGrmChunks * kGCMax * (size_t)n + npad * kGCMax);
 }
 
 void launch_grm_scatter(hipStream_t s, const double* Gc, int64_t ldg, int32_t n, int64_t j0, int k,

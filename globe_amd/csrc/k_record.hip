@@ -312,23 +312,27 @@ __global__ __launch_bounds__(kBlock) void k_dot_parts(int64_t nrec, const double
 }
 
 // out[c] = sum_j part[c][j], j ascending (one thread per column).
-__global__ void k_reduce_parts(const double* __restrict__ part, int nparts, int k, double* out,
-                               const int* skip) {
+// out[c] = sum of the nparts partials of column c: one block per column, thread t adds
+// partials t, t + 256, ... in order, then a fixed-order block sum (deterministic; a
+// thread per column would chain up to 1024 dependent loads).
+__global__ __launch_bounds__(kBlock) void k_reduce_parts(const double* __restrict__ part, int nparts,
+                                                         double* out, const int* skip) {
     if (skip && *skip) return;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= k) return;
+    const int c = blockIdx.x;
     double s = 0.0;
-    for (int j = 0; j < nparts; ++j) s += part[c * kMaxParts + j];
-    out[c] = s;
+    for (int j = threadIdx.x; j < nparts; j += kBlock) s += part[c * kMaxParts + j];
+    s = block_sum(s);
+    if (threadIdx.x == 0) out[c] = s;
 }
 
-// A^T B partials: part[(a*kb + b)][block]
+// A^T B partials: part[(a*kb + b)][block] = column a of A . column (a bsa + b) of B
+// (bsa = 0: the full gram; bsa = kb: column a of A against its own group of kb columns)
 __global__ __launch_bounds__(kBlock) void k_gram_parts(int64_t nrec, const double* __restrict__ A,
                                                        int64_t lda, const double* __restrict__ B,
-                                                       int64_t ldb, int kb, double* part) {
+                                                       int64_t ldb, int kb, int bsa, double* part) {
     const int a = blockIdx.y / kb, bb = blockIdx.y % kb;
     const double* ac = A + a * lda;
-    const double* bc = B + bb * ldb;
+    const double* bc = B + (int64_t)(a * bsa + bb) * ldb;
     double acc = 0.0;
     for (int64_t r = blockIdx.x * (int64_t)kBlock + threadIdx.x; r < nrec;
          r += (int64_t)gridDim.x * kBlock)
@@ -716,13 +720,13 @@ void launch_dot_parts(hipStream_t s, int64_t nrec, const double* a, int64_t lda,
 }
 
 void launch_reduce_parts(hipStream_t s, const double* part, int nparts, int k, double* out) {
-    k_reduce_parts<<<(k + 63) / 64, 64, 0, s>>>(part, nparts, k, out, nullptr);
+    if (k > 0) k_reduce_parts<<<k, kBlock, 0, s>>>(part, nparts, out, nullptr);
     LAUNCH_CHECK();
 }
 
 void launch_reduce_parts_skip(hipStream_t s, const double* part, int nparts, int k, double* out,
                               const int* skip) {
-    k_reduce_parts<<<(k + 63) / 64, 64, 0, s>>>(part, nparts, k, out, skip);
+    if (k > 0) k_reduce_parts<<<k, kBlock, 0, s>>>(part, nparts, out, skip);
     LAUNCH_CHECK();
 }
 
@@ -730,9 +734,19 @@ void launch_gram(hipStream_t s, int64_t nrec, const double* A, int64_t lda, int 
                  const double* B, int64_t ldb, int kb, double* part, double* out) {
     const int np = parts_for(nrec);
     dim3 g(np, ka * kb);
-    k_gram_parts<<<g, kBlock, 0, s>>>(nrec, A, lda, B, ldb, kb, part);
+    k_gram_parts<<<g, kBlock, 0, s>>>(nrec, A, lda, B, ldb, kb, 0, part);
     LAUNCH_CHECK();
-    k_reduce_parts<<<(ka * kb + 63) / 64, 64, 0, s>>>(part, np, ka * kb, out, nullptr);
+    if (ka * kb > 0) k_reduce_parts<<<ka * kb, kBlock, 0, s>>>(part, np, out, nullptr);
+    LAUNCH_CHECK();
+}
+
+void launch_gram_grouped(hipStream_t s, int64_t nrec, const double* A, int64_t lda, int ka,
+                         const double* B, int64_t ldb, int kb, double* part, double* out) {
+    const int np = parts_for(nrec);
+    dim3 g(np, ka * kb);
+    k_gram_parts<<<g, kBlock, 0, s>>>(nrec, A, lda, B, ldb, kb, kb, part);
+    LAUNCH_CHECK();
+    if (ka * kb > 0) k_reduce_parts<<<ka * kb, kBlock, 0, s>>>(part, np, out, nullptr);
     LAUNCH_CHECK();
 }
 

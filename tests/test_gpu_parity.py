@@ -322,9 +322,10 @@ def test_record_tiles_and_large_block_fallback(big_block, gpu_ctx_factory):
     assert rel(xg, xo) < 1e-7
 
 
-@pytest.mark.parametrize("k", [1, 2, 3, 8, 16, 21])
+@pytest.mark.parametrize("k", [1, 3, 16, 17, 21, 32, 33, 64])
 def test_grm_product_widths(k, gpu_ctx_factory):
-    """Tiled GRM product for every column-group width (16, 8, 4, 2, 1 and mixes)."""
+    """Tiled GRM product: one pass of 16 columns (1..16), of two column groups sharing
+    each tile (17..32), and several passes (33..64)."""
     _say("grm-widths", k)
     d = load_golden("grm_n180")
     g = device_for(d, gpu_ctx_factory)
