@@ -133,8 +133,10 @@ def serving_leg(g, c, kept, args):
 
 def aireml_leg(args):
     """AIREML on a dense GRM (N = 40,000, GRM from 80,000 synthetic SNPs built on the
-    device, ~320k records, y + 15 Rademacher probes as one 16-column block PCG, start
-    0.5 / 0.5 / 0.3 / 0 / 0.3): seconds per AIREML iteration over the first iterations."""
+    device, ~320k records, [X | y | 15 Rademacher probes] as one block PCG, start
+    0.5 / 0.5 / 0.3 / 0 / 0.3): seconds per AIREML iteration over the first iterations,
+    with the GRM product in FP64 and in mixed precision (int8 digits) from the same start
+    and probes."""
     import globe_amd as G
     from globe_amd import synth
     g = G.Globe(0)
@@ -151,13 +153,37 @@ def aireml_leg(args):
     g.grm_from_geno()
     t_grm = time.perf_counter() - t0
     log(f"aireml: GRM {c.n} x {c.n} from {kept} SNPs built in {t_grm:.2f}s")
-    fit = g.aireml_fit(y, (0.5, 0.5, 0.3, 0.0, 0.3), max_iter=args.aireml_iters, tol=1e-3, num_mc=15,
-                       probes="rademacher", seed=args.seed, pcg_tol=1e-5)
-    log(f"aireml: {fit['iters']} iterations, {fit['sec_per_iter']:.3f} s/iter, theta {np.round(fit['theta'], 4)}")
+    th0 = (0.5, 0.5, 0.3, 0.0, 0.3)
+    runs = {}
+    for name, prec in (("fp64", G.PREC_FP64), ("mixed", G.PREC_MIXED)):
+        if prec == G.PREC_MIXED:
+            # the int8 digit tiles of G are built by the first mixed product
+            t0 = time.perf_counter()
+            g.computeVx_multi(np.zeros((c.nrec, 1)), multGRM_uhat=True, precision=prec)
+            t_dig = time.perf_counter() - t0
+            log(f"aireml: int8 digit tiles of G built in {t_dig:.2f}s")
+        g.profile(True)
+        fit = g.aireml_fit(y, th0, max_iter=args.aireml_iters, tol=1e-3, num_mc=15, probes="rademacher",
+                           seed=args.seed, pcg_tol=1e-5, precision=prec)
+        ks = g.kernel_stats(3 if prec == G.PREC_MIXED else 2)
+        g.profile(False)
+        runs[name] = {"sec_per_iter": fit["sec_per_iter"], "iters": fit["iters"],
+                      "theta": list(map(float, fit["theta"])), "grm_passes": ks["launches"],
+                      "grm_pass_ms": ks["ms_avg"],
+                      "grm_pass_GBps": ks["bytes"] / (ks["ms_avg"] * 1e-3) / 1e9 if ks["ms_avg"] else None}
+        log(f"aireml {name}: {fit['iters']} iterations, {fit['sec_per_iter']:.3f} s/iter, GRM pass "
+            f"{ks['ms_avg']:.3f} ms x {ks['launches']}, theta {np.round(fit['theta'], 5)}")
+    t64, tm = np.array(runs["fp64"]["theta"]), np.array(runs["mixed"]["theta"])
     out = {"n": c.n, "m_snps": int(kept), "nrec": c.nrec, "probes": 15, "grm_build_s": t_grm,
-           "iters": fit["iters"], "sec_per_iter": fit["sec_per_iter"], "theta": list(map(float, fit["theta"])),
-           "precision": "fp64 (FP32 GRM, FP64 accumulation)",
-           "note": f"first {args.aireml_iters} iterations of the fit (maxiter 50, tol 1e-3 in the full run)"}
+           "iters": runs["fp64"]["iters"], "sec_per_iter": runs["fp64"]["sec_per_iter"],
+           "theta": runs["fp64"]["theta"], "precision": "fp64 (FP32 GRM on the FP64 matrix cores)",
+           "mixed": dict(runs["mixed"], precision="mixed (int8 digits of G and S0, relaxed 4/3/2)",
+                         digit_build_s=t_dig),
+           "mixed_vs_fp64_speedup": runs["fp64"]["sec_per_iter"] / runs["mixed"]["sec_per_iter"],
+           "mixed_theta_max_rel_diff": float(np.max(np.abs(tm - t64) / np.abs(t64))),
+           "fp64_grm_pass_ms": runs["fp64"]["grm_pass_ms"],
+           "note": f"first {args.aireml_iters} iterations of the fit from the same start and probes "
+                   "(maxiter 50, tol 1e-3 in the full run)"}
     g.close()
     return out
 

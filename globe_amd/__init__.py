@@ -127,6 +127,7 @@ def load_library(path: str = LIB_PATH):
         "globe_get_geno_rows": ([P, i64, i64, P, P], C.c_int),
         "globe_dev_copy": ([P, P, P, i64], C.c_int),
         "globe_profile_enable": ([P, C.c_int], C.c_int),
+        "globe_set_mixed_digits": ([P, C.c_int], C.c_int),
         "globe_profile_read": ([P, C.c_int, P], C.c_int),
         "globe_aireml_step": ([P, P, P, dbl, P], C.c_int),
         "globe_aireml_step_reduced": ([P, P, P, dbl, P], C.c_int),
@@ -419,6 +420,10 @@ class Globe:
 
     def synchronize(self):
         self._check(self.L.globe_synchronize(self.h))
+
+    def set_mixed_digits(self, digits):
+        """int8 digits (2..4) of mixed-precision genetic products outside a PCG solve."""
+        self._check(self.L.globe_set_mixed_digits(self.h, int(digits)))
 
     # ---- kernel timing
     def profile(self, on=True):

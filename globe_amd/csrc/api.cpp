@@ -105,14 +105,54 @@ void ensure_gdiag(globe_ctx* c) {
     c->gdiag_ok = true;
 }
 
+// int8 digit tiles of the GRM (4 N^2 bytes: the full square) for mixed products
+static void ensure_grm_digits(globe_ctx* c) {
+    if (c->grm8_ok) return;
+    const int32_t n = c->n;
+    c->d_grm8.ensure(grm_digit_bytes(n));
+    c->d_grm8_aux.ensure(grm_digit_aux_bytes(n));
+    launch_grm_digits(c->stream, c->d_grm.as<float>(), n, c->d_grm8_aux.p, c->d_grm8.as<int8_t>());
+    c->grm8_ok = true;
+}
+
 void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int precision,
                      const int* skip, const double* th) {
     const int32_t n = c->n;
     hipStream_t s = c->stream;
     if (c->gen == 2) {
-        c->w_grm_part.ensure(grm_product_part_bytes(n));
-        launch_grm_product(s, c->d_grm.as<float>(), n, S0, n, k, th, GZ, n, skip,
-                           c->w_grm_part.as<double>());
+        const bool mixed = precision == GLOBE_PREC_MIXED;
+        if (mixed) {
+            ensure_grm_digits(c);
+            ensure_gdiag(c);
+            c->w_grm8_part.ensure(grm_i8_part_bytes(n));
+        } else {
+            c->w_grm_part.ensure(grm_product_part_bytes(n));
+        }
+        // timing: which 2 = FP64 product, 3 = int8 digit product; algorithmic bytes = the
+        // FP32 triangle once, flops = 2 N^2 k
+        globe_ctx::ProfEvent e{};
+        if (c->prof_on) {
+            const double np = (double)((n + 63) / 64 * 64), L = c->limbs_mixed;
+            const double cols = 16.0 * ((k + 15) / 16);
+            HIP_CHECK(hipEventCreate(&e.a));
+            HIP_CHECK(hipEventCreate(&e.b));
+            e.pass = mixed ? 3 : 2;
+            e.iter = c->prof_iter;
+            e.bytes = 4.0 * (double)n * (n + 1) / 2;
+            e.flops = 2.0 * (double)n * n * k;
+            e.issued = mixed ? 2.0 * np * np * cols * L * (L + 1) / 2 : 2.0 * np * np * cols;
+            HIP_CHECK(hipEventRecord(e.a, s));
+        }
+        if (mixed)
+            launch_grm_product_i8(s, c->d_grm8.as<int8_t>(), grm_digit_rexp(c->d_grm8_aux.p, n), n, S0, n, k,
+                                  c->limbs_mixed, c->d_gdiag.as<double>(), th, GZ, n, skip, c->w_grm8_part.p);
+        else
+            launch_grm_product(s, c->d_grm.as<float>(), n, S0, n, k, th, GZ, n, skip,
+                               c->w_grm_part.as<double>());
+        if (c->prof_on) {
+            HIP_CHECK(hipEventRecord(e.b, s));
+            c->prof.push_back(e);
+        }
         return;
     }
     if (c->m == 0) {
@@ -308,7 +348,7 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
     const bool relax = o.precision == GLOBE_PREC_MIXED && !(o.flags & GLOBE_PCG_FIXED_DIGITS);
     struct LimbsGuard {
         globe_ctx* c;
-        ~LimbsGuard() { c->limbs_mixed = kLimbsMixed; }
+        ~LimbsGuard() { c->limbs_mixed = c->limbs_default; }
     } guard{c};
     c->limbs_mixed = kLimbsMixed;
     while (true) {
@@ -595,6 +635,7 @@ int globe_load_grm(globe_ctx* c, const float* lt) {
     HIP_CHECK(hipMemcpy(c->d_gdiag.p, gd.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     c->gdiag_ok = true;
     c->gen = 2;
+    c->grm8_ok = false;
     API_END(c)
 }
 
@@ -618,6 +659,7 @@ int globe_grm_from_geno(globe_ctx* c) {
     HIP_CHECK(hipStreamSynchronize(s));
     // the context now works from the GRM, as after globe_load_grm
     c->gen = 2;
+    c->grm8_ok = false;
     c->d_snp.release();
     c->d_ind.release();
     c->m = 0;
@@ -1116,6 +1158,14 @@ int globe_memcpy_d2h(globe_ctx* c, void* dst, const void* src, int64_t bytes) {
     require(dst && src && bytes >= 0, GLOBE_E_INVALID, "globe_memcpy_d2h: bad arguments");
     HIP_CHECK(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
+    API_END(c)
+}
+
+int globe_set_mixed_digits(globe_ctx* c, int digits) {
+    API_BEGIN(c)
+    require(digits >= 2 && digits <= kLimbsMixed, GLOBE_E_INVALID, "mixed digits must be 2 to 4");
+    c->limbs_default = digits;
+    c->limbs_mixed = digits;
     API_END(c)
 }
 

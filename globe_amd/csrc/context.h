@@ -33,14 +33,19 @@ struct globe_ctx {
     std::vector<float> qc_maf, h_freq;
     globe::DevBuf d_gdiag;
     globe::DevBuf w_grm_part;    // GRM product strip partials
+    // int8 digit tiles of the GRM for GLOBE_PREC_MIXED products, built on first use
+    globe::DevBuf d_grm8, d_grm8_aux, w_grm8_part;
+    bool grm8_ok = false;
     bool gdiag_ok = false;
     // genotype ingest: bytes of .bed rows per pipeline chunk, and the last ingest's
     // timings (seconds: total, host fill, host waiting on the device; chunks; threads)
     int64_t ingest_chunk = 256ll << 20;
     double ingest_stats[5] = {0, 0, 0, 0, 0};
 
-    // int8 digits of GLOBE_PREC_MIXED contractions (run_pcg lowers it per iteration)
+    // int8 digits of GLOBE_PREC_MIXED contractions (run_pcg lowers it per iteration), and
+    // the count used outside a PCG solve (globe_set_mixed_digits)
     int limbs_mixed = 4;
+    int limbs_default = 4;
 
     // variance components
     std::array<double, 5> theta{0, 0, 0, 0, 0};

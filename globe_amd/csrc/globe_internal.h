@@ -175,6 +175,20 @@ void launch_grm_product(hipStream_t s, const float* tiles, int32_t n, const doub
                         int k, const double* theta_cols, double* GZ, int64_t ldg, const int* skip,
                         double* part);
 size_t grm_product_part_bytes(int32_t n);
+
+// Mixed-precision GRM product (k_grm_i8.hip): int8 digit tiles of the full square with
+// per-row exponents, built from the FP32 tiles (aux: grm_digit_aux_bytes of scratch that
+// keeps the row exponents, read back by grm_digit_rexp).
+size_t grm_digit_bytes(int32_t n);
+size_t grm_digit_aux_bytes(int32_t n);
+void launch_grm_digits(hipStream_t s, const float* tiles, int32_t n, void* aux, int8_t* G8);
+const int32_t* grm_digit_rexp(const void* aux, int32_t n);
+// GZ = sig2g_c (G S0) with L (2..4) digits of G and S0; gdiag: the FP64 diagonal of G;
+// scratch: grm_i8_part_bytes(n)
+size_t grm_i8_part_bytes(int32_t n);
+void launch_grm_product_i8(hipStream_t s, const int8_t* G8, const int32_t* rexp, int32_t n, const double* S0,
+                           int64_t ldn, int k, int L, const double* gdiag, const double* theta_cols, double* GZ,
+                           int64_t ldg, const int* skip, void* scratch);
 // tile column j0 / 64 of G (FP32) for i >= j0 + c from the n x k columns Gc (j0 % 64 == 0)
 void launch_grm_scatter(hipStream_t s, const double* Gc, int64_t ldg, int32_t n, int64_t j0, int k,
                         float* tiles);

@@ -37,12 +37,18 @@ typedef enum {
 
 /* globe_vx modes */
 enum { GLOBE_VX_FULL = 0, GLOBE_VX_GRM_ONLY = 1 };
-/* Arithmetic of the packed-genotype contraction W (W^T z); everything else is FP64.
+/* Arithmetic of the genetic product; everything else is FP64.
+ * Packed genotypes, W (W^T z):
  *   FP64:      int8 matrix cores on 7 int8 digits of a 54-bit block-fixed-point z,
  *              exact int32 accumulation -- as accurate as an FP64 dot product;
  *   MIXED:     the same with 4 digits (30-bit fixed point);
  *   FP64_VALU: FP64 fused multiply-adds on the vector ALUs (validation path).
- * The GRM product always accumulates in FP64. */
+ * Dense GRM, G S0:
+ *   FP64, FP64_VALU: the FP32 entries on the FP64 matrix cores, FP64 accumulation;
+ *   MIXED:     int8 matrix cores on 4 int8 digits of G (per-row block fixed point of
+ *              the off-diagonal entries; the diagonal applied in FP64) and of S0, exact
+ *              int32 accumulation. The digit tiles (4 N^2 bytes of device memory) are
+ *              built on the first mixed product after the GRM is loaded. */
 enum { GLOBE_PREC_FP64 = 0, GLOBE_PREC_MIXED = 1, GLOBE_PREC_FP64_VALU = 2 };
 /* PCG stopping rules */
 enum {
@@ -63,8 +69,8 @@ typedef struct {
     int32_t flags;      /* GLOBE_PCG_* bits, 0 by default */
 } globe_pcg_opts;
 
-/* GLOBE_PREC_MIXED solves relax the digits of the genotype contraction as the residual
- * falls (4, then 3, then 2 int8 digits: inexact Krylov relaxation, the operator error of
+/* GLOBE_PREC_MIXED solves relax the digits of the genetic product as the residual falls
+ * (4, then 3, then 2 int8 digits: inexact Krylov relaxation, the operator error of
  * iteration j at most 0.01 tol |b| / |r_j|); this flag keeps 4 digits throughout. */
 #define GLOBE_PCG_FIXED_DIGITS 1
 
@@ -299,11 +305,21 @@ int globe_serve_finished(globe_ctx* ctx, int64_t* ids, int32_t max, int32_t* cou
 int globe_serve_take(globe_ctx* ctx, int64_t id, double* x, globe_serve_result* res);
 int globe_serve_close(globe_ctx* ctx);
 
+/* ---- mixed-precision digits (no reference counterpart) ----------------------------- */
+/* int8 digits (2..4, default 4) of GLOBE_PREC_MIXED genetic products applied outside a PCG
+ * solve (globe_vx and friends). PCG solves start at 4 and lower the count themselves as
+ * the residual falls, unless GLOBE_PCG_FIXED_DIGITS is set; they leave this value in
+ * place. Fewer digits trade accuracy (about 2^(1-8L) of each operand's scale) for speed. */
+int globe_set_mixed_digits(globe_ctx* ctx, int digits);
+
 /* ---- kernel timing (no reference counterpart) --------------------------------------- */
-/* When enabled, every packed-genotype contraction launch is bracketed by HIP events on
- * the context stream (launches that a finished PCG skips are discarded). which: 0 = pass 1
- * (SNP-major rows, W^T z), 1 = pass 2 (individual-major rows, W u). bytes / flops are
- * the algorithmic figures per launch: rows x ceil(K/4) packed bytes, 2 N M k flops. */
+/* When enabled, every genetic-product launch is bracketed by HIP events on the context
+ * stream (launches that a finished PCG skips are discarded). which: 0 = pass 1 of the
+ * packed-genotype contraction (SNP-major rows, W^T z), 1 = pass 2 (individual-major rows,
+ * W u), 2 = dense GRM product in FP64, 3 = dense GRM product on int8 digits (mixed).
+ * bytes / flops are the algorithmic figures per launch: rows x ceil(K/4) packed bytes and
+ * 2 N M k flops for the genotypes; 4 N(N+1)/2 bytes (the FP32 triangle once) and 2 N^2 k
+ * flops for the GRM. */
 typedef struct {
     int64_t launches;
     double ms_total;

@@ -421,3 +421,61 @@ def probes(kind, indiv, times, theta, nmc, seed):
             z3 = np.sqrt(-2.0 * np.log(unit(b[0]))) * np.cos(2 * np.pi * unit(b[1]))
             out[:, s] = np.sqrt(sg) * z1 + np.sqrt(se) * z3 + z2a * (l00 + l10 * times) + z2b * l11 * times
     return out
+
+
+# ------------------------------------------------------- mixed-precision GRM product
+def digits4(v):
+    """Four balanced base-256 digits of int64 v (|v| <= 2^30), most significant first:
+    v = sum_a d[a] 256^(3 - a), d[1..3] in [-128, 127] (k_grm_i8.hip, digits4)."""
+    v = np.asarray(v, dtype=np.int64).copy()
+    d = [None] * 4
+    for a in (3, 2, 1):
+        lo = ((v & 255) ^ 128) - 128
+        d[a] = lo
+        v = (v - lo) >> 8
+    d[0] = v
+    return np.stack(d)
+
+
+def _exp_of(m):
+    """frexp exponent e (m = f 2^e, f in [0.5, 1)), 0 where m == 0."""
+    return np.where(m > 0, np.frexp(m)[1], 0).astype(np.int64)
+
+
+def grm_mixed_product(lt, n, S0, digits, sig2g=1.0):
+    """sig2g G S0 as the mixed-precision device product computes it, restated with exact
+    integer arithmetic (k_grm_i8.hip): per-row block fixed point of the off-diagonal FP32
+    entries (scale 2^e_i, e_i the frexp exponent of the row's largest |G_ij|, j != i),
+    per-column block fixed point of S0, four balanced int8 digits of each 30-bit value,
+    the digit pairs a + b < L summed exactly by weight class over chunks of tile columns,
+    then combined in FP64 in the device's order, with G_ii S0_i added in FP64.
+    S0: n x k. Returns n x k."""
+    S0 = np.asarray(S0, dtype=np.float64).reshape(n, -1)
+    L = int(digits)
+    G32 = np.zeros((n, n), dtype=np.float32)
+    il = np.tril_indices(n)
+    G32[il] = np.asarray(lt, dtype=np.float32)
+    G32 = G32 + np.tril(G32, -1).T
+    gdiag = np.diag(G32).astype(np.float64)
+    off = G32.copy()
+    np.fill_diagonal(off, np.float32(0))
+    ei = _exp_of(np.abs(off).max(axis=1))
+    dG = digits4(np.rint(np.ldexp(off.astype(np.float64), (30 - ei)[:, None])).astype(np.int64))
+    ec = _exp_of(np.abs(S0).max(axis=0))
+    dS = digits4(np.rint(np.ldexp(S0, (30 - ec)[None, :])).astype(np.int64))
+    ntiles = (n + 63) // 64
+    H = max(4, (ntiles + 255) // 256)
+    total = np.zeros_like(S0)
+    for h in range(H):
+        j0, j1 = 64 * (h * ntiles // H), min(n, 64 * ((h + 1) * ntiles // H))
+        part = np.zeros_like(S0)
+        for w in range(L):
+            acc = np.zeros(S0.shape, dtype=np.int64)
+            for a in range(w + 1):
+                b = w - a
+                if a < L and b < L and j1 > j0:
+                    acc += dG[a][:, j0:j1] @ dS[b][j0:j1, :]
+            part = part + np.ldexp(acc.astype(np.float64), ei[:, None] + ec[None, :] - 12 - 8 * w)
+        total = total + part
+    total = total + gdiag[:, None] * S0
+    return sig2g * total
