@@ -136,13 +136,15 @@ __global__ __launch_bounds__(256) void k_grm_digits(const float* __restrict__ ti
         *reinterpret_cast<uint4*>(dst + a * kPlaneB + swz(r, 16 * p)) = *reinterpret_cast<const uint4*>(d[a]);
 }
 
-// cexp[c] = frexp exponent of max_j |S0[c][j]| (0 for a zero column); one block per column.
-__global__ __launch_bounds__(256) void k_grm_colexp(const double* __restrict__ S0, int64_t ldn, int32_t n,
-                                                    int32_t* __restrict__ cexp, const int* skip) {
+// cmax[c] = max_j |S0[c][j]| as its bit pattern (non-negative doubles order like their
+// bits), from a grid of (parts, columns) blocks; cmax starts at zero.
+__global__ __launch_bounds__(256) void k_grm_colmax(const double* __restrict__ S0, int64_t ldn, int32_t n,
+                                                    unsigned long long* __restrict__ cmax, const int* skip) {
     if (skip && *skip) return;
-    const int c = blockIdx.x;
+    const int c = blockIdx.y;
     double m = 0.0;
-    for (int64_t j = threadIdx.x; j < n; j += 256) m = fmax(m, fabs(S0[c * ldn + j]));
+    for (int64_t j = blockIdx.x * 256ll + threadIdx.x; j < n; j += (int64_t)gridDim.x * 256)
+        m = fmax(m, fabs(S0[c * ldn + j]));
     __shared__ double sh[256];
     sh[threadIdx.x] = m;
     __syncthreads();
@@ -150,18 +152,22 @@ __global__ __launch_bounds__(256) void k_grm_colexp(const double* __restrict__ S
         if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        int e = 0;
-        if (sh[0] > 0.0) (void)frexp(sh[0], &e);
-        cexp[c] = e;
-    }
+    if (threadIdx.x == 0 && sh[0] > 0.0) atomicMax(cmax + c, (unsigned long long)__double_as_longlong(sh[0]));
+}
+
+// frexp exponent of a column maximum held as bits (0 for a zero column)
+__device__ inline int col_exp(unsigned long long bits) {
+    int e = 0;
+    const double m = __longlong_as_double((long long)bits);
+    if (m > 0.0) (void)frexp(m, &e);
+    return e;
 }
 
 // S0 digits Sd[J][a][c][swizzled row] for rows j < 64 ntiles, columns c < cols (zero
 // past n and kk); one thread per (row, column).
 __global__ __launch_bounds__(256) void k_grm_s0_digits(const double* __restrict__ S0, int64_t ldn,
                                                        int32_t n, int kk, int cols, int32_t ntiles,
-                                                       const int32_t* __restrict__ cexp,
+                                                       const unsigned long long* __restrict__ cmax,
                                                        int8_t* __restrict__ Sd, const int* skip) {
     if (skip && *skip) return;
     const int64_t total = (int64_t)ntiles * kT * cols;
@@ -169,7 +175,7 @@ __global__ __launch_bounds__(256) void k_grm_s0_digits(const double* __restrict_
         const int64_t j = e % ((int64_t)ntiles * kT);
         const int c = (int)(e / ((int64_t)ntiles * kT));
         int v = 0;
-        if (c < kk && j < n) v = (int)rint(ldexp(S0[c * ldn + j], 30 - cexp[c]));
+        if (c < kk && j < n) v = (int)rint(ldexp(S0[c * ldn + j], 30 - col_exp(cmax[c])));
         int8_t d[4];
         digits4(v, d);
         const int64_t J = j / kT;
@@ -198,7 +204,8 @@ template <int CG, int L>
 __global__ __launch_bounds__(256 * CG) void k_grm_i8(const int8_t* __restrict__ G8,
                                                      const int8_t* __restrict__ Sd,
                                                      const int32_t* __restrict__ rexp,
-                                                     const int32_t* __restrict__ cexp, int kk, int32_t n,
+                                                     const unsigned long long* __restrict__ cmax, int kk,
+                                                     int32_t n,
                                                      int32_t ntiles, int H, double* __restrict__ part,
                                                      const int* skip) {
     if (skip && *skip) return;
@@ -264,7 +271,7 @@ __global__ __launch_bounds__(256 * CG) void k_grm_i8(const int8_t* __restrict__ 
     // D element v of lane l: (row 4 (l / 16) + v, column l % 16)
     const int col = 16 * cg + (lane & 15);
     if (col >= kk) return;
-    const int ec = cexp[col];
+    const int ec = col_exp(cmax[col]);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int64_t i = (int64_t)I * kT + 16 * rg + 4 * (lane >> 4) + v;
@@ -300,8 +307,9 @@ __global__ __launch_bounds__(256) void k_grm_i8_finish(const double* __restrict_
 
 template <int CG, int L>
 void launch_i8(hipStream_t s, dim3 g, const int8_t* G8, const int8_t* Sd, const int32_t* rexp,
-               const int32_t* cexp, int kk, int32_t n, int32_t ntiles, int H, double* part, const int* skip) {
-    k_grm_i8<CG, L><<<g, 256 * CG, 0, s>>>(G8, Sd, rexp, cexp, kk, n, ntiles, H, part, skip);
+               const unsigned long long* cmax, int kk, int32_t n, int32_t ntiles, int H, double* part,
+               const int* skip) {
+    k_grm_i8<CG, L><<<g, 256 * CG, 0, s>>>(G8, Sd, rexp, cmax, kk, n, ntiles, H, part, skip);
 }
 
 int chunks_for(int32_t ntiles) { return std::max(4, (ntiles + kMaxChunkTiles - 1) / kMaxChunkTiles); }
@@ -356,26 +364,29 @@ void launch_grm_product_i8(hipStream_t s, const int8_t* G8, const int32_t* rexp,
     const int H = chunks_for(ntiles);
     double* part = static_cast<double*>(scratch);
     int8_t* Sd = static_cast<int8_t*>(scratch) + align1k(sizeof(double) * H * kI8Cols * (size_t)n);
-    int32_t* cexp = reinterpret_cast<int32_t*>(Sd + align1k((size_t)ntiles * kDig * kI8Cols * kT));
+    unsigned long long* cmax =
+        reinterpret_cast<unsigned long long*>(Sd + align1k((size_t)ntiles * kDig * kI8Cols * kT));
+    const unsigned gm = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 2047) / 2048, 64));
     const dim3 g(ntiles, H);
     const unsigned gf = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
     for (int c = 0; c < k; c += kI8Cols) {
         const int kk = std::min(kI8Cols, k - c);
         const int CG = kk > 16 ? 2 : 1;
         const int cols = 16 * CG;
-        k_grm_colexp<<<kk, 256, 0, s>>>(S0 + c * ldn, ldn, n, cexp, skip);
+        launch_fill(s, reinterpret_cast<double*>(cmax), kk, 0.0);  // all-zero bits
+        k_grm_colmax<<<dim3(gm, kk), 256, 0, s>>>(S0 + c * ldn, ldn, n, cmax, skip);
         HIP_CHECK(hipGetLastError());
         const unsigned gd = (unsigned)std::min<int64_t>(((int64_t)ntiles * kT * cols + 255) / 256, 4096);
-        k_grm_s0_digits<<<gd, 256, 0, s>>>(S0 + c * ldn, ldn, n, kk, cols, ntiles, cexp, Sd, skip);
+        k_grm_s0_digits<<<gd, 256, 0, s>>>(S0 + c * ldn, ldn, n, kk, cols, ntiles, cmax, Sd, skip);
         HIP_CHECK(hipGetLastError());
         if (CG == 1) {
-            if (L == 2) launch_i8<1, 2>(s, g, G8, Sd, rexp, cexp, kk, n, ntiles, H, part, skip);
-            else if (L == 3) launch_i8<1, 3>(s, g, G8, Sd, rexp, cexp, kk, n, ntiles, H, part, skip);
-            else launch_i8<1, 4>(s, g, G8, Sd, rexp, cexp, kk, n, ntiles, H, part, skip);
+            if (L == 2) launch_i8<1, 2>(s, g, G8, Sd, rexp, cmax, kk, n, ntiles, H, part, skip);
+            else if (L == 3) launch_i8<1, 3>(s, g, G8, Sd, rexp, cmax, kk, n, ntiles, H, part, skip);
+            else launch_i8<1, 4>(s, g, G8, Sd, rexp, cmax, kk, n, ntiles, H, part, skip);
         } else {
-            if (L == 2) launch_i8<2, 2>(s, g, G8, Sd, rexp, cexp, kk, n, ntiles, H, part, skip);
-            else if (L == 3) launch_i8<2, 3>(s, g, G8, Sd, rexp, cexp, kk, n, ntiles, H, part, skip);
-            else launch_i8<2, 4>(s, g, G8, Sd, rexp, cexp, kk, n, ntiles, H, part, skip);
+            if (L == 2) launch_i8<2, 2>(s, g, G8, Sd, rexp, cmax, kk, n, ntiles, H, part, skip);
+            else if (L == 3) launch_i8<2, 3>(s, g, G8, Sd, rexp, cmax, kk, n, ntiles, H, part, skip);
+            else launch_i8<2, 4>(s, g, G8, Sd, rexp, cmax, kk, n, ntiles, H, part, skip);
         }
         HIP_CHECK(hipGetLastError());
         k_grm_i8_finish<<<dim3(gf, kk), 256, 0, s>>>(part, n, H, theta_cols + c * 5, gdiag, S0 + c * ldn, ldn,
