@@ -137,6 +137,7 @@ def load_library(path: str = LIB_PATH):
         "globe_aireml_fit_reduced": ([P, P, P, P, P, P, P, P, P, P], C.c_int),
         "globe_grm_from_geno": ([P], C.c_int),
         "globe_get_grm": ([P, P], C.c_int),
+        "globe_get_grm_rows": ([P, i64, i64, P], C.c_int),
         "globe_serve_open": ([P, P], C.c_int),
         "globe_serve_submit": ([P, P, P], C.c_int),
         "globe_serve_step": ([P, P, P, P], C.c_int),

@@ -26,7 +26,7 @@ LIB_SOURCES = ["k_record.hip", "k_geno.hip", "k_grm.hip", "k_grm_i8.hip", "k_mfm
                "serve.cpp", "ingest.cpp"]
 CLI_COMMON = ["io.cpp"]
 CLIS = {"runPCGsolver": "cli_pcgsolver.cpp", "runAIREML": "cli_aireml.cpp",
-        "runAIREML_reducedModel": "cli_aireml_reduced.cpp"}
+        "runAIREML_reducedModel": "cli_aireml_reduced.cpp", "makeGRM": "cli_make_grm.cpp"}
 
 DEVICE_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

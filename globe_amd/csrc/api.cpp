@@ -689,6 +689,26 @@ int globe_get_grm(globe_ctx* c, float* lt) {
     API_END(c)
 }
 
+int globe_get_grm_rows(globe_ctx* c, int64_t r0, int64_t r1, float* rows) {
+    API_BEGIN(c)
+    require(c->gen == 2, GLOBE_E_STATE, "no GRM in the context");
+    require(r0 >= 0 && r0 <= r1 && r1 <= c->n, GLOBE_E_INVALID, "rows must satisfy 0 <= r0 <= r1 <= n");
+    require(rows != nullptr || r0 == r1, GLOBE_E_INVALID, "null output");
+    hipStream_t s = c->stream;
+    DevBuf stage;
+    const int64_t base = r0 * (r0 + 1) / 2;
+    stage.ensure(sizeof(float) * (size_t)std::min<int64_t>(kGrmStageFloats, r1 * (r1 + 1) / 2 - base + c->n));
+    for (int64_t a = r0; a < r1;) {
+        const int64_t b = std::min(r1, grm_chunk_end(a, c->n, kGrmStageFloats));
+        const int64_t off = a * (a + 1) / 2, cnt = b * (b + 1) / 2 - off;
+        launch_grm_lt_from_tiles(s, c->d_grm.as<float>(), a, b, stage.as<float>());
+        HIP_CHECK(hipMemcpyAsync(rows + (off - base), stage.p, sizeof(float) * cnt, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        a = b;
+    }
+    API_END(c)
+}
+
 int globe_genetic_diag(globe_ctx* c, double* out) {
     API_BEGIN(c)
     ensure_gdiag(c);

@@ -7,7 +7,10 @@
 // Opt-in extensions: --probes gaussian|rademacher|exact (default gaussian: the
 // reference's tilde-V probes), --seed n, --pcgTol v (1e-5), --precision
 // fp64|mixed|fp64-valu, --device d, --geno <plinkPrefix> (packed genotypes instead of
-// the GRM; the reference's runAIREML only reads a GRM).
+// the GRM; the reference's runAIREML only reads a GRM), --writeSolutions (also write
+// <output_prefix>_beta.bin, _uhat.bin and _bhat.bin: the GLS / BLUP solve at the fitted
+// components, the files the reference's commented-out saveVector calls name,
+// varCompEst.cpp:1493-1495).
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -55,6 +58,7 @@ int main(int argc, char** argv) {
     o.pcg = globe_pcg_opts{1e-5, 1000, GLOBE_X0_PRECOND, GLOBE_RULE_ALL_COLUMNS, 0, GLOBE_PREC_FP64, 0};
     double theta[5] = {0.5, 0.3, 0.2, 0.1, 0.1};  // VarCompDataStruct defaults (:47-51)
     int device = 0;
+    bool write_solutions = false;
     std::string geno;
     for (int i = 4; i < argc; ++i) {
         const std::string a = argv[i];
@@ -74,6 +78,7 @@ int main(int argc, char** argv) {
         else if (a == "--pcgTol") o.pcg.tol = std::stod(next());
         else if (a == "--device") device = std::stoi(next());
         else if (a == "--geno") geno = next();
+        else if (a == "--writeSolutions") write_solutions = true;
         else if (a == "--probes") {
             const std::string v = next();
             if (v == "gaussian") o.probes = 0;
@@ -136,6 +141,18 @@ int main(int argc, char** argv) {
     char buf[256];
     std::snprintf(buf, sizeof(buf), "AIREML: %d iterations, %.4f s per iteration", iters, spi);
     emit(buf);
+    if (write_solutions) {
+        // fitMME's solve at the fitted components (x0 = Q^-1 b, all-columns rule)
+        check(globe_set_varcomp(ctx, theta), "variance components");
+        std::vector<double> beta(c.p + 1), uhat(c.n), bhat(2 * (size_t)c.n);
+        globe_pcg_stats st{};
+        check(globe_gls_blup(ctx, c.y.data(), &o.pcg, beta.data(), nullptr, uhat.data(), bhat.data(), &st),
+              "solutions");
+        globe_io::save_vector(beta, out + "_beta.bin");
+        globe_io::save_vector(uhat, out + "_uhat.bin");
+        globe_io::save_vector(bhat, out + "_bhat.bin");
+        emit("Solutions written to " + out + "_beta.bin, _uhat.bin and _bhat.bin.");
+    }
     globe_destroy(ctx);
     g_log.close();
     return 0;

@@ -130,6 +130,10 @@ int globe_load_grm(globe_ctx* ctx, const float* grm_lt);
 int globe_grm_from_geno(globe_ctx* ctx);
 /* The context's GRM as the n(n+1)/2 FP32 lower triangle, row-major (GCTA .grm.bin order). */
 int globe_get_grm(globe_ctx* ctx, float* grm_lt);
+/* Rows r0 .. r1 - 1 of that triangle (r1(r1+1)/2 - r0(r0+1)/2 floats, starting with
+ * element (r0, 0)), so a writer can stream a GRM larger than host memory (80 GB at
+ * N = 200k). The reference declares a GRM builder but ships none (PCGsolver.cpp:165-167). */
+int globe_get_grm_rows(globe_ctx* ctx, int64_t r0, int64_t r1, float* rows);
 /* Replaces computeGeneticDiagonal (PCGsolver.cpp:925-989): diag(G), n doubles. */
 int globe_genetic_diag(globe_ctx* ctx, double* out);
 

@@ -16,6 +16,7 @@ from helpers import golden_oracle, load_golden, rel
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PCG = os.path.join(ROOT, "globe_amd", "bin", "runPCGsolver")
 AIREML = os.path.join(ROOT, "globe_amd", "bin", "runAIREML")
+MAKEGRM = os.path.join(ROOT, "globe_amd", "bin", "makeGRM")
 SIG = ["--sig2e", "0.6", "--sig2g", "0.4", "--sig2b0", "0.4", "--sigb01", "0.2", "--sig2b1", "0.2"]
 
 
@@ -114,3 +115,71 @@ def test_runaireml_runs_and_logs(case_files):
     log = open(out + ".log").read()
     assert "Iteration 1 took" in log and "Final variance components" in log
     assert log.startswith("Command:")
+    assert not os.path.exists(out + "_beta.bin")  # the reference writes no solutions
+
+
+@pytest.mark.gpu
+def test_runaireml_write_solutions(case_files):
+    """--writeSolutions: GLS / BLUP at the fitted components into _beta/_uhat/_bhat.bin
+    (the reference's commented-out saveVector calls, varCompEst.cpp:1493-1495)."""
+    p = case_files
+    out = os.path.join(p["dir"], "aireml_sol")
+    print("[case] runAIREML --writeSolutions", flush=True)
+    r = run([AIREML, p["grm_input"], p["grm_grm"], out, "--maxiter", "2", "--numMC", "4", "--writeSolutions",
+             "--sig2e", "0.5", "--sig2g", "0.5", "--sig2b0", "0.3", "--sigb01", "0.0", "--sig2b1", "0.3"])
+    assert r.returncode == 0, r.stderr + r.stdout
+    log = open(out + ".log").read()
+    theta = []
+    for name in ("sig2e", "sig2g", "sig2b0", "sigb01", "sig2b1"):
+        line = next(ln for ln in log.splitlines() if ln.strip().startswith(name + " ="))
+        theta.append(float(line.split("=")[1].split("+/-")[0]))
+    d = load_golden("grm_n180")
+    res = {s: synth.read_vector_bin(f"{out}_{s}.bin") for s in ("beta", "uhat", "bhat")}
+    assert res["beta"].shape == (4,) and res["bhat"].shape == (2 * d["cohort"].n,)
+    # the log prints 6 significant digits of the components
+    ro = golden_oracle(d, theta=tuple(theta)).gls_blup(d["y"], tol=1e-8, x0_mode=1, rule=1)
+    for s in res:
+        assert rel(res[s], ro[s]) < 1e-4, s
+
+
+def test_makegrm_flag_and_file_errors(case_files, tmp_path):
+    p = case_files
+    r = run([MAKEGRM])
+    assert r.returncode == 1 and "Usage" in r.stderr
+    r = run([MAKEGRM, "--geno", p["geno"], "--output", str(tmp_path / "g"), "--bogus"])
+    assert r.returncode == 1 and "Unknown flag: --bogus" in r.stderr
+    r = run([MAKEGRM, "--geno", p["geno"], "--output", str(tmp_path / "g"), "--minMAF", "0.7"])
+    assert r.returncode == 1 and "Invalid MAF" in r.stderr
+    r = run([MAKEGRM, "--geno", str(tmp_path / "nothing"), "--output", str(tmp_path / "g")])
+    assert r.returncode == 1 and ".fam" in r.stderr
+    assert not os.path.exists(str(tmp_path / "g.grm.bin"))
+
+
+@pytest.mark.gpu
+def test_makegrm_writes_wwt_and_round_trips(case_files):
+    """makeGRM writes G = W W^T / M over the QC-kept SNPs as a .grm.bin; runPCGsolver
+    --grm on that file reproduces runPCGsolver --geno."""
+    p = case_files
+    d = load_golden("geno_n203")
+    c = d["cohort"]
+    out = os.path.join(p["dir"], "made")
+    print("[case] makeGRM", flush=True)
+    r = run([MAKEGRM, "--geno", p["geno"], "--output", out, "--minMAF", str(float(d["min_maf"])),
+             "--maxMissingRate", str(float(d["max_miss"]))])
+    assert r.returncode == 0, r.stderr + r.stdout
+    lt = np.fromfile(out + ".grm.bin", dtype=np.float32)
+    assert lt.shape == (c.n * (c.n + 1) // 2,)
+    W = synth.standardized(d["bed_rows"][d["keep"]], c.n, d["freq"])
+    G = (W @ W.T) / W.shape[1]
+    ref = G[np.tril_indices(c.n)]
+    assert np.max(np.abs(lt - ref)) <= 2.0 * np.finfo(np.float32).eps * np.max(np.abs(G))
+    # the written GRM drives the --grm path to the --geno solution
+    mm = ["--minMAF", str(float(d["min_maf"])), "--maxMissingRate", str(float(d["max_miss"]))]
+    res = {}
+    for kind, src in (("geno", ["--geno", p["geno"], *mm]), ("grm", ["--grm", out])):
+        o = os.path.join(p["dir"], "rt_" + kind)
+        r = run([PCG, "--input", p["input"], "--output", o, *SIG, *src])
+        assert r.returncode == 0, r.stderr + r.stdout
+        res[kind] = {s: synth.read_vector_bin(f"{o}_{s}.bin") for s in ("beta", "uhat", "bhat")}
+    for s in ("beta", "uhat", "bhat"):
+        assert rel(res["grm"][s], res["geno"][s]) < 1e-5, s
