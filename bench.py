@@ -20,7 +20,10 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -90,6 +93,96 @@ def cpu_baseline(g, c, kept, iters, sample_snps, threads):
                       f"Nrec={c.nrec}, k=1 on {sample_snps} of {kept} SNPs ({t_sample:.2f} s each), "
                       f"scaled linearly to M and by the GPU FP64 iteration count ({iters}); extrapolated",
             "sec_per_vx_extrapolated": t_full}
+
+
+def _reference_iteration_time(exe, c, y, rows, theta, threads, max_iters):
+    """Seconds per PCG iteration of the reference runPCGsolver on cohort c with the given
+    .bed rows: its --debug mode prints one line per iteration; stdout is a pseudo-terminal
+    so each line arrives as it is printed. The run is stopped after max_iters lines (or
+    at convergence). Returns (seconds per iteration, iterations timed, threads, seconds
+    of input parsing and set-up)."""
+    import pty
+    from globe_amd import synth
+    tmp = tempfile.mkdtemp(prefix="globe_cpu_ref_")
+    try:
+        prefix = os.path.join(tmp, "sample")
+        synth.write_plink(prefix, rows, c.n)
+        synth.write_text_input(prefix + ".txt", c, y)
+        cmd = [exe, "--input", prefix + ".txt", "--output", prefix + "_out", "--geno", prefix,
+               "--minMAF", "0", "--maxMissingRate", "1", "--debug"]
+        for f, v in zip(("--sig2e", "--sig2g", "--sig2b0", "--sigb01", "--sig2b1"), theta):
+            cmd += [f, repr(float(v))]
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        master, slave = pty.openpty()
+        t0 = time.perf_counter()
+        proc = subprocess.Popen(cmd, stdout=slave, stderr=slave, env=env, close_fds=True)
+        os.close(slave)
+        stamps, nthr, t_start, buf, done = [], threads, None, b"", False
+        try:
+            while len(stamps) < max_iters and not done:
+                try:
+                    chunk = os.read(master, 65536)
+                except OSError:
+                    break
+                if not chunk:
+                    break
+                now = time.perf_counter()
+                buf += chunk
+                *lines, buf = buf.split(b"\n")
+                for ln in lines:
+                    t = ln.decode(errors="replace")
+                    if "OpenMP will utilize up to" in t:
+                        nthr = int(t.split("up to")[1].split()[0])
+                    elif "Starting PCG solver" in t:
+                        t_start = now
+                    elif "[runPCG_multi] Iter " in t:
+                        stamps.append(now)
+                    elif "Converged at iteration" in t:
+                        stamps.append(now)
+                        done = True
+        finally:
+            proc.kill()
+            proc.wait(timeout=60)
+            os.close(master)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    if t_start is None or len(stamps) < 3:
+        return None
+    # iteration 1 ends at stamps[0]; time the ones after it
+    t_it = (stamps[-1] - stamps[0]) / (len(stamps) - 1)
+    return t_it, len(stamps) - 1, nthr, t_start - t0
+
+
+def cpu_baseline_reference(g, c, y, kept, iters, sample_snps, threads, theta, max_iters=7):
+    """The reference's own runPCGsolver (oracle/_ref, compiled from its sources by
+    oracle/Makefile) timed on the host cores: the same cohort and phenotype with the
+    first s kept SNPs written as a .bed, for two sample sizes s. Its PCG solves [X | y]
+    (p + 2 columns, FP32); the time per iteration is fitted as a + b s (record-space work
+    plus genotype passes) and evaluated at s = M, then multiplied by the GPU FP64
+    iteration count. Returns None when the binary is absent."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "runPCGsolver")
+    if not os.access(exe, os.X_OK):
+        return None
+    pts = []
+    for s in sample_snps:
+        rows = g.geno_rows(0, s)[:, : (c.n + 3) // 4]
+        r = _reference_iteration_time(exe, c, y, rows, theta, threads, max_iters)
+        if r is None:
+            return None
+        pts.append((s, *r))
+    (s1, t1, n1, nthr, _), (s2, t2, n2, _, setup) = pts[0], pts[-1]
+    b = (t2 - t1) / (s2 - s1)
+    a = t1 - b * s1
+    t_full = a + b * kept
+    k = c.X.shape[1] + 1
+    return {"value": k / (t_full * iters), "unit": "solves/s", "cores": nthr, "kind": "reference",
+            "sample": f"the reference runPCGsolver (FP32, OpenMP, {nthr} threads) on N={c.n}, Nrec={c.nrec}: "
+                      f"{n1} PCG iterations with {s1} SNPs ({t1:.3f} s each) and {n2} with {s2} SNPs "
+                      f"({t2:.3f} s each; input parsing and set-up {setup:.1f} s, not counted) of its "
+                      f"{k}-column [X | y] solve; per-iteration time a + b M fitted and evaluated at "
+                      f"M = {kept} ({t_full:.2f} s), times the GPU FP64 iteration count ({iters}); "
+                      f"{k} right-hand sides per solve; extrapolated",
+            "sec_per_iter_extrapolated": t_full}
 
 
 def serving_leg(g, c, kept, args):
@@ -248,6 +341,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--seed", type=int, default=20261206)
     ap.add_argument("--cpu-sample-snps", type=int, default=20_000)
+    ap.add_argument("--cpu-ref-snps", type=int, default=4_000,
+                    help="SNPs of the .bed the reference runPCGsolver is timed on")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=400,
                     help="requests replayed through the serving loop (0: skip)")
@@ -339,9 +434,18 @@ def main():
     traffic64, traffic64_src = measured_traffic("thin")
     cpu = None
     if not args.no_cpu:
-        log("CPU baseline ...")
-        cpu = cpu_baseline(g, c, kept, st64["iters"], min(args.cpu_sample_snps, kept),
-                           os.cpu_count() or 1)
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        log("CPU baseline: the reference runPCGsolver ...")
+        cpu = cpu_baseline_reference(g, c, y, kept, st64["iters"],
+                                     (min(args.cpu_ref_snps // 4, kept), min(args.cpu_ref_snps, kept)),
+                                     threads, (0.6, 0.4, 0.4, 0.2, 0.2))
+        log("CPU baseline: the oracle's FP32 restatement ...")
+        port = cpu_baseline(g, c, kept, st64["iters"], min(args.cpu_sample_snps, kept), threads)
+        if cpu is None:
+            cpu = port
+        else:
+            cpu["port"] = port
+        log(f"CPU baseline: {cpu['value']:.5f} solves/s ({cpu['kind']})")
     out = {
         "metric": "PCG solves/s at N=200k,M=500k (mixed vs FP64); AIREML iter s; p50/p99",
         "value": value,
