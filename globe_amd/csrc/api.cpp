@@ -649,11 +649,12 @@ int globe_grm_from_geno(globe_ctx* c) {
     HIP_CHECK(hipMemsetAsync(c->d_grm.p, 0, grm_tile_bytes(n), s));
     DevBuf GZ;
     GZ.ensure(sizeof(double) * (size_t)n * kb);
-    // G e_j in FP64 (W^T e_j is individual j's standardised row; W u runs on 7 int8
-    // digits, as accurate as FP64), rounded to FP32
+    // G e_j, rows i >= j (W^T e_j is individual j's standardised row; W u runs on 5
+    // int8 digits, about 1e-10 relative), rounded to FP32
     for (int64_t j0 = 0; j0 < n; j0 += kb) {
         const int k = (int)std::min<int64_t>(kb, n - j0);
-        geno_unit_columns(c, j0, k, GZ.as<double>());
+        // rows from j0 down to a multiple of 4096 (few distinct shapes for the tuner)
+        geno_unit_columns(c, j0, k, GZ.as<double>(), j0 / 4096 * 4096);
         launch_grm_scatter(s, GZ.as<double>(), n, n, j0, k, c->d_grm.as<float>());
     }
     HIP_CHECK(hipStreamSynchronize(s));

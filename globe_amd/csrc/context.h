@@ -131,12 +131,14 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
 void require(bool cond, int code, const std::string& msg);
 // packed-genotype contraction on the int8 matrix cores (k_mfma.hip): GZ = sig2g G S0
 // with `limbs` int8 digits per value of the right-hand side
-// (U_given: pass 1 already done by the caller, u = U_given, m x k; S0 unused)
+// (U_given: pass 1 already done by the caller, u = U_given, m x k; S0 unused; then only
+// rows row0 .. n - 1 of GZ are formed)
 void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, double* GZ,
                        int64_t ldg, const int* skip, int limbs, const double* theta_cols,
-                       const double* U_given = nullptr);
-// G [e_j0 .. e_j0+k-1] (n x k) from the resident genotypes, FP64-exact (GRM builder)
-void geno_unit_columns(globe_ctx* c, int64_t j0, int k, double* GZ);
+                       const double* U_given = nullptr, int64_t row0 = 0);
+// Rows i >= row0 of G [e_j0 .. e_j0+k-1] (n x k, row0 <= j0) from the resident genotypes
+// (GRM builder: the lower triangle needs rows i >= j only)
+void geno_unit_columns(globe_ctx* c, int64_t j0, int k, double* GZ, int64_t row0);
 // Probes generated on the device (k_probe.hip): kind 0 Gaussian tilde-V, 1 Rademacher,
 // 2 exact unit probes; seed = the Philox key.
 struct ProbeGen {

@@ -853,7 +853,8 @@ static std::vector<Variant> candidates(bool wide, int JT, int64_t nrows, int64_t
 
 // GZ (n x k) = sig2g G S0 for the packed-genotype cohort with `limbs` int8 digits.
 void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, double* GZ, int64_t ldg,
-                       const int* skip, int limbs, const double* theta_cols, const double* U_given) {
+                       const int* skip, int limbs, const double* theta_cols, const double* U_given,
+                       int64_t row0) {
     hipStream_t s = c->stream;
     const int64_t n = c->n, m = c->m;
     const int J = k * limbs;
@@ -873,9 +874,12 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
         c->cu_count = prop.multiProcessorCount;
     }
     const int ncu = c->cu_count;
+    // pass 2 forms rows r0 .. n - 1 (r0 > 0 only with U_given); fewer rows can take more
+    // splits, so the partial buffer is sized for the row count actually contracted
+    const int64_t r0 = U_given ? row0 : 0, n2 = n - r0;
     for (const Variant& v : candidates(wide, JT, m, nss1, ncu)) smax1 = std::max(smax1, v.S);
-    for (const Variant& v : candidates(wide, JT, n, nss2, ncu)) smax2 = std::max(smax2, v.S);
-    const size_t ob = std::max((size_t)smax1 * m, (size_t)smax2 * n) * J16 * 4;
+    for (const Variant& v : candidates(wide, JT, n2, nss2, ncu)) smax2 = std::max(smax2, v.S);
+    const size_t ob = std::max((size_t)smax1 * m, (size_t)smax2 * n2) * J16 * 4;
     const size_t bq = std::max(bq1, bq2);
     c->w_mix.ensure(round_up(bq, 256) + round_up(ob, 256) + 2 * kMaxCols * sizeof(int));
     uint8_t* Bq = c->w_mix.as<uint8_t>();
@@ -974,10 +978,10 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
                                               c->d_twof.as<double>(), Bq, part, skip);
     k_finish_cols<<<1, 256, 0, s>>>(part, qg2, k, 0, limbs, nullptr, c->w_C.as<double>(), skip);
     HIP_CHECK(hipGetLastError());
-    const int sp2 = contract(1, c->d_ind.as<uint8_t>(), c->pitch_i, n, m, Kpad2, nss2);
-    const int64_t eb2 = std::min<int64_t>((n + kEpiRows - 1) / kEpiRows, kMaxParts);
-    k_epilogue<<<(unsigned)eb2, 256, 0, s>>>(out, sp2, n, J16, limbs, k, sh2, 1, c->w_C.as<double>(),
-                                             nullptr, nullptr, nullptr, 0.0, GZ, ldg, nullptr, skip);
+    const int sp2 = contract(1, c->d_ind.as<uint8_t>() + r0 * c->pitch_i, c->pitch_i, n2, m, Kpad2, nss2);
+    const int64_t eb2 = std::min<int64_t>((n2 + kEpiRows - 1) / kEpiRows, kMaxParts);
+    k_epilogue<<<(unsigned)eb2, 256, 0, s>>>(out, sp2, n2, J16, limbs, k, sh2, 1, c->w_C.as<double>(),
+                                             nullptr, nullptr, nullptr, 0.0, GZ + r0, ldg, nullptr, skip);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -985,9 +989,14 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
 
 namespace globe {
 
-// GZ (n x k) = G [e_j0 .. e_j0+k-1] for the GRM builder: pass 1 is the direct row
-// extraction k_unit_u, pass 2 the FP64-exact contraction.
-void geno_unit_columns(globe_ctx* c, int64_t j0, int k, double* GZ) {
+// Rows row0 .. n - 1 of GZ (n x k) = G [e_j0 .. e_j0+k-1] for the GRM builder: pass 1 is
+// the direct row extraction k_unit_u, pass 2 the contraction on kBuilderLimbs digits.
+// u = W^T e_j / M has no large common offset (its entries inv_s^2 (g_sj - 2 f_s) / M
+// change sign with g_sj), so G_ij = sum_s g_is u_s - C is not a difference of large
+// terms, and 5 digits (u to 2^-38 of its column maximum) leave each entry about 1e-10
+// of itself from the FP64 value, far below its FP32 rounding.
+void geno_unit_columns(globe_ctx* c, int64_t j0, int k, double* GZ, int64_t row0) {
+    constexpr int kBuilderLimbs = 5;
     hipStream_t s = c->stream;
     const int64_t m = c->m;
     c->w_P1.ensure(sizeof(double) * (size_t)(m > 0 ? m : 1) * k);
@@ -996,11 +1005,11 @@ void geno_unit_columns(globe_ctx* c, int64_t j0, int k, double* GZ) {
     k_unit_u<<<dim3(gx, k), 256, 0, s>>>(c->d_ind.as<uint8_t>(), c->pitch_i, j0, m, c->d_twof.as<double>(),
                                          c->d_inv.as<double>(), 1.0 / (double)m, U);
     HIP_CHECK(hipGetLastError());
-    const int kg = 128 / kLimbsFP64;
+    const int kg = 128 / kBuilderLimbs;
     for (int c0 = 0; c0 < k; c0 += kg) {
         const int kk = std::min(kg, k - c0);
-        mfma_geno_product(c, nullptr, c->n, kk, GZ + (size_t)c0 * c->n, c->n, nullptr, kLimbsFP64, nullptr,
-                          U + (size_t)c0 * m);
+        mfma_geno_product(c, nullptr, c->n, kk, GZ + (size_t)c0 * c->n, c->n, nullptr, kBuilderLimbs, nullptr,
+                          U + (size_t)c0 * m, row0);
     }
 }
 

@@ -335,6 +335,27 @@ def test_grm_product_widths(k, gpu_ctx_factory):
     assert rel(g.computeVx_multi(x, multGRM_uhat=True), orc.vx(x, grm_only=True)) < 1e-12
 
 
+def test_grm_builder_at_40k_rows_offsets(gpu_ctx_factory):
+    """The builder forms only rows i >= j of each tile column, from a row offset (a
+    multiple of 4096) on; fewer rows let the contraction take more K-splits than the
+    full cohort, and its partial-sum buffer must be sized for them (N = 40,000 and
+    M = 80,000 SNPs, the AI-REML benchmark's shape, is the first size where they
+    differ). The built GRM's products and diagonal match the genotype path."""
+    _say("grm-builder-40k")
+    rng = np.random.default_rng(31)
+    c = synth.simulate_cohort(40_000, 0.5, rng)
+    g = gpu_ctx_factory()
+    g.set_cohort(c.indiv, c.times, c.X)
+    g.synth_bed(80_000, seed=31, f_lo=0.05, f_hi=0.5, miss=0.0, min_maf=0.0, max_miss=1.0)
+    g.set_varcomp((0.6, 0.4, 0.4, 0.2, 0.2))
+    x = rng.standard_normal((c.nrec, 3))
+    v_geno = g.computeVx_multi(x)
+    d_geno = g.computeGeneticDiagonal()
+    g.grm_from_geno()
+    assert rel(g.computeVx_multi(x), v_geno) < 1e-6
+    assert rel(g.computeGeneticDiagonal(), d_geno) < 1e-6
+
+
 def test_grm_builder_matches_wwt(gpu_ctx_factory):
     """G = W W^T / M built on the device from the resident genotypes equals the numpy
     product to FP32 rounding, and the context then solves from it like a loaded GRM."""
