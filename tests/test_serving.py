@@ -120,3 +120,43 @@ def test_serving_rejects_bad_requests(cohort, gpu_ctx_factory):
     with pytest.raises(G.GlobeError):  # unknown / unfinished request
         g._check(g.L.globe_serve_take(g.h, 12345, None, None))
     g.serve_close()
+
+
+def test_serving_admits_by_hbm_budget(cohort, gpu_ctx_factory):
+    """A budget that holds one request's vectors (6 record vectors of 8 bytes + the record
+    structure: 80 bytes per record) admits one at a time, FIFO, however many slots are
+    free; a request larger than the budget is refused at submission; each result equals
+    the same request served with an unlimited budget."""
+    _say("serve-budget")
+    d = cohort
+    n = d.cohort.n
+    rng = np.random.default_rng(9)
+    # whole-cohort requests of similar size: no two fit a budget of the largest one
+    reqs = [serving.make_request(rng, n, n, n, 2.0, 2.2) for _ in range(5)]
+    sizes = sorted(r[0].shape[0] for r in reqs)
+    assert sizes[0] + sizes[1] > sizes[-1]
+    cap = sizes[-1]
+    g = _device(d, gpu_ctx_factory)
+    free, _ = _serve(g, reqs, slots=4, precision=G.PREC_FP64, tol=1e-10, stagger=False, cap=cap)
+    per_rec = 80
+    g.serve_open(slots=4, max_records=cap, hbm_budget=per_rec * cap, tol=1e-10, max_iter=500,
+                 precision=G.PREC_FP64)
+    ids = [g.serve_submit(*r) for r in reqs]
+    order, out = [], {}
+    for _ in range(5000):
+        fin, active, queued = g.serve_step()
+        assert active <= 1
+        for rid in g.serve_finished():
+            x, st = g.serve_take(rid)
+            out[rid] = x
+            order.append(rid)
+        if len(out) == len(reqs):
+            break
+    assert order == ids  # first in, first admitted, first out
+    for q, rid in enumerate(ids):
+        assert rel(out[rid], free[q]) <= 1e-14, q
+    g.serve_close()
+    g.serve_open(slots=4, max_records=cap, hbm_budget=per_rec * 10, tol=1e-10)
+    with pytest.raises(G.GlobeError):  # larger than the whole budget
+        g.serve_submit(*reqs[0])
+    g.serve_close()
