@@ -160,17 +160,21 @@ void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int prec
         return;
     }
     if (precision != GLOBE_PREC_FP64_VALU) {
-        const int limbs = precision == GLOBE_PREC_MIXED ? c->limbs_mixed : kLimbsFP64;
-        if (k * limbs <= 128) {
-            mfma_geno_product(c, S0, n, k, GZ, n, skip, limbs, th);
-            return;
-        }
-        // wider blocks run in column groups that fit one contraction
-        const int kg = 128 / limbs;
-        for (int c0 = 0; c0 < k; c0 += kg) {
-            const int kk = std::min(kg, k - c0);
-            mfma_geno_product(c, S0 + (size_t)c0 * n, n, kk, GZ + (size_t)c0 * n, n, skip, limbs,
+        // digits per column: 7 for FP64; mixed: each column's own relaxed count while a
+        // solve sets them, else the uniform count
+        std::vector<int8_t> L((size_t)k);
+        const bool per_col = precision == GLOBE_PREC_MIXED && (int)c->limbs_col.size() >= k;
+        for (int q = 0; q < k; ++q)
+            L[q] = precision != GLOBE_PREC_MIXED ? (int8_t)kLimbsFP64
+                   : per_col                     ? c->limbs_col[q]
+                                                 : (int8_t)c->limbs_mixed;
+        // column groups whose digits fit one contraction (128 digit columns)
+        for (int c0 = 0; c0 < k;) {
+            int c1 = c0, J = 0;
+            while (c1 < k && J + L[c1] <= 128) J += L[c1++];
+            mfma_geno_product(c, S0 + (size_t)c0 * n, n, c1 - c0, GZ + (size_t)c0 * n, n, skip, L.data() + c0,
                               th + (size_t)c0 * 5);
+            c0 = c1;
         }
         return;
     }
@@ -252,8 +256,8 @@ void make_probes_dev(globe_ctx* c, int kind, int nmc, uint64_t seed, double* Y, 
 // PCG with the reference's control flow (PCGsolver.cpp:1324-1444 for rule 0,
 // varCompEst.cpp:453-528 for rule 1), all scalars on the device. The host enqueues
 // batches of iterations and reads back only the done flag between batches.
-// Digits of the mixed contraction for an iteration that starts at residual ratio
-// max_j |r_j| / |b_j| (inexact Krylov relaxation: Simoncini & Szyld, SIAM J. Sci.
+// Digits of the mixed contraction for a column whose iteration starts at residual ratio
+// |r_j| / |b_j| (inexact Krylov relaxation: Simoncini & Szyld, SIAM J. Sci.
 // Comput. 25 (2003); van den Eshof & Sleijpen, SIAM J. Matrix Anal. Appl. 26 (2004)).
 // An operator error of relative size eta in iteration j moves the true residual away
 // from the recursive one by about eta |r_j|, so eta_j <= 0.01 tol |b| / |r_j| keeps the
@@ -262,7 +266,7 @@ void make_probes_dev(globe_ctx* c, int kind, int nmc, uint64_t seed, double* Y, 
 // as eta(L) (the largest entry of a right-hand side is a few times its rms). Measured at
 // N = 200k, M = 500k, 32 RHS, tol 1e-6: 0.01 keeps the 11 iterations of the 4-digit
 // solve (0.1 costs a twelfth) with 3.3 digits on average.
-static int relaxed_limbs(double ratio, double tol) {
+int relaxed_limbs(double ratio, double tol) {
     const double allowed = 0.01 * tol / std::max(ratio, 1e-300);
     for (int L = 2; L < kLimbsMixed; ++L)
         if (8.0 * std::ldexp(1.0, -(8 * L - 2)) <= allowed) return L;
@@ -348,9 +352,13 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
     const bool relax = o.precision == GLOBE_PREC_MIXED && !(o.flags & GLOBE_PCG_FIXED_DIGITS);
     struct LimbsGuard {
         globe_ctx* c;
-        ~LimbsGuard() { c->limbs_mixed = c->limbs_default; }
+        ~LimbsGuard() {
+            c->limbs_mixed = c->limbs_default;
+            c->limbs_col.clear();
+        }
     } guard{c};
     c->limbs_mixed = kLimbsMixed;
+    c->limbs_col.clear();
     while (true) {
         for (int b = 0; b < batch; ++b) {
             c->prof_iter = queued + b;
@@ -363,11 +371,24 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
             launch_pcg_p(s, dst, nrec, p, z, nrec, k);
         }
         queued += batch;
-        // the scalar header: done, iters, ..., max_ratio
-        HIP_CHECK(hipMemcpyAsync(hs, dst, offsetof(PcgState, rs), hipMemcpyDeviceToHost, s));
+        // the scalar header (done, iters, ..., max_ratio), with relaxation also the
+        // per-column arrays up to the residual ratios
+        HIP_CHECK(hipMemcpyAsync(hs, dst, relax ? offsetof(PcgState, active) : offsetof(PcgState, rs),
+                                 hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
         if (hs->done || queued >= o.max_iter + 1) break;
-        if (relax) c->limbs_mixed = relaxed_limbs(hs->max_ratio, o.tol);
+        if (relax) {
+            // each column's digits from its own residual ratio (the columns are
+            // independent recurrences, so each gets the single-RHS bound)
+            c->limbs_col.assign((size_t)k, (int8_t)kLimbsMixed);
+            int lmax = 2;
+            for (int j = 0; j < k; ++j) {
+                const int L = relaxed_limbs(hs->ratio[j], o.tol);
+                c->limbs_col[j] = (int8_t)L;
+                lmax = std::max(lmax, L);
+            }
+            c->limbs_mixed = lmax;
+        }
         const auto tn = std::chrono::steady_clock::now();
         const double per_iter = std::chrono::duration<double>(tn - tb).count() / batch;
         tb = tn;

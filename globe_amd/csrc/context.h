@@ -42,10 +42,13 @@ struct globe_ctx {
     int64_t ingest_chunk = 256ll << 20;
     double ingest_stats[5] = {0, 0, 0, 0, 0};
 
-    // int8 digits of GLOBE_PREC_MIXED contractions (run_pcg lowers it per iteration), and
-    // the count used outside a PCG solve (globe_set_mixed_digits)
+    // int8 digits of GLOBE_PREC_MIXED contractions, and the count used outside a PCG
+    // solve (globe_set_mixed_digits); while a PCG solve or the serving loop runs,
+    // limbs_col holds each column's own count (relaxed on its residual), and
+    // limbs_mixed their maximum
     int limbs_mixed = 4;
     int limbs_default = 4;
+    std::vector<int8_t> limbs_col;
 
     // variance components
     std::array<double, 5> theta{0, 0, 0, 0, 0};
@@ -129,12 +132,15 @@ void ensure_gdiag(globe_ctx* c);
 void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64_t ldx,
              const globe_pcg_opts& o, globe_pcg_stats* st, double* col_ratio, int32_t* col_iters);
 void require(bool cond, int code, const std::string& msg);
+// int8 digits (2..4) a mixed-precision product may use in a PCG iteration that starts at
+// residual ratio |r| / |b| (inexact Krylov relaxation)
+int relaxed_limbs(double ratio, double tol);
 // packed-genotype contraction on the int8 matrix cores (k_mfma.hip): GZ = sig2g G S0
-// with `limbs` int8 digits per value of the right-hand side
-// (U_given: pass 1 already done by the caller, u = U_given, m x k; S0 unused; then only
-// rows row0 .. n - 1 of GZ are formed)
+// with col_limbs[c] int8 digits (1..7) for column c of the right-hand side, at most 128
+// digit columns in all (U_given: pass 1 already done by the caller, u = U_given, m x k;
+// S0 unused; then only rows row0 .. n - 1 of GZ are formed)
 void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, double* GZ,
-                       int64_t ldg, const int* skip, int limbs, const double* theta_cols,
+                       int64_t ldg, const int* skip, const int8_t* col_limbs, const double* theta_cols,
                        const double* U_given = nullptr, int64_t row0 = 0);
 // Rows i >= row0 of G [e_j0 .. e_j0+k-1] (n x k, row0 <= j0) from the resident genotypes
 // (GRM builder: the lower triangle needs rows i >= j only)

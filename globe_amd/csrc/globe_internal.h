@@ -54,6 +54,16 @@ struct PcgState {
     int active[kMaxCols], col_iters[kMaxCols];
 };
 
+// Digit layout of a k-column right-hand side in the int8 contraction: column c has L[c]
+// int8 digits (1..7) in digit columns off[c] .. off[c] + L[c] - 1; digit column j holds
+// digit dig[j] of column col[j] (col = 255: padding). Passed to kernels by value.
+struct LimbMap {
+    int8_t L[kMaxCols];
+    uint8_t off[kMaxCols];
+    uint8_t col[128];
+    uint8_t dig[128];
+};
+
 // ---------------------------------------------------------------- record space
 // Record-space structure seen by the record kernels. The records of an individual
 // are contiguous (a block); column c of a block PCG may carry its own structure (the

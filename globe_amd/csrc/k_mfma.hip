@@ -47,6 +47,8 @@
 // K-chunk of Bq in that XCD's L2. Partial sums per split go to out[split][row][J16]
 // and are added exactly (int64) in the epilogue.
 #include <algorithm>
+#include <cstring>
+#include <vector>
 #include <cstdio>
 #include <cstdlib>
 
@@ -490,9 +492,9 @@ __global__ __launch_bounds__(256) void k_colmax_parts(int64_t rows, const double
 }
 
 // One wave per column: max (want_max) or ordered sum of the column's partials; the
-// scale exponent sh_c = (8L - 2) - e_c with max|v| < 2^e_c when sh != nullptr.
+// scale exponent sh_c = (8 L_c - 2) - e_c with max|v| < 2^e_c when sh != nullptr.
 __global__ __launch_bounds__(256) void k_finish_cols(const double* __restrict__ part, int nparts,
-                                                     int k, int want_max, int limbs, int* sh,
+                                                     int k, int want_max, const LimbMap lm, int* sh,
                                                      double* out, const int* skip) {
     if (skip && *skip) return;
     const int lane = threadIdx.x & 63;
@@ -508,7 +510,7 @@ __global__ __launch_bounds__(256) void k_finish_cols(const double* __restrict__ 
             if (sh) {
                 int e = 0;
                 if (v > 0.0) (void)frexp(v, &e);  // v < 2^e
-                sh[c] = (v > 0.0 && isfinite(v)) ? (8 * limbs - 2) - e : 0;
+                sh[c] = (v > 0.0 && isfinite(v)) ? (8 * lm.L[c] - 2) - e : 0;
             }
         }
     }
@@ -518,14 +520,14 @@ __global__ __launch_bounds__(256) void k_finish_cols(const double* __restrict__ 
 // (wide = 0) or wide (wide = 1) kernel and emit per-block partials of
 // sum_kappa w_kappa q_kappa 2^-sh (T for pass 1 with w = 1, C for pass 2 with w = 2 f).
 __global__ __launch_bounds__(256) void k_quantize(const double* __restrict__ v, int64_t ldv,
-                                                  int64_t K, int64_t Kpad, int k, int limbs,
+                                                  int64_t K, int64_t Kpad, int k, const LimbMap lm,
                                                   int nfrag, int wide, const int* __restrict__ sh,
                                                   const double* __restrict__ wts,
                                                   uint8_t* __restrict__ Bq, double* part,
                                                   const int* skip) {
     if (skip && *skip) return;
     const int j = blockIdx.y;  // digit column
-    const int c = j / limbs, l = j % limbs;
+    const int c = lm.col[j], l = lm.dig[j];
     const bool live = c < k;
     const int fw = wide ? 32 : 16;
     const int jf = j / fw, jj = j % fw;
@@ -589,7 +591,7 @@ constexpr int kEpiRows = 64;
 
 __device__ inline double combine(const int32_t* __restrict__ out, int nsplit, int64_t split_stride,
                                  int64_t base, int limbs) {
-    if (limbs == 4) {
+    if (limbs == 4 && (base & 3) == 0) {
         // the 4 digit sums of a column are 16-byte aligned (J16 and the split stride are
         // multiples of 16): one vector load per split, same combination order
         long long s[4] = {0, 0, 0, 0};
@@ -617,7 +619,7 @@ __device__ inline double combine(const int32_t* __restrict__ out, int nsplit, in
 // pass 1 (mode 0): u = sig2g / M * inv^2 * (P - 2 f T), per-block colmax of u
 // pass 2 (mode 1): GZ = P - C
 __global__ __launch_bounds__(256) void k_epilogue(const int32_t* __restrict__ out, int nsplit,
-                                                  int64_t rows, int J16, int limbs, int k,
+                                                  int64_t rows, int J16, const LimbMap lm, int k,
                                                   const int* __restrict__ sh, int mode,
                                                   const double* __restrict__ TC,
                                                   const double* __restrict__ twof,
@@ -640,7 +642,7 @@ __global__ __launch_bounds__(256) void k_epilogue(const int32_t* __restrict__ ou
             double val = 0.0;
             if (r < rows) {
                 const double P =
-                    ldexp(combine(out, nsplit, split_stride, r * J16 + c * limbs, limbs), -sh[c]);
+                    ldexp(combine(out, nsplit, split_stride, r * J16 + lm.off[c], lm.L[c]), -sh[c]);
                 if (mode == 0) {
                     const double iv = inv[r];
                     val = th[c * 5 + 1] * inv_m * iv * iv * (P - twof[r] * TC[c]);
@@ -851,14 +853,29 @@ static std::vector<Variant> candidates(bool wide, int JT, int64_t nrows, int64_t
     return v;
 }
 
-// GZ (n x k) = sig2g G S0 for the packed-genotype cohort with `limbs` int8 digits.
+// GZ (n x k) = sig2g G S0 for the packed-genotype cohort, column c of S0 on col_limbs[c]
+// int8 digits.
 void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, double* GZ, int64_t ldg,
-                       const int* skip, int limbs, const double* theta_cols, const double* U_given,
-                       int64_t row0) {
+                       const int* skip, const int8_t* col_limbs, const double* theta_cols,
+                       const double* U_given, int64_t row0) {
     hipStream_t s = c->stream;
     const int64_t n = c->n, m = c->m;
-    const int J = k * limbs;
-    require(J <= 128, GLOBE_E_INVALID, "k * limbs must be <= 128 for the MFMA contraction");
+    require(k >= 1 && k <= kMaxCols, GLOBE_E_INVALID, "1 to 64 columns");
+    LimbMap lm;
+    std::memset(&lm, 0, sizeof(lm));
+    std::memset(lm.col, 255, sizeof(lm.col));
+    int J = 0;
+    for (int q = 0; q < k; ++q) {
+        require(col_limbs[q] >= 1 && col_limbs[q] <= kLimbsFP64, GLOBE_E_INVALID, "1 to 7 digits per column");
+        require(J + col_limbs[q] <= 128, GLOBE_E_INVALID, "at most 128 digit columns per MFMA contraction");
+        lm.L[q] = col_limbs[q];
+        lm.off[q] = (uint8_t)J;
+        for (int l = 0; l < col_limbs[q]; ++l) {
+            lm.col[J + l] = (uint8_t)q;
+            lm.dig[J + l] = (uint8_t)l;
+        }
+        J += col_limbs[q];
+    }
     const bool wide = J > 16;
     const int JT = wide ? (J + 31) / 32 : 1;  // column fragments of 32 digit columns
     const int J16 = wide ? 32 * JT : 16;
@@ -958,29 +975,29 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     // pass 1: P1[s][c] = sum_i g_is S0[c][i] on the SNP-major rows
     const int npn = parts_for(n);
     k_colmax_parts<<<dim3(npn, k), 256, 0, s>>>(n, S0, ldn, part, skip);
-    k_finish_cols<<<1, 256, 0, s>>>(part, npn, k, 1, limbs, sh1, nullptr, skip);
+    k_finish_cols<<<1, 256, 0, s>>>(part, npn, k, 1, lm, sh1, nullptr, skip);
     const int qg1 = parts_for(Kpad1);
-    k_quantize<<<dim3(qg1, J16), 256, 0, s>>>(S0, ldn, n, Kpad1, k, limbs, nfrag, wide, sh1, nullptr,
+    k_quantize<<<dim3(qg1, J16), 256, 0, s>>>(S0, ldn, n, Kpad1, k, lm, nfrag, wide, sh1, nullptr,
                                               Bq, part, skip);
-    k_finish_cols<<<1, 256, 0, s>>>(part, qg1, k, 0, limbs, nullptr, c->w_T.as<double>(), skip);
+    k_finish_cols<<<1, 256, 0, s>>>(part, qg1, k, 0, lm, nullptr, c->w_T.as<double>(), skip);
     HIP_CHECK(hipGetLastError());
     const int sp1 = contract(0, c->d_snp.as<uint8_t>(), c->pitch_s, m, n, Kpad1, nss1);
     eb1 = std::min<int64_t>((m + kEpiRows - 1) / kEpiRows, kMaxParts);
-    k_epilogue<<<(unsigned)eb1, 256, 0, s>>>(out, sp1, m, J16, limbs, k, sh1, 0, c->w_T.as<double>(),
+    k_epilogue<<<(unsigned)eb1, 256, 0, s>>>(out, sp1, m, J16, lm, k, sh1, 0, c->w_T.as<double>(),
                                              c->d_twof.as<double>(), c->d_inv.as<double>(), th,
                                              1.0 / (double)m, U, m, part, skip);
     HIP_CHECK(hipGetLastError());
     }
     // pass 2: GZ[c][i] = sum_s g_is u[c][s] - C[c] on the individual-major rows
-    k_finish_cols<<<1, 256, 0, s>>>(part, (int)eb1, k, 1, limbs, sh2, nullptr, skip);
+    k_finish_cols<<<1, 256, 0, s>>>(part, (int)eb1, k, 1, lm, sh2, nullptr, skip);
     const int qg2 = parts_for(Kpad2);
-    k_quantize<<<dim3(qg2, J16), 256, 0, s>>>(U, m, m, Kpad2, k, limbs, nfrag, wide, sh2,
+    k_quantize<<<dim3(qg2, J16), 256, 0, s>>>(U, m, m, Kpad2, k, lm, nfrag, wide, sh2,
                                               c->d_twof.as<double>(), Bq, part, skip);
-    k_finish_cols<<<1, 256, 0, s>>>(part, qg2, k, 0, limbs, nullptr, c->w_C.as<double>(), skip);
+    k_finish_cols<<<1, 256, 0, s>>>(part, qg2, k, 0, lm, nullptr, c->w_C.as<double>(), skip);
     HIP_CHECK(hipGetLastError());
     const int sp2 = contract(1, c->d_ind.as<uint8_t>() + r0 * c->pitch_i, c->pitch_i, n2, m, Kpad2, nss2);
     const int64_t eb2 = std::min<int64_t>((n2 + kEpiRows - 1) / kEpiRows, kMaxParts);
-    k_epilogue<<<(unsigned)eb2, 256, 0, s>>>(out, sp2, n2, J16, limbs, k, sh2, 1, c->w_C.as<double>(),
+    k_epilogue<<<(unsigned)eb2, 256, 0, s>>>(out, sp2, n2, J16, lm, k, sh2, 1, c->w_C.as<double>(),
                                              nullptr, nullptr, nullptr, 0.0, GZ + r0, ldg, nullptr, skip);
     HIP_CHECK(hipGetLastError());
 }
@@ -1008,7 +1025,8 @@ void geno_unit_columns(globe_ctx* c, int64_t j0, int k, double* GZ, int64_t row0
     const int kg = 128 / kBuilderLimbs;
     for (int c0 = 0; c0 < k; c0 += kg) {
         const int kk = std::min(kg, k - c0);
-        mfma_geno_product(c, nullptr, c->n, kk, GZ + (size_t)c0 * c->n, c->n, nullptr, kBuilderLimbs, nullptr,
+        const std::vector<int8_t> L((size_t)kk, (int8_t)kBuilderLimbs);
+        mfma_geno_product(c, nullptr, c->n, kk, GZ + (size_t)c0 * c->n, c->n, nullptr, L.data(), nullptr,
                           U + (size_t)c0 * m, row0);
     }
 }

@@ -338,6 +338,27 @@ int globe_serve_step(globe_ctx* c, int32_t* finished, int32_t* active, int32_t* 
         if (S.busy[j]) k = j + 1;
     int fin = 0;
     if (k > 0) {
+        // mixed precision: each slot's digits from its own residual ratio (4 in its first
+        // iteration), one digit column of zeros for an empty slot
+        struct LimbsGuard {
+            globe_ctx* c;
+            ~LimbsGuard() {
+                c->limbs_col.clear();
+                c->limbs_mixed = c->limbs_default;
+            }
+        } guard{c};
+        if (S.o.precision == GLOBE_PREC_MIXED) {
+            c->limbs_col.assign((size_t)k, (int8_t)1);
+            int lmax = 2;
+            for (int j = 0; j < k; ++j) {
+                if (!S.busy[j]) continue;
+                const int L = S.global_iter == S.admit_iter[j] ? kLimbsMixed
+                                                               : relaxed_limbs(S.h_state->ratio[j], S.o.tol);
+                c->limbs_col[j] = (int8_t)L;
+                lmax = std::max(lmax, L);
+            }
+            c->limbs_mixed = lmax;
+        }
         iterate(c, S, k);
         HIP_CHECK(hipMemcpyAsync(S.h_state, S.state.p, sizeof(PcgState), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
