@@ -120,10 +120,13 @@ struct ThinCfg {
     static_assert(kDigKB < W || kDigKB % W == 0, "digit chunks must spread evenly");
 };
 
+// Digit chunk lane l holds digit column l % 16; only the first `live` columns carry
+// digits, so the other lanes skip their copy: their LDS bytes keep whatever the slot
+// held, which only reaches output columns that no epilogue reads.
 template <int W, int RT, int SS>
 __device__ inline void thin_issue(const uint8_t* __restrict__ arow, int64_t frag_stride,
                                   const uint8_t* __restrict__ bstage, uint8_t* __restrict__ slot,
-                                  int wave, int lane) {
+                                  int wave, int lane, int live) {
     using C = ThinCfg<W, RT, SS>;
 #pragma unroll
     for (int q = 0; q < RT; ++q)
@@ -133,15 +136,17 @@ __device__ inline void thin_issue(const uint8_t* __restrict__ arow, int64_t frag
             __builtin_amdgcn_global_load_lds((const void*)(arow + q * frag_stride + 64 * h),
                                              (lds_void*)(slot + f * 1024), 16, 0, 0);
         }
+    const bool dig = (lane & 15) < live;  // live >= 1: every 16-lane group has an active lane
     if constexpr (C::kDigKB >= W) {
 #pragma unroll
         for (int d = 0; d < C::kDigPerWave; ++d) {
             const int ch = wave * C::kDigPerWave + d;
-            __builtin_amdgcn_global_load_lds((const void*)(bstage + ch * 1024 + lane * 16),
-                                             (lds_void*)(slot + C::kA + ch * 1024), 16, 0, 0);
+            if (dig)
+                __builtin_amdgcn_global_load_lds((const void*)(bstage + ch * 1024 + lane * 16),
+                                                 (lds_void*)(slot + C::kA + ch * 1024), 16, 0, 0);
         }
     } else {
-        if (wave < C::kDigKB)
+        if (wave < C::kDigKB && dig)
             __builtin_amdgcn_global_load_lds((const void*)(bstage + wave * 1024 + lane * 16),
                                              (lds_void*)(slot + C::kA + wave * 1024), 16, 0, 0);
     }
@@ -199,7 +204,8 @@ template <int W, int RT, int NS, int SS>
 __global__ __launch_bounds__(W * 64) void k_geno_thin(const uint8_t* __restrict__ rows, int64_t pitch,
                                                       int64_t nrows, int64_t n_st, int64_t U, int64_t G,
                                                       int P, const uint8_t* __restrict__ Bq,
-                                                      int32_t* __restrict__ out, const int* skip) {
+                                                      int32_t* __restrict__ out, int live,
+                                                      const int* skip) {
     if (skip && *skip) return;
     using C = ThinCfg<W, RT, SS>;
     __shared__ __attribute__((aligned(1024))) uint8_t lds[NS * C::kSlot];
@@ -219,7 +225,7 @@ __global__ __launch_bounds__(W * 64) void k_geno_thin(const uint8_t* __restrict_
     int64_t ir = lo / n_st, it = lo % n_st, issued = 0;
     auto issue_next = [&](int slot) {
         thin_issue<W, RT, SS>(rows + ir * kRB * pitch + lane_off + kRowStage * it, fstride,
-                              Bq + it * C::kB, lds + slot * C::kSlot, wave, lane);
+                              Bq + it * C::kB, lds + slot * C::kSlot, wave, lane, live);
         if (issued + 1 < nt) {
             ++issued;
             if (++it == n_st) {
@@ -706,11 +712,11 @@ constexpr ThinShape kThin[kNumThin] = {{4, 4, 3, 1},  {8, 2, 3, 2},  {8, 2, 4, 2
 
 template <int Q>
 void launch_thin_q(hipStream_t s, const Variant& v, const uint8_t* rows, int64_t pitch, int64_t nrows,
-                   int64_t nss, int64_t R, const uint8_t* Bq, int32_t* out, const int* skip) {
+                   int64_t nss, int64_t R, const uint8_t* Bq, int32_t* out, int live, const int* skip) {
     constexpr ThinShape t = kThin[Q];
     const int64_t nst = nss / t.SS;
     k_geno_thin<t.W, t.RT, t.NS, t.SS><<<(unsigned)v.G, t.W * 64, 0, s>>>(rows, pitch, nrows, nst, R * nst,
-                                                                         v.G, v.S, Bq, out, skip);
+                                                                         v.G, v.S, Bq, out, live, skip);
 }
 
 template <int Q>
@@ -782,7 +788,7 @@ void launch_wide(hipStream_t s, const Variant& v, unsigned grid, const uint8_t* 
 }
 
 void launch_variant(hipStream_t s, bool wide, int JT, const Variant& v, const uint8_t* rows,
-                    int64_t pitch, int64_t nrows, int64_t nss, const uint8_t* Bq, int32_t* out,
+                    int64_t pitch, int64_t nrows, int64_t nss, const uint8_t* Bq, int32_t* out, int live,
                     const int* skip) {
     const int64_t R = (nrows + rows_per_block(wide, v.cfg) - 1) / rows_per_block(wide, v.cfg);
     const int64_t per = (nss + v.S - 1) / v.S;
@@ -791,14 +797,14 @@ void launch_variant(hipStream_t s, bool wide, int JT, const Variant& v, const ui
         require(nss % kThin[v.cfg].SS == 0 && v.G >= 1 && v.G <= R * (nss / kThin[v.cfg].SS),
                 GLOBE_E_INVALID, "thin shape needs whole stages per row and at most one block per unit");
         switch (v.cfg) {
-            case 0: launch_thin_q<0>(s, v, rows, pitch, nrows, nss, R, Bq, out, skip); break;
-            case 1: launch_thin_q<1>(s, v, rows, pitch, nrows, nss, R, Bq, out, skip); break;
-            case 2: launch_thin_q<2>(s, v, rows, pitch, nrows, nss, R, Bq, out, skip); break;
-            case 3: launch_thin_q<3>(s, v, rows, pitch, nrows, nss, R, Bq, out, skip); break;
-            case 4: launch_thin_q<4>(s, v, rows, pitch, nrows, nss, R, Bq, out, skip); break;
-            case 5: launch_thin_q<5>(s, v, rows, pitch, nrows, nss, R, Bq, out, skip); break;
-            case 6: launch_thin_q<6>(s, v, rows, pitch, nrows, nss, R, Bq, out, skip); break;
-            default: launch_thin_q<7>(s, v, rows, pitch, nrows, nss, R, Bq, out, skip); break;
+            case 0: launch_thin_q<0>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip); break;
+            case 1: launch_thin_q<1>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip); break;
+            case 2: launch_thin_q<2>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip); break;
+            case 3: launch_thin_q<3>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip); break;
+            case 4: launch_thin_q<4>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip); break;
+            case 5: launch_thin_q<5>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip); break;
+            case 6: launch_thin_q<6>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip); break;
+            default: launch_thin_q<7>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip); break;
         }
     } else if (JT == 1) {
         launch_wide<1>(s, v, grid, rows, pitch, nrows, nss, per, R, Bq, out, skip);
@@ -927,7 +933,7 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
             int bi = 0;
             for (size_t q = 0; q < cand.size(); ++q) {
                 HIP_CHECK(hipEventRecord(e0, s));
-                launch_variant(s, wide, JT, cand[q], rows, pitch, nrows, nss, Bq, out, nullptr);
+                launch_variant(s, wide, JT, cand[q], rows, pitch, nrows, nss, Bq, out, J, nullptr);
                 HIP_CHECK(hipEventRecord(e1, s));
                 HIP_CHECK(hipEventSynchronize(e1));
                 float ms = 0.f;
@@ -955,11 +961,11 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
             e.flops = flops;
             e.issued = 2.0 * (double)nrows * (double)Kpad * J16;
             HIP_CHECK(hipEventRecord(e.a, s));
-            launch_variant(s, wide, JT, v, rows, pitch, nrows, nss, Bq, out, skip);
+            launch_variant(s, wide, JT, v, rows, pitch, nrows, nss, Bq, out, J, skip);
             HIP_CHECK(hipEventRecord(e.b, s));
             c->prof.push_back(e);
         } else {
-            launch_variant(s, wide, JT, v, rows, pitch, nrows, nss, Bq, out, skip);
+            launch_variant(s, wide, JT, v, rows, pitch, nrows, nss, Bq, out, J, skip);
         }
         return v.S;
     };
