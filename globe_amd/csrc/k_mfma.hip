@@ -919,8 +919,10 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     const double flops = 2.0 * (double)n * (double)m * k;
 
     // run one pass with the tuned variant (tuning it first if this shape is new)
+    // epi(S, skip) launches the pass's epilogue over S split partials: the tuner times
+    // each candidate together with it, since more splits mean more partials to combine
     auto contract = [&](int pass, const uint8_t* rows, int64_t pitch, int64_t nrows, int64_t K,
-                        int64_t Kpad, int64_t nss) -> int {
+                        int64_t Kpad, int64_t nss, const auto& epi) -> int {
         const std::string key = std::to_string(pass) + ":" + std::to_string(nrows) + ":" +
                                 std::to_string(Kpad) + ":" + std::to_string(J16);
         auto it = c->tuned.find(key);
@@ -934,6 +936,7 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
             for (size_t q = 0; q < cand.size(); ++q) {
                 HIP_CHECK(hipEventRecord(e0, s));
                 launch_variant(s, wide, JT, cand[q], rows, pitch, nrows, nss, Bq, out, J, nullptr);
+                epi(cand[q].S, nullptr);
                 HIP_CHECK(hipEventRecord(e1, s));
                 HIP_CHECK(hipEventSynchronize(e1));
                 float ms = 0.f;
@@ -987,12 +990,14 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
                                               Bq, part, skip);
     k_finish_cols<<<1, 256, 0, s>>>(part, qg1, k, 0, lm, nullptr, c->w_T.as<double>(), skip);
     HIP_CHECK(hipGetLastError());
-    const int sp1 = contract(0, c->d_snp.as<uint8_t>(), c->pitch_s, m, n, Kpad1, nss1);
     eb1 = std::min<int64_t>((m + kEpiRows - 1) / kEpiRows, kMaxParts);
-    k_epilogue<<<(unsigned)eb1, 256, 0, s>>>(out, sp1, m, J16, lm, k, sh1, 0, c->w_T.as<double>(),
-                                             c->d_twof.as<double>(), c->d_inv.as<double>(), th,
-                                             1.0 / (double)m, U, m, part, skip);
-    HIP_CHECK(hipGetLastError());
+    auto epi1 = [&](int S, const int* sk) {
+        k_epilogue<<<(unsigned)eb1, 256, 0, s>>>(out, S, m, J16, lm, k, sh1, 0, c->w_T.as<double>(),
+                                                 c->d_twof.as<double>(), c->d_inv.as<double>(), th,
+                                                 1.0 / (double)m, U, m, part, sk);
+        HIP_CHECK(hipGetLastError());
+    };
+    epi1(contract(0, c->d_snp.as<uint8_t>(), c->pitch_s, m, n, Kpad1, nss1, epi1), skip);
     }
     // pass 2: GZ[c][i] = sum_s g_is u[c][s] - C[c] on the individual-major rows
     k_finish_cols<<<1, 256, 0, s>>>(part, (int)eb1, k, 1, lm, sh2, nullptr, skip);
@@ -1001,11 +1006,13 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
                                               c->d_twof.as<double>(), Bq, part, skip);
     k_finish_cols<<<1, 256, 0, s>>>(part, qg2, k, 0, lm, nullptr, c->w_C.as<double>(), skip);
     HIP_CHECK(hipGetLastError());
-    const int sp2 = contract(1, c->d_ind.as<uint8_t>() + r0 * c->pitch_i, c->pitch_i, n2, m, Kpad2, nss2);
     const int64_t eb2 = std::min<int64_t>((n2 + kEpiRows - 1) / kEpiRows, kMaxParts);
-    k_epilogue<<<(unsigned)eb2, 256, 0, s>>>(out, sp2, n2, J16, lm, k, sh2, 1, c->w_C.as<double>(),
-                                             nullptr, nullptr, nullptr, 0.0, GZ + r0, ldg, nullptr, skip);
-    HIP_CHECK(hipGetLastError());
+    auto epi2 = [&](int S, const int* sk) {
+        k_epilogue<<<(unsigned)eb2, 256, 0, s>>>(out, S, n2, J16, lm, k, sh2, 1, c->w_C.as<double>(), nullptr,
+                                                 nullptr, nullptr, 0.0, GZ + r0, ldg, nullptr, sk);
+        HIP_CHECK(hipGetLastError());
+    };
+    epi2(contract(1, c->d_ind.as<uint8_t>() + r0 * c->pitch_i, c->pitch_i, n2, m, Kpad2, nss2, epi2), skip);
 }
 
 }  // namespace globe
