@@ -479,3 +479,96 @@ def grm_mixed_product(lt, n, S0, digits, sig2g=1.0):
         total = total + part
     total = total + gdiag[:, None] * S0
     return sig2g * total
+
+
+def aireml_fit_replay(orc, y, theta0, probes_of, step_fn, *, max_iter=50, tol=1e-3, pcg_tol=1e-10):
+    """The full-model AI-REML loop (main :1347-1491 and oneIter_AIREML :1148-1344 of
+    varCompEst.cpp), replayed on the oracle's FP64 evaluations (orc_aireml_eval):
+    probes_of(e) gives (probes, weights) of the e-th gradient evaluation (the device fit
+    numbers its evaluations the same way), step_fn(grad, ai, theta, delta) the
+    constrained step. Control flow as the reference: step halving on box / covariance
+    violations, the gradient-growth check, the rho test and trust-region update, the
+    AI matrix at an accepted point, the relative-change stop and the radius stop.
+    Restarts after > 5 rejections draw random points and are not replayed (an error).
+    Returns theta, the theta after every iteration, and the iteration count."""
+    th = np.array(theta0, dtype=np.float64)
+    nv = 5
+    eval_id = [0]
+
+    def evaluate(t):
+        orc.set_theta(tuple(t))
+        Y, W = probes_of(eval_id[0])
+        eval_id[0] += 1
+        e = orc.aireml_eval(y, Y, W, tol=pcg_tol)
+        return e["grad"], e["ai"]
+
+    delta_tr, dangerous, num_rej, gnt = 1e20, False, 0, 0.0
+    traj, it_done = [], 0
+    g = A = None
+    for it in range(max_iter):
+        if it == 0:
+            g, A = evaluate(th)
+        if it == 1:
+            gnt = 0.1 * np.linalg.norm(g)
+        if num_rej > 5:
+            raise RuntimeError("restart not replayed")
+        th_old = th.copy()
+        p = step_fn(g, A, th, delta_tr)
+        if p is None:
+            delta_tr *= 0.75
+            dangerous = True
+            num_rej += 1
+        else:
+            scale, accepted, converged, red = 1.0, False, False, 0
+            while not accepted and red <= 10:
+                sp = scale * p
+                tn = th + sp
+                violated = False
+                for j in range(nv):
+                    if j != 3 and tn[j] < 1e-6:
+                        violated = True
+                    if tn[j] > 10.0:
+                        violated = True
+                if tn[3] > np.sqrt(max(tn[2], 0.0) * max(tn[4], 0.0)):
+                    violated = True
+                if violated:
+                    scale *= 0.5
+                    red += 1
+                    continue
+                gn, _ = evaluate(tn)
+                dll = sp @ (0.5 * (g + gn))
+                pred = g @ sp - 0.5 * sp @ A @ sp
+                if scale == 1.0 and pred < 1e-2:
+                    dangerous, converged = False, True
+                    break
+                if np.linalg.norm(g) > gnt and np.linalg.norm(gn) > 2 * np.linalg.norm(g):
+                    scale *= 0.5
+                    red += 1
+                    continue
+                rho = dll / pred
+                npd = float(np.sum(np.abs(np.diag(A) * sp)))
+                if rho < 0.25:
+                    delta_tr = 0.25 * npd
+                elif rho > 0.99:
+                    delta_tr = max(delta_tr, 3.5 * npd)
+                if rho > 0.25:
+                    th, g = tn, gn
+                    # AI at the accepted point (computeDerivative_AImat, :1321-1330)
+                    orc.set_theta(tuple(th))
+                    A = orc.aireml_eval(y, np.zeros((len(y), 0)), np.zeros((len(y), 0)), tol=pcg_tol)["ai"]
+                    num_rej, accepted = 0, True
+                else:
+                    scale *= 0.5
+                    red += 1
+            if not accepted and not converged:
+                delta_tr *= 0.75
+                dangerous = True
+                num_rej += 1
+        traj.append(th.copy())
+        it_done = it + 1
+        if np.linalg.norm(th - th_old) / np.linalg.norm(th_old) < tol and not dangerous:
+            break
+        dangerous = False
+        if delta_tr < 1e-4:
+            break
+    return th, np.array(traj), it_done
