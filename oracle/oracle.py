@@ -484,8 +484,9 @@ def grm_mixed_product(lt, n, S0, digits, sig2g=1.0):
 def aireml_fit_replay(orc, y, theta0, probes_of, step_fn, *, max_iter=50, tol=1e-3, pcg_tol=1e-10):
     """The full-model AI-REML loop (main :1347-1491 and oneIter_AIREML :1148-1344 of
     varCompEst.cpp), replayed on the oracle's FP64 evaluations (orc_aireml_eval):
-    probes_of(e) gives (probes, weights) of the e-th gradient evaluation (the device fit
-    numbers its evaluations the same way), step_fn(grad, ai, theta, delta) the
+    probes_of(e, theta) gives (probes, weights, PCG start or None) of the e-th gradient
+    evaluation, at theta (the device fit numbers its evaluations the same way),
+    step_fn(grad, ai, theta, delta) the
     constrained step. Control flow as the reference: step halving on box / covariance
     violations, the gradient-growth check, the rho test and trust-region update, the
     AI matrix at an accepted point, the relative-change stop and the radius stop.
@@ -497,9 +498,9 @@ def aireml_fit_replay(orc, y, theta0, probes_of, step_fn, *, max_iter=50, tol=1e
 
     def evaluate(t):
         orc.set_theta(tuple(t))
-        Y, W = probes_of(eval_id[0])
+        Y, W, X0 = probes_of(eval_id[0], tuple(t))
         eval_id[0] += 1
-        e = orc.aireml_eval(y, Y, W, tol=pcg_tol)
+        e = orc.aireml_eval(y, Y, W, X0, tol=pcg_tol)
         return e["grad"], e["ai"]
 
     delta_tr, dangerous, num_rej, gnt = 1e20, False, 0, 0.0

@@ -73,7 +73,7 @@ def test_replay_reaches_dense_reml_optimum():
     c, lt, G, y = make_case(n=120, lam=2.0, m=3000, seed=43, theta=(0.5, 1.0, 0.4, 0.1, 0.2))
     orc = O.Oracle(c.indiv, c.times, c.X, (0.5, 0.5, 0.3, 0.0, 0.3), grm_lt=lt, n=c.n)
     unit = np.sqrt(c.nrec) * np.eye(c.nrec)
-    th, traj, its = O.aireml_fit_replay(orc, y, (0.5, 0.5, 0.3, 0.0, 0.3), lambda e: (unit, unit), step_fn,
+    th, traj, its = O.aireml_fit_replay(orc, y, (0.5, 0.5, 0.3, 0.0, 0.3), lambda e, t: (unit, unit, None), step_fn,
                                         max_iter=60, tol=1e-7, pcg_tol=1e-12)
     cons = [{"type": "ineq", "fun": lambda t: t[2] * t[4] - t[3] ** 2}]
     r = minimize(lambda t: -reml_loglik(t, c, G, y), th, method="SLSQP",
@@ -92,8 +92,11 @@ def test_replay_reaches_dense_reml_optimum():
 
 # ------------------------------------------------------------------------------ GPU
 @pytest.mark.gpu
-def test_device_fit_equals_replay_with_same_probes(gpu_ctx_factory):
-    _say("aireml-fit-replay")
+@pytest.mark.parametrize("kind", ["rademacher", "gaussian"])
+def test_device_fit_equals_replay_with_same_probes(kind, gpu_ctx_factory):
+    """Rademacher probes, and the reference's Gaussian tilde-V probes (drawn at each
+    evaluation's components, weights and PCG start tilde-V^-1 probe)."""
+    _say("aireml-fit-replay", kind)
     c, lt, G, y = make_case(n=300, lam=2.0, m=6000, seed=47, theta=(0.5, 1.0, 0.4, 0.1, 0.2))
     th0 = (0.5, 0.5, 0.3, 0.0, 0.3)
     seed, nmc = 20261203, 30
@@ -101,13 +104,16 @@ def test_device_fit_equals_replay_with_same_probes(gpu_ctx_factory):
     g.set_cohort(c.indiv, c.times, c.X)
     g.readFullGRM(lt)
     g.set_varcomp(th0)
-    fit = g.aireml_fit(y, th0, max_iter=10, tol=1e-3, num_mc=nmc, probes="rademacher", seed=seed,
-                       pcg_tol=1e-10)
+    fit = g.aireml_fit(y, th0, max_iter=10, tol=1e-3, num_mc=nmc, probes=kind, seed=seed, pcg_tol=1e-10)
     orc = O.Oracle(c.indiv, c.times, c.X, th0, grm_lt=lt, n=c.n)
 
-    def probes_of(e):
-        Y = O.probes(1, c.indiv, c.times, th0, nmc, probe_seed(seed, e))
-        return Y, Y
+    def probes_of(e, t):
+        if kind == "rademacher":
+            Y = O.probes(1, c.indiv, c.times, t, nmc, probe_seed(seed, e))
+            return Y, Y, None
+        Y = O.probes(0, c.indiv, c.times, t, nmc, probe_seed(seed, e))
+        W = orc.tildeV_solve(Y)  # orc is at t while evaluate() draws
+        return Y, W, W
 
     th, traj, its = O.aireml_fit_replay(orc, y, th0, probes_of, step_fn, max_iter=10, tol=1e-3, pcg_tol=1e-10)
     assert fit["iters"] == its
