@@ -118,3 +118,26 @@ def test_device_fit_equals_replay_with_same_probes(kind, gpu_ctx_factory):
     th, traj, its = O.aireml_fit_replay(orc, y, th0, probes_of, step_fn, max_iter=10, tol=1e-3, pcg_tol=1e-10)
     assert fit["iters"] == its
     assert np.max(np.abs(fit["theta"] - th) / np.abs(th)) < 1e-6, (fit["theta"], th)
+
+
+@pytest.mark.gpu
+def test_mixed_fit_within_1e4_of_fp64_on_genotypes(gpu_ctx_factory):
+    """Packed-genotype cohort: the mixed-precision fit (int8 digits relaxed per column)
+    and the FP64 fit from the same start and probes agree to 1e-4 on the variance
+    components."""
+    _say("aireml-fit-mixed-geno")
+    d = synth.make_geno_case(n=400, m=2000, lam=2, seed=49, miss=0.01)
+    c = d.cohort
+    th0 = (0.5, 0.5, 0.3, 0.0, 0.3)
+    out = {}
+    for prec in (globe_amd.PREC_FP64, globe_amd.PREC_MIXED):
+        g = gpu_ctx_factory()
+        g.set_cohort(c.indiv, c.times, c.X)
+        g.prepGenotypeInput(d.bed_rows, 0.01, 0.2)
+        g.set_varcomp(th0)
+        out[prec] = g.aireml_fit(d.y, th0, max_iter=6, tol=1e-3, num_mc=15, probes="rademacher", seed=7,
+                                 pcg_tol=1e-8, precision=prec)
+    t64, tm = out[globe_amd.PREC_FP64]["theta"], out[globe_amd.PREC_MIXED]["theta"]
+    assert out[globe_amd.PREC_FP64]["iters"] == out[globe_amd.PREC_MIXED]["iters"]
+    # relative, with components near 0 (the covariance sigb01) measured against 0.01
+    assert np.max(np.abs(tm - t64) / np.maximum(np.abs(t64), 1e-2)) < 1e-4, (t64, tm)
