@@ -140,7 +140,7 @@ void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int prec
             e.iter = c->prof_iter;
             e.bytes = 4.0 * (double)n * (n + 1) / 2;
             e.flops = 2.0 * (double)n * n * k;
-            e.issued = mixed ? 2.0 * np * np * cols * L * (L + 1) / 2 : 2.0 * np * np * cols;
+            e.issued = mixed ? 2.0 * np * np * cols * (L * (L + 1) / 2 + L - 1) : 2.0 * np * np * cols;
             HIP_CHECK(hipEventRecord(e.a, s));
         }
         if (mixed)

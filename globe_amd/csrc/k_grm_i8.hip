@@ -9,12 +9,15 @@
 // the entry is within 2^-6 of the row maximum; the diagonal, far larger than the rest of
 // its row, stays out of the digits and is applied in FP64 (G_ii S0_i). S0 gets the same
 // form per column (scale 2^e_c >= max_j |S0_jc|). With L digits of each, the product
-// keeps the digit pairs a + b < L, grouped by weight class w = a + b, each class
-// accumulated exactly in int32 over a chunk of at most 16384 columns of G
-// (|sum| <= 4 x 128 x 128 x 16384 = 2^30), then scaled by 2^(e_i + e_c - 12 - 8w) in
-// FP64. Truncating both operands to L digits leaves a relative error of about 2^(1 - 8L)
-// of the row and column scales; the PCG lowers L as its residual falls (inexact Krylov
-// relaxation), and L = 4 matches the FP64 product to about 1e-9.
+// keeps the digit pairs a, b < L with a + b <= L, grouped by weight class w = a + b,
+// each class accumulated exactly in int32 over a chunk of at most 16384 columns of G
+// (at most 4 pairs per class: |sum| <= 4 x 128 x 128 x 16384 = 2^30), then scaled by
+// 2^(e_i + e_c - 12 - 8w) in FP64. The pairs left out (a + b > L) weigh at most about
+// 2^(-8L - 4) of the scales each, below the truncation of each operand to L digits
+// (2^(1 - 8L)); keeping only a + b < L would leave the class w = L, about 2^(4 - 8L).
+// The PCG lowers L as its residual falls (inexact Krylov relaxation); L = 4 is within
+// about 1e-9 of the FP64 product. The matrix cores do L(L+1)/2 + L - 1 int8 products
+// per tile and column block, far from bounding the pass.
 //
 // Layout: digit tiles of the FULL square (both triangles, so no tile is read
 // transposed): tile (I, J) at (I ntiles + J) x 16 KB, four 4 KB planes (digit a, then
@@ -216,9 +219,9 @@ __global__ __launch_bounds__(256 * CG) void k_grm_i8(const int8_t* __restrict__ 
     const int rg = w & 3, cg = w >> 2;
     const int J0 = (int)((int64_t)h * ntiles / H), J1 = (int)((int64_t)(h + 1) * ntiles / H);
     const int nt = J1 - J0;
-    v4i acc[L];
+    v4i acc[L + 1];  // weight classes w = a + b = 0 .. L
 #pragma unroll
-    for (int q = 0; q < L; ++q) acc[q] = v4i{0, 0, 0, 0};
+    for (int q = 0; q <= L; ++q) acc[q] = v4i{0, 0, 0, 0};
     if (nt > 0) {
         // stage t: tile (I, J0 + t) and slab J0 + t (the last one repeated past the end,
         // into slots never read); copy p of a stage is 1 KB, waves take p = w, w + waves,
@@ -262,7 +265,7 @@ __global__ __launch_bounds__(256 * CG) void k_grm_i8(const int8_t* __restrict__ 
 #pragma unroll
             for (int a = 0; a < L; ++a)
 #pragma unroll
-                for (int b = 0; a + b < L; ++b)
+                for (int b = 0; b < L && a + b <= L; ++b)
                     acc[a + b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[a], B[b], acc[a + b], 0, 0, 0);
             cs = cs + 1 == kNS ? 0 : cs + 1;
         }
@@ -279,7 +282,7 @@ __global__ __launch_bounds__(256 * CG) void k_grm_i8(const int8_t* __restrict__ 
         const int ei = rexp[i];
         double val = 0.0;
 #pragma unroll
-        for (int q = 0; q < L; ++q) val = val + ldexp((double)acc[q][v], ei + ec - 12 - 8 * q);
+        for (int q = 0; q <= L; ++q) val = val + ldexp((double)acc[q][v], ei + ec - 12 - 8 * q);
         part[((int64_t)h * kI8Cols + col) * n + i] = val;
     }
 }

@@ -447,7 +447,8 @@ def grm_mixed_product(lt, n, S0, digits, sig2g=1.0):
     integer arithmetic (k_grm_i8.hip): per-row block fixed point of the off-diagonal FP32
     entries (scale 2^e_i, e_i the frexp exponent of the row's largest |G_ij|, j != i),
     per-column block fixed point of S0, four balanced int8 digits of each 30-bit value,
-    the digit pairs a + b < L summed exactly by weight class over chunks of tile columns,
+    the digit pairs a, b < L with a + b <= L summed exactly by weight class over chunks of
+    tile columns,
     then combined in FP64 in the device's order, with G_ii S0_i added in FP64.
     S0: n x k. Returns n x k."""
     S0 = np.asarray(S0, dtype=np.float64).reshape(n, -1)
@@ -469,7 +470,7 @@ def grm_mixed_product(lt, n, S0, digits, sig2g=1.0):
     for h in range(H):
         j0, j1 = 64 * (h * ntiles // H), min(n, 64 * ((h + 1) * ntiles // H))
         part = np.zeros_like(S0)
-        for w in range(L):
+        for w in range(L + 1):
             acc = np.zeros(S0.shape, dtype=np.int64)
             for a in range(w + 1):
                 b = w - a
