@@ -20,6 +20,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import select
 import shutil
 import subprocess
 import sys
@@ -118,8 +119,12 @@ def _reference_iteration_time(exe, c, y, rows, theta, threads, max_iters):
         proc = subprocess.Popen(cmd, stdout=slave, stderr=slave, env=env, close_fds=True)
         os.close(slave)
         stamps, nthr, t_start, buf, done = [], threads, None, b"", False
+        deadline = time.perf_counter() + 180.0  # the reference gets 3 minutes per sample
         try:
             while len(stamps) < max_iters and not done:
+                left = deadline - time.perf_counter()
+                if left <= 0 or not select.select([master], [], [], left)[0]:
+                    break
                 try:
                     chunk = os.read(master, 65536)
                 except OSError:
