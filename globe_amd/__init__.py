@@ -58,6 +58,12 @@ class AiremlOpts(C.Structure):
                 ("probes", C.c_int32), ("seed", C.c_uint64), ("pcg", PcgOpts)]
 
 
+class Loglik(C.Structure):
+    _fields_ = [("loglik", C.c_double), ("logdet_v", C.c_double), ("logdet_v_se", C.c_double),
+                ("logdet_q", C.c_double), ("logdet_xtvx", C.c_double), ("ypy", C.c_double),
+                ("probes", C.c_int32), ("max_steps", C.c_int32), ("seconds", C.c_double)]
+
+
 class Request(C.Structure):
     _fields_ = [("nrec", C.c_int64), ("indiv", C.c_void_p), ("times", C.c_void_p), ("y", C.c_void_p),
                 ("theta", C.c_double * 5)]
@@ -133,6 +139,7 @@ def load_library(path: str = LIB_PATH):
         "globe_aireml_step_reduced": ([P, P, P, dbl, P], C.c_int),
         "globe_aireml_probes": ([P, C.c_int, C.c_int, C.c_uint64, P, P], C.c_int),
         "globe_probe_seed": ([C.c_uint64, C.c_uint64], C.c_uint64),
+        "globe_reml_loglik": ([P, P, C.c_int, C.c_int, C.c_uint64, P, P], C.c_int),
         "globe_aireml_fit": ([P, P, P, P, P, P, P, P, P, P], C.c_int),
         "globe_aireml_fit_reduced": ([P, P, P, P, P, P, P, P, P, P], C.c_int),
         "globe_grm_from_geno": ([P], C.c_int),
@@ -392,6 +399,20 @@ class Globe:
         Wt = np.zeros((self.nrec, nmc), order="F")
         self._check(self.L.globe_aireml_probes(self.h, k, nmc, C.c_uint64(seed), _p(Y), _p(Wt)))
         return Y, Wt
+
+    def reml_loglik(self, y, *, probes="rademacher", nprobe=64, seed=1, tol=1e-8, max_iter=1000,
+                    precision=PREC_FP64):
+        """The REML log-likelihood at the current variance components: log|V| by
+        stochastic Lanczos quadrature over `nprobe` probe solves ("gaussian" or
+        "rademacher"), or exactly over all Nrec + 2N columns of the preconditioner's
+        factor ("exact"); the GLS terms from the fit of gls_blup."""
+        k = {"gaussian": 0, "rademacher": 1, "exact": 2}[probes]
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        o = PcgOpts(tol, max_iter, X0_ZERO, RULE_ALL_COLUMNS, 0, precision, 0)
+        r = Loglik()
+        self._check(self.L.globe_reml_loglik(self.h, _p(y), k, int(nprobe), C.c_uint64(seed), C.byref(o),
+                                             C.byref(r)))
+        return {f: getattr(r, f) for f, _ in Loglik._fields_}
 
     def geno_Wa(self, a):
         a = np.ascontiguousarray(a, dtype=np.float64)

@@ -274,7 +274,8 @@ int relaxed_limbs(double ratio, double tol) {
 }
 
 void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64_t ldx,
-             const globe_pcg_opts& o, globe_pcg_stats* st, double* col_ratio, int32_t* col_iters) {
+             const globe_pcg_opts& o, globe_pcg_stats* st, double* col_ratio, int32_t* col_iters,
+             double* hist, int hist_cap) {
     require(c->has_cohort && c->gen != 0, GLOBE_E_STATE, "cohort and genetics required");
     require(k >= 1 && k <= kMaxCols, GLOBE_E_INVALID, "k must be in [1, 64]");
     require(o.rule >= 0 && o.rule <= 2, GLOBE_E_INVALID, "unknown PCG rule");
@@ -309,6 +310,8 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
     hs->stagnation = o.stagnation;
     hs->k = k;
     hs->tol = o.tol;
+    hs->hist = hist;
+    hs->hist_cap = hist ? hist_cap : 0;
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
@@ -836,11 +839,6 @@ int globe_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int6
 
 // ------------------------------------------------------------------------ estimators
 namespace globe {
-
-struct Fit {
-    std::vector<double> beta, xtvx_inv;  // (p+1), (p+1)^2 row-major
-    globe_pcg_stats stats{};
-};
 
 // fitMME's closing steps (PCGsolver.cpp:262-290, varCompEst.cpp:636-665) from a solved
 // block in c->e_sol (ld nrec): columns 0..p hold V^-1 X, column p+1 V^-1 y. Forms

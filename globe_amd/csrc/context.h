@@ -93,6 +93,8 @@ struct globe_ctx {
 
     // estimator workspaces
     globe::DevBuf e_xy, e_sol, e_f, e_probe, e_w, e_sv, e_pv, e_us, e_fs, e_coef, e_small;
+    // log-likelihood: probes, their solutions and the solves' Lanczos coefficients
+    globe::DevBuf l_b, l_x, l_hist;
 
     // serving loop, when open
     ServeState* serve = nullptr;
@@ -129,8 +131,11 @@ void ensure_theta(globe_ctx* c, int k);
 void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int precision,
                      const int* skip, const double* theta_cols);
 void ensure_gdiag(globe_ctx* c);
+// hist (device, (1 + 2 hist_cap) kMaxCols doubles): the Lanczos coefficients of the
+// solve, laid out as PcgState::hist describes (nullptr: not kept)
 void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64_t ldx,
-             const globe_pcg_opts& o, globe_pcg_stats* st, double* col_ratio, int32_t* col_iters);
+             const globe_pcg_opts& o, globe_pcg_stats* st, double* col_ratio, int32_t* col_iters,
+             double* hist = nullptr, int hist_cap = 0);
 void require(bool cond, int code, const std::string& msg);
 // int8 digits (2..4) a mixed-precision product may use in a PCG iteration that starts at
 // residual ratio |r| / |b| (inexact Krylov relaxation)
@@ -160,6 +165,13 @@ int aireml_eval_impl(globe_ctx* c, const double* y, const double* probes, const 
                      const double* probe_x0, int nmc, const globe_pcg_opts* opts, bool want_ai,
                      double* grad, double* ai, double* first_terms, double* trace, int nvc,
                      const ProbeGen* gen);
+// fitMME (PCGsolver.cpp:262-290, varCompEst.cpp:636-665): solve V [X | y] (y_dev on the
+// device) into c->e_sol, then beta, (X^T V^-1 X)^-1 and V^-1 r into vinv_r_dev
+struct Fit {
+    std::vector<double> beta, xtvx_inv;  // (p+1), (p+1)^2 row-major
+    globe_pcg_stats stats{};
+};
+Fit fit_gls(globe_ctx* c, const double* y_dev, const globe_pcg_opts& o, double* vinv_r_dev);
 constexpr int kLimbsFP64 = 7;
 constexpr int kLimbsMixed = 4;
 

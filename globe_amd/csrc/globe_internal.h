@@ -52,6 +52,11 @@ struct PcgState {
     double tol, last, max_ratio, pad;
     double rs[kMaxCols], alpha[kMaxCols], beta[kMaxCols], bnorm[kMaxCols], ratio[kMaxCols];
     int active[kMaxCols], col_iters[kMaxCols];
+    // Lanczos coefficients of the solve (nullptr: not kept): hist[c] = r0.z0 of column
+    // c, then per iteration it < hist_cap the row pair hist[(1 + 2 it) kMaxCols + c] =
+    // alpha_it and hist[(2 + 2 it) kMaxCols + c] = beta_it of the columns still active
+    double* hist;
+    int hist_cap, hist_pad;
 };
 
 // Digit layout of a k-column right-hand side in the int8 contraction: column c has L[c]
@@ -170,6 +175,13 @@ void launch_reduce_parts_skip(hipStream_t s, const double* part, int nparts, int
 // ---------------------------------------------------------------- probes (k_probe.hip)
 // nmc columns (leading dimension ldy) of Philox-seeded probes: kind 0 Gaussian tilde-V at
 // theta (per-record individual and time), 1 Rademacher, 2 unit vectors sqrt(nrec) e_s
+// log-determinant probes b = F u, F F^T = Q (k_probe.hip); probe s is number first + s
+void launch_logdet_probes(hipStream_t s, int kind, int64_t nrec, int64_t first, int count, uint64_t seed,
+                          const int32_t* indiv, const double* times, const double* gdiag,
+                          const double theta[5], double* Y, int64_t ldy);
+// partial sums of log|Q_i| over the blocks into part; returns the partial count
+int launch_logdet_q(hipStream_t s, int32_t nblk, const int64_t* off, const int32_t* ind, const double* t,
+                    const double* gdiag, const double* theta, double* part);
 void launch_probes(hipStream_t s, int kind, int64_t nrec, int nmc, uint64_t seed, const int32_t* indiv,
                    const double* times, const double theta[5], double* Y, int64_t ldy);
 

@@ -76,7 +76,71 @@ __global__ __launch_bounds__(256) void k_probes(int kind, int64_t nrec, uint64_t
     }
 }
 
+// Probes b = F u of the log-determinant estimator, F F^T = Q (the block preconditioner:
+// Q_i = sig2e I + U_i C_i U_i^T, U_i = [1, t], C_i = [[sig2g G_ii + sig2b0, sigb01],
+// [sigb01, sig2b1]]): b_r = sqrt(sig2e) u3_r + ua_i (k00 + k10 t_r) + ub_i k11 t_r with
+// [[k00, 0], [k10, k11]] the Cholesky factor of C_i (a negative pivot clipped to 0).
+// Probe s has the global index g = first + s. kind 1: u3 = sign of bit 31 of word 0 of
+// (r, g, 3), ua / ub = signs of words 0 / 1 of (i, g, 4); kind 0: Box-Muller normals on
+// the same counters; kind 2: column g of F (g < nrec: sqrt(sig2e) e_g; else, for
+// g = nrec + 2 i + a, the column a of U_i K_i on the records of individual i).
+__global__ __launch_bounds__(256) void k_logdet_probes(int kind, int64_t nrec, int64_t first, uint64_t seed,
+                                                       const int32_t* __restrict__ indiv,
+                                                       const double* __restrict__ times,
+                                                       const double* __restrict__ gdiag, double se,
+                                                       double sg, double sb0, double sb01, double sb1,
+                                                       double* __restrict__ Y, int64_t ldy) {
+    const int s = blockIdx.y;
+    const int64_t g = first + s;
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int64_t r = blockIdx.x * 256ll + threadIdx.x; r < nrec; r += (int64_t)gridDim.x * 256) {
+        const int64_t i = indiv[r];
+        const double t = times[r];
+        const double c00 = sg * gdiag[i] + sb0;
+        const double k00 = c00 > 0.0 ? sqrt(c00) : 0.0;
+        const double k10 = k00 > 0.0 ? sb01 / k00 : 0.0;
+        const double d11 = sb1 - k10 * k10;
+        const double k11 = d11 > 0.0 ? sqrt(d11) : 0.0;
+        double y;
+        if (kind == 2) {
+            if (g < nrec) {
+                y = r == g ? sqrt(se) : 0.0;
+            } else {
+                const int64_t j = g - nrec;
+                y = i != j / 2 ? 0.0 : ((j & 1) == 0 ? k00 + k10 * t : k11 * t);
+            }
+        } else {
+            const U4 w = philox4x32_10((uint32_t)r, (uint32_t)(r >> 32), (uint32_t)g, 3u, k0, k1);
+            const U4 v = philox4x32_10((uint32_t)i, (uint32_t)(i >> 32), (uint32_t)g, 4u, k0, k1);
+            double u3, ua, ub;
+            if (kind == 1) {
+                u3 = (w.w[0] >> 31) ? 1.0 : -1.0;
+                ua = (v.w[0] >> 31) ? 1.0 : -1.0;
+                ub = (v.w[1] >> 31) ? 1.0 : -1.0;
+            } else {
+                u3 = sqrt(-2.0 * log(unit_open(w.w[0]))) * cos(two_pi * unit_open(w.w[1]));
+                const double ra = sqrt(-2.0 * log(unit_open(v.w[0])));
+                ua = ra * cos(two_pi * unit_open(v.w[1]));
+                ub = ra * sin(two_pi * unit_open(v.w[1]));
+            }
+            y = sqrt(se) * u3 + ua * (k00 + k10 * t) + ub * k11 * t;
+        }
+        Y[s * ldy + r] = y;
+    }
+}
+
 }  // namespace
+
+void launch_logdet_probes(hipStream_t s, int kind, int64_t nrec, int64_t first, int count, uint64_t seed,
+                          const int32_t* indiv, const double* times, const double* gdiag,
+                          const double theta[5], double* Y, int64_t ldy) {
+    if (nrec <= 0 || count <= 0) return;
+    const unsigned gx = (unsigned)std::min<int64_t>((nrec + 255) / 256, 1024);
+    k_logdet_probes<<<dim3(gx, count), 256, 0, s>>>(kind, nrec, first, seed, indiv, times, gdiag, theta[0],
+                                                    theta[1], theta[2], theta[3], theta[4], Y, ldy);
+    HIP_CHECK(hipGetLastError());
+}
 
 void launch_probes(hipStream_t s, int kind, int64_t nrec, int nmc, uint64_t seed, const int32_t* indiv,
                    const double* times, const double theta[5], double* Y, int64_t ldy) {

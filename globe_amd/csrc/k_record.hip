@@ -297,6 +297,37 @@ __global__ __launch_bounds__(kBlock) void k_precond(
     }
 }
 
+// log|Q_i| of the preconditioner blocks by Sylvester's determinant identity:
+// |sig2e I_n + U C U^T| = sig2e^(n - 2) |sig2e I_2 + C U^T U| (the 2x2 matrix k_precond
+// inverts); per-workgroup partial sums into part[blockIdx.x].
+__global__ __launch_bounds__(kBlock) void k_logdet_q(int32_t nblk, const int64_t* __restrict__ off,
+                                                     const int32_t* __restrict__ ind,
+                                                     const double* __restrict__ t,
+                                                     const double* __restrict__ gdiag,
+                                                     const double* __restrict__ th, double* part) {
+    const double se = th[0], sg = th[1], sb0 = th[2], sb01 = th[3], sb1 = th[4];
+    const double lse = log(se);
+    double acc = 0.0;
+    for (int64_t b = blockIdx.x * (int64_t)kBlock + threadIdx.x; b < nblk;
+         b += (int64_t)gridDim.x * kBlock) {
+        const int64_t r0 = off[b], r1 = off[b + 1];
+        double a01 = 0.0, a11 = 0.0;
+        for (int64_t q = r0; q < r1; ++q) {
+            a01 += t[q];
+            a11 += t[q] * t[q];
+        }
+        const double a00 = (double)(r1 - r0);
+        const double c00 = sg * gdiag[ind[b]] + sb0, c01 = sb01, c11 = sb1;
+        const double m00 = se + c00 * a00 + c01 * a01;
+        const double m01 = c00 * a01 + c01 * a11;
+        const double m10 = c01 * a00 + c11 * a01;
+        const double m11 = se + c01 * a01 + c11 * a11;
+        acc += (a00 - 2.0) * lse + log(m00 * m11 - m01 * m10);
+    }
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
 __global__ __launch_bounds__(kBlock) void k_dot_parts(int64_t nrec, const double* __restrict__ a,
                                                       int64_t lda, const double* __restrict__ b,
                                                       int64_t ldb, double* part) {
@@ -391,6 +422,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(PcgState* st, const double*
         st->alpha[c] = 0.0;
         st->beta[c] = 0.0;
         st->col_iters[c] = 0;
+        if (st->hist) st->hist[c] = rz;
         st->ratio[c] = nb > 0.0 ? nr / nb : 0.0;
         ratio_s[c] = nb > 1e-12 ? nr / nb : 0.0;
         int act = 1;
@@ -453,6 +485,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_alpha(PcgState* st, const double
     const int c = threadIdx.x;
     if (c >= k) return;
     st->alpha[c] = st->active[c] ? st->rs[c] / sums[c] : 0.0;
+    if (st->hist && st->active[c] && st->iters < st->hist_cap)
+        st->hist[(1 + 2 * (int64_t)st->iters) * kMaxCols + c] = st->alpha[c];
 }
 
 __global__ __launch_bounds__(kBlock) void k_pcg_xr(PcgState* st, int64_t nrec, double* __restrict__ x,
@@ -559,6 +593,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_beta(PcgState* st, const double*
     const double s = sums[c];
     st->beta[c] = s / st->rs[c];
     st->rs[c] = s;
+    // iters already counts this iteration
+    if (st->hist && st->iters - 1 < st->hist_cap)
+        st->hist[(2 + 2 * (int64_t)(st->iters - 1)) * kMaxCols + c] = st->beta[c];
 }
 
 __global__ __launch_bounds__(kBlock) void k_pcg_p(PcgState* st, int64_t nrec, double* __restrict__ p,
@@ -755,6 +792,14 @@ void launch_axpy_cols(hipStream_t s, int64_t nrec, double* y, int64_t ldy, const
     dim3 g(parts_for(nrec), ky);
     k_axpy_cols<<<g, kBlock, 0, s>>>(nrec, y, ldy, x, ldx, kx, coef, ky);
     LAUNCH_CHECK();
+}
+
+int launch_logdet_q(hipStream_t s, int32_t nblk, const int64_t* off, const int32_t* ind, const double* t,
+                    const double* gdiag, const double* theta, double* part) {
+    const int g = (int)std::min<int64_t>(std::max<int64_t>((nblk + kBlock - 1) / kBlock, 1), kMaxParts);
+    k_logdet_q<<<g, kBlock, 0, s>>>(nblk, off, ind, t, gdiag, theta, part);
+    HIP_CHECK(hipGetLastError());
+    return g;
 }
 
 void launch_pcg_admit(hipStream_t s, PcgState* st, int j, const double* part_bb, int nparts_bb,

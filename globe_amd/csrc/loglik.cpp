@@ -1,0 +1,242 @@
+// The REML log-likelihood at the context's variance components (globe_reml_loglik).
+//
+// The reference has none: its AI-REML loop tracks predicted and approximate changes of
+// the log-likelihood only (varCompEst.cpp:1221, 1277-1278). Here
+//   loglik = -1/2 (log|V| + log|X^T V^-1 X| + y^T P y + (Nrec - p - 1) log 2pi)
+// with the GLS terms from the same fit as globe_gls_blup and log|V| split as
+//   log|V| = log|Q| + log|Q^-1/2 V Q^-1/2|,
+// Q the block preconditioner. log|Q| is exact: Q_i = sig2e I + U_i C_i U_i^T per
+// individual, |Q_i| = sig2e^(n_i - 2) |sig2e I_2 + C_i U_i^T U_i| (k_logdet_q). The
+// second term is a stochastic Lanczos quadrature (Ubaru, Chen and Saad, SIAM J. Matrix
+// Anal. Appl. 38 (2017)): for a probe b = F u with F F^T = Q and E[u u^T] = I,
+// u' = Q^-1/2 b has E[u' u'^T] = I, and preconditioned CG on V x = b from x = 0 is the
+// Lanczos process of A = Q^-1/2 V Q^-1/2 started at u'. Its coefficients give the
+// Lanczos matrix T (Saad, Iterative Methods for Sparse Linear Systems, 2nd ed.,
+// sec. 6.7.3), and u'^T log(A) u' ~ |u'|^2 e1^T log(T) e1, |u'|^2 = b^T Q^-1 b = r0.z0
+// of the solve (Gauss quadrature of the spectral measure of A at u', Golub and Meurant).
+// The block PCG solves up to 64 probes per call; each column's alpha_j, beta_j stay on
+// the device (PcgState::hist) and come back once per call.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "context.h"
+
+namespace globe {
+
+namespace {
+
+// e1^T log(T) e1 for the symmetric tridiagonal T (diagonal d, off-diagonal e), by
+// implicit shifted QR steps with the Wilkinson shift (Golub and Van Loan, Matrix
+// Computations, 4th ed., sec. 8.3) on a dense copy, carrying only the first components
+// of the eigenvectors (Golub and Welsch, Math. Comp. 23 (1969)): every rotation R of
+// A <- R A R^T also updates v <- R v, v = e1 at the start, so at the end v_k is the first
+// component of the eigenvector of the eigenvalue A_kk. Returns false if T is not
+// positive definite or the iteration does not converge.
+bool log_quadrature(const std::vector<double>& d, const std::vector<double>& e, double& out) {
+    const int m = (int)d.size();
+    std::vector<double> A((size_t)m * m, 0.0), v((size_t)m, 0.0);
+    auto a = [&](int i, int j) -> double& { return A[(size_t)i * m + j]; };
+    for (int i = 0; i < m; ++i) a(i, i) = d[i];
+    for (int i = 0; i + 1 < m; ++i) a(i, i + 1) = a(i + 1, i) = e[i];
+    v[0] = 1.0;
+    constexpr double eps = 2.220446049250313e-16;
+    // rotation in the plane (k, k+1): rows k, k+1 <- [c s; -s c] rows, then the same on
+    // the columns; only indices lo-1..k+2 can be nonzero there, clipped to the window
+    auto rotate = [&](int k, double c, double s, int lo, int hi) {
+        const int j0 = std::max(lo, k - 1), j1 = std::min(hi, k + 2);
+        for (int j = j0; j <= j1; ++j) {
+            const double x = a(k, j), y = a(k + 1, j);
+            a(k, j) = c * x + s * y;
+            a(k + 1, j) = -s * x + c * y;
+        }
+        for (int j = j0; j <= j1; ++j) {
+            const double x = a(j, k), y = a(j, k + 1);
+            a(j, k) = c * x + s * y;
+            a(j, k + 1) = -s * x + c * y;
+        }
+        const double x = v[k], y = v[k + 1];
+        v[k] = c * x + s * y;
+        v[k + 1] = -s * x + c * y;
+    };
+    auto small = [&](int i) {  // a(i, i-1) negligible
+        return std::fabs(a(i, i - 1)) <= eps * (std::fabs(a(i, i)) + std::fabs(a(i - 1, i - 1)));
+    };
+    int hi = m - 1, steps = 0;
+    while (hi > 0) {
+        if (small(hi)) {
+            a(hi, hi - 1) = a(hi - 1, hi) = 0.0;
+            --hi;
+            continue;
+        }
+        int lo = hi - 1;
+        while (lo > 0 && !small(lo)) --lo;
+        if (lo > 0) a(lo, lo - 1) = a(lo - 1, lo) = 0.0;
+        if (++steps > 50 * m) return false;
+        // Wilkinson shift: the eigenvalue of the trailing 2x2 nearer a(hi, hi)
+        const double dd = 0.5 * (a(hi - 1, hi - 1) - a(hi, hi)), b = a(hi, hi - 1);
+        const double mu = a(hi, hi) - b * b / (dd + std::copysign(std::hypot(dd, b), dd));
+        double x = a(lo, lo) - mu, y = a(lo + 1, lo);
+        for (int k = lo; k < hi; ++k) {
+            const double r = std::hypot(x, y);
+            const double c = r > 0.0 ? x / r : 1.0, s = r > 0.0 ? y / r : 0.0;
+            rotate(k, c, s, lo, hi);
+            if (k > lo) a(k + 1, k - 1) = a(k - 1, k + 1) = 0.0;  // the bulge just chased
+            if (k + 1 < hi) {
+                x = a(k + 1, k);
+                y = a(k + 2, k);
+            }
+        }
+    }
+    double q = 0.0;
+    for (int k = 0; k < m; ++k) {
+        if (!(a(k, k) > 0.0)) return false;
+        q += v[k] * v[k] * std::log(a(k, k));
+    }
+    out = q;
+    return true;
+}
+
+// log|S| of a small symmetric positive definite S (n x n, row-major) by Cholesky
+bool logdet_spd(int n, std::vector<double> S, double& out) {
+    double ld = 0.0;
+    for (int j = 0; j < n; ++j) {
+        double djj = S[(size_t)j * n + j];
+        for (int k = 0; k < j; ++k) djj -= S[(size_t)j * n + k] * S[(size_t)j * n + k];
+        if (!(djj > 0.0)) return false;
+        const double ljj = std::sqrt(djj);
+        S[(size_t)j * n + j] = ljj;
+        ld += 2.0 * std::log(ljj);
+        for (int i = j + 1; i < n; ++i) {
+            double sij = S[(size_t)i * n + j];
+            for (int k = 0; k < j; ++k) sij -= S[(size_t)i * n + k] * S[(size_t)j * n + k];
+            S[(size_t)i * n + j] = sij / ljj;
+        }
+    }
+    out = ld;
+    return true;
+}
+
+}  // namespace
+
+}  // namespace globe
+
+using namespace globe;
+
+extern "C" int globe_reml_loglik(globe_ctx* c, const double* y, int kind, int nprobe, uint64_t seed,
+                                 const globe_pcg_opts* opts, globe_loglik* out) {
+    API_BEGIN(c)
+    const auto t0 = std::chrono::steady_clock::now();
+    require(y && opts && out, GLOBE_E_INVALID, "globe_reml_loglik: null argument");
+    require(c->has_cohort && c->gen != 0, GLOBE_E_STATE, "cohort and genetics required");
+    require(c->has_theta, GLOBE_E_STATE, "variance components not set");
+    require(kind >= 0 && kind <= 2, GLOBE_E_INVALID, "unknown probe kind");
+    require(kind == 2 || (nprobe >= 2 && nprobe <= (1 << 20)), GLOBE_E_INVALID,
+            "nprobe must be in [2, 2^20] for random probes");
+    require(opts->max_iter >= 1 && opts->max_iter <= 100000, GLOBE_E_INVALID, "max_iter must be in [1, 1e5]");
+    const double se = c->theta[0], sg = c->theta[1], sb0 = c->theta[2], sb01 = c->theta[3], sb1 = c->theta[4];
+    require(se > 0.0 && sg >= 0.0 && sb0 >= 0.0 && sb1 >= 0.0 && sb01 * sb01 <= sb0 * sb1, GLOBE_E_INVALID,
+            "the log-likelihood needs sig2e > 0, sig2g, sig2b0, sig2b1 >= 0 and sigb01^2 <= sig2b0 sig2b1");
+    const int64_t nrec = c->nrec;
+    const int q = c->p + 1;
+    hipStream_t s = c->stream;
+    ensure_gdiag(c);
+
+    // GLS terms: y^T P y = y^T V^-1 r and log|X^T V^-1 X|, from the fit globe_gls_blup runs
+    c->e_probe.ensure(sizeof(double) * nrec);
+    c->e_pv.ensure(sizeof(double) * nrec);
+    HIP_CHECK(hipMemcpy(c->e_probe.p, y, sizeof(double) * nrec, hipMemcpyHostToDevice));
+    Fit f = fit_gls(c, c->e_probe.as<double>(), *opts, c->e_pv.as<double>());
+    c->e_small.ensure(sizeof(double) * kMaxCols * kMaxCols);
+    c->w_part.ensure(sizeof(double) * kMaxParts * kMaxCols * kMaxCols);
+    launch_gram(s, nrec, c->e_probe.as<double>(), nrec, 1, c->e_pv.as<double>(), nrec, 1, c->w_part.as<double>(),
+                c->e_small.as<double>());
+    double ypy = 0.0;
+    HIP_CHECK(hipMemcpyAsync(&ypy, c->e_small.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    // log|Q| over the individuals' blocks
+    ensure_theta(c, kMaxCols);
+    const int ng = launch_logdet_q(s, c->n, c->d_blk_off.as<int64_t>(), c->d_blk_ind.as<int32_t>(),
+                                   c->d_times.as<double>(), c->d_gdiag.as<double>(), c->d_theta.as<double>(),
+                                   c->w_part.as<double>());
+    std::vector<double> lq((size_t)ng);
+    HIP_CHECK(hipMemcpyAsync(lq.data(), c->w_part.p, sizeof(double) * ng, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    double logdet_q = 0.0;
+    for (double v : lq) logdet_q += v;
+    double ld_inv = 0.0;
+    require(logdet_spd(q, f.xtvx_inv, ld_inv), GLOBE_E_NUMERIC, "X^T V^-1 X is not positive definite");
+
+    // log|Q^-1/2 V Q^-1/2| by Lanczos quadrature over the probe solves
+    const int64_t total = kind == 2 ? nrec + 2 * (int64_t)c->n : nprobe;
+    globe_pcg_opts o = *opts;
+    o.rule = GLOBE_RULE_PER_COLUMN;
+    o.x0_mode = GLOBE_X0_ZERO;
+    o.stagnation = 0;
+    const int cap = o.max_iter;
+    const size_t hist_n = (size_t)(1 + 2 * (size_t)cap) * kMaxCols;
+    c->l_b.ensure(sizeof(double) * nrec * kMaxCols);
+    c->l_x.ensure(sizeof(double) * nrec * kMaxCols);
+    c->l_hist.ensure(sizeof(double) * hist_n);
+    std::vector<double> hist;
+    std::vector<double> est;
+    est.reserve((size_t)std::min<int64_t>(total, 1 << 20));
+    int max_steps = 0;
+    std::vector<int32_t> col_iters(kMaxCols);
+    for (int64_t first = 0; first < total; first += kMaxCols) {
+        const int kb = (int)std::min<int64_t>(kMaxCols, total - first);
+        launch_logdet_probes(s, kind, nrec, first, kb, seed, c->d_indiv.as<int32_t>(), c->d_times.as<double>(),
+                             c->d_gdiag.as<double>(), c->theta.data(), c->l_b.as<double>(), nrec);
+        globe_pcg_stats st{};
+        run_pcg(c, c->l_b.as<double>(), nrec, kb, c->l_x.as<double>(), nrec, o, &st, nullptr, col_iters.data(),
+                c->l_hist.as<double>(), cap);
+        const int steps = std::min(st.iters, cap);
+        hist.resize((size_t)(1 + 2 * (size_t)steps) * kMaxCols);
+        HIP_CHECK(hipMemcpy(hist.data(), c->l_hist.p, sizeof(double) * hist.size(), hipMemcpyDeviceToHost));
+        for (int j = 0; j < kb; ++j) {
+            const int m = std::min(col_iters[j], steps);
+            const double rho0 = hist[j];
+            if (m == 0) {  // a zero probe (a column of F that vanishes) or one solved at x0 = 0
+                est.push_back(0.0);
+                continue;
+            }
+            max_steps = std::max(max_steps, m);
+            std::vector<double> d((size_t)m), e((size_t)(m - 1));
+            for (int it = 0; it < m; ++it) {
+                const double al = hist[(size_t)(1 + 2 * it) * kMaxCols + j];
+                require(al > 0.0, GLOBE_E_NUMERIC, "Lanczos breakdown: non-positive CG step");
+                d[it] = 1.0 / al;
+                if (it > 0) {
+                    const double alp = hist[(size_t)(1 + 2 * (it - 1)) * kMaxCols + j];
+                    const double bp = hist[(size_t)(2 + 2 * (it - 1)) * kMaxCols + j];
+                    require(bp > 0.0, GLOBE_E_NUMERIC, "Lanczos breakdown: non-positive CG beta");
+                    d[it] += bp / alp;
+                    e[it - 1] = std::sqrt(bp) / alp;
+                }
+            }
+            double qd = 0.0;
+            require(log_quadrature(d, e, qd), GLOBE_E_NUMERIC, "Lanczos matrix not positive definite");
+            est.push_back(rho0 * qd);
+        }
+    }
+    double sum = 0.0;
+    for (double v : est) sum += v;
+    double logdet_a = sum, se_a = 0.0;
+    if (kind != 2) {
+        const double t = (double)est.size(), mean = sum / t;
+        double ss = 0.0;
+        for (double v : est) ss += (v - mean) * (v - mean);
+        logdet_a = mean;
+        se_a = std::sqrt(ss / (t - 1.0) / t);
+    }
+    out->logdet_q = logdet_q;
+    out->logdet_v = logdet_q + logdet_a;
+    out->logdet_v_se = se_a;
+    out->logdet_xtvx = -ld_inv;
+    out->ypy = ypy;
+    out->loglik = -0.5 * (out->logdet_v + out->logdet_xtvx + ypy + (double)(nrec - q) * std::log(2.0 * M_PI));
+    out->probes = (int32_t)est.size();
+    out->max_steps = max_steps;
+    out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    API_END(c)
+}

@@ -259,6 +259,33 @@ int globe_aireml_fit_reduced(globe_ctx* ctx, const double* y, const globe_aireml
                              double theta[2], double se[2], double ai[4], int32_t* iters,
                              double* sec_per_iter, globe_log_fn log, void* user);
 
+/* ---- REML log-likelihood (no reference counterpart: the reference's AI-REML loop tracks
+ * only predicted and approximate changes of it, varCompEst.cpp:1221, 1277-1278) ---- */
+typedef struct {
+    double loglik;       /* -1/2 (log|V| + log|X^T V^-1 X| + y^T P y + (Nrec - p - 1) log 2pi) */
+    double logdet_v;     /* log|V|: logdet_q + the Lanczos quadrature estimate */
+    double logdet_v_se;  /* standard error of logdet_v over the probes (0 for exact probes) */
+    double logdet_q;     /* log|Q| of the block preconditioner, exact */
+    double logdet_xtvx;  /* log|X^T V^-1 X| */
+    double ypy;          /* y^T P y */
+    int32_t probes;      /* probe solves run */
+    int32_t max_steps;   /* most Lanczos steps (PCG iterations) of one probe */
+    double seconds;      /* wall time of the call */
+} globe_loglik;
+
+/* The REML log-likelihood at the context's variance components. log|V| = log|Q| +
+ * log|Q^-1/2 V Q^-1/2|: log|Q| exactly over the per-individual preconditioner blocks,
+ * the second term by stochastic Lanczos quadrature -- probes b = F u with F F^T = Q,
+ * solved V x = b by the block PCG (per-column rule, x0 = 0, opts->tol / max_iter /
+ * precision), each column's alpha/beta giving the Lanczos matrix T and the estimate
+ * (b^T Q^-1 b) e1^T log(T) e1. kind 0: Gaussian u, 1: Rademacher u (nprobe >= 2 each,
+ * estimate = their mean); 2: exact -- the Nrec + 2N columns of F (nprobe ignored), whose
+ * sum is log|Q^-1/2 V Q^-1/2| itself up to the solves' tolerance. Probes come from
+ * Philox4x32-10 keyed by `seed`. The fixed effects are solved as globe_gls_blup does.
+ * Needs sig2e > 0, sig2g >= 0, sig2b0 >= 0, sig2b1 >= 0 and sigb01^2 <= sig2b0 sig2b1. */
+int globe_reml_loglik(globe_ctx* ctx, const double* y, int kind, int nprobe, uint64_t seed,
+                      const globe_pcg_opts* opts, globe_loglik* out);
+
 /* ---- serving loop (no reference counterpart: the reference solves one trait per run) -- */
 /* One resident cohort (globe_set_cohort for N, genotypes or GRM loaded) serves solve
  * requests V_j x_j = y_j, each with its own individuals, records, times and variance

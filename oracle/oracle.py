@@ -574,3 +574,149 @@ def aireml_fit_replay(orc, y, theta0, probes_of, step_fn, *, max_iter=50, tol=1e
         if delta_tr < 1e-4:
             break
     return th, np.array(traj), it_done
+
+
+# ------------------------------------------------ REML log-likelihood (log|V| by SLQ)
+# The reference has no log-likelihood (its AI-REML loop tracks predicted and
+# approximate changes only, varCompEst.cpp:1221, 1277-1278). These restate the build's
+# estimator, globe_reml_loglik: log|V| = log|Q| + log|Q^-1/2 V Q^-1/2|, the first exact
+# over the per-individual preconditioner blocks Q_i (computePreconditioner_Block,
+# PCGsolver.cpp:991-1058), the second by stochastic Lanczos quadrature (Ubaru, Chen and
+# Saad, SIAM J. Matrix Anal. Appl. 38 (2017)) on the coefficients of preconditioned CG
+# solves V x = b with probes b = F u, F F^T = Q.
+def precond_blocks(indiv, times, gdiag, theta):
+    """Dense per-individual blocks Q_i = sig2g G_ii 11^T + U_i B U_i^T + sig2e I,
+    U_i = [1, t], B = [[sig2b0, sigb01], [sigb01, sig2b1]]: {individual: (records, Q_i)}."""
+    se, sg, sb0, sb01, sb1 = theta
+    indiv = np.asarray(indiv)
+    out = {}
+    for i in np.unique(indiv):
+        r = np.nonzero(indiv == i)[0]
+        t = np.asarray(times, dtype=np.float64)[r]
+        Q = (sg * gdiag[i] + sb0) + sb01 * (t[:, None] + t[None, :]) + sb1 * np.outer(t, t)
+        out[int(i)] = (r, Q + se * np.eye(r.size))
+    return out
+
+
+def logdet_precond(indiv, times, gdiag, theta):
+    """log|Q| = sum over individuals of log|Q_i| (numpy slogdet of each dense block)."""
+    tot = 0.0
+    for _, Q in precond_blocks(indiv, times, gdiag, theta).values():
+        s, ld = np.linalg.slogdet(Q)
+        assert s > 0
+        tot += ld
+    return tot
+
+
+def logdet_probes(kind, indiv, times, gdiag, theta, first, count, seed):
+    """The log-determinant probes b = F u as the device draws them (k_probe.hip,
+    k_logdet_probes), restated: F = [sqrt(sig2e) I | U_i K_i per individual], K_i the
+    lower Cholesky factor of C_i = [[sig2g G_ii + sig2b0, sigb01], [sigb01, sig2b1]]
+    (a negative pivot clipped to 0), so F F^T = Q. Probe s (global index first + s):
+    kind 1 Rademacher u (record signs from Philox counter (r, s, 3), the two individual
+    signs from bit 31 of words 0 and 1 of (i, s, 4)); kind 0 Gaussian u (Box-Muller on
+    the same counters); kind 2 the columns of F (s < nrec: sqrt(sig2e) e_s, then
+    U_i K_i e_a for s = nrec + 2 i + a). Returns nrec x count."""
+    indiv = np.asarray(indiv, dtype=np.int64)
+    t = np.asarray(times, dtype=np.float64)
+    nrec = indiv.shape[0]
+    se, sg, sb0, sb01, sb1 = theta
+    c00 = sg * np.asarray(gdiag, dtype=np.float64)[indiv] + sb0
+    k00 = np.sqrt(np.maximum(c00, 0.0))
+    k10 = np.where(k00 > 0, sb01 / np.where(k00 > 0, k00, 1.0), 0.0)
+    k11 = np.sqrt(np.maximum(sb1 - k10 * k10, 0.0))
+    k0, k1 = seed & 0xFFFFFFFF, seed >> 32
+    r = np.arange(nrec, dtype=np.uint64)
+    iu = indiv.astype(np.uint64)
+    unit = lambda w: (w.astype(np.float64) + 0.5) * 2.0 ** -32  # noqa: E731
+    out = np.zeros((nrec, count), order="F")
+    for j in range(count):
+        s = first + j
+        if kind == 2:
+            if s < nrec:
+                out[s, j] = np.sqrt(se)
+            else:
+                i, a = divmod(s - nrec, 2)
+                sel = indiv == i
+                out[sel, j] = (k00 + k10 * t)[sel] if a == 0 else (k11 * t)[sel]
+            continue
+        w = philox4x32_10(r, r >> np.uint64(32), s, 3, k0, k1)
+        v = philox4x32_10(iu, iu >> np.uint64(32), s, 4, k0, k1)
+        if kind == 1:
+            u3 = np.where(w[0] >> np.uint64(31), 1.0, -1.0)
+            ua = np.where(v[0] >> np.uint64(31), 1.0, -1.0)
+            ub = np.where(v[1] >> np.uint64(31), 1.0, -1.0)
+        else:
+            u3 = np.sqrt(-2.0 * np.log(unit(w[0]))) * np.cos(2 * np.pi * unit(w[1]))
+            ra = np.sqrt(-2.0 * np.log(unit(v[0])))
+            ua = ra * np.cos(2 * np.pi * unit(v[1]))
+            ub = ra * np.sin(2 * np.pi * unit(v[1]))
+        out[:, j] = np.sqrt(se) * u3 + ua * (k00 + k10 * t) + ub * k11 * t
+    return out
+
+
+def slq_exact_terms(V, Q, B):
+    """Per probe b (columns of B) the quadratic form the Lanczos quadrature converges
+    to: u^T log(A) u with u = Q^-1/2 b and A = Q^-1/2 V Q^-1/2 (dense eigendecompositions;
+    any square root of Q gives the same value, as u u^T -> Q^-1/2 b b^T Q^-1/2)."""
+    w, E = np.linalg.eigh(Q)
+    Qh = (E / np.sqrt(w)) @ E.T
+    lam, Z = np.linalg.eigh(Qh @ V @ Qh)
+    U = Z.T @ (Qh @ B)
+    return (np.log(lam)[:, None] * U * U).sum(axis=0)
+
+
+def pcg_lanczos(V, Qinv, b, tol, max_iter=1000):
+    """Preconditioned CG on V x = b from x = 0 (numpy, FP64) returning r0.z0 and the
+    coefficients alpha_0..alpha_{m-1}, beta_0..beta_{m-2} of its m iterations."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = Qinv @ r
+    p = z.copy()
+    rz = r @ z
+    rho0, al, be = rz, [], []
+    nb = np.linalg.norm(b)
+    for _ in range(max_iter):
+        vp = V @ p
+        a = rz / (p @ vp)
+        al.append(a)
+        x += a * p
+        r -= a * vp
+        if np.linalg.norm(r) < tol * nb:
+            break
+        z = Qinv @ r
+        rz2 = r @ z
+        be.append(rz2 / rz)
+        p = z + be[-1] * p
+        rz = rz2
+    return rho0, np.array(al), np.array(be)
+
+
+def lanczos_log_quadrature(alpha, beta):
+    """e1^T log(T) e1 of the Lanczos tridiagonal of a CG solve (Saad, Iterative Methods
+    for Sparse Linear Systems, 2nd ed., sec. 6.7.3): T_00 = 1/alpha_0, T_jj = 1/alpha_j +
+    beta_{j-1}/alpha_{j-1}, T_{j-1,j} = sqrt(beta_{j-1})/alpha_{j-1}; Gauss quadrature
+    sum_k tau_k^2 log theta_k over T's eigenpairs (scipy eigh_tridiagonal)."""
+    from scipy.linalg import eigh_tridiagonal
+    alpha = np.asarray(alpha, dtype=np.float64)
+    beta = np.asarray(beta, dtype=np.float64)
+    m = alpha.size
+    d = 1.0 / alpha
+    d[1:] += beta[: m - 1] / alpha[: m - 1]
+    e = np.sqrt(beta[: m - 1]) / alpha[: m - 1]
+    th, Z = eigh_tridiagonal(d, e)
+    return float((Z[0] ** 2 * np.log(th)).sum())
+
+
+def reml_loglik_dense(V, X, y):
+    """The REML log-likelihood -1/2 (log|V| + log|X^T V^-1 X| + y^T P y + (n - q) log 2pi),
+    P = V^-1 - V^-1 X (X^T V^-1 X)^-1 X^T V^-1, from dense V (numpy slogdet).
+    Returns (loglik, log|V|, log|X^T V^-1 X|, y^T P y)."""
+    Vi = np.linalg.inv(V)
+    XtViX = X.T @ Vi @ X
+    P = Vi - Vi @ X @ np.linalg.solve(XtViX, X.T @ Vi)
+    ld1 = np.linalg.slogdet(V)[1]
+    ld2 = np.linalg.slogdet(XtViX)[1]
+    ypy = float(y @ P @ y)
+    n, q = X.shape
+    return -0.5 * (ld1 + ld2 + ypy + (n - q) * np.log(2 * np.pi)), ld1, ld2, ypy

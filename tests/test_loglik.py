@@ -1,0 +1,184 @@
+"""The REML log-likelihood (globe_reml_loglik). The reference has none: its AI-REML
+loop tracks predicted and approximate changes only (varCompEst.cpp:1221, 1277-1278), so
+every figure here is pinned by dense closed forms on small cohorts.
+
+CPU: the oracle's pieces against the mathematics -- the probe factor F has F F^T = Q
+(the block preconditioner, computePreconditioner_Block PCGsolver.cpp:991-1058); log|Q|
+per block equals the dense determinant; the exact-probe sum is log|V|; the Lanczos
+quadrature of preconditioned CG coefficients equals the dense quadratic form u^T log(A) u;
+the random probes have covariance Q; the REML formula equals the likelihood of the
+error contrasts K^T y (Harville, Biometrika 61 (1974)).
+GPU: exact probes give log|V|, log|X^T V^-1 X|, y^T P y and the log-likelihood of the
+dense model; random probes give the oracle's quadratic forms of the same probes; mixed
+precision stays within 1e-4 of FP64.
+"""
+import numpy as np
+import pytest
+
+import globe_amd
+import oracle as O
+from globe_amd import synth
+from helpers import dense_V
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+def grm_case(n=50, lam=2.0, m=400, seed=61, theta=(0.5, 0.8, 0.4, 0.1, 0.2)):
+    rng = np.random.default_rng(seed)
+    c = synth.simulate_cohort(n, lam, rng)
+    g = synth.make_reduced_case(n=n, m=m, seed=seed + 1)
+    G = synth.grm_dense(g.grm_lt, n)
+    V, _ = dense_V(c, G, theta)
+    y = c.X @ np.array([1.0, 0.3, 0.01, 0.05])[: c.X.shape[1]] + np.linalg.cholesky(V) @ rng.standard_normal(c.nrec)
+    return c, g.grm_lt, G, V, y
+
+
+def dense_Q(c, gd, theta):
+    Q = np.zeros((c.nrec, c.nrec))
+    for r, Qi in O.precond_blocks(c.indiv, c.times, gd, theta).values():
+        Q[np.ix_(r, r)] = Qi
+    return Q
+
+
+# ------------------------------------------------------------------------------ CPU
+def test_oracle_probe_factor_and_logdet_q():
+    th = (0.5, 0.8, 0.4, 0.1, 0.2)
+    c, _, G, V, _ = grm_case()
+    gd = np.diag(G)
+    Q = dense_Q(c, gd, th)
+    F = O.logdet_probes(2, c.indiv, c.times, gd, th, 0, c.nrec + 2 * c.n, 1)
+    assert np.abs(F @ F.T - Q).max() < 1e-12 * np.abs(Q).max()
+    assert abs(O.logdet_precond(c.indiv, c.times, gd, th) - np.linalg.slogdet(Q)[1]) < 1e-10
+    # the exact probes sum to log|Q^-1/2 V Q^-1/2| = log|V| - log|Q|
+    tot = O.slq_exact_terms(V, Q, F).sum() + np.linalg.slogdet(Q)[1]
+    assert abs(tot - np.linalg.slogdet(V)[1]) < 1e-9 * abs(np.linalg.slogdet(V)[1])
+    # a zero slope variance: the second column of each K_i vanishes, F F^T = Q still
+    th0 = (0.5, 0.8, 0.4, 0.0, 0.0)
+    F0 = O.logdet_probes(2, c.indiv, c.times, gd, th0, 0, c.nrec + 2 * c.n, 1)
+    assert np.abs(F0 @ F0.T - dense_Q(c, gd, th0)).max() < 1e-12
+
+
+def test_oracle_lanczos_quadrature_equals_dense_quadratic_form():
+    th = (0.5, 0.8, 0.4, 0.1, 0.2)
+    c, _, G, V, _ = grm_case()
+    gd = np.diag(G)
+    Q = dense_Q(c, gd, th)
+    Qi = np.linalg.inv(Q)
+    B = O.logdet_probes(1, c.indiv, c.times, gd, th, 0, 6, 5)
+    exact = O.slq_exact_terms(V, Q, B)
+    for j in range(B.shape[1]):
+        rho0, al, be = O.pcg_lanczos(V, Qi, B[:, j], 1e-12)
+        assert abs(rho0 - B[:, j] @ Qi @ B[:, j]) < 1e-10 * rho0
+        assert abs(rho0 * O.lanczos_log_quadrature(al, be) - exact[j]) < 1e-9 * max(1.0, abs(exact[j]))
+    # a plausible slip (T_jj without the beta/alpha term) moves the quadrature off
+    rho0, al, be = O.pcg_lanczos(V, Qi, B[:, 0], 1e-12)
+    from scipy.linalg import eigh_tridiagonal
+    th_, Z = eigh_tridiagonal(1.0 / al, np.sqrt(be) / al[:-1])
+    assert abs(rho0 * (Z[0] ** 2 * np.log(th_)).sum() - exact[0]) > 1e-5
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_oracle_random_probes_have_covariance_q(kind):
+    th = (0.5, 0.8, 0.4, 0.1, 0.2)
+    c, _, G, _, _ = grm_case(n=20)
+    gd = np.diag(G)
+    Q = dense_Q(c, gd, th)
+    B = O.logdet_probes(kind, c.indiv, c.times, gd, th, 0, 6000, 11)
+    C = B @ B.T / B.shape[1]
+    assert np.abs(C - Q).max() < 0.08 * np.abs(Q).max()
+    # probes continue the stream: a batch from index 100 equals columns 100.. of one call
+    assert np.array_equal(O.logdet_probes(kind, c.indiv, c.times, gd, th, 100, 3, 11), B[:, 100:103])
+
+
+def test_oracle_reml_equals_error_contrast_likelihood():
+    c, _, _, V, y = grm_case()
+    X = c.X
+    ll, ld1, ld2, ypy = O.reml_loglik_dense(V, X, y)
+    # K: orthonormal basis of the complement of X; K^T y ~ N(0, K^T V K)
+    Qx, _ = np.linalg.qr(X, mode="complete")
+    K = Qx[:, X.shape[1]:]
+    S = K.T @ V @ K
+    z = K.T @ y
+    lc = -0.5 * (np.linalg.slogdet(S)[1] + z @ np.linalg.solve(S, z) + K.shape[1] * np.log(2 * np.pi))
+    assert abs(ll - (lc - 0.5 * np.linalg.slogdet(X.T @ X)[1])) < 1e-9 * abs(ll)
+
+
+# ------------------------------------------------------------------------------ GPU
+def _ctx_grm(factory, c, lt, theta):
+    g = factory()
+    g.set_cohort(c.indiv, c.times, c.X)
+    g.readFullGRM(lt)
+    g.set_varcomp(theta)
+    return g
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("theta", [(0.5, 0.8, 0.4, 0.1, 0.2), (0.7, 0.3, 0.2, 0.0, 0.0)])
+def test_exact_probes_give_dense_reml_loglik(theta, gpu_ctx_factory):
+    _say("loglik-exact", theta)
+    c, lt, G, _, y = grm_case()
+    V, _ = dense_V(c, G, theta)
+    g = _ctx_grm(gpu_ctx_factory, c, lt, theta)
+    r = g.reml_loglik(y, probes="exact", tol=1e-11)
+    ll, ld1, ld2, ypy = O.reml_loglik_dense(V, c.X, y)
+    gd = g.computeGeneticDiagonal()
+    assert abs(r["logdet_q"] - O.logdet_precond(c.indiv, c.times, gd, theta)) < 1e-11 * abs(r["logdet_q"])
+    assert r["probes"] == c.nrec + 2 * c.n and r["logdet_v_se"] == 0.0
+    assert abs(r["logdet_v"] - ld1) < 1e-8 * abs(ld1), (r["logdet_v"], ld1)
+    assert abs(r["logdet_xtvx"] - ld2) < 1e-8 * max(1.0, abs(ld2))
+    assert abs(r["ypy"] - ypy) < 1e-8 * ypy
+    assert abs(r["loglik"] - ll) < 1e-8 * abs(ll)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["rademacher", "gaussian"])
+def test_random_probes_equal_oracle_quadratic_forms(kind, gpu_ctx_factory):
+    _say("loglik-random", kind)
+    th = (0.5, 0.8, 0.4, 0.1, 0.2)
+    c, lt, G, V, y = grm_case(n=80)
+    g = _ctx_grm(gpu_ctx_factory, c, lt, th)
+    nprobe, seed = 100, 2026  # two block solves: 64 + 36 probes
+    r = g.reml_loglik(y, probes=kind, nprobe=nprobe, seed=seed, tol=1e-11)
+    gd = g.computeGeneticDiagonal()
+    Q = dense_Q(c, gd, th)
+    B = O.logdet_probes({"gaussian": 0, "rademacher": 1}[kind], c.indiv, c.times, gd, th, 0, nprobe, seed)
+    ex = O.slq_exact_terms(V, Q, B)
+    ldq = np.linalg.slogdet(Q)[1]
+    assert r["probes"] == nprobe
+    assert abs(r["logdet_v"] - (ldq + ex.mean())) < 1e-8 * abs(ldq + ex.mean())
+    assert abs(r["logdet_v_se"] - ex.std(ddof=1) / np.sqrt(nprobe)) < 1e-6 * r["logdet_v_se"]
+    # an unbiased estimate of log|V|
+    ld1 = np.linalg.slogdet(V)[1]
+    assert abs(r["logdet_v"] - ld1) < 5 * r["logdet_v_se"]
+
+
+@pytest.mark.gpu
+def test_mixed_loglik_within_1e4_of_fp64_on_genotypes(gpu_ctx_factory):
+    _say("loglik-mixed-geno")
+    d = synth.make_geno_case(n=400, m=2000, lam=2, seed=63, miss=0.01)
+    c = d.cohort
+    th = (0.5, 0.5, 0.3, 0.05, 0.2)
+    out = {}
+    for prec in (globe_amd.PREC_FP64, globe_amd.PREC_MIXED):
+        g = gpu_ctx_factory()
+        g.set_cohort(c.indiv, c.times, c.X)
+        g.prepGenotypeInput(d.bed_rows, 0.01, 0.2)
+        g.set_varcomp(th)
+        out[prec] = g.reml_loglik(d.y, probes="rademacher", nprobe=64, seed=5, tol=1e-8, precision=prec)
+    a, b = out[globe_amd.PREC_FP64], out[globe_amd.PREC_MIXED]
+    assert abs(a["loglik"] - b["loglik"]) < 1e-4 * abs(a["loglik"]), (a, b)
+    assert abs(a["logdet_q"] - b["logdet_q"]) < 1e-12 * abs(a["logdet_q"])
+
+
+@pytest.mark.gpu
+def test_loglik_rejects_bad_arguments(gpu_ctx_factory):
+    _say("loglik-errors")
+    c, lt, _, _, y = grm_case(n=20)
+    g = _ctx_grm(gpu_ctx_factory, c, lt, (0.5, 0.8, 0.1, 0.3, 0.2))  # sigb01^2 > sig2b0 sig2b1
+    with pytest.raises(globe_amd.GlobeError, match="sigb01"):
+        g.reml_loglik(y)
+    g.set_varcomp((0.5, 0.8, 0.4, 0.1, 0.2))
+    with pytest.raises(globe_amd.GlobeError, match="nprobe"):
+        g.reml_loglik(y, nprobe=1)
