@@ -10,7 +10,9 @@
 // the GRM; the reference's runAIREML only reads a GRM), --writeSolutions (also write
 // <output_prefix>_beta.bin, _uhat.bin and _bhat.bin: the GLS / BLUP solve at the fitted
 // components, the files the reference's commented-out saveVector calls name,
-// varCompEst.cpp:1493-1495).
+// varCompEst.cpp:1493-1495), --loglik n|exact (also print the REML log-likelihood at the
+// fitted components: log|V| by Lanczos quadrature over n Rademacher probe solves, or
+// exactly over every column of the preconditioner's factor; the reference prints none).
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -59,6 +61,7 @@ int main(int argc, char** argv) {
     double theta[5] = {0.5, 0.3, 0.2, 0.1, 0.1};  // VarCompDataStruct defaults (:47-51)
     int device = 0;
     bool write_solutions = false;
+    int loglik_probes = 0;  // 0: none, -1: exact
     std::string geno;
     for (int i = 4; i < argc; ++i) {
         const std::string a = argv[i];
@@ -79,6 +82,12 @@ int main(int argc, char** argv) {
         else if (a == "--device") device = std::stoi(next());
         else if (a == "--geno") geno = next();
         else if (a == "--writeSolutions") write_solutions = true;
+        else if (a == "--loglik") {
+            const std::string v = next();
+            loglik_probes = v == "exact" ? -1 : std::atoi(v.c_str());
+            if (!(loglik_probes == -1 || loglik_probes >= 2))
+                die("[Error] --loglik takes a probe count >= 2 or exact");
+        }
         else if (a == "--probes") {
             const std::string v = next();
             if (v == "gaussian") o.probes = 0;
@@ -152,6 +161,18 @@ int main(int argc, char** argv) {
         globe_io::save_vector(uhat, out + "_uhat.bin");
         globe_io::save_vector(bhat, out + "_bhat.bin");
         emit("Solutions written to " + out + "_beta.bin, _uhat.bin and _bhat.bin.");
+    }
+    if (loglik_probes != 0) {
+        check(globe_set_varcomp(ctx, theta), "variance components");
+        globe_pcg_opts lo = o.pcg;
+        lo.max_iter = 1000;
+        globe_loglik ll{};
+        check(globe_reml_loglik(ctx, c.y.data(), loglik_probes < 0 ? 2 : 1, loglik_probes, o.seed, &lo, &ll),
+              "log-likelihood");
+        std::snprintf(buf, sizeof(buf),
+                      "REML log-likelihood at the estimates: %.10g (log|V| = %.10g +/- %.3g, %d probes, %.3f s)",
+                      ll.loglik, ll.logdet_v, ll.logdet_v_se, ll.probes, ll.seconds);
+        emit(buf);
     }
     globe_destroy(ctx);
     g_log.close();

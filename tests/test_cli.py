@@ -142,6 +142,38 @@ def test_runaireml_write_solutions(case_files):
         assert rel(res[s], ro[s]) < 1e-4, s
 
 
+@pytest.mark.gpu
+def test_runaireml_loglik_exact(case_files):
+    """--loglik exact: the REML log-likelihood at the fitted components equals the dense
+    closed form at the components the log prints (6 significant digits)."""
+    p = case_files
+    out = os.path.join(p["dir"], "aireml_ll")
+    print("[case] runAIREML --loglik exact", flush=True)
+    r = run([AIREML, p["grm_input"], p["grm_grm"], out, "--maxiter", "2", "--numMC", "4", "--loglik", "exact",
+             "--pcgTol", "1e-10", "--sig2e", "0.5", "--sig2g", "0.5", "--sig2b0", "0.3", "--sigb01", "0.0",
+             "--sig2b1", "0.3"])
+    assert r.returncode == 0, r.stderr + r.stdout
+    log = open(out + ".log").read()
+    theta = []
+    for name in ("sig2e", "sig2g", "sig2b0", "sigb01", "sig2b1"):
+        line = next(ln for ln in log.splitlines() if ln.strip().startswith(name + " ="))
+        theta.append(float(line.split("=")[1].split("+/-")[0]))
+    line = next(ln for ln in log.splitlines() if ln.startswith("REML log-likelihood"))
+    ll = float(line.split(":")[1].split("(")[0])
+    d = load_golden("grm_n180")
+    c = d["cohort"]
+    G = synth.grm_dense(d["grm_lt"], c.n)
+    from helpers import dense_V
+    V, _ = dense_V(c, G, theta)
+    import oracle as O
+    ref = O.reml_loglik_dense(V, c.X, d["y"])[0]
+    # the printed components carry 6 digits; the log-likelihood moves by far less than
+    # 1e-6 of itself over that rounding near the fitted point
+    assert abs(ll - ref) < 1e-6 * abs(ref), (ll, ref)
+    r = run([AIREML, p["grm_input"], p["grm_grm"], out, "--maxiter", "1", "--loglik", "1"])
+    assert r.returncode == 1 and "--loglik" in r.stderr
+
+
 def test_makegrm_flag_and_file_errors(case_files, tmp_path):
     p = case_files
     r = run([MAKEGRM])
