@@ -298,6 +298,64 @@ def aireml_leg(args):
     return out
 
 
+def grm_leg(args):
+    """The --grm path at the genotype workload's cohort size (N = 200,000, ~1.6M
+    records): G = W W^T / M built on the device from args.grm_snps synthetic SNPs into
+    the HBM-resident tile-major FP32 lower triangle (80 GB), then PCG on the GRM product:
+    the FP64 1-RHS solve of y and 32-RHS block solves ([y | 31 Rademacher columns]) in
+    FP64 and mixed precision (int8 digit tiles of G, 160 GB, built by the first mixed
+    product). Runs after the genotype context is closed."""
+    import globe_amd as G
+    from globe_amd import synth
+    g = G.Globe(0)
+    rng = np.random.default_rng(args.seed + 13)
+    c = synth.simulate_cohort(args.n, args.lam, rng)
+    g.set_cohort(c.indiv, c.times, c.X)
+    kept = g.synth_bed(args.grm_snps, seed=args.seed + 13, f_lo=0.05, f_hi=0.5, miss=0.0,
+                       min_maf=0.0, max_miss=1.0)
+    theta = (0.6, 0.4, 0.4, 0.2, 0.2)
+    g.set_varcomp(theta)
+    u = np.sqrt(theta[1] / kept) * g.geno_Wa(rng.standard_normal(kept))
+    y = synth.simulate_phenotype(c, u, theta, rng)
+    t0 = time.perf_counter()
+    g.grm_from_geno()
+    t_build = time.perf_counter() - t0
+    log(f"grm: G {c.n} x {c.n} from {kept} SNPs built in {t_build:.2f}s")
+    nrec, k = c.nrec, args.k
+    B = np.asfortranarray(np.column_stack([y, rng.choice([-1.0, 1.0], size=(nrec, k - 1))]))
+    bbuf, xbuf = g.alloc(B.nbytes), g.alloc(B.nbytes)
+    bbuf.upload(B)
+    out = {"n": c.n, "nrec": nrec, "m_snps": int(kept), "build_s": t_build, "rhs": k, "tol": args.tol,
+           "grm_bytes": ((c.n + 63) // 64) * ((c.n + 63) // 64 + 1) // 2 * 4096 * 4}
+    runs = [("fp64_1rhs", 1, G.PREC_FP64, 2), ("fp64_block", k, G.PREC_FP64, 2), ("mixed_block", k, G.PREC_MIXED, 3)]
+    x0 = None
+    for name, kk, prec, which in runs:
+        try:
+            g.pcg_dev(bbuf, xbuf, kk, tol=args.tol, rule=G.RULE_PER_COLUMN, precision=prec)  # warm
+            g.profile(True)
+            st = g.pcg_dev(bbuf, xbuf, kk, tol=args.tol, rule=G.RULE_PER_COLUMN, precision=prec)
+            ks = g.kernel_stats(which)
+            g.profile(False)
+        except G.GlobeError as e:
+            out[name] = {"error": str(e)}
+            log(f"grm {name}: {e}")
+            continue
+        conv = int(np.sum(st["col_ratio"][:kk] < args.tol))
+        X = xbuf.download(np.zeros((nrec, k), order="F"))
+        r = {"solves_per_s": conv / st["seconds"], "seconds": st["seconds"], "iters": st["iters"],
+             "grm_pass_ms": ks["ms_avg"], "grm_passes": ks["launches"],
+             "grm_pass_GBps": ks["bytes"] / (ks["ms_avg"] * 1e-3) / 1e9 if ks["ms_avg"] else None}
+        if name == "fp64_1rhs":
+            x0 = X[:, 0].copy()
+        else:
+            r["y_rel_err_vs_fp64_1rhs"] = float(np.linalg.norm(X[:, 0] - x0) / np.linalg.norm(x0))
+        out[name] = r
+        log(f"grm {name}: {st['iters']} iters, {st['seconds']:.3f}s, {r['solves_per_s']:.2f} solves/s, "
+            f"GRM pass {ks['ms_avg']:.2f} ms x {ks['launches']}")
+    g.close()
+    return out
+
+
 def ingest_leg(g, c, args):
     """Genotype ingest from a PLINK file set and from a host buffer into a second
     context: the first args.ingest_snps kept rows of the resident synthetic cohort,
@@ -370,6 +428,8 @@ def main():
     ap.add_argument("--aireml-iters", type=int, default=3, help="AIREML iterations timed (0: skip)")
     ap.add_argument("--aireml-n", type=int, default=40_000)
     ap.add_argument("--aireml-m", type=int, default=80_000)
+    ap.add_argument("--grm-snps", type=int, default=20_000,
+                    help="SNPs of the device-built GRM of the N-individual --grm leg (0: skip)")
     ap.add_argument("--calib", action="store_true",
                     help="first run 3 plain 1 GiB device copies (counter calibration)")
     args = ap.parse_args()
@@ -499,8 +559,10 @@ def main():
         "aireml": aireml,
         "ingest": ingest,
     }
-    print(json.dumps(out))
     g.close()
+    if args.grm_snps > 0:
+        out["grm"] = grm_leg(args)
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":
