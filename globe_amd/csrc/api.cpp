@@ -125,6 +125,8 @@ void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int prec
             ensure_grm_digits(c);
             ensure_gdiag(c);
             c->w_grm8_part.ensure(grm_i8_part_bytes(n));
+        } else if (k <= 4) {
+            c->w_grm_sym.ensure(grm_sym_part_bytes(n, k));
         } else {
             c->w_grm_part.ensure(grm_product_part_bytes(n));
         }
@@ -140,12 +142,16 @@ void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int prec
             e.iter = c->prof_iter;
             e.bytes = 4.0 * (double)n * (n + 1) / 2;
             e.flops = 2.0 * (double)n * n * k;
-            e.issued = mixed ? 2.0 * np * np * cols * (L * (L + 1) / 2 + L - 1) : 2.0 * np * np * cols;
+            e.issued = mixed ? 2.0 * np * np * cols * (L * (L + 1) / 2 + L - 1)
+                             : (k <= 4 ? 2.0 * np * np * (k <= 2 ? k : 4) : 2.0 * np * np * cols);
             HIP_CHECK(hipEventRecord(e.a, s));
         }
         if (mixed)
             launch_grm_product_i8(s, c->d_grm8.as<int8_t>(), grm_digit_rexp(c->d_grm8_aux.p, n), n, S0, n, k,
                                   c->limbs_mixed, c->d_gdiag.as<double>(), th, GZ, n, skip, c->w_grm8_part.p);
+        else if (k <= 4)
+            launch_grm_product_sym(s, c->d_grm.as<float>(), n, S0, n, k, th, GZ, n, skip,
+                                   c->w_grm_sym.as<double>());
         else
             launch_grm_product(s, c->d_grm.as<float>(), n, S0, n, k, th, GZ, n, skip,
                                c->w_grm_part.as<double>());

@@ -33,6 +33,7 @@ struct globe_ctx {
     std::vector<float> qc_maf, h_freq;
     globe::DevBuf d_gdiag;
     globe::DevBuf w_grm_part;    // GRM product strip partials
+    globe::DevBuf w_grm_sym;     // few-column GRM product: row and transposed-tile partials
     // int8 digit tiles of the GRM for GLOBE_PREC_MIXED products, built on first use
     globe::DevBuf d_grm8, d_grm8_aux, w_grm8_part;
     bool grm8_ok = false;

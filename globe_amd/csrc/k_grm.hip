@@ -181,6 +181,119 @@ __global__ __launch_bounds__(256) void k_grm_finish(const double* __restrict__ p
     }
 }
 
+// Few-column product (k <= 4) that reads every stored tile once. A workgroup owns strip I
+// and one of kSymChunks ranges of J <= I (the longest strips are dispatched first); wave
+// w holds rows 16 w .. 16 w + 15 of each tile, lane = the tile column, so a row of a
+// tile is one 256-byte wave load. Per tile (I, J):
+//   rows:    acc[r][q] += G_IJ[r][lane] x_J[lane][q], summed over the lanes once per strip
+//            into rowp[h][q][i] (every workgroup writes its rows, zeros for an empty range);
+//   columns: for J < I, sum_r G_IJ[r][lane] x_I[r][q] over the 4 waves (LDS, fixed order)
+//            into colp[tile (I, J)][q][lane], the strip-J partial of the transposed tile.
+// The diagonal tile is stored whole and only contributes to rows. The finish adds the
+// chunk partials and the column partials of strip J in fixed order: deterministic.
+constexpr int kSymChunks = 8;
+
+template <int KC>
+__global__ __launch_bounds__(256) void k_grm_sym(const float* __restrict__ tiles, int32_t n, int32_t ntiles,
+                                                 const double* __restrict__ S0, int64_t ldn, int kk,
+                                                 double* __restrict__ rowp, double* __restrict__ colp,
+                                                 const int* skip) {
+    if (skip && *skip) return;
+    const int32_t I = ntiles - 1 - (int32_t)blockIdx.x;
+    const int h = blockIdx.y;
+    const int32_t J0 = (int32_t)((int64_t)h * (I + 1) / kSymChunks);
+    const int32_t J1 = (int32_t)((int64_t)(h + 1) * (I + 1) / kSymChunks);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __shared__ double xi[kGT][KC];
+    __shared__ double cred[2][4][kGT][KC];
+    for (int e = threadIdx.x; e < kGT * KC; e += 256) {
+        const int r = e / KC, q = e % KC;
+        const int64_t i = (int64_t)I * kGT + r;
+        xi[r][q] = (q < kk && i < n) ? S0[q * ldn + i] : 0.0;
+    }
+    __syncthreads();
+    double acc[16][KC];
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr)
+#pragma unroll
+        for (int q = 0; q < KC; ++q) acc[rr][q] = 0.0;
+    for (int32_t J = J0; J < J1; ++J) {
+        const float* t = tiles + tile_index(I, J) * (kGT * kGT) + (16 * w) * kGT + lane;
+        float gv[16];
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr) gv[rr] = t[rr * kGT];
+        const int64_t j = (int64_t)J * kGT + lane;
+        double xj[KC], cp[KC];
+#pragma unroll
+        for (int q = 0; q < KC; ++q) {
+            xj[q] = (q < kk && j < n) ? S0[q * ldn + j] : 0.0;
+            cp[q] = 0.0;
+        }
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr) {
+            const double g = (double)gv[rr];
+#pragma unroll
+            for (int q = 0; q < KC; ++q) {
+                acc[rr][q] += g * xj[q];
+                cp[q] += g * xi[16 * w + rr][q];
+            }
+        }
+        if (J < I) {
+            // double-buffered: the reduction of tile J reads cred[J & 1] while tile J + 1
+            // fills the other half; the barrier of J + 1 orders it before J + 2 refills
+            const int b = J & 1;
+#pragma unroll
+            for (int q = 0; q < KC; ++q) cred[b][w][lane][q] = cp[q];
+            __syncthreads();
+            if (threadIdx.x < kGT) {
+                double* dst = colp + tile_index(I, J) * (kGT * KC);
+#pragma unroll
+                for (int q = 0; q < KC; ++q)
+                    dst[q * kGT + lane] = ((cred[b][0][lane][q] + cred[b][1][lane][q]) + cred[b][2][lane][q]) +
+                                          cred[b][3][lane][q];
+            }
+        }
+    }
+    // each of the wave's 16 rows: the sum over the 64 lanes
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) {
+            double v = acc[rr][q];
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+            const int64_t i = (int64_t)I * kGT + 16 * w + rr;
+            if (lane == 0 && q < kk && i < n) rowp[((int64_t)h * KC + q) * n + i] = v;
+        }
+    }
+}
+
+// GZ[q][i] = sig2g_q (sum_h rowp[h][q][i] + sum_{I > J} colp[tile (I, J)][q][a]) for the
+// rows i = 64 J + a of strip J = blockIdx.x; wave w adds I = J + 1 + w, J + 5 + w, ...
+template <int KC>
+__global__ __launch_bounds__(256) void k_grm_sym_finish(const double* __restrict__ rowp,
+                                                        const double* __restrict__ colp, int32_t n,
+                                                        int32_t ntiles, const double* __restrict__ th,
+                                                        double* __restrict__ GZ, int64_t ldg, const int* skip) {
+    if (skip && *skip) return;
+    const int32_t J = blockIdx.x;
+    const int q = blockIdx.y;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __shared__ double red[4][kGT];
+    double v = 0.0;
+    for (int32_t I = J + 1 + w; I < ntiles; I += 4) v += colp[tile_index(I, J) * (kGT * KC) + q * kGT + lane];
+    red[w][lane] = v;
+    __syncthreads();
+    if (w != 0) return;
+    const int64_t i = (int64_t)J * kGT + lane;
+    if (i >= n) return;
+    double r = 0.0;
+#pragma unroll
+    for (int h = 0; h < kSymChunks; ++h) r += rowp[((int64_t)h * KC + q) * n + i];
+    const double c = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+    GZ[q * ldg + i] = th[q * 5 + 1] * (r + c);
+}
+
 // Packed rows r0 .. r1 - 1 of the triangle (lt holds them from lt_index(r0, 0) on) into
 // the tiles; one block per row. The diagonal tile gets both (a, b) and (b, a).
 __global__ __launch_bounds__(256) void k_grm_tiles_from_lt(const float* __restrict__ lt, int64_t r0,
@@ -289,6 +402,33 @@ void launch_grm_product(hipStream_t s, const float* tiles, int32_t n, const doub
                                                   skip);
         HIP_CHECK(hipGetLastError());
     }
+}
+
+size_t grm_sym_part_bytes(int32_t n, int k) {
+    const size_t T = (size_t)(n + kGT - 1) / kGT;
+    const size_t kc = k <= 1 ? 1 : (k <= 2 ? 2 : 4);
+    return sizeof(double) * kc * ((size_t)kSymChunks * (size_t)n + T * (T + 1) / 2 * kGT);
+}
+
+void launch_grm_product_sym(hipStream_t s, const float* tiles, int32_t n, const double* S0, int64_t ldn, int k,
+                            const double* theta_cols, double* GZ, int64_t ldg, const int* skip, double* scratch) {
+    if (n == 0) return;
+    const int32_t ntiles = (n + kGT - 1) / kGT;
+    const int kc = k <= 1 ? 1 : (k <= 2 ? 2 : 4);
+    double* rowp = scratch;
+    double* colp = scratch + (size_t)kSymChunks * kc * n;
+    const dim3 g(ntiles, kSymChunks), gf(ntiles, k);
+    if (kc == 1) {
+        k_grm_sym<1><<<g, 256, 0, s>>>(tiles, n, ntiles, S0, ldn, k, rowp, colp, skip);
+        k_grm_sym_finish<1><<<gf, 256, 0, s>>>(rowp, colp, n, ntiles, theta_cols, GZ, ldg, skip);
+    } else if (kc == 2) {
+        k_grm_sym<2><<<g, 256, 0, s>>>(tiles, n, ntiles, S0, ldn, k, rowp, colp, skip);
+        k_grm_sym_finish<2><<<gf, 256, 0, s>>>(rowp, colp, n, ntiles, theta_cols, GZ, ldg, skip);
+    } else {
+        k_grm_sym<4><<<g, 256, 0, s>>>(tiles, n, ntiles, S0, ldn, k, rowp, colp, skip);
+        k_grm_sym_finish<4><<<gf, 256, 0, s>>>(rowp, colp, n, ntiles, theta_cols, GZ, ldg, skip);
+    }
+    HIP_CHECK(hipGetLastError());
 }
 
 size_t grm_product_part_bytes(int32_t n) {
