@@ -278,7 +278,8 @@ typedef struct {
  * the second term by stochastic Lanczos quadrature -- probes b = F u with F F^T = Q,
  * solved V x = b by the block PCG (per-column rule, x0 = 0, opts->tol / max_iter /
  * precision), each column's alpha/beta giving the Lanczos matrix T and the estimate
- * (b^T Q^-1 b) e1^T log(T) e1. kind 0: Gaussian u, 1: Rademacher u (nprobe >= 2 each,
+ * (b^T Q^-1 b) e1^T (log(T) - T + I) e1 (the term T - I has the known mean tr(Q^-1 V) -
+ * Nrec = 0 and cuts the spread several-fold). kind 0: Gaussian u, 1: Rademacher u (nprobe >= 2 each,
  * estimate = their mean); 2: exact -- the Nrec + 2N columns of F (nprobe ignored), whose
  * sum is log|Q^-1/2 V Q^-1/2| itself up to the solves' tolerance. Probes come from
  * Philox4x32-10 keyed by `seed`. The fixed effects are solved as globe_gls_blup does.

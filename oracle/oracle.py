@@ -655,15 +655,18 @@ def logdet_probes(kind, indiv, times, gdiag, theta, first, count, seed):
     return out
 
 
-def slq_exact_terms(V, Q, B):
+def slq_exact_terms(V, Q, B, control=False):
     """Per probe b (columns of B) the quadratic form the Lanczos quadrature converges
     to: u^T log(A) u with u = Q^-1/2 b and A = Q^-1/2 V Q^-1/2 (dense eigendecompositions;
-    any square root of Q gives the same value, as u u^T -> Q^-1/2 b b^T Q^-1/2)."""
+    any square root of Q gives the same value, as u u^T -> Q^-1/2 b b^T Q^-1/2).
+    control=True: u^T (log(A) - A + I) u, the device's estimator (tr(A - I) = 0 when Q's
+    blocks are V's diagonal blocks)."""
     w, E = np.linalg.eigh(Q)
     Qh = (E / np.sqrt(w)) @ E.T
     lam, Z = np.linalg.eigh(Qh @ V @ Qh)
     U = Z.T @ (Qh @ B)
-    return (np.log(lam)[:, None] * U * U).sum(axis=0)
+    f = np.log(lam) - (lam - 1.0) if control else np.log(lam)
+    return (f[:, None] * U * U).sum(axis=0)
 
 
 def pcg_lanczos(V, Qinv, b, tol, max_iter=1000):
@@ -690,6 +693,11 @@ def pcg_lanczos(V, Qinv, b, tol, max_iter=1000):
         p = z + be[-1] * p
         rz = rz2
     return rho0, np.array(al), np.array(be)
+
+
+def lanczos_t00(alpha):
+    """T_00 = 1/alpha_0 = z0^T V z0 / r0.z0: the Rayleigh quotient of A at u."""
+    return 1.0 / float(np.asarray(alpha)[0])
 
 
 def lanczos_log_quadrature(alpha, beta):

@@ -54,6 +54,9 @@ def test_oracle_probe_factor_and_logdet_q():
     # the exact probes sum to log|Q^-1/2 V Q^-1/2| = log|V| - log|Q|
     tot = O.slq_exact_terms(V, Q, F).sum() + np.linalg.slogdet(Q)[1]
     assert abs(tot - np.linalg.slogdet(V)[1]) < 1e-9 * abs(np.linalg.slogdet(V)[1])
+    # and so do the control-variate forms (their A - I terms sum to tr(A - I) = 0)
+    tot = O.slq_exact_terms(V, Q, F, control=True).sum() + np.linalg.slogdet(Q)[1]
+    assert abs(tot - np.linalg.slogdet(V)[1]) < 1e-9 * abs(np.linalg.slogdet(V)[1])
     # a zero slope variance: the second column of each K_i vanishes, F F^T = Q still
     th0 = (0.5, 0.8, 0.4, 0.0, 0.0)
     F0 = O.logdet_probes(2, c.indiv, c.times, gd, th0, 0, c.nrec + 2 * c.n, 1)
@@ -72,6 +75,15 @@ def test_oracle_lanczos_quadrature_equals_dense_quadratic_form():
         rho0, al, be = O.pcg_lanczos(V, Qi, B[:, j], 1e-12)
         assert abs(rho0 - B[:, j] @ Qi @ B[:, j]) < 1e-10 * rho0
         assert abs(rho0 * O.lanczos_log_quadrature(al, be) - exact[j]) < 1e-9 * max(1.0, abs(exact[j]))
+    # the control variate: rho0 (T_00 - 1) = u^T (A - I) u, so the device's estimator
+    # rho0 (e1^T log(T) e1 - (T_00 - 1)) converges to the dense u^T (log A - A + I) u
+    excv = O.slq_exact_terms(V, Q, B, control=True)
+    for j in range(B.shape[1]):
+        rho0, al, be = O.pcg_lanczos(V, Qi, B[:, j], 1e-12)
+        cv = rho0 * (O.lanczos_log_quadrature(al, be) - (O.lanczos_t00(al) - 1.0))
+        assert abs(cv - excv[j]) < 1e-9 * max(1.0, abs(excv[j]))
+    # its mean is exact: tr(Q^-1 V) = Nrec (Q_i is V's diagonal block)
+    assert abs(np.trace(Qi @ V) - c.nrec) < 1e-9 * c.nrec
     # a plausible slip (T_jj without the beta/alpha term) moves the quadrature off
     rho0, al, be = O.pcg_lanczos(V, Qi, B[:, 0], 1e-12)
     from scipy.linalg import eigh_tridiagonal
@@ -144,14 +156,16 @@ def test_random_probes_equal_oracle_quadratic_forms(kind, gpu_ctx_factory):
     gd = g.computeGeneticDiagonal()
     Q = dense_Q(c, gd, th)
     B = O.logdet_probes({"gaussian": 0, "rademacher": 1}[kind], c.indiv, c.times, gd, th, 0, nprobe, seed)
-    ex = O.slq_exact_terms(V, Q, B)
+    ex = O.slq_exact_terms(V, Q, B, control=True)
     ldq = np.linalg.slogdet(Q)[1]
     assert r["probes"] == nprobe
     assert abs(r["logdet_v"] - (ldq + ex.mean())) < 1e-8 * abs(ldq + ex.mean())
     assert abs(r["logdet_v_se"] - ex.std(ddof=1) / np.sqrt(nprobe)) < 1e-6 * r["logdet_v_se"]
-    # an unbiased estimate of log|V|
+    # an unbiased estimate of log|V|, with a spread several times below that of the
+    # plain log quadratic forms of the same probes
     ld1 = np.linalg.slogdet(V)[1]
     assert abs(r["logdet_v"] - ld1) < 5 * r["logdet_v_se"]
+    assert r["logdet_v_se"] < 0.5 * O.slq_exact_terms(V, Q, B).std(ddof=1) / np.sqrt(nprobe)
 
 
 @pytest.mark.gpu
