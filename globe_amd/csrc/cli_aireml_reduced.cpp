@@ -8,7 +8,9 @@
 // writes no other file, as the reference. Unknown flags are ignored (as :113-128).
 // Opt-in extensions: --probes gaussian|rademacher|exact (default gaussian: the
 // reference's tilde-V probes), --seed n, --pcgTol v (1e-5), --precision
-// fp64|mixed|fp64-valu, --device d.
+// fp64|mixed|fp64-valu, --device d, --loglik n|exact (also print the REML log-likelihood
+// at the fitted components: log|V| by Lanczos quadrature over n Rademacher probe solves,
+// or exactly; the reference prints none).
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -55,6 +57,7 @@ int main(int argc, char** argv) {
     o.pcg = globe_pcg_opts{1e-5, 1000, GLOBE_X0_PRECOND, GLOBE_RULE_ALL_COLUMNS, 0, GLOBE_PREC_FP64, 0};
     double theta[2] = {0.5, 0.5};  // VarCompDataStruct defaults (:43-44)
     int device = 0;
+    int loglik_probes = 0;  // 0: none, -1: exact
     for (int i = 4; i < argc; ++i) {
         const std::string a = argv[i];
         auto next = [&]() -> std::string {
@@ -69,7 +72,12 @@ int main(int argc, char** argv) {
         else if (a == "--seed") o.seed = std::stoull(next());
         else if (a == "--pcgTol") o.pcg.tol = std::stod(next());
         else if (a == "--device") device = std::stoi(next());
-        else if (a == "--probes") {
+        else if (a == "--loglik") {
+            const std::string v = next();
+            loglik_probes = v == "exact" ? -1 : std::atoi(v.c_str());
+            if (!(loglik_probes == -1 || loglik_probes >= 2))
+                die("[Error] --loglik takes a probe count >= 2 or exact");
+        } else if (a == "--probes") {
             const std::string v = next();
             if (v == "gaussian") o.probes = 0;
             else if (v == "rademacher") o.probes = 1;
@@ -118,6 +126,19 @@ int main(int argc, char** argv) {
     char buf[256];
     std::snprintf(buf, sizeof(buf), "AIREML: %d iterations, %.4f s per iteration", iters, spi);
     emit(buf);
+    if (loglik_probes != 0) {
+        const double th5[5] = {theta[0], theta[1], 0.0, 0.0, 0.0};
+        check(globe_set_varcomp(ctx, th5), "variance components");
+        globe_pcg_opts lo = o.pcg;
+        lo.max_iter = 1000;
+        globe_loglik ll{};
+        check(globe_reml_loglik(ctx, c.y.data(), loglik_probes < 0 ? 2 : 1, loglik_probes, o.seed, &lo, &ll),
+              "log-likelihood");
+        std::snprintf(buf, sizeof(buf),
+                      "REML log-likelihood at the estimates: %.10g (log|V| = %.10g +/- %.3g, %d probes, %.3f s)",
+                      ll.loglik, ll.logdet_v, ll.logdet_v_se, ll.probes, ll.seconds);
+        emit(buf);
+    }
     globe_destroy(ctx);
     g_log.close();
     return 0;

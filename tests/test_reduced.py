@@ -254,7 +254,7 @@ def test_reduced_cli_runs_exact_and_logs(case, tmp_path):
     synth.write_grm_bin(grm, d.grm_lt)
     out = str(tmp_path / "o")
     r = _run([CLI, inp, grm, out, "--numMC", str(d.n), "--probes", "exact", "--tol", "1e-6",
-              "--pcgTol", "1e-10", "--sig2e", "0.5", "--sig2g", "0.5", "--bogus"])
+              "--pcgTol", "1e-10", "--sig2e", "0.5", "--sig2g", "0.5", "--bogus", "--loglik", "exact"])
     assert r.returncode == 0, r.stderr + r.stdout
     log = open(out + ".log").read()
     assert log.startswith("Command:") and "Final variance components" in log
@@ -263,3 +263,9 @@ def test_reduced_cli_runs_exact_and_logs(case, tmp_path):
     est = [ln for ln in log.splitlines() if ln.startswith("Estimated variance components:")][-1]
     vals = [float(x.split("=")[1]) for x in est.split(":", 1)[1].split(",")]
     assert rel(vals, th) < 1e-5
+    # --loglik exact: the REML log-likelihood at the printed estimates (dense closed form
+    # plus the -(n - q)/2 log 2 pi constant the build reports)
+    ln = next(x for x in log.splitlines() if x.startswith("REML log-likelihood"))
+    ll = float(ln.split(":")[1].split("(")[0])
+    want = reml_loglik(np.array(vals), d) - 0.5 * (d.n - d.covars.shape[1]) * np.log(2 * np.pi)
+    assert abs(ll - want) < 1e-6 * abs(want), (ll, want)
