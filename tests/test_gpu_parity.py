@@ -335,6 +335,34 @@ def test_grm_product_widths(k, gpu_ctx_factory):
     assert rel(g.computeVx_multi(x, multGRM_uhat=True), orc.vx(x, grm_only=True)) < 1e-12
 
 
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_grm_few_column_product_multi_tile(k, gpu_ctx_factory):
+    """Products of k <= 4 columns read each stored tile once (row strip in registers,
+    transposed strip through per-tile partials). n = 3,000: 47 tile strips, so each of a
+    strip's 8 chunks holds several tiles and both halves of the column-partial double
+    buffer are used. Against sig2g G J^T x from the dense G, and against the same columns
+    of a 16-column product (the strip kernel)."""
+    _say("grm-few-columns", k)
+    n = 3000
+    rng = np.random.default_rng(40 + k)
+    c = synth.simulate_cohort(n, 0.5, rng)
+    gcase = synth.make_reduced_case(n=n, m=500, seed=41)
+    G = synth.grm_dense(gcase.grm_lt, n)
+    theta = (0.6, 0.4, 0.4, 0.2, 0.2)
+    g = gpu_ctx_factory()
+    g.set_cohort(c.indiv, c.times, c.X)
+    g.readFullGRM(gcase.grm_lt)
+    g.set_varcomp(theta)
+    x = rng.standard_normal((c.nrec, 16))
+    J = np.zeros((c.nrec, n))
+    J[np.arange(c.nrec), c.indiv] = 1.0
+    want = theta[1] * G @ (J.T @ x[:, :k])
+    got = g.computeVx_multi(np.asfortranarray(x[:, :k]), multGRM_uhat=True)
+    assert rel(got, want) < 1e-12
+    wide = g.computeVx_multi(np.asfortranarray(x), multGRM_uhat=True)[:, :k]
+    assert rel(got, wide) < 1e-13
+
+
 def test_grm_builder_at_40k_rows_offsets(gpu_ctx_factory):
     """The builder forms only rows i >= j of each tile column, from a row offset (a
     multiple of 4096) on; fewer rows let the contraction take more K-splits than the
