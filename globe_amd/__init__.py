@@ -133,6 +133,8 @@ def load_library(path: str = LIB_PATH):
         "globe_get_geno_rows": ([P, i64, i64, P, P], C.c_int),
         "globe_dev_copy": ([P, P, P, i64], C.c_int),
         "globe_profile_enable": ([P, C.c_int], C.c_int),
+        "globe_force_variant": ([P, C.c_int], C.c_int),
+        "globe_variant_counts": ([P, P], C.c_int),
         "globe_set_mixed_digits": ([P, C.c_int], C.c_int),
         "globe_profile_read": ([P, C.c_int, P], C.c_int),
         "globe_aireml_step": ([P, P, P, dbl, P], C.c_int),
@@ -173,6 +175,30 @@ def _cm(a, dtype=np.float64):
 
 def device_count() -> int:
     return load_library().globe_device_count()
+
+
+REPLAY_PATH = os.path.join(HERE, "libglobe_plan_replay.so")
+
+
+def plan_replay(digit_cols, m, n, r0=0, ncu=256, thin_bpc=(1,) * 8, shrink=(0, 0, 0)):
+    """Host replay of both contraction passes of one product shape over every tuner
+    candidate (libglobe_plan_replay.so; no GPU): raises AssertionError with the first
+    read or store outside its buffer, or a partial slot left unwritten; returns the
+    replay's counters. shrink: bytes taken off the packed layouts, the digit matrix and
+    the partial buffer (to show that one access too many is caught)."""
+    L = C.CDLL(REPLAY_PATH)
+    f = L.globe_plan_replay
+    f.argtypes = [C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                  C.c_char_p, C.c_int]
+    f.restype = C.c_int
+    bpc = np.ascontiguousarray(thin_bpc, dtype=np.int32)
+    shr = np.ascontiguousarray(shrink, dtype=np.int64)
+    st = np.zeros(8, dtype=np.int64)
+    msg = C.create_string_buffer(512)
+    rc = f(int(digit_cols), int(m), int(n), int(r0), int(ncu), _p(bpc), _p(shr), _p(st), msg, 512)
+    assert rc == 0, msg.value.decode()
+    return {"cand1": int(st[0]), "cand2": int(st[1]), "blocks": int(st[2]), "row_margin1": int(st[3]),
+            "row_margin2": int(st[4]), "digit_bytes": int(st[5]), "part_bytes": int(st[6]), "wide": bool(st[7])}
 
 
 class DeviceBuffer:
@@ -446,6 +472,18 @@ class Globe:
     def set_mixed_digits(self, digits):
         """int8 digits (2..4) of mixed-precision genetic products outside a PCG solve."""
         self._check(self.L.globe_set_mixed_digits(self.h, int(digits)))
+
+    # ---- contraction variants
+    def force_variant(self, index):
+        """index >= 0: every genotype contraction runs candidate index % count of its
+        shape's tuner candidates (untimed); -1 returns to the tuned variants."""
+        self._check(self.L.globe_force_variant(self.h, int(index)))
+
+    def variant_counts(self):
+        """Candidates of pass 1 and pass 2 of the last genotype contraction."""
+        out = np.zeros(2, dtype=np.int32)
+        self._check(self.L.globe_variant_counts(self.h, _p(out)))
+        return int(out[0]), int(out[1])
 
     # ---- kernel timing
     def profile(self, on=True):

@@ -66,6 +66,11 @@ def build(verbose: bool = True, jobs: int = 8) -> str:
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp,
           *[objs[s] for s in LIB_SOURCES]])
     os.replace(tmp, LIB)
+    # host replay of the contraction's launch plans (a checker; no device code)
+    rep = os.path.join(HERE, "libglobe_plan_replay.so")
+    _run([HIPCC, "-O2", "-std=c++17", "-fPIC", "-shared", "-o", rep + ".tmp",
+          os.path.join(CSRC, "plan_replay.cpp")])
+    os.replace(rep + ".tmp", rep)
     for exe, src in CLIS.items():
         if src not in objs:
             continue

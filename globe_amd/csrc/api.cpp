@@ -493,6 +493,7 @@ int globe_set_cohort(globe_ctx* c, int64_t nrec, int32_t n, int32_t p, const int
     API_BEGIN(c)
     require(nrec > 0 && n > 0 && p >= 0 && indiv && times && X && ldx >= nrec, GLOBE_E_INVALID,
             "globe_set_cohort: bad sizes or null pointers");
+    require(c->serve == nullptr, GLOBE_E_STATE, "globe_set_cohort: close the serving loop first");
     // blocks = runs of equal individual (PCGsolver.cpp:1004-1011); each individual
     // must form exactly one run and every index 0..n-1 must occur.
     std::vector<int64_t> off;
@@ -571,9 +572,23 @@ int globe_set_cohort(globe_ctx* c, int64_t nrec, int32_t n, int32_t p, const int
         }
     }
     c->has_cohort = true;
-    // genetics must match n
-    if ((c->gen == 1 || c->gen == 2) && c->gdiag_ok && c->d_gdiag.bytes < sizeof(double) * (size_t)n)
+    c->fit_valid = false;
+    // Genotypes and GRMs are laid out for the N they were staged with (row pitch, row
+    // count, tiles, diag(G)); a cohort of another size releases them, so the next
+    // product fails with GLOBE_E_STATE instead of indexing buffers of the old size.
+    if (c->gen != 0 && c->gen_n != n) {
+        c->gen = 0;
+        c->gen_n = 0;
+        c->d_snp.release();
+        c->d_ind.release();
+        c->d_grm.release();
+        c->d_grm8.release();
+        c->d_grm8_aux.release();
+        c->m = 0;
+        c->grm8_ok = false;
         c->gdiag_ok = false;
+        c->tuned.clear();
+    }
     API_END(c)
 }
 
@@ -665,6 +680,7 @@ int globe_load_grm(globe_ctx* c, const float* lt) {
     HIP_CHECK(hipMemcpy(c->d_gdiag.p, gd.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     c->gdiag_ok = true;
     c->gen = 2;
+    c->gen_n = (int32_t)n;
     c->grm8_ok = false;
     API_END(c)
 }
@@ -690,6 +706,7 @@ int globe_grm_from_geno(globe_ctx* c) {
     HIP_CHECK(hipStreamSynchronize(s));
     // the context now works from the GRM, as after globe_load_grm
     c->gen = 2;
+    c->gen_n = n;
     c->grm8_ok = false;
     c->d_snp.release();
     c->d_ind.release();
@@ -1204,6 +1221,21 @@ int globe_memcpy_d2h(globe_ctx* c, void* dst, const void* src, int64_t bytes) {
     require(dst && src && bytes >= 0, GLOBE_E_INVALID, "globe_memcpy_d2h: bad arguments");
     HIP_CHECK(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
+    API_END(c)
+}
+
+int globe_force_variant(globe_ctx* c, int index) {
+    API_BEGIN(c)
+    require(index >= -1, GLOBE_E_INVALID, "variant index must be >= -1");
+    c->force_variant = index;
+    API_END(c)
+}
+
+int globe_variant_counts(globe_ctx* c, int32_t counts[2]) {
+    API_BEGIN(c)
+    require(counts != nullptr, GLOBE_E_INVALID, "null counts");
+    counts[0] = c->variant_count[0];
+    counts[1] = c->variant_count[1];
     API_END(c)
 }
 

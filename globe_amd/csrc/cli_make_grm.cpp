@@ -24,8 +24,12 @@
 
 namespace {
 
+// the partial output file, removed when the run fails
+std::string g_partial;
+
 [[noreturn]] void die(const std::string& msg) {
     std::cerr << msg << std::endl;
+    if (!g_partial.empty()) std::remove(g_partial.c_str());
     std::exit(EXIT_FAILURE);
 }
 
@@ -75,9 +79,13 @@ int main(int argc, char** argv) {
     } catch (const std::exception& e) {
         die(e.what());
     }
+    // written under a temporary name and renamed once complete, so a failed run never
+    // leaves a truncated GRM under the final name
     const std::string path = output + ".grm.bin";
-    std::ofstream f(path, std::ios::binary);
-    if (!f.is_open()) die("[Error] Error opening output file: " + path);
+    const std::string tmp = path + ".tmp";
+    std::ofstream f(tmp, std::ios::binary);
+    if (!f.is_open()) die("[Error] Error opening output file: " + tmp);
+    g_partial = tmp;
 
     globe_ctx* ctx = nullptr;
     if (globe_create(device, &ctx) != GLOBE_OK)
@@ -104,11 +112,13 @@ int main(int argc, char** argv) {
         buf.resize(cnt);
         check(ctx, globe_get_grm_rows(ctx, r0, r1, buf.data()), "GRM read-out");
         f.write(reinterpret_cast<const char*>(buf.data()), (std::streamsize)(cnt * sizeof(float)));
-        if (!f) die("[Error] Error writing " + path);
+        if (!f) die("[Error] Error writing " + tmp);
         r0 = r1;
     }
     f.close();
-    if (!f) die("[Error] Error writing " + path);
+    if (!f) die("[Error] Error writing " + tmp);
+    if (std::rename(tmp.c_str(), path.c_str()) != 0) die("[Error] Error renaming " + tmp + " to " + path);
+    g_partial.clear();
     const double t_all = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::cout << "[makeGRM] Wrote " << path << " (" << (size_t)n * (n + 1) / 2 * sizeof(float) << " bytes)";
     if (debug) std::cout << ": ingest + build " << t_build << " s, total " << t_all << " s";

@@ -24,8 +24,9 @@ struct globe_ctx {
     globe::DevBuf d_tile_blk, d_rec_blk;  // record tiles
     globe::RecMeta meta;                  // the cohort's record-space structure
 
-    // genetics: 0 none, 1 packed genotypes, 2 GRM
+    // genetics: 0 none, 1 packed genotypes, 2 GRM; gen_n = the N they were staged for
     int gen = 0;
+    int32_t gen_n = 0;
     int64_t m_total = 0, m = 0, pitch_s = 0, pitch_i = 0;
     globe::DevBuf d_snp, d_ind, d_freq, d_twof, d_inv, d_grm;
     std::vector<int64_t> qc_alt, qc_miss;
@@ -67,6 +68,11 @@ struct globe_ctx {
         int64_t grid;
     };
     std::map<std::string, Tuned> tuned;
+    // validation of the candidates: >= 0 runs candidate force_variant % count of every
+    // contraction instead of the tuned one; variant_count = candidates of the last
+    // contraction of pass 1 and pass 2
+    int force_variant = -1;
+    int variant_count[2] = {0, 0};
     int cu_count = 0;  // compute units of the device (stream-K grids)
 
     // PCG workspaces and state

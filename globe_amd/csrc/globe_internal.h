@@ -44,6 +44,10 @@ inline int parts_for(int64_t count) {
     return (int)(b > kMaxParts ? kMaxParts : b);
 }
 inline int64_t round_up(int64_t a, int64_t m) { return (a + m - 1) / m * m; }
+// Packed 2-bit layouts: bytes per row holding `count` genotypes, and rows allocated for
+// `rows` rows (zero rows past the last one, so row blocks read without bounds checks)
+inline int64_t packed_pitch(int64_t count) { return round_up((count + 3) / 4, kPitchAlign); }
+inline int64_t packed_rows(int64_t rows) { return round_up(rows > 1 ? rows : 1, kRowPad); }
 
 // Device-side PCG state: scalars then per-column arrays (k <= kMaxCols).
 struct PcgState {

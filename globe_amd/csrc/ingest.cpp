@@ -95,12 +95,12 @@ void stage_rows(globe_ctx* c, int64_t m, int64_t pitch_in, float min_maf, float 
     c->gen = 0;
     c->gdiag_ok = false;
     c->d_grm.release();
-    c->pitch_s = round_up((n + 3) / 4, kPitchAlign);
+    c->pitch_s = packed_pitch(n);
     c->d_snp.release();
     c->d_ind.release();
     // rows are padded to a multiple of kRowPad with zero (dosage-0) bytes so the
     // contraction kernels load whole row blocks without bounds checks
-    c->d_snp.ensure((size_t)round_up(std::max<int64_t>(m, 1), kRowPad) * c->pitch_s);
+    c->d_snp.ensure((size_t)packed_rows(m) * c->pitch_s);
     // the synthetic generator seeds each 1 GiB chunk of rows, so its chunking is fixed
     const int64_t target = hf ? c->ingest_chunk : (1ll << 30);
     const int64_t chunk_rows = std::max<int64_t>(1, std::min<int64_t>(m, target / pitch_in));
@@ -171,13 +171,13 @@ void stage_rows(globe_ctx* c, int64_t m, int64_t pitch_in, float min_maf, float 
     c->m_total = m;
     c->m = kept;
     {
-        const int64_t pad_rows = round_up(std::max<int64_t>(kept, 1), kRowPad) - kept;
+        const int64_t pad_rows = packed_rows(kept) - kept;
         HIP_CHECK(hipMemsetAsync(c->d_snp.as<uint8_t>() + kept * c->pitch_s, 0,
                                  (size_t)pad_rows * c->pitch_s, s));
     }
-    c->pitch_i = round_up((kept + 3) / 4, kPitchAlign);
-    c->d_ind.ensure((size_t)round_up(n, kRowPad) * c->pitch_i);
-    HIP_CHECK(hipMemsetAsync(c->d_ind.p, 0, (size_t)round_up(n, kRowPad) * c->pitch_i, s));
+    c->pitch_i = packed_pitch(kept);
+    c->d_ind.ensure((size_t)packed_rows(n) * c->pitch_i);
+    HIP_CHECK(hipMemsetAsync(c->d_ind.p, 0, (size_t)packed_rows(n) * c->pitch_i, s));
     launch_transpose_geno(s, c->d_snp.as<uint8_t>(), kept, c->pitch_s, n, c->d_ind.as<uint8_t>(), c->pitch_i);
     c->d_freq.ensure(sizeof(float) * std::max<int64_t>(kept, 1));
     c->d_twof.ensure(sizeof(double) * std::max<int64_t>(kept, 1));
@@ -188,6 +188,7 @@ void stage_rows(globe_ctx* c, int64_t m, int64_t pitch_in, float min_maf, float 
     }
     HIP_CHECK(hipStreamSynchronize(s));
     c->gen = 1;
+    c->gen_n = n;
     c->ingest_stats[0] = now_s() - t0;
     c->ingest_stats[1] = t_fill;
     c->ingest_stats[2] = t_wait;
@@ -307,7 +308,7 @@ int globe_synth_bed(globe_ctx* c, int64_t m, uint64_t seed, float f_lo, float f_
     require(m >= 0 && f_lo >= 0.f && f_hi <= 1.f && f_lo <= f_hi && miss >= 0.f && miss <= 1.f,
             GLOBE_E_INVALID, "globe_synth_bed: bad parameters");
     require(c->has_cohort, GLOBE_E_STATE, "cohort must be set before genotypes");
-    const int64_t pitch = round_up((c->n + 3) / 4, kPitchAlign);
+    const int64_t pitch = packed_pitch(c->n);
     DevFill gen = [&](uint8_t* dst, int64_t s0, int64_t cnt) {
         launch_synth_bed(c->stream, dst, cnt, pitch, c->n, seed + 0x51ED2701ull * (uint64_t)s0, f_lo, f_hi,
                          miss);

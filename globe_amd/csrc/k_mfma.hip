@@ -53,6 +53,7 @@
 #include <cstdlib>
 
 #include "context.h"
+#include "contract_plan.h"
 
 namespace globe {
 
@@ -78,17 +79,8 @@ __device__ inline uint32_t dword_of(const uint4& v, int s) {
     return s == 0 ? v.x : s == 1 ? v.y : s == 2 ? v.z : v.w;
 }
 
-// block b -> (row block, split)
-__device__ inline void block_coords(int64_t b, int64_t R, int S, int64_t& rb, int& sp) {
-    if (S <= 8) {
-        sp = (int)(b % S);
-        rb = b / S;
-    } else {
-        const int64_t t = b / 8;
-        sp = (int)(b % 8) + 8 * (int)(t / R);
-        rb = t % R;
-    }
-}
+using plan::ThinCfg;
+using plan::WideCfg;
 
 // ---------------------------------------------------------------- thin kernel
 // W waves x RT row fragments of 16 rows (16 W RT rows per block), one column fragment
@@ -103,28 +95,12 @@ __device__ inline void block_coords(int64_t b, int64_t R, int S, int64_t& rb, in
 // 64 bytes per row at a time stream at 5.2 TB/s, 128 bytes at a time at 6.1 TB/s. Row
 // blocks are read without bounds checks: the packed layouts are padded to a multiple of
 // 512 rows.
-template <int W, int RT, int SS>
-struct ThinCfg {
-    static constexpr int kRowKB = W * RT * SS;           // packed rows of a stage
-    static constexpr int kDigKB = 4 * SS;                // digits of a stage (4 K-steps per super-step)
-    static constexpr int kA = kRowKB * 1024;
-    static constexpr int kB = kDigKB * 1024;
-    static constexpr int kSlot = kA + kB;
-    // digit chunks per wave: spread evenly, or one each for waves < kDigKB when there
-    // are more waves than chunks (the other waves copy only rows)
-    static constexpr int kDigPerWave = kDigKB >= W ? kDigKB / W : 1;
-    static constexpr int kOps = RT * SS + kDigPerWave;  // copies per stage, digit waves
-    static constexpr int kOpsRowOnly = RT * SS;         // copies per stage, other waves
-    static_assert(W == 4 || W == 8 || W == 16, "4, 8 or 16 waves");
-    static_assert(SS == 1 || SS == 2, "1 or 2 super-steps per stage");
-    static_assert(kDigKB < W || kDigKB % W == 0, "digit chunks must spread evenly");
-};
 
 // Digit chunk lane l holds digit column l % 16; only the first `live` columns carry
 // digits, so the other lanes skip their copy: their LDS bytes keep whatever the slot
 // held, which only reaches output columns that no epilogue reads.
 template <int W, int RT, int SS>
-__device__ inline void thin_issue(const uint8_t* __restrict__ arow, int64_t frag_stride,
+__device__ inline void thin_issue(const uint8_t* __restrict__ arow, int64_t pitch,
                                   const uint8_t* __restrict__ bstage, uint8_t* __restrict__ slot,
                                   int wave, int lane, int live) {
     using C = ThinCfg<W, RT, SS>;
@@ -133,7 +109,7 @@ __device__ inline void thin_issue(const uint8_t* __restrict__ arow, int64_t frag
 #pragma unroll
         for (int h = 0; h < SS; ++h) {
             const int f = (wave * RT + q) * SS + h;
-            __builtin_amdgcn_global_load_lds((const void*)(arow + q * frag_stride + 64 * h),
+            __builtin_amdgcn_global_load_lds((const void*)(arow + plan::thin_frag_off(q, h, pitch)),
                                              (lds_void*)(slot + f * 1024), 16, 0, 0);
         }
     const bool dig = (lane & 15) < live;  // live >= 1: every 16-lane group has an active lane
@@ -142,12 +118,12 @@ __device__ inline void thin_issue(const uint8_t* __restrict__ arow, int64_t frag
         for (int d = 0; d < C::kDigPerWave; ++d) {
             const int ch = wave * C::kDigPerWave + d;
             if (dig)
-                __builtin_amdgcn_global_load_lds((const void*)(bstage + ch * 1024 + lane * 16),
+                __builtin_amdgcn_global_load_lds((const void*)(bstage + plan::dig_lane(ch, lane)),
                                                  (lds_void*)(slot + C::kA + ch * 1024), 16, 0, 0);
         }
     } else {
         if (wave < C::kDigKB && dig)
-            __builtin_amdgcn_global_load_lds((const void*)(bstage + wave * 1024 + lane * 16),
+            __builtin_amdgcn_global_load_lds((const void*)(bstage + plan::dig_lane(wave, lane)),
                                              (lds_void*)(slot + C::kA + wave * 1024), 16, 0, 0);
     }
 }
@@ -188,18 +164,9 @@ __device__ inline void thin_stage_barrier(int wave) {
         asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(kAheadRowOnly) : "memory");
 }
 
-// Stream-K over the units u = r n_st + t (row block r of 16 W RT rows, stage t of its
-// n_st stages), u < U = R n_st: block b takes units [b U / G, (b + 1) U / G) as one
-// continuous ring of stages, so every block streams the same number of bytes and no
-// round of blocks runs the chip partly idle. A row block is covered by consecutive
-// blocks: the one holding its stage 0 writes partial slot 0, the next slot 1, ...
-// (slot = b - first block of the row block), and the block that ends a row block zeroes
-// the slots above its own up to P - 1, so the epilogue adds exactly P integer slots for
-// every row and the sum does not depend on G.
-__device__ inline int64_t sk_block_of(int64_t u, int64_t U, int64_t G) {
-    return ((u + 1) * G - 1) / U;  // the b with b U / G <= u < (b + 1) U / G
-}
-
+// Stream-K over the units (row block, stage) of plan::sk_block_of: every block streams
+// the same number of bytes, and the partial slots of each row block are exactly P, so the
+// epilogue's sum does not depend on the grid size.
 template <int W, int RT, int NS, int SS>
 __global__ __launch_bounds__(W * 64) void k_geno_thin(const uint8_t* __restrict__ rows, int64_t pitch,
                                                       int64_t nrows, int64_t n_st, int64_t U, int64_t G,
@@ -210,29 +177,19 @@ __global__ __launch_bounds__(W * 64) void k_geno_thin(const uint8_t* __restrict_
     using C = ThinCfg<W, RT, SS>;
     __shared__ __attribute__((aligned(1024))) uint8_t lds[NS * C::kSlot];
     const int64_t b = blockIdx.x;
-    const int64_t lo = b * U / G, hi = (b + 1) * U / G;
+    const int64_t lo = plan::sk_lo(b, U, G), hi = plan::sk_lo(b + 1, U, G);
     const int64_t nt = hi - lo;
     if (nt <= 0) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, rr = lane & 15;
-    constexpr int64_t kRB = 16 * W * RT;  // rows per row block
-    constexpr int kRowStage = 64 * SS;    // packed bytes of a row per stage
-    const int64_t lane_off = (int64_t)(wave * 16 * RT + rr) * pitch + 16 * g;
-    const int64_t fstride = 16 * pitch;
+    constexpr int64_t kRB = C::kRB;  // rows per row block
+    const int64_t lane_off = plan::thin_lane_off(wave, RT, rr, g, pitch);
 
-    // issue cursor (row block, stage); it stops at the block's last unit, so the last
-    // NS - 1 issues repeat that unit into slots that are never read
-    int64_t ir = lo / n_st, it = lo % n_st, issued = 0;
+    plan::SkCursor cur(lo, nt, n_st);
     auto issue_next = [&](int slot) {
-        thin_issue<W, RT, SS>(rows + ir * kRB * pitch + lane_off + kRowStage * it, fstride,
-                              Bq + it * C::kB, lds + slot * C::kSlot, wave, lane, live);
-        if (issued + 1 < nt) {
-            ++issued;
-            if (++it == n_st) {
-                it = 0;
-                ++ir;
-            }
-        }
+        thin_issue<W, RT, SS>(rows + plan::thin_unit_off(cur.r, kRB, cur.t, SS, pitch) + lane_off, pitch,
+                              Bq + plan::dig_stage(cur.t, C::kB), lds + slot * C::kSlot, wave, lane, live);
+        cur.advance();
     };
     // consume cursor
     int64_t cr = lo / n_st, ct = lo % n_st;
@@ -252,17 +209,16 @@ __global__ __launch_bounds__(W * 64) void k_geno_thin(const uint8_t* __restrict_
         const bool row_end = ++ct == n_st;
         if (row_end || i == nt - 1) {
             // this block's piece of row block cr is complete
-            const int64_t slot = b - sk_block_of(cr * n_st, U, G);
-            const int64_t r0 = cr * kRB + wave * 16 * RT + 4 * g;
+            const int64_t slot = plan::sk_slot(b, cr, n_st, U, G);
 #pragma unroll
             for (int a = 0; a < RT; ++a)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int64_t r = r0 + a * 16 + v;
+                    const int64_t r = plan::thin_out_row(cr, kRB, wave, RT, a, v, g);
                     if (r < nrows) {
-                        out[(slot * nrows + r) * 16 + rr] = acc[a][v];
+                        out[plan::part_idx(slot, nrows, r, 16, rr)] = acc[a][v];
                         if (row_end)
-                            for (int64_t z = slot + 1; z < P; ++z) out[(z * nrows + r) * 16 + rr] = 0;
+                            for (int64_t z = slot + 1; z < P; ++z) out[plan::part_idx(z, nrows, r, 16, rr)] = 0;
                     }
                 }
 #pragma unroll
@@ -286,25 +242,6 @@ __global__ __launch_bounds__(W * 64) void k_geno_thin(const uint8_t* __restrict_
 // (W / CG) 32 RA rows x all 32 JT digit columns. Per stage a wave reads RA KB of
 // packed rows (4 K-steps each) and 4 CB KB of digits from LDS for 4 RA CB MFMAs, so
 // the taller tiles (RA = 4) need a third less LDS traffic per MFMA than RA = 2.
-template <int JT, int W, int CG, int RA>
-struct WideCfg {
-    static constexpr int kW = W, kJT = JT, kRA = RA;
-    static constexpr int kCB = JT / CG;                 // column fragments per wave
-    static constexpr int kRowGroups = W / CG;
-    static constexpr int kRows = kRowGroups * 32 * RA;  // rows per block
-    static constexpr int kAFrags = kRowGroups * RA;     // 1 KB row fragments per stage
-    static constexpr int kAPerWave = kAFrags / W;
-    static constexpr int kA = kAFrags * 1024;
-    static constexpr int kB = 4 * JT * 1024;            // digits of a stage
-    static constexpr int kSlot = kA + kB;
-    static constexpr int kBChunks = 4 * JT;             // 1 KB digit chunks
-    static constexpr int kBPerWave = kBChunks >= W ? kBChunks / W : 1;
-    static constexpr int kSink = kBChunks < W ? (W - kBChunks) * 1024 : 0;
-    static constexpr int kOps = kAPerWave + kBPerWave;  // copies per wave per stage
-    static_assert(JT % CG == 0 && W % CG == 0, "column groups");
-    static_assert(kAFrags % W == 0, "row fragments split evenly over waves");
-    static_assert(kBChunks < W || kBChunks % W == 0, "digit chunks split evenly over waves");
-};
 
 // Issue the global -> LDS copies of one stage into ring slot `slot`: kAPerWave row
 // fragments (lane l copies the 16 packed bytes of row l % 32, half l / 32; they land
@@ -325,10 +262,10 @@ __device__ inline void wide_issue(const uint8_t* __restrict__ arow, int64_t frag
     for (int q = 0; q < C::kBPerWave; ++q) {
         const int chunk = wave + C::kW * q;
         if (chunk < C::kBChunks)
-            __builtin_amdgcn_global_load_lds((const void*)(bstage + chunk * 1024 + lane * 16),
+            __builtin_amdgcn_global_load_lds((const void*)(bstage + plan::dig_lane(chunk, lane)),
                                              (lds_void*)(slot + C::kA + chunk * 1024), 16, 0, 0);
         else
-            __builtin_amdgcn_global_load_lds((const void*)(bstage + (chunk % C::kBChunks) * 1024 + lane * 16),
+            __builtin_amdgcn_global_load_lds((const void*)(bstage + plan::dig_lane(chunk % C::kBChunks, lane)),
                                              (lds_void*)(sink + (chunk - C::kBChunks) * 1024), 16, 0, 0);
     }
 }
@@ -389,21 +326,15 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(2))) voi
     using C = WideCfg<JT, W, CG, RA>;
     constexpr int CB = C::kCB;
     __shared__ __attribute__((aligned(1024))) uint8_t lds[NS * C::kSlot + C::kSink];
-    int64_t rb;
-    int sp;
-    block_coords(blockIdx.x, R, S, rb, sp);
+    const plan::WideBlock wb = plan::wide_block(blockIdx.x, R, S, C::kRows, n_ss, ss_per_split);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int rgroup = wave / CG, cgroup = wave % CG;
     const int g = lane >> 5, rr = lane & 31;
-    const int64_t blk_row = rb * C::kRows;
-    const int64_t ss0 = (int64_t)sp * ss_per_split;
-    const int64_t ss1 = ss0 + ss_per_split < n_ss ? ss0 + ss_per_split : n_ss;
-    const int64_t H0 = 2 * ss0;
-    const int64_t nt = ss1 > ss0 ? 2 * (ss1 - ss0) : 0;  // stages of this split
+    const int64_t blk_row = wb.blk_row, H0 = wb.H0, nt = wb.nt;
     // fragment f of the block = rows blk_row + 32 f .. + 31
-    const uint8_t* arow = rows + (blk_row + rr) * pitch + 16 * g + 32 * H0;
-    const int64_t fstride = 32 * pitch;
-    const uint8_t* bst = Bq + H0 * C::kB;
+    const uint8_t* arow = rows + plan::wide_row_base(blk_row, rr, g, H0, pitch);
+    const int64_t fstride = plan::wide_frag_stride(pitch);
+    const uint8_t* bst = Bq + plan::dig_stage(H0, C::kB);
     uint8_t* sink = lds + NS * C::kSlot;
 
     v16i acc[4][4];
@@ -419,7 +350,8 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(2))) voi
 #pragma unroll
         for (int u = 0; u < NS - 1; ++u) {
             const int64_t t = u < nt ? u : nt - 1;
-            wide_issue<C>(arow + 32 * t, fstride, bst + t * C::kB, lds + u * C::kSlot, sink, wave, lane);
+            wide_issue<C>(arow + plan::wide_stage_step(t), fstride, bst + plan::dig_stage(t, C::kB),
+                          lds + u * C::kSlot, sink, wave, lane);
         }
         for (int64_t t0 = 0; t0 < nt; t0 += NS) {
 #pragma unroll
@@ -430,7 +362,7 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(2))) voi
                 // refill the slot multiplied in the previous stage with stage t + NS - 1,
                 // issued before this stage's MFMAs (the scheduler may not sink it)
                 const int64_t tn = t + NS - 1 < nt ? t + NS - 1 : nt - 1;
-                wide_issue<C>(arow + 32 * tn, fstride, bst + tn * C::kB,
+                wide_issue<C>(arow + plan::wide_stage_step(tn), fstride, bst + plan::dig_stage(tn, C::kB),
                               lds + ((u + NS - 1) % NS) * C::kSlot, sink, wave, lane);
                 __builtin_amdgcn_sched_barrier(0);
                 wide_mma<C>(lds + u * C::kSlot, rgroup, cgroup, lane, acc);
@@ -440,16 +372,16 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(2))) voi
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     }
     constexpr int J16 = 32 * JT;
-    int32_t* o = out + (int64_t)sp * nrows * J16;
+    int32_t* o = out + plan::part_idx(wb.sp, nrows, 0, J16, 0);
     const int64_t row_base = blk_row + rgroup * 32 * RA;
 #pragma unroll
     for (int a = 0; a < RA; ++a)
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
-            const int64_t r = row_base + a * 32 + 8 * (v >> 2) + 4 * g + (v & 3);
+            const int64_t r = plan::wide_out_row(row_base, a, v, g);
             if (r < nrows)
 #pragma unroll
-                for (int b = 0; b < CB; ++b) o[r * J16 + (cgroup * CB + b) * 32 + rr] = acc[a][b][v];
+                for (int b = 0; b < CB; ++b) o[plan::part_idx(0, nrows, r, J16, (cgroup * CB + b) * 32 + rr)] = acc[a][b][v];
         }
 }
 
@@ -552,21 +484,7 @@ __global__ __launch_bounds__(256) void k_quantize(const double* __restrict__ v, 
                 t = (t - b) >> 8;
             }
         }
-        const int64_t ss = kap >> 8;
-        const int rem = (int)(kap & 255);
-        const int w = rem & 15;
-        const int qslot = 4 * (w & 3) + (w >> 2);
-        int64_t ks;
-        int ln;
-        if (wide) {
-            // stage 2 ss + rem / 128; within it lane half (rem / 64) % 2, K-step (rem / 16) % 4
-            ks = 8 * ss + 4 * (rem >> 7) + ((rem >> 4) & 3);
-            ln = 32 * ((rem >> 6) & 1) + jj;
-        } else {
-            ks = 4 * ss + ((rem >> 4) & 3);
-            ln = 16 * (rem >> 6) + jj;
-        }
-        Bq[((ks * nfrag + jf) * 64 + ln) * 16 + qslot] = (uint8_t)(int8_t)dig;
+        Bq[plan::bq_index(kap, jj, jf, nfrag, wide != 0)] = (uint8_t)(int8_t)dig;
     }
     if (live && l == 0) {
         const double r = block_sum(acc);
@@ -680,35 +598,13 @@ __global__ __launch_bounds__(256) void k_epilogue(const int32_t* __restrict__ ou
         for (int c = threadIdx.x; c < k; c += 256) part[c * kMaxParts + blockIdx.x] = cmax[c];
 }
 
-// Split-K: enough blocks to fill the chip (>= 4 per CU), each split >= 8 super-steps;
-// wide contractions split at least 8 ways so each XCD streams one K-chunk of Bq.
-int choose_split(int64_t R, int64_t n_ss, bool wide) {
-    int S = 1;
-    while (R * S < 1024 && S < 64) S <<= 1;
-    if (wide && S < 8) S = 8;
-    while (S > 1 && n_ss / S < 8) S >>= 1;
-    return S;
-}
-
-struct Variant {
-    int cfg;     // index into kThin (thin) or kWide (wide)
-    int S;       // partial-sum slots: split-K (wide), pieces per row block (thin)
-    int64_t G;   // thin: stream-K grid (blocks); wide: 0
-};
-
-// wide block shapes: waves, ring stages, column groups, row fragments per wave
-struct WideShape {
-    int W, NS, CG, RA;
-};
-constexpr WideShape kWide[4] = {{8, 4, 1, 2}, {4, 3, 1, 2}, {8, 4, 2, 4}, {4, 3, 2, 4}};
-// thin block shapes: waves, row fragments of 16 rows per wave, ring stages, super-steps
-// per stage
-struct ThinShape {
-    int W, RT, NS, SS;
-};
-constexpr int kNumThin = 8;
-constexpr ThinShape kThin[kNumThin] = {{4, 4, 3, 1},  {8, 2, 3, 2},  {8, 2, 4, 2}, {16, 1, 3, 2},
-                                       {16, 1, 4, 2}, {8, 1, 5, 2}, {4, 4, 4, 2}, {8, 1, 6, 2}};
+using plan::kThin;
+using plan::kWide;
+using plan::rows_per_block;
+using plan::ThinShape;
+using plan::Variant;
+using plan::WideShape;
+using plan::wide_fits;
 
 template <int Q>
 void launch_thin_q(hipStream_t s, const Variant& v, const uint8_t* rows, int64_t pitch, int64_t nrows,
@@ -728,38 +624,23 @@ int thin_blocks_per_cu() {
     return nb;
 }
 
+// resident blocks per CU of thin shape cfg (a property of the compiled kernel: queried once)
 int thin_occupancy(int cfg) {
+    static int cache[plan::kNumThin] = {0};
+    if (cache[cfg] > 0) return cache[cfg];
+    int nb = 0;
     switch (cfg) {
-        case 0: return thin_blocks_per_cu<0>();
-        case 1: return thin_blocks_per_cu<1>();
-        case 2: return thin_blocks_per_cu<2>();
-        case 3: return thin_blocks_per_cu<3>();
-        case 4: return thin_blocks_per_cu<4>();
-        case 5: return thin_blocks_per_cu<5>();
-        case 6: return thin_blocks_per_cu<6>();
-        default: return thin_blocks_per_cu<7>();
+        case 0: nb = thin_blocks_per_cu<0>(); break;
+        case 1: nb = thin_blocks_per_cu<1>(); break;
+        case 2: nb = thin_blocks_per_cu<2>(); break;
+        case 3: nb = thin_blocks_per_cu<3>(); break;
+        case 4: nb = thin_blocks_per_cu<4>(); break;
+        case 5: nb = thin_blocks_per_cu<5>(); break;
+        case 6: nb = thin_blocks_per_cu<6>(); break;
+        default: nb = thin_blocks_per_cu<7>(); break;
     }
-}
-
-// Pieces per row block of the stream-K split: the most blocks any row block spans.
-int thin_pieces(int64_t R, int64_t nst, int64_t G) {
-    const int64_t U = R * nst;
-    auto block_of = [&](int64_t u) { return ((u + 1) * G - 1) / U; };
-    int64_t P = 1;
-    for (int64_t r = 0; r < R; ++r) P = std::max(P, block_of(r * nst + nst - 1) - block_of(r * nst) + 1);
-    return (int)P;
-}
-
-int64_t rows_per_block(bool wide, int cfg) {
-    if (!wide) return (int64_t)16 * kThin[cfg].W * kThin[cfg].RT;
-    const WideShape& w = kWide[cfg];
-    return (int64_t)(w.W / w.CG) * 32 * w.RA;
-}
-
-// A wide shape fits JT column fragments when the column groups divide them and the
-// stage's 4 JT digit chunks spread evenly over the waves (or number fewer than them).
-constexpr bool wide_fits(int JT, int q) {
-    return JT % kWide[q].CG == 0 && (4 * JT < kWide[q].W || (4 * JT) % kWide[q].W == 0);
+    cache[cfg] = std::max(nb, 1);
+    return cache[cfg];
 }
 
 template <int JT, int Q>
@@ -821,42 +702,13 @@ void launch_variant(hipStream_t s, bool wide, int JT, const Variant& v, const ui
 }  // namespace
 
 // Tile autotuner: the first contraction of each shape (pass, rows, K, digit columns)
-// times every candidate variant once on the live data with HIP events and keeps the
-// fastest; every variant computes the same exact integer sums, so the choice never
-// changes a result. Candidates: the thin block shapes of kThin, the wide block shapes of
-// kWide (the RA = 4 ones need an even JT), each with split-K S/2, S, 2 S (and 4 S for
-// thin) around choose_split().
+// times every candidate variant of plan::candidates once on the live data with HIP
+// events and keeps the fastest; every variant computes the same exact integer sums, so
+// the choice never changes a result.
 static std::vector<Variant> candidates(bool wide, int JT, int64_t nrows, int64_t nss, int ncu) {
-    std::vector<Variant> v;
-    if (wide) {
-        for (int cfg = 0; cfg < 4; ++cfg) {
-            if (!wide_fits(JT, cfg)) continue;
-            const int64_t R = (nrows + rows_per_block(true, cfg) - 1) / rows_per_block(true, cfg);
-            const int S0 = choose_split(R, nss, true);
-            for (int S : {S0 / 2, S0, S0 * 2}) {
-                if (S < 1 || S > 64 || (S > 1 && nss / S < 4)) continue;
-                v.push_back({cfg, S, 0});
-            }
-        }
-        return v;
-    }
-    for (int cfg = 0; cfg < kNumThin; ++cfg) {
-        if (nss % kThin[cfg].SS != 0) continue;
-        const int64_t R = (nrows + rows_per_block(false, cfg) - 1) / rows_per_block(false, cfg);
-        const int64_t U = R * (nss / kThin[cfg].SS);
-        const int64_t resident = (int64_t)ncu * std::max(1, thin_occupancy(cfg));
-        // grids that fill the chip in exactly 1, 2 or 4 rounds, and one or two blocks
-        // per row block (rounds as the hardware dispatches them)
-        for (int64_t G : {resident, 2 * resident, 4 * resident, R, 2 * R}) {
-            if (G < 1 || G > U) continue;
-            const int P = thin_pieces(R, nss / kThin[cfg].SS, G);
-            if (P > 64) continue;
-            bool dup = false;
-            for (const Variant& w : v) dup |= w.cfg == cfg && w.G == G;
-            if (!dup) v.push_back({cfg, P, G});
-        }
-    }
-    return v;
+    int bpc[plan::kNumThin];
+    for (int q = 0; q < plan::kNumThin; ++q) bpc[q] = wide ? 1 : thin_occupancy(q);
+    return plan::candidates(wide, JT, nrows, nss, ncu, bpc);
 }
 
 // GZ (n x k) = sig2g G S0 for the packed-genotype cohort, column c of S0 on col_limbs[c]
@@ -882,15 +734,6 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
         }
         J += col_limbs[q];
     }
-    const bool wide = J > 16;
-    const int JT = wide ? (J + 31) / 32 : 1;  // column fragments of 32 digit columns
-    const int J16 = wide ? 32 * JT : 16;
-    const int nfrag = wide ? JT : 1;
-    const int64_t Kpad1 = c->pitch_s * 4, Kpad2 = c->pitch_i * 4;  // genotypes per padded row
-    const int64_t nss1 = Kpad1 / 256, nss2 = Kpad2 / 256;
-    // the largest split any candidate may use bounds the partial-sum buffer
-    const size_t bq1 = (size_t)Kpad1 * J16, bq2 = (size_t)Kpad2 * J16;
-    int smax1 = 1, smax2 = 1;
     if (c->cu_count == 0) {
         hipDeviceProp_t prop;
         HIP_CHECK(hipGetDeviceProperties(&prop, c->device));
@@ -900,10 +743,14 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     // pass 2 forms rows r0 .. n - 1 (r0 > 0 only with U_given); fewer rows can take more
     // splits, so the partial buffer is sized for the row count actually contracted
     const int64_t r0 = U_given ? row0 : 0, n2 = n - r0;
-    for (const Variant& v : candidates(wide, JT, m, nss1, ncu)) smax1 = std::max(smax1, v.S);
-    for (const Variant& v : candidates(wide, JT, n2, nss2, ncu)) smax2 = std::max(smax2, v.S);
-    const size_t ob = std::max((size_t)smax1 * m, (size_t)smax2 * n2) * J16 * 4;
-    const size_t bq = std::max(bq1, bq2);
+    int bpc[plan::kNumThin];
+    for (int q = 0; q < plan::kNumThin; ++q) bpc[q] = thin_occupancy(q);
+    const plan::Geometry geo = plan::contract_geometry(J, m, n, c->pitch_s, c->pitch_i, r0, ncu, bpc);
+    const bool wide = geo.wide;
+    const int JT = geo.JT, J16 = geo.J16, nfrag = geo.nfrag;
+    const int64_t Kpad1 = geo.pass[0].Kpad, Kpad2 = geo.pass[1].Kpad;
+    const int64_t nss1 = geo.pass[0].nss, nss2 = geo.pass[1].nss;
+    const size_t bq = (size_t)geo.bq_bytes, ob = (size_t)geo.part_bytes;
     c->w_mix.ensure(round_up(bq, 256) + round_up(ob, 256) + 2 * kMaxCols * sizeof(int));
     uint8_t* Bq = c->w_mix.as<uint8_t>();
     int32_t* out = reinterpret_cast<int32_t*>(Bq + round_up(bq, 256));
@@ -925,9 +772,16 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
                         int64_t Kpad, int64_t nss, const auto& epi) -> int {
         const std::string key = std::to_string(pass) + ":" + std::to_string(nrows) + ":" +
                                 std::to_string(Kpad) + ":" + std::to_string(J16);
+        const std::vector<Variant> cand = candidates(wide, JT, nrows, nss, ncu);
+        c->variant_count[pass] = (int)cand.size();
         auto it = c->tuned.find(key);
+        if (c->force_variant >= 0) {
+            // validation: a given candidate instead of the tuned one
+            const Variant v = cand[(size_t)c->force_variant % cand.size()];
+            launch_variant(s, wide, JT, v, rows, pitch, nrows, nss, Bq, out, J, skip);
+            return v.S;
+        }
         if (it == c->tuned.end()) {
-            const std::vector<Variant> cand = candidates(wide, JT, nrows, nss, ncu);
             hipEvent_t e0, e1;
             HIP_CHECK(hipEventCreate(&e0));
             HIP_CHECK(hipEventCreate(&e1));

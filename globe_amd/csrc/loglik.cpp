@@ -180,6 +180,10 @@ extern "C" int globe_reml_loglik(globe_ctx* c, const double* y, int kind, int np
     o.rule = GLOBE_RULE_PER_COLUMN;
     o.x0_mode = GLOBE_X0_ZERO;
     o.stagnation = 0;
+    // The quadrature needs the Lanczos matrix of one operator: relaxing the int8 digits
+    // between iterations would change the operator mid-solve, so mixed-precision probe
+    // solves keep a fixed digit count.
+    o.flags |= GLOBE_PCG_FIXED_DIGITS;
     const int cap = o.max_iter;
     const size_t hist_n = (size_t)(1 + 2 * (size_t)cap) * kMaxCols;
     c->l_b.ensure(sizeof(double) * nrec * kMaxCols);

@@ -344,6 +344,15 @@ int globe_serve_close(globe_ctx* ctx);
  * place. Fewer digits trade accuracy (about 2^(1-8L) of each operand's scale) for speed. */
 int globe_set_mixed_digits(globe_ctx* ctx, int digits);
 
+/* ---- contraction variants (no reference counterpart: validation of the autotuner) ---- */
+/* The first packed-genotype contraction of each shape times every candidate variant (block
+ * shape, split-K or stream-K grid) and keeps the fastest; all candidates compute the same
+ * exact integer sums. index >= 0: every later contraction runs candidate index % count of
+ * its shape's candidate list instead, untimed; -1 (the default) returns to the tuned
+ * variants. globe_variant_counts: candidates of pass 1 and pass 2 of the last contraction. */
+int globe_force_variant(globe_ctx* ctx, int index);
+int globe_variant_counts(globe_ctx* ctx, int32_t counts[2]);
+
 /* ---- kernel timing (no reference counterpart) --------------------------------------- */
 /* When enabled, every genetic-product launch is bracketed by HIP events on the context
  * stream (launches that a finished PCG skips are discarded). which: 0 = pass 1 of the

@@ -407,3 +407,33 @@ def test_grm_builder_matches_wwt(gpu_ctx_factory):
     orc = O.Oracle(c.indiv, c.times, c.X, d.theta, grm_lt=lt, n=c.n)
     assert rel(g.computeVx_multi(x), orc.vx(x)) < 1e-12
     assert rel(g.computeGeneticDiagonal(), np.diag(G)) < 1e-6
+
+
+@pytest.mark.parametrize("kind", ["geno", "grm"])
+def test_cohort_of_another_size_releases_genetics(kind, gpu_ctx_factory):
+    """Genotypes and GRMs are laid out for the N they were staged with: a cohort of
+    another N releases them, so the next product fails with GLOBE_E_STATE (before any
+    kernel reads buffers of the old size); a cohort of the same N keeps them."""
+    _say("cohort-resize", kind)
+    d = synth.make_geno_case(n=200, m=300, lam=2, seed=5) if kind == "geno" else \
+        synth.make_grm_case(n=200, m=300, lam=2, seed=5)
+    c = d.cohort
+    g = gpu_ctx_factory()
+    g.set_cohort(c.indiv, c.times, c.X)
+    if kind == "geno":
+        g.prepGenotypeInput(d.bed_rows, 0.01, 0.2)
+    else:
+        g.readFullGRM(d.grm_lt)
+    g.set_varcomp(d.theta)
+    x = np.random.default_rng(1).standard_normal((c.nrec, 2))
+    y0 = g.computeVx_multi(x)
+    # same N, records reordered within the cohort's rules: genetics stay
+    g.set_cohort(c.indiv, c.times, c.X)
+    assert np.array_equal(g.computeVx_multi(x), y0)
+    for n_new in (150, 260):
+        c2 = synth.simulate_cohort(n_new, 2.0, np.random.default_rng(n_new))
+        g.set_cohort(c2.indiv, c2.times, c2.X)
+        with pytest.raises(G.GlobeError) as e:
+            g.computeVx_multi(np.ones((c2.nrec, 1)))
+        assert e.value.code == 2  # GLOBE_E_STATE
+    g.close()

@@ -1,0 +1,101 @@
+"""Host replay of the packed-genotype contraction's launch plans (CPU, no GPU).
+
+The tuner times every candidate variant on live data, so every candidate of a shape
+runs on the GPU at least once, not only the winner. plan_replay walks every block of
+every candidate of both passes through the kernels' own index functions
+(globe_amd/csrc/contract_plan.h) and fails on the first read outside the padded packed
+layout, digit access outside the digit matrix, partial-sum store or epilogue read
+outside the split-partial buffer that the library allocates, or partial slot left
+unwritten. The shapes are those the benchmarks and the GRM builder run:
+N = 200k / M = 500k (the scale workload), N = 40k / M = 80k with the builder's
+4096-multiple row offsets (the AI-REML cohort), N = 200k from 20k SNPs (the --grm
+path's builder), plus small and ragged shapes of the tests.
+"""
+import pytest
+
+import globe_amd
+
+# digit-column counts: the geometry depends on J only through the fragment count (thin:
+# J <= 16; wide: 1 to 4 fragments of 32), so one count per class covers them all
+J_ALL = (7, 28, 63, 91, 128)
+# resident thin blocks per CU: the kernels' LDS admits 1 or 2; the replay takes both
+BPC = ((1,) * 8, (2,) * 8, (2, 1, 1, 1, 1, 1, 2, 1))
+
+
+def _lds_bpc():
+    """Blocks per CU each thin shape's LDS ring admits (160 KB per CU)."""
+    shapes = [(4, 4, 3, 1), (8, 2, 3, 2), (8, 2, 4, 2), (16, 1, 3, 2), (16, 1, 4, 2), (8, 1, 5, 2),
+              (4, 4, 4, 2), (8, 1, 6, 2)]
+    out = []
+    for w, rt, ns, ss in shapes:
+        slot = w * rt * ss * 1024 + 4 * ss * 1024
+        out.append(max(1, min(2, (160 * 1024) // (ns * slot))))
+    return tuple(out)
+
+
+def _replay_all(m, n, r0s=(0,), js=J_ALL, ncus=(256,)):
+    seen_s = set()
+    for j in js:
+        for bpc in BPC + (_lds_bpc(),):
+            for ncu in ncus:
+                for r0 in r0s:
+                    st = globe_amd.plan_replay(j, m, n, r0=r0, ncu=ncu, thin_bpc=bpc)
+                    assert st["row_margin1"] <= 0 and st["row_margin2"] <= 0
+                    seen_s.add((st["wide"], st["cand1"], st["cand2"]))
+    return seen_s
+
+
+def test_replay_scale_workload_all_candidates():
+    """N = 200k, M = 500k (499,939 kept): the headline's thin and wide candidates."""
+    _replay_all(499939, 200000)
+
+
+def test_replay_aireml_cohort_builder_offsets():
+    """N = 40k, M = 80k: the GRM builder contracts rows r0 .. n - 1 from every multiple of
+    4096 (fewer rows admit more splits: the shape of the one fault of the first builder)."""
+    r0s = tuple(range(0, 40000, 4096))
+    _replay_all(79990, 40000, r0s=r0s, js=(125, 70))
+
+
+def test_replay_grm_path_builder():
+    """N = 200k from 20k SNPs (the --grm path at scale): builder offsets up to 196,608."""
+    r0s = tuple(range(0, 200000, 4096 * 6)) + (196608,)
+    _replay_all(20000, 200000, r0s=r0s, js=(125, 70))
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (3, 5), (900, 301), (2000, 400), (513, 1025), (40000, 20000),
+                                 (4097, 70001)])
+def test_replay_small_and_ragged(m, n):
+    _replay_all(m, n, r0s=(0,) if n < 4096 else (0, 4096), ncus=(256, 80))
+
+
+def test_replay_candidate_sets_reach_the_large_variants():
+    """The N = 20k / M = 40k cohort of the forced-variant GPU test has S > 8 split-K
+    candidates and multi-piece stream-K candidates, so that test runs those paths."""
+    st = globe_amd.plan_replay(128, 40000, 20000)
+    assert st["wide"] and st["cand1"] >= 9 and st["cand2"] >= 9
+    st = globe_amd.plan_replay(7, 40000, 20000)
+    assert not st["wide"] and st["cand1"] >= 10
+
+
+def test_replay_detects_an_undersized_buffer():
+    """The replay is not vacuous: the digit matrix and the split-partial buffer are
+    exactly as large as the farthest access of some candidate, and the packed layouts
+    exceed the farthest row read by the replay's reported margin, so taking 16 bytes
+    more than that margin off the layouts, 16 off the digit matrix or 4 off the
+    partial buffer makes the replay fail."""
+    for j, m, n in ((128, 40000, 20000), (7, 40000, 20000), (125, 79990, 40000)):
+        st = globe_amd.plan_replay(j, m, n)
+        slack = -max(st["row_margin1"], st["row_margin2"])
+        for shrink, what in (((slack + 16, 0, 0), "packed-row read"), ((0, 16, 0), "digit"),
+                             ((0, 0, 4), "partial")):
+            with pytest.raises(AssertionError, match=what):
+                globe_amd.plan_replay(j, m, n, shrink=shrink)
+        globe_amd.plan_replay(j, m, n, shrink=(slack, 0, 0))
+
+
+def test_replay_rejects_bad_shapes():
+    with pytest.raises(AssertionError):
+        globe_amd.plan_replay(129, 40000, 20000)
+    with pytest.raises(AssertionError):
+        globe_amd.plan_replay(128, 40000, 20000, r0=100)
