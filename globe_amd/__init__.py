@@ -116,6 +116,7 @@ def load_library(path: str = LIB_PATH):
         "globe_set_varcomp": ([P, P], C.c_int),
         "globe_vx": ([P, P, i64, C.c_int, P, i64, C.c_int, C.c_int], C.c_int),
         "globe_vx_dev": ([P, P, i64, C.c_int, P, i64, C.c_int, C.c_int], C.c_int),
+        "globe_vx_dev_noalloc": ([P, P, i64, C.c_int, P, i64, C.c_int, C.c_int], C.c_int),
         "globe_precond_apply": ([P, P, i64, C.c_int, P, i64], C.c_int),
         "globe_pcg": ([P, P, i64, C.c_int, P, i64, P, P, P, P], C.c_int),
         "globe_pcg_dev": ([P, P, i64, C.c_int, P, i64, P, P, P, P], C.c_int),

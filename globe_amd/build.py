@@ -23,7 +23,7 @@ HIPCC = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
 LIB_SOURCES = ["k_record.hip", "k_geno.hip", "k_grm.hip", "k_grm_i8.hip", "k_mfma.hip", "k_probe.hip", "api.cpp", "aireml.cpp",
-               "serve.cpp", "ingest.cpp", "loglik.cpp"]
+               "serve.cpp", "ingest.cpp", "loglik.cpp", "noalloc.hip"]
 CLI_COMMON = ["io.cpp"]
 CLIS = {"runPCGsolver": "cli_pcgsolver.cpp", "runAIREML": "cli_aireml.cpp",
         "runAIREML_reducedModel": "cli_aireml_reduced.cpp", "makeGRM": "cli_make_grm.cpp"}

@@ -24,8 +24,14 @@ void hip_check(hipError_t e, const char* what) {
                     std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// set while a no-allocation entry point runs on this thread: any device allocation is
+// then a bug, reported instead of performed
+thread_local bool t_noalloc = false;
+void set_noalloc(bool on) { t_noalloc = on; }
+
 void DevBuf::ensure(size_t b) {
     if (b <= bytes && p) return;
+    require(!t_noalloc, GLOBE_E_STATE, "device allocation inside a no-allocation call");
     release();
     if (b == 0) b = 256;
     hipError_t e = hipMalloc(&p, b);
@@ -47,6 +53,24 @@ void DevBuf::release() {
 
 void require(bool cond, int code, const std::string& msg) {
     if (!cond) throw Error(code, msg);
+}
+
+void* ws_bytes(globe_ctx* c, DevBuf& b, size_t bytes) {
+    globe_ctx::Arena& a = c->arena;
+    if (!a.base) {
+        b.ensure(bytes);
+        return b.p;
+    }
+    for (int q = 0; q < a.nslot; ++q)
+        if (a.slot[q].key == &b && a.slot[q].bytes >= bytes) return a.slot[q].p;
+    const size_t off = (size_t)round_up((int64_t)a.used, 256);
+    require(off + bytes <= a.cap && a.nslot < 32, GLOBE_E_INVALID,
+            "the call needs more than the " + std::to_string(a.cap >> 20) +
+                " MiB of static scratch of the no-allocation entry point (use the allocating one)");
+    void* p = a.base + off;
+    a.used = off + bytes;
+    a.slot[a.nslot++] = {&b, p, bytes};
+    return p;
 }
 
 // Gauss-Jordan inverse with partial pivoting (the reference inverts X^T V^-1 X and
@@ -121,19 +145,21 @@ void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int prec
     hipStream_t s = c->stream;
     if (c->gen == 2) {
         const bool mixed = precision == GLOBE_PREC_MIXED;
+        void* scratch = nullptr;
         if (mixed) {
             ensure_grm_digits(c);
             ensure_gdiag(c);
-            c->w_grm8_part.ensure(grm_i8_part_bytes(n));
+            scratch = ws_bytes(c, c->w_grm8_part, grm_i8_part_bytes(n));
         } else if (k <= 4) {
-            c->w_grm_sym.ensure(grm_sym_part_bytes(n, k));
+            scratch = ws_bytes(c, c->w_grm_sym, grm_sym_part_bytes(n, k));
         } else {
-            c->w_grm_part.ensure(grm_product_part_bytes(n));
+            scratch = ws_bytes(c, c->w_grm_part, grm_product_part_bytes(n));
         }
         // timing: which 2 = FP64 product, 3 = int8 digit product; algorithmic bytes = the
         // FP32 triangle once, flops = 2 N^2 k
         globe_ctx::ProfEvent e{};
-        if (c->prof_on) {
+        const bool prof = c->prof_on && !c->arena.base;
+        if (prof) {
             const double np = (double)((n + 63) / 64 * 64), L = c->limbs_mixed;
             const double cols = 16.0 * ((k + 15) / 16);
             HIP_CHECK(hipEventCreate(&e.a));
@@ -148,14 +174,14 @@ void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int prec
         }
         if (mixed)
             launch_grm_product_i8(s, c->d_grm8.as<int8_t>(), grm_digit_rexp(c->d_grm8_aux.p, n), n, S0, n, k,
-                                  c->limbs_mixed, c->d_gdiag.as<double>(), th, GZ, n, skip, c->w_grm8_part.p);
+                                  c->limbs_mixed, c->d_gdiag.as<double>(), th, GZ, n, skip, scratch);
         else if (k <= 4)
             launch_grm_product_sym(s, c->d_grm.as<float>(), n, S0, n, k, th, GZ, n, skip,
-                                   c->w_grm_sym.as<double>());
+                                   static_cast<double*>(scratch));
         else
             launch_grm_product(s, c->d_grm.as<float>(), n, S0, n, k, th, GZ, n, skip,
-                               c->w_grm_part.as<double>());
-        if (c->prof_on) {
+                               static_cast<double*>(scratch));
+        if (prof) {
             HIP_CHECK(hipEventRecord(e.b, s));
             c->prof.push_back(e);
         }
@@ -185,26 +211,22 @@ void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int prec
         return;
     }
     const int64_t m = c->m;
-    c->w_P1.ensure(sizeof(double) * (size_t)m * k);
-    c->w_U.ensure(sizeof(double) * (size_t)m * k);
-    c->w_P2.ensure(sizeof(double) * (size_t)n * k);
-    c->w_T.ensure(sizeof(double) * kMaxCols);
-    c->w_C.ensure(sizeof(double) * kMaxCols);
-    c->w_part2.ensure(sizeof(double) * kMaxParts * kMaxCols);
-    double* P1 = c->w_P1.as<double>();
-    double* U = c->w_U.as<double>();
-    double* P2 = c->w_P2.as<double>();
-    double* part = c->w_part2.as<double>();
+    double* P1 = ws<double>(c, c->w_P1, sizeof(double) * (size_t)m * k);
+    double* U = ws<double>(c, c->w_U, sizeof(double) * (size_t)m * k);
+    double* P2 = ws<double>(c, c->w_P2, sizeof(double) * (size_t)n * k);
+    double* T = ws<double>(c, c->w_T, sizeof(double) * kMaxCols);
+    double* Cs = ws<double>(c, c->w_C, sizeof(double) * kMaxCols);
+    double* part = ws<double>(c, c->w_part2, sizeof(double) * kMaxParts * kMaxCols);
     // pass 1 (PCGsolver.cpp:1143-1171): P1[c][s] = sum_i g_is S0[c][i]
     launch_rowdot(s, c->d_snp.as<uint8_t>(), c->pitch_s, m, n, S0, n, k, P1, m, skip);
     launch_colsum_parts(s, n, S0, n, k, part, skip);
-    launch_reduce_parts_skip(s, part, parts_for(n), k, c->w_T.as<double>(), skip);
-    launch_center_snp(s, m, k, P1, m, c->w_T.as<double>(), c->d_twof.as<double>(),
-                      c->d_inv.as<double>(), th, 1.0 / (double)m, U, m, part, skip);
-    launch_reduce_parts_skip(s, part, parts_for(m), k, c->w_C.as<double>(), skip);
+    launch_reduce_parts_skip(s, part, parts_for(n), k, T, skip);
+    launch_center_snp(s, m, k, P1, m, T, c->d_twof.as<double>(), c->d_inv.as<double>(), th, 1.0 / (double)m, U, m,
+                      part, skip);
+    launch_reduce_parts_skip(s, part, parts_for(m), k, Cs, skip);
     // pass 2 (PCGsolver.cpp:1173-1199): GZ[c][i] = sum_s g_is u[c][s] - C[c]
     launch_rowdot(s, c->d_ind.as<uint8_t>(), c->pitch_i, n, m, U, m, k, P2, n, skip);
-    launch_center_ind(s, n, k, P2, n, c->w_C.as<double>(), GZ, n, skip);
+    launch_center_ind(s, n, k, P2, n, Cs, GZ, n, skip);
 }
 
 void op_vx(globe_ctx* c, const double* x, int64_t ldx, int k, double* y, int64_t ldy, int mode,
@@ -214,12 +236,9 @@ void op_vx(globe_ctx* c, const double* x, int64_t ldx, int k, double* y, int64_t
     require(k >= 1 && k <= kMaxCols, GLOBE_E_INVALID, "k must be in [1, 64]");
     ensure_theta(c, k);
     const int32_t n = c->n;
-    c->w_S0.ensure(sizeof(double) * (size_t)n * k);
-    c->w_S1.ensure(sizeof(double) * (size_t)n * k);
-    c->w_GZ.ensure(sizeof(double) * (size_t)n * k);
-    double* S0 = c->w_S0.as<double>();
-    double* S1 = c->w_S1.as<double>();
-    double* GZ = c->w_GZ.as<double>();
+    double* S0 = ws<double>(c, c->w_S0, sizeof(double) * (size_t)n * k);
+    double* S1 = ws<double>(c, c->w_S1, sizeof(double) * (size_t)n * k);
+    double* GZ = ws<double>(c, c->w_GZ, sizeof(double) * (size_t)n * k);
     hipStream_t s = c->stream;
     launch_seg_sums(s, c->meta, x, ldx, k, S0, S1, n, skip);
     if (mode == GLOBE_VX_GRM_ONLY) {
@@ -465,7 +484,14 @@ int globe_create(int device, globe_ctx** out) {
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
         hipHostMalloc((void**)&c->h_state, sizeof(PcgState), hipHostMallocDefault) != hipSuccess) {
-        delete c;
+        globe_destroy(c);
+        return GLOBE_E_DEVICE;
+    }
+    // the device address of the static scratch of the no-allocation entry point
+    try {
+        c->scratch = scratch_address(device, &c->scratch_bytes);
+    } catch (const Error&) {
+        globe_destroy(c);
         return GLOBE_E_DEVICE;
     }
     *out = c;

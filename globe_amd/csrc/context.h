@@ -105,6 +105,23 @@ struct globe_ctx {
 
     // serving loop, when open
     ServeState* serve = nullptr;
+
+    // No-allocation calls (globe_vx_dev_noalloc): the device's static scratch, whose
+    // address hipGetSymbolAddress returned at globe_create; while `arena.base` is set,
+    // every workspace of the call is a slice of it (globe::ws) and nothing is allocated.
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    struct Arena {
+        uint8_t* base = nullptr;
+        size_t cap = 0, used = 0;
+        struct Slot {
+            const void* key;
+            void* p;
+            size_t bytes;
+        };
+        Slot slot[32];
+        int nslot = 0;
+    } arena;
 };
 
 // Body of an ABI entry point: HIP errors and globe::Error become status codes with
@@ -127,7 +144,15 @@ struct globe_ctx {
 
 namespace globe {
 
+// Workspace `b` of at least `bytes` for the current call: the context's grow-only buffer,
+// or, inside a no-allocation call, a 256-byte-aligned slice of the static scratch (the
+// same slice each time the call asks for the same workspace).
+void* ws_bytes(globe_ctx* c, DevBuf& b, size_t bytes);
+template <class T> T* ws(globe_ctx* c, DevBuf& b, size_t bytes) { return static_cast<T*>(ws_bytes(c, b, bytes)); }
+
 void serve_release(globe_ctx* c);
+// device address and size of the no-allocation scratch on `device` (the current device)
+void* scratch_address(int device, size_t* bytes);
 
 // device operator: y = V x (mode FULL) or sig2g G J^T x (GRM_ONLY); optional fused
 // partials of sum x.y per column; skip = device done flag (nullptr: always run).

@@ -750,38 +750,64 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     const int JT = geo.JT, J16 = geo.J16, nfrag = geo.nfrag;
     const int64_t Kpad1 = geo.pass[0].Kpad, Kpad2 = geo.pass[1].Kpad;
     const int64_t nss1 = geo.pass[0].nss, nss2 = geo.pass[1].nss;
-    const size_t bq = (size_t)geo.bq_bytes, ob = (size_t)geo.part_bytes;
-    c->w_mix.ensure(round_up(bq, 256) + round_up(ob, 256) + 2 * kMaxCols * sizeof(int));
-    uint8_t* Bq = c->w_mix.as<uint8_t>();
+    // The variant of each pass: a forced candidate (validation), the tuned one, or -- in a
+    // no-allocation call, which neither times candidates nor sizes the partials for all
+    // of them -- the candidate with the fewest partial slots. Every candidate computes the
+    // same integer sums, so the choice never changes a result.
+    const bool noalloc = c->arena.base != nullptr;
+    auto key_of = [&](int pass, int64_t nrows, int64_t Kpad) {
+        return std::to_string(pass) + ":" + std::to_string(nrows) + ":" + std::to_string(Kpad) + ":" +
+               std::to_string(J16);
+    };
+    auto preset = [&](int pass, int64_t nrows, int64_t Kpad, int64_t nss, Variant& v) -> bool {
+        const std::vector<Variant> cand = candidates(wide, JT, nrows, nss, ncu);
+        c->variant_count[pass] = (int)cand.size();
+        if (c->force_variant >= 0) {
+            v = cand[(size_t)c->force_variant % cand.size()];
+            return true;
+        }
+        if (noalloc) {
+            v = cand[0];
+            for (const Variant& w : cand)
+                if (w.S < v.S) v = w;
+            return true;
+        }
+        auto it = c->tuned.find(key_of(pass, nrows, Kpad));
+        if (it == c->tuned.end()) return false;
+        v = Variant{it->second.cfg, it->second.parts, it->second.grid};
+        return true;
+    };
+    Variant vp[2];
+    bool fixed[2] = {false, false};
+    if (!U_given) fixed[0] = preset(0, m, Kpad1, nss1, vp[0]);
+    fixed[1] = preset(1, n2, Kpad2, nss2, vp[1]);
+    const size_t bq = (size_t)geo.bq_bytes;
+    const size_t ob = noalloc ? (size_t)std::max((U_given ? 0 : vp[0].S) * m, (int64_t)vp[1].S * n2) * J16 * 4
+                              : (size_t)geo.part_bytes;
+    uint8_t* Bq = ws<uint8_t>(c, c->w_mix, round_up(bq, 256) + round_up(ob, 256) + 2 * kMaxCols * sizeof(int));
     int32_t* out = reinterpret_cast<int32_t*>(Bq + round_up(bq, 256));
     int* sh1 = reinterpret_cast<int*>(Bq + round_up(bq, 256) + round_up(ob, 256));
     int* sh2 = sh1 + kMaxCols;
-    c->w_U.ensure(sizeof(double) * (size_t)(m > 0 ? m : 1) * k);
-    c->w_T.ensure(sizeof(double) * kMaxCols);
-    c->w_C.ensure(sizeof(double) * kMaxCols);
-    c->w_part2.ensure(sizeof(double) * kMaxParts * kMaxCols);
-    double* part = c->w_part2.as<double>();
-    double* U = c->w_U.as<double>();
+    double* U = ws<double>(c, c->w_U, sizeof(double) * (size_t)(m > 0 ? m : 1) * k);
+    double* Tc = ws<double>(c, c->w_T, sizeof(double) * kMaxCols);
+    double* Cc = ws<double>(c, c->w_C, sizeof(double) * kMaxCols);
+    double* part = ws<double>(c, c->w_part2, sizeof(double) * kMaxParts * kMaxCols);
     const double* th = theta_cols;
     const double flops = 2.0 * (double)n * (double)m * k;
 
-    // run one pass with the tuned variant (tuning it first if this shape is new)
-    // epi(S, skip) launches the pass's epilogue over S split partials: the tuner times
-    // each candidate together with it, since more splits mean more partials to combine
+    // run one pass with its preset variant, or tune the shape first (epi(S, skip) launches
+    // the pass's epilogue over S split partials: the tuner times each candidate together
+    // with it, since more splits mean more partials to combine)
     auto contract = [&](int pass, const uint8_t* rows, int64_t pitch, int64_t nrows, int64_t K,
                         int64_t Kpad, int64_t nss, const auto& epi) -> int {
-        const std::string key = std::to_string(pass) + ":" + std::to_string(nrows) + ":" +
-                                std::to_string(Kpad) + ":" + std::to_string(J16);
-        const std::vector<Variant> cand = candidates(wide, JT, nrows, nss, ncu);
-        c->variant_count[pass] = (int)cand.size();
-        auto it = c->tuned.find(key);
-        if (c->force_variant >= 0) {
-            // validation: a given candidate instead of the tuned one
-            const Variant v = cand[(size_t)c->force_variant % cand.size()];
-            launch_variant(s, wide, JT, v, rows, pitch, nrows, nss, Bq, out, J, skip);
-            return v.S;
+        if (fixed[pass] && (c->force_variant >= 0 || noalloc)) {
+            launch_variant(s, wide, JT, vp[pass], rows, pitch, nrows, nss, Bq, out, J, skip);
+            return vp[pass].S;
         }
+        const std::string key = key_of(pass, nrows, Kpad);
+        auto it = c->tuned.find(key);
         if (it == c->tuned.end()) {
+            const std::vector<Variant> cand = candidates(wide, JT, nrows, nss, ncu);
             hipEvent_t e0, e1;
             HIP_CHECK(hipEventCreate(&e0));
             HIP_CHECK(hipEventCreate(&e1));
@@ -808,7 +834,7 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
             it = c->tuned.emplace(key, globe_ctx::Tuned{cand[bi].cfg, cand[bi].S, cand[bi].G}).first;
         }
         const Variant v{it->second.cfg, it->second.parts, it->second.grid};
-        if (c->prof_on) {
+        if (c->prof_on && !noalloc) {
             globe_ctx::ProfEvent e{};
             HIP_CHECK(hipEventCreate(&e.a));
             HIP_CHECK(hipEventCreate(&e.b));
@@ -842,11 +868,11 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     const int qg1 = parts_for(Kpad1);
     k_quantize<<<dim3(qg1, J16), 256, 0, s>>>(S0, ldn, n, Kpad1, k, lm, nfrag, wide, sh1, nullptr,
                                               Bq, part, skip);
-    k_finish_cols<<<1, 256, 0, s>>>(part, qg1, k, 0, lm, nullptr, c->w_T.as<double>(), skip);
+    k_finish_cols<<<1, 256, 0, s>>>(part, qg1, k, 0, lm, nullptr, Tc, skip);
     HIP_CHECK(hipGetLastError());
     eb1 = std::min<int64_t>((m + kEpiRows - 1) / kEpiRows, kMaxParts);
     auto epi1 = [&](int S, const int* sk) {
-        k_epilogue<<<(unsigned)eb1, 256, 0, s>>>(out, S, m, J16, lm, k, sh1, 0, c->w_T.as<double>(),
+        k_epilogue<<<(unsigned)eb1, 256, 0, s>>>(out, S, m, J16, lm, k, sh1, 0, Tc,
                                                  c->d_twof.as<double>(), c->d_inv.as<double>(), th,
                                                  1.0 / (double)m, U, m, part, sk);
         HIP_CHECK(hipGetLastError());
@@ -858,11 +884,11 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     const int qg2 = parts_for(Kpad2);
     k_quantize<<<dim3(qg2, J16), 256, 0, s>>>(U, m, m, Kpad2, k, lm, nfrag, wide, sh2,
                                               c->d_twof.as<double>(), Bq, part, skip);
-    k_finish_cols<<<1, 256, 0, s>>>(part, qg2, k, 0, lm, nullptr, c->w_C.as<double>(), skip);
+    k_finish_cols<<<1, 256, 0, s>>>(part, qg2, k, 0, lm, nullptr, Cc, skip);
     HIP_CHECK(hipGetLastError());
     const int64_t eb2 = std::min<int64_t>((n2 + kEpiRows - 1) / kEpiRows, kMaxParts);
     auto epi2 = [&](int S, const int* sk) {
-        k_epilogue<<<(unsigned)eb2, 256, 0, s>>>(out, S, n2, J16, lm, k, sh2, 1, c->w_C.as<double>(), nullptr,
+        k_epilogue<<<(unsigned)eb2, 256, 0, s>>>(out, S, n2, J16, lm, k, sh2, 1, Cc, nullptr,
                                                  nullptr, nullptr, 0.0, GZ + r0, ldg, nullptr, sk);
         HIP_CHECK(hipGetLastError());
     };

@@ -151,6 +151,21 @@ int globe_vx(globe_ctx* ctx, const double* X, int64_t ldx, int k, double* Y, int
              int mode, int precision);
 int globe_vx_dev(globe_ctx* ctx, const double* X, int64_t ldx, int k, double* Y, int64_t ldy,
                  int mode, int precision);
+/* No counterpart in the reference: globe_vx_dev for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same operator, modes and precisions, same kernels, results bit-identical to
+ * globe_vx_dev; every workspace is a slice of one static 128 MiB device array that each
+ * device the library runs on holds until the process ends, so nothing is allocated and
+ * nothing is tuned (the contraction runs its candidate with the fewest partial slots).
+ * The work is enqueued on the context stream and the call returns without waiting
+ * (globe_synchronize waits). GLOBE_E_INVALID when the call needs more than the scratch
+ * (24 N k bytes of individual-space vectors, 8 M k of SNP-space ones, the contraction's
+ * digit matrix and split partials: N = 20k, M = 40k fits up to k = 32, N = 200k,
+ * M = 500k fits k = 1 in FP64); GLOBE_E_STATE when the call would have to allocate
+ * (mixed-precision GRM digits not built yet). The array is shared by every context on
+ * a device and keeps the last call's contents: calls that overlap on one device race. */
+int globe_vx_dev_noalloc(globe_ctx* ctx, const double* X, int64_t ldx, int k, double* Y, int64_t ldy,
+                         int mode, int precision);
 /* Replaces applyPreconditioner (PCGsolver.cpp:1060-1093, varCompEst.cpp:432-450):
  * Z = Q^-1 R per individual block, all k columns. */
 int globe_precond_apply(globe_ctx* ctx, const double* R, int64_t ldr, int k, double* Z, int64_t ldz);
