@@ -34,6 +34,26 @@ __device__ inline double block_sum(double v) {
     return r;
 }
 
+// Grid-wide reduction ticket: every workgroup calls this once after storing its
+// partials (thread 0 stores them); it returns true, in all threads, in the one
+// workgroup that arrives last. The release half of the agent-scope atomic publishes each
+// workgroup's partials past its XCD's L2, and the acquire fence of the last workgroup
+// makes all of them visible to it, so it can add them in a fixed order (deterministic).
+// The last workgroup resets the ticket for the next launch.
+__device__ inline bool arrive_last(unsigned int* ticket) {
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int total = gridDim.x * gridDim.y;
+        const unsigned int prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = prev + 1 == total;
+        if (last) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return last != 0;
+}
+
 // ---------------------------------------------------------------------------
 // S0[c][i] = sum_{r in block} x[c][r], S1[c][i] = sum t_r x[c][r]
 // (PCGsolver.cpp:1119-1125 scatter, and the t-weighted sum the factorized time
@@ -62,14 +82,125 @@ __global__ __launch_bounds__(kBlock) void k_seg_sums(int32_t nblk, const int64_t
     }
 }
 
+// ---------------------------------------------------------------- PCG control
+// PCG per PCGsolver.cpp:1343-1443 / varCompEst.cpp:482-527 with every scalar on
+// the device; once `done` is set every later kernel of the solve is a no-op, so the
+// host may enqueue iterations ahead of the convergence test.
+// sums[c] = sum_j part[c][j] for c < k: one wave per column, lanes over partials,
+// then a fixed shuffle tree (deterministic). Ends with a barrier.
+__device__ void wave_col_sums(const double* __restrict__ part, int nparts, int k, double* sums) {
+    const int lane = threadIdx.x & 63;
+    for (int c = threadIdx.x >> 6; c < k; c += blockDim.x >> 6) {
+        double v = 0.0;
+        for (int j = lane; j < nparts; j += 64) v += part[c * kMaxParts + j];
+        v = wave_sum(v);
+        if (lane == 0) sums[c] = v;
+    }
+    __syncthreads();
+}
+
+// alpha_c = r.z / p.Vp of the active columns (the p.Vp partials of k columns)
+__device__ void pcg_alpha_body(PcgState* st, const double* __restrict__ part_pvp, int nparts, int k) {
+    __shared__ double sums[kMaxCols];
+    wave_col_sums(part_pvp, nparts, k, sums);
+    const int c = threadIdx.x;
+    if (c >= k) return;
+    st->alpha[c] = st->active[c] ? st->rs[c] / sums[c] : 0.0;
+    if (st->hist && st->active[c] && st->iters < st->hist_cap)
+        st->hist[(1 + 2 * (int64_t)st->iters) * kMaxCols + c] = st->alpha[c];
+}
+
+// Convergence test after the x/r update of one iteration.
+//   rule 0 (PCGsolver.cpp:1388-1415): max ratio over columns with |b| > 1e-12 below tol;
+//          stall exit when the max ratio improves by < 5e-3 of its last value for 10
+//          consecutive iterations.
+//   rule 1 (varCompEst.cpp:505): every column |r| < tol |b|.
+//   rule 2: per-column freeze; done when no column is active.
+__device__ void pcg_check_body(PcgState* st, const double* __restrict__ part_rr, int nparts, int k) {
+    __shared__ double nr_s[kMaxCols];
+    wave_col_sums(part_rr, nparts, k, nr_s);
+    if (threadIdx.x < k) nr_s[threadIdx.x] = sqrt(nr_s[threadIdx.x]);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const int it = st->iters + 1;
+    int done = 0;
+    double maxr = 0.0;
+    if (st->rule == GLOBE_RULE_MAX_RATIO) {
+        for (int j = 0; j < k; ++j) {
+            const double nb = st->bnorm[j];
+            const double ratio = nb > 1e-12 ? nr_s[j] / nb : 0.0;
+            st->ratio[j] = ratio;
+            if (ratio > maxr) maxr = ratio;
+        }
+        if (maxr < st->tol) {
+            done = 1;
+        } else if (st->stagnation) {
+            const double impr = st->last - maxr, req = 5e-3 * st->last;
+            st->stag = impr < req ? st->stag + 1 : 0;
+            st->last = maxr;
+            if (st->stag >= 10) {
+                done = 1;
+                st->stalled = 1;
+            }
+        } else {
+            st->last = maxr;
+        }
+        for (int j = 0; j < k; ++j) st->col_iters[j] = it;
+    } else if (st->rule == GLOBE_RULE_ALL_COLUMNS) {
+        int all = 1;
+        for (int j = 0; j < k; ++j) {
+            const double nb = st->bnorm[j];
+            if (!(nr_s[j] < st->tol * nb)) all = 0;
+            const double ratio = nb > 0.0 ? nr_s[j] / nb : 0.0;
+            st->ratio[j] = ratio;
+            if (ratio > maxr) maxr = ratio;
+            st->col_iters[j] = it;
+        }
+        done = all;
+    } else {
+        int any = 0;
+        for (int j = 0; j < k; ++j) {
+            const double nb = st->bnorm[j];
+            const double ratio = nb > 1e-12 ? nr_s[j] / nb : 0.0;
+            if (st->active[j]) {
+                st->ratio[j] = nb > 0.0 ? nr_s[j] / nb : 0.0;
+                st->col_iters[j] = it;
+                if (nr_s[j] < st->tol * nb) st->active[j] = 0;
+            }
+            if (ratio > maxr) maxr = ratio;
+            any |= st->active[j];
+        }
+        done = !any;
+    }
+    st->max_ratio = maxr;
+    st->iters = it;
+    if (it >= st->max_iter) done = 1;
+    st->done = done;
+}
+
+// beta_c = r.z_new / r.z_old of the active columns (the r.z partials of k columns)
+__device__ void pcg_beta_body(PcgState* st, const double* __restrict__ part_rz, int nparts, int k) {
+    __shared__ double sums[kMaxCols];
+    wave_col_sums(part_rz, nparts, k, sums);
+    const int c = threadIdx.x;
+    if (c >= k || !st->active[c]) return;
+    const double s = sums[c];
+    st->beta[c] = s / st->rs[c];
+    st->rs[c] = s;
+    // iters already counts this iteration
+    if (st->hist && st->iters - 1 < st->hist_cap)
+        st->hist[(2 + 2 * (int64_t)(st->iters - 1)) * kMaxCols + c] = st->beta[c];
+}
+
 // y[c][r] = GZ_i + sig2b0 S0_i + sigb01 (t S0_i + S1_i) + sig2b1 t S1_i + sig2e x_r
 // (the factorized form of PCGsolver.cpp:1220-1265). Optional fused partial of
-// sum_r x[c][r] y[c][r] (the p . Vp of PCG, PCGsolver.cpp:1381).
+// sum_r x[c][r] y[c][r] (the p . Vp of PCG, PCGsolver.cpp:1381), and with a PCG state
+// the grid reduction of those partials into alpha.
 __global__ __launch_bounds__(kBlock) void k_record_out(
     RecMeta m, const double* __restrict__ x, int64_t ldx, const double* __restrict__ GZ,
     const double* __restrict__ S0, const double* __restrict__ S1, int64_t ldn,
     const double* __restrict__ th, double* __restrict__ y, int64_t ldy, double* part_dot,
-    const int* skip) {
+    const int* skip, PcgState* ast) {
     if (skip && *skip) return;
     const int c = blockIdx.y;
     const double se = th[c * 5 + 0], sb0 = th[c * 5 + 2], sb01 = th[c * 5 + 3],
@@ -97,6 +228,8 @@ __global__ __launch_bounds__(kBlock) void k_record_out(
     if (part_dot) {
         double s = block_sum(acc);
         if (threadIdx.x == 0) part_dot[c * kMaxParts + blockIdx.x] = s;
+        // inside PCG the last workgroup forms alpha from all p.Vp partials
+        if (ast && arrive_last(&ast->ticket[0])) pcg_alpha_body(ast, part_dot, gridDim.x, gridDim.y);
     }
 }
 
@@ -147,14 +280,16 @@ __global__ __launch_bounds__(kBlock) void k_seg_sums_tiled(RecMeta m, const doub
 __global__ __launch_bounds__(kBlock) void k_precond_tiled(
     RecMeta m, const double* __restrict__ gdiag, const double* __restrict__ th,
     const double* __restrict__ r, int64_t ldr, double* __restrict__ z, int64_t ldz,
-    double* part_rz, double* part_rr, const PcgState* stt, int tilde_v) {
+    double* part_rz, double* part_rr, PcgState* stt, int tilde_v, int fuse_beta) {
     if (stt && stt->done) return;
     const int c = blockIdx.y;
     if (stt && !stt->active[c]) {
+        // frozen column: keep the partial slots defined
         if (threadIdx.x == 0) {
             if (part_rz) part_rz[c * kMaxParts + blockIdx.x] = 0.0;
             if (part_rr) part_rr[c * kMaxParts + blockIdx.x] = 0.0;
         }
+        if (fuse_beta && arrive_last(&stt->ticket[2])) pcg_beta_body(stt, part_rz, gridDim.x, gridDim.y);
         return;
     }
     extern __shared__ double sm[];
@@ -225,6 +360,7 @@ __global__ __launch_bounds__(kBlock) void k_precond_tiled(
         double s = block_sum(arr);
         if (threadIdx.x == 0) part_rr[c * kMaxParts + blockIdx.x] = s;
     }
+    if (fuse_beta && arrive_last(&stt->ticket[2])) pcg_beta_body(stt, part_rz, gridDim.x, gridDim.y);
 }
 
 // z = Q_i^-1 r per individual block by the 2x2 Woodbury form of
@@ -233,12 +369,13 @@ __global__ __launch_bounds__(kBlock) void k_precond_tiled(
 // (PCGsolver.cpp:1026-1045, applied at :1060-1093):
 //   w = U^T r, c = (sig2e I + C U^T U)^-1 C w, z = (r - U c) / sig2e.
 // tilde_v = 1 uses G_ii := 1 (tilde-V, varCompEst.cpp:839-858).
-// Fused partials: sum r.z and sum r.r per column.
+// Fused partials: sum r.z and sum r.r per column; inside PCG (fuse_beta) the grid
+// reduction of the r.z partials into beta.
 __global__ __launch_bounds__(kBlock) void k_precond(
     int32_t nblk, const int64_t* __restrict__ off, const int32_t* __restrict__ ind,
     const double* __restrict__ t, const double* __restrict__ gdiag, const double* __restrict__ th,
     const double* __restrict__ r, int64_t ldr, double* __restrict__ z, int64_t ldz,
-    double* part_rz, double* part_rr, const PcgState* st, int tilde_v) {
+    double* part_rz, double* part_rr, PcgState* st, int tilde_v, int fuse_beta) {
     if (st && st->done) return;
     const int c = blockIdx.y;
     if (st && !st->active[c]) {
@@ -247,6 +384,7 @@ __global__ __launch_bounds__(kBlock) void k_precond(
             if (part_rz) part_rz[c * kMaxParts + blockIdx.x] = 0.0;
             if (part_rr) part_rr[c * kMaxParts + blockIdx.x] = 0.0;
         }
+        if (fuse_beta && arrive_last(&st->ticket[2])) pcg_beta_body(st, part_rz, gridDim.x, gridDim.y);
         return;
     }
     const double se = th[c * 5 + 0], sg = th[c * 5 + 1], sb0 = th[c * 5 + 2],
@@ -295,6 +433,7 @@ __global__ __launch_bounds__(kBlock) void k_precond(
         double s = block_sum(arr);
         if (threadIdx.x == 0) part_rr[c * kMaxParts + blockIdx.x] = s;
     }
+    if (fuse_beta && arrive_last(&st->ticket[2])) pcg_beta_body(st, part_rz, gridDim.x, gridDim.y);
 }
 
 // log|Q_i| of the preconditioner blocks by Sylvester's determinant identity:
@@ -386,23 +525,6 @@ __global__ __launch_bounds__(kBlock) void k_axpy_cols(int64_t nrec, double* __re
     }
 }
 
-// ---------------------------------------------------------------- PCG control
-// PCG per PCGsolver.cpp:1343-1443 / varCompEst.cpp:482-527 with every scalar on
-// the device; once `done` is set every later kernel of the solve is a no-op, so the
-// host may enqueue iterations ahead of the convergence test.
-// sums[c] = sum_j part[c][j] for c < k: one wave per column, lanes over partials,
-// then a fixed shuffle tree (deterministic). Ends with a barrier.
-__device__ void wave_col_sums(const double* __restrict__ part, int nparts, int k, double* sums) {
-    const int lane = threadIdx.x & 63;
-    for (int c = threadIdx.x >> 6; c < k; c += blockDim.x >> 6) {
-        double v = 0.0;
-        for (int j = lane; j < nparts; j += 64) v += part[c * kMaxParts + j];
-        v = wave_sum(v);
-        if (lane == 0) sums[c] = v;
-    }
-    __syncthreads();
-}
-
 __global__ __launch_bounds__(kBlock) void k_pcg_init(PcgState* st, const double* __restrict__ part_bb,
                                                      const double* __restrict__ part_rr,
                                                      const double* __restrict__ part_rz, int nparts,
@@ -477,18 +599,6 @@ __global__ __launch_bounds__(64) void k_pcg_admit(PcgState* st, int j, const dou
     if (act) st->done = 0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_pcg_alpha(PcgState* st, const double* __restrict__ part_pvp,
-                                                      int nparts, int k) {
-    if (st->done) return;
-    __shared__ double sums[kMaxCols];
-    wave_col_sums(part_pvp, nparts, k, sums);
-    const int c = threadIdx.x;
-    if (c >= k) return;
-    st->alpha[c] = st->active[c] ? st->rs[c] / sums[c] : 0.0;
-    if (st->hist && st->active[c] && st->iters < st->hist_cap)
-        st->hist[(1 + 2 * (int64_t)st->iters) * kMaxCols + c] = st->alpha[c];
-}
-
 __global__ __launch_bounds__(kBlock) void k_pcg_xr(PcgState* st, int64_t nrec, double* __restrict__ x,
                                                    int64_t ldx, double* __restrict__ r,
                                                    const double* __restrict__ p,
@@ -511,91 +621,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_xr(PcgState* st, int64_t nrec, d
     }
     double s = block_sum(acc);
     if (threadIdx.x == 0) part_rr[c * kMaxParts + blockIdx.x] = s;
-}
-
-// Convergence test after the x/r update of one iteration.
-//   rule 0 (PCGsolver.cpp:1388-1415): max ratio over columns with |b| > 1e-12 below tol;
-//          stall exit when the max ratio improves by < 5e-3 of its last value for 10
-//          consecutive iterations.
-//   rule 1 (varCompEst.cpp:505): every column |r| < tol |b|.
-//   rule 2: per-column freeze; done when no column is active.
-__global__ __launch_bounds__(kBlock) void k_pcg_check(PcgState* st, const double* __restrict__ part_rr,
-                                                      int nparts, int k) {
-    if (st->done) return;
-    __shared__ double nr_s[kMaxCols];
-    wave_col_sums(part_rr, nparts, k, nr_s);
-    if (threadIdx.x < k) nr_s[threadIdx.x] = sqrt(nr_s[threadIdx.x]);
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    const int it = st->iters + 1;
-    int done = 0;
-    double maxr = 0.0;
-    if (st->rule == GLOBE_RULE_MAX_RATIO) {
-        for (int j = 0; j < k; ++j) {
-            const double nb = st->bnorm[j];
-            const double ratio = nb > 1e-12 ? nr_s[j] / nb : 0.0;
-            st->ratio[j] = ratio;
-            if (ratio > maxr) maxr = ratio;
-        }
-        if (maxr < st->tol) {
-            done = 1;
-        } else if (st->stagnation) {
-            const double impr = st->last - maxr, req = 5e-3 * st->last;
-            st->stag = impr < req ? st->stag + 1 : 0;
-            st->last = maxr;
-            if (st->stag >= 10) {
-                done = 1;
-                st->stalled = 1;
-            }
-        } else {
-            st->last = maxr;
-        }
-        for (int j = 0; j < k; ++j) st->col_iters[j] = it;
-    } else if (st->rule == GLOBE_RULE_ALL_COLUMNS) {
-        int all = 1;
-        for (int j = 0; j < k; ++j) {
-            const double nb = st->bnorm[j];
-            if (!(nr_s[j] < st->tol * nb)) all = 0;
-            const double ratio = nb > 0.0 ? nr_s[j] / nb : 0.0;
-            st->ratio[j] = ratio;
-            if (ratio > maxr) maxr = ratio;
-            st->col_iters[j] = it;
-        }
-        done = all;
-    } else {
-        int any = 0;
-        for (int j = 0; j < k; ++j) {
-            const double nb = st->bnorm[j];
-            const double ratio = nb > 1e-12 ? nr_s[j] / nb : 0.0;
-            if (st->active[j]) {
-                st->ratio[j] = nb > 0.0 ? nr_s[j] / nb : 0.0;
-                st->col_iters[j] = it;
-                if (nr_s[j] < st->tol * nb) st->active[j] = 0;
-            }
-            if (ratio > maxr) maxr = ratio;
-            any |= st->active[j];
-        }
-        done = !any;
-    }
-    st->max_ratio = maxr;
-    st->iters = it;
-    if (it >= st->max_iter) done = 1;
-    st->done = done;
-}
-
-__global__ __launch_bounds__(kBlock) void k_pcg_beta(PcgState* st, const double* __restrict__ part_rz,
-                                                     int nparts, int k) {
-    if (st->done) return;
-    __shared__ double sums[kMaxCols];
-    wave_col_sums(part_rz, nparts, k, sums);
-    const int c = threadIdx.x;
-    if (c >= k || !st->active[c]) return;
-    const double s = sums[c];
-    st->beta[c] = s / st->rs[c];
-    st->rs[c] = s;
-    // iters already counts this iteration
-    if (st->hist && st->iters - 1 < st->hist_cap)
-        st->hist[(2 + 2 * (int64_t)(st->iters - 1)) * kMaxCols + c] = st->beta[c];
+    // the last workgroup runs the convergence test on all r.r partials
+    if (arrive_last(&st->ticket[1])) pcg_check_body(st, part_rr, gridDim.x, gridDim.y);
 }
 
 __global__ __launch_bounds__(kBlock) void k_pcg_p(PcgState* st, int64_t nrec, double* __restrict__ p,
@@ -728,23 +755,27 @@ void launch_seg_sums(hipStream_t s, const RecMeta& m, const double* x, int64_t l
 void launch_record_out(hipStream_t s, const RecMeta& m, const double* x, int64_t ldx, int k,
                        const double* GZ, const double* S0, const double* S1, int64_t ldn,
                        const double* theta_cols, double* y, int64_t ldy, double* part_dot,
-                       const int* skip) {
+                       const int* skip, PcgState* alpha_st) {
     dim3 g(parts_for(m.nrec), k);
-    k_record_out<<<g, kBlock, 0, s>>>(m, x, ldx, GZ, S0, S1, ldn, theta_cols, y, ldy, part_dot, skip);
+    k_record_out<<<g, kBlock, 0, s>>>(m, x, ldx, GZ, S0, S1, ldn, theta_cols, y, ldy, part_dot, skip,
+                                      part_dot ? alpha_st : nullptr);
     LAUNCH_CHECK();
 }
 
 void launch_precond(hipStream_t s, const RecMeta& m, const double* gdiag, const double* theta_cols,
                     const double* r, int64_t ldr, int k, double* z, int64_t ldz, double* part_rz,
-                    double* part_rr, const PcgState* st, int tilde_v) {
+                    double* part_rr, const PcgState* st, int tilde_v, bool fuse_beta) {
+    require(!fuse_beta || (st && part_rz), GLOBE_E_INVALID, "beta needs the PCG state and r.z partials");
+    PcgState* sw = const_cast<PcgState*>(st);
+    const int fb = fuse_beta ? 1 : 0;
     if (m.cap > 0) {
         dim3 g(precond_parts(m), k);
         k_precond_tiled<<<g, kBlock, sizeof(double) * 4 * m.cap, s>>>(m, gdiag, theta_cols, r, ldr, z,
-                                                                       ldz, part_rz, part_rr, st, tilde_v);
+                                                                       ldz, part_rz, part_rr, sw, tilde_v, fb);
     } else {
         dim3 g(parts_for(m.nblk), k);
         k_precond<<<g, kBlock, 0, s>>>(m.nblk, m.blk_off, m.blk_ind, m.times, gdiag, theta_cols, r,
-                                       ldr, z, ldz, part_rz, part_rr, st, tilde_v);
+                                       ldr, z, ldz, part_rz, part_rr, sw, tilde_v, fb);
     }
     LAUNCH_CHECK();
 }
@@ -814,10 +845,6 @@ void launch_pcg_init(hipStream_t s, PcgState* st, const double* part_bb, const d
     LAUNCH_CHECK();
 }
 
-void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* part_pvp, int nparts, int k) {
-    k_pcg_alpha<<<1, kBlock, 0, s>>>(st, part_pvp, nparts, k);
-    LAUNCH_CHECK();
-}
 
 void launch_pcg_xr(hipStream_t s, PcgState* st, int64_t nrec, double* x, int64_t ldx, double* r,
                    const double* p, const double* vp, int64_t ld, int k, double* part_rr) {
@@ -826,15 +853,7 @@ void launch_pcg_xr(hipStream_t s, PcgState* st, int64_t nrec, double* x, int64_t
     LAUNCH_CHECK();
 }
 
-void launch_pcg_check(hipStream_t s, PcgState* st, const double* part_rr, int nparts, int k) {
-    k_pcg_check<<<1, kBlock, 0, s>>>(st, part_rr, nparts, k);
-    LAUNCH_CHECK();
-}
 
-void launch_pcg_beta(hipStream_t s, PcgState* st, const double* part_rz, int nparts, int k) {
-    k_pcg_beta<<<1, kBlock, 0, s>>>(st, part_rz, nparts, k);
-    LAUNCH_CHECK();
-}
 
 void launch_pcg_p(hipStream_t s, PcgState* st, int64_t nrec, double* p, const double* z,
                   int64_t ld, int k) {

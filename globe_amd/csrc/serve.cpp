@@ -204,21 +204,17 @@ void iterate(globe_ctx* c, ServeState& S, int k) {
     double* part_a = S.part_a.as<double>();
     double* part_b = S.part_b.as<double>();
     double* part_c = S.part_c.as<double>();
-    const int np = parts_for(cap), npb = precond_parts(m);
     // V p: individuals outside a request's subset keep S0 = S1 = 0
     launch_fill(s, S0, (int64_t)n * k, 0.0);
     launch_fill(s, S1, (int64_t)n * k, 0.0);
     launch_seg_sums(s, m, S.P.as<double>(), cap, k, S0, S1, n, skip);
     genetic_product(c, S0, k, GZ, S.o.precision, skip, th);
     launch_record_out(s, m, S.P.as<double>(), cap, k, GZ, S0, S1, n, th, S.VP.as<double>(), cap, part_a,
-                      skip);
-    launch_pcg_alpha(s, st, part_a, np, k);
+                      skip, st);
     launch_pcg_xr(s, st, cap, S.X.as<double>(), cap, S.R.as<double>(), S.P.as<double>(),
                   S.VP.as<double>(), cap, k, part_b);
-    launch_pcg_check(s, st, part_b, np, k);
     launch_precond(s, m, c->d_gdiag.as<double>(), th, S.R.as<double>(), cap, k, S.Z.as<double>(), cap,
-                   part_c, nullptr, st, 0);
-    launch_pcg_beta(s, st, part_c, npb, k);
+                   part_c, nullptr, st, 0, true);
     launch_pcg_p(s, st, cap, S.P.as<double>(), S.Z.as<double>(), cap, k);
     ++S.global_iter;
 }
