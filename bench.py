@@ -158,12 +158,25 @@ def _reference_iteration_time(exe, c, y, rows, theta, threads, max_iters):
     return t_it, len(stamps) - 1, nthr, t_start - t0
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline_reference(g, c, y, kept, iters, sample_snps, threads, theta, max_iters=7):
     """The reference's own runPCGsolver (oracle/_ref, compiled from its sources by
     oracle/Makefile) timed on the host cores: the same cohort and phenotype with the
-    first s kept SNPs written as a .bed, for two sample sizes s. Its PCG solves [X | y]
-    (p + 2 columns, FP32); the time per iteration is fitted as a + b s (record-space work
-    plus genotype passes) and evaluated at s = M, then multiplied by the GPU FP64
+    first s kept SNPs written as a .bed, for several sample sizes s up to one of its
+    50,000-SNP chunks (PCGsolver.cpp:1135; 2.5 GB at N = 200k, beyond the host caches).
+    Its PCG solves [X | y] (p + 2 columns, FP32); the time per iteration is fitted by least
+    squares as a + b s (record-space work plus genotype passes), the fit's residuals at the
+    samples are reported, and it is evaluated at s = M and multiplied by the GPU FP64
     iteration count. Returns None when the binary is absent."""
     exe = os.path.join(ROOT, "oracle", "_ref", "runPCGsolver")
     if not os.access(exe, os.X_OK):
@@ -172,21 +185,34 @@ def cpu_baseline_reference(g, c, y, kept, iters, sample_snps, threads, theta, ma
     for s in sample_snps:
         rows = g.geno_rows(0, s)[:, : (c.n + 3) // 4]
         r = _reference_iteration_time(exe, c, y, rows, theta, threads, max_iters)
+        del rows
         if r is None:
-            return None
+            log(f"CPU baseline: the reference gave no timing at {s} SNPs")
+            continue
         pts.append((s, *r))
-    (s1, t1, n1, nthr, _), (s2, t2, n2, _, setup) = pts[0], pts[-1]
-    b = (t2 - t1) / (s2 - s1)
-    a = t1 - b * s1
+        log(f"CPU baseline: reference at {s} SNPs: {r[0]:.3f} s per PCG iteration ({r[1]} timed, "
+            f"{r[2]} threads, set-up {r[3]:.1f} s)")
+    if len(pts) < 2:
+        return None
+    S = np.array([p[0] for p in pts], dtype=np.float64)
+    T = np.array([p[1] for p in pts])
+    A = np.column_stack([np.ones_like(S), S])
+    (a, b), *_ = np.linalg.lstsq(A, T, rcond=None)
+    resid = T - (a + b * S)
     t_full = a + b * kept
+    nthr = pts[0][3]
     k = c.X.shape[1] + 1
+    desc = "; ".join(f"{p[2]} iterations at {p[0]} SNPs, {p[1]:.3f} s each" for p in pts)
     return {"value": k / (t_full * iters), "unit": "solves/s", "cores": nthr, "kind": "reference",
-            "sample": f"the reference runPCGsolver (FP32, OpenMP, {nthr} threads) on N={c.n}, Nrec={c.nrec}: "
-                      f"{n1} PCG iterations with {s1} SNPs ({t1:.3f} s each) and {n2} with {s2} SNPs "
-                      f"({t2:.3f} s each; input parsing and set-up {setup:.1f} s, not counted) of its "
-                      f"{k}-column [X | y] solve; per-iteration time a + b M fitted and evaluated at "
-                      f"M = {kept} ({t_full:.2f} s), times the GPU FP64 iteration count ({iters}); "
-                      f"{k} right-hand sides per solve; extrapolated",
+            "cpu_model": _cpu_model(),
+            "sample": f"the reference runPCGsolver (FP32, OpenMP, {nthr} threads, {_cpu_model()}) on N={c.n}, "
+                      f"Nrec={c.nrec}, its {k}-column [X | y] solve: {desc} (input parsing and set-up not "
+                      f"counted); per-iteration time a + b M fitted by least squares (a = {a:.3f} s, b = "
+                      f"{b * 1e6:.3f} us per SNP; residuals {', '.join(f'{x:+.3f}' for x in resid)} s) and "
+                      f"evaluated at M = {kept} ({t_full:.2f} s), times the GPU FP64 iteration count "
+                      f"({iters}); {k} right-hand sides per solve; extrapolated",
+            "fit": {"a_s": float(a), "b_s_per_snp": float(b), "snps": [int(x) for x in S],
+                    "sec_per_iter": [float(x) for x in T], "residual_s": [float(x) for x in resid]},
             "sec_per_iter_extrapolated": t_full}
 
 
@@ -206,35 +232,80 @@ def serving_leg(g, c, kept, args):
     log(f"serving: {len(reqs)} requests generated in {time.time() - t0:.1f}s "
         f"(records {min(r[0].shape[0] for r in reqs)}..{cap})")
     import globe_amd as G
+    # the record pool (request vectors and structure) gets a share of the HBM left free
+    # by the resident cohort and the solver's workspaces
+    free_hbm, total_hbm = g.device_memory()
+    budget = int(args.serve_hbm_frac * free_hbm)
+
     def run(batch, rate):
-        g.serve_open(slots=args.serve_slots, max_records=cap, tol=args.tol, max_iter=1000,
+        g.serve_open(slots=args.serve_slots, max_records=cap, hbm_budget=budget, tol=args.tol, max_iter=1000,
                      precision=G.PREC_MIXED)
         _, st = serving.replay(g, batch, rate, seed=args.seed)
+        use = g.serve_usage()
         g.serve_close()
         lat = st["latency_s"]
         r = {"requests": len(batch), "arrival_rate_per_s": rate, "solves_per_s": len(batch) / st["wall_s"],
              "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
              "mean_iters": float(np.mean(st["iters"])), "mean_active_slots": st["mean_active"],
-             "block_iterations": st["steps"], "wall_s": st["wall_s"]}
+             "block_iterations": st["steps"], "wall_s": st["wall_s"],
+             "pool_peak_bytes": use["peak"], "steps_waiting_for_hbm": use["waits"]}
         log(f"serving at {rate:g}/s: {r['solves_per_s']:.1f} solves/s, p50 {r['p50_s']:.3f}s, "
-            f"p99 {r['p99_s']:.3f}s, mean active {r['mean_active_slots']:.1f}")
+            f"p99 {r['p99_s']:.3f}s, mean active {r['mean_active_slots']:.1f}, pool peak "
+            f"{use['peak'] / 1e9:.2f} of {use['pool'] / 1e9:.2f} GB, waits for HBM {use['waits']}")
         return r
 
     out = run(reqs, args.serve_rate)
-    # half of the same requests offered 50x faster (a burst the loop cannot clear as it
+    # a tenth of the same requests offered 50x faster (a burst the loop cannot clear as it
     # arrives): throughput and latency with all slots busy
-    out["saturated"] = run(reqs[: max(1, len(reqs) // 2)], 50.0 * args.serve_rate)
+    out["saturated"] = run(reqs[: max(1, len(reqs) // 10)], 50.0 * args.serve_rate)
     out.update({"slots": args.serve_slots, "precision": "mixed", "tol": args.tol,
+                "hbm_budget_bytes": budget, "hbm_free_bytes": free_hbm,
                 "data": "synthetic requests (subset, records, components, trait) on the resident cohort"})
     return out
+
+
+def aireml_cpu_baseline(g, c, theta, grm_launches, grm_cols, iters, sample_indiv, threads):
+    """runAIREML's operator on the CPU: the FP32 OpenMP record-pair restatement of
+    computeVx_multi_VC (varCompEst.cpp:322-352, oracle/) timed with 3 applications at
+    k = 1 and 3 at k = 16 columns on the records of the first sample_indiv individuals
+    (the leading block of the device's GRM), fitted as a + b k per application and scaled
+    by (Nrec / Nrec_sample)^2 (every record pair is visited). Seconds per AIREML iteration =
+    the GPU fit's operator applications per iteration (GRM product launches, and their
+    columns) at that cost; extrapolated."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    ns = min(sample_indiv, c.n)
+    nr = int(np.searchsorted(c.indiv, ns))  # records are contiguous per individual, in order
+    sub = g.grm_rows(0, ns)
+    orc = O.Oracle(c.indiv[:nr], c.times[:nr], c.X[:nr], theta, grm_lt=sub, n=ns)
+    rng = np.random.default_rng(0)
+    t_k = {}
+    for k in (1, 16):
+        x = rng.standard_normal((nr, k)).astype(np.float32)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            orc.vx_vc_ref32(x)
+        t_k[k] = (time.perf_counter() - t0) / 3
+    b = (t_k[16] - t_k[1]) / 15.0
+    a = t_k[1] - b
+    scale = (c.nrec / nr) ** 2
+    t_iter = (a * grm_launches + b * grm_cols) * scale / iters
+    return {"sec_per_iter": t_iter, "unit": "s/iteration", "cores": threads, "kind": "port",
+            "sample": f"3 + 3 applications of the FP32 record-pair restatement of computeVx_multi_VC on the "
+                      f"{nr} records of the first {ns} individuals (k = 1: {t_k[1]:.2f} s, k = 16: "
+                      f"{t_k[16]:.2f} s each), a + b k fitted, scaled by (Nrec / {nr})^2 = {scale:.0f} and by "
+                      f"the GPU fit's {grm_launches / iters:.1f} operator applications ({grm_cols / iters:.0f} "
+                      f"columns) per iteration; extrapolated"}
 
 
 def aireml_leg(args):
     """AIREML on a dense GRM (N = 40,000, GRM from 80,000 synthetic SNPs built on the
     device, ~320k records, [X | y | 15 Rademacher probes] as one block PCG, start
-    0.5 / 0.5 / 0.3 / 0 / 0.3): seconds per AIREML iteration over the first iterations,
-    with the GRM product in FP64 and in mixed precision (int8 digits) from the same start
-    and probes."""
+    0.5 / 0.5 / 0.3 / 0 / 0.3) run to convergence (maxiter 50, tol 1e-3): iterations and
+    seconds per AIREML iteration with the GRM product in FP64 and in mixed precision (int8
+    digits) from the same start and probes, and the reference operator's CPU cost per
+    iteration."""
     import globe_amd as G
     from globe_amd import synth
     g = G.Globe(0)
@@ -265,6 +336,9 @@ def aireml_leg(args):
                            seed=args.seed, pcg_tol=1e-5, precision=prec)
         ks = g.kernel_stats(3 if prec == G.PREC_MIXED else 2)
         g.profile(False)
+        if name == "fp64":
+            grm_launches = ks["launches"]
+            grm_cols = ks["flops"] * ks["launches"] / (2.0 * c.n * c.n)
         runs[name] = {"sec_per_iter": fit["sec_per_iter"], "iters": fit["iters"],
                       "theta": list(map(float, fit["theta"])), "grm_passes": ks["launches"],
                       "grm_pass_ms": ks["ms_avg"],
@@ -281,6 +355,13 @@ def aireml_leg(args):
                                                    "max_steps", "seconds")}
         log(f"aireml {name}: REML log-likelihood {ll['loglik']:.6f} (log|V| {ll['logdet_v']:.4f} +/- "
             f"{ll['logdet_v_se']:.3g}, {ll['max_steps']} Lanczos steps) in {ll['seconds']:.3f}s")
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        log("aireml: CPU baseline (record-pair operator) ...")
+        cpu = aireml_cpu_baseline(g, c, theta, grm_launches, grm_cols, runs["fp64"]["iters"],
+                                  args.aireml_cpu_indiv, threads)
+        log(f"aireml: CPU baseline {cpu['sec_per_iter']:.0f} s per iteration ({cpu['kind']})")
     t64, tm = np.array(runs["fp64"]["theta"]), np.array(runs["mixed"]["theta"])
     out = {"n": c.n, "m_snps": int(kept), "nrec": c.nrec, "probes": 15, "grm_build_s": t_grm,
            "iters": runs["fp64"]["iters"], "sec_per_iter": runs["fp64"]["sec_per_iter"],
@@ -292,8 +373,49 @@ def aireml_leg(args):
            "fp64_grm_pass_ms": runs["fp64"]["grm_pass_ms"],
            "loglik": dict(runs["fp64"]["loglik"], probes=64, at="the FP64 fit's components",
                           mixed=runs["mixed"]["loglik"]),
-           "note": f"first {args.aireml_iters} iterations of the fit from the same start and probes "
-                   "(maxiter 50, tol 1e-3 in the full run)"}
+           "cpu_baseline": cpu,
+           "note": f"the fit from the same start and probes, maxiter {args.aireml_iters}, tol 1e-3"}
+    g.close()
+    return out
+
+
+def single_rhs_leg(args):
+    """The single-RHS runPCGsolver --geno workload at N = 50,000, M = 100,000 SNPs (1%
+    missing, QC minMAF 0.01 / maxMissingRate 0.05), 1 + Poisson(7) records (~400k): the
+    solve of y in FP64 (7-digit int8 contraction), in the FP64 vector-ALU form and in mixed
+    precision (4 digits, relaxed), tol 1e-6 -- solves/s, iterations, the packed-genotype
+    passes' time and HBM rate, and each solution's error against the FP64 one."""
+    import globe_amd as G
+    g = G.Globe(0)
+    c, y, kept = build_cohort(g, args.cfg1_n, args.cfg1_m, args.lam, args.seed + 17)
+    nrec = c.nrec
+    b1, x1 = g.alloc(nrec * 8), g.alloc(nrec * 8)
+    b1.upload(np.ascontiguousarray(y))
+    out = {"n": c.n, "m_snps": args.cfg1_m, "m_kept": int(kept), "nrec": nrec, "tol": args.tol}
+    ref = None
+    for name, prec in (("fp64", G.PREC_FP64), ("fp64_valu", G.PREC_FP64_VALU), ("mixed", G.PREC_MIXED)):
+        g.pcg_dev(b1, x1, 1, tol=args.tol, rule=G.RULE_PER_COLUMN, precision=prec)  # warm (tuning)
+        g.profile(True)
+        reps, t_sum = 5, 0.0
+        for _ in range(reps):
+            st = g.pcg_dev(b1, x1, 1, tol=args.tol, rule=G.RULE_PER_COLUMN, precision=prec)
+            t_sum += st["seconds"]
+        p = [g.kernel_stats(0), g.kernel_stats(1)]
+        g.profile(False)
+        x = x1.download(np.zeros(nrec))
+        r = {"solves_per_s": reps / t_sum, "seconds": t_sum / reps, "iters": st["iters"]}
+        if prec != G.PREC_FP64_VALU:
+            r.update({"pass1_ms": p[0]["ms_avg"], "pass2_ms": p[1]["ms_avg"],
+                      "pass_GBps": [q["bytes"] / (q["ms_avg"] * 1e-3) / 1e9 if q["launches"] else None for q in p],
+                      "hbm_frac": [q["bytes"] / (q["ms_avg"] * 1e-3) / 1e9 / PEAK_HBM_GBPS if q["launches"] else None
+                                   for q in p]})
+        if ref is None:
+            ref = x
+        else:
+            r["rel_err_vs_fp64"] = float(np.linalg.norm(x - ref) / np.linalg.norm(ref))
+        out[name] = r
+        log(f"single-RHS {name}: {st['iters']} iters, {r['seconds'] * 1e3:.1f} ms, {r['solves_per_s']:.1f} solves/s"
+            + (f", passes {p[0]['ms_avg']:.3f} / {p[1]['ms_avg']:.3f} ms" if "pass1_ms" in r else ""))
     g.close()
     return out
 
@@ -416,18 +538,24 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--seed", type=int, default=20261206)
     ap.add_argument("--cpu-sample-snps", type=int, default=20_000)
-    ap.add_argument("--cpu-ref-snps", type=int, default=4_000,
-                    help="SNPs of the .bed the reference runPCGsolver is timed on")
+    ap.add_argument("--cpu-ref-snps", type=int, nargs="+", default=[1_000, 4_000, 50_000],
+                    help="SNP samples on which the reference runPCGsolver is timed")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=400,
+    ap.add_argument("--serve-requests", type=int, default=2000,
                     help="requests replayed through the serving loop (0: skip)")
+    ap.add_argument("--serve-hbm-frac", type=float, default=0.05,
+                    help="share of the free HBM given to the serving loop's record pool")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-slots", type=int, default=32)
     ap.add_argument("--ingest-snps", type=int, default=40_000,
                     help="SNPs of the .bed written and ingested (0: skip)")
-    ap.add_argument("--aireml-iters", type=int, default=3, help="AIREML iterations timed (0: skip)")
+    ap.add_argument("--aireml-iters", type=int, default=50, help="AIREML maxiter (0: skip)")
+    ap.add_argument("--aireml-cpu-indiv", type=int, default=10_000,
+                    help="individuals whose records time the CPU record-pair operator")
     ap.add_argument("--aireml-n", type=int, default=40_000)
     ap.add_argument("--aireml-m", type=int, default=80_000)
+    ap.add_argument("--cfg1-n", type=int, default=50_000, help="single-RHS workload individuals (0: skip)")
+    ap.add_argument("--cfg1-m", type=int, default=100_000, help="single-RHS workload SNPs")
     ap.add_argument("--grm-snps", type=int, default=20_000,
                     help="SNPs of the device-built GRM of the N-individual --grm leg (0: skip)")
     ap.add_argument("--calib", action="store_true",
@@ -468,6 +596,13 @@ def main():
     x64 = x1.download(np.zeros(nrec))
     log(f"FP64 1-RHS: {st64['iters']} iters, {st64['seconds']:.3f}s, pass1 {p64[0]['ms_avg']:.3f} ms, "
         f"pass2 {p64[1]['ms_avg']:.3f} ms")
+    # the plain FP64 form of the same solve: FP64 FMAs over the dosages on the vector ALUs
+    g.profile(False)
+    stv = g.pcg_dev(b1, x1, 1, tol=args.tol, rule=G.RULE_PER_COLUMN, precision=G.PREC_FP64_VALU)
+    xv = x1.download(np.zeros(nrec))
+    valu = {"solves_per_s": 1.0 / stv["seconds"], "iters": stv["iters"], "seconds": stv["seconds"],
+            "rel_err_vs_fp64": float(np.linalg.norm(xv - x64) / np.linalg.norm(x64))}
+    log(f"FP64 VALU 1-RHS: {stv['iters']} iters, {stv['seconds']:.3f}s")
 
     for w in range(args.warmup):
         log(f"warmup {w}")
@@ -514,7 +649,7 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         log("CPU baseline: the reference runPCGsolver ...")
         cpu = cpu_baseline_reference(g, c, y, kept, st64["iters"],
-                                     (min(args.cpu_ref_snps // 4, kept), min(args.cpu_ref_snps, kept)),
+                                     sorted({min(s_, kept) for s_ in args.cpu_ref_snps}),
                                      threads, (0.6, 0.4, 0.4, 0.2, 0.2))
         log("CPU baseline: the oracle's FP32 restatement ...")
         port = cpu_baseline(g, c, kept, st64["iters"], min(args.cpu_sample_snps, kept), threads)
@@ -545,7 +680,9 @@ def main():
                       "pass1_GBps": gbps64[0], "pass2_GBps": gbps64[1],
                       "hbm_frac": [x / PEAK_HBM_GBPS for x in gbps64],
                       "traffic": traffic64, "traffic_source": traffic64_src},
+        "fp64_valu_1rhs": valu,
         "mixed_vs_fp64_speedup": value * st64["seconds"],
+        "mixed_vs_fp64_valu_speedup": value * stv["seconds"],
         "mixed_solution_rel_err_y": err_y,
         "mixed_iters": [st["iters"] for st in stats],
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_INT8_TOPS,
@@ -560,6 +697,8 @@ def main():
         "ingest": ingest,
     }
     g.close()
+    if args.cfg1_n > 0:
+        out["single_rhs"] = single_rhs_leg(args)
     if args.grm_snps > 0:
         out["grm"] = grm_leg(args)
     print(json.dumps(out))

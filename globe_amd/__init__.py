@@ -135,6 +135,7 @@ def load_library(path: str = LIB_PATH):
         "globe_dev_copy": ([P, P, P, i64], C.c_int),
         "globe_profile_enable": ([P, C.c_int], C.c_int),
         "globe_force_variant": ([P, C.c_int], C.c_int),
+        "globe_device_memory": ([P, P, P], C.c_int),
         "globe_variant_counts": ([P, P], C.c_int),
         "globe_set_mixed_digits": ([P, C.c_int], C.c_int),
         "globe_profile_read": ([P, C.c_int, P], C.c_int),
@@ -154,6 +155,7 @@ def load_library(path: str = LIB_PATH):
         "globe_serve_take": ([P, i64, P, P], C.c_int),
         "globe_serve_finished": ([P, P, i32, P], C.c_int),
         "globe_serve_close": ([P], C.c_int),
+        "globe_serve_usage": ([P, P], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -474,6 +476,12 @@ class Globe:
         """int8 digits (2..4) of mixed-precision genetic products outside a PCG solve."""
         self._check(self.L.globe_set_mixed_digits(self.h, int(digits)))
 
+    def device_memory(self):
+        """(free, total) bytes of device memory."""
+        f, t = C.c_int64(), C.c_int64()
+        self._check(self.L.globe_device_memory(self.h, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
     # ---- contraction variants
     def force_variant(self, index):
         """index >= 0: every genotype contraction runs candidate index % count of its
@@ -501,6 +509,12 @@ class Globe:
     def grm_from_geno(self):
         """Build G = W W^T / M from the resident genotypes; the context then uses the GRM."""
         self._check(self.L.globe_grm_from_geno(self.h))
+
+    def grm_rows(self, r0, r1):
+        """Rows r0 .. r1 - 1 of the GRM's packed lower triangle (.grm.bin order)."""
+        out = np.zeros(r1 * (r1 + 1) // 2 - r0 * (r0 + 1) // 2, dtype=np.float32)
+        self._check(self.L.globe_get_grm_rows(self.h, int(r0), int(r1), _p(out)))
+        return out
 
     def grm_lt(self):
         lt = np.zeros(self.n * (self.n + 1) // 2, dtype=np.float32)
@@ -545,6 +559,13 @@ class Globe:
         del self._serve_nrec[rid]
         return x, {"id": res.id, "iters": res.iters, "converged": bool(res.converged), "ratio": res.ratio,
                    "t_submit": res.t_submit, "t_admit": res.t_admit, "t_done": res.t_done}
+
+    def serve_usage(self):
+        """Record pool: bytes in use, peak bytes in use, pool bytes, steps the queue's head
+        waited for room."""
+        out = np.zeros(4, dtype=np.int64)
+        self._check(self.L.globe_serve_usage(self.h, _p(out)))
+        return {"in_use": int(out[0]), "peak": int(out[1]), "pool": int(out[2]), "waits": int(out[3])}
 
     def serve_close(self):
         self._check(self.L.globe_serve_close(self.h))

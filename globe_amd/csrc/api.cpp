@@ -1250,6 +1250,15 @@ int globe_memcpy_d2h(globe_ctx* c, void* dst, const void* src, int64_t bytes) {
     API_END(c)
 }
 
+int globe_device_memory(globe_ctx* c, int64_t* free_bytes, int64_t* total_bytes) {
+    API_BEGIN(c)
+    size_t f = 0, t = 0;
+    HIP_CHECK(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = (int64_t)f;
+    if (total_bytes) *total_bytes = (int64_t)t;
+    API_END(c)
+}
+
 int globe_force_variant(globe_ctx* c, int index) {
     API_BEGIN(c)
     require(index >= -1, GLOBE_E_INVALID, "variant index must be >= -1");
@@ -1297,10 +1306,13 @@ int globe_profile_read(globe_ctx* c, int which, globe_kernel_stats* out) {
         st.launches += 1;
         st.ms_total += ms;
         st.bytes_per_launch = e.bytes;
-        st.flops_per_launch = e.flops;
+        st.flops_per_launch += e.flops;
         st.issued_ops_per_launch += e.issued;
     }
-    if (st.launches) st.issued_ops_per_launch /= (double)st.launches;
+    if (st.launches) {
+        st.flops_per_launch /= (double)st.launches;
+        st.issued_ops_per_launch /= (double)st.launches;
+    }
     *out = st;
     API_END(c)
 }

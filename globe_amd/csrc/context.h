@@ -181,6 +181,9 @@ int relaxed_limbs(double ratio, double tol);
 void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, double* GZ,
                        int64_t ldg, const int* skip, const int8_t* col_limbs, const double* theta_cols,
                        const double* U_given = nullptr, int64_t row0 = 0);
+// tune the packed-genotype contraction for every digit-column width a block of up to
+// max_cols columns reaches at `precision` (theta_cols: [max_cols][5] components)
+void pretune_geno_product(globe_ctx* c, int max_cols, int precision, const double* theta_cols);
 // Rows i >= row0 of G [e_j0 .. e_j0+k-1] (n x k, row0 <= j0) from the resident genotypes
 // (GRM builder: the lower triangle needs rows i >= j only)
 void geno_unit_columns(globe_ctx* c, int64_t j0, int k, double* GZ, int64_t row0);

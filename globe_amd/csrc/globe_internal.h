@@ -98,6 +98,10 @@ struct RecMeta {
     const int32_t* ntiles_col = nullptr;  // per-column tile counts (device) or null
     int32_t ntiles = 0;                   // tiles (max over columns when ntiles_col is set)
     const int64_t* nrec_col = nullptr;    // per-column record counts (device) or null
+    // per-column start (device, elements) of every record-space array -- structure and
+    // vectors -- when the columns live at their own offsets of one pool (the serving
+    // loop); null: column c starts at c times the stride of each array
+    const int64_t* col_off = nullptr;
     int64_t nrec = 0;                     // records (row count of the record matrices)
     int32_t nblk = 0;                     // blocks (shared structure)
     int32_t cap = 0;                      // records per tile; 0: thread-per-block kernels
@@ -130,6 +134,9 @@ void launch_gram_grouped(hipStream_t s, int64_t nrec, const double* A, int64_t l
                          const double* B, int64_t ldb, int kb, double* part, double* out);
 void launch_axpy_cols(hipStream_t s, int64_t nrec, double* y, int64_t ldy, const double* x,
                       int64_t ldx, int kx, const double* coef, int ky);
+// pooled column j: offset, tile and record counts, components (written on the device)
+void launch_set_column(hipStream_t s, int j, int64_t off, int32_t ntiles, int64_t nrec, const double theta[5],
+                       int64_t* col_off, int32_t* ntiles_col, int64_t* nrec_col, double* theta_cols);
 void launch_pcg_init(hipStream_t s, PcgState* st, const double* part_bb, const double* part_rr,
                      const double* part_rz, int nparts, int k);
 void launch_pcg_admit(hipStream_t s, PcgState* st, int j, const double* part_bb, int nparts_bb,
@@ -137,9 +144,12 @@ void launch_pcg_admit(hipStream_t s, PcgState* st, int j, const double* part_bb,
 // x += alpha p, r -= alpha Vp per active column with the r.r partials; the last
 // workgroup then runs the convergence test (k_pcg_check's rules) on them
 void launch_pcg_xr(hipStream_t s, PcgState* st, int64_t nrec, double* x, int64_t ldx, double* r,
-                   const double* p, const double* vp, int64_t ld, int k, double* part_rr);
+                   const double* p, const double* vp, int64_t ld, int k, double* part_rr,
+                   const int64_t* col_off = nullptr, const int64_t* col_cnt = nullptr);
+// col_off / col_cnt (device, may be null): column starts and record counts of pooled
+// columns (otherwise column c starts at c ld and holds nrec records)
 void launch_pcg_p(hipStream_t s, PcgState* st, int64_t nrec, double* p, const double* z,
-                  int64_t ld, int k);
+                  int64_t ld, int k, const int64_t* col_off = nullptr, const int64_t* col_cnt = nullptr);
 void launch_bhat(hipStream_t s, int32_t nblk, const int64_t* blk_off, const int32_t* blk_ind,
                  const double* times, const double* v, const double* theta, double* bhat);
 void launch_f_theta(hipStream_t s, int32_t nblk, const int64_t* blk_off, const int32_t* blk_ind,

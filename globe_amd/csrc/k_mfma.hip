@@ -492,6 +492,18 @@ __global__ __launch_bounds__(256) void k_quantize(const double* __restrict__ v, 
     }
 }
 
+// Pseudo-random right-hand sides in (-1, 1) (splitmix64 of the element index) for the
+// tuner's warm-up: live-like digits rather than zeros, whose MFMAs draw less power.
+__global__ __launch_bounds__(256) void k_tune_fill(double* __restrict__ v, int64_t count, uint64_t seed) {
+    for (int64_t q = blockIdx.x * 256ll + threadIdx.x; q < count; q += (int64_t)gridDim.x * 256) {
+        uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(q + 1);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        v[q] = (double)(int64_t)(z >> 11) * 0x1.0p-52 - 1.0;
+    }
+}
+
 // Pass 1 of G e_j for the unit vectors e_j0 .. e_j0+k-1 without a contraction: W^T e_j
 // is individual j's standardised row, so u[c][s] = inv_s^2 / M (g_{s, j0+c} - 2 f_s), the
 // pass-1 epilogue's value (sig2g = 1, T = 1). g from the individual-major dosage rows.
@@ -898,6 +910,35 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
 }  // namespace globe
 
 namespace globe {
+
+// Tune the contraction for every digit-column width class (thin, 1 to 4 wide fragments of
+// 32) that a block of up to max_cols columns can reach at `precision`, so that no shape is
+// first tuned inside a running solve.
+void pretune_geno_product(globe_ctx* c, int max_cols, int precision, const double* theta_cols) {
+    if (c->gen != 1 || c->m == 0 || precision == GLOBE_PREC_FP64_VALU) return;
+    const int L = precision == GLOBE_PREC_FP64 ? kLimbsFP64 : kLimbsMixed;
+    const int maxJ = std::min(128, L * std::max(1, max_cols));
+    const int64_t n = c->n;
+    const int kmax = (128 + L - 1) / L;
+    c->w_P2.ensure(sizeof(double) * (size_t)n * kmax);
+    c->w_P1.ensure(sizeof(double) * (size_t)n * kmax);
+    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n * kmax + 255) / 256, 2048));
+    k_tune_fill<<<gx, 256, 0, c->stream>>>(c->w_P2.as<double>(), n * kmax, 0x5EED5EEDull);
+    HIP_CHECK(hipGetLastError());
+    // classes of J: [1, 16] thin, then 1, 2, 3, 4 wide fragments
+    const int kClass[5][2] = {{1, 16}, {17, 32}, {33, 64}, {65, 96}, {97, 128}};
+    for (const auto& cl : kClass) {
+        const int lo = cl[0], hi = cl[1];
+        if (lo > maxJ) break;
+        const int J = std::min(hi, maxJ);
+        const int kk = (J + L - 1) / L;
+        std::vector<int8_t> limbs((size_t)kk, (int8_t)L);
+        limbs[(size_t)kk - 1] = (int8_t)(J - L * (kk - 1));
+        mfma_geno_product(c, c->w_P2.as<double>(), n, kk, c->w_P1.as<double>(), n, nullptr, limbs.data(),
+                          theta_cols);
+    }
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+}
 
 // Rows row0 .. n - 1 of GZ (n x k) = G [e_j0 .. e_j0+k-1] for the GRM builder: pass 1 is
 // the direct row extraction k_unit_u, pass 2 the contraction on kBuilderLimbs digits.

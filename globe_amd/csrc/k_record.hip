@@ -54,6 +54,10 @@ __device__ inline bool arrive_last(unsigned int* ticket) {
     return last != 0;
 }
 
+// Start of column c of a record-space array with column stride ld: its own pool offset
+// when the columns are pooled, else c ld.
+__device__ inline int64_t col_start(const RecMeta& m, int c, int64_t ld) { return m.col_off ? m.col_off[c] : c * ld; }
+
 // ---------------------------------------------------------------------------
 // S0[c][i] = sum_{r in block} x[c][r], S1[c][i] = sum t_r x[c][r]
 // (PCGsolver.cpp:1119-1125 scatter, and the t-weighted sum the factorized time
@@ -205,16 +209,18 @@ __global__ __launch_bounds__(kBlock) void k_record_out(
     const int c = blockIdx.y;
     const double se = th[c * 5 + 0], sb0 = th[c * 5 + 2], sb01 = th[c * 5 + 3],
                  sb1 = th[c * 5 + 4];
-    const int32_t* indiv = m.indiv + c * m.rec_ld;
-    const double* t = m.times + c * m.rec_ld;
+    const int32_t* indiv = m.indiv + col_start(m, c, m.rec_ld);
+    const double* t = m.times + col_start(m, c, m.rec_ld);
     const int64_t nvalid = m.nrec_col ? m.nrec_col[c] : m.nrec;
-    const double* xc = x + c * ldx;
-    double* yc = y + c * ldy;
+    const double* xc = x + col_start(m, c, ldx);
+    double* yc = y + col_start(m, c, ldy);
+    // pooled columns hold exactly their own records; strided ones are padded to m.nrec
+    const int64_t nloop = m.col_off ? nvalid : m.nrec;
     const double* g = GZ + c * ldn;
     const double* s0 = S0 + c * ldn;
     const double* s1 = S1 + c * ldn;
     double acc = 0.0;
-    for (int64_t r = blockIdx.x * (int64_t)kBlock + threadIdx.x; r < m.nrec;
+    for (int64_t r = blockIdx.x * (int64_t)kBlock + threadIdx.x; r < nloop;
          r += (int64_t)gridDim.x * kBlock) {
         double v = 0.0;  // records past a column's own count are padding
         if (r < nvalid) {
@@ -246,11 +252,11 @@ __global__ __launch_bounds__(kBlock) void k_seg_sums_tiled(RecMeta m, const doub
     double* sx = sm;
     double* st = sm + m.cap;
     const int c = blockIdx.y;
-    const double* xc = x + c * ldx;
-    const double* t = m.times + c * m.rec_ld;
-    const int64_t* off = m.blk_off + c * m.blk_ld;
-    const int32_t* ind = m.blk_ind + c * m.blk_ld;
-    const int32_t* tile_blk = m.tile_blk + c * m.tile_ld;
+    const double* xc = x + col_start(m, c, ldx);
+    const double* t = m.times + col_start(m, c, m.rec_ld);
+    const int64_t* off = m.blk_off + col_start(m, c, m.blk_ld);
+    const int32_t* ind = m.blk_ind + col_start(m, c, m.blk_ld);
+    const int32_t* tile_blk = m.tile_blk + col_start(m, c, m.tile_ld);
     const int32_t ntiles = m.ntiles_col ? m.ntiles_col[c] : m.ntiles;
     for (int32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int32_t b0 = tile_blk[tile], b1 = tile_blk[tile + 1];
@@ -299,13 +305,13 @@ __global__ __launch_bounds__(kBlock) void k_precond_tiled(
     double* sc1 = sm + 3 * m.cap;
     const double se = th[c * 5 + 0], sg = th[c * 5 + 1], sb0 = th[c * 5 + 2],
                  sb01 = th[c * 5 + 3], sb1 = th[c * 5 + 4];
-    const double* rc = r + c * ldr;
-    double* zc = z + c * ldz;
-    const double* t = m.times + c * m.rec_ld;
-    const int32_t* rec_blk = m.rec_blk + c * m.rec_ld;
-    const int64_t* off = m.blk_off + c * m.blk_ld;
-    const int32_t* ind = m.blk_ind + c * m.blk_ld;
-    const int32_t* tile_blk = m.tile_blk + c * m.tile_ld;
+    const double* rc = r + col_start(m, c, ldr);
+    double* zc = z + col_start(m, c, ldz);
+    const double* t = m.times + col_start(m, c, m.rec_ld);
+    const int32_t* rec_blk = m.rec_blk + col_start(m, c, m.rec_ld);
+    const int64_t* off = m.blk_off + col_start(m, c, m.blk_ld);
+    const int32_t* ind = m.blk_ind + col_start(m, c, m.blk_ld);
+    const int32_t* tile_blk = m.tile_blk + col_start(m, c, m.tile_ld);
     const int32_t ntiles = m.ntiles_col ? m.ntiles_col[c] : m.ntiles;
     const double inv_se = 1.0 / se;
     double arz = 0.0, arr = 0.0;
@@ -599,23 +605,44 @@ __global__ __launch_bounds__(64) void k_pcg_admit(PcgState* st, int j, const dou
     if (act) st->done = 0;
 }
 
+// Pooled columns (col_off set): column c starts at col_off[c] of every vector and holds
+// col_cnt[c] records; otherwise it starts at c ld (c ldx for x) and holds nrec.
+// Place column j of a pooled block (the serving loop): its pool offset, tile and record
+// counts and variance components, passed by value so that no host buffer is read.
+__global__ void k_set_column(int j, int64_t off, int32_t ntiles, int64_t nrec, double t0, double t1, double t2,
+                             double t3, double t4, int64_t* col_off, int32_t* ntiles_col, int64_t* nrec_col,
+                             double* theta) {
+    if (threadIdx.x != 0) return;
+    col_off[j] = off;
+    ntiles_col[j] = ntiles;
+    nrec_col[j] = nrec;
+    theta[5 * j + 0] = t0;
+    theta[5 * j + 1] = t1;
+    theta[5 * j + 2] = t2;
+    theta[5 * j + 3] = t3;
+    theta[5 * j + 4] = t4;
+}
+
 __global__ __launch_bounds__(kBlock) void k_pcg_xr(PcgState* st, int64_t nrec, double* __restrict__ x,
                                                    int64_t ldx, double* __restrict__ r,
                                                    const double* __restrict__ p,
                                                    const double* __restrict__ vp, int64_t ld,
-                                                   double* part_rr) {
+                                                   double* part_rr, const int64_t* __restrict__ col_off,
+                                                   const int64_t* __restrict__ col_cnt) {
     if (st->done) return;
     const int c = blockIdx.y;
     const double a = st->alpha[c];
     const bool act = st->active[c] != 0;
+    const int64_t o = col_off ? col_off[c] : c * ld, ox = col_off ? col_off[c] : c * ldx;
+    const int64_t nc = col_cnt ? col_cnt[c] : nrec;
     double acc = 0.0;
-    for (int64_t q = blockIdx.x * (int64_t)kBlock + threadIdx.x; q < nrec;
+    for (int64_t q = blockIdx.x * (int64_t)kBlock + threadIdx.x; q < nc;
          q += (int64_t)gridDim.x * kBlock) {
-        double rv = r[c * ld + q];
+        double rv = r[o + q];
         if (act) {
-            x[c * ldx + q] += a * p[c * ld + q];
-            rv -= a * vp[c * ld + q];
-            r[c * ld + q] = rv;
+            x[ox + q] += a * p[o + q];
+            rv -= a * vp[o + q];
+            r[o + q] = rv;
         }
         acc += rv * rv;
     }
@@ -626,14 +653,18 @@ __global__ __launch_bounds__(kBlock) void k_pcg_xr(PcgState* st, int64_t nrec, d
 }
 
 __global__ __launch_bounds__(kBlock) void k_pcg_p(PcgState* st, int64_t nrec, double* __restrict__ p,
-                                                  const double* __restrict__ z, int64_t ld) {
+                                                  const double* __restrict__ z, int64_t ld,
+                                                  const int64_t* __restrict__ col_off,
+                                                  const int64_t* __restrict__ col_cnt) {
     if (st->done) return;
     const int c = blockIdx.y;
     if (!st->active[c]) return;
     const double b = st->beta[c];
-    for (int64_t q = blockIdx.x * (int64_t)kBlock + threadIdx.x; q < nrec;
+    const int64_t o = col_off ? col_off[c] : c * ld;
+    const int64_t nc = col_cnt ? col_cnt[c] : nrec;
+    for (int64_t q = blockIdx.x * (int64_t)kBlock + threadIdx.x; q < nc;
          q += (int64_t)gridDim.x * kBlock)
-        p[c * ld + q] = z[c * ld + q] + b * p[c * ld + q];
+        p[o + q] = z[o + q] + b * p[o + q];
 }
 
 // bhat[2i] = sum v_r (sig2b0 + sigb01 t_r), bhat[2i+1] = sum v_r (sigb01 + sig2b1 t_r)
@@ -833,6 +864,13 @@ int launch_logdet_q(hipStream_t s, int32_t nblk, const int64_t* off, const int32
     return g;
 }
 
+void launch_set_column(hipStream_t s, int j, int64_t off, int32_t ntiles, int64_t nrec, const double theta[5],
+                       int64_t* col_off, int32_t* ntiles_col, int64_t* nrec_col, double* theta_cols) {
+    k_set_column<<<1, 64, 0, s>>>(j, off, ntiles, nrec, theta[0], theta[1], theta[2], theta[3], theta[4], col_off,
+                                  ntiles_col, nrec_col, theta_cols);
+    LAUNCH_CHECK();
+}
+
 void launch_pcg_admit(hipStream_t s, PcgState* st, int j, const double* part_bb, int nparts_bb,
                       const double* part_rr, const double* part_rz, int nparts) {
     k_pcg_admit<<<1, 64, 0, s>>>(st, j, part_bb, nparts_bb, part_rr, part_rz, nparts);
@@ -847,18 +885,19 @@ void launch_pcg_init(hipStream_t s, PcgState* st, const double* part_bb, const d
 
 
 void launch_pcg_xr(hipStream_t s, PcgState* st, int64_t nrec, double* x, int64_t ldx, double* r,
-                   const double* p, const double* vp, int64_t ld, int k, double* part_rr) {
+                   const double* p, const double* vp, int64_t ld, int k, double* part_rr,
+                   const int64_t* col_off, const int64_t* col_cnt) {
     dim3 g(parts_for(nrec), k);
-    k_pcg_xr<<<g, kBlock, 0, s>>>(st, nrec, x, ldx, r, p, vp, ld, part_rr);
+    k_pcg_xr<<<g, kBlock, 0, s>>>(st, nrec, x, ldx, r, p, vp, ld, part_rr, col_off, col_cnt);
     LAUNCH_CHECK();
 }
 
 
 
 void launch_pcg_p(hipStream_t s, PcgState* st, int64_t nrec, double* p, const double* z,
-                  int64_t ld, int k) {
+                  int64_t ld, int k, const int64_t* col_off, const int64_t* col_cnt) {
     dim3 g(parts_for(nrec), k);
-    k_pcg_p<<<g, kBlock, 0, s>>>(st, nrec, p, z, ld);
+    k_pcg_p<<<g, kBlock, 0, s>>>(st, nrec, p, z, ld, col_off, col_cnt);
     LAUNCH_CHECK();
 }
 

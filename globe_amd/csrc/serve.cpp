@@ -1,14 +1,25 @@
 // Serving loop: one resident cohort, solve requests admitted into the columns of one
 // shared block PCG as slots and HBM free up (continuous batching).
 //
-// Slot j owns column j of every record-space matrix ([slots][cap] doubles: b, x, r,
-// z, p, V p) and its own record structure at the same column stride (individual and
-// time per record, individual blocks, LDS tiles) -- the kernels read it through a
-// RecMeta with per-column strides and counts. Records past a request's own count are
-// zero in b, x, r, z, p and V p, so the column-wise PCG kernels run over the full
-// capacity unchanged. The genotype contraction takes all occupied columns at once;
-// its fixed-point scale, the PCG scalars and the record structure are per column, so
-// a request's arithmetic does not depend on which other requests share the block.
+// Memory: one record pool sized by the HBM budget holds every admitted request's
+// record-space data -- its record structure (individual and time per record, individual
+// blocks, LDS tiles) and its six PCG vectors (b, x, r, z, p, V p) -- as one contiguous
+// range of nrec + 1 elements rounded up to 64, placed first-fit. Column j of the block
+// reads its range through a per-column offset (RecMeta::col_off), and every record-space
+// kernel loops over that column's own records, so a request costs the HBM and the
+// record-space work of its own size. A request is admitted (FIFO) when a slot is free
+// and a pool range of its size is.
+//
+// Time: admissions are staged one step ahead. After the step's iteration is enqueued,
+// the next requests' data is uploaded on a copy stream while the GPU iterates; the next
+// step joins them to the block (the compute stream waits on the copies' event, then
+// starts each new column's recurrence) before its iteration. Finished columns are copied
+// out after each iteration and their ranges returned to the pool.
+//
+// The genotype contraction takes all occupied columns at once; its fixed-point scale,
+// the PCG scalars and the record structure are per column, and the reductions of a column
+// are split over a grid fixed by the per-request record cap, so a request's arithmetic
+// does not depend on which other requests share the block.
 #include <algorithm>
 #include <chrono>
 #include <climits>
@@ -27,47 +38,51 @@ struct ServeState {
         std::vector<double> times, y;
         double theta[5] = {0, 0, 0, 0, 0};
         double t_submit = 0;
-        int64_t bytes = 0;
+        int64_t span = 0;   // pool elements
     };
     struct Done {
         std::vector<double> x;
         globe_serve_result res{};
     };
+    enum SlotState { kFree = 0, kStaged = 1, kBusy = 2 };
     globe_serve_opts o{};
     int slots = 0;
-    int64_t cap = 0;
-    globe::DevBuf B, X, R, Z, P, VP;                           // [slots][cap] doubles
-    // per-slot structure: per record [slots][cap], per block and per tile [slots][cap + 1]
-    // (blk_off, blk_ind and tile_blk share the block stride blk_ld = tile_ld = cap + 1)
-    globe::DevBuf indiv, times, rec_blk, blk_off, blk_ind, tile_blk;
-    globe::DevBuf ntiles_col, nrec_col, theta, state;
+    int64_t cap = 0;        // records of the largest request
+    int64_t pool = 0;       // pool elements
+    globe::DevBuf B, X, R, Z, P, VP;                          // [pool] doubles
+    globe::DevBuf indiv, times, rec_blk, blk_off, blk_ind, tile_blk;  // [pool]
+    globe::DevBuf col_off, ntiles_col, nrec_col, theta, state;
     globe::DevBuf part_a, part_b, part_c;
+    hipStream_t copy = nullptr;   // uploads of staged requests
+    hipEvent_t staged_ev = nullptr;
+    std::map<int64_t, int64_t> free_ranges;  // pool offset -> length
     std::deque<Request> queue;
     std::vector<Request> slot_req;
-    std::vector<char> busy;
+    std::vector<int> slot_state;
+    std::vector<int64_t> slot_off;
     std::vector<int> admit_iter;
     std::vector<double> t_admit;
     std::map<int64_t, Done> done;
-    int64_t next_id = 0, bytes_in_use = 0, budget = 0;
+    int64_t next_id = 0;
     int global_iter = 0;
+    int64_t peak_in_use = 0, waits_for_room = 0;
+    bool staged_pending = false;
     std::chrono::steady_clock::time_point t0;
     globe::PcgState* h_state = nullptr;
 
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
-    // record structure of slots [j0, j0 + k)
+    // record structure of columns j0 .. (pooled)
     globe::RecMeta meta(int j0) const {
         globe::RecMeta m;
-        m.indiv = indiv.as<int32_t>() + j0 * cap;
-        m.times = times.as<double>() + j0 * cap;
-        m.rec_blk = rec_blk.as<int32_t>() + j0 * cap;
-        m.rec_ld = cap;
-        m.blk_off = blk_off.as<int64_t>() + j0 * (cap + 1);
-        m.blk_ind = blk_ind.as<int32_t>() + j0 * (cap + 1);
-        m.blk_ld = cap + 1;
-        m.tile_blk = tile_blk.as<int32_t>() + j0 * (cap + 1);
-        m.tile_ld = cap + 1;
+        m.indiv = indiv.as<int32_t>();
+        m.times = times.as<double>();
+        m.rec_blk = rec_blk.as<int32_t>();
+        m.blk_off = blk_off.as<int64_t>();
+        m.blk_ind = blk_ind.as<int32_t>();
+        m.tile_blk = tile_blk.as<int32_t>();
+        m.col_off = col_off.as<int64_t>() + j0;
         m.ntiles_col = ntiles_col.as<int32_t>() + j0;
         // two consecutive tiles hold more than kTileRecords records, so a column has
         // at most 2 cap / kTileRecords + 1 tiles (the launch grid covers that many)
@@ -77,20 +92,50 @@ struct ServeState {
         m.cap = globe::kTileRecords;
         return m;
     }
+    // first-fit range of `len` pool elements, or -1
+    int64_t alloc(int64_t len) {
+        for (auto it = free_ranges.begin(); it != free_ranges.end(); ++it) {
+            if (it->second < len) continue;
+            const int64_t off = it->first, rest = it->second - len;
+            free_ranges.erase(it);
+            if (rest > 0) free_ranges.emplace(off + len, rest);
+            return off;
+        }
+        return -1;
+    }
+    void release(int64_t off, int64_t len) {
+        auto it = free_ranges.emplace(off, len).first;
+        auto nx = std::next(it);
+        if (nx != free_ranges.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_ranges.erase(nx);
+        }
+        if (it != free_ranges.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_ranges.erase(it);
+            }
+        }
+    }
 };
 
 namespace globe {
 
 namespace {
 
+// device bytes per pool element: six FP64 vectors and the record structure (individual,
+// time, block of each record; block offsets, block individuals, tile starts)
+constexpr int64_t kPoolElemBytes = 6 * 8 + 4 + 8 + 4 + 8 + 4 + 4;
+
+// pool elements a request of nrec records occupies: its records, one more block offset,
+// rounded to 64 so that every range starts 512-byte aligned
+int64_t request_span(int64_t nrec) { return round_up(nrec + 1, 64); }
+
 ServeState& server(globe_ctx* c) {
     require(c->serve != nullptr, GLOBE_E_STATE, "serving loop not open (globe_serve_open)");
     return *c->serve;
 }
-
-// bytes of device memory a request occupies while admitted: six record vectors and
-// its record structure
-int64_t request_bytes(int64_t nrec) { return nrec * (6 * 8 + 4 + 8 + 4 + 8 + 4 + 4); }
 
 void prepare(globe_ctx* c, const ServeState& S, const globe_request* q, ServeState::Request& r) {
     require(q != nullptr && q->nrec >= 1, GLOBE_E_INVALID, "empty request");
@@ -138,84 +183,107 @@ void prepare(globe_ctx* c, const ServeState& S, const globe_request* q, ServeSta
     }
     r.tile_blk.push_back(r.nblk);
     r.ntiles = (int32_t)r.tile_blk.size() - 1;
-    r.bytes = request_bytes(nrec);
+    r.span = request_span(nrec);
 }
 
-// place request r in free slot j and start its column
-void admit(globe_ctx* c, ServeState& S, int j, ServeState::Request&& r) {
-    hipStream_t s = c->stream;
-    const int64_t cap = S.cap, nrec = r.nrec;
-    auto h2d = [&](void* dst, const void* src, size_t bytes) {
-        if (bytes) HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
-    };
-    h2d(S.indiv.as<int32_t>() + j * cap, r.indiv.data(), sizeof(int32_t) * nrec);
-    h2d(S.times.as<double>() + j * cap, r.times.data(), sizeof(double) * nrec);
-    h2d(S.rec_blk.as<int32_t>() + j * cap, r.rec_blk.data(), sizeof(int32_t) * nrec);
-    h2d(S.blk_off.as<int64_t>() + j * (cap + 1), r.blk_off.data(), sizeof(int64_t) * r.blk_off.size());
-    h2d(S.blk_ind.as<int32_t>() + j * (cap + 1), r.blk_ind.data(), sizeof(int32_t) * r.blk_ind.size());
-    h2d(S.tile_blk.as<int32_t>() + j * (cap + 1), r.tile_blk.data(), sizeof(int32_t) * r.tile_blk.size());
-    h2d(S.ntiles_col.as<int32_t>() + j, &r.ntiles, sizeof(int32_t));
-    h2d(S.nrec_col.as<int64_t>() + j, &r.nrec, sizeof(int64_t));
-    h2d(S.theta.as<double>() + 5 * j, r.theta, sizeof(double) * 5);
-    double* b = S.B.as<double>() + j * cap;
-    double* x = S.X.as<double>() + j * cap;
-    double* rr = S.R.as<double>() + j * cap;
-    double* z = S.Z.as<double>() + j * cap;
-    double* p = S.P.as<double>() + j * cap;
-    h2d(b, r.y.data(), sizeof(double) * nrec);
-    if (nrec < cap) {
-        launch_fill(s, b + nrec, cap - nrec, 0.0);
-        launch_fill(s, z + nrec, cap - nrec, 0.0);
+// Reserve a slot and a pool range for the queue's head and upload its data on the copy
+// stream (it joins the block at the next admit_staged).
+bool stage_next(ServeState& S) {
+    if (S.queue.empty()) return false;
+    int j = 0;
+    while (j < S.slots && S.slot_state[j] != ServeState::kFree) ++j;
+    if (j == S.slots) return false;
+    const int64_t off = S.alloc(S.queue.front().span);
+    if (off < 0) {  // first in, first admitted: the head waits for room
+        ++S.waits_for_room;
+        return false;
     }
-    launch_fill(s, x, cap, 0.0);
-    launch_copy_cols(s, cap, b, cap, rr, cap, 1);
-    // z = Q^-1 r, partial r.z and r.r; |b| partials; p = z; then the column's scalars
-    const RecMeta m = S.meta(j);
-    launch_precond(s, m, c->d_gdiag.as<double>(), S.theta.as<double>() + 5 * j, rr, cap, 1, z, cap,
-                   S.part_c.as<double>(), S.part_b.as<double>(), nullptr, 0);
-    launch_dot_parts(s, cap, b, cap, b, cap, 1, S.part_a.as<double>());
-    launch_copy_cols(s, cap, z, cap, p, cap, 1);
-    launch_pcg_admit(s, S.state.as<PcgState>(), j, S.part_a.as<double>(), parts_for(cap),
-                     S.part_b.as<double>(), S.part_c.as<double>(), precond_parts(m));
-    // the copies read the request's host vectors: finish them before r is moved
-    HIP_CHECK(hipStreamSynchronize(s));
-    S.bytes_in_use += r.bytes;
-    S.busy[j] = 1;
-    S.admit_iter[j] = S.global_iter;
-    S.t_admit[j] = S.now();
-    S.slot_req[j] = std::move(r);
+    ServeState::Request r = std::move(S.queue.front());
+    S.queue.pop_front();
+    const int64_t nrec = r.nrec;
+    auto h2d = [&](void* dst, const void* src, size_t bytes) {
+        if (bytes) HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S.copy));
+    };
+    h2d(S.indiv.as<int32_t>() + off, r.indiv.data(), sizeof(int32_t) * nrec);
+    h2d(S.times.as<double>() + off, r.times.data(), sizeof(double) * nrec);
+    h2d(S.rec_blk.as<int32_t>() + off, r.rec_blk.data(), sizeof(int32_t) * nrec);
+    h2d(S.blk_off.as<int64_t>() + off, r.blk_off.data(), sizeof(int64_t) * r.blk_off.size());
+    h2d(S.blk_ind.as<int32_t>() + off, r.blk_ind.data(), sizeof(int32_t) * r.blk_ind.size());
+    h2d(S.tile_blk.as<int32_t>() + off, r.tile_blk.data(), sizeof(int32_t) * r.tile_blk.size());
+    h2d(S.B.as<double>() + off, r.y.data(), sizeof(double) * nrec);
+    S.slot_state[j] = ServeState::kStaged;
+    S.slot_off[j] = off;
+    S.slot_req[j] = std::move(r);  // its host vectors stay alive until the slot retires
+    S.staged_pending = true;
+    int64_t f = 0;
+    for (const auto& kv : S.free_ranges) f += kv.second;
+    S.peak_in_use = std::max(S.peak_in_use, S.pool - f);
+    return true;
 }
 
-// one iteration of the block PCG over slots [0, k)
+// Start the columns of every staged request: the compute stream waits for the uploads,
+// then each column's x = 0, r = b, z = Q^-1 r, p = z and its recurrence scalars.
+void admit_staged(globe_ctx* c, ServeState& S) {
+    if (!S.staged_pending) return;
+    hipStream_t s = c->stream;
+    HIP_CHECK(hipEventRecord(S.staged_ev, S.copy));
+    HIP_CHECK(hipStreamWaitEvent(s, S.staged_ev, 0));
+    for (int j = 0; j < S.slots; ++j) {
+        if (S.slot_state[j] != ServeState::kStaged) continue;
+        const ServeState::Request& r = S.slot_req[j];
+        const int64_t off = S.slot_off[j], nrec = r.nrec;
+        launch_set_column(s, j, off, r.ntiles, nrec, r.theta, S.col_off.as<int64_t>(), S.ntiles_col.as<int32_t>(),
+                          S.nrec_col.as<int64_t>(), S.theta.as<double>());
+        double* b = S.B.as<double>() + off;
+        double* x = S.X.as<double>() + off;
+        double* rr = S.R.as<double>() + off;
+        double* z = S.Z.as<double>() + off;
+        double* p = S.P.as<double>() + off;
+        launch_fill(s, x, nrec, 0.0);
+        launch_copy_cols(s, nrec, b, nrec, rr, nrec, 1);
+        // z = Q^-1 r with the partial r.z and r.r; |b| partials; p = z; then the scalars
+        const RecMeta m = S.meta(j);
+        launch_precond(s, m, c->d_gdiag.as<double>(), S.theta.as<double>() + 5 * j, S.R.as<double>(), 0, 1,
+                       S.Z.as<double>(), 0, S.part_c.as<double>(), S.part_b.as<double>(), nullptr, 0);
+        launch_dot_parts(s, nrec, b, nrec, b, nrec, 1, S.part_a.as<double>());
+        launch_copy_cols(s, nrec, z, nrec, p, nrec, 1);
+        launch_pcg_admit(s, S.state.as<PcgState>(), j, S.part_a.as<double>(), parts_for(nrec),
+                         S.part_b.as<double>(), S.part_c.as<double>(), precond_parts(m));
+        S.slot_state[j] = ServeState::kBusy;
+        S.admit_iter[j] = S.global_iter;
+        S.t_admit[j] = S.now();
+    }
+    S.staged_pending = false;
+}
+
+// one iteration of the block PCG over columns [0, k)
 void iterate(globe_ctx* c, ServeState& S, int k) {
     hipStream_t s = c->stream;
     const int32_t n = c->n;
     const int64_t cap = S.cap;
     PcgState* st = S.state.as<PcgState>();
     const int* skip = &st->done;
-    c->w_S0.ensure(sizeof(double) * (size_t)n * k);
-    c->w_S1.ensure(sizeof(double) * (size_t)n * k);
-    c->w_GZ.ensure(sizeof(double) * (size_t)n * k);
-    double* S0 = c->w_S0.as<double>();
-    double* S1 = c->w_S1.as<double>();
-    double* GZ = c->w_GZ.as<double>();
+    double* S0 = ws<double>(c, c->w_S0, sizeof(double) * (size_t)n * k);
+    double* S1 = ws<double>(c, c->w_S1, sizeof(double) * (size_t)n * k);
+    double* GZ = ws<double>(c, c->w_GZ, sizeof(double) * (size_t)n * k);
     const RecMeta m = S.meta(0);
     const double* th = S.theta.as<double>();
     double* part_a = S.part_a.as<double>();
     double* part_b = S.part_b.as<double>();
     double* part_c = S.part_c.as<double>();
+    const int64_t* off = S.col_off.as<int64_t>();
+    const int64_t* cnt = S.nrec_col.as<int64_t>();
     // V p: individuals outside a request's subset keep S0 = S1 = 0
     launch_fill(s, S0, (int64_t)n * k, 0.0);
     launch_fill(s, S1, (int64_t)n * k, 0.0);
-    launch_seg_sums(s, m, S.P.as<double>(), cap, k, S0, S1, n, skip);
+    launch_seg_sums(s, m, S.P.as<double>(), 0, k, S0, S1, n, skip);
     genetic_product(c, S0, k, GZ, S.o.precision, skip, th);
-    launch_record_out(s, m, S.P.as<double>(), cap, k, GZ, S0, S1, n, th, S.VP.as<double>(), cap, part_a,
-                      skip, st);
-    launch_pcg_xr(s, st, cap, S.X.as<double>(), cap, S.R.as<double>(), S.P.as<double>(),
-                  S.VP.as<double>(), cap, k, part_b);
-    launch_precond(s, m, c->d_gdiag.as<double>(), th, S.R.as<double>(), cap, k, S.Z.as<double>(), cap,
-                   part_c, nullptr, st, 0, true);
-    launch_pcg_p(s, st, cap, S.P.as<double>(), S.Z.as<double>(), cap, k);
+    launch_record_out(s, m, S.P.as<double>(), 0, k, GZ, S0, S1, n, th, S.VP.as<double>(), 0, part_a, skip, st);
+    launch_pcg_xr(s, st, cap, S.X.as<double>(), 0, S.R.as<double>(), S.P.as<double>(), S.VP.as<double>(), 0, k,
+                  part_b, off, cnt);
+    launch_precond(s, m, c->d_gdiag.as<double>(), th, S.R.as<double>(), 0, k, S.Z.as<double>(), 0, part_c,
+                   nullptr, st, 0, true);
+    launch_pcg_p(s, st, cap, S.P.as<double>(), S.Z.as<double>(), 0, k, off, cnt);
     ++S.global_iter;
 }
 
@@ -223,8 +291,12 @@ void iterate(globe_ctx* c, ServeState& S, int k) {
 
 void serve_release(globe_ctx* c) {
     if (!c->serve) return;
-    if (c->serve->h_state) (void)hipHostFree(c->serve->h_state);
-    delete c->serve;
+    ServeState* S = c->serve;
+    if (S->copy) (void)hipStreamSynchronize(S->copy);
+    if (S->staged_ev) (void)hipEventDestroy(S->staged_ev);
+    if (S->copy) (void)hipStreamDestroy(S->copy);
+    if (S->h_state) (void)hipHostFree(S->h_state);
+    delete S;
     c->serve = nullptr;
 }
 
@@ -240,6 +312,7 @@ int globe_serve_open(globe_ctx* c, const globe_serve_opts* o) {
     require(c->has_cohort && c->gen != 0, GLOBE_E_STATE, "cohort and genetics required");
     require(o->slots >= 1 && o->slots <= kMaxCols, GLOBE_E_INVALID, "slots must be in [1, 64]");
     require(o->max_records >= 1, GLOBE_E_INVALID, "max_records must be positive");
+    require(o->hbm_budget >= 0, GLOBE_E_INVALID, "hbm_budget must be >= 0");
     require(o->tol > 0.0 && o->max_iter >= 1, GLOBE_E_INVALID, "tol and max_iter must be positive");
     require(o->precision >= GLOBE_PREC_FP64 && o->precision <= GLOBE_PREC_FP64_VALU, GLOBE_E_INVALID,
             "unknown precision");
@@ -248,33 +321,32 @@ int globe_serve_open(globe_ctx* c, const globe_serve_opts* o) {
     c->serve = S;
     S->o = *o;
     S->slots = o->slots;
-    S->cap = round_up(o->max_records, 64);
-    S->budget = o->hbm_budget > 0 ? o->hbm_budget : request_bytes(S->cap) * S->slots;
-    const size_t vec = sizeof(double) * (size_t)S->cap * S->slots;
-    for (DevBuf* d : {&S->B, &S->X, &S->R, &S->Z, &S->P, &S->VP}) d->ensure(vec);
-    S->indiv.ensure(sizeof(int32_t) * (size_t)S->cap * S->slots);
-    S->times.ensure(sizeof(double) * (size_t)S->cap * S->slots);
-    S->rec_blk.ensure(sizeof(int32_t) * (size_t)S->cap * S->slots);
-    S->blk_off.ensure(sizeof(int64_t) * (size_t)(S->cap + 1) * S->slots);
-    S->blk_ind.ensure(sizeof(int32_t) * (size_t)(S->cap + 1) * S->slots);
-    S->tile_blk.ensure(sizeof(int32_t) * (size_t)(S->cap + 1) * S->slots);
+    S->cap = o->max_records;
+    S->pool = o->hbm_budget > 0 ? o->hbm_budget / kPoolElemBytes : request_span(S->cap) * S->slots;
+    require(S->pool >= 64, GLOBE_E_INVALID, "hbm_budget holds no request");
+    S->free_ranges.emplace(0, S->pool);
+    const size_t pool = (size_t)S->pool;
+    for (DevBuf* d : {&S->B, &S->X, &S->R, &S->Z, &S->P, &S->VP}) d->ensure(sizeof(double) * pool);
+    S->indiv.ensure(sizeof(int32_t) * pool);
+    S->times.ensure(sizeof(double) * pool);
+    S->rec_blk.ensure(sizeof(int32_t) * pool);
+    S->blk_off.ensure(sizeof(int64_t) * pool);
+    S->blk_ind.ensure(sizeof(int32_t) * pool);
+    S->tile_blk.ensure(sizeof(int32_t) * pool);
+    S->col_off.ensure(sizeof(int64_t) * kMaxCols);
     S->ntiles_col.ensure(sizeof(int32_t) * kMaxCols);
     S->nrec_col.ensure(sizeof(int64_t) * kMaxCols);
     S->theta.ensure(sizeof(double) * 5 * kMaxCols);
     S->state.ensure(sizeof(PcgState));
     for (DevBuf* d : {&S->part_a, &S->part_b, &S->part_c}) d->ensure(sizeof(double) * kMaxParts * kMaxCols);
     hipStream_t s = c->stream;
-    for (DevBuf* d : {&S->B, &S->X, &S->R, &S->Z, &S->P, &S->VP})
-        launch_fill(s, d->as<double>(), (int64_t)S->cap * S->slots, 0.0);
-    HIP_CHECK(hipMemsetAsync(S->ntiles_col.p, 0, sizeof(int32_t) * kMaxCols, s));
-    HIP_CHECK(hipMemsetAsync(S->nrec_col.p, 0, sizeof(int64_t) * kMaxCols, s));
-    // free slots carry harmless components until a request takes them
-    std::vector<double> th((size_t)5 * kMaxCols);
-    for (int j = 0; j < kMaxCols; ++j) {
-        th[5 * j + 0] = 1.0;
-        th[5 * j + 1] = 1.0;
-    }
-    HIP_CHECK(hipMemcpyAsync(S->theta.p, th.data(), th.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreateWithFlags(&S->staged_ev, hipEventDisableTiming));
+    // free slots: no records, harmless components
+    const double th_free[5] = {1.0, 1.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < kMaxCols; ++j)
+        launch_set_column(s, j, 0, 0, 0, th_free, S->col_off.as<int64_t>(), S->ntiles_col.as<int32_t>(),
+                          S->nrec_col.as<int64_t>(), S->theta.as<double>());
     HIP_CHECK(hipHostMalloc((void**)&S->h_state, sizeof(PcgState), hipHostMallocDefault));
     std::memset(S->h_state, 0, sizeof(PcgState));
     S->h_state->max_iter = INT_MAX;
@@ -283,22 +355,15 @@ int globe_serve_open(globe_ctx* c, const globe_serve_opts* o) {
     S->h_state->tol = o->tol;
     S->h_state->done = 1;
     HIP_CHECK(hipMemcpyAsync(S->state.p, S->h_state, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
     S->slot_req.resize(S->slots);
-    S->busy.assign(S->slots, 0);
+    S->slot_state.assign(S->slots, ServeState::kFree);
+    S->slot_off.assign(S->slots, 0);
     S->admit_iter.assign(S->slots, 0);
     S->t_admit.assign(S->slots, 0.0);
     ensure_gdiag(c);
     // tune the contraction for every width the block can take before requests arrive
-    c->w_S0.ensure(sizeof(double) * (size_t)c->n * S->slots);
-    c->w_GZ.ensure(sizeof(double) * (size_t)c->n * S->slots);
-    launch_fill(s, c->w_S0.as<double>(), (int64_t)c->n * S->slots, 0.0);
-    for (int k : {8, 16, 32, 64}) {
-        const int kk = std::min(k, S->slots);
-        genetic_product(c, c->w_S0.as<double>(), kk, c->w_GZ.as<double>(), o->precision, nullptr,
-                        S->theta.as<double>());
-        if (kk == S->slots) break;
-    }
-    HIP_CHECK(hipStreamSynchronize(s));
+    pretune_geno_product(c, S->slots, o->precision, S->theta.as<double>());
     S->t0 = std::chrono::steady_clock::now();
     API_END(c)
 }
@@ -308,7 +373,7 @@ int globe_serve_submit(globe_ctx* c, const globe_request* q, int64_t* id) {
     ServeState& S = server(c);
     ServeState::Request r;
     prepare(c, S, q, r);
-    require(r.bytes <= S.budget, GLOBE_E_INVALID, "request larger than the HBM budget");
+    require(r.span <= S.pool, GLOBE_E_INVALID, "request larger than the HBM budget");
     r.id = S.next_id++;
     r.t_submit = S.now();
     if (id) *id = r.id;
@@ -320,22 +385,23 @@ int globe_serve_step(globe_ctx* c, int32_t* finished, int32_t* active, int32_t* 
     API_BEGIN(c)
     ServeState& S = server(c);
     hipStream_t s = c->stream;
-    // admit FIFO while a slot is free and the budget holds
-    while (!S.queue.empty()) {
-        int j = 0;
-        while (j < S.slots && S.busy[j]) ++j;
-        if (j == S.slots || S.bytes_in_use + S.queue.front().bytes > S.budget) break;
-        ServeState::Request r = std::move(S.queue.front());
-        S.queue.pop_front();
-        admit(c, S, j, std::move(r));
-    }
-    int k = 0;
-    for (int j = 0; j < S.slots; ++j)
-        if (S.busy[j]) k = j + 1;
+    auto busy_cols = [&]() {
+        int k = 0;
+        for (int j = 0; j < S.slots; ++j)
+            if (S.slot_state[j] == ServeState::kBusy) k = j + 1;
+        return k;
+    };
+    // an idle block takes the queue's head at once; a running one joins the requests
+    // staged during the previous iteration
+    if (busy_cols() == 0 && !S.staged_pending)
+        while (stage_next(S)) {
+        }
+    admit_staged(c, S);
+    const int k = busy_cols();
     int fin = 0;
     if (k > 0) {
         // mixed precision: each slot's digits from its own residual ratio (4 in its first
-        // iteration), one digit column of zeros for an empty slot
+        // iteration), one digit column of zeros for a slot that holds no request
         struct LimbsGuard {
             globe_ctx* c;
             ~LimbsGuard() {
@@ -347,7 +413,7 @@ int globe_serve_step(globe_ctx* c, int32_t* finished, int32_t* active, int32_t* 
             c->limbs_col.assign((size_t)k, (int8_t)1);
             int lmax = 2;
             for (int j = 0; j < k; ++j) {
-                if (!S.busy[j]) continue;
+                if (S.slot_state[j] != ServeState::kBusy) continue;
                 const int L = S.global_iter == S.admit_iter[j] ? kLimbsMixed
                                                                : relaxed_limbs(S.h_state->ratio[j], S.o.tol);
                 c->limbs_col[j] = (int8_t)L;
@@ -357,17 +423,24 @@ int globe_serve_step(globe_ctx* c, int32_t* finished, int32_t* active, int32_t* 
         }
         iterate(c, S, k);
         HIP_CHECK(hipMemcpyAsync(S.h_state, S.state.p, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    }
+    // the next requests upload while the iteration runs
+    while (stage_next(S)) {
+    }
+    if (k > 0) {
         HIP_CHECK(hipStreamSynchronize(s));
         const double t = S.now();
+        std::vector<int> retired;
+        const double th_free[5] = {1.0, 1.0, 0.0, 0.0, 0.0};
         for (int j = 0; j < k; ++j) {
-            if (!S.busy[j]) continue;
+            if (S.slot_state[j] != ServeState::kBusy) continue;
             const int its = S.global_iter - S.admit_iter[j];
             const bool conv = S.h_state->active[j] == 0;
             if (!conv && its < S.o.max_iter) continue;
             ServeState::Request& r = S.slot_req[j];
             ServeState::Done d;
             d.x.resize(r.nrec);
-            HIP_CHECK(hipMemcpyAsync(d.x.data(), S.X.as<double>() + j * S.cap, sizeof(double) * r.nrec,
+            HIP_CHECK(hipMemcpyAsync(d.x.data(), S.X.as<double>() + S.slot_off[j], sizeof(double) * r.nrec,
                                      hipMemcpyDeviceToHost, s));
             d.res.id = r.id;
             d.res.iters = its;
@@ -376,31 +449,32 @@ int globe_serve_step(globe_ctx* c, int32_t* finished, int32_t* active, int32_t* 
             d.res.t_submit = r.t_submit;
             d.res.t_admit = S.t_admit[j];
             d.res.t_done = t;
-            // retire the column: it stops contributing and takes no further work
-            int32_t zero32 = 0;
-            int64_t zero64 = 0;
-            HIP_CHECK(hipMemcpyAsync(S.ntiles_col.as<int32_t>() + j, &zero32, sizeof(int32_t),
-                                     hipMemcpyHostToDevice, s));
-            HIP_CHECK(hipMemcpyAsync(S.nrec_col.as<int64_t>() + j, &zero64, sizeof(int64_t),
-                                     hipMemcpyHostToDevice, s));
-            launch_fill(s, S.P.as<double>() + j * S.cap, S.cap, 0.0);
+            // retire the column: no records, no further work
+            launch_set_column(s, j, 0, 0, 0, th_free, S.col_off.as<int64_t>(), S.ntiles_col.as<int32_t>(),
+                              S.nrec_col.as<int64_t>(), S.theta.as<double>());
             if (!conv) {
                 int32_t off = 0;
                 HIP_CHECK(hipMemcpyAsync(&S.state.as<PcgState>()->active[j], &off, sizeof(int32_t),
                                          hipMemcpyHostToDevice, s));
+                HIP_CHECK(hipStreamSynchronize(s));
             }
-            HIP_CHECK(hipStreamSynchronize(s));
-            S.bytes_in_use -= r.bytes;
-            S.busy[j] = 0;
             S.done.emplace(r.id, std::move(d));
-            r = ServeState::Request();
+            retired.push_back(j);
             ++fin;
+        }
+        if (!retired.empty()) {
+            HIP_CHECK(hipStreamSynchronize(s));
+            for (int j : retired) {
+                S.release(S.slot_off[j], S.slot_req[j].span);
+                S.slot_state[j] = ServeState::kFree;
+                S.slot_req[j] = ServeState::Request();
+            }
         }
     }
     if (finished) *finished = fin;
     if (active) {
         int a = 0;
-        for (int j = 0; j < S.slots; ++j) a += S.busy[j] ? 1 : 0;
+        for (int j = 0; j < S.slots; ++j) a += S.slot_state[j] == ServeState::kBusy ? 1 : 0;
         *active = a;
     }
     if (queued) *queued = (int32_t)S.queue.size();
@@ -428,6 +502,19 @@ int globe_serve_take(globe_ctx* c, int64_t id, double* x, globe_serve_result* re
     if (x) std::memcpy(x, it->second.x.data(), sizeof(double) * it->second.x.size());
     if (res) *res = it->second.res;
     S.done.erase(it);
+    API_END(c)
+}
+
+int globe_serve_usage(globe_ctx* c, int64_t out[4]) {
+    API_BEGIN(c)
+    ServeState& S = server(c);
+    require(out != nullptr, GLOBE_E_INVALID, "null output");
+    int64_t f = 0;
+    for (const auto& kv : S.free_ranges) f += kv.second;
+    out[0] = (S.pool - f) * kPoolElemBytes;
+    out[1] = S.peak_in_use * kPoolElemBytes;
+    out[2] = S.pool * kPoolElemBytes;
+    out[3] = S.waits_for_room;
     API_END(c)
 }
 

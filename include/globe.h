@@ -322,7 +322,10 @@ typedef struct {
 typedef struct {
     int32_t slots;        /* columns of the shared block PCG (1..64) */
     int64_t max_records;  /* record capacity of a slot */
-    int64_t hbm_budget;   /* bytes of request vectors resident at once (0: slots x capacity) */
+    int64_t hbm_budget;   /* bytes of the record pool that holds every admitted request's
+                             record structure and PCG vectors: 80 bytes per element, a
+                             request takes nrec + 1 elements rounded up to 64 (0: room for
+                             `slots` requests of max_records) */
     double tol;           /* per-request |r| < tol |b| */
     int32_t max_iter;     /* per request */
     int32_t precision;    /* GLOBE_PREC_* of the genotype contraction */
@@ -338,6 +341,9 @@ typedef struct {
     double t_done;
 } globe_serve_result;
 
+/* Requests are admitted first in, first out, each into a free slot and a first-fit range
+ * of the record pool; a request's data is uploaded while the block iterates and it joins
+ * at the next step. */
 int globe_serve_open(globe_ctx* ctx, const globe_serve_opts* opts);
 /* Copies and queues a request; *id identifies it in globe_serve_take. */
 int globe_serve_submit(globe_ctx* ctx, const globe_request* req, int64_t* id);
@@ -350,6 +356,9 @@ int globe_serve_finished(globe_ctx* ctx, int64_t* ids, int32_t max, int32_t* cou
 /* Hands over a finished request: x (its nrec values) and its result; GLOBE_E_STATE if it
  * has not finished. Each finished request can be taken once. */
 int globe_serve_take(globe_ctx* ctx, int64_t id, double* x, globe_serve_result* res);
+/* Record-pool use: out[0] bytes in use, out[1] the most in use at once, out[2] the pool's
+ * bytes, out[3] steps at which the queue's head waited for pool room with a slot free. */
+int globe_serve_usage(globe_ctx* ctx, int64_t out[4]);
 int globe_serve_close(globe_ctx* ctx);
 
 /* ---- mixed-precision digits (no reference counterpart) ----------------------------- */
@@ -380,7 +389,7 @@ typedef struct {
     int64_t launches;
     double ms_total;
     double bytes_per_launch;
-    double flops_per_launch;
+    double flops_per_launch;      /* averaged over the launches */
     double issued_ops_per_launch; /* int8 MFMA operations actually issued (digits, padding),
                                      averaged over the launches */
 } globe_kernel_stats;
@@ -390,6 +399,9 @@ int globe_profile_read(globe_ctx* ctx, int which, globe_kernel_stats* out);
 /* ---- device memory and synthetic cohorts (no reference counterpart) ---------------- */
 /* Device buffers for the *_dev entry points and for measurement with inputs resident in HBM. */
 int globe_dev_alloc(globe_ctx* ctx, int64_t bytes, void** ptr);
+/* Free and total device memory of the context's device, bytes (e.g. to size a serving
+ * loop's HBM budget). */
+int globe_device_memory(globe_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
 int globe_dev_free(globe_ctx* ctx, void* ptr);
 int globe_memcpy_h2d(globe_ctx* ctx, void* dst, const void* src, int64_t bytes);
 int globe_memcpy_d2h(globe_ctx* ctx, void* dst, const void* src, int64_t bytes);

@@ -389,6 +389,44 @@ void orc_vx_ref32(const orc_model* md, const float* x, int k, float* y, int grm_
     free(st); free(Z); free(GZ);
 }
 
+/* FP32 OpenMP restatement of computeVx_multi_VC (varCompEst.cpp:322-352) with
+ * computeV_ij (:309-319), kept as the CPU timing baseline of runAIREML's operator: for
+ * every record pair j <= i, V_ij = sig2g A(indiv_i, indiv_j) from the packed lower
+ * triangle (getElement_vecLT, :304-306) + the time term when the records share an
+ * individual + sig2e on the diagonal, accumulated into both rows in thread-local
+ * result matrices (schedule dynamic over i) that are added at the end. Needs the GRM. */
+void orc_vx_vc_ref32(const orc_model* md, const float* x, int k, float* y) {
+    const int64_t nrec = md->nrec;
+    for (int64_t q = 0; q < nrec * k; ++q) y[q] = 0.0f;
+    if (!md->use_grm) return;
+    const float sg = (float)md->theta[1], sb0 = (float)md->theta[2], sb01 = (float)md->theta[3],
+                sb1 = (float)md->theta[4], se = (float)md->theta[0];
+#pragma omp parallel
+    {
+        float* loc = (float*)calloc((size_t)nrec * k, sizeof(float));
+#pragma omp for schedule(dynamic)
+        for (long i = 0; i < (long)nrec; ++i) {
+            const int ii = md->indiv[i];
+            const float ti = (float)md->times[i];
+            for (long j = 0; j <= i; ++j) {
+                const int ij = md->indiv[j];
+                const float tj = (float)md->times[j];
+                const size_t a = ii < ij ? (size_t)ij * (ij + 1) / 2 + ii : (size_t)ii * (ii + 1) / 2 + ij;
+                float v = sg * md->grm_lt[a];
+                v += ii == ij ? sb0 + sb01 * (ti + tj) + sb1 * ti * tj : 0.0f;
+                v += i == j ? se : 0.0f;
+                for (int c = 0; c < k; ++c) {
+                    loc[(size_t)c * nrec + i] += v * x[(size_t)c * nrec + j];
+                    if (i != j) loc[(size_t)c * nrec + j] += v * x[(size_t)c * nrec + i];
+                }
+            }
+        }
+#pragma omp critical
+        for (int64_t q = 0; q < nrec * k; ++q) y[q] += loc[q];
+        free(loc);
+    }
+}
+
 /* Gauss-Jordan inverse with partial pivoting (stands in for Eigen's
  * partialPivLu-based inverse(), thirdParty/eigen/Eigen/src/LU/InverseImpl.h:23-28).
  * Returns 0 on success, 1 if a pivot is exactly zero. */

@@ -55,6 +55,7 @@ void orc_vx(const orc_model* md, const double* x, int k, double* y, int grm_only
 void orc_vx_dense(const orc_model* md, const double* x, int k, double* y);
 void orc_geno_Wa(const orc_model* md, const double* a, double* out);
 void orc_vx_ref32(const orc_model* md, const float* x, int k, float* y, int grm_only);
+void orc_vx_vc_ref32(const orc_model* md, const float* x, int k, float* y);
 
 /* --- preconditioner ----------------------------------------------------------- */
 int64_t orc_blocks(const orc_model* md, int64_t* starts /* nrec+1 */);

@@ -64,6 +64,7 @@ def _declare(L):
     L.orc_vx_dense.argtypes = [P, P, C.c_int, P]
     L.orc_geno_Wa.argtypes = [P, P, P]
     L.orc_vx_ref32.argtypes = [P, P, C.c_int, P, C.c_int]
+    L.orc_vx_vc_ref32.argtypes = [P, P, C.c_int, P]
     L.orc_blocks.argtypes = [P, P]
     L.orc_blocks.restype = C.c_int64
     L.orc_precond_size.argtypes = [P]
@@ -152,6 +153,15 @@ class Oracle:
         k = x.shape[1]
         y = np.zeros((self.n if grm_only else self.nrec, k), order="F", dtype=np.float32)
         lib().orc_vx_ref32(self.ref, _ptr(x), k, _ptr(y), int(grm_only))
+        return y
+
+    def vx_vc_ref32(self, x):
+        """FP32 record-pair restatement of computeVx_multi_VC (varCompEst.cpp:322-352):
+        the CPU timing baseline of runAIREML's operator (GRM models only)."""
+        x = np.asfortranarray(np.asarray(x, dtype=np.float32).reshape(self.nrec, -1))
+        k = x.shape[1]
+        y = np.zeros((self.nrec, k), dtype=np.float32, order="F")
+        lib().orc_vx_vc_ref32(self.ref, _ptr(x), k, _ptr(y))
         return y
 
     def geno_Wa(self, a):

@@ -89,6 +89,13 @@ def test_vx_matches_explicit_dense_V(name):
     assert abs((y.T @ orc.vx(x[:, :1]))[0, 0] - (x[:, :1].T @ orc.vx(y))[0, 0]) < 1e-10
     # FP32 restatement kept for CPU timing agrees to FP32 rounding
     assert rel(orc.vx_ref32(x), V @ x) < 1e-5
+    if "grm_lt" in d:
+        # and so does the record-pair form of runAIREML's operator (computeVx_multi_VC),
+        # every pair V_ij built on the fly; dropping the time term or the residual fails
+        y32 = orc.vx_vc_ref32(x)
+        assert rel(y32, V @ x) < 1e-5
+        se = d["theta"][0]
+        assert rel(y32, (V - se * np.eye(c.nrec)) @ x) > 1e-3
 
 
 def test_preconditioner_is_dense_block_inverse():

@@ -123,10 +123,11 @@ def test_serving_rejects_bad_requests(cohort, gpu_ctx_factory):
 
 
 def test_serving_admits_by_hbm_budget(cohort, gpu_ctx_factory):
-    """A budget that holds one request's vectors (6 record vectors of 8 bytes + the record
-    structure: 80 bytes per record) admits one at a time, FIFO, however many slots are
-    free; a request larger than the budget is refused at submission; each result equals
-    the same request served with an unlimited budget."""
+    """A budget whose record pool holds one request (80 bytes per pool element: 6 record
+    vectors of 8 bytes + the record structure; a request takes nrec + 1 elements rounded
+    up to 64) admits one at a time, FIFO, however many slots are free; a request larger
+    than the budget is refused at submission; each result equals the same request served
+    with an unlimited budget."""
     _say("serve-budget")
     d = cohort
     n = d.cohort.n
@@ -139,7 +140,12 @@ def test_serving_admits_by_hbm_budget(cohort, gpu_ctx_factory):
     g = _device(d, gpu_ctx_factory)
     free, _ = _serve(g, reqs, slots=4, precision=G.PREC_FP64, tol=1e-10, stagger=False, cap=cap)
     per_rec = 80
-    g.serve_open(slots=4, max_records=cap, hbm_budget=per_rec * cap, tol=1e-10, max_iter=500,
+
+    def span(nrec):
+        return (nrec + 1 + 63) // 64 * 64
+
+    assert span(sizes[0]) + span(sizes[1]) > span(cap)
+    g.serve_open(slots=4, max_records=cap, hbm_budget=per_rec * span(cap), tol=1e-10, max_iter=500,
                  precision=G.PREC_FP64)
     ids = [g.serve_submit(*r) for r in reqs]
     order, out = [], {}
@@ -156,7 +162,49 @@ def test_serving_admits_by_hbm_budget(cohort, gpu_ctx_factory):
     for q, rid in enumerate(ids):
         assert rel(out[rid], free[q]) <= 1e-14, q
     g.serve_close()
-    g.serve_open(slots=4, max_records=cap, hbm_budget=per_rec * 10, tol=1e-10)
+    g.serve_open(slots=4, max_records=cap, hbm_budget=per_rec * 64, tol=1e-10)
     with pytest.raises(G.GlobeError):  # larger than the whole budget
         g.serve_submit(*reqs[0])
     g.serve_close()
+    with pytest.raises(G.GlobeError):  # a budget that holds no request at all
+        g.serve_open(slots=4, max_records=cap, hbm_budget=per_rec * 10, tol=1e-10)
+
+
+def test_serving_pool_admits_by_request_size(cohort, gpu_ctx_factory):
+    """Requests take pool ranges of their own size, not a slot of the largest one: with a
+    pool that holds one large request, several small requests run together while the
+    large one waits its turn (FIFO), and every result equals the same request served
+    alone."""
+    _say("serve-pool")
+    d = cohort
+    n = d.cohort.n
+    rng = np.random.default_rng(21)
+    big = serving.make_request(rng, n, n, n, 3.0, 3.0)
+    smalls = [serving.make_request(rng, n, 30, 40, 1.0, 2.0) for _ in range(4)]
+    reqs = smalls[:2] + [big] + smalls[2:]
+    cap = big[0].shape[0]
+
+    def span(nrec):
+        return (nrec + 1 + 63) // 64 * 64
+
+    assert 4 * max(span(r[0].shape[0]) for r in smalls) <= span(cap)
+    g = _device(d, gpu_ctx_factory)
+    alone = [_serve(g, [r], slots=4, precision=G.PREC_FP64, tol=1e-10, stagger=False, cap=cap)[0][0]
+             for r in reqs]
+    g.serve_open(slots=4, max_records=cap, hbm_budget=80 * span(cap), tol=1e-10, max_iter=500,
+                 precision=G.PREC_FP64)
+    ids = [g.serve_submit(*r) for r in reqs]
+    out, most_active, order = {}, 0, []
+    for _ in range(5000):
+        _, active, _ = g.serve_step()
+        most_active = max(most_active, active)
+        for rid in g.serve_finished():
+            out[rid] = g.serve_take(rid)[0]
+            order.append(rid)
+        if len(out) == len(reqs):
+            break
+    g.serve_close()
+    assert most_active >= 2  # the two small requests ran side by side
+    assert order.index(ids[2]) > order.index(ids[1])  # the large one waited for room
+    for q, rid in enumerate(ids):
+        assert rel(out[rid], alone[q]) <= 1e-14, q
