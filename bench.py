@@ -40,6 +40,17 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+MARK = False
+
+
+def mark(g):
+    """With --mark: a 16-byte device copy (k_copy16) that delimits a leg in a kernel trace."""
+    if MARK:
+        buf = g.alloc(256)
+        g._check(g.L.globe_dev_copy(g.h, buf.ptr, buf.ptr + 128, 16))
+        buf.free()
+
+
 def build_cohort(g, n, m, lam, seed):
     from globe_amd import synth
     rng = np.random.default_rng(seed)
@@ -240,7 +251,9 @@ def serving_leg(g, c, kept, args):
     def run(batch, rate):
         g.serve_open(slots=args.serve_slots, max_records=cap, hbm_budget=budget, tol=args.tol, max_iter=1000,
                      precision=G.PREC_MIXED)
+        mark(g)
         _, st = serving.replay(g, batch, rate, seed=args.seed)
+        mark(g)
         use = g.serve_usage()
         g.serve_close()
         lat = st["latency_s"]
@@ -332,8 +345,10 @@ def aireml_leg(args):
             t_dig = time.perf_counter() - t0
             log(f"aireml: int8 digit tiles of G built in {t_dig:.2f}s")
         g.profile(True)
+        mark(g)
         fit = g.aireml_fit(y, th0, max_iter=args.aireml_iters, tol=1e-3, num_mc=15, probes="rademacher",
                            seed=args.seed, pcg_tol=1e-5, precision=prec)
+        mark(g)
         ks = g.kernel_stats(3 if prec == G.PREC_MIXED else 2)
         g.profile(False)
         if name == "fp64":
@@ -541,6 +556,8 @@ def main():
     ap.add_argument("--cpu-ref-snps", type=int, nargs="+", default=[1_000, 4_000, 50_000],
                     help="SNP samples on which the reference runPCGsolver is timed")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mark", action="store_true",
+                    help="delimit the serving replays and AI-REML fits with 16-byte device copies")
     ap.add_argument("--serve-requests", type=int, default=2000,
                     help="requests replayed through the serving loop (0: skip)")
     ap.add_argument("--serve-hbm-frac", type=float, default=0.05,
@@ -561,6 +578,8 @@ def main():
     ap.add_argument("--calib", action="store_true",
                     help="first run 3 plain 1 GiB device copies (counter calibration)")
     args = ap.parse_args()
+    global MARK
+    MARK = args.mark
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; SNP-sharded runs over several "
                                    "GPUs are not implemented", "n_gpus": args.gpus}))

@@ -230,7 +230,7 @@ void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int prec
 }
 
 void op_vx(globe_ctx* c, const double* x, int64_t ldx, int k, double* y, int64_t ldy, int mode,
-           int precision, const int* skip, double* part_dot, PcgState* alpha_st) {
+           int precision, const int* skip, double* part_dot) {
     require(c->has_cohort, GLOBE_E_STATE, "cohort not set (globe_set_cohort)");
     require(c->gen != 0, GLOBE_E_STATE, "no genetics loaded (globe_load_bed / globe_load_grm)");
     require(k >= 1 && k <= kMaxCols, GLOBE_E_INVALID, "k must be in [1, 64]");
@@ -248,15 +248,15 @@ void op_vx(globe_ctx* c, const double* x, int64_t ldx, int k, double* y, int64_t
     }
     genetic_product(c, S0, k, GZ, precision, skip, c->d_theta.as<double>());
     launch_record_out(s, c->meta, x, ldx, k, GZ, S0, S1, n, c->d_theta.as<double>(), y, ldy, part_dot,
-                      skip, alpha_st);
+                      skip);
 }
 
 static void precond(globe_ctx* c, const double* r, int64_t ldr, int k, double* z, int64_t ldz,
-                    double* part_rz, double* part_rr, const PcgState* st, bool fuse_beta = false) {
+                    double* part_rz, double* part_rr, const PcgState* st) {
     ensure_gdiag(c);
     ensure_theta(c, k);
     launch_precond(c->stream, c->meta, c->d_gdiag.as<double>(), c->d_theta.as<double>(), r, ldr, k, z,
-                   ldz, part_rz, part_rr, st, 0, fuse_beta);
+                   ldz, part_rz, part_rr, st, 0);
 }
 
 // nmc probes (nrec x nmc) and their trace weights at the context's variance components:
@@ -390,12 +390,14 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
     while (true) {
         for (int b = 0; b < batch; ++b) {
             c->prof_iter = queued + b;
-            // three grid reductions per iteration, each fused into the kernel that produces
-            // its partials: p.Vp -> alpha (output kernel of V p), r.r -> convergence test
-            // (x/r update), r.z -> beta (preconditioner)
-            op_vx(c, p, nrec, k, vp, nrec, GLOBE_VX_FULL, o.precision, skip, part_a, dst);
+            // the dot partials come fused out of the kernels that produce their operands
+            // (V p output, x/r update, preconditioner); one block reduces each of them
+            op_vx(c, p, nrec, k, vp, nrec, GLOBE_VX_FULL, o.precision, skip, part_a);
+            launch_pcg_alpha(s, dst, part_a, np, k);
             launch_pcg_xr(s, dst, nrec, X, ldx, r, p, vp, nrec, k, part_b);
-            precond(c, r, nrec, k, z, nrec, part_c, nullptr, dst, true);
+            launch_pcg_check(s, dst, part_b, np, k);
+            precond(c, r, nrec, k, z, nrec, part_c, nullptr, dst);
+            launch_pcg_beta(s, dst, part_c, npb, k);
             launch_pcg_p(s, dst, nrec, p, z, nrec, k);
         }
         queued += batch;

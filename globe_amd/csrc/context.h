@@ -156,10 +156,8 @@ void* scratch_address(int device, size_t* bytes);
 
 // device operator: y = V x (mode FULL) or sig2g G J^T x (GRM_ONLY); optional fused
 // partials of sum x.y per column; skip = device done flag (nullptr: always run).
-// alpha_st (PCG only): the output kernel's last workgroup also forms alpha from the fused
-// p.Vp partials into that state
 void op_vx(globe_ctx* c, const double* x, int64_t ldx, int k, double* y, int64_t ldy, int mode,
-           int precision, const int* skip, double* part_dot, PcgState* alpha_st = nullptr);
+           int precision, const int* skip, double* part_dot);
 void ensure_theta(globe_ctx* c, int k);
 // GZ (n x k) = sig2g_c G S0 per column with the per-column components theta_cols [k][5]
 void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int precision,

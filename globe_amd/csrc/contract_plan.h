@@ -16,12 +16,17 @@ namespace globe {
 namespace plan {
 
 // ---------------------------------------------------------------- block shapes
-// wide kernel (k_geno_wide): waves, ring stages, column groups, row fragments per wave
+// wide kernel (k_geno_wide): waves, ring stages, column groups, row fragments per wave,
+// and PF: 1 reads the digits from LDS two K-steps ahead of their MFMAs, pinned in that
+// order (0: one K-step ahead, as the compiler schedules). PF changes only the order of
+// the LDS reads.
 struct WideShape {
-    int W, NS, CG, RA;
+    int W, NS, CG, RA, PF;
 };
-constexpr int kNumWide = 4;
-constexpr WideShape kWide[kNumWide] = {{8, 4, 1, 2}, {4, 3, 1, 2}, {8, 4, 2, 4}, {4, 3, 2, 4}};
+constexpr int kNumWide = 10;
+constexpr WideShape kWide[kNumWide] = {{8, 4, 1, 2, 0}, {4, 3, 1, 2, 0}, {8, 4, 2, 4, 0}, {4, 3, 2, 4, 0},
+                                       {8, 4, 1, 2, 1}, {4, 3, 1, 2, 1}, {8, 4, 2, 4, 1}, {4, 3, 2, 4, 1},
+                                       {4, 5, 1, 2, 1}, {8, 6, 1, 2, 1}};
 // thin kernel (k_geno_thin): waves, row fragments of 16 rows per wave, ring stages,
 // super-steps (256 genotypes = 64 packed bytes of a row) per stage
 struct ThinShape {
@@ -75,10 +80,14 @@ struct ThinCfg {
     static_assert(kDigKB < W || kDigKB % W == 0, "digit chunks must spread evenly");
 };
 
-// A wide shape fits JT column fragments when the column groups divide them and the
-// stage's 4 JT digit chunks spread evenly over the waves (or number fewer than them).
+// A wide shape fits JT column fragments when the column groups divide them, the stage's
+// 4 JT digit chunks spread evenly over the waves (or number fewer than them) and its ring
+// of NS stages fits the 160 KB of LDS.
 constexpr bool wide_fits(int JT, int q) {
-    return JT % kWide[q].CG == 0 && (4 * JT < kWide[q].W || (4 * JT) % kWide[q].W == 0);
+    return JT % kWide[q].CG == 0 && (4 * JT < kWide[q].W || (4 * JT) % kWide[q].W == 0) &&
+           kWide[q].NS * ((kWide[q].W / kWide[q].CG) * kWide[q].RA + 4 * JT) * 1024 +
+                   (4 * JT < kWide[q].W ? (kWide[q].W - 4 * JT) * 1024 : 0) <=
+               160 * 1024;
 }
 
 inline int64_t rows_per_block(bool wide, int cfg) {

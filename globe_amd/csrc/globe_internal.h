@@ -62,9 +62,6 @@ struct PcgState {
     // alpha_it and hist[(2 + 2 it) kMaxCols + c] = beta_it of the columns still active
     double* hist;
     int hist_cap, hist_pad;
-    // grid-reduction tickets of the fused kernels (record_out + alpha, x/r update +
-    // convergence test, preconditioner + beta): zero between launches
-    unsigned int ticket[4];
 };
 
 // Digit layout of a k-column right-hand side in the int8 contraction: column c has L[c]
@@ -112,17 +109,14 @@ constexpr int kTileMaxRecords = 2048;  // LDS: 4 doubles per record slot (64 KB)
 int precond_parts(const RecMeta& m);
 void launch_seg_sums(hipStream_t s, const RecMeta& m, const double* x, int64_t ldx, int k,
                      double* S0, double* S1, int64_t ldn, const int* skip);
-// alpha_st: inside PCG, the last workgroup also forms alpha = r.z / p.Vp from the p.Vp
-// partials (part_dot) into that state
 void launch_record_out(hipStream_t s, const RecMeta& m, const double* x, int64_t ldx, int k,
                        const double* GZ, const double* S0, const double* S1, int64_t ldn,
                        const double* theta_cols, double* y, int64_t ldy, double* part_dot,
-                       const int* skip, PcgState* alpha_st = nullptr);
-// st: inside PCG (frozen columns skipped, nothing when done); fuse_beta: the last
-// workgroup also forms beta = r.z_new / r.z_old from the part_rz partials
+                       const int* skip);
+// st: inside PCG (frozen columns skipped, nothing when done)
 void launch_precond(hipStream_t s, const RecMeta& m, const double* gdiag, const double* theta_cols,
                     const double* r, int64_t ldr, int k, double* z, int64_t ldz, double* part_rz,
-                    double* part_rr, const PcgState* st, int tilde_v, bool fuse_beta = false);
+                    double* part_rr, const PcgState* st, int tilde_v);
 void launch_dot_parts(hipStream_t s, int64_t nrec, const double* a, int64_t lda, const double* b,
                       int64_t ldb, int k, double* part);
 void launch_reduce_parts(hipStream_t s, const double* part, int nparts, int k, double* out);
@@ -141,8 +135,11 @@ void launch_pcg_init(hipStream_t s, PcgState* st, const double* part_bb, const d
                      const double* part_rz, int nparts, int k);
 void launch_pcg_admit(hipStream_t s, PcgState* st, int j, const double* part_bb, int nparts_bb,
                       const double* part_rr, const double* part_rz, int nparts);
-// x += alpha p, r -= alpha Vp per active column with the r.r partials; the last
-// workgroup then runs the convergence test (k_pcg_check's rules) on them
+// one-block reductions of the dot partials: alpha, the convergence test, beta
+void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* part_pvp, int nparts, int k);
+void launch_pcg_check(hipStream_t s, PcgState* st, const double* part_rr, int nparts, int k);
+void launch_pcg_beta(hipStream_t s, PcgState* st, const double* part_rz, int nparts, int k);
+// x += alpha p, r -= alpha Vp per active column with the r.r partials
 void launch_pcg_xr(hipStream_t s, PcgState* st, int64_t nrec, double* x, int64_t ldx, double* r,
                    const double* p, const double* vp, int64_t ld, int k, double* part_rr,
                    const int64_t* col_off = nullptr, const int64_t* col_cnt = nullptr);

@@ -152,6 +152,12 @@ __device__ inline void thin_mma(const uint8_t* __restrict__ slot, int wave, int 
 // copies of the NS - 2 later stages in flight), then barrier: after it, every wave's
 // copies of the stage are visible in LDS. Waves that copy no digits issue fewer copies
 // per stage, so their count is smaller.
+// The copies into LDS count in lgkmcnt as well as vmcnt, so the LDS counter waits for the
+// same number of later copies (capped at its 4-bit field; waiting for fewer is safe): no
+// LDS read is outstanding here, since every read of the previous stage fed an MFMA, and
+// lgkmcnt(0) would drain the copies of the later stages too.
+constexpr int lgkm_cap(int n) { return n < 15 ? n : 15; }
+
 template <int W, int RT, int NS, int SS>
 __device__ inline void thin_stage_barrier(int wave) {
     using C = ThinCfg<W, RT, SS>;
@@ -159,9 +165,10 @@ __device__ inline void thin_stage_barrier(int wave) {
     constexpr int kAheadRowOnly = (NS - 2) * C::kOpsRowOnly;
     static_assert(kAhead < 64, "vmcnt field");
     if (C::kDigKB >= W || wave < C::kDigKB)
-        asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(kAhead) : "memory");
+        asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(%1)\n\ts_barrier" ::"n"(kAhead), "n"(lgkm_cap(kAhead)) : "memory");
     else
-        asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(kAheadRowOnly) : "memory");
+        asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(%1)\n\ts_barrier" ::"n"(kAheadRowOnly),
+                     "n"(lgkm_cap(kAheadRowOnly)) : "memory");
 }
 
 // Stream-K over the units (row block, stage) of plan::sk_block_of: every block streams
@@ -271,8 +278,9 @@ __device__ inline void wide_issue(const uint8_t* __restrict__ arow, int64_t frag
 }
 
 // 4 K-steps of one stage from LDS: the wave's RA row fragments, then its digit
-// fragments one K-step ahead of the MFMAs that use them.
-template <class C>
+// fragments -- one K-step ahead of the MFMAs that use them (PF = 0), or two K-steps
+// ahead with the reads pinned before the MFMAs they overlap (PF = 1).
+template <class C, int PF>
 __device__ inline void wide_mma(const uint8_t* __restrict__ slot, int rgroup, int cgroup, int lane,
                                 v16i (&acc)[4][4]) {
     constexpr int RA = C::kRA, CB = C::kCB, JT = C::kJT;
@@ -281,6 +289,32 @@ __device__ inline void wide_mma(const uint8_t* __restrict__ slot, int rgroup, in
     for (int a = 0; a < RA; ++a)
         av[a] = *reinterpret_cast<const uint4*>(slot + (rgroup * RA + a) * 1024 + lane * 16);
     const uint8_t* cl = slot + C::kA + (cgroup * CB) * 1024 + lane * 16;
+    if constexpr (PF == 1) {
+        // digits two K-steps ahead: at most 2 + 3 CB reads in flight (the LDS counter
+        // tracks 15), each issued before the MFMAs it runs behind
+        v4i bf[4][CB];
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int b = 0; b < CB; ++b) bf[s][b] = *reinterpret_cast<const v4i*>(cl + (s * JT + b) * 1024);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (s + 2 < 4)
+#pragma unroll
+                for (int b = 0; b < CB; ++b)
+                    bf[s + 2][b] = *reinterpret_cast<const v4i*>(cl + ((s + 2) * JT + b) * 1024);
+            // keep the reads ahead of this K-step's MFMAs (the scheduler sinks them to their uses)
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int a = 0; a < RA; ++a) {
+                const v4i ad = decode16(dword_of(av[a], s));
+#pragma unroll
+                for (int b = 0; b < CB; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ad, bf[s][b], acc[a][b], 0, 0, 0);
+            }
+        }
+        return;
+    }
     v4i bcur[CB];
 #pragma unroll
     for (int b = 0; b < CB; ++b) bcur[b] = *reinterpret_cast<const v4i*>(cl + b * 1024);
@@ -312,12 +346,12 @@ template <class C, int NS>
 __device__ inline void wide_stage_barrier() {
     constexpr int kAhead = (NS - 2) * C::kOps;
     static_assert(kAhead < 64, "vmcnt field");
-    asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(kAhead) : "memory");
+    asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(%1)\n\ts_barrier" ::"n"(kAhead), "n"(lgkm_cap(kAhead)) : "memory");
 }
 
 // All operands pass through LDS (no loaded value lives in VGPRs across a stage), so
 // the only waits on the copies are the explicit per-stage ones above.
-template <int JT, int W, int NS, int CG, int RA>
+template <int JT, int W, int NS, int CG, int RA, int PF>
 __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(2))) void k_geno_wide(
     const uint8_t* __restrict__ rows, int64_t pitch, int64_t nrows, int64_t n_ss,
     int64_t ss_per_split, int64_t R, int S, const uint8_t* __restrict__ Bq,
@@ -365,7 +399,7 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(2))) voi
                 wide_issue<C>(arow + plan::wide_stage_step(tn), fstride, bst + plan::dig_stage(tn, C::kB),
                               lds + ((u + NS - 1) % NS) * C::kSlot, sink, wave, lane);
                 __builtin_amdgcn_sched_barrier(0);
-                wide_mma<C>(lds + u * C::kSlot, rgroup, cgroup, lane, acc);
+                wide_mma<C, PF>(lds + u * C::kSlot, rgroup, cgroup, lane, acc);
             }
         }
         // no copy into LDS may be in flight when the block ends
@@ -661,7 +695,7 @@ void launch_wide_q(hipStream_t s, unsigned grid, const uint8_t* rows, int64_t pi
                    const int* skip) {
     if constexpr (wide_fits(JT, Q)) {
         constexpr WideShape w = kWide[Q];
-        k_geno_wide<JT, w.W, w.NS, w.CG, w.RA><<<grid, w.W * 64, 0, s>>>(rows, pitch, nrows, nss, per, R,
+        k_geno_wide<JT, w.W, w.NS, w.CG, w.RA, w.PF><<<grid, w.W * 64, 0, s>>>(rows, pitch, nrows, nss, per, R,
                                                                         S, Bq, out, skip);
     } else {
         require(false, GLOBE_E_INVALID, "wide contraction shape not available for this width");
@@ -676,7 +710,13 @@ void launch_wide(hipStream_t s, const Variant& v, unsigned grid, const uint8_t* 
         case 0: launch_wide_q<JT, 0>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
         case 1: launch_wide_q<JT, 1>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
         case 2: launch_wide_q<JT, 2>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
-        default: launch_wide_q<JT, 3>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
+        case 3: launch_wide_q<JT, 3>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
+        case 4: launch_wide_q<JT, 4>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
+        case 5: launch_wide_q<JT, 5>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
+        case 6: launch_wide_q<JT, 6>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
+        case 7: launch_wide_q<JT, 7>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
+        case 8: launch_wide_q<JT, 8>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
+        default: launch_wide_q<JT, 9>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
     }
 }
 

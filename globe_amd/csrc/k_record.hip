@@ -34,26 +34,6 @@ __device__ inline double block_sum(double v) {
     return r;
 }
 
-// Grid-wide reduction ticket: every workgroup calls this once after storing its
-// partials (thread 0 stores them); it returns true, in all threads, in the one
-// workgroup that arrives last. The release half of the agent-scope atomic publishes each
-// workgroup's partials past its XCD's L2, and the acquire fence of the last workgroup
-// makes all of them visible to it, so it can add them in a fixed order (deterministic).
-// The last workgroup resets the ticket for the next launch.
-__device__ inline bool arrive_last(unsigned int* ticket) {
-    __shared__ int last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned int total = gridDim.x * gridDim.y;
-        const unsigned int prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = prev + 1 == total;
-        if (last) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return last != 0;
-}
-
 // Start of column c of a record-space array with column stride ld: its own pool offset
 // when the columns are pooled, else c ld.
 __device__ inline int64_t col_start(const RecMeta& m, int c, int64_t ld) { return m.col_off ? m.col_off[c] : c * ld; }
@@ -198,13 +178,12 @@ __device__ void pcg_beta_body(PcgState* st, const double* __restrict__ part_rz, 
 
 // y[c][r] = GZ_i + sig2b0 S0_i + sigb01 (t S0_i + S1_i) + sig2b1 t S1_i + sig2e x_r
 // (the factorized form of PCGsolver.cpp:1220-1265). Optional fused partial of
-// sum_r x[c][r] y[c][r] (the p . Vp of PCG, PCGsolver.cpp:1381), and with a PCG state
-// the grid reduction of those partials into alpha.
+// sum_r x[c][r] y[c][r] (the p . Vp of PCG, PCGsolver.cpp:1381).
 __global__ __launch_bounds__(kBlock) void k_record_out(
     RecMeta m, const double* __restrict__ x, int64_t ldx, const double* __restrict__ GZ,
     const double* __restrict__ S0, const double* __restrict__ S1, int64_t ldn,
     const double* __restrict__ th, double* __restrict__ y, int64_t ldy, double* part_dot,
-    const int* skip, PcgState* ast) {
+    const int* skip) {
     if (skip && *skip) return;
     const int c = blockIdx.y;
     const double se = th[c * 5 + 0], sb0 = th[c * 5 + 2], sb01 = th[c * 5 + 3],
@@ -234,8 +213,6 @@ __global__ __launch_bounds__(kBlock) void k_record_out(
     if (part_dot) {
         double s = block_sum(acc);
         if (threadIdx.x == 0) part_dot[c * kMaxParts + blockIdx.x] = s;
-        // inside PCG the last workgroup forms alpha from all p.Vp partials
-        if (ast && arrive_last(&ast->ticket[0])) pcg_alpha_body(ast, part_dot, gridDim.x, gridDim.y);
     }
 }
 
@@ -286,7 +263,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_sums_tiled(RecMeta m, const doub
 __global__ __launch_bounds__(kBlock) void k_precond_tiled(
     RecMeta m, const double* __restrict__ gdiag, const double* __restrict__ th,
     const double* __restrict__ r, int64_t ldr, double* __restrict__ z, int64_t ldz,
-    double* part_rz, double* part_rr, PcgState* stt, int tilde_v, int fuse_beta) {
+    double* part_rz, double* part_rr, const PcgState* stt, int tilde_v) {
     if (stt && stt->done) return;
     const int c = blockIdx.y;
     if (stt && !stt->active[c]) {
@@ -295,7 +272,6 @@ __global__ __launch_bounds__(kBlock) void k_precond_tiled(
             if (part_rz) part_rz[c * kMaxParts + blockIdx.x] = 0.0;
             if (part_rr) part_rr[c * kMaxParts + blockIdx.x] = 0.0;
         }
-        if (fuse_beta && arrive_last(&stt->ticket[2])) pcg_beta_body(stt, part_rz, gridDim.x, gridDim.y);
         return;
     }
     extern __shared__ double sm[];
@@ -366,7 +342,6 @@ __global__ __launch_bounds__(kBlock) void k_precond_tiled(
         double s = block_sum(arr);
         if (threadIdx.x == 0) part_rr[c * kMaxParts + blockIdx.x] = s;
     }
-    if (fuse_beta && arrive_last(&stt->ticket[2])) pcg_beta_body(stt, part_rz, gridDim.x, gridDim.y);
 }
 
 // z = Q_i^-1 r per individual block by the 2x2 Woodbury form of
@@ -375,13 +350,12 @@ __global__ __launch_bounds__(kBlock) void k_precond_tiled(
 // (PCGsolver.cpp:1026-1045, applied at :1060-1093):
 //   w = U^T r, c = (sig2e I + C U^T U)^-1 C w, z = (r - U c) / sig2e.
 // tilde_v = 1 uses G_ii := 1 (tilde-V, varCompEst.cpp:839-858).
-// Fused partials: sum r.z and sum r.r per column; inside PCG (fuse_beta) the grid
-// reduction of the r.z partials into beta.
+// Fused partials: sum r.z and sum r.r per column.
 __global__ __launch_bounds__(kBlock) void k_precond(
     int32_t nblk, const int64_t* __restrict__ off, const int32_t* __restrict__ ind,
     const double* __restrict__ t, const double* __restrict__ gdiag, const double* __restrict__ th,
     const double* __restrict__ r, int64_t ldr, double* __restrict__ z, int64_t ldz,
-    double* part_rz, double* part_rr, PcgState* st, int tilde_v, int fuse_beta) {
+    double* part_rz, double* part_rr, const PcgState* st, int tilde_v) {
     if (st && st->done) return;
     const int c = blockIdx.y;
     if (st && !st->active[c]) {
@@ -390,7 +364,6 @@ __global__ __launch_bounds__(kBlock) void k_precond(
             if (part_rz) part_rz[c * kMaxParts + blockIdx.x] = 0.0;
             if (part_rr) part_rr[c * kMaxParts + blockIdx.x] = 0.0;
         }
-        if (fuse_beta && arrive_last(&st->ticket[2])) pcg_beta_body(st, part_rz, gridDim.x, gridDim.y);
         return;
     }
     const double se = th[c * 5 + 0], sg = th[c * 5 + 1], sb0 = th[c * 5 + 2],
@@ -439,7 +412,6 @@ __global__ __launch_bounds__(kBlock) void k_precond(
         double s = block_sum(arr);
         if (threadIdx.x == 0) part_rr[c * kMaxParts + blockIdx.x] = s;
     }
-    if (fuse_beta && arrive_last(&st->ticket[2])) pcg_beta_body(st, part_rz, gridDim.x, gridDim.y);
 }
 
 // log|Q_i| of the preconditioner blocks by Sylvester's determinant identity:
@@ -623,6 +595,26 @@ __global__ void k_set_column(int j, int64_t off, int32_t ntiles, int64_t nrec, d
     theta[5 * j + 4] = t4;
 }
 
+// One-block reductions of PCG's dot partials: alpha after V p, the convergence test
+// after the x/r update, beta after the preconditioner.
+__global__ __launch_bounds__(kBlock) void k_pcg_alpha(PcgState* st, const double* __restrict__ part_pvp,
+                                                      int nparts, int k) {
+    if (st->done) return;
+    pcg_alpha_body(st, part_pvp, nparts, k);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcg_check(PcgState* st, const double* __restrict__ part_rr,
+                                                      int nparts, int k) {
+    if (st->done) return;
+    pcg_check_body(st, part_rr, nparts, k);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcg_beta(PcgState* st, const double* __restrict__ part_rz,
+                                                     int nparts, int k) {
+    if (st->done) return;
+    pcg_beta_body(st, part_rz, nparts, k);
+}
+
 __global__ __launch_bounds__(kBlock) void k_pcg_xr(PcgState* st, int64_t nrec, double* __restrict__ x,
                                                    int64_t ldx, double* __restrict__ r,
                                                    const double* __restrict__ p,
@@ -648,8 +640,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_xr(PcgState* st, int64_t nrec, d
     }
     double s = block_sum(acc);
     if (threadIdx.x == 0) part_rr[c * kMaxParts + blockIdx.x] = s;
-    // the last workgroup runs the convergence test on all r.r partials
-    if (arrive_last(&st->ticket[1])) pcg_check_body(st, part_rr, gridDim.x, gridDim.y);
 }
 
 __global__ __launch_bounds__(kBlock) void k_pcg_p(PcgState* st, int64_t nrec, double* __restrict__ p,
@@ -786,27 +776,23 @@ void launch_seg_sums(hipStream_t s, const RecMeta& m, const double* x, int64_t l
 void launch_record_out(hipStream_t s, const RecMeta& m, const double* x, int64_t ldx, int k,
                        const double* GZ, const double* S0, const double* S1, int64_t ldn,
                        const double* theta_cols, double* y, int64_t ldy, double* part_dot,
-                       const int* skip, PcgState* alpha_st) {
+                       const int* skip) {
     dim3 g(parts_for(m.nrec), k);
-    k_record_out<<<g, kBlock, 0, s>>>(m, x, ldx, GZ, S0, S1, ldn, theta_cols, y, ldy, part_dot, skip,
-                                      part_dot ? alpha_st : nullptr);
+    k_record_out<<<g, kBlock, 0, s>>>(m, x, ldx, GZ, S0, S1, ldn, theta_cols, y, ldy, part_dot, skip);
     LAUNCH_CHECK();
 }
 
 void launch_precond(hipStream_t s, const RecMeta& m, const double* gdiag, const double* theta_cols,
                     const double* r, int64_t ldr, int k, double* z, int64_t ldz, double* part_rz,
-                    double* part_rr, const PcgState* st, int tilde_v, bool fuse_beta) {
-    require(!fuse_beta || (st && part_rz), GLOBE_E_INVALID, "beta needs the PCG state and r.z partials");
-    PcgState* sw = const_cast<PcgState*>(st);
-    const int fb = fuse_beta ? 1 : 0;
+                    double* part_rr, const PcgState* st, int tilde_v) {
     if (m.cap > 0) {
         dim3 g(precond_parts(m), k);
         k_precond_tiled<<<g, kBlock, sizeof(double) * 4 * m.cap, s>>>(m, gdiag, theta_cols, r, ldr, z,
-                                                                       ldz, part_rz, part_rr, sw, tilde_v, fb);
+                                                                       ldz, part_rz, part_rr, st, tilde_v);
     } else {
         dim3 g(parts_for(m.nblk), k);
         k_precond<<<g, kBlock, 0, s>>>(m.nblk, m.blk_off, m.blk_ind, m.times, gdiag, theta_cols, r,
-                                       ldr, z, ldz, part_rz, part_rr, sw, tilde_v, fb);
+                                       ldr, z, ldz, part_rz, part_rr, st, tilde_v);
     }
     LAUNCH_CHECK();
 }
@@ -883,6 +869,21 @@ void launch_pcg_init(hipStream_t s, PcgState* st, const double* part_bb, const d
     LAUNCH_CHECK();
 }
 
+
+void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* part_pvp, int nparts, int k) {
+    k_pcg_alpha<<<1, kBlock, 0, s>>>(st, part_pvp, nparts, k);
+    LAUNCH_CHECK();
+}
+
+void launch_pcg_check(hipStream_t s, PcgState* st, const double* part_rr, int nparts, int k) {
+    k_pcg_check<<<1, kBlock, 0, s>>>(st, part_rr, nparts, k);
+    LAUNCH_CHECK();
+}
+
+void launch_pcg_beta(hipStream_t s, PcgState* st, const double* part_rz, int nparts, int k) {
+    k_pcg_beta<<<1, kBlock, 0, s>>>(st, part_rz, nparts, k);
+    LAUNCH_CHECK();
+}
 
 void launch_pcg_xr(hipStream_t s, PcgState* st, int64_t nrec, double* x, int64_t ldx, double* r,
                    const double* p, const double* vp, int64_t ld, int k, double* part_rr,

@@ -195,7 +195,13 @@ void wide_jt(const PassGeom& p, int64_t rb, int64_t bqb, int64_t pw, const Varia
         case 0: wide_q<JT, 0>(p, rb, bqb, pw, v, st); break;
         case 1: wide_q<JT, 1>(p, rb, bqb, pw, v, st); break;
         case 2: wide_q<JT, 2>(p, rb, bqb, pw, v, st); break;
-        default: wide_q<JT, 3>(p, rb, bqb, pw, v, st); break;
+        case 3: wide_q<JT, 3>(p, rb, bqb, pw, v, st); break;
+        case 4: wide_q<JT, 4>(p, rb, bqb, pw, v, st); break;
+        case 5: wide_q<JT, 5>(p, rb, bqb, pw, v, st); break;
+        case 6: wide_q<JT, 6>(p, rb, bqb, pw, v, st); break;
+        case 7: wide_q<JT, 7>(p, rb, bqb, pw, v, st); break;
+        case 8: wide_q<JT, 8>(p, rb, bqb, pw, v, st); break;
+        default: wide_q<JT, 9>(p, rb, bqb, pw, v, st); break;
     }
 }
 

@@ -278,11 +278,14 @@ void iterate(globe_ctx* c, ServeState& S, int k) {
     launch_fill(s, S1, (int64_t)n * k, 0.0);
     launch_seg_sums(s, m, S.P.as<double>(), 0, k, S0, S1, n, skip);
     genetic_product(c, S0, k, GZ, S.o.precision, skip, th);
-    launch_record_out(s, m, S.P.as<double>(), 0, k, GZ, S0, S1, n, th, S.VP.as<double>(), 0, part_a, skip, st);
+    launch_record_out(s, m, S.P.as<double>(), 0, k, GZ, S0, S1, n, th, S.VP.as<double>(), 0, part_a, skip);
+    launch_pcg_alpha(s, st, part_a, parts_for(cap), k);
     launch_pcg_xr(s, st, cap, S.X.as<double>(), 0, S.R.as<double>(), S.P.as<double>(), S.VP.as<double>(), 0, k,
                   part_b, off, cnt);
+    launch_pcg_check(s, st, part_b, parts_for(cap), k);
     launch_precond(s, m, c->d_gdiag.as<double>(), th, S.R.as<double>(), 0, k, S.Z.as<double>(), 0, part_c,
-                   nullptr, st, 0, true);
+                   nullptr, st, 0);
+    launch_pcg_beta(s, st, part_c, precond_parts(m), k);
     launch_pcg_p(s, st, cap, S.P.as<double>(), S.Z.as<double>(), 0, k, off, cnt);
     ++S.global_iter;
 }

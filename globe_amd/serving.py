@@ -86,3 +86,48 @@ def replay(g, requests, rate, seed=0):
     wall = time.perf_counter() - t0
     return sols, {"latency_s": lat, "iters": iters, "wall_s": wall, "steps": steps,
                   "mean_active": occupancy / max(steps, 1)}
+
+
+def main(argv=None):
+    """Replay synthetic requests on a synthetic resident cohort (genotypes generated in
+    HBM): python -m globe_amd.serving --n 200000 --m 500000 --requests 200 --rate 1000.
+    A 16-byte device copy before and after the replay marks its span in a kernel trace."""
+    import argparse
+    import json
+    import globe_amd as G
+    from globe_amd import synth
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=200_000)
+    ap.add_argument("--m", type=int, default=500_000)
+    ap.add_argument("--lam", type=float, default=7.0)
+    ap.add_argument("--requests", type=int, default=200)
+    ap.add_argument("--rate", type=float, default=1000.0)
+    ap.add_argument("--slots", type=int, default=32)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--seed", type=int, default=20261206)
+    a = ap.parse_args(argv)
+    rng = np.random.default_rng(a.seed)
+    g = G.Globe(0)
+    c = synth.simulate_cohort(a.n, a.lam, rng)
+    g.set_cohort(c.indiv, c.times, c.X)
+    kept = g.synth_bed(a.m, seed=a.seed, f_lo=0.01, f_hi=0.5, miss=0.01, min_maf=0.01, max_miss=0.05)
+    g.set_varcomp((0.6, 0.4, 0.4, 0.2, 0.2))
+    u_pool = [np.sqrt(1.0 / kept) * g.geno_Wa(rng.standard_normal(kept)) for _ in range(4)]
+    reqs = [make_request(rng, a.n, min(5_000, a.n), a.n, 2.0, 10.0, u_pool) for _ in range(a.requests)]
+    cap = max(r[0].shape[0] for r in reqs)
+    g.serve_open(slots=a.slots, max_records=cap, tol=a.tol, max_iter=1000, precision=G.PREC_MIXED)
+    marker = g.alloc(256)
+    g._check(g.L.globe_dev_copy(g.h, marker.ptr, marker.ptr + 128, 16))
+    _, st = replay(g, reqs, a.rate, seed=a.seed)
+    g._check(g.L.globe_dev_copy(g.h, marker.ptr, marker.ptr + 128, 16))
+    g.serve_close()
+    lat = st["latency_s"]
+    print(json.dumps({"requests": a.requests, "rate": a.rate, "wall_s": st["wall_s"],
+                      "solves_per_s": a.requests / st["wall_s"], "p50_s": float(np.percentile(lat, 50)),
+                      "p99_s": float(np.percentile(lat, 99)), "steps": st["steps"],
+                      "mean_active": st["mean_active"]}), flush=True)
+    g.close()
+
+
+if __name__ == "__main__":
+    main()
