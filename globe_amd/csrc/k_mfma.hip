@@ -152,10 +152,12 @@ __device__ inline void thin_mma(const uint8_t* __restrict__ slot, int wave, int 
 // copies of the NS - 2 later stages in flight), then barrier: after it, every wave's
 // copies of the stage are visible in LDS. Waves that copy no digits issue fewer copies
 // per stage, so their count is smaller.
-// The copies into LDS count in lgkmcnt as well as vmcnt, so the LDS counter waits for the
-// same number of later copies (capped at its 4-bit field; waiting for fewer is safe): no
-// LDS read is outstanding here, since every read of the previous stage fed an MFMA, and
-// lgkmcnt(0) would drain the copies of the later stages too.
+// The compiler treats a copy into LDS as an event of the LDS counter as well as of the
+// memory counter (with one in flight it waits lgkmcnt(0) before any LDS read), so the
+// per-stage wait leaves the later stages' copies in flight on both counters (the LDS one
+// capped at its 4-bit field; waiting for fewer is safe). No LDS read is outstanding
+// here -- every read of the previous stage fed an MFMA -- so this is exact whether or not
+// the hardware counts the copies there, and lgkmcnt(0) could drain the later copies.
 constexpr int lgkm_cap(int n) { return n < 15 ? n : 15; }
 
 template <int W, int RT, int NS, int SS>
