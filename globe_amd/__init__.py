@@ -536,6 +536,8 @@ class Globe:
         r = Request(indiv.shape[0], indiv.ctypes.data, times.ctypes.data, y.ctypes.data,
                     (C.c_double * 5)(*theta))
         rid = C.c_int64()
+        # the request's id is recorded before the call returns to the caller, so a stepping
+        # thread can take it as soon as it is finished
         self._check(self.L.globe_serve_submit(self.h, C.byref(r), C.byref(rid)))
         self._serve_nrec[rid.value] = indiv.shape[0]
         return rid.value
@@ -553,7 +555,10 @@ class Globe:
         return [int(v) for v in ids[:n.value]]
 
     def serve_take(self, rid):
-        x = np.zeros(self._serve_nrec[rid])
+        nrec = self._serve_nrec.get(rid)
+        if nrec is None:
+            raise KeyError(f"request {rid}: serve_submit has not returned it to this handle")
+        x = np.zeros(nrec)
         res = ServeResult()
         self._check(self.L.globe_serve_take(self.h, rid, _p(x), C.byref(res)))
         del self._serve_nrec[rid]

@@ -3,6 +3,7 @@
 #pragma once
 #include <array>
 #include <map>
+#include <mutex>
 
 #include "globe_internal.h"
 
@@ -12,6 +13,7 @@ struct globe_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    std::mutex err_mu;  // globe_serve_submit may fail on a second thread
 
     // cohort
     bool has_cohort = false;
@@ -134,10 +136,12 @@ struct globe_ctx {
     return GLOBE_OK;                                   \
     }                                                  \
     catch (const globe::Error& e) {                    \
+        std::lock_guard<std::mutex> lk((ctx)->err_mu); \
         (ctx)->err = e.what();                         \
         return e.code;                                 \
     }                                                  \
     catch (const std::exception& e) {                  \
+        std::lock_guard<std::mutex> lk((ctx)->err_mu); \
         (ctx)->err = e.what();                         \
         return GLOBE_E_INVALID;                        \
     }

@@ -26,6 +26,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <mutex>
 
 #include "context.h"
 
@@ -56,6 +57,9 @@ struct ServeState {
     hipStream_t copy = nullptr;   // uploads of staged requests
     hipEvent_t staged_ev = nullptr;
     std::map<int64_t, int64_t> free_ranges;  // pool offset -> length
+    // the queue and the id counter are shared with globe_serve_submit, which may run on
+    // another thread (request intake) while the loop steps
+    std::mutex qmu;
     std::deque<Request> queue;
     std::vector<Request> slot_req;
     std::vector<int> slot_state;
@@ -189,17 +193,22 @@ void prepare(globe_ctx* c, const ServeState& S, const globe_request* q, ServeSta
 // Reserve a slot and a pool range for the queue's head and upload its data on the copy
 // stream (it joins the block at the next admit_staged).
 bool stage_next(ServeState& S) {
-    if (S.queue.empty()) return false;
     int j = 0;
     while (j < S.slots && S.slot_state[j] != ServeState::kFree) ++j;
     if (j == S.slots) return false;
-    const int64_t off = S.alloc(S.queue.front().span);
-    if (off < 0) {  // first in, first admitted: the head waits for room
-        ++S.waits_for_room;
-        return false;
+    ServeState::Request r;
+    int64_t off = -1;
+    {
+        std::lock_guard<std::mutex> lk(S.qmu);
+        if (S.queue.empty()) return false;
+        off = S.alloc(S.queue.front().span);
+        if (off < 0) {  // first in, first admitted: the head waits for room
+            ++S.waits_for_room;
+            return false;
+        }
+        r = std::move(S.queue.front());
+        S.queue.pop_front();
     }
-    ServeState::Request r = std::move(S.queue.front());
-    S.queue.pop_front();
     const int64_t nrec = r.nrec;
     auto h2d = [&](void* dst, const void* src, size_t bytes) {
         if (bytes) HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S.copy));
@@ -377,6 +386,7 @@ int globe_serve_submit(globe_ctx* c, const globe_request* q, int64_t* id) {
     ServeState::Request r;
     prepare(c, S, q, r);
     require(r.span <= S.pool, GLOBE_E_INVALID, "request larger than the HBM budget");
+    std::lock_guard<std::mutex> lk(S.qmu);
     r.id = S.next_id++;
     r.t_submit = S.now();
     if (id) *id = r.id;
@@ -480,7 +490,10 @@ int globe_serve_step(globe_ctx* c, int32_t* finished, int32_t* active, int32_t* 
         for (int j = 0; j < S.slots; ++j) a += S.slot_state[j] == ServeState::kBusy ? 1 : 0;
         *active = a;
     }
-    if (queued) *queued = (int32_t)S.queue.size();
+    if (queued) {
+        std::lock_guard<std::mutex> lk(S.qmu);
+        *queued = (int32_t)S.queue.size();
+    }
     API_END(c)
 }
 

@@ -51,8 +51,11 @@ def make_request(rng, n_cohort, size_lo, size_hi, lam_lo, lam_hi, u_pool=None):
 
 
 def replay(g, requests, rate, seed=0):
-    """Submit requests at Poisson arrival times (rate per second) and run the loop until
-    all finish. Returns (solutions, stats) with per-request latency and loop counters."""
+    """Submit requests at Poisson arrival times (rate per second) from an intake thread
+    (globe_serve_submit prepares each request on the host while the loop steps the GPU)
+    and run the loop until all finish. Returns (solutions, stats) with per-request
+    latency (completion - arrival) and loop counters."""
+    import threading
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(requests)))
     arrivals -= arrivals[0]
@@ -60,29 +63,52 @@ def replay(g, requests, rate, seed=0):
     lat = np.zeros(len(requests))
     iters = np.zeros(len(requests), dtype=np.int64)
     idx = {}
-    nxt = 0
+    lock = threading.Lock()
+    finished_intake = threading.Event()
+    failure = []
+    t0 = time.perf_counter()
+
+    def intake():
+        try:
+            for q, req in enumerate(requests):
+                wait = arrivals[q] - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(wait)
+                rid = g.serve_submit(*req)
+                with lock:
+                    idx[rid] = q
+        except Exception as e:  # surfaced by the stepping thread
+            failure.append(e)
+        finally:
+            finished_intake.set()
+
+    th = threading.Thread(target=intake, daemon=True)
+    th.start()
     steps = 0
     occupancy = 0
-    t0 = time.perf_counter()
-    while nxt < len(requests) or idx:
-        now = time.perf_counter() - t0
-        while nxt < len(requests) and arrivals[nxt] <= now:
-            indiv, times, y, theta = requests[nxt]
-            idx[g.serve_submit(indiv, times, y, theta)] = nxt
-            nxt += 1
+    done = 0
+    while done < len(requests):
+        if failure:
+            raise failure[0]
         _, active, queued = g.serve_step()
-        steps += 1
-        occupancy += active
+        if active:
+            steps += 1
+            occupancy += active
         for rid in g.serve_finished():
+            with lock:
+                q = idx.pop(rid, None)
+            if q is None:  # its id is not recorded yet: take it at the next step
+                continue
             x, st = g.serve_take(rid)
-            q = idx.pop(rid)
             sols[q] = x
             lat[q] = (time.perf_counter() - t0) - arrivals[q]
             iters[q] = st["iters"]
-        if not idx and nxt < len(requests):
-            wait = arrivals[nxt] - (time.perf_counter() - t0)
-            if wait > 0:
-                time.sleep(wait)
+            done += 1
+        if not active and not queued:
+            if finished_intake.is_set() and done >= len(requests):
+                break
+            time.sleep(2e-4)  # idle: leave the interpreter to the intake thread
+    th.join()
     wall = time.perf_counter() - t0
     return sols, {"latency_s": lat, "iters": iters, "wall_s": wall, "steps": steps,
                   "mean_active": occupancy / max(steps, 1)}

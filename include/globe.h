@@ -7,8 +7,8 @@
  * explicit leading dimension (the reference keeps Eigen column-major float
  * matrices); record-space matrices have nrec rows, individual-space ones n rows.
  *
- * One context per host thread (not re-entrant); all device work of a context is
- * ordered on one HIP stream. Every call returns a globe_status; on failure
+ * One context per host thread (not re-entrant; the one exception is globe_serve_submit,
+ * see the serving loop); all device work of a context is ordered on one HIP stream. Every call returns a globe_status; on failure
  * globe_last_error() describes it. Nothing here exits the process (the
  * reference calls exit(1) on bad input; the CLIs built on this ABI keep that).
  *
@@ -345,7 +345,9 @@ typedef struct {
  * of the record pool; a request's data is uploaded while the block iterates and it joins
  * at the next step. */
 int globe_serve_open(globe_ctx* ctx, const globe_serve_opts* opts);
-/* Copies and queues a request; *id identifies it in globe_serve_take. */
+/* Copies, validates and queues a request; *id identifies it in globe_serve_take. This one
+ * call may run on a second host thread (request intake) concurrently with the thread that
+ * calls globe_serve_step, so requests are prepared on the host while the GPU iterates. */
 int globe_serve_submit(globe_ctx* ctx, const globe_request* req, int64_t* id);
 /* Admits queued requests (FIFO) into free slots while the HBM budget allows, runs one
  * iteration of the block PCG over the occupied slots and retires the requests whose
