@@ -17,16 +17,20 @@ namespace plan {
 
 // ---------------------------------------------------------------- block shapes
 // wide kernel (k_geno_wide): waves, ring stages, column groups, row fragments per wave,
-// and PF: 1 reads the digits from LDS two K-steps ahead of their MFMAs, pinned in that
-// order (0: one K-step ahead, as the compiler schedules). PF changes only the order of
-// the LDS reads.
+// PF: 1 reads the digits from LDS two K-steps ahead of their MFMAs, pinned in that
+// order (0: one K-step ahead, as the compiler schedules; PF changes only the order of
+// the LDS reads), and KS: K-blocks of 128 genotypes per stage. KS = 2 stages a whole
+// super-step (64 packed bytes of a row per copy instruction instead of 32: the row
+// stream reads each 128-byte line in two requests rather than four) and halves the
+// barriers per MFMA.
 struct WideShape {
-    int W, NS, CG, RA, PF;
+    int W, NS, CG, RA, PF, KS;
 };
-constexpr int kNumWide = 10;
-constexpr WideShape kWide[kNumWide] = {{8, 4, 1, 2, 0}, {4, 3, 1, 2, 0}, {8, 4, 2, 4, 0}, {4, 3, 2, 4, 0},
-                                       {8, 4, 1, 2, 1}, {4, 3, 1, 2, 1}, {8, 4, 2, 4, 1}, {4, 3, 2, 4, 1},
-                                       {4, 5, 1, 2, 1}, {8, 6, 1, 2, 1}};
+constexpr int kNumWide = 14;
+constexpr WideShape kWide[kNumWide] = {
+    {8, 4, 1, 2, 0, 1}, {4, 3, 1, 2, 0, 1}, {8, 4, 2, 4, 0, 1}, {4, 3, 2, 4, 0, 1}, {8, 4, 1, 2, 1, 1},
+    {4, 3, 1, 2, 1, 1}, {8, 4, 2, 4, 1, 1}, {4, 3, 2, 4, 1, 1}, {4, 5, 1, 2, 1, 1}, {8, 6, 1, 2, 1, 1},
+    {8, 2, 1, 2, 1, 2}, {8, 2, 2, 4, 1, 2}, {4, 2, 1, 2, 1, 2}, {8, 3, 1, 2, 1, 2}};
 // thin kernel (k_geno_thin): waves, row fragments of 16 rows per wave, ring stages,
 // super-steps (256 genotypes = 64 packed bytes of a row) per stage
 struct ThinShape {
@@ -37,27 +41,30 @@ constexpr ThinShape kThin[kNumThin] = {{4, 4, 3, 1},  {8, 2, 3, 2},  {8, 2, 4, 2
                                        {16, 1, 4, 2}, {8, 1, 5, 2}, {4, 4, 4, 2}, {8, 1, 6, 2}};
 
 // Per-stage layout of the wide kernel <JT column fragments of 32 digit columns, W waves,
-// CG column groups, RA row fragments of 32 rows per wave>: the waves form W / CG row groups
-// x CG column groups; a wave owns RA row fragments and CB = JT / CG column fragments, so a
-// block covers (W / CG) 32 RA rows x all 32 JT digit columns.
-template <int JT, int W, int CG, int RA>
+// CG column groups, RA row fragments of 32 rows per wave, KS K-blocks of 128 genotypes
+// per stage>: the waves form W / CG row groups x CG column groups; a wave owns RA row
+// fragments and CB = JT / CG column fragments, so a block covers (W / CG) 32 RA rows x all
+// 32 JT digit columns. A row fragment takes KS KB of a stage (32 rows x 32 KS packed
+// bytes), the digits 4 JT KS chunks of 1 KB (K-step major, then column fragment).
+template <int JT, int W, int CG, int RA, int KS = 1>
 struct WideCfg {
-    static constexpr int kW = W, kJT = JT, kRA = RA;
+    static constexpr int kW = W, kJT = JT, kRA = RA, kKS = KS;
     static constexpr int kCB = JT / CG;                 // column fragments per wave
     static constexpr int kRowGroups = W / CG;
     static constexpr int kRows = kRowGroups * 32 * RA;  // rows per block
-    static constexpr int kAFrags = kRowGroups * RA;     // 1 KB row fragments per stage
-    static constexpr int kAPerWave = kAFrags / W;
-    static constexpr int kA = kAFrags * 1024;
-    static constexpr int kB = 4 * JT * 1024;            // digits of a stage
+    static constexpr int kAFrags = kRowGroups * RA;     // row fragments per stage
+    static constexpr int kAPerWave = kAFrags / W;       // row fragments copied per wave
+    static constexpr int kA = kAFrags * 1024 * KS;
+    static constexpr int kB = 4 * JT * 1024 * KS;       // digits of a stage
     static constexpr int kSlot = kA + kB;
-    static constexpr int kBChunks = 4 * JT;             // 1 KB digit chunks
+    static constexpr int kBChunks = 4 * JT * KS;        // 1 KB digit chunks
     static constexpr int kBPerWave = kBChunks >= W ? kBChunks / W : 1;
     static constexpr int kSink = kBChunks < W ? (W - kBChunks) * 1024 : 0;
-    static constexpr int kOps = kAPerWave + kBPerWave;  // copies per wave per stage
+    static constexpr int kOps = KS * kAPerWave + kBPerWave;  // copies per wave per stage
     static_assert(JT % CG == 0 && W % CG == 0, "column groups");
     static_assert(kAFrags % W == 0, "row fragments split evenly over waves");
     static_assert(kBChunks < W || kBChunks % W == 0, "digit chunks split evenly over waves");
+    static_assert(KS == 1 || KS == 2, "one or two K-blocks per stage");
 };
 
 // Per-stage layout of the thin kernel <W waves, RT row fragments of 16 rows per wave, SS
@@ -81,12 +88,13 @@ struct ThinCfg {
 };
 
 // A wide shape fits JT column fragments when the column groups divide them, the stage's
-// 4 JT digit chunks spread evenly over the waves (or number fewer than them) and its ring
-// of NS stages fits the 160 KB of LDS.
+// 4 JT KS digit chunks spread evenly over the waves (or number fewer than them) and its
+// ring of NS stages fits the 160 KB of LDS.
 constexpr bool wide_fits(int JT, int q) {
-    return JT % kWide[q].CG == 0 && (4 * JT < kWide[q].W || (4 * JT) % kWide[q].W == 0) &&
-           kWide[q].NS * ((kWide[q].W / kWide[q].CG) * kWide[q].RA + 4 * JT) * 1024 +
-                   (4 * JT < kWide[q].W ? (kWide[q].W - 4 * JT) * 1024 : 0) <=
+    return JT % kWide[q].CG == 0 &&
+           (4 * JT * kWide[q].KS < kWide[q].W || (4 * JT * kWide[q].KS) % kWide[q].W == 0) &&
+           kWide[q].NS * ((kWide[q].W / kWide[q].CG) * kWide[q].RA + 4 * JT) * kWide[q].KS * 1024 +
+                   (4 * JT * kWide[q].KS < kWide[q].W ? (kWide[q].W - 4 * JT * kWide[q].KS) * 1024 : 0) <=
                160 * 1024;
 }
 
@@ -137,6 +145,24 @@ __host__ __device__ inline int64_t wide_row_base(int64_t blk_row, int rr, int g,
 }
 __host__ __device__ inline int64_t wide_stage_step(int64_t t) { return 32 * t; }
 __host__ __device__ inline int64_t wide_frag_stride(int64_t pitch) { return 32 * pitch; }
+// Packed-row reads with 256-genotype stages (KS = 2): row fragment f of a stage holds 32
+// rows x 64 packed bytes in 2 KB of LDS, row r at 64 r with its 16-byte chunk c at
+// position c ^ wide_swz(r), so that the MFMA reads of one chunk by rows 0..31 spread over
+// the LDS banks. Copy instruction i (0, 1) of a fragment: lane -> row 16 i + lane / 4, LDS
+// position lane % 4, i.e. every row's 4 lanes read its 64 contiguous bytes and the 64
+// lanes land as one contiguous KB. Offsets are from the block's first row at its first
+// stage, wide_row_base(blk_row, 0, 0, H0, pitch); t counts 256-genotype stages.
+__host__ __device__ constexpr int wide_swz(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
+__host__ __device__ inline int wide2_copy_row(int i, int lane) { return 16 * i + (lane >> 2); }
+__host__ __device__ inline int wide2_copy_chunk(int i, int lane) {
+    return (lane & 3) ^ wide_swz(wide2_copy_row(i, lane));
+}
+__host__ __device__ inline int64_t wide2_lane_off(int f, int i, int lane, int64_t pitch) {
+    return (int64_t)(32 * f + wide2_copy_row(i, lane)) * pitch + 16 * wide2_copy_chunk(i, lane);
+}
+__host__ __device__ inline int64_t wide2_stage_step(int64_t t) { return 64 * t; }
+// LDS byte of row rr's chunk c in a 2 KB fragment
+__host__ __device__ inline int wide2_lds_chunk(int rr, int c) { return rr * 64 + ((c ^ wide_swz(rr)) << 4); }
 // Digit reads: lane copies the 16 bytes at dig_stage(H, kB) + dig_lane(chunk, lane) from
 // 1 KB chunk `chunk` of stage H (kB bytes of digits per stage; both kernels).
 __host__ __device__ inline int64_t dig_stage(int64_t H, int kB) { return H * kB; }

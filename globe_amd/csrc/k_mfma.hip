@@ -340,6 +340,75 @@ __device__ inline void wide_mma(const uint8_t* __restrict__ slot, int rgroup, in
     }
 }
 
+// 256-genotype stages (KS = 2, plan::wide2_*): each of the wave's row fragments takes two
+// copy instructions of 16 rows x 64 packed bytes; abase is the block's first row at its
+// first stage (uniform), aoff the lane's 32-bit offsets from it.
+template <class C>
+__device__ inline void wide2_issue(const uint8_t* __restrict__ abase, const uint32_t (&aoff)[C::kAPerWave][2],
+                                   const uint8_t* __restrict__ bbase, uint32_t boff, uint8_t* __restrict__ slot,
+                                   uint8_t* __restrict__ sink, int wave, int64_t t) {
+    const uint8_t* at = abase + plan::wide2_stage_step(t);
+#pragma unroll
+    for (int q = 0; q < C::kAPerWave; ++q) {
+        const int f = wave + C::kW * q;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            __builtin_amdgcn_global_load_lds((const void*)(at + aoff[q][i]), (lds_void*)(slot + f * 2048 + i * 1024),
+                                             16, 0, 0);
+    }
+    const uint8_t* bt = bbase + plan::dig_stage(t, C::kB) + boff;
+#pragma unroll
+    for (int q = 0; q < C::kBPerWave; ++q) {
+        const int chunk = wave + C::kW * q;
+        if (chunk < C::kBChunks)
+            __builtin_amdgcn_global_load_lds((const void*)(bt + chunk * 1024), (lds_void*)(slot + C::kA + chunk * 1024),
+                                             16, 0, 0);
+        else
+            __builtin_amdgcn_global_load_lds((const void*)(bt + (chunk % C::kBChunks) * 1024),
+                                             (lds_void*)(sink + (chunk - C::kBChunks) * 1024), 16, 0, 0);
+    }
+}
+
+// 8 K-steps of a 256-genotype stage: K-block kb (0, 1) of row fragment a is chunk
+// 2 kb + g of row rr (the 16 packed bytes that K-steps 4 kb .. 4 kb + 3 of the 128-genotype
+// layout read); digit chunk of K-step ks, column fragment b: ks JT + b. Digits two K-steps
+// ahead, pinned as in wide_mma<C, 1>.
+template <class C>
+__device__ inline void wide2_mma(const uint8_t* __restrict__ slot, int rgroup, int cgroup, int lane,
+                                 v16i (&acc)[4][4]) {
+    constexpr int RA = C::kRA, CB = C::kCB, JT = C::kJT;
+    const int rr = lane & 31, g = lane >> 5;
+    const uint8_t* cl = slot + C::kA + (cgroup * CB) * 1024 + lane * 16;
+    v4i bf[8][CB];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int b = 0; b < CB; ++b) bf[s][b] = *reinterpret_cast<const v4i*>(cl + (s * JT + b) * 1024);
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        uint4 av[RA];
+#pragma unroll
+        for (int a = 0; a < RA; ++a)
+            av[a] = *reinterpret_cast<const uint4*>(slot + (rgroup * RA + a) * 2048 + plan::wide2_lds_chunk(rr, 2 * kb + g));
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int ks = 4 * kb + s;
+            if (ks + 2 < 8)
+#pragma unroll
+                for (int b = 0; b < CB; ++b)
+                    bf[ks + 2][b] = *reinterpret_cast<const v4i*>(cl + ((ks + 2) * JT + b) * 1024);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int a = 0; a < RA; ++a) {
+                const v4i ad = decode16(dword_of(av[a], s));
+#pragma unroll
+                for (int b = 0; b < CB; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ad, bf[ks][b], acc[a][b], 0, 0, 0);
+            }
+        }
+    }
+}
+
 // Wait until this wave's copies of the stage about to be multiplied have landed (at
 // most the copies of the NS - 2 later stages still in flight) and its LDS reads of
 // the previous stage are complete, then synchronize the block, so that every wave's
@@ -353,25 +422,42 @@ __device__ inline void wide_stage_barrier() {
 
 // All operands pass through LDS (no loaded value lives in VGPRs across a stage), so
 // the only waits on the copies are the explicit per-stage ones above.
-template <int JT, int W, int NS, int CG, int RA, int PF>
+template <int JT, int W, int NS, int CG, int RA, int PF, int KS>
 __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(2))) void k_geno_wide(
-    const uint8_t* __restrict__ rows, int64_t pitch, int64_t nrows, int64_t n_ss,
-    int64_t ss_per_split, int64_t R, int S, const uint8_t* __restrict__ Bq,
-    int32_t* __restrict__ out, const int* skip) {
+    const uint8_t* __restrict__ rows, int64_t pitch, int64_t nrows, int64_t n_ss, int64_t ss_per_split,
+    int64_t R, int S, const uint8_t* __restrict__ Bq, int32_t* __restrict__ out, const int* skip) {
     if (skip && *skip) return;
-    using C = WideCfg<JT, W, CG, RA>;
+    using C = WideCfg<JT, W, CG, RA, KS>;
     constexpr int CB = C::kCB;
     __shared__ __attribute__((aligned(1024))) uint8_t lds[NS * C::kSlot + C::kSink];
     const plan::WideBlock wb = plan::wide_block(blockIdx.x, R, S, C::kRows, n_ss, ss_per_split);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int rgroup = wave / CG, cgroup = wave % CG;
     const int g = lane >> 5, rr = lane & 31;
-    const int64_t blk_row = wb.blk_row, H0 = wb.H0, nt = wb.nt;
-    // fragment f of the block = rows blk_row + 32 f .. + 31
+    const int64_t blk_row = wb.blk_row, H0 = wb.H0, nt = wb.nt / KS;  // stages of 128 KS genotypes
+    // KS = 1: fragment f of the block = rows blk_row + 32 f .. + 31, lane row pointer
     const uint8_t* arow = rows + plan::wide_row_base(blk_row, rr, g, H0, pitch);
     const int64_t fstride = plan::wide_frag_stride(pitch);
-    const uint8_t* bst = Bq + plan::dig_stage(H0, C::kB);
+    // KS = 2: uniform block base and the lane's 32-bit offsets (the launch checks that a
+    // block's rows span less than 4 GB)
+    const uint8_t* abase = rows + plan::wide_row_base(blk_row, 0, 0, H0, pitch);
+    uint32_t aoff[C::kAPerWave][2];
+    if constexpr (KS == 2) {
+#pragma unroll
+        for (int q = 0; q < C::kAPerWave; ++q)
+#pragma unroll
+            for (int i = 0; i < 2; ++i) aoff[q][i] = (uint32_t)plan::wide2_lane_off(wave + W * q, i, lane, pitch);
+    }
+    const uint8_t* bst = Bq + plan::dig_stage(H0 / KS, C::kB);
+    const uint32_t boff = (uint32_t)(lane * 16);
     uint8_t* sink = lds + NS * C::kSlot;
+    auto issue = [&](int64_t t, uint8_t* slot) {
+        if constexpr (KS == 1)
+            wide_issue<C>(arow + plan::wide_stage_step(t), fstride, bst + plan::dig_stage(t, C::kB), slot, sink, wave,
+                          lane);
+        else
+            wide2_issue<C>(abase, aoff, bst, boff, slot, sink, wave, t);
+    };
 
     v16i acc[4][4];
 #pragma unroll
@@ -384,11 +470,7 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(2))) voi
         // prologue: stages 0 .. NS - 2 (clamped to the last stage; a clamped copy
         // lands in a slot that is never multiplied)
 #pragma unroll
-        for (int u = 0; u < NS - 1; ++u) {
-            const int64_t t = u < nt ? u : nt - 1;
-            wide_issue<C>(arow + plan::wide_stage_step(t), fstride, bst + plan::dig_stage(t, C::kB),
-                          lds + u * C::kSlot, sink, wave, lane);
-        }
+        for (int u = 0; u < NS - 1; ++u) issue(u < nt ? u : nt - 1, lds + u * C::kSlot);
         for (int64_t t0 = 0; t0 < nt; t0 += NS) {
 #pragma unroll
             for (int u = 0; u < NS; ++u) {
@@ -397,11 +479,12 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(2))) voi
                 wide_stage_barrier<C, NS>();
                 // refill the slot multiplied in the previous stage with stage t + NS - 1,
                 // issued before this stage's MFMAs (the scheduler may not sink it)
-                const int64_t tn = t + NS - 1 < nt ? t + NS - 1 : nt - 1;
-                wide_issue<C>(arow + plan::wide_stage_step(tn), fstride, bst + plan::dig_stage(tn, C::kB),
-                              lds + ((u + NS - 1) % NS) * C::kSlot, sink, wave, lane);
+                issue(t + NS - 1 < nt ? t + NS - 1 : nt - 1, lds + ((u + NS - 1) % NS) * C::kSlot);
                 __builtin_amdgcn_sched_barrier(0);
-                wide_mma<C, PF>(lds + u * C::kSlot, rgroup, cgroup, lane, acc);
+                if constexpr (KS == 1)
+                    wide_mma<C, PF>(lds + u * C::kSlot, rgroup, cgroup, lane, acc);
+                else
+                    wide2_mma<C>(lds + u * C::kSlot, rgroup, cgroup, lane, acc);
             }
         }
         // no copy into LDS may be in flight when the block ends
@@ -697,8 +780,11 @@ void launch_wide_q(hipStream_t s, unsigned grid, const uint8_t* rows, int64_t pi
                    const int* skip) {
     if constexpr (wide_fits(JT, Q)) {
         constexpr WideShape w = kWide[Q];
-        k_geno_wide<JT, w.W, w.NS, w.CG, w.RA, w.PF><<<grid, w.W * 64, 0, s>>>(rows, pitch, nrows, nss, per, R,
-                                                                        S, Bq, out, skip);
+        // 256-genotype stages address a block's rows with 32-bit lane offsets
+        require(w.KS == 1 || (int64_t)rows_per_block(true, Q) * pitch < ((int64_t)1 << 32), GLOBE_E_INVALID,
+                "rows of a contraction block span 4 GB");
+        k_geno_wide<JT, w.W, w.NS, w.CG, w.RA, w.PF, w.KS><<<grid, w.W * 64, 0, s>>>(rows, pitch, nrows, nss, per,
+                                                                                    R, S, Bq, out, skip);
     } else {
         require(false, GLOBE_E_INVALID, "wide contraction shape not available for this width");
     }
@@ -718,7 +804,11 @@ void launch_wide(hipStream_t s, const Variant& v, unsigned grid, const uint8_t* 
         case 6: launch_wide_q<JT, 6>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
         case 7: launch_wide_q<JT, 7>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
         case 8: launch_wide_q<JT, 8>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
-        default: launch_wide_q<JT, 9>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
+        case 9: launch_wide_q<JT, 9>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
+        case 10: launch_wide_q<JT, 10>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
+        case 11: launch_wide_q<JT, 11>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
+        case 12: launch_wide_q<JT, 12>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
+        default: launch_wide_q<JT, 13>(s, grid, rows, pitch, nrows, nss, per, R, v.S, Bq, out, skip); break;
     }
 }
 

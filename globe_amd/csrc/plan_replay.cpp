@@ -49,10 +49,10 @@ struct Range {
 };
 
 // one wide launch: R row blocks x S splits
-template <int JT, int W, int CG, int RA>
+template <int JT, int W, int CG, int RA, int KS>
 void walk_wide(const PassGeom& p, int64_t row_bytes, int64_t bq_bytes, int64_t part_words, const Variant& v,
                int64_t* stats) {
-    using C = WideCfg<JT, W, CG, RA>;
+    using C = WideCfg<JT, W, CG, RA, KS>;
     const int64_t R = variant_rowblocks(true, v, p.nrows);
     const int S = v.S;
     const int64_t per = (p.nss + S - 1) / S;
@@ -62,23 +62,41 @@ void walk_wide(const PassGeom& p, int64_t row_bytes, int64_t bq_bytes, int64_t p
     std::vector<int64_t> kcov((size_t)R, 0);
     Range rows, dig;
     constexpr int J16 = 32 * JT;
+    // KS = 2: the lane offsets of both copy instructions of the first and the last
+    // fragment (the chunk swizzle is not monotone in the lane; the offsets do not depend
+    // on the block), walked once
+    Range lane_off;
+    if (KS == 2)
+        for (int f : {0, C::kAFrags - 1})
+            for (int i = 0; i < 2; ++i)
+                for (int lane = 0; lane < 64; ++lane) lane_off.add(wide2_lane_off(f, i, lane, p.pitch), 16);
+    check(KS == 1 || (lane_off.lo >= 0 && lane_off.hi - 16 < ((int64_t)1 << 32)), "lane offset outside 32 bits");
     for (int64_t b = 0; b < grid; ++b) {
         const WideBlock wb = wide_block(b, R, S, C::kRows, p.nss, per);
         check(wb.sp >= 0 && wb.sp < S && wb.rb >= 0 && wb.rb < R, "block coordinates out of range");
         check(!seen[(size_t)(wb.rb * S + wb.sp)]++, "two blocks share a (row block, split)");
         kcov[(size_t)wb.rb] += wb.nt;
-        if (wb.nt > 0) {
+        check(wb.nt % KS == 0 && wb.H0 % KS == 0, "split not in whole stages");
+        const int64_t nt = wb.nt / KS;
+        if (nt > 0 && KS == 1) {
             // copies at stages 0 .. nt - 1 (the ring's clamped copies repeat nt - 1)
-            for (int64_t t : {(int64_t)0, wb.nt - 1})
+            for (int64_t t : {(int64_t)0, nt - 1})
                 for (int f : {0, C::kAFrags - 1})
                     for (int lane : {0, 63}) {
                         const int64_t base = wide_row_base(wb.blk_row, lane & 31, lane >> 5, wb.H0, p.pitch);
                         rows.add(base + wide_stage_step(t) + f * wide_frag_stride(p.pitch), 16);
                     }
-            for (int64_t t : {(int64_t)0, wb.nt - 1})
-                for (int chunk : {0, C::kBChunks - 1})
-                    for (int lane : {0, 63}) dig.add(dig_stage(wb.H0 + t, C::kB) + dig_lane(chunk, lane), 16);
+        } else if (nt > 0) {
+            const int64_t base = wide_row_base(wb.blk_row, 0, 0, wb.H0, p.pitch);
+            for (int64_t t : {(int64_t)0, nt - 1}) {
+                rows.add(base + wide2_stage_step(t) + lane_off.lo, 16);
+                rows.add(base + wide2_stage_step(t) + lane_off.hi - 16, 16);
+            }
         }
+        if (nt > 0)
+            for (int64_t t : {(int64_t)0, nt - 1})
+                for (int chunk : {0, C::kBChunks - 1})
+                    for (int lane : {0, 63}) dig.add(dig_stage(wb.H0 / KS + t, C::kB) + dig_lane(chunk, lane), 16);
         // stores: rows of the block below nrows, every digit column of the fragment
         for (int rg : {0, W / CG - 1})
             for (int a : {0, RA - 1})
@@ -183,7 +201,10 @@ template <int JT, int Q>
 void wide_q(const PassGeom& p, int64_t rb, int64_t bqb, int64_t pw, const Variant& v, int64_t* st) {
     if constexpr (wide_fits(JT, Q)) {
         constexpr WideShape w = kWide[Q];
-        walk_wide<JT, w.W, w.CG, w.RA>(p, rb, bqb, pw, v, st);
+        static_assert(w.NS * WideCfg<JT, w.W, w.CG, w.RA, w.KS>::kSlot + WideCfg<JT, w.W, w.CG, w.RA, w.KS>::kSink <=
+                          160 * 1024,
+                      "ring exceeds the LDS");
+        walk_wide<JT, w.W, w.CG, w.RA, w.KS>(p, rb, bqb, pw, v, st);
     } else {
         check(false, "wide shape listed for a width it does not fit");
     }
@@ -201,7 +222,11 @@ void wide_jt(const PassGeom& p, int64_t rb, int64_t bqb, int64_t pw, const Varia
         case 6: wide_q<JT, 6>(p, rb, bqb, pw, v, st); break;
         case 7: wide_q<JT, 7>(p, rb, bqb, pw, v, st); break;
         case 8: wide_q<JT, 8>(p, rb, bqb, pw, v, st); break;
-        default: wide_q<JT, 9>(p, rb, bqb, pw, v, st); break;
+        case 9: wide_q<JT, 9>(p, rb, bqb, pw, v, st); break;
+        case 10: wide_q<JT, 10>(p, rb, bqb, pw, v, st); break;
+        case 11: wide_q<JT, 11>(p, rb, bqb, pw, v, st); break;
+        case 12: wide_q<JT, 12>(p, rb, bqb, pw, v, st); break;
+        default: wide_q<JT, 13>(p, rb, bqb, pw, v, st); break;
     }
 }
 
