@@ -723,12 +723,13 @@ int globe_grm_from_geno(globe_ctx* c) {
     HIP_CHECK(hipMemsetAsync(c->d_grm.p, 0, grm_tile_bytes(n), s));
     DevBuf GZ;
     GZ.ensure(sizeof(double) * (size_t)n * kb);
-    // G e_j, rows i >= j (W^T e_j is individual j's standardised row; W u runs on 5
+    // G e_j, rows i >= j (W^T e_j is individual j's standardised row; W u runs on 4
     // int8 digits, about 1e-10 relative), rounded to FP32
     for (int64_t j0 = 0; j0 < n; j0 += kb) {
         const int k = (int)std::min<int64_t>(kb, n - j0);
-        // rows from j0 down to a multiple of 4096 (few distinct shapes for the tuner)
-        geno_unit_columns(c, j0, k, GZ.as<double>(), j0 / 4096 * 4096);
+        // rows from j0 down to a multiple of 512 (the packed layouts' row padding); the
+        // contraction reuses the block shape tuned for all rows on the shorter ranges
+        geno_unit_columns(c, j0, k, GZ.as<double>(), j0 / kRowPad * kRowPad);
         launch_grm_scatter(s, GZ.as<double>(), n, n, j0, k, c->d_grm.as<float>());
     }
     HIP_CHECK(hipStreamSynchronize(s));

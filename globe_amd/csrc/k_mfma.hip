@@ -47,6 +47,7 @@
 // K-chunk of Bq in that XCD's L2. Partial sums per split go to out[split][row][J16]
 // and are added exactly (int64) in the epilogue.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 #include <cstdio>
@@ -550,17 +551,27 @@ __global__ __launch_bounds__(256) void k_colmax_parts(int64_t rows, const double
 
 // One wave per column: max (want_max) or ordered sum of the column's partials; the
 // scale exponent sh_c = (8 L_c - 2) - e_c with max|v| < 2^e_c when sh != nullptr.
-__global__ __launch_bounds__(256) void k_finish_cols(const double* __restrict__ part, int nparts,
-                                                     int k, int want_max, const LimbMap lm, int* sh,
-                                                     double* out, const int* skip) {
+// One wave per column (16 waves): each lane loads its up to 16 partials at once (so the
+// loads are in flight together) and adds them in the order j = lane, lane + 64, ..., then
+// the wave reduces; slots j >= nparts read as 0, which changes neither the maximum of
+// non-negative values nor the sum.
+constexpr int kFinishThreads = 1024;
+__global__ __launch_bounds__(kFinishThreads) void k_finish_cols(const double* __restrict__ part, int nparts,
+                                                                int k, int want_max, const LimbMap lm, int* sh,
+                                                                double* out, const int* skip) {
     if (skip && *skip) return;
     const int lane = threadIdx.x & 63;
-    for (int c = threadIdx.x >> 6; c < k; c += 4) {
-        double v = 0.0;
-        for (int j = lane; j < nparts; j += 64) {
-            const double x = part[c * kMaxParts + j];
-            v = want_max ? fmax(v, x) : v + x;
+    constexpr int kPer = kMaxParts / 64;
+    for (int c = threadIdx.x >> 6; c < k; c += kFinishThreads / 64) {
+        double x[kPer];
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int j = lane + 64 * i;
+            x[i] = j < nparts ? part[c * kMaxParts + j] : 0.0;
         }
+        double v = 0.0;
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) v = want_max ? fmax(v, x[i]) : v + x[i];
         v = want_max ? wave_max(v) : wave_sum(v);
         if (lane == 0) {
             if (out) out[c] = v;
@@ -572,47 +583,54 @@ __global__ __launch_bounds__(256) void k_finish_cols(const double* __restrict__ 
         }
     }
 }
-
-// Quantize v (k columns, K rows valid, Kpad total) into the digit layout of the thin
-// (wide = 0) or wide (wide = 1) kernel and emit per-block partials of
-// sum_kappa w_kappa q_kappa 2^-sh (T for pass 1 with w = 1, C for pass 2 with w = 2 f).
+// Digit matrix of k columns: one thread per genotype kap of column c = blockIdx.y (the
+// reads of v are coalesced). It rounds the value to the column's block fixed point once
+// (q = round(z 2^sh_c)), splits q into its L_c balanced base-256 digits and stores digit d
+// at plan::bq_index of digit column off_c + d; the 16 genotypes of a 16-byte chunk are
+// 16 consecutive threads, so each store instruction writes whole chunks. Genotypes
+// K .. Kpad - 1 get digit 0; the last grid row (blockIdx.y == k) zeroes the unused digit
+// columns J .. J16 - 1. part[c][block]: the block's sum of wts[kap] q 2^-sh_c (wts = 1
+// when null), in a fixed order.
+__device__ inline int balanced_digit(long long& t) {
+    int b = (int)(t & 0xFF);
+    if (b >= 128) b -= 256;
+    t = (t - b) >> 8;
+    return b;
+}
 __global__ __launch_bounds__(256) void k_quantize(const double* __restrict__ v, int64_t ldv,
                                                   int64_t K, int64_t Kpad, int k, const LimbMap lm,
-                                                  int nfrag, int wide, const int* __restrict__ sh,
+                                                  int nfrag, int wide, int J, int J16,
+                                                  const int* __restrict__ sh,
                                                   const double* __restrict__ wts,
                                                   uint8_t* __restrict__ Bq, double* part,
                                                   const int* skip) {
     if (skip && *skip) return;
-    const int j = blockIdx.y;  // digit column
-    const int c = lm.col[j], l = lm.dig[j];
-    const bool live = c < k;
+    const int c = blockIdx.y;
     const int fw = wide ? 32 : 16;
-    const int jf = j / fw, jj = j % fw;
     double acc = 0.0;
     for (int64_t kap = blockIdx.x * 256ll + threadIdx.x; kap < Kpad; kap += (int64_t)gridDim.x * 256) {
-        int dig = 0;
-        if (live && kap < K) {
-            const double z = v[c * ldv + kap];
-            const long long q = llrint(ldexp(z, sh[c]));
-            if (l == 0) acc += (wts ? wts[kap] : 1.0) * ldexp((double)q, -sh[c]);
-            long long t = q;
-            for (int d = 0; d <= l; ++d) {
-                int b = (int)(t & 0xFF);
-                if (b >= 128) b -= 256;
-                dig = b;
-                t = (t - b) >> 8;
-            }
+        if (c == k) {
+            for (int j = J; j < J16; ++j) Bq[plan::bq_index(kap, j % fw, j / fw, nfrag, wide != 0)] = 0;
+            continue;
         }
-        Bq[plan::bq_index(kap, jj, jf, nfrag, wide != 0)] = (uint8_t)(int8_t)dig;
+        const int L = lm.L[c], off = lm.off[c], e = sh[c];
+        long long q = 0;
+        if (kap < K) {
+            q = llrint(ldexp(v[c * ldv + kap], e));
+            acc += (wts ? wts[kap] : 1.0) * ldexp((double)q, -e);
+        }
+        for (int d = 0; d < L; ++d) {
+            const int j = off + d;
+            Bq[plan::bq_index(kap, j % fw, j / fw, nfrag, wide != 0)] = (uint8_t)(int8_t)balanced_digit(q);
+        }
     }
-    if (live && l == 0) {
+    if (c < k) {
         const double r = block_sum(acc);
         if (threadIdx.x == 0) part[c * kMaxParts + blockIdx.x] = r;
     }
 }
-
-// Pseudo-random right-hand sides in (-1, 1) (splitmix64 of the element index) for the
-// tuner's warm-up: live-like digits rather than zeros, whose MFMAs draw less power.
+// blocks per column of k_quantize
+inline int quantize_parts(int64_t Kpad) { return parts_for(Kpad); }
 __global__ __launch_bounds__(256) void k_tune_fill(double* __restrict__ v, int64_t count, uint64_t seed) {
     for (int64_t q = blockIdx.x * 256ll + threadIdx.x; q < count; q += (int64_t)gridDim.x * 256) {
         uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(q + 1);
@@ -950,6 +968,22 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
         }
         const std::string key = key_of(pass, nrows, Kpad);
         auto it = c->tuned.find(key);
+        if (it == c->tuned.end() && wide && pass == 1 && r0 > 0) {
+            // the GRM builder's rows r0 .. n - 1 (r0 a multiple of 512): the block shape
+            // tuned for all n rows, with that shape's candidate split for the shorter
+            // range nearest the tuned one (fewer is taken on a tie: fewer partials to
+            // combine), so each row range is not tuned anew
+            auto full = c->tuned.find(key_of(pass, n, Kpad));
+            if (full != c->tuned.end()) {
+                const int cfg = full->second.cfg, S_full = full->second.parts;
+                int S = 0;
+                for (const Variant& w : candidates(wide, JT, nrows, nss, ncu))
+                    if (w.cfg == cfg && (S == 0 || std::abs(std::log2((double)w.S / S_full)) <
+                                                       std::abs(std::log2((double)S / S_full))))
+                        S = w.S;
+                if (S > 0) it = c->tuned.emplace(key, globe_ctx::Tuned{cfg, S, 0}).first;
+            }
+        }
         if (it == c->tuned.end()) {
             const std::vector<Variant> cand = candidates(wide, JT, nrows, nss, ncu);
             hipEvent_t e0, e1;
@@ -1008,11 +1042,11 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     // pass 1: P1[s][c] = sum_i g_is S0[c][i] on the SNP-major rows
     const int npn = parts_for(n);
     k_colmax_parts<<<dim3(npn, k), 256, 0, s>>>(n, S0, ldn, part, skip);
-    k_finish_cols<<<1, 256, 0, s>>>(part, npn, k, 1, lm, sh1, nullptr, skip);
-    const int qg1 = parts_for(Kpad1);
-    k_quantize<<<dim3(qg1, J16), 256, 0, s>>>(S0, ldn, n, Kpad1, k, lm, nfrag, wide, sh1, nullptr,
-                                              Bq, part, skip);
-    k_finish_cols<<<1, 256, 0, s>>>(part, qg1, k, 0, lm, nullptr, Tc, skip);
+    k_finish_cols<<<1, kFinishThreads, 0, s>>>(part, npn, k, 1, lm, sh1, nullptr, skip);
+    const int qg1 = quantize_parts(Kpad1);
+    k_quantize<<<dim3(qg1, k + 1), 256, 0, s>>>(S0, ldn, n, Kpad1, k, lm, nfrag, wide, J, J16, sh1, nullptr,
+                                               Bq, part, skip);
+    k_finish_cols<<<1, kFinishThreads, 0, s>>>(part, qg1, k, 0, lm, nullptr, Tc, skip);
     HIP_CHECK(hipGetLastError());
     eb1 = std::min<int64_t>((m + kEpiRows - 1) / kEpiRows, kMaxParts);
     auto epi1 = [&](int S, const int* sk) {
@@ -1024,11 +1058,11 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     epi1(contract(0, c->d_snp.as<uint8_t>(), c->pitch_s, m, n, Kpad1, nss1, epi1), skip);
     }
     // pass 2: GZ[c][i] = sum_s g_is u[c][s] - C[c] on the individual-major rows
-    k_finish_cols<<<1, 256, 0, s>>>(part, (int)eb1, k, 1, lm, sh2, nullptr, skip);
-    const int qg2 = parts_for(Kpad2);
-    k_quantize<<<dim3(qg2, J16), 256, 0, s>>>(U, m, m, Kpad2, k, lm, nfrag, wide, sh2,
-                                              c->d_twof.as<double>(), Bq, part, skip);
-    k_finish_cols<<<1, 256, 0, s>>>(part, qg2, k, 0, lm, nullptr, Cc, skip);
+    k_finish_cols<<<1, kFinishThreads, 0, s>>>(part, (int)eb1, k, 1, lm, sh2, nullptr, skip);
+    const int qg2 = quantize_parts(Kpad2);
+    k_quantize<<<dim3(qg2, k + 1), 256, 0, s>>>(U, m, m, Kpad2, k, lm, nfrag, wide, J, J16, sh2,
+                                               c->d_twof.as<double>(), Bq, part, skip);
+    k_finish_cols<<<1, kFinishThreads, 0, s>>>(part, qg2, k, 0, lm, nullptr, Cc, skip);
     HIP_CHECK(hipGetLastError());
     const int64_t eb2 = std::min<int64_t>((n2 + kEpiRows - 1) / kEpiRows, kMaxParts);
     auto epi2 = [&](int S, const int* sk) {
@@ -1076,10 +1110,13 @@ void pretune_geno_product(globe_ctx* c, int max_cols, int precision, const doubl
 // the direct row extraction k_unit_u, pass 2 the contraction on kBuilderLimbs digits.
 // u = W^T e_j / M has no large common offset (its entries inv_s^2 (g_sj - 2 f_s) / M
 // change sign with g_sj), so G_ij = sum_s g_is u_s - C is not a difference of large
-// terms, and 5 digits (u to 2^-38 of its column maximum) leave each entry about 1e-10
-// of itself from the FP64 value, far below its FP32 rounding.
+// terms. 4 digits hold u to 2^-30 of its column maximum (|u| <= 2 inv_s^2 / M, at most
+// 101 / M at MAF 0.01): the M rounding errors of an entry add as a random walk, about
+// 1e-10, and even all aligned (2 M 2^-31 max|u| <= 1e-7) stay at the entry's FP32
+// rounding. 32 columns of 4 digits fill one 128-column contraction, and 4-digit columns
+// take the epilogue's vector path.
 void geno_unit_columns(globe_ctx* c, int64_t j0, int k, double* GZ, int64_t row0) {
-    constexpr int kBuilderLimbs = 5;
+    constexpr int kBuilderLimbs = 4;
     hipStream_t s = c->stream;
     const int64_t m = c->m;
     c->w_P1.ensure(sizeof(double) * (size_t)(m > 0 ? m : 1) * k);

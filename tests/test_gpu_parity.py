@@ -365,7 +365,7 @@ def test_grm_few_column_product_multi_tile(k, gpu_ctx_factory):
 
 def test_grm_builder_at_40k_rows_offsets(gpu_ctx_factory):
     """The builder forms only rows i >= j of each tile column, from a row offset (a
-    multiple of 4096) on; fewer rows let the contraction take more K-splits than the
+    multiple of 512) on; fewer rows let the contraction take more K-splits than the
     full cohort, and its partial-sum buffer must be sized for them (N = 40,000 and
     M = 80,000 SNPs, the AI-REML benchmark's shape, is the first size where they
     differ). The built GRM's products and diagonal match the genotype path."""

@@ -8,7 +8,7 @@ layout, digit access outside the digit matrix, partial-sum store or epilogue rea
 outside the split-partial buffer that the library allocates, or partial slot left
 unwritten. The shapes are those the benchmarks and the GRM builder run:
 N = 200k / M = 500k (the scale workload), N = 40k / M = 80k with the builder's
-4096-multiple row offsets (the AI-REML cohort), N = 200k from 20k SNPs (the --grm
+512-multiple row offsets (the AI-REML cohort), N = 200k from 20k SNPs (the --grm
 path's builder), plus small and ragged shapes of the tests.
 """
 import pytest
@@ -50,17 +50,23 @@ def test_replay_scale_workload_all_candidates():
     _replay_all(499939, 200000)
 
 
+def _builder_offsets(n, stride):
+    """Row offsets r0 of the builder's launches: multiples of 512 (every stride-th one,
+    and all of the last 16, where the fewest rows admit the most splits)."""
+    last = n // 512 * 512 if n % 512 else n - 512
+    return tuple(sorted(set(range(0, n, stride)) | set(range(max(0, last - 15 * 512), last + 1, 512))))
+
+
 def test_replay_aireml_cohort_builder_offsets():
-    """N = 40k, M = 80k: the GRM builder contracts rows r0 .. n - 1 from every multiple of
-    4096 (fewer rows admit more splits: the shape of the one fault of the first builder)."""
-    r0s = tuple(range(0, 40000, 4096))
-    _replay_all(79990, 40000, r0s=r0s, js=(125, 70))
+    """N = 40k, M = 80k: the GRM builder contracts rows r0 .. n - 1 from multiples of 512
+    below each tile column (fewer rows admit more splits: the shape of the one fault of the
+    first builder), 32 columns of 4 digits (and 32 digit columns for an 8-column rest)."""
+    _replay_all(79990, 40000, r0s=_builder_offsets(40000, 4096), js=(128, 32))
 
 
 def test_replay_grm_path_builder():
-    """N = 200k from 20k SNPs (the --grm path at scale): builder offsets up to 196,608."""
-    r0s = tuple(range(0, 200000, 4096 * 6)) + (196608,)
-    _replay_all(20000, 200000, r0s=r0s, js=(125, 70))
+    """N = 200k from 20k SNPs (the --grm path at scale): builder offsets up to 199,168."""
+    _replay_all(20000, 200000, r0s=_builder_offsets(200000, 4096 * 6), js=(128, 32))
 
 
 @pytest.mark.parametrize("m,n", [(1, 1), (3, 5), (900, 301), (2000, 400), (513, 1025), (40000, 20000),

@@ -77,12 +77,15 @@ def test_every_candidate_bit_identical_and_vs_valu(gpu_ctx_factory):
 
 
 def test_grm_builder_every_candidate(gpu_ctx_factory):
-    """The builder's pass 2 runs on rows r0 .. n - 1 from multiples of 4096 (5 digits, 125
-    and 70 digit columns); every forced candidate gives the same GRM bit for bit."""
+    """The builder's pass 2 runs on rows r0 .. n - 1 from multiples of 512 (4 digits: 128
+    digit columns, and 32 for the last 8 columns of 9000); every forced candidate gives the
+    same GRM bit for bit (a forced index wraps modulo the shape's candidate count, so the
+    indices up to the largest count cover every candidate of every launch shape)."""
     n, m = 9000, 6000
     _say("builder variants", n, m)
     ref = None
-    for i in [-1] + list(range(12)):
+    most = max(G.plan_replay(128, m, n, r0)["cand2"] for r0 in range(0, n, 512))
+    for i in [-1] + list(range(most)):
         g, c, _ = _cohort_ctx(gpu_ctx_factory, n, m, 1.0, 11)
         g.force_variant(i)
         g.grm_from_geno()
