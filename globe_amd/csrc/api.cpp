@@ -564,14 +564,15 @@ int globe_set_cohort(globe_ctx* c, int64_t nrec, int32_t n, int32_t p, const int
     HIP_CHECK(hipMemcpy(c->d_X.p, c->h_X.data(), sizeof(double) * nrec * (p + 1), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(c->d_blk_off.p, off.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(c->d_blk_ind.p, ind.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
-    // record tiles: greedy runs of whole blocks of at most kTileRecords records
+    // record tiles: greedy runs of whole blocks, at most kTileRecords records and
+    // kTileBlocks blocks each
     {
         std::vector<int32_t> tile_blk{0}, rec_blk((size_t)nrec);
         int64_t maxblk = 0, in_tile = 0;
         for (int32_t b = 0; b < n; ++b) {
             const int64_t len = off[b + 1] - off[b];
             maxblk = std::max(maxblk, len);
-            if (in_tile > 0 && in_tile + len > kTileRecords) {
+            if (in_tile > 0 && (in_tile + len > kTileRecords || b - tile_blk.back() == kTileBlocks)) {
                 tile_blk.push_back(b);
                 in_tile = 0;
             }

@@ -104,7 +104,13 @@ struct RecMeta {
     int32_t cap = 0;                      // records per tile; 0: thread-per-block kernels
 };
 constexpr int kTileRecords = 2048;
-constexpr int kTileMaxRecords = 2048;  // LDS: 4 doubles per record slot (64 KB)
+constexpr int kTileMaxRecords = 2048;  // LDS: 2 doubles per record slot (32 KB)
+// individuals (blocks) per tile: the preconditioner keeps 2 doubles per block in LDS
+// (8 KB), so its footprint is 40 KB and 4 workgroups fit a CU
+constexpr int kTileBlocks = 512;
+// Tiles end at kTileRecords records or kTileBlocks blocks, so two consecutive tiles hold
+// at least kTileBlocks records: a column of R records has at most this many tiles.
+inline int64_t max_tiles(int64_t records) { return 2 * records / kTileBlocks + 1; }
 // partial-sum slots written by launch_precond
 int precond_parts(const RecMeta& m);
 void launch_seg_sums(hipStream_t s, const RecMeta& m, const double* x, int64_t ldx, int k,

@@ -70,13 +70,24 @@ __global__ __launch_bounds__(kBlock) void k_seg_sums(int32_t nblk, const int64_t
 // PCG per PCGsolver.cpp:1343-1443 / varCompEst.cpp:482-527 with every scalar on
 // the device; once `done` is set every later kernel of the solve is a no-op, so the
 // host may enqueue iterations ahead of the convergence test.
-// sums[c] = sum_j part[c][j] for c < k: one wave per column, lanes over partials,
-// then a fixed shuffle tree (deterministic). Ends with a barrier.
+// sums[c] = sum_j part[c][j] for c < k: one wave per column; each lane loads its up to
+// kMaxParts / 64 partials at once (so the loads are in flight together), adds them in
+// the order j = lane, lane + 64, ... (slots j >= nparts read as 0), then a fixed
+// shuffle tree (deterministic). Ends with a barrier.
+constexpr int kPcgThreads = 1024;  // one-block PCG kernels: 16 waves, so k <= 64 columns take <= 4 rounds
 __device__ void wave_col_sums(const double* __restrict__ part, int nparts, int k, double* sums) {
     const int lane = threadIdx.x & 63;
+    constexpr int kPer = kMaxParts / 64;
     for (int c = threadIdx.x >> 6; c < k; c += blockDim.x >> 6) {
+        double x[kPer];
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int j = lane + 64 * i;
+            x[i] = j < nparts ? part[c * kMaxParts + j] : 0.0;
+        }
         double v = 0.0;
-        for (int j = lane; j < nparts; j += 64) v += part[c * kMaxParts + j];
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) v += x[i];
         v = wave_sum(v);
         if (lane == 0) sums[c] = v;
     }
@@ -277,8 +288,8 @@ __global__ __launch_bounds__(kBlock) void k_precond_tiled(
     extern __shared__ double sm[];
     double* sr = sm;               // r of the tile's records
     double* st = sm + m.cap;       // t of the tile's records
-    double* sc0 = sm + 2 * m.cap;  // per block: Woodbury coefficients
-    double* sc1 = sm + 3 * m.cap;
+    double* sc0 = sm + 2 * m.cap;  // per block (at most kTileBlocks): Woodbury coefficients
+    double* sc1 = sc0 + kTileBlocks;
     const double se = th[c * 5 + 0], sg = th[c * 5 + 1], sb0 = th[c * 5 + 2],
                  sb01 = th[c * 5 + 3], sb1 = th[c * 5 + 4];
     const double* rc = r + col_start(m, c, ldr);
@@ -503,7 +514,7 @@ __global__ __launch_bounds__(kBlock) void k_axpy_cols(int64_t nrec, double* __re
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_pcg_init(PcgState* st, const double* __restrict__ part_bb,
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgState* st, const double* __restrict__ part_bb,
                                                      const double* __restrict__ part_rr,
                                                      const double* __restrict__ part_rz, int nparts,
                                                      int k) {
@@ -597,19 +608,19 @@ __global__ void k_set_column(int j, int64_t off, int32_t ntiles, int64_t nrec, d
 
 // One-block reductions of PCG's dot partials: alpha after V p, the convergence test
 // after the x/r update, beta after the preconditioner.
-__global__ __launch_bounds__(kBlock) void k_pcg_alpha(PcgState* st, const double* __restrict__ part_pvp,
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_alpha(PcgState* st, const double* __restrict__ part_pvp,
                                                       int nparts, int k) {
     if (st->done) return;
     pcg_alpha_body(st, part_pvp, nparts, k);
 }
 
-__global__ __launch_bounds__(kBlock) void k_pcg_check(PcgState* st, const double* __restrict__ part_rr,
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_check(PcgState* st, const double* __restrict__ part_rr,
                                                       int nparts, int k) {
     if (st->done) return;
     pcg_check_body(st, part_rr, nparts, k);
 }
 
-__global__ __launch_bounds__(kBlock) void k_pcg_beta(PcgState* st, const double* __restrict__ part_rz,
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_beta(PcgState* st, const double* __restrict__ part_rz,
                                                      int nparts, int k) {
     if (st->done) return;
     pcg_beta_body(st, part_rz, nparts, k);
@@ -787,7 +798,7 @@ void launch_precond(hipStream_t s, const RecMeta& m, const double* gdiag, const 
                     double* part_rr, const PcgState* st, int tilde_v) {
     if (m.cap > 0) {
         dim3 g(precond_parts(m), k);
-        k_precond_tiled<<<g, kBlock, sizeof(double) * 4 * m.cap, s>>>(m, gdiag, theta_cols, r, ldr, z,
+        k_precond_tiled<<<g, kBlock, sizeof(double) * (2 * m.cap + 2 * kTileBlocks), s>>>(m, gdiag, theta_cols, r, ldr, z,
                                                                        ldz, part_rz, part_rr, st, tilde_v);
     } else {
         dim3 g(parts_for(m.nblk), k);
@@ -865,23 +876,23 @@ void launch_pcg_admit(hipStream_t s, PcgState* st, int j, const double* part_bb,
 
 void launch_pcg_init(hipStream_t s, PcgState* st, const double* part_bb, const double* part_rr,
                      const double* part_rz, int nparts, int k) {
-    k_pcg_init<<<1, kBlock, 0, s>>>(st, part_bb, part_rr, part_rz, nparts, k);
+    k_pcg_init<<<1, kPcgThreads, 0, s>>>(st, part_bb, part_rr, part_rz, nparts, k);
     LAUNCH_CHECK();
 }
 
 
 void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* part_pvp, int nparts, int k) {
-    k_pcg_alpha<<<1, kBlock, 0, s>>>(st, part_pvp, nparts, k);
+    k_pcg_alpha<<<1, kPcgThreads, 0, s>>>(st, part_pvp, nparts, k);
     LAUNCH_CHECK();
 }
 
 void launch_pcg_check(hipStream_t s, PcgState* st, const double* part_rr, int nparts, int k) {
-    k_pcg_check<<<1, kBlock, 0, s>>>(st, part_rr, nparts, k);
+    k_pcg_check<<<1, kPcgThreads, 0, s>>>(st, part_rr, nparts, k);
     LAUNCH_CHECK();
 }
 
 void launch_pcg_beta(hipStream_t s, PcgState* st, const double* part_rz, int nparts, int k) {
-    k_pcg_beta<<<1, kBlock, 0, s>>>(st, part_rz, nparts, k);
+    k_pcg_beta<<<1, kPcgThreads, 0, s>>>(st, part_rz, nparts, k);
     LAUNCH_CHECK();
 }
 

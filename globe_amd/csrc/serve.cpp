@@ -88,9 +88,8 @@ struct ServeState {
         m.tile_blk = tile_blk.as<int32_t>();
         m.col_off = col_off.as<int64_t>() + j0;
         m.ntiles_col = ntiles_col.as<int32_t>() + j0;
-        // two consecutive tiles hold more than kTileRecords records, so a column has
-        // at most 2 cap / kTileRecords + 1 tiles (the launch grid covers that many)
-        m.ntiles = (int32_t)(2 * cap / globe::kTileRecords + 1);
+        // the launch grid covers the most tiles a column of cap records can have
+        m.ntiles = (int32_t)globe::max_tiles(cap);
         m.nrec_col = nrec_col.as<int64_t>() + j0;
         m.nrec = cap;
         m.cap = globe::kTileRecords;
@@ -179,7 +178,7 @@ void prepare(globe_ctx* c, const ServeState& S, const globe_request* q, ServeSta
             "individual with too many records");
     for (int32_t q2 = 0; q2 < r.nblk; ++q2) {
         const int64_t len = r.blk_off[q2 + 1] - r.blk_off[q2];
-        if (in_tile > 0 && in_tile + len > kTileRecords) {
+        if (in_tile > 0 && (in_tile + len > kTileRecords || q2 - r.tile_blk.back() == kTileBlocks)) {
             r.tile_blk.push_back(q2);
             in_tile = 0;
         }
