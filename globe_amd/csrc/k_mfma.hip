@@ -660,37 +660,14 @@ __global__ __launch_bounds__(256) void k_unit_u(const uint8_t* __restrict__ ind,
 // Epilogues: a block takes 64 rows x k columns. Phase 1 reads the digit sums with
 // consecutive threads on consecutive digits (coalesced), phase 2 writes the FP64
 // results column-major with consecutive threads on consecutive rows.
-constexpr int kEpiRows = 64;
+// Epilogue tiles of kEpiRows rows. The int32 partial sums of the S splits for a tile's
+// rows are one contiguous run of kEpiRows J16 words per split: each thread adds 16
+// consecutive words over the splits in int64 (exact, so the order does not matter) with
+// four 16-byte loads per split, the sums go to LDS, and one thread per (row, column)
+// combines the column's L digit sums in FP64 in digit order, v += s_l 2^(8 l).
+constexpr int kEpiRows = 32;
+constexpr int kEpiWords = 16;  // partial words per thread and split
 
-__device__ inline double combine(const int32_t* __restrict__ out, int nsplit, int64_t split_stride,
-                                 int64_t base, int limbs) {
-    if (limbs == 4 && (base & 3) == 0) {
-        // the 4 digit sums of a column are 16-byte aligned (J16 and the split stride are
-        // multiples of 16): one vector load per split, same combination order
-        long long s[4] = {0, 0, 0, 0};
-        for (int sp = 0; sp < nsplit; ++sp) {
-            const int4 d = *reinterpret_cast<const int4*>(out + sp * split_stride + base);
-            s[0] += d.x;
-            s[1] += d.y;
-            s[2] += d.z;
-            s[3] += d.w;
-        }
-        double v = 0.0;
-#pragma unroll
-        for (int l = 0; l < 4; ++l) v += ldexp((double)s[l], 8 * l);
-        return v;
-    }
-    double v = 0.0;
-    for (int l = 0; l < limbs; ++l) {
-        long long s = 0;
-        for (int sp = 0; sp < nsplit; ++sp) s += out[sp * split_stride + base + l];
-        v += ldexp((double)s, 8 * l);
-    }
-    return v;
-}
-
-// pass 1 (mode 0): u = sig2g / M * inv^2 * (P - 2 f T), per-block colmax of u
-// pass 2 (mode 1): GZ = P - C
 __global__ __launch_bounds__(256) void k_epilogue(const int32_t* __restrict__ out, int nsplit,
                                                   int64_t rows, int J16, const LimbMap lm, int k,
                                                   const int* __restrict__ sh, int mode,
@@ -701,21 +678,46 @@ __global__ __launch_bounds__(256) void k_epilogue(const int32_t* __restrict__ ou
                                                   double* __restrict__ res, int64_t ldr,
                                                   double* part, const int* skip) {
     if (skip && *skip) return;
+    __shared__ long long dsum[kEpiRows * 128];
     __shared__ double tile[kEpiRows][kMaxCols + 1];
     __shared__ double cmax[kMaxCols];
     const int64_t split_stride = rows * J16;
     const int64_t ntiles = (rows + kEpiRows - 1) / kEpiRows;
+    // J16 is a multiple of 16 (16 or 32 JT), so 16 words never straddle a row
+    const int nthr = kEpiRows * J16 / kEpiWords;
     for (int c = threadIdx.x; c < k; c += 256) cmax[c] = 0.0;
     for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int64_t r0 = t * kEpiRows;
+        __syncthreads();
+        if (threadIdx.x < nthr) {
+            const int w0 = threadIdx.x * kEpiWords;
+            long long sm[kEpiWords];
+#pragma unroll
+            for (int i = 0; i < kEpiWords; ++i) sm[i] = 0;
+            if (r0 + w0 / J16 < rows) {
+                const int32_t* src = out + r0 * J16 + w0;
+                for (int sp = 0; sp < nsplit; ++sp) {
+                    const int4* q = reinterpret_cast<const int4*>(src + sp * split_stride);
+                    const int4 a = q[0], b = q[1], c4 = q[2], d = q[3];
+                    sm[0] += a.x; sm[1] += a.y; sm[2] += a.z; sm[3] += a.w;
+                    sm[4] += b.x; sm[5] += b.y; sm[6] += b.z; sm[7] += b.w;
+                    sm[8] += c4.x; sm[9] += c4.y; sm[10] += c4.z; sm[11] += c4.w;
+                    sm[12] += d.x; sm[13] += d.y; sm[14] += d.z; sm[15] += d.w;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < kEpiWords; ++i) dsum[w0 + i] = sm[i];
+        }
         __syncthreads();
         for (int q = threadIdx.x; q < kEpiRows * k; q += 256) {
             const int rl = q / k, c = q % k;
             const int64_t r = r0 + rl;
             double val = 0.0;
             if (r < rows) {
-                const double P =
-                    ldexp(combine(out, nsplit, split_stride, r * J16 + lm.off[c], lm.L[c]), -sh[c]);
+                const long long* ds = dsum + rl * J16 + lm.off[c];
+                double v = 0.0;
+                for (int l = 0; l < lm.L[c]; ++l) v += ldexp((double)ds[l], 8 * l);
+                const double P = ldexp(v, -sh[c]);
                 if (mode == 0) {
                     const double iv = inv[r];
                     val = th[c * 5 + 1] * inv_m * iv * iv * (P - twof[r] * TC[c]);
@@ -736,7 +738,7 @@ __global__ __launch_bounds__(256) void k_epilogue(const int32_t* __restrict__ ou
             for (int c = threadIdx.x >> 6; c < k; c += 4) {
                 double m = 0.0;
                 const int rl = threadIdx.x & 63;
-                if (r0 + rl < rows) m = fabs(tile[rl][c]);
+                if (rl < kEpiRows && r0 + rl < rows) m = fabs(tile[rl][c]);
                 m = wave_max(m);
                 if ((threadIdx.x & 63) == 0) cmax[c] = fmax(cmax[c], m);
             }
