@@ -514,7 +514,15 @@ void globe_destroy(globe_ctx* c) {
     delete c;
 }
 
-const char* globe_last_error(const globe_ctx* c) { return c ? c->err.c_str() : "null context"; }
+// The message is copied under the context's error lock into a per-thread buffer, so a
+// submit failing on an intake thread cannot free the string another thread is reading.
+const char* globe_last_error(const globe_ctx* c) {
+    if (!c) return "null context";
+    thread_local std::string msg;
+    std::lock_guard<std::mutex> lk(const_cast<globe_ctx*>(c)->err_mu);
+    msg = c->err;
+    return msg.c_str();
+}
 
 int globe_set_cohort(globe_ctx* c, int64_t nrec, int32_t n, int32_t p, const int32_t* indiv,
                      const double* times, const double* X, int64_t ldx) {

@@ -85,6 +85,8 @@ typedef struct {
 /* No reference counterpart (the reference is one host process without a device). */
 int globe_create(int device, globe_ctx** out);
 void globe_destroy(globe_ctx* ctx);
+/* The last failure's message on this context, valid on the calling thread until its next
+ * globe_last_error call. */
 const char* globe_last_error(const globe_ctx* ctx);
 int globe_abi_version(void);
 int globe_device_count(void);

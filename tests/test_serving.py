@@ -208,3 +208,30 @@ def test_serving_pool_admits_by_request_size(cohort, gpu_ctx_factory):
     assert order.index(ids[2]) > order.index(ids[1])  # the large one waited for room
     for q, rid in enumerate(ids):
         assert rel(out[rid], alone[q]) <= 1e-14, q
+
+
+def test_serving_submissions_from_an_intake_thread(cohort, gpu_ctx_factory):
+    """globe_serve_submit runs on an intake thread (serving.replay: Poisson arrivals
+    submitted from a second host thread while this one steps the loop): every request
+    comes back once, converged, equal to the same request served alone to 1e-14, and a
+    request the loop rejects fails on the intake thread with its own message."""
+    _say("serve-intake-thread")
+    d = cohort
+    reqs = _requests(d.cohort.n, 24, seed=13)
+    cap = max(r[0].shape[0] for r in reqs)
+    g = _device(d, gpu_ctx_factory)
+    g.serve_open(slots=6, max_records=cap, tol=1e-10, max_iter=500, precision=G.PREC_FP64)
+    sols, st = serving.replay(g, reqs, rate=2000.0, seed=1)
+    g.serve_close()
+    assert all(x is not None for x in sols)
+    assert st["iters"].min() >= 1 and st["steps"] >= 1
+    for q in (0, 7, 23):
+        alone, info = _serve(g, [reqs[q]], slots=6, precision=G.PREC_FP64, tol=1e-10, stagger=False, cap=cap)
+        assert info[0]["converged"]
+        assert rel(sols[q], alone[0]) <= 1e-14, q
+    # an invalid request raised on the intake thread reaches the stepping thread
+    g.serve_open(slots=2, max_records=cap, tol=1e-8)
+    bad = (np.array([0, 1, 0], np.int32), np.zeros(3), np.ones(3), (0.5, 0.4, 0.3, 0.0, 0.2))
+    with pytest.raises(G.GlobeError, match="contiguous"):
+        serving.replay(g, [reqs[0], bad], rate=2000.0, seed=2)
+    g.serve_close()
