@@ -38,6 +38,30 @@ __device__ inline double block_sum(double v) {
 // when the columns are pooled, else c ld.
 __device__ inline int64_t col_start(const RecMeta& m, int c, int64_t ld) { return m.col_off ? m.col_off[c] : c * ld; }
 
+// A tile's records (at most kTileRecords = kTileIters x kBlock) go to LDS: each thread
+// issues all of its loads before its LDS stores, so the tile costs one memory round trip
+// rather than one per record a thread copies.
+constexpr int kTileIters = kTileRecords / kBlock;
+static_assert(kTileIters * kBlock == kTileRecords, "tile records spread evenly over the block");
+__device__ inline void stage_tile(const double* __restrict__ a, const double* __restrict__ b, int nr,
+                                  double* __restrict__ sa, double* __restrict__ sb) {
+    double va[kTileIters], vb[kTileIters];
+#pragma unroll
+    for (int i = 0; i < kTileIters; ++i) {
+        const int q = threadIdx.x + kBlock * i;
+        va[i] = q < nr ? a[q] : 0.0;
+        vb[i] = q < nr ? b[q] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < kTileIters; ++i) {
+        const int q = threadIdx.x + kBlock * i;
+        if (q < nr) {
+            sa[q] = va[i];
+            sb[q] = vb[i];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // S0[c][i] = sum_{r in block} x[c][r], S1[c][i] = sum t_r x[c][r]
 // (PCGsolver.cpp:1119-1125 scatter, and the t-weighted sum the factorized time
@@ -210,16 +234,30 @@ __global__ __launch_bounds__(kBlock) void k_record_out(
     const double* s0 = S0 + c * ldn;
     const double* s1 = S1 + c * ldn;
     double acc = 0.0;
-    for (int64_t r = blockIdx.x * (int64_t)kBlock + threadIdx.x; r < nloop;
-         r += (int64_t)gridDim.x * kBlock) {
-        double v = 0.0;  // records past a column's own count are padding
-        if (r < nvalid) {
-            const int32_t i = indiv[r];
-            const double tr = t[r], a = s0[i], b = s1[i], xv = xc[r];
-            v = g[i] + sb0 * a + sb01 * (tr * a + b) + sb1 * tr * b + se * xv;
-            acc += xv * v;
+    // the thread's records r0, r0 + stride, ... in groups of 4: the individual indices of
+    // a group are loaded before its gathers (the same records in the same order)
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    constexpr int kU = 4;
+    for (int64_t r0 = blockIdx.x * (int64_t)kBlock + threadIdx.x; r0 < nloop; r0 += kU * stride) {
+        int32_t ii[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int64_t r = r0 + u * stride;
+            ii[u] = r < nvalid ? indiv[r] : 0;
         }
-        yc[r] = v;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int64_t r = r0 + u * stride;
+            if (r >= nloop) break;
+            double v = 0.0;  // records past a column's own count are padding
+            if (r < nvalid) {
+                const int32_t i = ii[u];
+                const double tr = t[r], a = s0[i], b = s1[i], xv = xc[r];
+                v = g[i] + sb0 * a + sb01 * (tr * a + b) + sb1 * tr * b + se * xv;
+                acc += xv * v;
+            }
+            yc[r] = v;
+        }
     }
     if (part_dot) {
         double s = block_sum(acc);
@@ -251,10 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_sums_tiled(RecMeta m, const doub
         const int64_t r0 = off[b0];
         const int nr = (int)(off[b1] - r0);
         __syncthreads();
-        for (int q = threadIdx.x; q < nr; q += kBlock) {
-            sx[q] = xc[r0 + q];
-            st[q] = t[r0 + q];
-        }
+        stage_tile(xc + r0, t + r0, nr, sx, st);
         __syncthreads();
         for (int32_t b = b0 + threadIdx.x; b < b1; b += kBlock) {
             const int q0 = (int)(off[b] - r0), q1 = (int)(off[b + 1] - r0);
@@ -306,11 +341,15 @@ __global__ __launch_bounds__(kBlock) void k_precond_tiled(
         const int32_t b0 = tile_blk[tile], b1 = tile_blk[tile + 1];
         const int64_t r0 = off[b0];
         const int nr = (int)(off[b1] - r0);
-        __syncthreads();
-        for (int q = threadIdx.x; q < nr; q += kBlock) {
-            sr[q] = rc[r0 + q];
-            st[q] = t[r0 + q];
+        // the blocks of this thread's records, loaded with the tile (used after the barrier)
+        int32_t rb[kTileIters];
+#pragma unroll
+        for (int i = 0; i < kTileIters; ++i) {
+            const int q = threadIdx.x + kBlock * i;
+            rb[i] = q < nr ? rec_blk[r0 + q] : 0;
         }
+        __syncthreads();
+        stage_tile(rc + r0, t + r0, nr, sr, st);
         __syncthreads();
         for (int32_t b = b0 + threadIdx.x; b < b1; b += kBlock) {
             const int q0 = (int)(off[b] - r0), q1 = (int)(off[b + 1] - r0);
@@ -336,8 +375,11 @@ __global__ __launch_bounds__(kBlock) void k_precond_tiled(
             sc1[b - b0] = (m00 * v1 - m10 * v0) / det;
         }
         __syncthreads();
-        for (int q = threadIdx.x; q < nr; q += kBlock) {
-            const int bl = rec_blk[r0 + q] - b0;
+#pragma unroll
+        for (int i = 0; i < kTileIters; ++i) {
+            const int q = threadIdx.x + kBlock * i;
+            if (q >= nr) break;
+            const int bl = rb[i] - b0;
             const double rv = sr[q];
             const double zv = (rv - sc0[bl] - sc1[bl] * st[q]) * inv_se;
             zc[r0 + q] = zv;
