@@ -47,6 +47,9 @@
 // K-chunk of Bq in that XCD's L2. Partial sums per split go to out[split][row][J16]
 // and are added exactly (int64) in the epilogue.
 #include <algorithm>
+#include <tuple>
+#include <mutex>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -869,10 +872,39 @@ void launch_variant(hipStream_t s, bool wide, int JT, const Variant& v, const ui
 // times every candidate variant of plan::candidates once on the live data with HIP
 // events and keeps the fastest; every variant computes the same exact integer sums, so
 // the choice never changes a result.
+// The candidate lists and a product's geometry are pure functions of the shape (and of the
+// compiled kernels' occupancy, fixed for the process). The thin lists walk every row block
+// for each grid, about 0.7 ms of host time per product at N = 200k, so both are computed
+// once per shape and kept for the process.
+static std::mutex g_plan_mu;
 static std::vector<Variant> candidates(bool wide, int JT, int64_t nrows, int64_t nss, int ncu) {
+    static std::map<std::tuple<bool, int, int64_t, int64_t, int>, std::vector<Variant>> memo;
+    const auto key = std::make_tuple(wide, wide ? JT : 0, nrows, nss, ncu);
+    {
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        auto it = memo.find(key);
+        if (it != memo.end()) return it->second;
+    }
     int bpc[plan::kNumThin];
     for (int q = 0; q < plan::kNumThin; ++q) bpc[q] = wide ? 1 : thin_occupancy(q);
-    return plan::candidates(wide, JT, nrows, nss, ncu, bpc);
+    std::vector<Variant> v = plan::candidates(wide, JT, nrows, nss, ncu, bpc);
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    return memo.emplace(key, std::move(v)).first->second;
+}
+static plan::Geometry product_geometry(int J, int64_t m, int64_t n, int64_t pitch_s, int64_t pitch_i, int64_t r0,
+                                       int ncu) {
+    static std::map<std::tuple<int, int64_t, int64_t, int64_t, int64_t, int64_t, int>, plan::Geometry> memo;
+    const auto key = std::make_tuple(J, m, n, pitch_s, pitch_i, r0, ncu);
+    {
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        auto it = memo.find(key);
+        if (it != memo.end()) return it->second;
+    }
+    int bpc[plan::kNumThin];
+    for (int q = 0; q < plan::kNumThin; ++q) bpc[q] = thin_occupancy(q);
+    const plan::Geometry geo = plan::contract_geometry(J, m, n, pitch_s, pitch_i, r0, ncu, bpc);
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    return memo.emplace(key, geo).first->second;
 }
 
 // GZ (n x k) = sig2g G S0 for the packed-genotype cohort, column c of S0 on col_limbs[c]
@@ -907,9 +939,7 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     // pass 2 forms rows r0 .. n - 1 (r0 > 0 only with U_given); fewer rows can take more
     // splits, so the partial buffer is sized for the row count actually contracted
     const int64_t r0 = U_given ? row0 : 0, n2 = n - r0;
-    int bpc[plan::kNumThin];
-    for (int q = 0; q < plan::kNumThin; ++q) bpc[q] = thin_occupancy(q);
-    const plan::Geometry geo = plan::contract_geometry(J, m, n, c->pitch_s, c->pitch_i, r0, ncu, bpc);
+    const plan::Geometry geo = product_geometry(J, m, n, c->pitch_s, c->pitch_i, r0, ncu);
     const bool wide = geo.wide;
     const int JT = geo.JT, J16 = geo.J16, nfrag = geo.nfrag;
     const int64_t Kpad1 = geo.pass[0].Kpad, Kpad2 = geo.pass[1].Kpad;
