@@ -359,14 +359,8 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
     // partial slot counts: dots over records use np, precond uses npb
     // (k_pcg_init reads up to max(np, npb) slots; unused slots are zero-filled below)
     const int nmax = std::max(np, npb);
-    if (npb < nmax) {
-        for (int j = 0; j < k; ++j) {
-            launch_fill(s, part_b + (size_t)j * kMaxParts + npb, nmax - npb, 0.0);
-            launch_fill(s, part_c + (size_t)j * kMaxParts + npb, nmax - npb, 0.0);
-        }
-    }
-    if (np < nmax)
-        for (int j = 0; j < k; ++j) launch_fill(s, part_a + (size_t)j * kMaxParts + np, nmax - np, 0.0);
+    launch_zero_slots(s, part_b, part_c, nullptr, k, npb, nmax);
+    launch_zero_slots(s, part_a, nullptr, nullptr, k, np, nmax);
     launch_pcg_init(s, dst, part_a, part_b, part_c, nmax, k);
     const int* skip = &dst->done;
     // Iterations are enqueued in batches that double while an iteration is short

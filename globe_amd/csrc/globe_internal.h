@@ -161,6 +161,8 @@ void launch_f_theta(hipStream_t s, int32_t nblk, const int64_t* blk_off, const i
 void launch_copy_cols(hipStream_t s, int64_t rows, const double* src, int64_t lds, double* dst,
                       int64_t ldd, int k);
 void launch_fill(hipStream_t s, double* dst, int64_t count, double v);
+// zero partial slots [from, to) of k columns of up to three partial arrays (null: skipped)
+void launch_zero_slots(hipStream_t s, double* a, double* b, double* c, int k, int from, int to);
 void launch_copy16(hipStream_t s, void* dst, const void* src, int64_t bytes);
 void launch_sub_cols(hipStream_t s, int64_t rows, const double* a, int64_t lda, const double* b,
                      int64_t ldb, double* d, int64_t ldd, int k);

@@ -805,6 +805,17 @@ __global__ __launch_bounds__(kBlock) void k_fill(double* dst, int64_t count, dou
         dst[q] = v;
 }
 
+// Zero the partial slots [from, to) of every column of up to three partial arrays
+// (kMaxParts slots per column; null arrays are skipped): one launch for all of them.
+__global__ __launch_bounds__(kBlock) void k_zero_slots(double* a, double* b, double* c, int k, int from, int to) {
+    double* arr = blockIdx.y == 0 ? a : blockIdx.y == 1 ? b : c;
+    if (!arr) return;
+    for (int q = threadIdx.x; q < k * (to - from); q += kBlock) {
+        const int j = q / (to - from), slot = from + q % (to - from);
+        arr[j * kMaxParts + slot] = 0.0;
+    }
+}
+
 }  // namespace
 
 #define LAUNCH_CHECK() HIP_CHECK(hipGetLastError())
@@ -998,6 +1009,12 @@ void launch_copy16(hipStream_t s, void* dst, const void* src, int64_t bytes) {
 
 void launch_fill(hipStream_t s, double* dst, int64_t count, double v) {
     k_fill<<<parts_for(count), kBlock, 0, s>>>(dst, count, v);
+    LAUNCH_CHECK();
+}
+
+void launch_zero_slots(hipStream_t s, double* a, double* b, double* c, int k, int from, int to) {
+    if (to <= from || k <= 0) return;
+    k_zero_slots<<<dim3(1, 3), kBlock, 0, s>>>(a, b, c, k, from, to);
     LAUNCH_CHECK();
 }
 
