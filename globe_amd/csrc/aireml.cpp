@@ -426,6 +426,15 @@ int fit_impl(globe_ctx* c, const double* y, const globe_aireml_opts* opts, int n
         if (!full)
             globe::require(c->has_cohort && c->nrec == c->n, GLOBE_E_INVALID,
                            "the reduced model takes exactly one record per individual");
+        // y stays fixed for the whole fit: one upload, device copies in every evaluation
+        globe::require(c->has_cohort, GLOBE_E_STATE, "no cohort");
+        c->e_yfit.ensure(sizeof(double) * (size_t)c->nrec);
+        HIP_CHECK(hipMemcpy(c->e_yfit.p, y, sizeof(double) * (size_t)c->nrec, hipMemcpyHostToDevice));
+        c->yfit_host = y;
+        struct YfitGuard {
+            globe_ctx* c;
+            ~YfitGuard() { c->yfit_host = nullptr; }
+        } yfit_guard{c};
         double th[KMAX];
         std::memcpy(th, theta, sizeof(double) * nv);
         Eval cur{};

@@ -1058,6 +1058,16 @@ namespace globe {
 // sig2b1 = 0 and one record per individual, so V = sig2g A + sig2e I and f_theta has
 // the columns V^-1 r and A V^-1 r, compute_f_theta_i :524-548); grad, first_terms and
 // trace hold nvc values, ai nvc x nvc row-major.
+// y (nrec) into dst on the device: from the copy the running AI-REML fit uploaded once,
+// else from the host.
+static void upload_y(globe_ctx* c, const double* y, double* dst) {
+    if (y && y == c->yfit_host && c->e_yfit.p) {
+        launch_copy_cols(c->stream, c->nrec, c->e_yfit.as<double>(), c->nrec, dst, c->nrec, 1);
+        return;
+    }
+    HIP_CHECK(hipMemcpy(dst, y, sizeof(double) * c->nrec, hipMemcpyHostToDevice));
+}
+
 int aireml_eval_impl(globe_ctx* c, const double* y, const double* probes, const double* weights,
                      const double* probe_x0, int nmc, const globe_pcg_opts* opts, bool want_ai,
                      double* grad, double* ai, double* first_terms, double* trace, int nvc,
@@ -1138,7 +1148,7 @@ int aireml_eval_impl(globe_ctx* c, const double* y, const double* probes, const 
         double* xy = c->e_xy.as<double>();
         double* sol = c->e_sol.as<double>();
         launch_copy_cols(s, nrec, c->d_X.as<double>(), nrec, xy, nrec, q);
-        HIP_CHECK(hipMemcpy(xy + (size_t)q * nrec, y, sizeof(double) * nrec, hipMemcpyHostToDevice));
+        upload_y(c, y, xy + (size_t)q * nrec);
         sv = sol + (size_t)kf * nrec;
         load_probes(xy + (size_t)kf * nrec, sv);
         precond(c, xy, nrec, ot.x0_mode == GLOBE_X0_GIVEN ? kf : kb, sol, nrec, nullptr, nullptr, nullptr);
@@ -1154,7 +1164,7 @@ int aireml_eval_impl(globe_ctx* c, const double* y, const double* probes, const 
         op_vx(c, vr, nrec, 1 + nmc, us, n, GLOBE_VX_GRM_ONLY, o.precision, nullptr, nullptr);
     } else if (y) {
         // 1. fitMME at theta (varCompEst.cpp:587-685): x0 = Q^-1 b
-        HIP_CHECK(hipMemcpy(c->e_probe.p, y, sizeof(double) * nrec, hipMemcpyHostToDevice));
+        upload_y(c, y, c->e_probe.as<double>());
         f = fit_gls(c, c->e_probe.as<double>(), o, py);
         op_vx(c, py, nrec, 1, us, n, GLOBE_VX_GRM_ONLY, o.precision, nullptr, nullptr);
     } else {

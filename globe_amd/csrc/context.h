@@ -102,6 +102,10 @@ struct globe_ctx {
 
     // estimator workspaces
     globe::DevBuf e_xy, e_sol, e_f, e_probe, e_w, e_sv, e_pv, e_us, e_fs, e_coef, e_small;
+    // y of the AI-REML fit in progress (globe_aireml_fit holds y fixed for the whole fit
+    // and uploads it once); yfit_host is the caller's array it came from, null otherwise
+    globe::DevBuf e_yfit;
+    const double* yfit_host = nullptr;
     // log-likelihood: probes, their solutions and the solves' Lanczos coefficients
     globe::DevBuf l_b, l_x, l_hist;
 
