@@ -396,21 +396,26 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
         }
         queued += batch;
         // the scalar header (done, iters, ..., max_ratio), with relaxation also the
-        // per-column arrays up to the residual ratios
-        HIP_CHECK(hipMemcpyAsync(hs, dst, relax ? offsetof(PcgState, active) : offsetof(PcgState, rs),
+        // per-column arrays up to the residual ratios and the active flags
+        HIP_CHECK(hipMemcpyAsync(hs, dst, relax ? offsetof(PcgState, col_iters) : offsetof(PcgState, rs),
                                  hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
         if (hs->done || queued >= o.max_iter + 1) break;
         if (relax) {
             // each column's digits from its own residual ratio (the columns are
-            // independent recurrences, so each gets the single-RHS bound)
+            // independent recurrences, so each gets the single-RHS bound); a column the
+            // per-column rule has frozen takes none: its V p is never read again (its x
+            // and r stay, its alpha is 0)
             c->limbs_col.assign((size_t)k, (int8_t)kLimbsMixed);
-            int lmax = 2;
+            int lmax = 2, live = 0;
             for (int j = 0; j < k; ++j) {
-                const int L = relaxed_limbs(hs->ratio[j], o.tol);
+                const bool frozen = o.rule == GLOBE_RULE_PER_COLUMN && !hs->active[j];
+                const int L = frozen ? 0 : relaxed_limbs(hs->ratio[j], o.tol);
                 c->limbs_col[j] = (int8_t)L;
                 lmax = std::max(lmax, L);
+                live += L > 0;
             }
+            if (!live) c->limbs_col[0] = 2;  // the batch's launches are no-ops once done
             c->limbs_mixed = lmax;
         }
         const auto tn = std::chrono::steady_clock::now();

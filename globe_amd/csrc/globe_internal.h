@@ -65,8 +65,9 @@ struct PcgState {
 };
 
 // Digit layout of a k-column right-hand side in the int8 contraction: column c has L[c]
-// int8 digits (1..7) in digit columns off[c] .. off[c] + L[c] - 1; digit column j holds
-// digit dig[j] of column col[j] (col = 255: padding). Passed to kernels by value.
+// int8 digits (0..7; 0: a column whose product is not needed, computed as zero) in digit
+// columns off[c] .. off[c] + L[c] - 1; digit column j holds digit dig[j] of column col[j]
+// (col = 255: padding). Passed to kernels by value.
 struct LimbMap {
     int8_t L[kMaxCols];
     uint8_t off[kMaxCols];

@@ -920,7 +920,9 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     std::memset(lm.col, 255, sizeof(lm.col));
     int J = 0;
     for (int q = 0; q < k; ++q) {
-        require(col_limbs[q] >= 1 && col_limbs[q] <= kLimbsFP64, GLOBE_E_INVALID, "1 to 7 digits per column");
+        // 0 digits: a column whose product is not needed (a frozen PCG column); its
+        // result is the product of a zero vector
+        require(col_limbs[q] >= 0 && col_limbs[q] <= kLimbsFP64, GLOBE_E_INVALID, "0 to 7 digits per column");
         require(J + col_limbs[q] <= 128, GLOBE_E_INVALID, "at most 128 digit columns per MFMA contraction");
         lm.L[q] = col_limbs[q];
         lm.off[q] = (uint8_t)J;
@@ -930,6 +932,7 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
         }
         J += col_limbs[q];
     }
+    require(J >= 1, GLOBE_E_INVALID, "a contraction needs at least one digit column");
     if (c->cu_count == 0) {
         hipDeviceProp_t prop;
         HIP_CHECK(hipGetDeviceProperties(&prop, c->device));

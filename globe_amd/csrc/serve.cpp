@@ -413,7 +413,8 @@ int globe_serve_step(globe_ctx* c, int32_t* finished, int32_t* active, int32_t* 
     int fin = 0;
     if (k > 0) {
         // mixed precision: each slot's digits from its own residual ratio (4 in its first
-        // iteration), one digit column of zeros for a slot that holds no request
+        // iteration), none for a slot that holds no request (slot k - 1 is busy, so the
+        // contraction keeps at least one digit column)
         struct LimbsGuard {
             globe_ctx* c;
             ~LimbsGuard() {
@@ -422,7 +423,7 @@ int globe_serve_step(globe_ctx* c, int32_t* finished, int32_t* active, int32_t* 
             }
         } guard{c};
         if (S.o.precision == GLOBE_PREC_MIXED) {
-            c->limbs_col.assign((size_t)k, (int8_t)1);
+            c->limbs_col.assign((size_t)k, (int8_t)0);
             int lmax = 2;
             for (int j = 0; j < k; ++j) {
                 if (S.slot_state[j] != ServeState::kBusy) continue;
