@@ -138,6 +138,7 @@ def load_library(path: str = LIB_PATH):
         "globe_device_memory": ([P, P, P], C.c_int),
         "globe_variant_counts": ([P, P], C.c_int),
         "globe_set_mixed_digits": ([P, C.c_int], C.c_int),
+        "globe_set_graphs": ([P, C.c_int], C.c_int),
         "globe_profile_read": ([P, C.c_int, P], C.c_int),
         "globe_aireml_step": ([P, P, P, dbl, P], C.c_int),
         "globe_aireml_step_reduced": ([P, P, P, dbl, P], C.c_int),
@@ -475,6 +476,11 @@ class Globe:
     def set_mixed_digits(self, digits):
         """int8 digits (2..4) of mixed-precision genetic products outside a PCG solve."""
         self._check(self.L.globe_set_mixed_digits(self.h, int(digits)))
+
+    def set_graphs(self, on=True):
+        """PCG iterations replayed from captured HIP graphs (on, the default) or enqueued
+        kernel by kernel (off); the results are identical."""
+        self._check(self.L.globe_set_graphs(self.h, int(on)))
 
     def device_memory(self):
         """(free, total) bytes of device memory."""

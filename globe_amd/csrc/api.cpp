@@ -5,10 +5,12 @@
 // AIREML ((p+1)^2 and 5x5 systems), and error reporting. Every vector operation
 // runs on the device stream of the context.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstddef>
 #include <cstring>
+#include <list>
 #include <numeric>
 
 #include "context.h"
@@ -28,11 +30,19 @@ void hip_check(hipError_t e, const char* what) {
 // then a bug, reported instead of performed
 thread_local bool t_noalloc = false;
 void set_noalloc(bool on) { t_noalloc = on; }
+// set while an iteration is captured into a HIP graph: an allocation there would be a
+// workspace the graph does not own (every workspace is sized by an eager run first)
+thread_local bool t_capturing = false;
+// moves on every device allocation and release: captured graphs hold workspace addresses
+// and are discarded once it has moved
+std::atomic<uint64_t> g_alloc_epoch{0};
 
 void DevBuf::ensure(size_t b) {
     if (b <= bytes && p) return;
     require(!t_noalloc, GLOBE_E_STATE, "device allocation inside a no-allocation call");
+    require(!t_capturing, GLOBE_E_STATE, "device allocation inside a captured PCG iteration");
     release();
+    ++g_alloc_epoch;
     if (b == 0) b = 256;
     hipError_t e = hipMalloc(&p, b);
     if (e != hipSuccess) {
@@ -46,7 +56,10 @@ void DevBuf::ensure(size_t b) {
 }
 
 void DevBuf::release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+        (void)hipFree(p);
+        ++g_alloc_epoch;
+    }
     p = nullptr;
     bytes = 0;
 }
@@ -170,7 +183,9 @@ void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int prec
             e.flops = 2.0 * (double)n * n * k;
             e.issued = mixed ? 2.0 * np * np * cols * (L * (L + 1) / 2 + L - 1)
                              : (k <= 4 ? 2.0 * np * np * (k <= 2 ? k : 4) : 2.0 * np * np * cols);
-            HIP_CHECK(hipEventRecord(e.a, s));
+            // external: inside a captured PCG iteration the record is a graph node that
+            // stamps the event on every replay
+            HIP_CHECK(hipEventRecordWithFlags(e.a, s, hipEventRecordExternal));
         }
         if (mixed)
             launch_grm_product_i8(s, c->d_grm8.as<int8_t>(), grm_digit_rexp(c->d_grm8_aux.p, n), n, S0, n, k,
@@ -182,7 +197,7 @@ void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int prec
             launch_grm_product(s, c->d_grm.as<float>(), n, S0, n, k, th, GZ, n, skip,
                                static_cast<double*>(scratch));
         if (prof) {
-            HIP_CHECK(hipEventRecord(e.b, s));
+            HIP_CHECK(hipEventRecordWithFlags(e.b, s, hipEventRecordExternal));
             c->prof.push_back(e);
         }
         return;
@@ -298,6 +313,46 @@ int relaxed_limbs(double ratio, double tol) {
     return kLimbsMixed;
 }
 
+namespace {
+
+// One PCG iteration's launches replayed from a captured HIP graph. Within one solve the
+// launch sequence of an iteration depends only on its digit configuration (each
+// column's int8 digits in a mixed solve), so a graph is kept per configuration. A
+// configuration first runs eagerly (that run tunes every contraction shape it needs and
+// sizes every workspace) and is captured on its next use. Captured kernel arguments are
+// workspace addresses: a device allocation or release since the capture discards it.
+struct IterGraphs {
+    struct Entry {
+        std::vector<int8_t> key;
+        uint64_t warm = ~0ull;   // allocation epoch of its last eager run
+        uint64_t epoch = 0;      // allocation epoch at capture
+        hipGraphExec_t exec = nullptr;
+        std::vector<globe_ctx::ProfEvent> prof;  // timing events recorded by the graph
+    };
+    std::list<Entry> e;  // stable addresses: a batch keeps pointers to the entries it launched
+    ~IterGraphs() {
+        for (Entry& x : e) drop(x);
+    }
+    static void drop(Entry& x) {
+        if (x.exec) (void)hipGraphExecDestroy(x.exec);
+        x.exec = nullptr;
+        for (auto& p : x.prof) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        x.prof.clear();
+    }
+    Entry& find(const std::vector<int8_t>& key) {
+        for (Entry& x : e)
+            if (x.key == key) return x;
+        e.push_back(Entry{});
+        e.back().key = key;
+        return e.back();
+    }
+};
+
+}  // namespace
+
 void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64_t ldx,
              const globe_pcg_opts& o, globe_pcg_stats* st, double* col_ratio, int32_t* col_iters,
              double* hist, int hist_cap) {
@@ -381,18 +436,72 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
     } guard{c};
     c->limbs_mixed = kLimbsMixed;
     c->limbs_col.clear();
+    auto iteration = [&]() {
+        // the dot partials come fused out of the kernels that produce their operands
+        // (V p output, x/r update, preconditioner); one block reduces each of them
+        op_vx(c, p, nrec, k, vp, nrec, GLOBE_VX_FULL, o.precision, skip, part_a);
+        launch_pcg_alpha(s, dst, part_a, np, k);
+        launch_pcg_xr(s, dst, nrec, X, ldx, r, p, vp, nrec, k, part_b);
+        launch_pcg_check(s, dst, part_b, np, k);
+        precond(c, r, nrec, k, z, nrec, part_c, nullptr, dst);
+        launch_pcg_beta(s, dst, part_c, npb, k);
+        launch_pcg_p(s, dst, nrec, p, z, nrec, k);
+    };
+    // graphs (not inside a no-allocation call, whose workspaces are arena slices); with
+    // timing on, one iteration per batch, so the host reads each replay's events before
+    // the next replay records them again
+    const bool graphs = c->graphs_on && !c->arena.base;
+    IterGraphs ig;
+    std::vector<IterGraphs::Entry*> launched;
     while (true) {
+        if (graphs && c->prof_on) batch = 1;
+        launched.clear();
         for (int b = 0; b < batch; ++b) {
             c->prof_iter = queued + b;
-            // the dot partials come fused out of the kernels that produce their operands
-            // (V p output, x/r update, preconditioner); one block reduces each of them
-            op_vx(c, p, nrec, k, vp, nrec, GLOBE_VX_FULL, o.precision, skip, part_a);
-            launch_pcg_alpha(s, dst, part_a, np, k);
-            launch_pcg_xr(s, dst, nrec, X, ldx, r, p, vp, nrec, k, part_b);
-            launch_pcg_check(s, dst, part_b, np, k);
-            precond(c, r, nrec, k, z, nrec, part_c, nullptr, dst);
-            launch_pcg_beta(s, dst, part_c, npb, k);
-            launch_pcg_p(s, dst, nrec, p, z, nrec, k);
+            if (!graphs) {
+                iteration();
+                continue;
+            }
+            std::vector<int8_t> key(c->limbs_col);
+            key.push_back((int8_t)c->limbs_mixed);
+            IterGraphs::Entry& g = ig.find(key);
+            const uint64_t now = g_alloc_epoch.load();
+            if (g.exec && g.epoch != now) IterGraphs::drop(g);
+            if (!g.exec && g.warm == now) {
+                // capture: the same launches, recorded instead of run
+                const size_t pm = c->prof.size();
+                hipGraph_t graph = nullptr;
+                HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                t_capturing = true;
+                try {
+                    iteration();
+                } catch (...) {
+                    t_capturing = false;
+                    (void)hipStreamEndCapture(s, &graph);
+                    if (graph) (void)hipGraphDestroy(graph);
+                    for (size_t q = pm; q < c->prof.size(); ++q) {
+                        (void)hipEventDestroy(c->prof[q].a);
+                        (void)hipEventDestroy(c->prof[q].b);
+                    }
+                    c->prof.resize(pm);
+                    throw;
+                }
+                t_capturing = false;
+                HIP_CHECK(hipStreamEndCapture(s, &graph));
+                const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                HIP_CHECK(ie);
+                g.epoch = now;
+                g.prof.assign(c->prof.begin() + pm, c->prof.end());
+                c->prof.resize(pm);
+            }
+            if (g.exec) {
+                HIP_CHECK(hipGraphLaunch(g.exec, s));
+                launched.push_back(&g);
+            } else {
+                iteration();
+                g.warm = g_alloc_epoch.load();
+            }
         }
         queued += batch;
         // the scalar header (done, iters, ..., max_ratio), with relaxation also the
@@ -400,6 +509,15 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
         HIP_CHECK(hipMemcpyAsync(hs, dst, relax ? offsetof(PcgState, col_iters) : offsetof(PcgState, rs),
                                  hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
+        // timing events of a replayed graph: read now (batch 1), before a replay re-records them
+        for (IterGraphs::Entry* g : launched)
+            for (const auto& e : g->prof) {
+                globe_ctx::ProfEvent t = e;
+                t.a = t.b = nullptr;
+                t.iter = queued - 1;
+                HIP_CHECK(hipEventElapsedTime(&t.ms, e.a, e.b));
+                c->prof.push_back(t);
+            }
         if (hs->done || queued >= o.max_iter + 1) break;
         if (relax) {
             // each column's digits from its own residual ratio (the columns are
@@ -434,7 +552,7 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
             const auto& e = c->prof[q];
             if (e.iter < hs->iters) {
                 keep.push_back(e);
-            } else {
+            } else if (e.a) {
                 (void)hipEventDestroy(e.a);
                 (void)hipEventDestroy(e.b);
             }
@@ -505,6 +623,7 @@ void globe_destroy(globe_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     serve_release(c);
     for (auto& e : c->prof) {
+        if (!e.a) continue;
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
     }
@@ -1303,10 +1422,17 @@ int globe_set_mixed_digits(globe_ctx* c, int digits) {
     API_END(c)
 }
 
+int globe_set_graphs(globe_ctx* c, int on) {
+    API_BEGIN(c)
+    c->graphs_on = on != 0;
+    API_END(c)
+}
+
 int globe_profile_enable(globe_ctx* c, int on) {
     API_BEGIN(c)
     HIP_CHECK(hipStreamSynchronize(c->stream));
     for (auto& e : c->prof) {
+        if (!e.a) continue;
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
     }
@@ -1322,8 +1448,8 @@ int globe_profile_read(globe_ctx* c, int which, globe_kernel_stats* out) {
     globe_kernel_stats st{};
     for (const auto& e : c->prof) {
         if (e.pass != which) continue;
-        float ms = 0.f;
-        HIP_CHECK(hipEventElapsedTime(&ms, e.a, e.b));
+        float ms = e.ms;
+        if (e.a) HIP_CHECK(hipEventElapsedTime(&ms, e.a, e.b));
         st.launches += 1;
         st.ms_total += ms;
         st.bytes_per_launch = e.bytes;

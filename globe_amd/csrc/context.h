@@ -77,16 +77,21 @@ struct globe_ctx {
     int variant_count[2] = {0, 0};
     int cu_count = 0;  // compute units of the device (stream-K grids)
 
+    // PCG iterations replayed from captured HIP graphs (globe_set_graphs; on by default)
+    bool graphs_on = true;
+
     // PCG workspaces and state
     globe::DevBuf p_state, p_r, p_z, p_p, p_vp;
     globe::PcgState* h_state = nullptr;  // pinned mirror
 
     // kernel timing: event pairs around contraction launches, tagged with the PCG
-    // iteration that issued them (-1 outside PCG)
+    // iteration that issued them (-1 outside PCG); a launch replayed from a graph keeps
+    // its measured time instead (a = b = null)
     struct ProfEvent {
         hipEvent_t a, b;
         int pass, iter;
         double bytes, flops, issued;
+        float ms = 0.f;
     };
     bool prof_on = false;
     int prof_iter = -1;

@@ -1056,9 +1056,11 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
             e.bytes = (double)nrows * (double)((K + 3) / 4);
             e.flops = flops;
             e.issued = 2.0 * (double)nrows * (double)Kpad * J16;
-            HIP_CHECK(hipEventRecord(e.a, s));
+            // external: inside a captured PCG iteration the record is a graph node that
+            // stamps the event on every replay
+            HIP_CHECK(hipEventRecordWithFlags(e.a, s, hipEventRecordExternal));
             launch_variant(s, wide, JT, v, rows, pitch, nrows, nss, Bq, out, J, skip);
-            HIP_CHECK(hipEventRecord(e.b, s));
+            HIP_CHECK(hipEventRecordWithFlags(e.b, s, hipEventRecordExternal));
             c->prof.push_back(e);
         } else {
             launch_variant(s, wide, JT, v, rows, pitch, nrows, nss, Bq, out, J, skip);

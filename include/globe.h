@@ -372,6 +372,12 @@ int globe_serve_close(globe_ctx* ctx);
  * place. Fewer digits trade accuracy (about 2^(1-8L) of each operand's scale) for speed. */
 int globe_set_mixed_digits(globe_ctx* ctx, int digits);
 
+/* ---- PCG iteration graphs (no reference counterpart) ------------------------------- */
+/* on (the default): a PCG solve captures one iteration's launches into a HIP graph per
+ * digit configuration (after running that configuration once eagerly) and replays it;
+ * off: every iteration's kernels are enqueued one by one. Results are identical. */
+int globe_set_graphs(globe_ctx* ctx, int on);
+
 /* ---- contraction variants (no reference counterpart: validation of the autotuner) ---- */
 /* The first packed-genotype contraction of each shape times every candidate variant (block
  * shape, split-K or stream-K grid) and keeps the fastest; all candidates compute the same
