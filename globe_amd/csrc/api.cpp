@@ -245,7 +245,7 @@ void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int prec
 }
 
 void op_vx(globe_ctx* c, const double* x, int64_t ldx, int k, double* y, int64_t ldy, int mode,
-           int precision, const int* skip, double* part_dot) {
+           int precision, const int* skip, double* part_dot, const double* pz, PcgState* pst) {
     require(c->has_cohort, GLOBE_E_STATE, "cohort not set (globe_set_cohort)");
     require(c->gen != 0, GLOBE_E_STATE, "no genetics loaded (globe_load_bed / globe_load_grm)");
     require(k >= 1 && k <= kMaxCols, GLOBE_E_INVALID, "k must be in [1, 64]");
@@ -255,7 +255,16 @@ void op_vx(globe_ctx* c, const double* x, int64_t ldx, int k, double* y, int64_t
     double* S1 = ws<double>(c, c->w_S1, sizeof(double) * (size_t)n * k);
     double* GZ = ws<double>(c, c->w_GZ, sizeof(double) * (size_t)n * k);
     hipStream_t s = c->stream;
-    launch_seg_sums(s, c->meta, x, ldx, k, S0, S1, n, skip);
+    if (pst) {
+        // x is PCG's search direction: p = z + beta p first (fused into the sums when the
+        // records are tiled)
+        if (!launch_seg_sums_pcg(s, c->meta, const_cast<double*>(x), pz, ldx, k, pst, S0, S1, n)) {
+            launch_pcg_p(s, pst, c->nrec, const_cast<double*>(x), pz, ldx, k);
+            launch_seg_sums(s, c->meta, x, ldx, k, S0, S1, n, skip);
+        }
+    } else {
+        launch_seg_sums(s, c->meta, x, ldx, k, S0, S1, n, skip);
+    }
     if (mode == GLOBE_VX_GRM_ONLY) {
         genetic_product(c, S0, k, GZ, precision, skip, c->d_theta.as<double>());
         launch_copy_cols(s, n, GZ, n, y, ldy, k);
@@ -436,16 +445,23 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
     } guard{c};
     c->limbs_mixed = kLimbsMixed;
     c->limbs_col.clear();
+    const bool tiled = c->meta.cap > 0;
     auto iteration = [&]() {
-        // the dot partials come fused out of the kernels that produce their operands
-        // (V p output, x/r update, preconditioner); one block reduces each of them
-        op_vx(c, p, nrec, k, vp, nrec, GLOBE_VX_FULL, o.precision, skip, part_a);
+        // p = z + beta p (beta = 0 in the first iteration, where p = z already) fused into
+        // the segmented sums of V p; the dot partials come out of the kernels that produce
+        // their operands (V p output; x/r update + preconditioner, one pass); two
+        // single-block reductions: alpha, then the convergence test with beta
+        op_vx(c, p, nrec, k, vp, nrec, GLOBE_VX_FULL, o.precision, skip, part_a, z, dst);
         launch_pcg_alpha(s, dst, part_a, np, k);
-        launch_pcg_xr(s, dst, nrec, X, ldx, r, p, vp, nrec, k, part_b);
-        launch_pcg_check(s, dst, part_b, np, k);
-        precond(c, r, nrec, k, z, nrec, part_c, nullptr, dst);
-        launch_pcg_beta(s, dst, part_c, npb, k);
-        launch_pcg_p(s, dst, nrec, p, z, nrec, k);
+        if (tiled) {
+            launch_pcg_xr_precond(s, c->meta, c->d_gdiag.as<double>(), c->d_theta.as<double>(), dst, X, ldx, r,
+                                  p, vp, nrec, k, z, part_c, part_b);
+            launch_pcg_check_beta(s, dst, part_b, npb, part_c, npb, k);
+        } else {
+            launch_pcg_xr(s, dst, nrec, X, ldx, r, p, vp, nrec, k, part_b);
+            precond(c, r, nrec, k, z, nrec, part_c, nullptr, dst);
+            launch_pcg_check_beta(s, dst, part_b, np, part_c, npb, k);
+        }
     };
     // graphs (not inside a no-allocation call, whose workspaces are arena slices); with
     // timing on, one iteration per batch, so the host reads each replay's events before

@@ -168,9 +168,12 @@ void serve_release(globe_ctx* c);
 void* scratch_address(int device, size_t* bytes);
 
 // device operator: y = V x (mode FULL) or sig2g G J^T x (GRM_ONLY); optional fused
-// partials of sum x.y per column; skip = device done flag (nullptr: always run).
+// partials of sum x.y per column; skip = device done flag (nullptr: always run). Inside
+// PCG (pst set), x is the search direction p, first updated in place to z + beta p on the
+// active columns (pz = z, same leading dimension).
 void op_vx(globe_ctx* c, const double* x, int64_t ldx, int k, double* y, int64_t ldy, int mode,
-           int precision, const int* skip, double* part_dot);
+           int precision, const int* skip, double* part_dot, const double* pz = nullptr,
+           PcgState* pst = nullptr);
 void ensure_theta(globe_ctx* c, int k);
 // GZ (n x k) = sig2g_c G S0 per column with the per-column components theta_cols [k][5]
 void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int precision,

@@ -116,6 +116,16 @@ inline int64_t max_tiles(int64_t records) { return 2 * records / kTileBlocks + 1
 int precond_parts(const RecMeta& m);
 void launch_seg_sums(hipStream_t s, const RecMeta& m, const double* x, int64_t ldx, int k,
                      double* S0, double* S1, int64_t ldn, const int* skip);
+// PCG forms on tiled record structure (return false, launching nothing, on the
+// thread-per-block structure): the segmented sums of the search direction after its
+// update p = z + beta p (active columns, skipped once the solve is done), and the x/r
+// update fused with the preconditioner (r.z and r.r partials of the new r, in
+// precond_parts(m) slots)
+bool launch_seg_sums_pcg(hipStream_t s, const RecMeta& m, double* p, const double* z, int64_t ld, int k,
+                         PcgState* st, double* S0, double* S1, int64_t ldn);
+bool launch_pcg_xr_precond(hipStream_t s, const RecMeta& m, const double* gdiag, const double* theta_cols,
+                           PcgState* st, double* x, int64_t ldx, double* r, const double* p, const double* vp,
+                           int64_t ld, int k, double* z, double* part_rz, double* part_rr);
 void launch_record_out(hipStream_t s, const RecMeta& m, const double* x, int64_t ldx, int k,
                        const double* GZ, const double* S0, const double* S1, int64_t ldn,
                        const double* theta_cols, double* y, int64_t ldy, double* part_dot,
@@ -146,6 +156,9 @@ void launch_pcg_admit(hipStream_t s, PcgState* st, int j, const double* part_bb,
 void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* part_pvp, int nparts, int k);
 void launch_pcg_check(hipStream_t s, PcgState* st, const double* part_rr, int nparts, int k);
 void launch_pcg_beta(hipStream_t s, PcgState* st, const double* part_rz, int nparts, int k);
+// the convergence test, then beta unless the solve has ended
+void launch_pcg_check_beta(hipStream_t s, PcgState* st, const double* part_rr, int nparts_rr,
+                           const double* part_rz, int nparts_rz, int k);
 // x += alpha p, r -= alpha Vp per active column with the r.r partials
 void launch_pcg_xr(hipStream_t s, PcgState* st, int64_t nrec, double* x, int64_t ldx, double* r,
                    const double* p, const double* vp, int64_t ld, int k, double* part_rr,

@@ -269,16 +269,22 @@ __global__ __launch_bounds__(kBlock) void k_record_out(
 // of whole individual blocks in LDS with coalesced loads, one thread per block sums its
 // records from LDS in record order (so the block sums equal the thread-per-block
 // kernels' bit for bit), and the preconditioner writes z back one thread per record.
-__global__ __launch_bounds__(kBlock) void k_seg_sums_tiled(RecMeta m, const double* __restrict__ x,
+// With pst (inside PCG), x is the search direction p and is first updated in place,
+// p = z + beta p for the active columns (PCGsolver.cpp:1427), so the sums see the new p.
+__global__ __launch_bounds__(kBlock) void k_seg_sums_tiled(RecMeta m, double* __restrict__ x,
                                                            int64_t ldx, double* __restrict__ S0,
                                                            double* __restrict__ S1, int64_t ldn,
-                                                           const int* skip) {
+                                                           const int* skip, const double* __restrict__ z,
+                                                           const PcgState* pst) {
     if (skip && *skip) return;
     extern __shared__ double sm[];
     double* sx = sm;
     double* st = sm + m.cap;
     const int c = blockIdx.y;
-    const double* xc = x + col_start(m, c, ldx);
+    const bool upd = pst && pst->active[c];
+    const double beta = upd ? pst->beta[c] : 0.0;
+    double* xc = x + col_start(m, c, ldx);
+    const double* zc = upd ? z + col_start(m, c, ldx) : nullptr;
     const double* t = m.times + col_start(m, c, m.rec_ld);
     const int64_t* off = m.blk_off + col_start(m, c, m.blk_ld);
     const int32_t* ind = m.blk_ind + col_start(m, c, m.blk_ld);
@@ -289,7 +295,28 @@ __global__ __launch_bounds__(kBlock) void k_seg_sums_tiled(RecMeta m, const doub
         const int64_t r0 = off[b0];
         const int nr = (int)(off[b1] - r0);
         __syncthreads();
-        stage_tile(xc + r0, t + r0, nr, sx, st);
+        if (upd) {
+            double vp[kTileIters], vz[kTileIters], vt[kTileIters];
+#pragma unroll
+            for (int i = 0; i < kTileIters; ++i) {
+                const int q = threadIdx.x + kBlock * i;
+                vp[i] = q < nr ? xc[r0 + q] : 0.0;
+                vz[i] = q < nr ? zc[r0 + q] : 0.0;
+                vt[i] = q < nr ? t[r0 + q] : 0.0;
+            }
+#pragma unroll
+            for (int i = 0; i < kTileIters; ++i) {
+                const int q = threadIdx.x + kBlock * i;
+                if (q < nr) {
+                    const double pv = vz[i] + beta * vp[i];
+                    xc[r0 + q] = pv;
+                    sx[q] = pv;
+                    st[q] = vt[i];
+                }
+            }
+        } else {
+            stage_tile(xc + r0, t + r0, nr, sx, st);
+        }
         __syncthreads();
         for (int32_t b = b0 + threadIdx.x; b < b1; b += kBlock) {
             const int q0 = (int)(off[b] - r0), q1 = (int)(off[b + 1] - r0);
@@ -395,6 +422,122 @@ __global__ __launch_bounds__(kBlock) void k_precond_tiled(
         double s = block_sum(arr);
         if (threadIdx.x == 0) part_rr[c * kMaxParts + blockIdx.x] = s;
     }
+}
+
+// One PCG step after alpha, fused: x += alpha p and r -= alpha Vp per active column
+// (PCGsolver.cpp:1383-1386), then z = Q^-1 r by the preconditioner of k_precond_tiled on
+// the updated r, with the r.r and r.z partials of the new r. Frozen columns (per-column
+// rule) only give their r.r partials. The convergence test and beta follow in one
+// single-block kernel (k_pcg_check_beta): a column that the test freezes has its z
+// computed here but never used.
+__global__ __launch_bounds__(kBlock) void k_pcg_xr_precond(
+    RecMeta m, const double* __restrict__ gdiag, const double* __restrict__ th, const PcgState* stt,
+    double* __restrict__ x, int64_t ldx, double* __restrict__ r, const double* __restrict__ p,
+    const double* __restrict__ vp, int64_t ld, double* __restrict__ z, double* part_rz, double* part_rr) {
+    if (stt->done) return;
+    const int c = blockIdx.y;
+    const bool act = stt->active[c] != 0;
+    const double alpha = stt->alpha[c];
+    extern __shared__ double sm[];
+    double* sr = sm;               // r of the tile's records (updated)
+    double* st = sm + m.cap;       // t of the tile's records
+    double* sc0 = sm + 2 * m.cap;  // per block (at most kTileBlocks): Woodbury coefficients
+    double* sc1 = sc0 + kTileBlocks;
+    const double se = th[c * 5 + 0], sg = th[c * 5 + 1], sb0 = th[c * 5 + 2],
+                 sb01 = th[c * 5 + 3], sb1 = th[c * 5 + 4];
+    double* rc = r + col_start(m, c, ld);
+    double* xc = x + col_start(m, c, ldx);
+    const double* pc = p + col_start(m, c, ld);
+    const double* vc = vp + col_start(m, c, ld);
+    double* zc = z + col_start(m, c, ld);
+    const double* t = m.times + col_start(m, c, m.rec_ld);
+    const int32_t* rec_blk = m.rec_blk + col_start(m, c, m.rec_ld);
+    const int64_t* off = m.blk_off + col_start(m, c, m.blk_ld);
+    const int32_t* ind = m.blk_ind + col_start(m, c, m.blk_ld);
+    const int32_t* tile_blk = m.tile_blk + col_start(m, c, m.tile_ld);
+    const int32_t ntiles = m.ntiles_col ? m.ntiles_col[c] : m.ntiles;
+    const double inv_se = 1.0 / se;
+    double arz = 0.0, arr = 0.0;
+    for (int32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int32_t b0 = tile_blk[tile], b1 = tile_blk[tile + 1];
+        const int64_t r0 = off[b0];
+        const int nr = (int)(off[b1] - r0);
+        if (!act) {
+            for (int q = threadIdx.x; q < nr; q += kBlock) {
+                const double rv = rc[r0 + q];
+                arr += rv * rv;
+            }
+            continue;
+        }
+        // all of this thread's loads of the tile before any store
+        int32_t rb[kTileIters];
+        double vr[kTileIters], vv[kTileIters], vpp[kTileIters], vx[kTileIters], vt[kTileIters];
+#pragma unroll
+        for (int i = 0; i < kTileIters; ++i) {
+            const int q = threadIdx.x + kBlock * i;
+            const bool in = q < nr;
+            rb[i] = in ? rec_blk[r0 + q] : 0;
+            vr[i] = in ? rc[r0 + q] : 0.0;
+            vv[i] = in ? vc[r0 + q] : 0.0;
+            vpp[i] = in ? pc[r0 + q] : 0.0;
+            vx[i] = in ? xc[r0 + q] : 0.0;
+            vt[i] = in ? t[r0 + q] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kTileIters; ++i) {
+            const int q = threadIdx.x + kBlock * i;
+            if (q < nr) {
+                const double xv = vx[i] + alpha * vpp[i];
+                const double rv = vr[i] - alpha * vv[i];
+                xc[r0 + q] = xv;
+                rc[r0 + q] = rv;
+                vr[i] = rv;
+                sr[q] = rv;
+                st[q] = vt[i];
+                arr += rv * rv;
+            }
+        }
+        __syncthreads();
+        for (int32_t b = b0 + threadIdx.x; b < b1; b += kBlock) {
+            const int q0 = (int)(off[b] - r0), q1 = (int)(off[b + 1] - r0);
+            double a01 = 0.0, a11 = 0.0, w0 = 0.0, w1 = 0.0;
+            for (int q = q0; q < q1; ++q) {
+                const double tq = st[q], rv = sr[q];
+                a01 += tq;
+                a11 += tq * tq;
+                w0 += rv;
+                w1 += tq * rv;
+            }
+            const double a00 = (double)(q1 - q0);
+            const double gv = sg * gdiag[ind[b]];
+            const double c00 = gv + sb0, c01 = sb01, c11 = sb1;
+            const double m00 = se + c00 * a00 + c01 * a01;
+            const double m01 = c00 * a01 + c01 * a11;
+            const double m10 = c01 * a00 + c11 * a01;
+            const double m11 = se + c01 * a01 + c11 * a11;
+            const double v0 = c00 * w0 + c01 * w1;
+            const double v1 = c01 * w0 + c11 * w1;
+            const double det = m00 * m11 - m01 * m10;
+            sc0[b - b0] = (m11 * v0 - m01 * v1) / det;
+            sc1[b - b0] = (m00 * v1 - m10 * v0) / det;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kTileIters; ++i) {
+            const int q = threadIdx.x + kBlock * i;
+            if (q >= nr) break;
+            const int bl = rb[i] - b0;
+            const double rv = vr[i];
+            const double zv = (rv - sc0[bl] - sc1[bl] * vt[i]) * inv_se;
+            zc[r0 + q] = zv;
+            arz += rv * zv;
+        }
+    }
+    double s = block_sum(arz);
+    if (threadIdx.x == 0) part_rz[c * kMaxParts + blockIdx.x] = s;
+    s = block_sum(arr);
+    if (threadIdx.x == 0) part_rr[c * kMaxParts + blockIdx.x] = s;
 }
 
 // z = Q_i^-1 r per individual block by the 2x2 Woodbury form of
@@ -668,6 +811,17 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_beta(PcgState* st, const do
     pcg_beta_body(st, part_rz, nparts, k);
 }
 
+// The convergence test, then beta if the solve goes on (one launch per iteration)
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_check_beta(PcgState* st, const double* __restrict__ part_rr,
+                                                           int nparts_rr, const double* __restrict__ part_rz,
+                                                           int nparts_rz, int k) {
+    if (st->done) return;
+    pcg_check_body(st, part_rr, nparts_rr, k);
+    __syncthreads();
+    if (st->done) return;
+    pcg_beta_body(st, part_rz, nparts_rz, k);
+}
+
 __global__ __launch_bounds__(kBlock) void k_pcg_xr(PcgState* st, int64_t nrec, double* __restrict__ x,
                                                    int64_t ldx, double* __restrict__ r,
                                                    const double* __restrict__ p,
@@ -828,13 +982,34 @@ void launch_seg_sums(hipStream_t s, const RecMeta& m, const double* x, int64_t l
                      double* S0, double* S1, int64_t ldn, const int* skip) {
     if (m.cap > 0) {
         dim3 g(precond_parts(m), k);
-        k_seg_sums_tiled<<<g, kBlock, sizeof(double) * 2 * m.cap, s>>>(m, x, ldx, S0, S1, ldn, skip);
+        k_seg_sums_tiled<<<g, kBlock, sizeof(double) * 2 * m.cap, s>>>(m, const_cast<double*>(x), ldx, S0, S1,
+                                                                        ldn, skip, nullptr, nullptr);
     } else {
         dim3 g(parts_for(m.nblk), k);
         k_seg_sums<<<g, kBlock, 0, s>>>(m.nblk, m.blk_off, m.blk_ind, m.times, x, ldx, S0, S1, ldn,
                                          skip);
     }
     LAUNCH_CHECK();
+}
+
+bool launch_seg_sums_pcg(hipStream_t s, const RecMeta& m, double* p, const double* z, int64_t ld, int k,
+                         PcgState* st, double* S0, double* S1, int64_t ldn) {
+    if (m.cap == 0) return false;
+    dim3 g(precond_parts(m), k);
+    k_seg_sums_tiled<<<g, kBlock, sizeof(double) * 2 * m.cap, s>>>(m, p, ld, S0, S1, ldn, &st->done, z, st);
+    LAUNCH_CHECK();
+    return true;
+}
+
+bool launch_pcg_xr_precond(hipStream_t s, const RecMeta& m, const double* gdiag, const double* theta_cols,
+                           PcgState* st, double* x, int64_t ldx, double* r, const double* p, const double* vp,
+                           int64_t ld, int k, double* z, double* part_rz, double* part_rr) {
+    if (m.cap == 0) return false;
+    dim3 g(precond_parts(m), k);
+    k_pcg_xr_precond<<<g, kBlock, sizeof(double) * (2 * m.cap + 2 * kTileBlocks), s>>>(
+        m, gdiag, theta_cols, st, x, ldx, r, p, vp, ld, z, part_rz, part_rr);
+    LAUNCH_CHECK();
+    return true;
 }
 
 void launch_record_out(hipStream_t s, const RecMeta& m, const double* x, int64_t ldx, int k,
@@ -941,6 +1116,12 @@ void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* part_pvp, int n
 
 void launch_pcg_check(hipStream_t s, PcgState* st, const double* part_rr, int nparts, int k) {
     k_pcg_check<<<1, kPcgThreads, 0, s>>>(st, part_rr, nparts, k);
+    LAUNCH_CHECK();
+}
+
+void launch_pcg_check_beta(hipStream_t s, PcgState* st, const double* part_rr, int nparts_rr,
+                           const double* part_rz, int nparts_rz, int k) {
+    k_pcg_check_beta<<<1, kPcgThreads, 0, s>>>(st, part_rr, nparts_rr, part_rz, nparts_rz, k);
     LAUNCH_CHECK();
 }
 

@@ -279,22 +279,19 @@ void iterate(globe_ctx* c, ServeState& S, int k) {
     double* part_a = S.part_a.as<double>();
     double* part_b = S.part_b.as<double>();
     double* part_c = S.part_c.as<double>();
-    const int64_t* off = S.col_off.as<int64_t>();
-    const int64_t* cnt = S.nrec_col.as<int64_t>();
     // V p: individuals outside a request's subset keep S0 = S1 = 0
     launch_fill(s, S0, (int64_t)n * k, 0.0);
     launch_fill(s, S1, (int64_t)n * k, 0.0);
-    launch_seg_sums(s, m, S.P.as<double>(), 0, k, S0, S1, n, skip);
+    // p = z + beta p of the active slots fused into the sums (an admitted slot starts
+    // with p = z and beta = 0), the x/r update fused with the preconditioner, and the
+    // convergence test with beta in one block (the slots' records are always tiled)
+    launch_seg_sums_pcg(s, m, S.P.as<double>(), S.Z.as<double>(), 0, k, st, S0, S1, n);
     genetic_product(c, S0, k, GZ, S.o.precision, skip, th);
     launch_record_out(s, m, S.P.as<double>(), 0, k, GZ, S0, S1, n, th, S.VP.as<double>(), 0, part_a, skip);
     launch_pcg_alpha(s, st, part_a, parts_for(cap), k);
-    launch_pcg_xr(s, st, cap, S.X.as<double>(), 0, S.R.as<double>(), S.P.as<double>(), S.VP.as<double>(), 0, k,
-                  part_b, off, cnt);
-    launch_pcg_check(s, st, part_b, parts_for(cap), k);
-    launch_precond(s, m, c->d_gdiag.as<double>(), th, S.R.as<double>(), 0, k, S.Z.as<double>(), 0, part_c,
-                   nullptr, st, 0);
-    launch_pcg_beta(s, st, part_c, precond_parts(m), k);
-    launch_pcg_p(s, st, cap, S.P.as<double>(), S.Z.as<double>(), 0, k, off, cnt);
+    launch_pcg_xr_precond(s, m, c->d_gdiag.as<double>(), th, st, S.X.as<double>(), 0, S.R.as<double>(),
+                          S.P.as<double>(), S.VP.as<double>(), 0, k, S.Z.as<double>(), part_c, part_b);
+    launch_pcg_check_beta(s, st, part_b, precond_parts(m), part_c, precond_parts(m), k);
     ++S.global_iter;
 }
 
