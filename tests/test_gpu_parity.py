@@ -81,6 +81,16 @@ def test_genetic_diag(name, gpu_ctx_factory):
     assert rel(g.computeGeneticDiagonal(), golden_oracle(d).genetic_diag()) < 1e-13
 
 
+@pytest.mark.parametrize("name", GENO)
+def test_geno_Wa_matches_oracle(name, gpu_ctx_factory):
+    """globe_geno_Wa (the synthetic phenotypes' genetic values) against the oracle's W a."""
+    _say("geno_Wa", name)
+    d = load_golden(name)
+    g = device_for(d, gpu_ctx_factory)
+    a = np.random.default_rng(12).standard_normal(g.m)
+    assert rel(g.geno_Wa(a), golden_oracle(d).geno_Wa(a)) < 1e-12
+
+
 @pytest.mark.parametrize("name", GENO + ["grm_n180"])
 @pytest.mark.parametrize("prec", [G.PREC_FP64, G.PREC_FP64_VALU])
 def test_vx_fp64(name, prec, gpu_ctx_factory):

@@ -98,6 +98,21 @@ def test_vx_matches_explicit_dense_V(name):
         assert rel(y32, (V - se * np.eye(c.nrec)) @ x) > 1e-3
 
 
+def test_geno_Wa_matches_dense_standardized_genotypes():
+    """The phenotype recipe's W a (standardized genotypes of the kept SNPs times a
+    SNP-space vector) against the dense FP64 W built independently by synth.standardized,
+    with missing calls and QC-dropped SNPs; a W without its centering fails."""
+    d = synth.make_geno_case(n=301, m=700, lam=3, seed=41, miss=0.02)
+    n = d.cohort.n
+    W = synth.standardized(d.bed_rows[d.keep], n, d.freq)
+    a = np.random.default_rng(6).standard_normal(W.shape[1])
+    got = O.oracle_from_synth(d).geno_Wa(a)
+    assert rel(got, W @ a) < 1e-12
+    codes = synth.unpack_codes(d.bed_rows[d.keep], n)
+    uncentred = synth.DOSAGE_OF_CODE[codes].astype(np.float64).T
+    assert rel(got, uncentred @ a) > 1e-3
+
+
 def test_preconditioner_is_dense_block_inverse():
     d = load_golden("geno_n150_odd")
     orc = golden_oracle(d)
