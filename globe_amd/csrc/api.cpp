@@ -449,15 +449,16 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
     auto iteration = [&]() {
         // p = z + beta p (beta = 0 in the first iteration, where p = z already) fused into
         // the segmented sums of V p; the dot partials come out of the kernels that produce
-        // their operands (V p output; x/r update + preconditioner, one pass); two
-        // single-block reductions: alpha, then the convergence test with beta
+        // their operands (V p output; x/r update + preconditioner, one pass, which also
+        // reduces alpha in every workgroup); one single-block reduction: the convergence
+        // test with beta
         op_vx(c, p, nrec, k, vp, nrec, GLOBE_VX_FULL, o.precision, skip, part_a, z, dst);
-        launch_pcg_alpha(s, dst, part_a, np, k);
         if (tiled) {
-            launch_pcg_xr_precond(s, c->meta, c->d_gdiag.as<double>(), c->d_theta.as<double>(), dst, X, ldx, r,
-                                  p, vp, nrec, k, z, part_c, part_b);
+            launch_pcg_xr_precond(s, c->meta, c->d_gdiag.as<double>(), c->d_theta.as<double>(), dst, part_a, np, X,
+                                  ldx, r, p, vp, nrec, k, z, part_c, part_b);
             launch_pcg_check_beta(s, dst, part_b, npb, part_c, npb, k);
         } else {
+            launch_pcg_alpha(s, dst, part_a, np, k);
             launch_pcg_xr(s, dst, nrec, X, ldx, r, p, vp, nrec, k, part_b);
             precond(c, r, nrec, k, z, nrec, part_c, nullptr, dst);
             launch_pcg_check_beta(s, dst, part_b, np, part_c, npb, k);

@@ -118,14 +118,15 @@ void launch_seg_sums(hipStream_t s, const RecMeta& m, const double* x, int64_t l
                      double* S0, double* S1, int64_t ldn, const int* skip);
 // PCG forms on tiled record structure (return false, launching nothing, on the
 // thread-per-block structure): the segmented sums of the search direction after its
-// update p = z + beta p (active columns, skipped once the solve is done), and the x/r
-// update fused with the preconditioner (r.z and r.r partials of the new r, in
-// precond_parts(m) slots)
+// update p = z + beta p (active columns, skipped once the solve is done), and alpha from
+// the np p.Vp partials with the x/r update and the preconditioner (r.z and r.r partials
+// of the new r, in precond_parts(m) slots)
 bool launch_seg_sums_pcg(hipStream_t s, const RecMeta& m, double* p, const double* z, int64_t ld, int k,
                          PcgState* st, double* S0, double* S1, int64_t ldn);
 bool launch_pcg_xr_precond(hipStream_t s, const RecMeta& m, const double* gdiag, const double* theta_cols,
-                           PcgState* st, double* x, int64_t ldx, double* r, const double* p, const double* vp,
-                           int64_t ld, int k, double* z, double* part_rz, double* part_rr);
+                           PcgState* st, const double* part_pvp, int np, double* x, int64_t ldx, double* r,
+                           const double* p, const double* vp, int64_t ld, int k, double* z, double* part_rz,
+                           double* part_rr);
 void launch_record_out(hipStream_t s, const RecMeta& m, const double* x, int64_t ldx, int k,
                        const double* GZ, const double* S0, const double* S1, int64_t ldn,
                        const double* theta_cols, double* y, int64_t ldy, double* part_dot,

@@ -99,20 +99,26 @@ __global__ __launch_bounds__(kBlock) void k_seg_sums(int32_t nblk, const int64_t
 // the order j = lane, lane + 64, ... (slots j >= nparts read as 0), then a fixed
 // shuffle tree (deterministic). Ends with a barrier.
 constexpr int kPcgThreads = 1024;  // one-block PCG kernels: 16 waves, so k <= 64 columns take <= 4 rounds
-__device__ void wave_col_sums(const double* __restrict__ part, int nparts, int k, double* sums) {
+// The sum of column c's partials by one wave (valid in lane 0): lane l adds slots
+// l, l + 64, ... in order, then the shuffle tree.
+__device__ inline double wave_col_sum(const double* __restrict__ part, int nparts, int c) {
     const int lane = threadIdx.x & 63;
     constexpr int kPer = kMaxParts / 64;
+    double x[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        const int j = lane + 64 * i;
+        x[i] = j < nparts ? part[c * kMaxParts + j] : 0.0;
+    }
+    double v = 0.0;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) v += x[i];
+    return wave_sum(v);
+}
+__device__ void wave_col_sums(const double* __restrict__ part, int nparts, int k, double* sums) {
+    const int lane = threadIdx.x & 63;
     for (int c = threadIdx.x >> 6; c < k; c += blockDim.x >> 6) {
-        double x[kPer];
-#pragma unroll
-        for (int i = 0; i < kPer; ++i) {
-            const int j = lane + 64 * i;
-            x[i] = j < nparts ? part[c * kMaxParts + j] : 0.0;
-        }
-        double v = 0.0;
-#pragma unroll
-        for (int i = 0; i < kPer; ++i) v += x[i];
-        v = wave_sum(v);
+        const double v = wave_col_sum(part, nparts, c);
         if (lane == 0) sums[c] = v;
     }
     __syncthreads();
@@ -424,20 +430,37 @@ __global__ __launch_bounds__(kBlock) void k_precond_tiled(
     }
 }
 
-// One PCG step after alpha, fused: x += alpha p and r -= alpha Vp per active column
-// (PCGsolver.cpp:1383-1386), then z = Q^-1 r by the preconditioner of k_precond_tiled on
-// the updated r, with the r.r and r.z partials of the new r. Frozen columns (per-column
-// rule) only give their r.r partials. The convergence test and beta follow in one
-// single-block kernel (k_pcg_check_beta): a column that the test freezes has its z
-// computed here but never used.
+// One PCG step after V p, fused: alpha = r.z / p.Vp (every workgroup reduces the p.Vp
+// partials of its column itself, in the order of wave_col_sums, so all of them hold the
+// same alpha; workgroup 0 of the column records it), x += alpha p and r -= alpha Vp per
+// active column (PCGsolver.cpp:1381-1386), then z = Q^-1 r by the preconditioner of
+// k_precond_tiled on the updated r, with the r.r and r.z partials of the new r. Frozen
+// columns (per-column rule) only give their r.r partials. The convergence test and beta
+// follow in one single-block kernel (k_pcg_check_beta), the only grid-wide reduction of
+// an iteration: a column that the test freezes has its z computed here but never used.
 __global__ __launch_bounds__(kBlock) void k_pcg_xr_precond(
-    RecMeta m, const double* __restrict__ gdiag, const double* __restrict__ th, const PcgState* stt,
-    double* __restrict__ x, int64_t ldx, double* __restrict__ r, const double* __restrict__ p,
-    const double* __restrict__ vp, int64_t ld, double* __restrict__ z, double* part_rz, double* part_rr) {
+    RecMeta m, const double* __restrict__ gdiag, const double* __restrict__ th, PcgState* stt,
+    const double* __restrict__ part_pvp, int np, double* __restrict__ x, int64_t ldx,
+    double* __restrict__ r, const double* __restrict__ p, const double* __restrict__ vp, int64_t ld,
+    double* __restrict__ z, double* part_rz, double* part_rr) {
     if (stt->done) return;
     const int c = blockIdx.y;
     const bool act = stt->active[c] != 0;
-    const double alpha = stt->alpha[c];
+    __shared__ double alpha_s;
+    if (threadIdx.x < 64) {
+        const double sum = wave_col_sum(part_pvp, np, c);
+        if (threadIdx.x == 0) {
+            const double a = act ? stt->rs[c] / sum : 0.0;
+            alpha_s = a;
+            if (blockIdx.x == 0) {
+                stt->alpha[c] = a;
+                if (stt->hist && act && stt->iters < stt->hist_cap)
+                    stt->hist[(1 + 2 * (int64_t)stt->iters) * kMaxCols + c] = a;
+            }
+        }
+    }
+    __syncthreads();
+    const double alpha = alpha_s;
     extern __shared__ double sm[];
     double* sr = sm;               // r of the tile's records (updated)
     double* st = sm + m.cap;       // t of the tile's records
@@ -1002,12 +1025,13 @@ bool launch_seg_sums_pcg(hipStream_t s, const RecMeta& m, double* p, const doubl
 }
 
 bool launch_pcg_xr_precond(hipStream_t s, const RecMeta& m, const double* gdiag, const double* theta_cols,
-                           PcgState* st, double* x, int64_t ldx, double* r, const double* p, const double* vp,
-                           int64_t ld, int k, double* z, double* part_rz, double* part_rr) {
+                           PcgState* st, const double* part_pvp, int np, double* x, int64_t ldx, double* r,
+                           const double* p, const double* vp, int64_t ld, int k, double* z, double* part_rz,
+                           double* part_rr) {
     if (m.cap == 0) return false;
     dim3 g(precond_parts(m), k);
     k_pcg_xr_precond<<<g, kBlock, sizeof(double) * (2 * m.cap + 2 * kTileBlocks), s>>>(
-        m, gdiag, theta_cols, st, x, ldx, r, p, vp, ld, z, part_rz, part_rr);
+        m, gdiag, theta_cols, st, part_pvp, np, x, ldx, r, p, vp, ld, z, part_rz, part_rr);
     LAUNCH_CHECK();
     return true;
 }

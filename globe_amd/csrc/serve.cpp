@@ -283,14 +283,14 @@ void iterate(globe_ctx* c, ServeState& S, int k) {
     launch_fill(s, S0, (int64_t)n * k, 0.0);
     launch_fill(s, S1, (int64_t)n * k, 0.0);
     // p = z + beta p of the active slots fused into the sums (an admitted slot starts
-    // with p = z and beta = 0), the x/r update fused with the preconditioner, and the
+    // with p = z and beta = 0), alpha with the x/r update and the preconditioner, and the
     // convergence test with beta in one block (the slots' records are always tiled)
     launch_seg_sums_pcg(s, m, S.P.as<double>(), S.Z.as<double>(), 0, k, st, S0, S1, n);
     genetic_product(c, S0, k, GZ, S.o.precision, skip, th);
     launch_record_out(s, m, S.P.as<double>(), 0, k, GZ, S0, S1, n, th, S.VP.as<double>(), 0, part_a, skip);
-    launch_pcg_alpha(s, st, part_a, parts_for(cap), k);
-    launch_pcg_xr_precond(s, m, c->d_gdiag.as<double>(), th, st, S.X.as<double>(), 0, S.R.as<double>(),
-                          S.P.as<double>(), S.VP.as<double>(), 0, k, S.Z.as<double>(), part_c, part_b);
+    launch_pcg_xr_precond(s, m, c->d_gdiag.as<double>(), th, st, part_a, parts_for(cap), S.X.as<double>(), 0,
+                          S.R.as<double>(), S.P.as<double>(), S.VP.as<double>(), 0, k, S.Z.as<double>(), part_c,
+                          part_b);
     launch_pcg_check_beta(s, st, part_b, precond_parts(m), part_c, precond_parts(m), k);
     ++S.global_iter;
 }
