@@ -157,6 +157,8 @@ void launch_pcg_admit(hipStream_t s, PcgState* st, int j, const double* part_bb,
 void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* part_pvp, int nparts, int k);
 void launch_pcg_check(hipStream_t s, PcgState* st, const double* part_rr, int nparts, int k);
 void launch_pcg_beta(hipStream_t s, PcgState* st, const double* part_rz, int nparts, int k);
+// column j of a block PCG stops iterating
+void launch_pcg_deactivate(hipStream_t s, PcgState* st, int j);
 // the convergence test, then beta unless the solve has ended
 void launch_pcg_check_beta(hipStream_t s, PcgState* st, const double* part_rr, int nparts_rr,
                            const double* part_rz, int nparts_rz, int k);

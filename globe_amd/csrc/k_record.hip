@@ -814,6 +814,11 @@ __global__ void k_set_column(int j, int64_t off, int32_t ntiles, int64_t nrec, d
     theta[5 * j + 4] = t4;
 }
 
+// Column j stops iterating (the serving loop's iteration cap).
+__global__ void k_pcg_deactivate(PcgState* st, int j) {
+    if (threadIdx.x == 0) st->active[j] = 0;
+}
+
 // One-block reductions of PCG's dot partials: alpha after V p, the convergence test
 // after the x/r update, beta after the preconditioner.
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_alpha(PcgState* st, const double* __restrict__ part_pvp,
@@ -1140,6 +1145,11 @@ void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* part_pvp, int n
 
 void launch_pcg_check(hipStream_t s, PcgState* st, const double* part_rr, int nparts, int k) {
     k_pcg_check<<<1, kPcgThreads, 0, s>>>(st, part_rr, nparts, k);
+    LAUNCH_CHECK();
+}
+
+void launch_pcg_deactivate(hipStream_t s, PcgState* st, int j) {
+    k_pcg_deactivate<<<1, 64, 0, s>>>(st, j);
     LAUNCH_CHECK();
 }
 

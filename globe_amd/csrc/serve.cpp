@@ -11,10 +11,11 @@
 // and a pool range of its size is.
 //
 // Time: admissions are staged one step ahead. After the step's iteration is enqueued,
-// the next requests' data is uploaded on a copy stream while the GPU iterates; the next
-// step joins them to the block (the compute stream waits on the copies' event, then
-// starts each new column's recurrence) before its iteration. Finished columns are copied
-// out after each iteration and their ranges returned to the pool.
+// the next requests' data is uploaded on a copy stream while the GPU iterates, and the
+// step returns without waiting for the iteration. The next step waits for it, copies out
+// the columns it finished and returns their ranges to the pool, joins the staged
+// requests to the block (the compute stream waits on the copies' event, then starts each
+// new column's recurrence) and enqueues its own iteration.
 //
 // The genotype contraction takes all occupied columns at once; its fixed-point scale,
 // the PCG scalars and the record structure are per column, and the reductions of a column
@@ -71,6 +72,7 @@ struct ServeState {
     int global_iter = 0;
     int64_t peak_in_use = 0, waits_for_room = 0;
     bool staged_pending = false;
+    bool inflight = false;  // an iteration enqueued by the last step, not yet waited for
     std::chrono::steady_clock::time_point t0;
     globe::PcgState* h_state = nullptr;
 
@@ -394,6 +396,51 @@ int globe_serve_step(globe_ctx* c, int32_t* finished, int32_t* active, int32_t* 
     API_BEGIN(c)
     ServeState& S = server(c);
     hipStream_t s = c->stream;
+    int fin = 0;
+    // The iteration the previous step enqueued: wait for it, then retire the columns its
+    // state shows finished (converged, or at the iteration cap). The step returns without
+    // waiting for its own iteration, so the caller's work between steps (taking results,
+    // its intake) overlaps the GPU.
+    if (S.inflight) {
+        S.inflight = false;
+        HIP_CHECK(hipStreamSynchronize(s));
+        const double t = S.now();
+        std::vector<int> retired;
+        const double th_free[5] = {1.0, 1.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < S.slots; ++j) {
+            if (S.slot_state[j] != ServeState::kBusy) continue;
+            const int its = S.global_iter - S.admit_iter[j];
+            const bool conv = S.h_state->active[j] == 0;
+            if (!conv && its < S.o.max_iter) continue;
+            ServeState::Request& r = S.slot_req[j];
+            ServeState::Done d;
+            d.x.resize(r.nrec);
+            HIP_CHECK(hipMemcpyAsync(d.x.data(), S.X.as<double>() + S.slot_off[j], sizeof(double) * r.nrec,
+                                     hipMemcpyDeviceToHost, s));
+            d.res.id = r.id;
+            d.res.iters = its;
+            d.res.converged = conv ? 1 : 0;
+            d.res.ratio = S.h_state->ratio[j];
+            d.res.t_submit = r.t_submit;
+            d.res.t_admit = S.t_admit[j];
+            d.res.t_done = t;
+            // retire the column: no records, no further work
+            launch_set_column(s, j, 0, 0, 0, th_free, S.col_off.as<int64_t>(), S.ntiles_col.as<int32_t>(),
+                              S.nrec_col.as<int64_t>(), S.theta.as<double>());
+            if (!conv) launch_pcg_deactivate(s, S.state.as<PcgState>(), j);
+            S.done.emplace(r.id, std::move(d));
+            retired.push_back(j);
+            ++fin;
+        }
+        if (!retired.empty()) {
+            HIP_CHECK(hipStreamSynchronize(s));
+            for (int j : retired) {
+                S.release(S.slot_off[j], S.slot_req[j].span);
+                S.slot_state[j] = ServeState::kFree;
+                S.slot_req[j] = ServeState::Request();
+            }
+        }
+    }
     auto busy_cols = [&]() {
         int k = 0;
         for (int j = 0; j < S.slots; ++j)
@@ -407,7 +454,6 @@ int globe_serve_step(globe_ctx* c, int32_t* finished, int32_t* active, int32_t* 
         }
     admit_staged(c, S);
     const int k = busy_cols();
-    int fin = 0;
     if (k > 0) {
         // mixed precision: each slot's digits from its own residual ratio (4 in its first
         // iteration), none for a slot that holds no request (slot k - 1 is busy, so the
@@ -433,53 +479,10 @@ int globe_serve_step(globe_ctx* c, int32_t* finished, int32_t* active, int32_t* 
         }
         iterate(c, S, k);
         HIP_CHECK(hipMemcpyAsync(S.h_state, S.state.p, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+        S.inflight = true;
     }
     // the next requests upload while the iteration runs
     while (stage_next(S)) {
-    }
-    if (k > 0) {
-        HIP_CHECK(hipStreamSynchronize(s));
-        const double t = S.now();
-        std::vector<int> retired;
-        const double th_free[5] = {1.0, 1.0, 0.0, 0.0, 0.0};
-        for (int j = 0; j < k; ++j) {
-            if (S.slot_state[j] != ServeState::kBusy) continue;
-            const int its = S.global_iter - S.admit_iter[j];
-            const bool conv = S.h_state->active[j] == 0;
-            if (!conv && its < S.o.max_iter) continue;
-            ServeState::Request& r = S.slot_req[j];
-            ServeState::Done d;
-            d.x.resize(r.nrec);
-            HIP_CHECK(hipMemcpyAsync(d.x.data(), S.X.as<double>() + S.slot_off[j], sizeof(double) * r.nrec,
-                                     hipMemcpyDeviceToHost, s));
-            d.res.id = r.id;
-            d.res.iters = its;
-            d.res.converged = conv ? 1 : 0;
-            d.res.ratio = S.h_state->ratio[j];
-            d.res.t_submit = r.t_submit;
-            d.res.t_admit = S.t_admit[j];
-            d.res.t_done = t;
-            // retire the column: no records, no further work
-            launch_set_column(s, j, 0, 0, 0, th_free, S.col_off.as<int64_t>(), S.ntiles_col.as<int32_t>(),
-                              S.nrec_col.as<int64_t>(), S.theta.as<double>());
-            if (!conv) {
-                int32_t off = 0;
-                HIP_CHECK(hipMemcpyAsync(&S.state.as<PcgState>()->active[j], &off, sizeof(int32_t),
-                                         hipMemcpyHostToDevice, s));
-                HIP_CHECK(hipStreamSynchronize(s));
-            }
-            S.done.emplace(r.id, std::move(d));
-            retired.push_back(j);
-            ++fin;
-        }
-        if (!retired.empty()) {
-            HIP_CHECK(hipStreamSynchronize(s));
-            for (int j : retired) {
-                S.release(S.slot_off[j], S.slot_req[j].span);
-                S.slot_state[j] = ServeState::kFree;
-                S.slot_req[j] = ServeState::Request();
-            }
-        }
     }
     if (finished) *finished = fin;
     if (active) {
