@@ -139,6 +139,7 @@ def load_library(path: str = LIB_PATH):
         "globe_variant_counts": ([P, P], C.c_int),
         "globe_set_mixed_digits": ([P, C.c_int], C.c_int),
         "globe_set_graphs": ([P, C.c_int], C.c_int),
+        "globe_set_row_gather": ([P, C.c_int], C.c_int),
         "globe_profile_read": ([P, C.c_int, P], C.c_int),
         "globe_aireml_step": ([P, P, P, dbl, P], C.c_int),
         "globe_aireml_step_reduced": ([P, P, P, dbl, P], C.c_int),
@@ -481,6 +482,11 @@ class Globe:
         """PCG iterations replayed from captured HIP graphs (on, the default) or enqueued
         kernel by kernel (off); the results are identical."""
         self._check(self.L.globe_set_graphs(self.h, int(on)))
+
+    def set_row_gather(self, on=True):
+        """Serving loop: form only the requests' individuals' rows in the genotype
+        product's individual-major pass (on, the default) or every row (off)."""
+        self._check(self.L.globe_set_row_gather(self.h, int(on)))
 
     def device_memory(self):
         """(free, total) bytes of device memory."""

@@ -220,7 +220,8 @@ void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int prec
             int c1 = c0, J = 0;
             while (c1 < k && J + L[c1] <= 128) J += L[c1++];
             mfma_geno_product(c, S0 + (size_t)c0 * n, n, c1 - c0, GZ + (size_t)c0 * n, n, skip, L.data() + c0,
-                              th + (size_t)c0 * 5);
+                              th + (size_t)c0 * 5, nullptr, 0, c->gather_n > 0 ? c->gather_idx : nullptr,
+                              c->gather_n);
             c0 = c1;
         }
         return;
@@ -1442,6 +1443,12 @@ int globe_set_mixed_digits(globe_ctx* c, int digits) {
 int globe_set_graphs(globe_ctx* c, int on) {
     API_BEGIN(c)
     c->graphs_on = on != 0;
+    API_END(c)
+}
+
+int globe_set_row_gather(globe_ctx* c, int on) {
+    API_BEGIN(c)
+    c->gather_on = on != 0;
     API_END(c)
 }
 

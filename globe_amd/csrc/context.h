@@ -79,6 +79,12 @@ struct globe_ctx {
 
     // PCG iterations replayed from captured HIP graphs (globe_set_graphs; on by default)
     bool graphs_on = true;
+    // Rows of the genotype product's individual-major pass that a caller needs (the
+    // serving loop: its requests' individuals), ng_rows entries (0: all rows); thin
+    // contractions form only these rows of G S0. globe_set_row_gather switches it off.
+    const int32_t* gather_idx = nullptr;
+    int64_t gather_n = 0;
+    bool gather_on = true;
 
     // PCG workspaces and state
     globe::DevBuf p_state, p_r, p_z, p_p, p_vp;
@@ -191,10 +197,13 @@ int relaxed_limbs(double ratio, double tol);
 // packed-genotype contraction on the int8 matrix cores (k_mfma.hip): GZ = sig2g G S0
 // with col_limbs[c] int8 digits (1..7) for column c of the right-hand side, at most 128
 // digit columns in all (U_given: pass 1 already done by the caller, u = U_given, m x k;
-// S0 unused; then only rows row0 .. n - 1 of GZ are formed)
+// S0 unused; then only rows row0 .. n - 1 of GZ are formed). ridx (device, ng entries,
+// ng a multiple of 512): when the contraction is thin (at most 16 digit columns), only
+// rows ridx[0 .. ng) of GZ are formed (the other rows keep what they held)
 void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, double* GZ,
                        int64_t ldg, const int* skip, const int8_t* col_limbs, const double* theta_cols,
-                       const double* U_given = nullptr, int64_t row0 = 0);
+                       const double* U_given = nullptr, int64_t row0 = 0, const int32_t* ridx = nullptr,
+                       int64_t ng = 0);
 // tune the packed-genotype contraction for every digit-column width a block of up to
 // max_cols columns reaches at `precision` (theta_cols: [max_cols][5] components)
 void pretune_geno_product(globe_ctx* c, int max_cols, int precision, const double* theta_cols);

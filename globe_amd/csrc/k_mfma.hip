@@ -132,6 +132,38 @@ __device__ inline void thin_issue(const uint8_t* __restrict__ arow, int64_t pitc
     }
 }
 
+// thin_issue for gathered rows: arq[q] is this lane's row of fragment q at super-step 0
+// of the row block (the rows of a row block are any rows of the layout, listed by the
+// caller), koff the stage's packed-byte offset within a row.
+template <int W, int RT, int SS>
+__device__ inline void thin_issue_ga(const uint8_t* const (&arq)[RT], int64_t koff,
+                                     const uint8_t* __restrict__ bstage, uint8_t* __restrict__ slot,
+                                     int wave, int lane, int live) {
+    using C = ThinCfg<W, RT, SS>;
+#pragma unroll
+    for (int q = 0; q < RT; ++q)
+#pragma unroll
+        for (int h = 0; h < SS; ++h) {
+            const int f = (wave * RT + q) * SS + h;
+            __builtin_amdgcn_global_load_lds((const void*)(arq[q] + koff + 64 * h), (lds_void*)(slot + f * 1024),
+                                             16, 0, 0);
+        }
+    const bool dig = (lane & 15) < live;
+    if constexpr (C::kDigKB >= W) {
+#pragma unroll
+        for (int d = 0; d < C::kDigPerWave; ++d) {
+            const int ch = wave * C::kDigPerWave + d;
+            if (dig)
+                __builtin_amdgcn_global_load_lds((const void*)(bstage + plan::dig_lane(ch, lane)),
+                                                 (lds_void*)(slot + C::kA + ch * 1024), 16, 0, 0);
+        }
+    } else {
+        if (wave < C::kDigKB && dig)
+            __builtin_amdgcn_global_load_lds((const void*)(bstage + plan::dig_lane(wave, lane)),
+                                             (lds_void*)(slot + C::kA + wave * 1024), 16, 0, 0);
+    }
+}
+
 template <int W, int RT, int SS>
 __device__ inline void thin_mma(const uint8_t* __restrict__ slot, int wave, int lane,
                                 v4i (&acc)[RT]) {
@@ -180,12 +212,14 @@ __device__ inline void thin_stage_barrier(int wave) {
 // Stream-K over the units (row block, stage) of plan::sk_block_of: every block streams
 // the same number of bytes, and the partial slots of each row block are exactly P, so the
 // epilogue's sum does not depend on the grid size.
-template <int W, int RT, int NS, int SS>
+// GA: the rows of row block r are rows ridx[r kRB ..] of the layout (ridx lists kRB R of
+// them), not rows r kRB ..; row i of the output is the product of row ridx[i].
+template <int W, int RT, int NS, int SS, bool GA>
 __global__ __launch_bounds__(W * 64) void k_geno_thin(const uint8_t* __restrict__ rows, int64_t pitch,
                                                       int64_t nrows, int64_t n_st, int64_t U, int64_t G,
                                                       int P, const uint8_t* __restrict__ Bq,
                                                       int32_t* __restrict__ out, int live,
-                                                      const int* skip) {
+                                                      const int* skip, const int32_t* __restrict__ ridx) {
     if (skip && *skip) return;
     using C = ThinCfg<W, RT, SS>;
     __shared__ __attribute__((aligned(1024))) uint8_t lds[NS * C::kSlot];
@@ -199,9 +233,24 @@ __global__ __launch_bounds__(W * 64) void k_geno_thin(const uint8_t* __restrict_
     const int64_t lane_off = plan::thin_lane_off(wave, RT, rr, g, pitch);
 
     plan::SkCursor cur(lo, nt, n_st);
+    // gathered rows: the lane's row of each fragment, looked up when the copies move to
+    // another row block (once per n_st stages)
+    const uint8_t* arq[RT];
+    int64_t arq_r = -1;
     auto issue_next = [&](int slot) {
-        thin_issue<W, RT, SS>(rows + plan::thin_unit_off(cur.r, kRB, cur.t, SS, pitch) + lane_off, pitch,
-                              Bq + plan::dig_stage(cur.t, C::kB), lds + slot * C::kSlot, wave, lane, live);
+        if constexpr (GA) {
+            if (cur.r != arq_r) {
+                arq_r = cur.r;
+#pragma unroll
+                for (int q = 0; q < RT; ++q)
+                    arq[q] = rows + (int64_t)ridx[cur.r * kRB + wave * 16 * RT + q * 16 + rr] * pitch + 16 * g;
+            }
+            thin_issue_ga<W, RT, SS>(arq, 64 * SS * cur.t, Bq + plan::dig_stage(cur.t, C::kB), lds + slot * C::kSlot,
+                                     wave, lane, live);
+        } else {
+            thin_issue<W, RT, SS>(rows + plan::thin_unit_off(cur.r, kRB, cur.t, SS, pitch) + lane_off, pitch,
+                                  Bq + plan::dig_stage(cur.t, C::kB), lds + slot * C::kSlot, wave, lane, live);
+        }
         cur.advance();
     };
     // consume cursor
@@ -679,7 +728,8 @@ __global__ __launch_bounds__(256) void k_epilogue(const int32_t* __restrict__ ou
                                                   const double* __restrict__ inv,
                                                   const double* __restrict__ th, double inv_m,
                                                   double* __restrict__ res, int64_t ldr,
-                                                  double* part, const int* skip) {
+                                                  double* part, const int* skip,
+                                                  const int32_t* __restrict__ ridx) {
     if (skip && *skip) return;
     __shared__ long long dsum[kEpiRows * 128];
     __shared__ double tile[kEpiRows][kMaxCols + 1];
@@ -734,7 +784,7 @@ __global__ __launch_bounds__(256) void k_epilogue(const int32_t* __restrict__ ou
         for (int q = threadIdx.x; q < kEpiRows * k; q += 256) {
             const int c = q / kEpiRows, rl = q % kEpiRows;
             const int64_t r = r0 + rl;
-            if (r < rows) res[c * ldr + r] = tile[rl][c];
+            if (r < rows) res[c * ldr + (ridx ? (int64_t)ridx[r] : r)] = tile[rl][c];
         }
         if (mode == 0) {
             // running per-column maxima (one wave per column, fixed order)
@@ -762,18 +812,23 @@ using plan::wide_fits;
 
 template <int Q>
 void launch_thin_q(hipStream_t s, const Variant& v, const uint8_t* rows, int64_t pitch, int64_t nrows,
-                   int64_t nss, int64_t R, const uint8_t* Bq, int32_t* out, int live, const int* skip) {
+                   int64_t nss, int64_t R, const uint8_t* Bq, int32_t* out, int live, const int* skip,
+                   const int32_t* ridx) {
     constexpr ThinShape t = kThin[Q];
     const int64_t nst = nss / t.SS;
-    k_geno_thin<t.W, t.RT, t.NS, t.SS><<<(unsigned)v.G, t.W * 64, 0, s>>>(rows, pitch, nrows, nst, R * nst,
-                                                                         v.G, v.S, Bq, out, live, skip);
+    if (ridx)
+        k_geno_thin<t.W, t.RT, t.NS, t.SS, true><<<(unsigned)v.G, t.W * 64, 0, s>>>(
+            rows, pitch, nrows, nst, R * nst, v.G, v.S, Bq, out, live, skip, ridx);
+    else
+        k_geno_thin<t.W, t.RT, t.NS, t.SS, false><<<(unsigned)v.G, t.W * 64, 0, s>>>(
+            rows, pitch, nrows, nst, R * nst, v.G, v.S, Bq, out, live, skip, nullptr);
 }
 
 template <int Q>
 int thin_blocks_per_cu() {
     constexpr ThinShape t = kThin[Q];
     int nb = 0;
-    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_geno_thin<t.W, t.RT, t.NS, t.SS>,
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_geno_thin<t.W, t.RT, t.NS, t.SS, false>,
                                                            t.W * 64, 0));
     return nb;
 }
@@ -837,22 +892,23 @@ void launch_wide(hipStream_t s, const Variant& v, unsigned grid, const uint8_t* 
 
 void launch_variant(hipStream_t s, bool wide, int JT, const Variant& v, const uint8_t* rows,
                     int64_t pitch, int64_t nrows, int64_t nss, const uint8_t* Bq, int32_t* out, int live,
-                    const int* skip) {
+                    const int* skip, const int32_t* ridx = nullptr) {
     const int64_t R = (nrows + rows_per_block(wide, v.cfg) - 1) / rows_per_block(wide, v.cfg);
     const int64_t per = (nss + v.S - 1) / v.S;
     const unsigned grid = (unsigned)(R * v.S);
+    require(!ridx || !wide, GLOBE_E_INVALID, "gathered rows take the thin contraction only");
     if (!wide) {
         require(nss % kThin[v.cfg].SS == 0 && v.G >= 1 && v.G <= R * (nss / kThin[v.cfg].SS),
                 GLOBE_E_INVALID, "thin shape needs whole stages per row and at most one block per unit");
         switch (v.cfg) {
-            case 0: launch_thin_q<0>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip); break;
-            case 1: launch_thin_q<1>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip); break;
-            case 2: launch_thin_q<2>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip); break;
-            case 3: launch_thin_q<3>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip); break;
-            case 4: launch_thin_q<4>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip); break;
-            case 5: launch_thin_q<5>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip); break;
-            case 6: launch_thin_q<6>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip); break;
-            default: launch_thin_q<7>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip); break;
+            case 0: launch_thin_q<0>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip, ridx); break;
+            case 1: launch_thin_q<1>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip, ridx); break;
+            case 2: launch_thin_q<2>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip, ridx); break;
+            case 3: launch_thin_q<3>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip, ridx); break;
+            case 4: launch_thin_q<4>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip, ridx); break;
+            case 5: launch_thin_q<5>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip, ridx); break;
+            case 6: launch_thin_q<6>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip, ridx); break;
+            default: launch_thin_q<7>(s, v, rows, pitch, nrows, nss, R, Bq, out, live, skip, ridx); break;
         }
     } else if (JT == 1) {
         launch_wide<1>(s, v, grid, rows, pitch, nrows, nss, per, R, Bq, out, skip);
@@ -911,7 +967,7 @@ static plan::Geometry product_geometry(int J, int64_t m, int64_t n, int64_t pitc
 // int8 digits.
 void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, double* GZ, int64_t ldg,
                        const int* skip, const int8_t* col_limbs, const double* theta_cols,
-                       const double* U_given, int64_t row0) {
+                       const double* U_given, int64_t row0, const int32_t* ridx, int64_t ng) {
     hipStream_t s = c->stream;
     const int64_t n = c->n, m = c->m;
     require(k >= 1 && k <= kMaxCols, GLOBE_E_INVALID, "1 to 64 columns");
@@ -939,10 +995,15 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
         c->cu_count = prop.multiProcessorCount;
     }
     const int ncu = c->cu_count;
-    // pass 2 forms rows r0 .. n - 1 (r0 > 0 only with U_given); fewer rows can take more
-    // splits, so the partial buffer is sized for the row count actually contracted
-    const int64_t r0 = U_given ? row0 : 0, n2 = n - r0;
-    const plan::Geometry geo = product_geometry(J, m, n, c->pitch_s, c->pitch_i, r0, ncu);
+    // pass 2 forms rows r0 .. n - 1 (r0 > 0 only with U_given), or, gathered, the ng rows
+    // ridx[0 .. ng) (thin contractions only: a wide one forms every row); fewer rows can
+    // take more splits, so the partial buffer is sized for the row count actually contracted
+    const bool gather = ridx != nullptr && !U_given && J <= 16;
+    if (gather)
+        require(ng > 0 && ng % kRowPad == 0 && ng <= c->n + kRowPad, GLOBE_E_INVALID,
+                "gathered rows: a positive multiple of 512, at most n + 512");
+    const int64_t r0 = U_given ? row0 : 0, n2 = gather ? ng : n - r0;
+    const plan::Geometry geo = product_geometry(J, m, gather ? ng : n, c->pitch_s, c->pitch_i, r0, ncu);
     const bool wide = geo.wide;
     const int JT = geo.JT, J16 = geo.J16, nfrag = geo.nfrag;
     const int64_t Kpad1 = geo.pass[0].Kpad, Kpad2 = geo.pass[1].Kpad;
@@ -978,6 +1039,23 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     bool fixed[2] = {false, false};
     if (!U_given) fixed[0] = preset(0, m, Kpad1, nss1, vp[0]);
     fixed[1] = preset(1, n2, Kpad2, nss2, vp[1]);
+    if (gather && !fixed[1]) {
+        // gathered rows are never timed (their count changes from call to call): the thin
+        // shape tuned for all n rows, with that shape's stream-K grid nearest two rounds of
+        // resident blocks
+        const std::vector<Variant> cand = candidates(false, 1, n2, nss2, ncu);
+        auto full = c->tuned.find(key_of(1, n, Kpad2));
+        const int cfg = full != c->tuned.end() ? full->second.cfg : cand[0].cfg;
+        const int64_t target = 2 * (int64_t)ncu * thin_occupancy(cfg);
+        vp[1] = cand[0];
+        bool have = false;
+        for (const Variant& w : cand) {
+            if (w.cfg != cfg) continue;
+            if (!have || std::llabs(w.G - target) < std::llabs(vp[1].G - target)) vp[1] = w;
+            have = true;
+        }
+        fixed[1] = true;
+    }
     const size_t bq = (size_t)geo.bq_bytes;
     const size_t ob = noalloc ? (size_t)std::max((U_given ? 0 : vp[0].S) * m, (int64_t)vp[1].S * n2) * J16 * 4
                               : (size_t)geo.part_bytes;
@@ -997,6 +1075,27 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     // with it, since more splits mean more partials to combine)
     auto contract = [&](int pass, const uint8_t* rows, int64_t pitch, int64_t nrows, int64_t K,
                         int64_t Kpad, int64_t nss, const auto& epi) -> int {
+        if (pass == 1 && gather) {
+            // timed like a tuned launch when profiling
+            globe_ctx::ProfEvent e{};
+            const bool prof = c->prof_on && !noalloc;
+            if (prof) {
+                HIP_CHECK(hipEventCreate(&e.a));
+                HIP_CHECK(hipEventCreate(&e.b));
+                e.pass = pass;
+                e.iter = c->prof_iter;
+                e.bytes = (double)nrows * (double)((K + 3) / 4);
+                e.flops = 2.0 * (double)nrows * (double)K * k;
+                e.issued = 2.0 * (double)nrows * (double)Kpad * J16;
+                HIP_CHECK(hipEventRecordWithFlags(e.a, s, hipEventRecordExternal));
+            }
+            launch_variant(s, wide, JT, vp[pass], rows, pitch, nrows, nss, Bq, out, J, skip, ridx);
+            if (prof) {
+                HIP_CHECK(hipEventRecordWithFlags(e.b, s, hipEventRecordExternal));
+                c->prof.push_back(e);
+            }
+            return vp[pass].S;
+        }
         if (fixed[pass] && (c->force_variant >= 0 || noalloc)) {
             launch_variant(s, wide, JT, vp[pass], rows, pitch, nrows, nss, Bq, out, J, skip);
             return vp[pass].S;
@@ -1089,7 +1188,7 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     auto epi1 = [&](int S, const int* sk) {
         k_epilogue<<<(unsigned)eb1, 256, 0, s>>>(out, S, m, J16, lm, k, sh1, 0, Tc,
                                                  c->d_twof.as<double>(), c->d_inv.as<double>(), th,
-                                                 1.0 / (double)m, U, m, part, sk);
+                                                 1.0 / (double)m, U, m, part, sk, nullptr);
         HIP_CHECK(hipGetLastError());
     };
     epi1(contract(0, c->d_snp.as<uint8_t>(), c->pitch_s, m, n, Kpad1, nss1, epi1), skip);
@@ -1104,7 +1203,8 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
     const int64_t eb2 = std::min<int64_t>((n2 + kEpiRows - 1) / kEpiRows, kMaxParts);
     auto epi2 = [&](int S, const int* sk) {
         k_epilogue<<<(unsigned)eb2, 256, 0, s>>>(out, S, n2, J16, lm, k, sh2, 1, Cc, nullptr,
-                                                 nullptr, nullptr, 0.0, GZ + r0, ldg, nullptr, sk);
+                                                 nullptr, nullptr, 0.0, GZ + r0, ldg, nullptr, sk,
+                                                 gather ? ridx : nullptr);
         HIP_CHECK(hipGetLastError());
     };
     epi2(contract(1, c->d_ind.as<uint8_t>() + r0 * c->pitch_i, c->pitch_i, n2, m, Kpad2, nss2, epi2), skip);

@@ -73,6 +73,13 @@ struct ServeState {
     int64_t peak_in_use = 0, waits_for_room = 0;
     bool staged_pending = false;
     bool inflight = false;  // an iteration enqueued by the last step, not yet waited for
+    // rows of the genotype product the block needs: the busy requests' individuals in
+    // ascending order, padded to a multiple of 512 with individuals outside them (whose
+    // rows of G S0 no record reads); rows_n = 0: every row
+    std::vector<int32_t> rows_host;
+    globe::DevBuf rows_dev;
+    int64_t rows_n = 0;
+    bool rows_dirty = true, rows_on = true;
     std::chrono::steady_clock::time_point t0;
     globe::PcgState* h_state = nullptr;
 
@@ -262,8 +269,42 @@ void admit_staged(globe_ctx* c, ServeState& S) {
         S.slot_state[j] = ServeState::kBusy;
         S.admit_iter[j] = S.global_iter;
         S.t_admit[j] = S.now();
+        S.rows_dirty = true;
     }
     S.staged_pending = false;
+}
+
+// The rows of the individual-major pass the busy requests need (their individuals), when
+// they are at most 60% of the cohort; otherwise every row. Called with nothing in flight
+// on the stream that reads rows_dev.
+void update_rows(globe_ctx* c, ServeState& S) {
+    S.rows_dirty = false;
+    S.rows_on = c->gather_on;
+    S.rows_n = 0;
+    if (!c->gather_on || c->gen != 1) return;
+    const int32_t n = c->n;
+    std::vector<uint8_t> in((size_t)n, 0);
+    int64_t cnt = 0;
+    for (int j = 0; j < S.slots; ++j) {
+        if (S.slot_state[j] != ServeState::kBusy) continue;
+        for (const int32_t i : S.slot_req[j].blk_ind)
+            if (!in[(size_t)i]) {
+                in[(size_t)i] = 1;
+                ++cnt;
+            }
+    }
+    const int64_t ng = round_up(cnt, kRowPad);
+    if (cnt == 0 || 10 * cnt > 6 * (int64_t)n || ng > n) return;
+    S.rows_host.clear();
+    S.rows_host.reserve((size_t)ng);
+    for (int32_t i = 0; i < n; ++i)
+        if (in[(size_t)i]) S.rows_host.push_back(i);
+    for (int32_t i = 0; i < n && (int64_t)S.rows_host.size() < ng; ++i)
+        if (!in[(size_t)i]) S.rows_host.push_back(i);
+    S.rows_dev.ensure(sizeof(int32_t) * (size_t)ng);
+    HIP_CHECK(hipMemcpyAsync(S.rows_dev.p, S.rows_host.data(), sizeof(int32_t) * (size_t)ng,
+                             hipMemcpyHostToDevice, c->stream));
+    S.rows_n = ng;
 }
 
 // one iteration of the block PCG over columns [0, k)
@@ -288,7 +329,19 @@ void iterate(globe_ctx* c, ServeState& S, int k) {
     // with p = z and beta = 0), alpha with the x/r update and the preconditioner, and the
     // convergence test with beta in one block (the slots' records are always tiled)
     launch_seg_sums_pcg(s, m, S.P.as<double>(), S.Z.as<double>(), 0, k, st, S0, S1, n);
-    genetic_product(c, S0, k, GZ, S.o.precision, skip, th);
+    {
+        // the individual-major pass forms only the requests' individuals' rows
+        struct RowsGuard {
+            globe_ctx* c;
+            ~RowsGuard() {
+                c->gather_idx = nullptr;
+                c->gather_n = 0;
+            }
+        } rows_guard{c};
+        c->gather_idx = S.rows_dev.as<int32_t>();
+        c->gather_n = S.rows_n;
+        genetic_product(c, S0, k, GZ, S.o.precision, skip, th);
+    }
     launch_record_out(s, m, S.P.as<double>(), 0, k, GZ, S0, S1, n, th, S.VP.as<double>(), 0, part_a, skip);
     launch_pcg_xr_precond(s, m, c->d_gdiag.as<double>(), th, st, part_a, parts_for(cap), S.X.as<double>(), 0,
                           S.R.as<double>(), S.P.as<double>(), S.VP.as<double>(), 0, k, S.Z.as<double>(), part_c,
@@ -439,6 +492,7 @@ int globe_serve_step(globe_ctx* c, int32_t* finished, int32_t* active, int32_t* 
                 S.slot_state[j] = ServeState::kFree;
                 S.slot_req[j] = ServeState::Request();
             }
+            S.rows_dirty = true;
         }
     }
     auto busy_cols = [&]() {
@@ -453,6 +507,7 @@ int globe_serve_step(globe_ctx* c, int32_t* finished, int32_t* active, int32_t* 
         while (stage_next(S)) {
         }
     admit_staged(c, S);
+    if (S.rows_dirty || S.rows_on != c->gather_on) update_rows(c, S);
     const int k = busy_cols();
     if (k > 0) {
         // mixed precision: each slot's digits from its own residual ratio (4 in its first

@@ -378,6 +378,13 @@ int globe_set_mixed_digits(globe_ctx* ctx, int digits);
  * off: every iteration's kernels are enqueued one by one. Results are identical. */
 int globe_set_graphs(globe_ctx* ctx, int on);
 
+/* ---- serving: rows of the genotype product (no reference counterpart) -------------- */
+/* on (the default): while at most 60% of the cohort's individuals belong to the serving
+ * loop's requests and the contraction is thin (at most 16 digit columns), the
+ * individual-major pass forms only their rows of G S0 (the other rows are never read);
+ * off: every row. Results are identical. */
+int globe_set_row_gather(globe_ctx* ctx, int on);
+
 /* ---- contraction variants (no reference counterpart: validation of the autotuner) ---- */
 /* The first packed-genotype contraction of each shape times every candidate variant (block
  * shape, split-K or stream-K grid) and keeps the fastest; all candidates compute the same
