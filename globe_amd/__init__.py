@@ -479,8 +479,8 @@ class Globe:
         self._check(self.L.globe_set_mixed_digits(self.h, int(digits)))
 
     def set_graphs(self, on=True):
-        """PCG iterations replayed from captured HIP graphs (on, the default) or enqueued
-        kernel by kernel (off); the results are identical."""
+        """PCG iterations replayed from captured HIP graphs (on) or enqueued kernel by
+        kernel (off, the default); the results are identical."""
         self._check(self.L.globe_set_graphs(self.h, int(on)))
 
     def set_row_gather(self, on=True):

@@ -77,8 +77,9 @@ struct globe_ctx {
     int variant_count[2] = {0, 0};
     int cu_count = 0;  // compute units of the device (stream-K grids)
 
-    // PCG iterations replayed from captured HIP graphs (globe_set_graphs; on by default)
-    bool graphs_on = true;
+    // PCG iterations replayed from captured HIP graphs (globe_set_graphs; off by default:
+    // a replay runs no faster than the eager stream on ROCm 7.2)
+    bool graphs_on = false;
     // Rows of the genotype product's individual-major pass that a caller needs (the
     // serving loop: its requests' individuals), ng_rows entries (0: all rows); thin
     // contractions form only these rows of G S0. globe_set_row_gather switches it off.

@@ -12,10 +12,11 @@
 //
 // Time: admissions are staged one step ahead. After the step's iteration is enqueued,
 // the next requests' data is uploaded on a copy stream while the GPU iterates, and the
-// step returns without waiting for the iteration. The next step waits for it, copies out
-// the columns it finished and returns their ranges to the pool, joins the staged
-// requests to the block (the compute stream waits on the copies' event, then starts each
-// new column's recurrence) and enqueues its own iteration.
+// step returns without waiting for the iteration. The next step waits for it, takes the
+// columns it finished out of the block, joins the staged requests (the compute stream
+// waits on the uploads' event, then starts each new column's recurrence) and enqueues
+// its own iteration; only then does it copy out the finished columns' solutions (on the
+// copy stream, while the GPU iterates) and return their ranges to the pool.
 //
 // The genotype contraction takes all occupied columns at once; its fixed-point scale,
 // the PCG scalars and the record structure are per column, and the reductions of a column
@@ -46,7 +47,13 @@ struct ServeState {
         std::vector<double> x;
         globe_serve_result res{};
     };
-    enum SlotState { kFree = 0, kStaged = 1, kBusy = 2 };
+    // kRetiring: finished in the last iteration; its column holds no records any more and
+    // its x is copied out while the next iteration runs, then the slot is free
+    enum SlotState { kFree = 0, kStaged = 1, kBusy = 2, kRetiring = 3 };
+    struct Retire {
+        int slot = 0;
+        globe_serve_result res{};
+    };
     globe_serve_opts o{};
     int slots = 0;
     int64_t cap = 0;        // records of the largest request
@@ -77,9 +84,16 @@ struct ServeState {
     // ascending order, padded to a multiple of 512 with individuals outside them (whose
     // rows of G S0 no record reads); rows_n = 0: every row
     std::vector<int32_t> rows_host;
-    globe::DevBuf rows_dev;
+    globe::DevBuf rows_dev;   // n + 512 entries, allocated at open
     int64_t rows_n = 0;
     bool rows_dirty = true, rows_on = true;
+    // per individual: how many staged, busy or retiring requests hold it; union_cnt: how
+    // many individuals some request holds (a request's counts are added when it is
+    // staged and removed when its slot frees, so the rows the gather takes are a superset
+    // of the busy requests' individuals, which changes no row's value)
+    bool count_rows = false;
+    std::vector<int32_t> ind_refs;
+    int64_t union_cnt = 0;
     std::chrono::steady_clock::time_point t0;
     globe::PcgState* h_state = nullptr;
 
@@ -114,6 +128,17 @@ struct ServeState {
             return off;
         }
         return -1;
+    }
+    void add_rows(const Request& r, int d) {
+        if (!count_rows) return;
+        if (d > 0) {
+            for (const int32_t i : r.blk_ind)
+                if (ind_refs[(size_t)i]++ == 0) ++union_cnt;
+        } else {
+            for (const int32_t i : r.blk_ind)
+                if (--ind_refs[(size_t)i] == 0) --union_cnt;
+        }
+        rows_dirty = true;
     }
     void release(int64_t off, int64_t len) {
         auto it = free_ranges.emplace(off, len).first;
@@ -230,6 +255,7 @@ bool stage_next(ServeState& S) {
     h2d(S.B.as<double>() + off, r.y.data(), sizeof(double) * nrec);
     S.slot_state[j] = ServeState::kStaged;
     S.slot_off[j] = off;
+    S.add_rows(r, +1);
     S.slot_req[j] = std::move(r);  // its host vectors stay alive until the slot retires
     S.staged_pending = true;
     int64_t f = 0;
@@ -274,34 +300,25 @@ void admit_staged(globe_ctx* c, ServeState& S) {
     S.staged_pending = false;
 }
 
-// The rows of the individual-major pass the busy requests need (their individuals), when
-// they are at most 60% of the cohort; otherwise every row. Called with nothing in flight
-// on the stream that reads rows_dev.
+// The rows of the individual-major pass the block needs (the individuals its requests
+// hold), when they are at most 60% of the cohort; otherwise every row. Called with
+// nothing in flight on the stream that reads rows_dev.
 void update_rows(globe_ctx* c, ServeState& S) {
     S.rows_dirty = false;
     S.rows_on = c->gather_on;
     S.rows_n = 0;
-    if (!c->gather_on || c->gen != 1) return;
+    if (!c->gather_on || !S.count_rows || c->gen != 1) return;
     const int32_t n = c->n;
-    std::vector<uint8_t> in((size_t)n, 0);
-    int64_t cnt = 0;
-    for (int j = 0; j < S.slots; ++j) {
-        if (S.slot_state[j] != ServeState::kBusy) continue;
-        for (const int32_t i : S.slot_req[j].blk_ind)
-            if (!in[(size_t)i]) {
-                in[(size_t)i] = 1;
-                ++cnt;
-            }
-    }
+    const int64_t cnt = S.union_cnt;
     const int64_t ng = round_up(cnt, kRowPad);
     if (cnt == 0 || 10 * cnt > 6 * (int64_t)n || ng > n) return;
     S.rows_host.clear();
     S.rows_host.reserve((size_t)ng);
+    const int32_t* refs = S.ind_refs.data();
     for (int32_t i = 0; i < n; ++i)
-        if (in[(size_t)i]) S.rows_host.push_back(i);
+        if (refs[i] > 0) S.rows_host.push_back(i);
     for (int32_t i = 0; i < n && (int64_t)S.rows_host.size() < ng; ++i)
-        if (!in[(size_t)i]) S.rows_host.push_back(i);
-    S.rows_dev.ensure(sizeof(int32_t) * (size_t)ng);
+        if (refs[i] == 0) S.rows_host.push_back(i);
     HIP_CHECK(hipMemcpyAsync(S.rows_dev.p, S.rows_host.data(), sizeof(int32_t) * (size_t)ng,
                              hipMemcpyHostToDevice, c->stream));
     S.rows_n = ng;
@@ -402,6 +419,11 @@ int globe_serve_open(globe_ctx* c, const globe_serve_opts* o) {
     S->theta.ensure(sizeof(double) * 5 * kMaxCols);
     S->state.ensure(sizeof(PcgState));
     for (DevBuf* d : {&S->part_a, &S->part_b, &S->part_c}) d->ensure(sizeof(double) * kMaxParts * kMaxCols);
+    S->count_rows = c->gen == 1;
+    if (S->count_rows) {
+        S->ind_refs.assign((size_t)c->n, 0);
+        S->rows_dev.ensure(sizeof(int32_t) * ((size_t)c->n + kRowPad));
+    }
     hipStream_t s = c->stream;
     HIP_CHECK(hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking));
     HIP_CHECK(hipEventCreateWithFlags(&S->staged_ev, hipEventDisableTiming));
@@ -453,46 +475,36 @@ int globe_serve_step(globe_ctx* c, int32_t* finished, int32_t* active, int32_t* 
     // The iteration the previous step enqueued: wait for it, then retire the columns its
     // state shows finished (converged, or at the iteration cap). The step returns without
     // waiting for its own iteration, so the caller's work between steps (taking results,
-    // its intake) overlaps the GPU.
+    // its intake) overlaps the GPU. A retired column's records leave the block before the
+    // next iteration; its solution no longer changes, so it is copied out on the copy
+    // stream while that iteration runs, and only then does its slot and range free up.
+    std::vector<ServeState::Retire> retiring;
     if (S.inflight) {
         S.inflight = false;
         HIP_CHECK(hipStreamSynchronize(s));
         const double t = S.now();
-        std::vector<int> retired;
         const double th_free[5] = {1.0, 1.0, 0.0, 0.0, 0.0};
         for (int j = 0; j < S.slots; ++j) {
             if (S.slot_state[j] != ServeState::kBusy) continue;
             const int its = S.global_iter - S.admit_iter[j];
             const bool conv = S.h_state->active[j] == 0;
             if (!conv && its < S.o.max_iter) continue;
-            ServeState::Request& r = S.slot_req[j];
-            ServeState::Done d;
-            d.x.resize(r.nrec);
-            HIP_CHECK(hipMemcpyAsync(d.x.data(), S.X.as<double>() + S.slot_off[j], sizeof(double) * r.nrec,
-                                     hipMemcpyDeviceToHost, s));
-            d.res.id = r.id;
-            d.res.iters = its;
-            d.res.converged = conv ? 1 : 0;
-            d.res.ratio = S.h_state->ratio[j];
-            d.res.t_submit = r.t_submit;
-            d.res.t_admit = S.t_admit[j];
-            d.res.t_done = t;
-            // retire the column: no records, no further work
+            const ServeState::Request& r = S.slot_req[j];
+            ServeState::Retire q;
+            q.slot = j;
+            q.res.id = r.id;
+            q.res.iters = its;
+            q.res.converged = conv ? 1 : 0;
+            q.res.ratio = S.h_state->ratio[j];
+            q.res.t_submit = r.t_submit;
+            q.res.t_admit = S.t_admit[j];
+            q.res.t_done = t;
+            // the column holds no records and does no further work
             launch_set_column(s, j, 0, 0, 0, th_free, S.col_off.as<int64_t>(), S.ntiles_col.as<int32_t>(),
                               S.nrec_col.as<int64_t>(), S.theta.as<double>());
             if (!conv) launch_pcg_deactivate(s, S.state.as<PcgState>(), j);
-            S.done.emplace(r.id, std::move(d));
-            retired.push_back(j);
-            ++fin;
-        }
-        if (!retired.empty()) {
-            HIP_CHECK(hipStreamSynchronize(s));
-            for (int j : retired) {
-                S.release(S.slot_off[j], S.slot_req[j].span);
-                S.slot_state[j] = ServeState::kFree;
-                S.slot_req[j] = ServeState::Request();
-            }
-            S.rows_dirty = true;
+            S.slot_state[j] = ServeState::kRetiring;
+            retiring.push_back(q);
         }
     }
     auto busy_cols = [&]() {
@@ -536,6 +548,29 @@ int globe_serve_step(globe_ctx* c, int32_t* finished, int32_t* active, int32_t* 
         HIP_CHECK(hipMemcpyAsync(S.h_state, S.state.p, sizeof(PcgState), hipMemcpyDeviceToHost, s));
         S.inflight = true;
     }
+    // the retired columns' solutions (the iteration just enqueued writes none of their
+    // elements); their ranges return to the pool once the copies are done, ahead of any
+    // upload into them on the same stream
+    if (!retiring.empty()) {
+        for (const ServeState::Retire& q : retiring) {
+            const ServeState::Request& r = S.slot_req[q.slot];
+            ServeState::Done d;
+            d.x.resize(r.nrec);
+            HIP_CHECK(hipMemcpyAsync(d.x.data(), S.X.as<double>() + S.slot_off[q.slot], sizeof(double) * r.nrec,
+                                     hipMemcpyDeviceToHost, S.copy));
+            d.res = q.res;
+            S.done.emplace(q.res.id, std::move(d));
+        }
+        HIP_CHECK(hipStreamSynchronize(S.copy));
+        for (const ServeState::Retire& q : retiring) {
+            const int j = q.slot;
+            S.release(S.slot_off[j], S.slot_req[j].span);
+            S.add_rows(S.slot_req[j], -1);
+            S.slot_state[j] = ServeState::kFree;
+            S.slot_req[j] = ServeState::Request();
+            ++fin;
+        }
+    }
     // the next requests upload while the iteration runs
     while (stage_next(S)) {
     }
@@ -545,9 +580,11 @@ int globe_serve_step(globe_ctx* c, int32_t* finished, int32_t* active, int32_t* 
         for (int j = 0; j < S.slots; ++j) a += S.slot_state[j] == ServeState::kBusy ? 1 : 0;
         *active = a;
     }
-    if (queued) {
+    if (queued) {  // waiting: in the queue, or staged to join at the next step
+        int q = 0;
+        for (int j = 0; j < S.slots; ++j) q += S.slot_state[j] == ServeState::kStaged ? 1 : 0;
         std::lock_guard<std::mutex> lk(S.qmu);
-        *queued = (int32_t)S.queue.size();
+        *queued = (int32_t)S.queue.size() + q;
     }
     API_END(c)
 }

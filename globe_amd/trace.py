@@ -6,7 +6,10 @@ run wrote, this reports for every pair of markers the span between them, the tim
 least one kernel was running (overlapping kernels merged), the idle time split into short
 (< 20 us, launch-to-launch) and long gaps, and the kernels that took the most time.
 
-    python -m globe_amd.trace <trace dir> [--top 8]
+    python -m globe_amd.trace <trace dir> [--top 8] [--gaps 5]
+
+With --gaps, the long gaps of each leg are also grouped by the kernels that end before
+and start after them (a device-to-host copy before a gap is a host round trip).
 
 counters: per-kernel tables of the contraction kernels from ``rocprofv3 --pmc`` runs of
 ``bench.py --calib`` (one directory per pass): clock, MFMA busy share, issued int8 ops,
@@ -58,15 +61,22 @@ def segments(rows, short_gap_ns=20_000):
         busy = 0
         short = 0
         cs = ce = None
-        for s, e, _ in seg:
+        prev = None
+        pairs = defaultdict(lambda: [0, 0])
+        for s, e, n in seg:
             if ce is None or s > ce:
                 if ce is not None:
                     busy += ce - cs
                     if s - ce < short_gap_ns:
                         short += s - ce
+                    else:
+                        g = pairs[(short_name(prev), short_name(n))]
+                        g[0] += 1
+                        g[1] += s - ce
                 cs, ce = s, e
             else:
                 ce = max(ce, e)
+            prev = n
         if ce is not None:
             busy += ce - cs
         per = defaultdict(lambda: [0, 0])
@@ -79,7 +89,9 @@ def segments(rows, short_gap_ns=20_000):
                     "busy_frac": busy / span if span else 0.0, "short_gaps_s": short / 1e9,
                     "long_gaps_s": (span - busy - short) / 1e9,
                     "by_kernel": sorted(((k, c, t / 1e9) for k, (c, t) in per.items()),
-                                        key=lambda x: -x[2])})
+                                        key=lambda x: -x[2]),
+                    "long_gaps_by_pair": sorted(((a, b, c, t / 1e9) for (a, b), (c, t) in pairs.items()),
+                                                key=lambda x: -x[3])})
     return out
 
 
@@ -238,6 +250,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("trace_dir")
     ap.add_argument("--top", type=int, default=8)
+    ap.add_argument("--gaps", type=int, default=0, help="also list the N largest long-gap kernel pairs")
     a = ap.parse_args(argv)
     segs = segments(load(a.trace_dir))
     print(f"{len(segs)} marked legs")
@@ -247,6 +260,10 @@ def main(argv=None):
               f"longer gaps {1e3 * s['long_gaps_s']:.1f} ms")
         for k, c, t in s["by_kernel"][:a.top]:
             print(f"    {1e3 * t:10.2f} ms {c:7d}  {k}")
+        if a.gaps:
+            print("  long gaps by the kernels either side:")
+            for k0, k1, c, t in s["long_gaps_by_pair"][:a.gaps]:
+                print(f"    {1e3 * t:10.2f} ms {c:7d}  {k0} -> {k1}")
 
 
 if __name__ == "__main__":

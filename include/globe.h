@@ -353,7 +353,9 @@ int globe_serve_open(globe_ctx* ctx, const globe_serve_opts* opts);
 int globe_serve_submit(globe_ctx* ctx, const globe_request* req, int64_t* id);
 /* Admits queued requests (FIFO) into free slots while the HBM budget allows, runs one
  * iteration of the block PCG over the occupied slots and retires the requests whose
- * column converged (or reached max_iter). Counts after the step may be NULL. */
+ * column converged (or reached max_iter). Counts after the step may be NULL: finished
+ * (results ready this step), active (requests in the block), queued (requests waiting,
+ * including those staged to join at the next step). */
 int globe_serve_step(globe_ctx* ctx, int32_t* finished, int32_t* active, int32_t* queued);
 /* Ids of finished requests not yet taken (up to max, ascending); *count gets how many. */
 int globe_serve_finished(globe_ctx* ctx, int64_t* ids, int32_t max, int32_t* count);
@@ -373,9 +375,9 @@ int globe_serve_close(globe_ctx* ctx);
 int globe_set_mixed_digits(globe_ctx* ctx, int digits);
 
 /* ---- PCG iteration graphs (no reference counterpart) ------------------------------- */
-/* on (the default): a PCG solve captures one iteration's launches into a HIP graph per
- * digit configuration (after running that configuration once eagerly) and replays it;
- * off: every iteration's kernels are enqueued one by one. Results are identical. */
+/* on: a PCG solve captures one iteration's launches into a HIP graph per digit
+ * configuration (after running that configuration once eagerly) and replays it; off (the
+ * default): every iteration's kernels are enqueued one by one. Results are identical. */
 int globe_set_graphs(globe_ctx* ctx, int on);
 
 /* ---- serving: rows of the genotype product (no reference counterpart) -------------- */
