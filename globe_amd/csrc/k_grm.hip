@@ -11,11 +11,12 @@
 //
 // Product: a workgroup owns a strip of 64 output rows and one of kGrmChunks ranges of the
 // strip's tiles J: tile (I, J) for J <= I, tile (J, I) read transposed for J > I. Tiles
-// and the matching 64 x 16 slice of S0 (pre-transposed to rows of 16 doubles) go global
-// -> LDS (global_load_lds_dwordx4) into a ring of kGS slots, one wait + barrier per tile,
-// and each tile is multiplied on the FP64 matrix cores while the next one lands. A group
-// of up to 32 columns takes one pass over the triangle (blocks of 8 waves for 17-32
-// columns: two column groups share each tile). The strip partials of the
+// and the matching 64-row slice of S0 (pre-transposed to rows of 16, 20, 24 or 32
+// doubles) go global -> LDS (global_load_lds_dwordx4) into a ring of kGS slots, one wait +
+// barrier per tile, and each tile is multiplied on the FP64 matrix cores while the next
+// one lands. A group of up to 32 columns takes one pass over the triangle: 16 columns on
+// the 16 x 16 MFMA; 17-24 with one or two more groups of 4 on the 4 x 4 MFMA; 25-32 in
+// blocks of 8 waves (two column groups of 16 share each tile). The strip partials of the
 // chunks are added in chunk order, so the result is deterministic. Each stored tile is
 // read twice (once for its row strip, once for its column strip); the triangle is far
 // larger than any cache.
@@ -35,17 +36,24 @@ constexpr int kGS = 2;         // ring slots: one tile in flight while one is mu
 constexpr int kTileB = kGT * kGT * 4;                 // FP32 tile bytes
 
 // A block of 4 CG waves: wave w owns rows 16 (w % 4) .. +15 of the strip and column
-// group w / 4 (16 columns); the CG groups share each tile.
-template <int CG>
+// group w / 4 (16 columns); the CG groups share each tile. E > 0 (with CG = 1): each wave
+// also owns E groups of 4 more columns (16 + 4 e ..) on the 4 x 4 FP64 MFMA, so 17-24
+// columns cost 1.25-1.5 times the MFMA work of 16 instead of twice.
+template <int CG, int E = 0>
 struct GrmCfg {
-    static constexpr int kCols = kGC * CG;                // columns per pass
+    static_assert(E == 0 || CG == 1, "4-column groups extend one 16-column group");
+    static constexpr int kCols = kGC * CG + 4 * E;        // columns per pass (S0 slice row)
     static constexpr int kWaves = 4 * CG;
     static constexpr int kSliceB = kGT * kCols * 8;       // S0 slice bytes
     static constexpr int kSlotB = kTileB + kSliceB;
     static constexpr int kTilePerWave = 16 / kWaves;      // 1 KB tile pieces per wave
-    static constexpr int kSlicePerWave = kSliceB / 1024 / kWaves;
+    static constexpr int kSlicePieces = kSliceB / 1024;   // 1 KB slice pieces, strided over the waves
+    static constexpr int kSlicePerWave = (kSlicePieces + kWaves - 1) / kWaves;
     static constexpr int kCopies = kTilePerWave + kSlicePerWave;
-    static_assert(16 % kWaves == 0 && (kSliceB / 1024) % kWaves == 0, "pieces spread evenly");
+    static_assert(16 % kWaves == 0 && kSliceB % 1024 == 0, "whole 1 KB pieces");
+    // with more than two ring slots the wait counts a wave's copies of later stages, which
+    // needs every wave to issue the same number
+    static_assert(kGS == 2 || kSlicePieces % kWaves == 0, "pieces spread evenly");
 };
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -74,11 +82,11 @@ __global__ __launch_bounds__(256) void k_grm_s0t(const double* __restrict__ S0, 
 
 // Copies of one stage into ring slot `slot` by wave w: tile pieces (1 KB = 4 tile rows,
 // lane l: row 4 p + l / 16, its piece (l % 16 - row) & 15, so the rows land rotated)
-// p = kTilePerWave w .., and kSlicePerWave 1 KB pieces of the S0T slice.
-template <int CG>
+// p = kTilePerWave w .., and the 1 KB pieces w, w + kWaves, .. of the S0T slice.
+template <int CG, int E>
 __device__ inline void grm_issue(const float* __restrict__ tile, const double* __restrict__ s0t,
                                  uint8_t* __restrict__ slot, int w, int lane) {
-    using C = GrmCfg<CG>;
+    using C = GrmCfg<CG, E>;
 #pragma unroll
     for (int q = 0; q < C::kTilePerWave; ++q) {
         const int p = C::kTilePerWave * w + q;
@@ -89,9 +97,10 @@ __device__ inline void grm_issue(const float* __restrict__ tile, const double* _
     }
 #pragma unroll
     for (int q = 0; q < C::kSlicePerWave; ++q) {
-        const int p = C::kSlicePerWave * w + q;
-        __builtin_amdgcn_global_load_lds((const void*)((const uint8_t*)s0t + p * 1024 + lane * 16),
-                                         (lds_void*)(slot + kTileB + p * 1024), 16, 0, 0);
+        const int p = w + C::kWaves * q;
+        if (p < C::kSlicePieces)  // uniform in the wave
+            __builtin_amdgcn_global_load_lds((const void*)((const uint8_t*)s0t + p * 1024 + lane * 16),
+                                             (lds_void*)(slot + kTileB + p * 1024), 16, 0, 0);
     }
 }
 
@@ -100,29 +109,41 @@ __device__ inline void grm_issue(const float* __restrict__ tile, const double* _
 // (TRANS: the stored tile read transposed), B lane l = S0[j0 + 4 s + l / 16][16 cg + l % 16],
 // D element v of lane l = (row 4 v + l / 16, column l % 16) (v_mfma_f64_16x16x4f64,
 // probed on gfx950).
-template <int CG, bool TRANS>
-__device__ inline void grm_mma(const uint8_t* __restrict__ slot, int rg, int cg, int lane, v4d& acc) {
-    constexpr int kCols = GrmCfg<CG>::kCols;
+// With E > 0, 16 E more of the 4-block v_mfma_f64_4x4x4f64 (probed on gfx950: block b of
+// lanes 4 b .. 4 b + 3 + 16 kk; A lane 16 kk + 4 b + r = (row r, k kk) of block b, B lane
+// 16 kk + 4 b + c = (k kk, column c), D lane 16 r + 4 b + c = (row r, column c)): block b
+// takes rows 16 rg + 4 b .. + 3, so its A operand is the 16 x 16 form's, lane for lane, and
+// every block reads the same 4 columns 16 + 4 e .. of S0; accx[e] of lane l = (row
+// 16 rg + 4 ((l >> 2) & 3) + l / 16, column 16 + 4 e + l % 4).
+template <int CG, int E, bool TRANS>
+__device__ inline void grm_mma(const uint8_t* __restrict__ slot, int rg, int cg, int lane, v4d& acc,
+                               double (&accx)[2]) {
+    constexpr int kCols = GrmCfg<CG, E>::kCols;
     const float* g = reinterpret_cast<const float*>(slot);
-    const double* sb = reinterpret_cast<const double*>(slot + kTileB) + (lane >> 4) * kCols + kGC * cg + (lane & 15);
+    const double* srow = reinterpret_cast<const double*>(slot + kTileB) + (lane >> 4) * kCols;
+    const double* sb = srow + kGC * cg + (lane & 15);
+    const double* sx = srow + kGC + (lane & 3);
     const int a = 16 * rg + (lane & 15);
 #pragma unroll
     for (int s4 = 0; s4 < kGT / 4; ++s4) {
         const int b = 4 * s4 + (lane >> 4);
-        const float gv = TRANS ? g[rot(b, a)] : g[rot(a, b)];
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)gv, sb[4 * s4 * kCols], acc, 0, 0, 0);
+        const double gv = (double)(TRANS ? g[rot(b, a)] : g[rot(a, b)]);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(gv, sb[4 * s4 * kCols], acc, 0, 0, 0);
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            accx[e] = __builtin_amdgcn_mfma_f64_4x4x4f64(gv, sx[4 * s4 * kCols + 4 * e], accx[e], 0, 0, 0);
     }
 }
 
-// One pass of up to 16 CG columns (S0T zero past kk): part[h][c][i] = sum over the tiles
-// J of chunk h of (G_IJ S0_J)[i][c].
-template <int CG>
+// One pass of up to 16 CG + 4 E columns (S0T zero past kk): part[h][c][i] = sum over the
+// tiles J of chunk h of (G_IJ S0_J)[i][c].
+template <int CG, int E>
 __global__ __launch_bounds__(256 * CG) void k_grm_tiles(const float* __restrict__ tiles, int32_t n,
                                                         const double* __restrict__ S0T, int kk,
                                                         int32_t ntiles, double* __restrict__ part,
                                                         const int* skip) {
     if (skip && *skip) return;
-    using C = GrmCfg<CG>;
+    using C = GrmCfg<CG, E>;
     __shared__ __attribute__((aligned(1024))) uint8_t lds[kGS * C::kSlotB];
     const int I = blockIdx.x, h = blockIdx.y;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -130,12 +151,13 @@ __global__ __launch_bounds__(256 * CG) void k_grm_tiles(const float* __restrict_
     const int J0 = (int)((int64_t)h * ntiles / kGrmChunks), J1 = (int)((int64_t)(h + 1) * ntiles / kGrmChunks);
     const int nt = J1 - J0;
     v4d acc = {0.0, 0.0, 0.0, 0.0};
+    double accx[2] = {0.0, 0.0};
     if (nt > 0) {
         // stage t: tile J0 + t (repeating the last one past the end, into slots never read)
         auto issue = [&](int t, int slot) {
             const int64_t J = J0 + (t < nt ? t : nt - 1);
             const float* tile = tiles + (J <= I ? tile_index(I, J) : tile_index(J, I)) * (kGT * kGT);
-            grm_issue<CG>(tile, S0T + J * kGT * C::kCols, lds + slot * C::kSlotB, w, lane);
+            grm_issue<CG, E>(tile, S0T + J * kGT * C::kCols, lds + slot * C::kSlotB, w, lane);
         };
 #pragma unroll
         for (int u = 0; u < kGS - 1; ++u) issue(u, u);
@@ -149,19 +171,26 @@ __global__ __launch_bounds__(256 * CG) void k_grm_tiles(const float* __restrict_
             issue(t + kGS - 1, cs == 0 ? kGS - 1 : cs - 1);
             __builtin_amdgcn_sched_barrier(0);
             const uint8_t* slot = lds + cs * C::kSlotB;
-            if (J0 + t <= I) grm_mma<CG, false>(slot, rg, cg, lane, acc);
-            else grm_mma<CG, true>(slot, rg, cg, lane, acc);
+            if (J0 + t <= I) grm_mma<CG, E, false>(slot, rg, cg, lane, acc, accx);
+            else grm_mma<CG, E, true>(slot, rg, cg, lane, acc, accx);
             cs = cs + 1 == kGS ? 0 : cs + 1;
         }
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     }
     const int col = kGC * cg + (lane & 15);
-    if (col >= kk) return;
     const int64_t i0 = (int64_t)I * kGT;
+    if (col < kk) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const int64_t i = i0 + 16 * rg + 4 * v + (lane >> 4);
-        if (i < n) part[((int64_t)h * kGCMax + col) * n + i] = acc[v];
+        for (int v = 0; v < 4; ++v) {
+            const int64_t i = i0 + 16 * rg + 4 * v + (lane >> 4);
+            if (i < n) part[((int64_t)h * kGCMax + col) * n + i] = acc[v];
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int cx = kGC + 4 * e + (lane & 3);
+        const int64_t i = i0 + 16 * rg + 4 * ((lane >> 2) & 3) + (lane >> 4);
+        if (cx < kk && i < n) part[((int64_t)h * kGCMax + cx) * n + i] = accx[e];
     }
 }
 
@@ -389,14 +418,19 @@ void launch_grm_product(hipStream_t s, const float* tiles, int32_t n, const doub
     double* S0T = part + (size_t)kGrmChunks * kGCMax * n;
     for (int c = 0; c < k; c += kGCMax) {
         const int kk = std::min(kGCMax, k - c);
-        const int cols = kk > kGC ? 2 * kGC : kGC;
+        // 16 columns; 16 + 4 or 16 + 8 (4-column MFMA groups); two groups of 16
+        const int cols = kk <= kGC ? kGC : kk <= kGC + 4 ? kGC + 4 : kk <= kGC + 8 ? kGC + 8 : 2 * kGC;
         const unsigned gt = (unsigned)std::min<int64_t>((npad * cols + 255) / 256, 1024);
         k_grm_s0t<<<gt, 256, 0, s>>>(S0 + c * ldn, ldn, n, kk, cols, npad, S0T, skip);
         HIP_CHECK(hipGetLastError());
         if (cols == kGC)
-            k_grm_tiles<1><<<g, 256, 0, s>>>(tiles, n, S0T, kk, ntiles, part, skip);
+            k_grm_tiles<1, 0><<<g, 256, 0, s>>>(tiles, n, S0T, kk, ntiles, part, skip);
+        else if (cols == kGC + 4)
+            k_grm_tiles<1, 1><<<g, 256, 0, s>>>(tiles, n, S0T, kk, ntiles, part, skip);
+        else if (cols == kGC + 8)
+            k_grm_tiles<1, 2><<<g, 256, 0, s>>>(tiles, n, S0T, kk, ntiles, part, skip);
         else
-            k_grm_tiles<2><<<g, 512, 0, s>>>(tiles, n, S0T, kk, ntiles, part, skip);
+            k_grm_tiles<2, 0><<<g, 512, 0, s>>>(tiles, n, S0T, kk, ntiles, part, skip);
         HIP_CHECK(hipGetLastError());
         k_grm_finish<<<dim3(gf, kk), 256, 0, s>>>(part, n, kGCMax, theta_cols + c * 5, GZ + c * ldg, ldg,
                                                   skip);

@@ -332,10 +332,11 @@ def test_record_tiles_and_large_block_fallback(big_block, gpu_ctx_factory):
     assert rel(xg, xo) < 1e-7
 
 
-@pytest.mark.parametrize("k", [1, 3, 16, 17, 21, 32, 33, 64])
+@pytest.mark.parametrize("k", [1, 3, 16, 17, 20, 21, 24, 25, 32, 33, 52, 64])
 def test_grm_product_widths(k, gpu_ctx_factory):
-    """Tiled GRM product: one pass of 16 columns (1..16), of two column groups sharing
-    each tile (17..32), and several passes (33..64)."""
+    """Tiled GRM product: one pass of 16 columns (1..16), of 16 plus one or two groups of
+    4 on the 4 x 4 MFMA (17..20, 21..24), of two column groups sharing each tile (25..32),
+    and several passes (33..64; 52 = 32 + 20)."""
     _say("grm-widths", k)
     d = load_golden("grm_n180")
     g = device_for(d, gpu_ctx_factory)
@@ -371,6 +372,40 @@ def test_grm_few_column_product_multi_tile(k, gpu_ctx_factory):
     assert rel(got, want) < 1e-12
     wide = g.computeVx_multi(np.asfortranarray(x), multGRM_uhat=True)[:, :k]
     assert rel(got, wide) < 1e-13
+
+
+@pytest.mark.parametrize("k", [17, 20, 22, 24])
+def test_grm_four_column_groups_multi_tile(k, gpu_ctx_factory):
+    """17-24 columns: 16 on the 16 x 16 FP64 MFMA plus groups of 4 on the 4 x 4 one. n =
+    3,000 (47 tile strips, tiles read both ways). Against sig2g G J^T x from the dense G,
+    and against the same columns of a 32-column product (two 16-column groups)."""
+    _say("grm-4col-groups", k)
+    n = 3000
+    rng = np.random.default_rng(60 + k)
+    c = synth.simulate_cohort(n, 0.5, rng)
+    gcase = synth.make_reduced_case(n=n, m=500, seed=61)
+    G = synth.grm_dense(gcase.grm_lt, n)
+    theta = (0.6, 0.4, 0.4, 0.2, 0.2)
+    g = gpu_ctx_factory()
+    g.set_cohort(c.indiv, c.times, c.X)
+    g.readFullGRM(gcase.grm_lt)
+    g.set_varcomp(theta)
+    x = rng.standard_normal((c.nrec, 32))
+    J = np.zeros((c.nrec, n))
+    J[np.arange(c.nrec), c.indiv] = 1.0
+    want = theta[1] * G @ (J.T @ x[:, :k])
+    got = g.computeVx_multi(np.asfortranarray(x[:, :k]), multGRM_uhat=True)
+    assert rel(got, want) < 1e-12
+    wide = g.computeVx_multi(np.asfortranarray(x), multGRM_uhat=True)[:, :k]
+    assert rel(got, wide) < 1e-13
+    # each column on its own scale: a column of zeros and a tiny one stay exact
+    x2 = x[:, :k].copy()
+    x2[:, k - 1] = 0.0
+    x2[:, 16] *= 1e-9
+    got2 = g.computeVx_multi(np.asfortranarray(x2), multGRM_uhat=True)
+    want2 = theta[1] * G @ (J.T @ x2)
+    assert np.all(got2[:, k - 1] == 0.0)
+    assert rel(got2[:, 16], want2[:, 16]) < 1e-12
 
 
 def test_grm_builder_at_40k_rows_offsets(gpu_ctx_factory):
