@@ -174,7 +174,7 @@ void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int prec
         const bool prof = c->prof_on && !c->arena.base;
         if (prof) {
             const double np = (double)((n + 63) / 64 * 64), L = c->limbs_mixed;
-            const double cols = 16.0 * ((k + 15) / 16);
+            const double cols = mixed ? 16.0 * ((k + 15) / 16) : (double)grm_padded_cols(k);
             HIP_CHECK(hipEventCreate(&e.a));
             HIP_CHECK(hipEventCreate(&e.b));
             e.pass = mixed ? 3 : 2;

@@ -244,6 +244,8 @@ void launch_grm_product(hipStream_t s, const float* tiles, int32_t n, const doub
                         int k, const double* theta_cols, double* GZ, int64_t ldg, const int* skip,
                         double* part);
 size_t grm_product_part_bytes(int32_t n);
+// columns the FP64 strip product computes for k columns (16, 20, 24 or 32 per pass of 32)
+int grm_padded_cols(int k);
 
 // Mixed-precision GRM product (k_grm_i8.hip): int8 digit tiles of the full square with
 // per-row exponents, built from the FP32 tiles (aux: grm_digit_aux_bytes of scratch that

@@ -419,7 +419,7 @@ void launch_grm_product(hipStream_t s, const float* tiles, int32_t n, const doub
     for (int c = 0; c < k; c += kGCMax) {
         const int kk = std::min(kGCMax, k - c);
         // 16 columns; 16 + 4 or 16 + 8 (4-column MFMA groups); two groups of 16
-        const int cols = kk <= kGC ? kGC : kk <= kGC + 4 ? kGC + 4 : kk <= kGC + 8 ? kGC + 8 : 2 * kGC;
+        const int cols = grm_padded_cols(kk);
         const unsigned gt = (unsigned)std::min<int64_t>((npad * cols + 255) / 256, 1024);
         k_grm_s0t<<<gt, 256, 0, s>>>(S0 + c * ldn, ldn, n, kk, cols, npad, S0T, skip);
         HIP_CHECK(hipGetLastError());
@@ -463,6 +463,15 @@ void launch_grm_product_sym(hipStream_t s, const float* tiles, int32_t n, const 
         k_grm_sym_finish<4><<<gf, 256, 0, s>>>(rowp, colp, n, ntiles, theta_cols, GZ, ldg, skip);
     }
     HIP_CHECK(hipGetLastError());
+}
+
+int grm_padded_cols(int k) {
+    int cols = 0;
+    for (int c = 0; c < k; c += kGCMax) {
+        const int kk = std::min(kGCMax, k - c);
+        cols += kk <= kGC ? kGC : kk <= kGC + 4 ? kGC + 4 : kk <= kGC + 8 ? kGC + 8 : 2 * kGC;
+    }
+    return cols;
 }
 
 size_t grm_product_part_bytes(int32_t n) {

@@ -93,6 +93,44 @@ def test_serving_mix_equals_alone_and_oracle(cohort, gpu_ctx_factory):
         assert rel(mix[q], xo[:, 0]) < 1e-7, q
 
 
+def test_serving_iteration_cap(cohort, gpu_ctx_factory):
+    """Requests that reach max_iter retire unconverged after exactly max_iter iterations
+    (their column is deactivated and leaves the block), holding the iterate the oracle's
+    PCG reaches in as many iterations; in a block they equal the same request alone."""
+    _say("serve-cap")
+    d = cohort
+    reqs = _requests(d.cohort.n, 6, seed=9)
+    g = _device(d, gpu_ctx_factory)
+    cap = max(r[0].shape[0] for r in reqs)
+
+    def run(rs):
+        g.serve_open(slots=3, max_records=cap, tol=1e-15, max_iter=4, precision=G.PREC_FP64)
+        ids = {g.serve_submit(*r): q for q, r in enumerate(rs)}
+        out, info = [None] * len(rs), [None] * len(rs)
+        for _ in range(1000):
+            g.serve_step()
+            for rid in g.serve_finished():
+                x, st = g.serve_take(rid)
+                q = ids.pop(rid)
+                out[q], info[q] = x, st
+            if not ids:
+                break
+        g.serve_close()
+        assert not ids
+        return out, info
+
+    mix, info = run(reqs)
+    for q, st in enumerate(info):
+        assert not st["converged"] and st["iters"] == 4, (q, st)
+    for q in (0, 3, 5):
+        alone, _ = run([reqs[q]])
+        assert rel(mix[q], alone[0]) <= 1e-14, q
+        orc = _oracle_for(d, reqs[q])
+        xo, so = orc.pcg(np.asarray(reqs[q][2])[:, None], tol=1e-15, max_iter=4, rule=2, stagnation=False)
+        assert so["iters"] == 4
+        assert rel(mix[q], xo[:, 0]) < 1e-9, q
+
+
 def test_serving_mixed_precision_within_1e6(cohort, gpu_ctx_factory):
     _say("serve-mixed")
     d = cohort
