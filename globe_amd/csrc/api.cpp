@@ -163,7 +163,7 @@ void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int prec
             ensure_grm_digits(c);
             ensure_gdiag(c);
             scratch = ws_bytes(c, c->w_grm8_part, grm_i8_part_bytes(n));
-        } else if (k <= 4) {
+        } else if (k <= kSymMaxCols) {
             scratch = ws_bytes(c, c->w_grm_sym, grm_sym_part_bytes(n, k));
         } else {
             scratch = ws_bytes(c, c->w_grm_part, grm_product_part_bytes(n));
@@ -182,7 +182,8 @@ void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int prec
             e.bytes = 4.0 * (double)n * (n + 1) / 2;
             e.flops = 2.0 * (double)n * n * k;
             e.issued = mixed ? 2.0 * np * np * cols * (L * (L + 1) / 2 + L - 1)
-                             : (k <= 4 ? 2.0 * np * np * (k <= 2 ? k : 4) : 2.0 * np * np * cols);
+                             : (k <= kSymMaxCols ? 2.0 * np * np * (k <= 2 ? k : k <= 4 ? 4 : 8)
+                                                 : 2.0 * np * np * cols);
             // external: inside a captured PCG iteration the record is a graph node that
             // stamps the event on every replay
             HIP_CHECK(hipEventRecordWithFlags(e.a, s, hipEventRecordExternal));
@@ -190,7 +191,7 @@ void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int prec
         if (mixed)
             launch_grm_product_i8(s, c->d_grm8.as<int8_t>(), grm_digit_rexp(c->d_grm8_aux.p, n), n, S0, n, k,
                                   c->limbs_mixed, c->d_gdiag.as<double>(), th, GZ, n, skip, scratch);
-        else if (k <= 4)
+        else if (k <= kSymMaxCols)
             launch_grm_product_sym(s, c->d_grm.as<float>(), n, S0, n, k, th, GZ, n, skip,
                                    static_cast<double*>(scratch));
         else
@@ -522,10 +523,11 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
             }
         }
         queued += batch;
-        // the scalar header (done, iters, ..., max_ratio), with relaxation also the
-        // per-column arrays up to the residual ratios and the active flags
-        HIP_CHECK(hipMemcpyAsync(hs, dst, relax ? offsetof(PcgState, col_iters) : offsetof(PcgState, rs),
-                                 hipMemcpyDeviceToHost, s));
+        // the whole state (3 KB: the header, and the per-column ratios and flags the
+        // relaxation and the caller read) after every batch, so the last batch's copy is
+        // the final one; the end event goes with it
+        HIP_CHECK(hipEventRecord(e1, s));
+        HIP_CHECK(hipMemcpyAsync(hs, dst, sizeof(PcgState), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
         // timing events of a replayed graph: read now (batch 1), before a replay re-records them
         for (IterGraphs::Entry* g : launched)
@@ -559,9 +561,6 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
         tb = tn;
         batch = per_iter > 2e-3 ? 1 : std::min(batch * 2, 16);
     }
-    HIP_CHECK(hipEventRecord(e1, s));
-    HIP_CHECK(hipMemcpyAsync(hs, dst, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
     c->prof_iter = -1;
     if (c->prof_on) {
         // launches of iterations after convergence returned at once: drop them

@@ -236,7 +236,8 @@ size_t grm_tile_bytes(int32_t n);
 void launch_grm_tiles_from_lt(hipStream_t s, const float* lt_rows, int64_t r0, int64_t r1, float* tiles);
 void launch_grm_lt_from_tiles(hipStream_t s, const float* tiles, int64_t r0, int64_t r1, float* lt_rows);
 // part: grm_product_part_bytes(n) of scratch for the strip partials and S0 slices
-// k <= 4 columns: every stored tile read once (scratch: grm_sym_part_bytes(n, k))
+// k <= kSymMaxCols columns: every stored tile read once (scratch: grm_sym_part_bytes(n, k))
+constexpr int kSymMaxCols = 8;
 size_t grm_sym_part_bytes(int32_t n, int k);
 void launch_grm_product_sym(hipStream_t s, const float* tiles, int32_t n, const double* S0, int64_t ldn, int k,
                             const double* theta_cols, double* GZ, int64_t ldg, const int* skip, double* scratch);

@@ -19,7 +19,9 @@
 // blocks of 8 waves (two column groups of 16 share each tile). The strip partials of the
 // chunks are added in chunk order, so the result is deterministic. Each stored tile is
 // read twice (once for its row strip, once for its column strip); the triangle is far
-// larger than any cache.
+// larger than any cache. Products of at most 8 columns read each stored tile once
+// instead: its transposed product goes to a per-tile partial of 64 x k doubles that a
+// finish kernel adds per strip in fixed order (k_grm_sym, k_grm_symx).
 #include <algorithm>
 
 #include "globe_internal.h"
@@ -210,7 +212,7 @@ __global__ __launch_bounds__(256) void k_grm_finish(const double* __restrict__ p
     }
 }
 
-// Few-column product (k <= 4) that reads every stored tile once. A workgroup owns strip I
+// One- or two-column product that reads every stored tile once. A workgroup owns strip I
 // and one of kSymChunks ranges of J <= I (the longest strips are dispatched first); wave
 // w holds rows 16 w .. 16 w + 15 of each tile, lane = the tile column, so a row of a
 // tile is one 256-byte wave load. Per tile (I, J):
@@ -321,6 +323,113 @@ __global__ __launch_bounds__(256) void k_grm_sym_finish(const double* __restrict
     for (int h = 0; h < kSymChunks; ++h) r += rowp[((int64_t)h * KC + q) * n + i];
     const double c = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
     GZ[q * ldg + i] = th[q * 5 + 1] * (r + c);
+}
+
+// Few-column product (3 <= k <= 8) on the 4 x 4 FP64 MFMA, reading every stored tile once.
+// Work split as in k_grm_sym (strip I, one of kSymChunks ranges of J <= I, the longest
+// strips first); the tiles and the matching 64-row slices of S0T (rows of 4 E doubles)
+// stream global -> LDS through a 2-slot ring as in k_grm_tiles, and x_I (the strip's own
+// slice) sits in LDS. Per tile (I, J), wave w:
+//   rows:    rows 16 w .. + 15 of G_IJ x_J (16 E MFMAs over the tile's 64 columns), kept in
+//            registers over the range and written once into rowp[h][q][i];
+//   columns: for J < I, tile columns 16 w .. + 15 of G_IJ^T x_I (16 E MFMAs over the
+//            tile's 64 rows, the transposed LDS read), complete for the tile, written into
+//            colp[tile (I, J)][q][a].
+// Lane layout of v_mfma_f64_4x4x4f64 as in grm_mma: block b of an instruction takes rows
+// (or tile columns) 16 w + 4 b .. + 3. The partials have k_grm_sym<KC>'s layout, so
+// k_grm_sym_finish<KC> adds them in the same fixed order.
+template <int KC>  // 4 or 8
+__global__ __launch_bounds__(256) void k_grm_symx(const float* __restrict__ tiles, int32_t n, int32_t ntiles,
+                                                  const double* __restrict__ S0T, int kk,
+                                                  double* __restrict__ rowp, double* __restrict__ colp,
+                                                  const int* skip) {
+    if (skip && *skip) return;
+    constexpr int E = (KC + 3) / 4;       // 4-column MFMA groups
+    constexpr int kC4 = 4 * E;             // S0T row length
+    constexpr int kSliceB = kGT * kC4 * 8;
+    constexpr int kSlotB = kTileB + kSliceB;
+    static_assert(kSliceB % 1024 == 0 && kSliceB / 1024 <= 4, "one slice piece per wave at most");
+    __shared__ __attribute__((aligned(1024))) uint8_t lds[kGS * kSlotB + kSliceB];
+    const int32_t I = ntiles - 1 - (int32_t)blockIdx.x;
+    const int h = blockIdx.y;
+    const int32_t J0 = (int32_t)((int64_t)h * (I + 1) / kSymChunks);
+    const int32_t J1 = (int32_t)((int64_t)(h + 1) * (I + 1) / kSymChunks);
+    const int nt = J1 - J0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double accr[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) accr[e] = 0.0;
+    if (nt > 0) {
+        auto issue = [&](int t, int slot) {
+            const int64_t J = J0 + (t < nt ? t : nt - 1);
+            const float* tile = tiles + tile_index(I, J) * (kGT * kGT);
+            uint8_t* dst = lds + slot * kSlotB;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int p = 4 * w + q;
+                const int a = 4 * p + (lane >> 4);
+                const int piece = ((lane & 15) - a) & 15;
+                __builtin_amdgcn_global_load_lds((const void*)(tile + a * kGT + 4 * piece),
+                                                 (lds_void*)(dst + p * 1024), 16, 0, 0);
+            }
+            if (w < kSliceB / 1024)
+                __builtin_amdgcn_global_load_lds(
+                    (const void*)((const uint8_t*)(S0T + J * kGT * kC4) + w * 1024 + lane * 16),
+                    (lds_void*)(dst + kTileB + w * 1024), 16, 0, 0);
+        };
+        issue(0, 0);
+        // x_I: the strip's own slice, read by the column products
+        double* xi = reinterpret_cast<double*>(lds + kGS * kSlotB);
+        for (int e = threadIdx.x; e < kGT * kC4; e += 256) xi[e] = S0T[(int64_t)I * kGT * kC4 + e];
+        const double* xis = xi + (lane >> 4) * kC4 + (lane & 3);
+        int cs = 0;
+        for (int t = 0; t < nt; ++t) {
+            // this wave's copies of stage t have landed; after the barrier every wave's
+            // have (and x_I is in place), and every wave is done with the other slot
+            asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+            issue(t + 1, cs ^ 1);
+            __builtin_amdgcn_sched_barrier(0);
+            const uint8_t* slot = lds + cs * kSlotB;
+            const float* g = reinterpret_cast<const float*>(slot);
+            const double* xj = reinterpret_cast<const double*>(slot + kTileB) + (lane >> 4) * kC4 + (lane & 3);
+            const int a = 16 * w + (lane & 15);
+#pragma unroll
+            for (int s4 = 0; s4 < kGT / 4; ++s4) {
+                const double gv = (double)g[rot(a, 4 * s4 + (lane >> 4))];
+#pragma unroll
+                for (int e = 0; e < E; ++e)
+                    accr[e] = __builtin_amdgcn_mfma_f64_4x4x4f64(gv, xj[4 * s4 * kC4 + 4 * e], accr[e], 0, 0, 0);
+            }
+            const int32_t J = J0 + t;
+            if (J < I) {
+                double accc[E];
+#pragma unroll
+                for (int e = 0; e < E; ++e) accc[e] = 0.0;
+#pragma unroll
+                for (int s4 = 0; s4 < kGT / 4; ++s4) {
+                    const double gv = (double)g[rot(4 * s4 + (lane >> 4), a)];
+#pragma unroll
+                    for (int e = 0; e < E; ++e)
+                        accc[e] = __builtin_amdgcn_mfma_f64_4x4x4f64(gv, xis[4 * s4 * kC4 + 4 * e], accc[e], 0, 0, 0);
+                }
+                double* dst = colp + tile_index(I, J) * (kGT * KC);
+                const int col = 16 * w + 4 * ((lane >> 2) & 3) + (lane >> 4);
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int q = 4 * e + (lane & 3);
+                    if (q < kk) dst[q * kGT + col] = accc[e];
+                }
+            }
+            cs ^= 1;
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    }
+    const int64_t i = (int64_t)I * kGT + 16 * w + 4 * ((lane >> 2) & 3) + (lane >> 4);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int q = 4 * e + (lane & 3);
+        if (q < kk && i < n) rowp[((int64_t)h * KC + q) * n + i] = accr[e];
+    }
 }
 
 // Packed rows r0 .. r1 - 1 of the triangle (lt holds them from lt_index(r0, 0) on) into
@@ -438,29 +547,51 @@ void launch_grm_product(hipStream_t s, const float* tiles, int32_t n, const doub
     }
 }
 
+// columns of the few-column product's partials: 1, 2, 4 or 8
+static int sym_cols(int k) { return k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : 8; }
+
 size_t grm_sym_part_bytes(int32_t n, int k) {
     const size_t T = (size_t)(n + kGT - 1) / kGT;
-    const size_t kc = k <= 1 ? 1 : (k <= 2 ? 2 : 4);
-    return sizeof(double) * kc * ((size_t)kSymChunks * (size_t)n + T * (T + 1) / 2 * kGT);
+    const size_t kc = (size_t)sym_cols(k);
+    // row partials, column partials, S0T (rows of 4 or 8 doubles)
+    return sizeof(double) * (kc * ((size_t)kSymChunks * (size_t)n + T * (T + 1) / 2 * kGT) +
+                             T * kGT * (kc <= 4 ? 4 : 8));
 }
 
 void launch_grm_product_sym(hipStream_t s, const float* tiles, int32_t n, const double* S0, int64_t ldn, int k,
                             const double* theta_cols, double* GZ, int64_t ldg, const int* skip, double* scratch) {
     if (n == 0) return;
     const int32_t ntiles = (n + kGT - 1) / kGT;
-    const int kc = k <= 1 ? 1 : (k <= 2 ? 2 : 4);
+    const int kc = sym_cols(k);
+    const int c4 = kc <= 4 ? 4 : 8;
+    const int64_t T = ntiles;
     double* rowp = scratch;
-    double* colp = scratch + (size_t)kSymChunks * kc * n;
+    double* colp = rowp + (size_t)kSymChunks * kc * n;
+    double* S0T = colp + (size_t)(T * (T + 1) / 2) * kGT * kc;
     const dim3 g(ntiles, kSymChunks), gf(ntiles, k);
-    if (kc == 1) {
-        k_grm_sym<1><<<g, 256, 0, s>>>(tiles, n, ntiles, S0, ldn, k, rowp, colp, skip);
-        k_grm_sym_finish<1><<<gf, 256, 0, s>>>(rowp, colp, n, ntiles, theta_cols, GZ, ldg, skip);
-    } else if (kc == 2) {
-        k_grm_sym<2><<<g, 256, 0, s>>>(tiles, n, ntiles, S0, ldn, k, rowp, colp, skip);
-        k_grm_sym_finish<2><<<gf, 256, 0, s>>>(rowp, colp, n, ntiles, theta_cols, GZ, ldg, skip);
-    } else {
-        k_grm_sym<4><<<g, 256, 0, s>>>(tiles, n, ntiles, S0, ldn, k, rowp, colp, skip);
+    if (kc <= 2) {
+        // one or two columns: the vector-ALU form (lane = tile column, the x values in
+        // registers) streams as fast as the MFMA one and needs no transposed S0
+        if (kc == 1) {
+            k_grm_sym<1><<<g, 256, 0, s>>>(tiles, n, ntiles, S0, ldn, k, rowp, colp, skip);
+            k_grm_sym_finish<1><<<gf, 256, 0, s>>>(rowp, colp, n, ntiles, theta_cols, GZ, ldg, skip);
+        } else {
+            k_grm_sym<2><<<g, 256, 0, s>>>(tiles, n, ntiles, S0, ldn, k, rowp, colp, skip);
+            k_grm_sym_finish<2><<<gf, 256, 0, s>>>(rowp, colp, n, ntiles, theta_cols, GZ, ldg, skip);
+        }
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
+    const int64_t npad = T * kGT;
+    const unsigned gt = (unsigned)std::min<int64_t>((npad * c4 + 255) / 256, 1024);
+    k_grm_s0t<<<gt, 256, 0, s>>>(S0, ldn, n, k, c4, npad, S0T, skip);
+    HIP_CHECK(hipGetLastError());
+    if (kc == 4) {
+        k_grm_symx<4><<<g, 256, 0, s>>>(tiles, n, ntiles, S0T, k, rowp, colp, skip);
         k_grm_sym_finish<4><<<gf, 256, 0, s>>>(rowp, colp, n, ntiles, theta_cols, GZ, ldg, skip);
+    } else {
+        k_grm_symx<8><<<g, 256, 0, s>>>(tiles, n, ntiles, S0T, k, rowp, colp, skip);
+        k_grm_sym_finish<8><<<gf, 256, 0, s>>>(rowp, colp, n, ntiles, theta_cols, GZ, ldg, skip);
     }
     HIP_CHECK(hipGetLastError());
 }

@@ -332,7 +332,7 @@ def test_record_tiles_and_large_block_fallback(big_block, gpu_ctx_factory):
     assert rel(xg, xo) < 1e-7
 
 
-@pytest.mark.parametrize("k", [1, 3, 16, 17, 20, 21, 24, 25, 32, 33, 52, 64])
+@pytest.mark.parametrize("k", [1, 3, 5, 8, 9, 16, 17, 20, 21, 24, 25, 32, 33, 52, 64])
 def test_grm_product_widths(k, gpu_ctx_factory):
     """Tiled GRM product: one pass of 16 columns (1..16), of 16 plus one or two groups of
     4 on the 4 x 4 MFMA (17..20, 21..24), of two column groups sharing each tile (25..32),
@@ -346,13 +346,13 @@ def test_grm_product_widths(k, gpu_ctx_factory):
     assert rel(g.computeVx_multi(x, multGRM_uhat=True), orc.vx(x, grm_only=True)) < 1e-12
 
 
-@pytest.mark.parametrize("k", [1, 2, 3, 4])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6, 8])
 def test_grm_few_column_product_multi_tile(k, gpu_ctx_factory):
-    """Products of k <= 4 columns read each stored tile once (row strip in registers,
-    transposed strip through per-tile partials). n = 3,000: 47 tile strips, so each of a
-    strip's 8 chunks holds several tiles and both halves of the column-partial double
-    buffer are used. Against sig2g G J^T x from the dense G, and against the same columns
-    of a 16-column product (the strip kernel)."""
+    """Products of k <= 8 columns read each stored tile once (row strip in registers,
+    transposed strip through per-tile partials, both on the 4 x 4 FP64 MFMA). n = 3,000:
+    47 tile strips, so each of a strip's 8 chunks holds several tiles and both ring slots
+    are used. Against sig2g G J^T x from the dense G, and against the same columns of a
+    16-column product (the strip kernel)."""
     _say("grm-few-columns", k)
     n = 3000
     rng = np.random.default_rng(40 + k)
