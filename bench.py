@@ -261,10 +261,15 @@ def serving_leg(g, c, kept, args):
              "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
              "mean_iters": float(np.mean(st["iters"])), "mean_active_slots": st["mean_active"],
              "block_iterations": st["steps"], "wall_s": st["wall_s"],
-             "pool_peak_bytes": use["peak"], "steps_waiting_for_hbm": use["waits"]}
+             "pool_peak_bytes": use["peak"], "steps_waiting_for_hbm": use["waits"],
+             # latency parts, p50 / p99: arrival to queued (host preparation on the intake
+             # thread), queued to admitted, admitted to done
+             "parts_s": {k: [float(np.percentile(st[k + "_s"], 50)), float(np.percentile(st[k + "_s"], 99))]
+                         for k in ("intake", "queue", "service")}}
         log(f"serving at {rate:g}/s: {r['solves_per_s']:.1f} solves/s, p50 {r['p50_s']:.3f}s, "
             f"p99 {r['p99_s']:.3f}s, mean active {r['mean_active_slots']:.1f}, pool peak "
-            f"{use['peak'] / 1e9:.2f} of {use['pool'] / 1e9:.2f} GB, waits for HBM {use['waits']}")
+            f"{use['peak'] / 1e9:.2f} of {use['pool'] / 1e9:.2f} GB, waits for HBM {use['waits']}; "
+            + ", ".join(f"{k} {v[0]:.3f}/{v[1]:.3f}s" for k, v in r["parts_s"].items()))
         return r
 
     out = run(reqs, args.serve_rate)

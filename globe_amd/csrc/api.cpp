@@ -64,10 +64,6 @@ void DevBuf::release() {
     bytes = 0;
 }
 
-void require(bool cond, int code, const std::string& msg) {
-    if (!cond) throw Error(code, msg);
-}
-
 void* ws_bytes(globe_ctx* c, DevBuf& b, size_t bytes) {
     globe_ctx::Arena& a = c->arena;
     if (!a.base) {

@@ -191,7 +191,6 @@ void ensure_gdiag(globe_ctx* c);
 void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64_t ldx,
              const globe_pcg_opts& o, globe_pcg_stats* st, double* col_ratio, int32_t* col_iters,
              double* hist = nullptr, int hist_cap = 0);
-void require(bool cond, int code, const std::string& msg);
 // int8 digits (2..4) a mixed-precision product may use in a PCG iteration that starts at
 // residual ratio |r| / |b| (inexact Krylov relaxation)
 int relaxed_limbs(double ratio, double tol);

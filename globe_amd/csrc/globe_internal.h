@@ -19,7 +19,14 @@ struct Error : std::runtime_error {
 };
 
 void hip_check(hipError_t e, const char* what);
-void require(bool cond, int code, const std::string& msg);
+// A failed check throws Error(code, msg). The literal form builds no string unless the
+// check fails (checks sit in per-record loops of request and input validation).
+inline void require(bool cond, int code, const char* msg) {
+    if (!cond) throw Error(code, msg);
+}
+inline void require(bool cond, int code, const std::string& msg) {
+    if (!cond) throw Error(code, msg);
+}
 #define HIP_CHECK(x) ::globe::hip_check((x), #x)
 
 // Grow-only device allocation.

@@ -54,7 +54,9 @@ def replay(g, requests, rate, seed=0):
     """Submit requests at Poisson arrival times (rate per second) from an intake thread
     (globe_serve_submit prepares each request on the host while the loop steps the GPU)
     and run the loop until all finish. Returns (solutions, stats) with per-request
-    latency (completion - arrival) and loop counters."""
+    latency (completion - arrival), its parts (intake: arrival to queued, the request
+    prepared on the host; queue: queued to admitted; service: admitted to done, on the
+    loop's clock) and loop counters."""
     import threading
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(requests)))
@@ -62,6 +64,9 @@ def replay(g, requests, rate, seed=0):
     sols = [None] * len(requests)
     lat = np.zeros(len(requests))
     iters = np.zeros(len(requests), dtype=np.int64)
+    intake_s = np.zeros(len(requests))
+    queue_s = np.zeros(len(requests))
+    service_s = np.zeros(len(requests))
     idx = {}
     lock = threading.Lock()
     finished_intake = threading.Event()
@@ -75,6 +80,7 @@ def replay(g, requests, rate, seed=0):
                 if wait > 0:
                     time.sleep(wait)
                 rid = g.serve_submit(*req)
+                intake_s[q] = (time.perf_counter() - t0) - arrivals[q]
                 with lock:
                     idx[rid] = q
         except Exception as e:  # surfaced by the stepping thread
@@ -103,6 +109,8 @@ def replay(g, requests, rate, seed=0):
             sols[q] = x
             lat[q] = (time.perf_counter() - t0) - arrivals[q]
             iters[q] = st["iters"]
+            queue_s[q] = st["t_admit"] - st["t_submit"]
+            service_s[q] = st["t_done"] - st["t_admit"]
             done += 1
         if not active and not queued:
             if finished_intake.is_set() and done >= len(requests):
@@ -111,7 +119,8 @@ def replay(g, requests, rate, seed=0):
     th.join()
     wall = time.perf_counter() - t0
     return sols, {"latency_s": lat, "iters": iters, "wall_s": wall, "steps": steps,
-                  "mean_active": occupancy / max(steps, 1)}
+                  "mean_active": occupancy / max(steps, 1), "intake_s": intake_s, "queue_s": queue_s,
+                  "service_s": service_s}
 
 
 def main(argv=None):
