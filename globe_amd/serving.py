@@ -50,13 +50,14 @@ def make_request(rng, n_cohort, size_lo, size_hi, lam_lo, lam_hi, u_pool=None):
     return indiv, times, y, theta
 
 
-def replay(g, requests, rate, seed=0):
-    """Submit requests at Poisson arrival times (rate per second) from an intake thread
-    (globe_serve_submit prepares each request on the host while the loop steps the GPU)
-    and run the loop until all finish. Returns (solutions, stats) with per-request
-    latency (completion - arrival), its parts (intake: arrival to queued, the request
-    prepared on the host; queue: queued to admitted; service: admitted to done, on the
-    loop's clock) and loop counters."""
+def replay(g, requests, rate, seed=0, intake_threads=4):
+    """Submit requests at Poisson arrival times (rate per second) from intake threads
+    (request q on thread q mod intake_threads; globe_serve_submit prepares each request on
+    the host while the loop steps the GPU) and run the loop until all finish. Returns
+    (solutions, stats) with per-request latency (completion - arrival), its parts (intake:
+    arrival to queued, the request prepared on the host; queue: queued to admitted;
+    service: admitted to done, on the loop's clock) and loop counters. A request the loop
+    rejects stops the intake and raises here."""
     import threading
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(requests)))
@@ -69,54 +70,58 @@ def replay(g, requests, rate, seed=0):
     service_s = np.zeros(len(requests))
     idx = {}
     lock = threading.Lock()
-    finished_intake = threading.Event()
+    stop = threading.Event()
     failure = []
+    nthreads = max(1, min(int(intake_threads), len(requests)))
     t0 = time.perf_counter()
 
-    def intake():
+    def intake(t):
         try:
-            for q, req in enumerate(requests):
+            for q in range(t, len(requests), nthreads):
                 wait = arrivals[q] - (time.perf_counter() - t0)
-                if wait > 0:
-                    time.sleep(wait)
-                rid = g.serve_submit(*req)
+                if (wait > 0 and stop.wait(wait)) or stop.is_set():
+                    return
+                rid = g.serve_submit(*requests[q])
                 intake_s[q] = (time.perf_counter() - t0) - arrivals[q]
                 with lock:
                     idx[rid] = q
         except Exception as e:  # surfaced by the stepping thread
             failure.append(e)
-        finally:
-            finished_intake.set()
+            stop.set()
 
-    th = threading.Thread(target=intake, daemon=True)
-    th.start()
+    threads = [threading.Thread(target=intake, args=(t,), daemon=True) for t in range(nthreads)]
+    for th in threads:
+        th.start()
     steps = 0
     occupancy = 0
     done = 0
-    while done < len(requests):
-        if failure:
-            raise failure[0]
-        _, active, queued = g.serve_step()
-        if active:
-            steps += 1
-            occupancy += active
-        for rid in g.serve_finished():
-            with lock:
-                q = idx.pop(rid, None)
-            if q is None:  # its id is not recorded yet: take it at the next step
-                continue
-            x, st = g.serve_take(rid)
-            sols[q] = x
-            lat[q] = (time.perf_counter() - t0) - arrivals[q]
-            iters[q] = st["iters"]
-            queue_s[q] = st["t_admit"] - st["t_submit"]
-            service_s[q] = st["t_done"] - st["t_admit"]
-            done += 1
-        if not active and not queued:
-            if finished_intake.is_set() and done >= len(requests):
-                break
-            time.sleep(2e-4)  # idle: leave the interpreter to the intake thread
-    th.join()
+    try:
+        while done < len(requests):
+            if failure:
+                raise failure[0]
+            _, active, queued = g.serve_step()
+            if active:
+                steps += 1
+                occupancy += active
+            for rid in g.serve_finished():
+                with lock:
+                    q = idx.pop(rid, None)
+                if q is None:  # its id is not recorded yet: take it at the next step
+                    continue
+                x, st = g.serve_take(rid)
+                sols[q] = x
+                lat[q] = (time.perf_counter() - t0) - arrivals[q]
+                iters[q] = st["iters"]
+                queue_s[q] = st["t_admit"] - st["t_submit"]
+                service_s[q] = st["t_done"] - st["t_admit"]
+                done += 1
+            if not active and not queued:
+                time.sleep(2e-4)  # idle: leave the interpreter to the intake threads
+    finally:
+        # no submit may overlap the caller's next call (globe_serve_close)
+        stop.set()
+        for th in threads:
+            th.join()
     wall = time.perf_counter() - t0
     return sols, {"latency_s": lat, "iters": iters, "wall_s": wall, "steps": steps,
                   "mean_active": occupancy / max(steps, 1), "intake_s": intake_s, "queue_s": queue_s,

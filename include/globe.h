@@ -348,8 +348,9 @@ typedef struct {
  * at the next step. */
 int globe_serve_open(globe_ctx* ctx, const globe_serve_opts* opts);
 /* Copies, validates and queues a request; *id identifies it in globe_serve_take. This one
- * call may run on a second host thread (request intake) concurrently with the thread that
- * calls globe_serve_step, so requests are prepared on the host while the GPU iterates. */
+ * call may run on other host threads (request intake, one or several at once) concurrently
+ * with the thread that calls globe_serve_step, so requests are prepared on the host while
+ * the GPU iterates. */
 int globe_serve_submit(globe_ctx* ctx, const globe_request* req, int64_t* id);
 /* Admits queued requests (FIFO) into free slots while the HBM budget allows, runs one
  * iteration of the block PCG over the occupied slots and retires the requests whose
