@@ -668,12 +668,14 @@ int globe_set_cohort(globe_ctx* c, int64_t nrec, int32_t n, int32_t p, const int
     std::vector<uint8_t> seen((size_t)n, 0);
     for (int64_t r = 0; r < nrec; ++r) {
         const int32_t i = indiv[r];
-        require(i >= 0 && i < n, GLOBE_E_INVALID,
-                "individual index " + std::to_string(i) + " at record " + std::to_string(r) +
-                    " outside [0, n)");
+        // per-record checks: the message is built only on failure
+        if (i < 0 || i >= n)
+            throw Error(GLOBE_E_INVALID, "individual index " + std::to_string(i) + " at record " +
+                                             std::to_string(r) + " outside [0, n)");
         if (r == 0 || indiv[r - 1] != i) {
-            require(!seen[i], GLOBE_E_INVALID,
-                    "records of individual " + std::to_string(i + 1) + " are not contiguous");
+            if (seen[i])
+                throw Error(GLOBE_E_INVALID,
+                            "records of individual " + std::to_string(i + 1) + " are not contiguous");
             seen[i] = 1;
             off.push_back(r);
             ind.push_back(i);

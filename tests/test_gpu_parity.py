@@ -282,11 +282,12 @@ def test_invalid_cohorts_rejected(gpu_ctx_factory):
     g = gpu_ctx_factory()
     t = np.zeros(4)
     X = np.ones((4, 2))
-    with pytest.raises(G.GlobeError):  # individual 0 appears in two runs
-        g.set_cohort(np.array([0, 1, 0, 1]), t, X)
+    with pytest.raises(G.GlobeError, match="records of individual 1 are not contiguous"):
+        g.set_cohort(np.array([0, 1, 0, 1]), t, X)  # individual 0 appears in two runs
     ind = np.array([0, 0, 1, 3], dtype=np.int32)  # id 3 outside [0, n = 3)
     Xf = np.asfortranarray(X)
     assert g.L.globe_set_cohort(g.h, 4, 3, 1, ind.ctypes.data, t.ctypes.data, Xf.ctypes.data, 4) == 1
+    assert b"individual index 3 at record 3 outside [0, n)" in g.L.globe_last_error(g.h)
     ind = np.array([0, 0, 2, 2], dtype=np.int32)  # n = 3 but id 1 never occurs
     assert g.L.globe_set_cohort(g.h, 4, 3, 1, ind.ctypes.data, t.ctypes.data, Xf.ctypes.data, 4) == 1
     g.set_cohort(np.array([0, 0, 1, 1], dtype=np.int32), t, X)
