@@ -301,10 +301,10 @@ __global__ __launch_bounds__(256) void k_grm_sym(const float* __restrict__ tiles
 
 // GZ[q][i] = sig2g_q (sum_h rowp[h][q][i] + sum_{I > J} colp[tile (I, J)][q][a]) for the
 // rows i = 64 J + a of strip J = blockIdx.x; wave w adds I = J + 1 + w, J + 5 + w, ...
-template <int KC>
+// (kc: the partials' column count)
 __global__ __launch_bounds__(256) void k_grm_sym_finish(const double* __restrict__ rowp,
                                                         const double* __restrict__ colp, int32_t n,
-                                                        int32_t ntiles, const double* __restrict__ th,
+                                                        int32_t ntiles, int kc, const double* __restrict__ th,
                                                         double* __restrict__ GZ, int64_t ldg, const int* skip) {
     if (skip && *skip) return;
     const int32_t J = blockIdx.x;
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(256) void k_grm_sym_finish(const double* __restrict
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     __shared__ double red[4][kGT];
     double v = 0.0;
-    for (int32_t I = J + 1 + w; I < ntiles; I += 4) v += colp[tile_index(I, J) * (kGT * KC) + q * kGT + lane];
+    for (int32_t I = J + 1 + w; I < ntiles; I += 4) v += colp[tile_index(I, J) * (kGT * kc) + q * kGT + lane];
     red[w][lane] = v;
     __syncthreads();
     if (w != 0) return;
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(256) void k_grm_sym_finish(const double* __restrict
     if (i >= n) return;
     double r = 0.0;
 #pragma unroll
-    for (int h = 0; h < kSymChunks; ++h) r += rowp[((int64_t)h * KC + q) * n + i];
+    for (int h = 0; h < kSymChunks; ++h) r += rowp[((int64_t)h * kc + q) * n + i];
     const double c = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
     GZ[q * ldg + i] = th[q * 5 + 1] * (r + c);
 }
@@ -336,9 +336,9 @@ __global__ __launch_bounds__(256) void k_grm_sym_finish(const double* __restrict
 //            tile's 64 rows, the transposed LDS read), complete for the tile, written into
 //            colp[tile (I, J)][q][a].
 // Lane layout of v_mfma_f64_4x4x4f64 as in grm_mma: block b of an instruction takes rows
-// (or tile columns) 16 w + 4 b .. + 3. The partials have k_grm_sym<KC>'s layout, so
-// k_grm_sym_finish<KC> adds them in the same fixed order.
-template <int KC>  // 4 or 8
+// (or tile columns) 16 w + 4 b .. + 3. The partials have k_grm_sym's layout with kk
+// columns, so k_grm_sym_finish adds them in the same fixed order.
+template <int KC>  // 4 or 8: columns of the MFMA groups (kk <= KC)
 __global__ __launch_bounds__(256) void k_grm_symx(const float* __restrict__ tiles, int32_t n, int32_t ntiles,
                                                   const double* __restrict__ S0T, int kk,
                                                   double* __restrict__ rowp, double* __restrict__ colp,
@@ -412,7 +412,7 @@ __global__ __launch_bounds__(256) void k_grm_symx(const float* __restrict__ tile
                     for (int e = 0; e < E; ++e)
                         accc[e] = __builtin_amdgcn_mfma_f64_4x4x4f64(gv, xis[4 * s4 * kC4 + 4 * e], accc[e], 0, 0, 0);
                 }
-                double* dst = colp + tile_index(I, J) * (kGT * KC);
+                double* dst = colp + tile_index(I, J) * (kGT * kk);
                 const int col = 16 * w + 4 * ((lane >> 2) & 3) + (lane >> 4);
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
@@ -428,7 +428,7 @@ __global__ __launch_bounds__(256) void k_grm_symx(const float* __restrict__ tile
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int q = 4 * e + (lane & 3);
-        if (q < kk && i < n) rowp[((int64_t)h * KC + q) * n + i] = accr[e];
+        if (q < kk && i < n) rowp[((int64_t)h * kk + q) * n + i] = accr[e];
     }
 }
 
@@ -547,52 +547,43 @@ void launch_grm_product(hipStream_t s, const float* tiles, int32_t n, const doub
     }
 }
 
-// columns of the few-column product's partials: 1, 2, 4 or 8
-static int sym_cols(int k) { return k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : 8; }
-
 size_t grm_sym_part_bytes(int32_t n, int k) {
     const size_t T = (size_t)(n + kGT - 1) / kGT;
-    const size_t kc = (size_t)sym_cols(k);
-    // row partials, column partials, S0T (rows of 4 or 8 doubles)
+    const size_t kc = (size_t)k;
+    // row partials, column partials (k columns each), S0T (rows of 4 or 8 doubles)
     return sizeof(double) * (kc * ((size_t)kSymChunks * (size_t)n + T * (T + 1) / 2 * kGT) +
-                             T * kGT * (kc <= 4 ? 4 : 8));
+                             T * kGT * (k <= 4 ? 4 : 8));
 }
 
 void launch_grm_product_sym(hipStream_t s, const float* tiles, int32_t n, const double* S0, int64_t ldn, int k,
                             const double* theta_cols, double* GZ, int64_t ldg, const int* skip, double* scratch) {
     if (n == 0) return;
     const int32_t ntiles = (n + kGT - 1) / kGT;
-    const int kc = sym_cols(k);
-    const int c4 = kc <= 4 ? 4 : 8;
+    const int c4 = k <= 4 ? 4 : 8;
     const int64_t T = ntiles;
     double* rowp = scratch;
-    double* colp = rowp + (size_t)kSymChunks * kc * n;
-    double* S0T = colp + (size_t)(T * (T + 1) / 2) * kGT * kc;
+    double* colp = rowp + (size_t)kSymChunks * k * n;
+    double* S0T = colp + (size_t)(T * (T + 1) / 2) * kGT * k;
     const dim3 g(ntiles, kSymChunks), gf(ntiles, k);
-    if (kc <= 2) {
+    if (k <= 2) {
         // one or two columns: the vector-ALU form (lane = tile column, the x values in
         // registers) streams as fast as the MFMA one and needs no transposed S0
-        if (kc == 1) {
+        if (k == 1)
             k_grm_sym<1><<<g, 256, 0, s>>>(tiles, n, ntiles, S0, ldn, k, rowp, colp, skip);
-            k_grm_sym_finish<1><<<gf, 256, 0, s>>>(rowp, colp, n, ntiles, theta_cols, GZ, ldg, skip);
-        } else {
+        else
             k_grm_sym<2><<<g, 256, 0, s>>>(tiles, n, ntiles, S0, ldn, k, rowp, colp, skip);
-            k_grm_sym_finish<2><<<gf, 256, 0, s>>>(rowp, colp, n, ntiles, theta_cols, GZ, ldg, skip);
-        }
-        HIP_CHECK(hipGetLastError());
-        return;
-    }
-    const int64_t npad = T * kGT;
-    const unsigned gt = (unsigned)std::min<int64_t>((npad * c4 + 255) / 256, 1024);
-    k_grm_s0t<<<gt, 256, 0, s>>>(S0, ldn, n, k, c4, npad, S0T, skip);
-    HIP_CHECK(hipGetLastError());
-    if (kc == 4) {
-        k_grm_symx<4><<<g, 256, 0, s>>>(tiles, n, ntiles, S0T, k, rowp, colp, skip);
-        k_grm_sym_finish<4><<<gf, 256, 0, s>>>(rowp, colp, n, ntiles, theta_cols, GZ, ldg, skip);
     } else {
-        k_grm_symx<8><<<g, 256, 0, s>>>(tiles, n, ntiles, S0T, k, rowp, colp, skip);
-        k_grm_sym_finish<8><<<gf, 256, 0, s>>>(rowp, colp, n, ntiles, theta_cols, GZ, ldg, skip);
+        const int64_t npad = T * kGT;
+        const unsigned gt = (unsigned)std::min<int64_t>((npad * c4 + 255) / 256, 1024);
+        k_grm_s0t<<<gt, 256, 0, s>>>(S0, ldn, n, k, c4, npad, S0T, skip);
+        HIP_CHECK(hipGetLastError());
+        if (k <= 4)
+            k_grm_symx<4><<<g, 256, 0, s>>>(tiles, n, ntiles, S0T, k, rowp, colp, skip);
+        else
+            k_grm_symx<8><<<g, 256, 0, s>>>(tiles, n, ntiles, S0T, k, rowp, colp, skip);
     }
+    HIP_CHECK(hipGetLastError());
+    k_grm_sym_finish<<<gf, 256, 0, s>>>(rowp, colp, n, ntiles, k, theta_cols, GZ, ldg, skip);
     HIP_CHECK(hipGetLastError());
 }
 
