@@ -141,9 +141,9 @@ def test_mixed_grm_aireml_within_1e4(case, gpu_ctx_factory):
 def test_grm_products_at_70k_individuals(gpu_ctx_factory):
     """N = 70,000 (1,094 tile rows): the mixed product sums its int32 digit classes over
     5 chunks of at most 256 tile columns (the int32 bound), the FP64 product over its 4;
-    both agree, for 1 to 40 columns (two passes), with 4 digits within 1e-8. The GRM is
-    built on the device from 2,000 synthetic SNPs (9.6 GB of FP32 tiles, 19 GB of digit
-    tiles)."""
+    both agree, for 1 to 40 columns (two passes), with 4 digits within 1e-8, and the
+    read-once 5-column FP64 product agrees with the strip kernel. The GRM is built on the
+    device from 2,000 synthetic SNPs (9.6 GB of FP32 tiles, 19 GB of digit tiles)."""
     _say("grm-70k")
     rng = np.random.default_rng(70)
     c = synth.simulate_cohort(70_000, 0.2, rng)
@@ -157,3 +157,9 @@ def test_grm_products_at_70k_individuals(gpu_ctx_factory):
         v64 = g.computeVx_multi(x, multGRM_uhat=True, precision=G.PREC_FP64)
         vm = g.computeVx_multi(x, multGRM_uhat=True, precision=G.PREC_MIXED)
         assert rel(vm, v64) < 1e-8, k
+    # the read-once 4 x 4 MFMA product (5 columns) against the same columns of the
+    # 16-column strip product, at 1,094 tile strips
+    x = np.asfortranarray(rng.standard_normal((c.nrec, 16)))
+    wide = g.computeVx_multi(x, multGRM_uhat=True, precision=G.PREC_FP64)[:, :5]
+    five = g.computeVx_multi(np.asfortranarray(x[:, :5]), multGRM_uhat=True, precision=G.PREC_FP64)
+    assert rel(five, wide) < 1e-13
