@@ -426,9 +426,15 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
     launch_pcg_init(s, dst, part_a, part_b, part_c, nmax, k);
     const int* skip = &dst->done;
     // Iterations are enqueued in batches that double while an iteration is short
-    // (launch-bound problems); when one iteration takes over 2 ms the host checks the
-    // done flag after every iteration, so no skipped launches are issued.
+    // (launch-bound problems); with a fixed operator the batch is also capped one above
+    // the iterations the residual's geometric decay over the last batch predicts are left
+    // (iterations enqueued past convergence return at once, but their launches still
+    // cost; a relaxed mixed solve keeps doubling, since each batch boundary it adds is a
+    // host round trip that its changing digits make hard to predict); when one iteration
+    // takes over 2 ms the host checks the done flag after every iteration, so no skipped
+    // launches are issued.
     int batch = 1, queued = 0;
+    double last_ratio = 1.0;
     const size_t prof_mark = c->prof.size();
     auto tb = std::chrono::steady_clock::now();
     // mixed precision: digits per batch from the residual ratio at its start (4 for the
@@ -555,7 +561,14 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
         const auto tn = std::chrono::steady_clock::now();
         const double per_iter = std::chrono::duration<double>(tn - tb).count() / batch;
         tb = tn;
-        batch = per_iter > 2e-3 ? 1 : std::min(batch * 2, 16);
+        int left = 16;
+        const double ratio = hs->max_ratio;
+        if (!relax && ratio > 0.0 && ratio < last_ratio && ratio > o.tol) {
+            const double per = std::log(ratio / last_ratio) / batch;  // log of the decay per iteration
+            left = (int)std::min(16.0, std::ceil(std::log(o.tol / ratio) / per) + 1.0);
+        }
+        last_ratio = ratio;
+        batch = per_iter > 2e-3 ? 1 : std::max(1, std::min({batch * 2, 16, left}));
     }
     c->prof_iter = -1;
     if (c->prof_on) {
