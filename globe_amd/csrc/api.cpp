@@ -154,15 +154,21 @@ void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int prec
     hipStream_t s = c->stream;
     if (c->gen == 2) {
         const bool mixed = precision == GLOBE_PREC_MIXED;
+        // FP64: up to 8 columns read each stored tile once; wider products run passes of
+        // up to 32 columns over the strips, and a last group of at most 8 columns (k = 33 to
+        // 40, 65 to 72, ...) takes the read-once product instead of a strip pass
+        const int tail = mixed || k <= kSymMaxCols ? 0 : (k % 32 <= kSymMaxCols ? k % 32 : 0);
+        const int kstrip = mixed || k <= kSymMaxCols ? 0 : k - tail;
+        const int ksym = mixed ? 0 : (k <= kSymMaxCols ? k : tail);
         void* scratch = nullptr;
+        void* scratch_sym = nullptr;
         if (mixed) {
             ensure_grm_digits(c);
             ensure_gdiag(c);
             scratch = ws_bytes(c, c->w_grm8_part, grm_i8_part_bytes(n));
-        } else if (k <= kSymMaxCols) {
-            scratch = ws_bytes(c, c->w_grm_sym, grm_sym_part_bytes(n, k));
         } else {
-            scratch = ws_bytes(c, c->w_grm_part, grm_product_part_bytes(n));
+            if (kstrip > 0) scratch = ws_bytes(c, c->w_grm_part, grm_product_part_bytes(n));
+            if (ksym > 0) scratch_sym = ws_bytes(c, c->w_grm_sym, grm_sym_part_bytes(n, ksym));
         }
         // timing: which 2 = FP64 product, 3 = int8 digit product; algorithmic bytes = the
         // FP32 triangle once, flops = 2 N^2 k
@@ -170,16 +176,16 @@ void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int prec
         const bool prof = c->prof_on && !c->arena.base;
         if (prof) {
             const double np = (double)((n + 63) / 64 * 64), L = c->limbs_mixed;
-            const double cols = mixed ? 16.0 * ((k + 15) / 16) : (double)grm_padded_cols(k);
+            const double cols = 16.0 * ((k + 15) / 16);
             HIP_CHECK(hipEventCreate(&e.a));
             HIP_CHECK(hipEventCreate(&e.b));
             e.pass = mixed ? 3 : 2;
             e.iter = c->prof_iter;
             e.bytes = 4.0 * (double)n * (n + 1) / 2;
             e.flops = 2.0 * (double)n * n * k;
+            const int symc = ksym <= 2 ? ksym : ksym <= 4 ? 4 : 8;
             e.issued = mixed ? 2.0 * np * np * cols * (L * (L + 1) / 2 + L - 1)
-                             : (k <= kSymMaxCols ? 2.0 * np * np * (k <= 2 ? k : k <= 4 ? 4 : 8)
-                                                 : 2.0 * np * np * cols);
+                             : 2.0 * np * np * ((kstrip > 0 ? grm_padded_cols(kstrip) : 0) + symc);
             // external: inside a captured PCG iteration the record is a graph node that
             // stamps the event on every replay
             HIP_CHECK(hipEventRecordWithFlags(e.a, s, hipEventRecordExternal));
@@ -187,12 +193,15 @@ void genetic_product(globe_ctx* c, const double* S0, int k, double* GZ, int prec
         if (mixed)
             launch_grm_product_i8(s, c->d_grm8.as<int8_t>(), grm_digit_rexp(c->d_grm8_aux.p, n), n, S0, n, k,
                                   c->limbs_mixed, c->d_gdiag.as<double>(), th, GZ, n, skip, scratch);
-        else if (k <= kSymMaxCols)
-            launch_grm_product_sym(s, c->d_grm.as<float>(), n, S0, n, k, th, GZ, n, skip,
+        else {
+            if (kstrip > 0)
+                launch_grm_product(s, c->d_grm.as<float>(), n, S0, n, kstrip, th, GZ, n, skip,
                                    static_cast<double*>(scratch));
-        else
-            launch_grm_product(s, c->d_grm.as<float>(), n, S0, n, k, th, GZ, n, skip,
-                               static_cast<double*>(scratch));
+            if (ksym > 0)
+                launch_grm_product_sym(s, c->d_grm.as<float>(), n, S0 + (size_t)kstrip * n, n, ksym,
+                                       th + 5 * kstrip, GZ + (size_t)kstrip * n, n, skip,
+                                       static_cast<double*>(scratch_sym));
+        }
         if (prof) {
             HIP_CHECK(hipEventRecordWithFlags(e.b, s, hipEventRecordExternal));
             c->prof.push_back(e);
