@@ -435,15 +435,18 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
     launch_pcg_init(s, dst, part_a, part_b, part_c, nmax, k);
     const int* skip = &dst->done;
     // Iterations are enqueued in batches that double while an iteration is short
-    // (launch-bound problems); with a fixed operator the batch is also capped one above
-    // the iterations the residual's geometric decay over the last batch predicts are left
-    // (iterations enqueued past convergence return at once, but their launches still
-    // cost; a relaxed mixed solve keeps doubling, since each batch boundary it adds is a
-    // host round trip that its changing digits make hard to predict); when one iteration
-    // takes over 2 ms the host checks the done flag after every iteration, so no skipped
-    // launches are issued.
+    // (launch-bound problems), capped one above the iterations the residual's geometric
+    // decay over the last batch predicts are left (iterations enqueued past convergence
+    // return at once, but their launches still cost). A relaxed mixed solve whose product
+    // costs less with fewer digits (a GRM, or 17 or more digit columns) also ends a batch
+    // where that decay predicts its digits can drop, since a batch keeps the digits of its
+    // start; one whose product does not (a thin genotype pass, bound by the rows it
+    // streams) keeps doubling, as each batch boundary is a host round trip. When one
+    // iteration takes over 2 ms the host checks the done flag after every iteration, so no
+    // skipped launches are issued.
     int batch = 1, queued = 0;
     double last_ratio = 1.0;
+    const bool digits_matter = c->gen == 2 || kLimbsMixed * k > 16;
     const size_t prof_mark = c->prof.size();
     auto tb = std::chrono::steady_clock::now();
     // mixed precision: digits per batch from the residual ratio at its start (4 for the
@@ -572,9 +575,15 @@ void run_pcg(globe_ctx* c, const double* B, int64_t ldb, int k, double* X, int64
         tb = tn;
         int left = 16;
         const double ratio = hs->max_ratio;
-        if (!relax && ratio > 0.0 && ratio < last_ratio && ratio > o.tol) {
+        if ((!relax || digits_matter) && ratio > 0.0 && ratio < last_ratio && ratio > o.tol) {
             const double per = std::log(ratio / last_ratio) / batch;  // log of the decay per iteration
             left = (int)std::min(16.0, std::ceil(std::log(o.tol / ratio) / per) + 1.0);
+            const int L = relax ? relaxed_limbs(ratio, o.tol) : 2;
+            if (L > 2) {
+                // the ratio below which relaxed_limbs gives one digit fewer
+                const double drop = 0.01 * o.tol / (8.0 * std::ldexp(1.0, -(8 * (L - 1) - 2)));
+                left = std::min(left, std::max(1, (int)std::ceil(std::log(drop / ratio) / per)));
+            }
         }
         last_ratio = ratio;
         batch = per_iter > 2e-3 ? 1 : std::max(1, std::min({batch * 2, 16, left}));
