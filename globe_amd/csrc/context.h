@@ -80,9 +80,10 @@ struct globe_ctx {
     // PCG iterations replayed from captured HIP graphs (globe_set_graphs; off by default:
     // a replay runs no faster than the eager stream on ROCm 7.2)
     bool graphs_on = false;
-    // Rows of the genotype product's individual-major pass that a caller needs (the
-    // serving loop: its requests' individuals), ng_rows entries (0: all rows); thin
-    // contractions form only these rows of G S0. globe_set_row_gather switches it off.
+    // Individuals a caller's genotype products involve (the serving loop: its requests'
+    // individuals; S0 is zero elsewhere), gather_n entries (0: all); thin contractions
+    // run pass 1 over their rows of the individual-major layout only and form only their
+    // rows of G S0. globe_set_row_gather switches it off.
     const int32_t* gather_idx = nullptr;
     int64_t gather_n = 0;
     bool gather_on = true;
@@ -198,8 +199,9 @@ int relaxed_limbs(double ratio, double tol);
 // with col_limbs[c] int8 digits (1..7) for column c of the right-hand side, at most 128
 // digit columns in all (U_given: pass 1 already done by the caller, u = U_given, m x k;
 // S0 unused; then only rows row0 .. n - 1 of GZ are formed). ridx (device, ng entries,
-// ng a multiple of 512): when the contraction is thin (at most 16 digit columns), only
-// rows ridx[0 .. ng) of GZ are formed (the other rows keep what they held)
+// ng a multiple of 512, S0 zero outside them): when the contraction is thin (at most 16
+// digit columns), pass 1 contracts over those individuals only and only rows
+// ridx[0 .. ng) of GZ are formed (the other rows keep what they held)
 void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, double* GZ,
                        int64_t ldg, const int* skip, const int8_t* col_limbs, const double* theta_cols,
                        const double* U_given = nullptr, int64_t row0 = 0, const int32_t* ridx = nullptr,

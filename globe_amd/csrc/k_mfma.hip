@@ -297,6 +297,180 @@ __global__ __launch_bounds__(W * 64) void k_geno_thin(const uint8_t* __restrict_
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
+// ---------------------------------------------------------------- gathered pass 1
+// Pass 1 (u = W^T S0 over SNPs) when S0 is zero outside a list of individuals ridx[0 ..
+// ng): instead of every individual's column of the SNP-major layout, read only the listed
+// individuals' rows of the individual-major layout. The contraction then runs over
+// individuals, which are rows of that layout, so the MFMA operand (16 individuals of one
+// SNP per lane) is a column of a staged tile: gfx950's transposing LDS read
+// ds_read_b64_tr_b8 supplies it. Lane s of a 16-lane group gives the address of the 8
+// bytes at (row s >> 1, half s & 1) of an 8-row x 16-byte block, and lane m receives byte
+// m of those 8 rows (probed on gfx950); two reads give 16 individuals. Each byte holds 4
+// SNPs, so one pair of reads feeds 4 MFMAs (SNP 4 c + j of byte c, j = 0..3).
+//
+// A workgroup of 8 waves owns RB bytes of the rows (4 RB SNPs; wave w the 16-byte columns
+// RB / 128 w ..) and split sp of the listed individuals (64-individual stages lo .. hi),
+// streamed one stage at a time: each stage copies 64 rows x RB bytes and the 16 x 64
+// digit block global -> LDS into a ring of kTgNS slots, one wait + barrier per stage. The
+// copy lanes take their rows' indices 16 stages at a time into registers (two groups
+// held), so no index load waits inside the ring. 16-byte piece p of row r sits at slot p ^
+// tg_swz(r), and digit chunk g of column n at g ^ ((n >> 2) & 3), so the transposing reads
+// and the digit reads touch each LDS bank once per 16 lanes.
+// Dt: digits [16 columns][ldd], column n of listed individual t at n ldd + t (0 past J).
+// out[sp][s][16]: the split's int32 sums for SNP s (every split writes all its rows).
+constexpr int kTgNS = 3;      // ring slots
+constexpr int kTgGroup = 16;  // stages per group of row indices
+
+typedef int v2i_t __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) const uint8_t lds_cu8;
+
+// LDS byte address of a pointer into a __shared__ array
+__device__ inline uint32_t lds_addr(const uint8_t* p) { return (uint32_t)(uintptr_t)(lds_cu8*)p; }
+
+template <int RB>
+__device__ inline int tg_row_slot(int r, int p) {
+    // 128-byte rows: rows r and r + 2 share banks; 256-byte rows: every row does
+    const int f = RB == 128 ? ((r >> 1) & 7) : (r & 7);
+    return r * RB + ((p ^ f) << 4);
+}
+__device__ inline int tg_dig_slot(int nn, int g) { return nn * 64 + ((g ^ ((nn >> 2) & 3)) << 4); }
+
+template <int RB>
+__global__ __launch_bounds__(512) void k_geno_tg(const uint8_t* __restrict__ ind, int64_t pitch, int64_t m,
+                                                 const int32_t* __restrict__ ridx, int64_t ng,
+                                                 const int8_t* __restrict__ Dt, int64_t ldd, int nsplit,
+                                                 int32_t* __restrict__ out, const int* skip) {
+    if (skip && *skip) return;
+    constexpr int kTile = 64 * RB;            // staged rows
+    constexpr int kSlot = kTile + 1024;       // + digits
+    constexpr int kPP = RB / 16;              // 16-byte pieces per row
+    constexpr int kRowsPerKB = 1024 / RB;     // rows per copy instruction
+    constexpr int kCopies = RB / 128;         // row copies per wave and stage (= its columns)
+    static_assert(RB == 128 || RB == 256, "128- or 256-byte rows");
+    __shared__ __attribute__((aligned(1024))) uint8_t lds[kTgNS * kSlot];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t b0 = (int64_t)blockIdx.x * RB;  // byte offset of the workgroup's SNPs
+    const int sp = blockIdx.y;
+    const int64_t nst = ng / 64;
+    const int64_t lo = sp * nst / nsplit, hi = (sp + 1) * nst / nsplit;
+    const int64_t nt = hi - lo;
+    // copy roles: copy instruction i of wave w fills LDS bytes (8 i + w) KB: rows
+    // cr[i] = (8 i + w) kRowsPerKB + lane / kPP, piece slot lane % kPP (the global piece
+    // swizzled into it, clamped inside the row for a last workgroup past the pitch); digit
+    // column dn = 2 w + lane / 4 and its chunk slot lane % 4 (lanes 0..7)
+    int cr[kCopies];
+    const uint8_t* ibase[kCopies];
+#pragma unroll
+    for (int i = 0; i < kCopies; ++i) {
+        cr[i] = (8 * i + w) * kRowsPerKB + lane / kPP;
+        const int q = lane % kPP;
+        const int f = RB == 128 ? ((cr[i] >> 1) & 7) : (cr[i] & 7);
+        const int64_t off = b0 + 16 * (q ^ f);
+        ibase[i] = ind + (off + 16 <= pitch ? off : pitch - 16);
+    }
+    const int dn = 2 * w + (lane >> 2), dq = lane & 3;
+    const int dg = dq ^ ((dn >> 2) & 3);
+    const int8_t* const dbase = Dt + (int64_t)dn * ldd + 16 * dg;
+    v4i acc[kCopies][4];
+#pragma unroll
+    for (int i = 0; i < kCopies; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = v4i{0, 0, 0, 0};
+    const int g = lane >> 4, sl = lane & 15;
+    if (nt > 0) {
+        // the copy rows' indices for stage t of the split (the last stage's past the end)
+        auto row_of = [&](int64_t t, int i) -> int32_t {
+            return ridx[(lo + (t < nt ? t : nt - 1)) * 64 + cr[i]];
+        };
+        auto issue = [&](int64_t t, const int32_t (&r)[kCopies], int slot) {
+            const int64_t t0 = (lo + (t < nt ? t : nt - 1)) * 64;
+            uint8_t* dst = lds + slot * kSlot;
+#pragma unroll
+            for (int i = 0; i < kCopies; ++i)
+                __builtin_amdgcn_global_load_lds((const void*)(ibase[i] + (int64_t)r[i] * pitch),
+                                                 (lds_void*)(dst + 1024 * (8 * i + w)), 16, 0, 0);
+            if (lane < 8)
+                __builtin_amdgcn_global_load_lds((const void*)(dbase + t0), (lds_void*)(dst + kTile + 128 * w),
+                                                 16, 0, 0);
+        };
+        int32_t cur[kTgGroup][kCopies], nxt[kTgGroup][kCopies];
+#pragma unroll
+        for (int q = 0; q < kTgGroup; ++q)
+#pragma unroll
+            for (int i = 0; i < kCopies; ++i) {
+                cur[q][i] = row_of(q, i);
+                nxt[q][i] = row_of(kTgGroup + q, i);
+            }
+#pragma unroll
+        for (int u = 0; u < kTgNS - 1; ++u) issue(u, cur[u], u);
+        int cs = 0;
+        for (int64_t t0 = 0; t0 < nt; t0 += kTgGroup) {
+#pragma unroll
+            for (int q = 0; q < kTgGroup; ++q) {
+                const int64_t t = t0 + q;
+                if (t >= nt) continue;  // uniform in the workgroup
+                // this wave's copies of stage t have landed (the later stage's are in
+                // flight); after the barrier every wave's have, and no wave still reads the
+                // slot the next copies overwrite
+                asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"((kCopies + 1) * (kTgNS - 2)) : "memory");
+                constexpr int ahead = kTgNS - 1;
+                if (q + ahead < kTgGroup)
+                    issue(t + ahead, cur[(q + ahead) % kTgGroup], cs == 0 ? kTgNS - 1 : cs - 1);
+                else
+                    issue(t + ahead, nxt[(q + ahead) % kTgGroup], cs == 0 ? kTgNS - 1 : cs - 1);
+                __builtin_amdgcn_sched_barrier(0);
+                const uint8_t* slot = lds + cs * kSlot;
+                const v4i bv = *reinterpret_cast<const v4i*>(slot + kTile + tg_dig_slot(sl, g));
+                const int rr = 16 * g + (sl >> 1);
+#pragma unroll
+                for (int i = 0; i < kCopies; ++i) {
+                    const int pc = kCopies * w + i;  // the 16-byte column of every row
+                    // written as asm: the compiler would otherwise wait for every copy in
+                    // flight (vmcnt(0)) before a transposing read, taking it for a read of
+                    // the copies' destination; both reads complete inside the block
+                    const uint32_t a1 = lds_addr(slot + tg_row_slot<RB>(rr, pc) + 8 * (sl & 1));
+                    const uint32_t a2 = lds_addr(slot + tg_row_slot<RB>(rr + 8, pc) + 8 * (sl & 1));
+                    v2i_t lo8, hi8;
+                    asm volatile("ds_read_b64_tr_b8 %0, %2\n\tds_read_b64_tr_b8 %1, %3\n\ts_waitcnt lgkmcnt(0)"
+                                 : "=&v"(lo8), "=&v"(hi8)
+                                 : "v"(a1), "v"(a2)
+                                 : "memory");
+                    const uint32_t wd[4] = {(uint32_t)lo8[0], (uint32_t)lo8[1], (uint32_t)hi8[0], (uint32_t)hi8[1]};
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        v4i av;
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) av[e] = (int)((wd[e] >> (2 * j)) & kFieldMask);
+                        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bv, acc[i][j], 0, 0, 0);
+                    }
+                }
+                cs = cs + 1 == kTgNS ? 0 : cs + 1;
+            }
+            // the next group's indices move up; the one after is read now, 16 stages
+            // before its first use
+#pragma unroll
+            for (int q = 0; q < kTgGroup; ++q)
+#pragma unroll
+                for (int i = 0; i < kCopies; ++i) {
+                    cur[q][i] = nxt[q][i];
+                    nxt[q][i] = row_of(t0 + 2 * kTgGroup + q, i);
+                }
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    }
+    // accumulator element v of lane (g, column sl) for column piece pc: MFMA row 4 g + v =
+    // byte 16 pc + 4 g + v of the workgroup's RB, SNP 4 (byte) + j
+#pragma unroll
+    for (int i = 0; i < kCopies; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t snp = 4 * (b0 + 16 * (kCopies * w + i) + 4 * g + v) + j;
+                if (snp < m) out[plan::part_idx(sp, m, snp, 16, sl)] = acc[i][j][v];
+            }
+}
+
 // ---------------------------------------------------------------- wide kernel
 // Block shape <W waves, NS ring stages, CG column groups, RA row fragments per wave>:
 // the waves form W / CG row groups x CG column groups; a wave owns RA row fragments of
@@ -681,6 +855,25 @@ __global__ __launch_bounds__(256) void k_quantize(const double* __restrict__ v, 
         if (threadIdx.x == 0) part[c * kMaxParts + blockIdx.x] = r;
     }
 }
+// Dt[n][t] (n < 16, t < ng): digit lm.dig[n] of column lm.col[n] of round(S0[c][ridx[t]]
+// 2^sh_c), the digits k_quantize forms for that individual; 0 for n >= J.
+__global__ __launch_bounds__(256) void k_quantize_gather(const double* __restrict__ v, int64_t ldv,
+                                                         const int32_t* __restrict__ ridx, int64_t ng,
+                                                         const LimbMap lm, int J, const int* __restrict__ sh,
+                                                         int8_t* __restrict__ Dt, int64_t ldd, const int* skip) {
+    if (skip && *skip) return;
+    const int nn = blockIdx.y;
+    for (int64_t t = blockIdx.x * 256ll + threadIdx.x; t < ng; t += (int64_t)gridDim.x * 256) {
+        int dig = 0;
+        if (nn < J) {
+            const int c = lm.col[nn], d = lm.dig[nn];
+            long long q = llrint(ldexp(v[c * ldv + ridx[t]], sh[c]));
+            for (int a = 0; a <= d; ++a) dig = balanced_digit(q);
+        }
+        Dt[nn * ldd + t] = (int8_t)dig;
+    }
+}
+
 // blocks per column of k_quantize
 inline int quantize_parts(int64_t Kpad) { return parts_for(Kpad); }
 __global__ __launch_bounds__(256) void k_tune_fill(double* __restrict__ v, int64_t count, uint64_t seed) {
@@ -1191,7 +1384,46 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
                                                  1.0 / (double)m, U, m, part, sk, nullptr);
         HIP_CHECK(hipGetLastError());
     };
-    epi1(contract(0, c->d_snp.as<uint8_t>(), c->pitch_s, m, n, Kpad1, nss1, epi1), skip);
+    if (gather) {
+        // S0 is zero outside the listed individuals: contract over their rows of the
+        // individual-major layout (k_geno_tg) instead of every individual's column of the
+        // SNP-major one. The digits are k_quantize's for those individuals (k_quantize
+        // above still forms T, the column sums); Bq's pass-1 digits are not read, so the
+        // listed individuals' digits take its place. Same integer sums, same epilogue.
+        require(J16 == 16 && (size_t)16 * (size_t)ng <= bq, GLOBE_E_INVALID, "gathered pass 1 shape");
+        int8_t* Dt = reinterpret_cast<int8_t*>(Bq);
+        k_quantize_gather<<<dim3((unsigned)std::min<int64_t>((ng + 255) / 256, 1024), 16), 256, 0, s>>>(
+            S0, ldn, ridx, ng, lm, J, sh1, Dt, ng, skip);
+        HIP_CHECK(hipGetLastError());
+        constexpr int kRB = 256;                                  // row bytes per workgroup
+        const int64_t R = (c->pitch_i + kRB - 1) / kRB;           // workgroups of 4 kRB SNPs
+        const int64_t nst = ng / 64;
+        const int64_t pmax = std::max<int64_t>(1, (int64_t)(ob / ((size_t)m * 16 * sizeof(int32_t))));
+        // enough workgroups for four per CU, each split at least 16 stages long
+        const int P = (int)std::max<int64_t>(1, std::min({(4 * (int64_t)ncu + R - 1) / R, nst / 16, pmax, (int64_t)8}));
+        globe_ctx::ProfEvent e{};
+        const bool prof = c->prof_on && !noalloc;
+        if (prof) {
+            HIP_CHECK(hipEventCreate(&e.a));
+            HIP_CHECK(hipEventCreate(&e.b));
+            e.pass = 0;
+            e.iter = c->prof_iter;
+            e.bytes = (double)ng * (double)((m + 3) / 4);
+            e.flops = 2.0 * (double)ng * (double)m * k;
+            e.issued = 2.0 * (double)ng * (double)(4 * c->pitch_i) * J16;
+            HIP_CHECK(hipEventRecordWithFlags(e.a, s, hipEventRecordExternal));
+        }
+        k_geno_tg<kRB><<<dim3((unsigned)R, (unsigned)P), 512, 0, s>>>(c->d_ind.as<uint8_t>(), c->pitch_i, m, ridx,
+                                                                     ng, Dt, ng, P, out, skip);
+        HIP_CHECK(hipGetLastError());
+        if (prof) {
+            HIP_CHECK(hipEventRecordWithFlags(e.b, s, hipEventRecordExternal));
+            c->prof.push_back(e);
+        }
+        epi1(P, skip);
+    } else {
+        epi1(contract(0, c->d_snp.as<uint8_t>(), c->pitch_s, m, n, Kpad1, nss1, epi1), skip);
+    }
     }
     // pass 2: GZ[c][i] = sum_s g_is u[c][s] - C[c] on the individual-major rows
     k_finish_cols<<<1, kFinishThreads, 0, s>>>(part, (int)eb1, k, 1, lm, sh2, nullptr, skip);

@@ -383,9 +383,10 @@ int globe_set_graphs(globe_ctx* ctx, int on);
 
 /* ---- serving: rows of the genotype product (no reference counterpart) -------------- */
 /* on (the default): while at most 60% of the cohort's individuals belong to the serving
- * loop's requests and the contraction is thin (at most 16 digit columns), the
- * individual-major pass forms only their rows of G S0 (the other rows are never read);
- * off: every row. Results are identical. */
+ * loop's requests and the contraction is thin (at most 16 digit columns), both genotype
+ * passes read only their rows of the individual-major layout: pass 1 contracts over those
+ * individuals only (S0 is zero elsewhere), pass 2 forms only their rows of G S0 (the
+ * other rows are never read); off: every individual and row. Results are identical. */
 int globe_set_row_gather(globe_ctx* ctx, int on);
 
 /* ---- contraction variants (no reference counterpart: validation of the autotuner) ---- */
