@@ -382,7 +382,7 @@ int globe_set_mixed_digits(globe_ctx* ctx, int digits);
 int globe_set_graphs(globe_ctx* ctx, int on);
 
 /* ---- serving: rows of the genotype product (no reference counterpart) -------------- */
-/* on (the default): while at most 60% of the cohort's individuals belong to the serving
+/* on (the default): while at most 80% of the cohort's individuals belong to the serving
  * loop's requests and the contraction is thin (at most 16 digit columns), both genotype
  * passes read only their rows of the individual-major layout: pass 1 contracts over those
  * individuals only (S0 is zero elsewhere), pass 2 forms only their rows of G S0 (the
