@@ -206,6 +206,29 @@ def plan_replay(digit_cols, m, n, r0=0, ncu=256, thin_bpc=(1,) * 8, shrink=(0, 0
             "row_margin2": int(st[4]), "digit_bytes": int(st[5]), "part_bytes": int(st[6]), "wide": bool(st[7])}
 
 
+def plan_replay_tg(m, n, ng, ncu=256, thin_bpc=(1,) * 8, shrink=(0, 0, 0)):
+    """Host replay of the gathered pass 1 (k_geno_tg) over ng listed individuals of an
+    n-individual cohort with m kept SNPs (libglobe_plan_replay.so; no GPU): raises
+    AssertionError with the first read or store outside its buffer, LDS slot filled
+    twice or read for the wrong piece, or SNP a split does not store exactly once;
+    returns the launch (workgroups, splits), the row-read margin and the digit and
+    partial buffers' bytes. shrink: bytes taken off the packed layout, the digit matrix
+    and the partial buffer."""
+    L = C.CDLL(REPLAY_PATH)
+    f = L.globe_plan_replay_tg
+    f.argtypes = [C.c_int64, C.c_int64, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p,
+                  C.c_int]
+    f.restype = C.c_int
+    bpc = np.ascontiguousarray(thin_bpc, dtype=np.int32)
+    shr = np.ascontiguousarray(shrink, dtype=np.int64)
+    st = np.zeros(5, dtype=np.int64)
+    msg = C.create_string_buffer(512)
+    rc = f(int(m), int(n), int(ng), int(ncu), _p(bpc), _p(shr), _p(st), msg, 512)
+    assert rc == 0, msg.value.decode()
+    return {"workgroups": int(st[0]), "splits": int(st[1]), "row_margin": int(st[2]), "digit_bytes": int(st[3]),
+            "part_bytes": int(st[4])}
+
+
 class DeviceBuffer:
     """Device allocation owned by a context (for the *_dev entry points)."""
 

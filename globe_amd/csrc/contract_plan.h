@@ -317,6 +317,61 @@ __host__ __device__ inline int64_t bq_index(int64_t kap, int jj, int jf, int nfr
     return ((ks * nfrag + jf) * 64 + ln) * 16 + qslot;
 }
 
+// ---------------------------------------------------------------- gathered pass 1
+// k_geno_tg: pass 1 over the rows of ng listed individuals of the individual-major layout.
+// Workgroup (x, sp) owns bytes RB x .. RB x + RB - 1 of every row (4 RB SNPs) and split sp
+// of the ng / 64 stages of 64 listed individuals; its 8 waves copy each stage's 64 rows x
+// RB bytes into LDS with 1 KB copy instructions (RB / 128 per wave), 16-byte pieces
+// swizzled by row.
+template <int RB> __host__ __device__ constexpr int tg_pieces() { return RB / 16; }
+template <int RB> __host__ __device__ constexpr int tg_copies() { return RB / 128; }
+// LDS slot of piece p in row r is p ^ tg_swz(r): 128-byte rows put rows r and r + 2 in the
+// same banks, 256-byte rows every row
+template <int RB> __host__ __device__ inline int tg_swz(int r) { return RB == 128 ? ((r >> 1) & 7) : (r & 7); }
+// stage row (0..63) filled by copy instruction i of wave w, lane `lane`
+template <int RB> __host__ __device__ inline int tg_copy_row(int i, int w, int lane) {
+    return (8 * i + w) * (1024 / RB) + lane / tg_pieces<RB>();
+}
+// byte offset inside its row of the global piece that lane `lane` copies for stage row r
+// (the lane's LDS slot is lane % pieces, so the piece is that slot's swizzle partner),
+// clamped inside the pitch for the last workgroup of a pitch that RB does not divide
+template <int RB> __host__ __device__ inline int64_t tg_piece_off(int64_t b0, int r, int lane, int64_t pitch) {
+    const int64_t off = b0 + 16 * ((lane % tg_pieces<RB>()) ^ tg_swz<RB>(r));
+    return off + 16 <= pitch ? off : pitch - 16;
+}
+// LDS byte of logical piece p of stage row r
+template <int RB> __host__ __device__ inline int tg_row_slot(int r, int p) {
+    return r * RB + ((p ^ tg_swz<RB>(r)) << 4);
+}
+// digits: lanes 0..7 of wave w copy digit column n = 2 w + lane / 4, 16-byte chunk
+// g = (lane & 3) ^ ((n >> 2) & 3) of a stage's 64 individuals (column n at n ldd)
+__host__ __device__ inline int tg_dig_col(int w, int lane) { return 2 * w + (lane >> 2); }
+__host__ __device__ inline int64_t tg_dig_off(int w, int lane, int64_t ldd) {
+    const int n = tg_dig_col(w, lane);
+    return (int64_t)n * ldd + 16 * ((lane & 3) ^ ((n >> 2) & 3));
+}
+// LDS byte of digit chunk g of column n
+__host__ __device__ inline int tg_dig_slot(int n, int g) { return n * 64 + ((g ^ ((n >> 2) & 3)) << 4); }
+// first stage of split sp of nst stages in nsplit splits
+__host__ __device__ inline int64_t tg_split_lo(int64_t sp, int64_t nst, int64_t nsplit) { return sp * nst / nsplit; }
+// SNP of accumulator element v of MFMA group g, copy column i of wave w, field j
+template <int RB> __host__ __device__ inline int64_t tg_out_snp(int64_t b0, int w, int i, int g, int v, int j) {
+    return 4 * (b0 + 16 * (tg_copies<RB>() * w + i) + 4 * g + v) + j;
+}
+// Launch: R workgroups across the pitch and P splits of the listed individuals -- four
+// workgroups per CU, each split at least 16 stages long, at most 8 splits and at most
+// the slots of m x 16 int32 partials that the partial buffer holds (part_bytes)
+struct TgLaunch {
+    int64_t R;
+    int P;
+};
+template <int RB> inline TgLaunch tg_launch(int64_t pitch, int64_t m, int64_t ng, int ncu, int64_t part_bytes) {
+    const int64_t R = (pitch + RB - 1) / RB;
+    const int64_t pmax = std::max<int64_t>(1, part_bytes / (m * 16 * (int64_t)sizeof(int32_t)));
+    const int64_t P = std::max<int64_t>(1, std::min({(4 * (int64_t)ncu + R - 1) / R, ng / 64 / 16, pmax, (int64_t)8}));
+    return {R, (int)P};
+}
+
 // ---------------------------------------------------------------- geometry of a product
 // One contraction pass: `nrows` rows of `pitch` packed bytes (the launch's row pointer
 // may start inside the layout), Kpad genotypes per padded row, nss super-steps of 256.

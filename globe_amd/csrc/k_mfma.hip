@@ -328,14 +328,6 @@ typedef __attribute__((address_space(3))) const uint8_t lds_cu8;
 __device__ inline uint32_t lds_addr(const uint8_t* p) { return (uint32_t)(uintptr_t)(lds_cu8*)p; }
 
 template <int RB>
-__device__ inline int tg_row_slot(int r, int p) {
-    // 128-byte rows: rows r and r + 2 share banks; 256-byte rows: every row does
-    const int f = RB == 128 ? ((r >> 1) & 7) : (r & 7);
-    return r * RB + ((p ^ f) << 4);
-}
-__device__ inline int tg_dig_slot(int nn, int g) { return nn * 64 + ((g ^ ((nn >> 2) & 3)) << 4); }
-
-template <int RB>
 __global__ __launch_bounds__(512) void k_geno_tg(const uint8_t* __restrict__ ind, int64_t pitch, int64_t m,
                                                  const int32_t* __restrict__ ridx, int64_t ng,
                                                  const int8_t* __restrict__ Dt, int64_t ldd, int nsplit,
@@ -343,34 +335,27 @@ __global__ __launch_bounds__(512) void k_geno_tg(const uint8_t* __restrict__ ind
     if (skip && *skip) return;
     constexpr int kTile = 64 * RB;            // staged rows
     constexpr int kSlot = kTile + 1024;       // + digits
-    constexpr int kPP = RB / 16;              // 16-byte pieces per row
-    constexpr int kRowsPerKB = 1024 / RB;     // rows per copy instruction
-    constexpr int kCopies = RB / 128;         // row copies per wave and stage (= its columns)
+    constexpr int kCopies = plan::tg_copies<RB>();  // row copies per wave and stage (= its columns)
     static_assert(RB == 128 || RB == 256, "128- or 256-byte rows");
     __shared__ __attribute__((aligned(1024))) uint8_t lds[kTgNS * kSlot];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t b0 = (int64_t)blockIdx.x * RB;  // byte offset of the workgroup's SNPs
     const int sp = blockIdx.y;
     const int64_t nst = ng / 64;
-    const int64_t lo = sp * nst / nsplit, hi = (sp + 1) * nst / nsplit;
+    const int64_t lo = plan::tg_split_lo(sp, nst, nsplit), hi = plan::tg_split_lo(sp + 1, nst, nsplit);
     const int64_t nt = hi - lo;
-    // copy roles: copy instruction i of wave w fills LDS bytes (8 i + w) KB: rows
-    // cr[i] = (8 i + w) kRowsPerKB + lane / kPP, piece slot lane % kPP (the global piece
-    // swizzled into it, clamped inside the row for a last workgroup past the pitch); digit
-    // column dn = 2 w + lane / 4 and its chunk slot lane % 4 (lanes 0..7)
+    // copy roles: copy instruction i of wave w fills LDS bytes (8 i + w) KB: stage row
+    // cr[i], piece slot lane % pieces (the global piece swizzled into it, clamped inside
+    // the row for a last workgroup past the pitch); lanes 0..7 copy a chunk of a digit
+    // column
     int cr[kCopies];
     const uint8_t* ibase[kCopies];
 #pragma unroll
     for (int i = 0; i < kCopies; ++i) {
-        cr[i] = (8 * i + w) * kRowsPerKB + lane / kPP;
-        const int q = lane % kPP;
-        const int f = RB == 128 ? ((cr[i] >> 1) & 7) : (cr[i] & 7);
-        const int64_t off = b0 + 16 * (q ^ f);
-        ibase[i] = ind + (off + 16 <= pitch ? off : pitch - 16);
+        cr[i] = plan::tg_copy_row<RB>(i, w, lane);
+        ibase[i] = ind + plan::tg_piece_off<RB>(b0, cr[i], lane, pitch);
     }
-    const int dn = 2 * w + (lane >> 2), dq = lane & 3;
-    const int dg = dq ^ ((dn >> 2) & 3);
-    const int8_t* const dbase = Dt + (int64_t)dn * ldd + 16 * dg;
+    const int8_t* const dbase = Dt + (lane < 8 ? plan::tg_dig_off(w, lane, ldd) : 0);
     v4i acc[kCopies][4];
 #pragma unroll
     for (int i = 0; i < kCopies; ++i)
@@ -420,7 +405,7 @@ __global__ __launch_bounds__(512) void k_geno_tg(const uint8_t* __restrict__ ind
                     issue(t + ahead, nxt[(q + ahead) % kTgGroup], cs == 0 ? kTgNS - 1 : cs - 1);
                 __builtin_amdgcn_sched_barrier(0);
                 const uint8_t* slot = lds + cs * kSlot;
-                const v4i bv = *reinterpret_cast<const v4i*>(slot + kTile + tg_dig_slot(sl, g));
+                const v4i bv = *reinterpret_cast<const v4i*>(slot + kTile + plan::tg_dig_slot(sl, g));
                 const int rr = 16 * g + (sl >> 1);
 #pragma unroll
                 for (int i = 0; i < kCopies; ++i) {
@@ -428,8 +413,8 @@ __global__ __launch_bounds__(512) void k_geno_tg(const uint8_t* __restrict__ ind
                     // written as asm: the compiler would otherwise wait for every copy in
                     // flight (vmcnt(0)) before a transposing read, taking it for a read of
                     // the copies' destination; both reads complete inside the block
-                    const uint32_t a1 = lds_addr(slot + tg_row_slot<RB>(rr, pc) + 8 * (sl & 1));
-                    const uint32_t a2 = lds_addr(slot + tg_row_slot<RB>(rr + 8, pc) + 8 * (sl & 1));
+                    const uint32_t a1 = lds_addr(slot + plan::tg_row_slot<RB>(rr, pc) + 8 * (sl & 1));
+                    const uint32_t a2 = lds_addr(slot + plan::tg_row_slot<RB>(rr + 8, pc) + 8 * (sl & 1));
                     v2i_t lo8, hi8;
                     asm volatile("ds_read_b64_tr_b8 %0, %2\n\tds_read_b64_tr_b8 %1, %3\n\ts_waitcnt lgkmcnt(0)"
                                  : "=&v"(lo8), "=&v"(hi8)
@@ -466,7 +451,7 @@ __global__ __launch_bounds__(512) void k_geno_tg(const uint8_t* __restrict__ ind
         for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int64_t snp = 4 * (b0 + 16 * (kCopies * w + i) + 4 * g + v) + j;
+                const int64_t snp = plan::tg_out_snp<RB>(b0, w, i, g, v, j);
                 if (snp < m) out[plan::part_idx(sp, m, snp, 16, sl)] = acc[i][j][v];
             }
 }
@@ -1395,12 +1380,10 @@ void mfma_geno_product(globe_ctx* c, const double* S0, int64_t ldn, int k, doubl
         k_quantize_gather<<<dim3((unsigned)std::min<int64_t>((ng + 255) / 256, 1024), 16), 256, 0, s>>>(
             S0, ldn, ridx, ng, lm, J, sh1, Dt, ng, skip);
         HIP_CHECK(hipGetLastError());
-        constexpr int kRB = 256;                                  // row bytes per workgroup
-        const int64_t R = (c->pitch_i + kRB - 1) / kRB;           // workgroups of 4 kRB SNPs
-        const int64_t nst = ng / 64;
-        const int64_t pmax = std::max<int64_t>(1, (int64_t)(ob / ((size_t)m * 16 * sizeof(int32_t))));
-        // enough workgroups for four per CU, each split at least 16 stages long
-        const int P = (int)std::max<int64_t>(1, std::min({(4 * (int64_t)ncu + R - 1) / R, nst / 16, pmax, (int64_t)8}));
+        constexpr int kRB = 256;  // row bytes per workgroup
+        const plan::TgLaunch tl = plan::tg_launch<kRB>(c->pitch_i, m, ng, ncu, (int64_t)ob);
+        const int64_t R = tl.R;
+        const int P = tl.P;
         globe_ctx::ProfEvent e{};
         const bool prof = c->prof_on && !noalloc;
         if (prof) {

@@ -259,9 +259,141 @@ void walk(bool wide, int JT, const PassGeom& p, int64_t rb, int64_t bqb, int64_t
     }
 }
 
+// One gathered pass-1 launch (k_geno_tg<RB>) over ng listed individuals: R workgroups
+// across the individual-major pitch, P splits of the stages. Walks every workgroup, copy
+// lane and accumulator element through the kernel's index functions.
+template <int RB>
+void walk_tg(int64_t m, int64_t n, int64_t ng, int64_t pitch, int ncu, int64_t row_bytes, int64_t dig_bytes,
+             int64_t part_bytes, int64_t* stats) {
+    const TgLaunch tl = tg_launch<RB>(pitch, m, ng, ncu, part_bytes);
+    const int64_t R = tl.R, P = tl.P, nst = ng / 64;
+    constexpr int kTile = 64 * RB;
+    check(R >= 1 && R * RB >= pitch && (R - 1) * RB < pitch, "workgroups do not tile the pitch");
+    check(P >= 1 && P <= 8 && (P == 1 || nst / P >= 16), "split count");
+    check(P * m * 16 * (int64_t)sizeof(int32_t) <= part_bytes, "partial slots exceed the partial buffer");
+    // splits tile the stages
+    for (int64_t sp = 0; sp < P; ++sp)
+        check(tg_split_lo(sp + 1, nst, P) >= tg_split_lo(sp, nst, P), "splits out of order");
+    check(tg_split_lo(0, nst, P) == 0 && tg_split_lo(P, nst, P) == nst, "splits do not tile the stages");
+    // copies: a stage row is filled by one (copy, wave, lane) per LDS slot, and the slot
+    // holds the global piece that the transposing reads of its logical piece expect
+    std::vector<int> owner((size_t)kTile / 16, -1);
+    for (int i = 0; i < tg_copies<RB>(); ++i)
+        for (int w = 0; w < 8; ++w)
+            for (int lane = 0; lane < 64; ++lane) {
+                const int r = tg_copy_row<RB>(i, w, lane);
+                check(r >= 0 && r < 64, "copy row outside the stage");
+                const int64_t lds = 1024 * (8 * i + w) + 16 * lane;  // where the copy lands
+                check(lds == (int64_t)r * RB + 16 * (lane % tg_pieces<RB>()), "copy lands off its row");
+                int& o = owner[(size_t)(lds / 16)];
+                check(o < 0, "two copies fill one LDS slot");
+                o = (int)(16 * tg_piece_off<RB>(0, r, lane, INT64_MAX / 2) / 16);
+            }
+    for (int r = 0; r < 64; ++r)
+        for (int pc = 0; pc < tg_pieces<RB>(); ++pc) {
+            const int at = tg_row_slot<RB>(r, pc);
+            check(at >= r * RB && at + 16 <= (r + 1) * RB, "read slot outside its row");
+            check(owner[(size_t)(at / 16)] == 16 * pc, "read slot does not hold the piece it reads");
+        }
+    // transposing reads of a lane (group g, column sl): rows 16 g + sl / 2 and + 8
+    for (int g = 0; g < 4; ++g)
+        for (int sl = 0; sl < 16; ++sl) {
+            const int rr = 16 * g + (sl >> 1);
+            check(rr + 8 < 64, "transposing read past the stage");
+            check(tg_dig_slot(sl, g) + 16 <= 1024, "digit read outside the stage's digits");
+        }
+    // digit copies: lanes 0..7 of every wave fill the 16 x 64 digit block once
+    std::vector<int> dseen(16 * 4, 0);
+    for (int w = 0; w < 8; ++w)
+        for (int lane = 0; lane < 8; ++lane) {
+            const int64_t off = tg_dig_off(w, lane, ng);
+            const int nn = tg_dig_col(w, lane);
+            check(nn >= 0 && nn < 16, "digit column");
+            const int64_t chunk = (off - nn * ng) / 16;
+            check(chunk >= 0 && chunk < 4 && (off - nn * ng) % 16 == 0, "digit chunk");
+            ++dseen[(size_t)(nn * 4 + chunk)];
+            // stage t of the split starts at individual 64 t
+            check(off >= 0 && off + 64 * (nst - 1) + 16 <= 16 * ng && 16 * ng <= dig_bytes,
+                  "digit read outside the digit matrix");
+            // and lands where the reads of (column nn, chunk) look
+            check(1024 * 0 + 128 * w + 16 * lane == tg_dig_slot(nn, (int)chunk), "digit copy lands off its slot");
+        }
+    for (int v : dseen) check(v == 1, "digit block not filled exactly once");
+    // row reads: every listed row < n, at a clamped piece of every workgroup
+    Range rows;
+    for (int64_t x = 0; x < R; ++x)
+        for (int i = 0; i < tg_copies<RB>(); ++i)
+            for (int w = 0; w < 8; ++w)
+                for (int lane = 0; lane < 64; ++lane) {
+                    const int r = tg_copy_row<RB>(i, w, lane);
+                    const int64_t off = tg_piece_off<RB>(x * RB, r, lane, pitch);
+                    check(off >= 0 && off + 16 <= pitch, "piece outside its row");
+                    rows.add(off, 16);
+                    rows.add((n - 1) * pitch + off, 16);
+                }
+    check(rows.lo >= 0 && rows.hi <= row_bytes, "packed-row read outside the padded layout");
+    // index reads: stage rows of stages 0 .. nst - 1 (later stages clamp to the last)
+    check(64 * (nst - 1) + 63 < ng, "index read past the listed individuals");
+    // stores: every SNP < m of every split exactly once per column
+    std::vector<uint8_t> hit((size_t)(P * m), 0);
+    for (int64_t sp = 0; sp < P; ++sp)
+        for (int64_t x = 0; x < R; ++x)
+            for (int w = 0; w < 8; ++w)
+                for (int i = 0; i < tg_copies<RB>(); ++i)
+                    for (int g = 0; g < 4; ++g)
+                        for (int v = 0; v < 4; ++v)
+                            for (int j = 0; j < 4; ++j) {
+                                const int64_t snp = tg_out_snp<RB>(x * RB, w, i, g, v, j);
+                                check(snp >= 0 && snp < 4 * R * RB, "SNP outside the workgroups");
+                                if (snp >= m) continue;
+                                for (int sl : {0, 15}) {
+                                    const int64_t idx = part_idx(sp, m, snp, 16, sl);
+                                    check(idx >= 0 && (idx + 1) * 4 <= part_bytes, "partial store outside the buffer");
+                                }
+                                check(!hit[(size_t)(sp * m + snp)]++, "SNP stored twice by a split");
+                            }
+    for (uint8_t h : hit) check(h == 1, "SNP not stored by a split");
+    check((part_idx(P - 1, m, m - 1, 16, 15) + 1) * 4 <= part_bytes, "epilogue read outside the partials");
+    stats[0] = R;
+    stats[1] = P;
+    stats[2] = rows.hi - row_bytes;
+}
+
 }  // namespace
 
 extern "C" {
+
+// Replays the gathered pass 1 of a thin product (k_quantize_gather + k_geno_tg<256>) over
+// ng listed individuals (a positive multiple of 512, at most n + 512) of a cohort of n
+// individuals and m kept SNPs, with the buffers mfma_geno_product allocates for it
+// (the digit matrix and split partials of the product's geometry on ng rows; the launch
+// takes fewer splits from a smaller partial buffer). stats (5): [0] workgroups across the
+// pitch, [1] splits, [2] largest packed-row byte read + 1 over the layout's bytes, [3]
+// digit bytes, [4] partial bytes. shrink (3, may be NULL) as for globe_plan_replay.
+// Returns 0, or 1 with the first violation in msg.
+int globe_plan_replay_tg(int64_t m, int64_t n, int64_t ng, int ncu, const int* thin_bpc, const int64_t* shrink,
+                         int64_t* stats, char* msg, int msglen) {
+    std::string err;
+    std::memset(stats, 0, 5 * sizeof(int64_t));
+    try {
+        check(m >= 1 && n >= 1 && ng > 0 && ng % kRowPad == 0 && ng <= n + kRowPad && ncu >= 1, "arguments");
+        const int64_t pitch_s = packed_pitch(n), pitch_i = packed_pitch(m);
+        const Geometry geo = contract_geometry(16, m, ng, pitch_s, pitch_i, 0, ncu, thin_bpc);
+        check(!geo.wide && geo.J16 == 16, "a gathered pass 1 is thin");
+        const int64_t sh[3] = {shrink ? shrink[0] : 0, shrink ? shrink[1] : 0, shrink ? shrink[2] : 0};
+        const int64_t row_bytes = packed_rows(n) * pitch_i - sh[0];
+        const int64_t dig_bytes = geo.bq_bytes - sh[1];
+        // the digit transpose: column nn of listed individual t at nn ng + t
+        check(15 * ng + ng - 1 < dig_bytes, "gathered digit write outside the digit matrix");
+        stats[3] = geo.bq_bytes;
+        stats[4] = geo.part_bytes;
+        walk_tg<256>(m, n, ng, pitch_i, ncu, row_bytes, dig_bytes, geo.part_bytes - sh[2], stats);
+    } catch (const Fail& f) {
+        err = f.msg;
+    }
+    if (msg && msglen > 0) std::snprintf(msg, (size_t)msglen, "%s", err.c_str());
+    return err.empty() ? 0 : 1;
+}
 
 // Replays both passes of one product shape: J digit columns (1..128), m kept SNPs, n
 // individuals, pass 2 from row r0 (the GRM builder; 0 otherwise), ncu compute units and

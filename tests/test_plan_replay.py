@@ -105,3 +105,63 @@ def test_replay_rejects_bad_shapes():
         globe_amd.plan_replay(129, 40000, 20000)
     with pytest.raises(AssertionError):
         globe_amd.plan_replay(128, 40000, 20000, r0=100)
+
+
+def _gather_sizes(n):
+    """Listed-individual counts of the gathered pass: 512 up to the cohort padded to 512
+    (every 512-multiple for small cohorts, a spread plus the largest for large ones)."""
+    top = (n + 511) // 512 * 512
+    sizes = range(512, top + 1, 512)
+    if len(sizes) > 40:
+        sizes = sorted(set(list(sizes[:8]) + list(sizes[::max(1, len(sizes) // 24)]) + list(sizes[-4:])))
+    return sizes
+
+
+@pytest.mark.parametrize("m,n", [(499939, 200000), (2000, 3000), (2101, 2500), (2302, 2500), (1, 1), (5, 513), (900, 400),
+                                 (4097, 70001), (79990, 40000)])
+def test_replay_gathered_pass1(m, n):
+    """k_geno_tg over every listed-individual count the serving loop can pass: the
+    workgroups tile the individual-major pitch (a last one past the pitch reads clamped
+    pieces), the splits tile the stages, each stage row's LDS slots are filled once with
+    the pieces the transposing reads expect, and every SNP < m of every split is stored
+    once inside the partial buffer."""
+    seen = set()
+    for ng in _gather_sizes(n):
+        for ncu in (256, 80):
+            for bpc in BPC:
+                st = globe_amd.plan_replay_tg(m, n, ng, ncu=ncu, thin_bpc=bpc)
+                assert st["row_margin"] <= 0
+                seen.add((st["workgroups"], st["splits"]))
+    if n >= 40000:
+        assert max(p for _, p in seen) > 1, seen  # split launches are replayed too
+
+
+def test_replay_gathered_pass1_detects_an_undersized_buffer():
+    """Taking 16 bytes more than the reported margin off the packed layout, or 16 bytes
+    off a digit matrix used to its last byte, fails the replay; a smaller partial
+    buffer gives fewer splits, down to one slot of m x 16 sums, and one short of that
+    fails."""
+    m, n, ng = 2101, 2500, 2560
+    st = globe_amd.plan_replay_tg(m, n, ng)
+    with pytest.raises(AssertionError, match="packed-row read"):
+        globe_amd.plan_replay_tg(m, n, ng, shrink=(-st["row_margin"] + 16, 0, 0))
+    globe_amd.plan_replay_tg(m, n, ng, shrink=(-st["row_margin"], 0, 0))
+    # 2,048 individuals and a 256-byte pitch: the digit matrix is 16 x 2,048 bytes
+    st = globe_amd.plan_replay_tg(900, 2048, 2048)
+    assert st["digit_bytes"] == 16 * 2048
+    with pytest.raises(AssertionError, match="digit"):
+        globe_amd.plan_replay_tg(900, 2048, 2048, shrink=(0, 16, 0))
+    m, n, ng = 499939, 200000, 65536
+    st = globe_amd.plan_replay_tg(m, n, ng)
+    assert st["splits"] > 1
+    slot = m * 16 * 4
+    fewer = globe_amd.plan_replay_tg(m, n, ng, shrink=(0, 0, st["part_bytes"] - slot))
+    assert fewer["splits"] == 1
+    with pytest.raises(AssertionError, match="partial"):
+        globe_amd.plan_replay_tg(m, n, ng, shrink=(0, 0, st["part_bytes"] - slot + 4))
+
+
+def test_replay_gathered_pass1_rejects_bad_counts():
+    for ng in (0, 100, 4096):
+        with pytest.raises(AssertionError):
+            globe_amd.plan_replay_tg(2000, 3000, ng)

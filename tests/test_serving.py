@@ -309,3 +309,42 @@ def test_serving_row_gather_equals_all_rows(gpu_ctx_factory):
         g.close()
         for q in range(len(reqs)):
             assert np.array_equal(x[q], out[False][q]), (v, q)
+
+
+def test_serving_row_gather_ragged_pitch(gpu_ctx_factory):
+    """A cohort whose individual-major pitch (640 bytes) is not a multiple of the gathered
+    pass's 256 bytes per workgroup, so its last workgroup reads clamped pieces past the
+    pitch and drops their SNPs: gathered and full passes agree bit for bit in mixed
+    precision (the last packed byte ragged too), and an FP64 request (7 digits, gathered
+    alone in the block) matches the
+    oracle of its own system."""
+    _say("serving row gather, ragged pitch")
+    d = synth.make_geno_case(n=2500, m=2302, lam=3, seed=23, miss=0.01)
+    rng = np.random.default_rng(8)
+    reqs = [serving.make_request(rng, 2500, 30, 300, 1.0, 4.0) for _ in range(6)]
+    out = {}
+    for on in (False, True):
+        g = _device(d, gpu_ctx_factory)
+        pitch = ((g.m + 3) // 4 + 127) // 128 * 128  # individual-major row bytes
+        assert pitch % 256 != 0 and g.m % 4 != 0, (g.m, pitch)
+        g.set_row_gather(on)
+        g.profile(True)
+        out[on], info = _serve(g, reqs, slots=3, precision=G.PREC_MIXED, tol=1e-8)
+        ks = g.kernel_stats(1)
+        g.profile(False)
+        if on:
+            assert 0 < ks["bytes"] < 2500 * ((g.m + 3) // 4), ks
+        g.close()
+        assert all(st["converged"] for st in info)
+    for q in range(len(reqs)):
+        assert np.array_equal(out[True][q], out[False][q]), q
+    g = _device(d, gpu_ctx_factory)
+    g.profile(True)
+    x, info = _serve(g, [reqs[2]], slots=2, precision=G.PREC_FP64, tol=1e-10, stagger=False)
+    ks = g.kernel_stats(1)
+    g.close()
+    assert 0 < ks["bytes"] < 2500 * ((g.m + 3) // 4), ks
+    assert info[0]["converged"]
+    orc = _oracle_for(d, reqs[2])
+    xo, _ = orc.pcg(np.asarray(reqs[2][2])[:, None], tol=1e-10, rule=2, stagnation=False)
+    assert rel(x[0], xo[:, 0]) < 1e-7
