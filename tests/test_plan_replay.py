@@ -124,7 +124,10 @@ def test_replay_gathered_pass1(m, n):
     workgroups tile the individual-major pitch (a last one past the pitch reads clamped
     pieces), the splits tile the stages, each stage row's LDS slots are filled once with
     the pieces the transposing reads expect, and every SNP < m of every split is stored
-    once inside the partial buffer."""
+    once inside the partial buffer.
+    The gathered pass 2 (a thin candidate over the ng listed rows) is every candidate of
+    a thin product on ng rows, replayed for the same counts (at the scale cohort for one
+    CU count and residency only: each replay there also walks pass 1's 500k rows)."""
     seen = set()
     for ng in _gather_sizes(n):
         for ncu in (256, 80):
@@ -132,6 +135,9 @@ def test_replay_gathered_pass1(m, n):
                 st = globe_amd.plan_replay_tg(m, n, ng, ncu=ncu, thin_bpc=bpc)
                 assert st["row_margin"] <= 0
                 seen.add((st["workgroups"], st["splits"]))
+                if m < 100000 or (ncu, bpc) == (256, BPC[0]):
+                    st2 = globe_amd.plan_replay(7, m, ng, ncu=ncu, thin_bpc=bpc)
+                    assert not st2["wide"] and st2["row_margin2"] <= 0
     if n >= 40000:
         assert max(p for _, p in seen) > 1, seen  # split launches are replayed too
 
