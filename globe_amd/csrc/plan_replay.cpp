@@ -287,7 +287,7 @@ void walk_tg(int64_t m, int64_t n, int64_t ng, int64_t pitch, int ncu, int64_t r
                 check(lds == (int64_t)r * RB + 16 * (lane % tg_pieces<RB>()), "copy lands off its row");
                 int& o = owner[(size_t)(lds / 16)];
                 check(o < 0, "two copies fill one LDS slot");
-                o = (int)(16 * tg_piece_off<RB>(0, r, lane, INT64_MAX / 2) / 16);
+                o = (int)tg_piece_off<RB>(0, r, lane, INT64_MAX / 2);  // the piece's byte offset
             }
     for (int r = 0; r < 64; ++r)
         for (int pc = 0; pc < tg_pieces<RB>(); ++pc) {
@@ -316,7 +316,7 @@ void walk_tg(int64_t m, int64_t n, int64_t ng, int64_t pitch, int ncu, int64_t r
             check(off >= 0 && off + 64 * (nst - 1) + 16 <= 16 * ng && 16 * ng <= dig_bytes,
                   "digit read outside the digit matrix");
             // and lands where the reads of (column nn, chunk) look
-            check(1024 * 0 + 128 * w + 16 * lane == tg_dig_slot(nn, (int)chunk), "digit copy lands off its slot");
+            check(128 * w + 16 * lane == tg_dig_slot(nn, (int)chunk), "digit copy lands off its slot");
         }
     for (int v : dseen) check(v == 1, "digit block not filled exactly once");
     // row reads: every listed row < n, at a clamped piece of every workgroup
