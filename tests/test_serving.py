@@ -315,8 +315,8 @@ def test_serving_row_gather_ragged_pitch(gpu_ctx_factory):
     """A cohort whose individual-major pitch (640 bytes) is not a multiple of the gathered
     pass's 256 bytes per workgroup, so its last workgroup reads clamped pieces past the
     pitch and drops their SNPs: gathered and full passes agree bit for bit in mixed
-    precision (the last packed byte ragged too), and an FP64 request (7 digits, gathered
-    alone in the block) matches the
+    precision (the last packed byte ragged too) and in FP64 with two requests in the
+    block, and an FP64 request (7 digits, gathered alone in the block) matches the
     oracle of its own system."""
     _say("serving row gather, ragged pitch")
     d = synth.make_geno_case(n=2500, m=2302, lam=3, seed=23, miss=0.01)
@@ -338,6 +338,16 @@ def test_serving_row_gather_ragged_pitch(gpu_ctx_factory):
         assert all(st["converged"] for st in info)
     for q in range(len(reqs)):
         assert np.array_equal(out[True][q], out[False][q]), q
+    # FP64 with two requests in the block: 14 digit columns, still one gathered group
+    x64 = {}
+    for on in (False, True):
+        g = _device(d, gpu_ctx_factory)
+        g.set_row_gather(on)
+        x64[on], info = _serve(g, reqs[:2], slots=2, precision=G.PREC_FP64, tol=1e-10, stagger=False)
+        g.close()
+        assert all(st["converged"] for st in info)
+    for q in range(2):
+        assert np.array_equal(x64[True][q], x64[False][q]), q
     g = _device(d, gpu_ctx_factory)
     g.profile(True)
     x, info = _serve(g, [reqs[2]], slots=2, precision=G.PREC_FP64, tol=1e-10, stagger=False)
